@@ -1,0 +1,210 @@
+/*
+ * vc_abi.h -- C ABI of libvc_hip.so, the MI355X (gfx950) implementation of
+ * the patchV1 simulation step.
+ *
+ * Every entry point:
+ *   - takes device pointers and sizes only (no framework types), plus a
+ *     `stream` argument that is a hipStream_t passed as void* (NULL = the
+ *     null stream);
+ *   - allocates nothing: the caller owns every buffer;
+ *   - returns 0 on success or a negative VC_E_* code; vc_last_error() returns
+ *     a message describing the most recent failure on the calling thread.
+ *     Nothing calls exit().
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   vc_lgn_precompute      store_PM + parvo_maxConvol
+ *                            src/discrete_input_convol.cu:427-744, 746-881;
+ *                            launched at src/patch.cu:5243-5320
+ *   vc_lgn_frame_response  the per-frame part of LGN_convol_parvo
+ *                            src/discrete_input_convol.cu:1003-1223
+ *   vc_lgn_steps           the temporal part of LGN_convol_parvo (taps x
+ *                            frames, :1143-1340) fused with LGN_nonlinear and
+ *                            get_spike (:1587-1821), for a run of steps
+ *   vc_v1_step             compute_V_collect_spike_learnFF_fast
+ *                            src/coredynamics.cu:1235-2059 (stages c-FF, d, e)
+ *                            fused with recal_G_mat_nd_fast for the previous
+ *                            step's spikes, src/coredynamics.cu:2609-2922
+ *   vc_v1_rec_apply        recal_G_mat_nd_fast on its own (the pending
+ *                            recurrent increments of the last step)
+ * The data-structure layouts are documented field by field below.
+ */
+#ifndef VC_ABI_H
+#define VC_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VC_MAX_NTYPE 2      /* src/CONST.h:14  MAX_NTYPE   */
+#define VC_MAX_NG    2      /* src/CONST.h:27-30 MAX_NGTYPE_* */
+
+#define VC_OK             0
+#define VC_E_INVALID    (-1)   /* bad argument or size           */
+#define VC_E_UNSUPPORTED (-2)  /* a reference feature not built  */
+#define VC_E_HIP        (-3)   /* a HIP runtime call failed      */
+
+/* ------------------------------------------------------------------ V1 -- */
+
+/* Neuron model and synapse parameters; one copy per simulation (or trial).
+ * Types: neuron i of a block has type j = first j with (i % blockSize) <
+ * typeAcc[j] (src/coredynamics.cu:1335-1342); E neurons are local ids < nE. */
+typedef struct {
+    uint32_t nV1, nblock, blockSize, nE;     /* nE = typeAcc[nTypeE-1]          */
+    uint32_t nType;
+    uint32_t typeAcc[VC_MAX_NTYPE];
+    uint32_t ngFF, ngE, ngI;                  /* receptor counts (<= VC_MAX_NG)  */
+    int32_t  iModel;                          /* 1: AdEx, w stored; 0: w not kept */
+    float dt;                                 /* ms                              */
+    /* per type (src/patch.cu:180-374 cfg keys of the same names) */
+    float vR[VC_MAX_NTYPE], vThres[VC_MAX_NTYPE], gL[VC_MAX_NTYPE], C[VC_MAX_NTYPE];
+    float tRef[VC_MAX_NTYPE], vT[VC_MAX_NTYPE], deltaT[VC_MAX_NTYPE];
+    float tau_w[VC_MAX_NTYPE], a[VC_MAX_NTYPE], b[VC_MAX_NTYPE];
+    float pFF[VC_MAX_NTYPE * VC_MAX_NG];      /* [type][ig] receptor split       */
+    float pE[VC_MAX_NTYPE * VC_MAX_NG];
+    float pI[VC_MAX_NTYPE * VC_MAX_NG];
+    /* ConductanceShape (src/condShape.h:13-41): dod = rise/(decay-rise) */
+    float riseFF[VC_MAX_NG], decayFF[VC_MAX_NG], dodFF[VC_MAX_NG];
+    float riseE[VC_MAX_NG],  decayE[VC_MAX_NG],  dodE[VC_MAX_NG];
+    float riseI[VC_MAX_NG],  decayI[VC_MAX_NG],  dodI[VC_MAX_NG];
+    /* background Poisson drive (extension; not in the reference): per type a
+     * rate lambda = rate*dt events per step, p0 = exp(-lambda) (precomputed
+     * on the host), strength and receptor split.  bgStrength 0 = off. */
+    float bgStrength[VC_MAX_NTYPE], bgLambda[VC_MAX_NTYPE], bgP0[VC_MAX_NTYPE];
+    float pBG[VC_MAX_NTYPE * VC_MAX_NG];
+    uint32_t seed_lo, seed_hi, trial;         /* Philox key and trial word       */
+} vc_v1_params;
+
+/* Persistent V1 state, all [..][nV1] float32, receptor-major. */
+typedef struct {
+    float *v, *w, *tBack;
+    float *gFF, *hFF;          /* [ngFF][nV1] */
+    float *gE, *hE;            /* [ngE][nV1]  */
+    float *gI, *hI;            /* [ngI][nV1]  */
+    const float *tonicDep;     /* [nV1]       */
+} vc_v1_state;
+
+/* LGN->V1 synapses, ELL layout transposed like the reference's sLGN
+ * (src/patch.h:30-77): entry i of neuron n at [i*nV1 + n]. */
+typedef struct {
+    const uint32_t *nLGN;      /* [nV1] inputs of each neuron (<= maxLGN)       */
+    const float *sLGN;         /* [maxLGN][nV1] strength, already * sRatioLGN   */
+    const uint32_t *idLGN;     /* [maxLGN][nV1] LGN cell index                  */
+    uint32_t maxLGN;
+} vc_ff_conn;
+
+/* V1->V1 near-neighbour synapses in "masked CSR": the reference's dense
+ * V1_conMat[nblock][maxNear][blockSize pre][blockSize post]
+ * (src/patch.cu:3441-3461) with only its strictly positive entries kept.
+ * Row r = (b*maxNear + ib)*blockSize + pre, split into blockSize/64 segments
+ * of 64 posts (one wavefront each):
+ *   rowMask[r*nSeg + s] bit l  <=> conMat[...][pre][s*64 + l] > 0
+ *   rowOff [r*nSeg + s]        =   index in w[] of the first kept entry of
+ *                                  segment s (entries in post order)           */
+typedef struct {
+    uint32_t maxNear;           /* nearNeighborBlock                            */
+    const uint32_t *nNear;      /* [nblock] near neighbours of each block       */
+    const uint32_t *nbr;        /* [nblock][maxNear] neighbour block ids        */
+    const uint64_t *rowMask;    /* [nblock*maxNear*blockSize][blockSize/64]     */
+    const uint32_t *rowOff;     /* [nblock*maxNear*blockSize][blockSize/64]     */
+    const float *w;             /* kept strengths                                */
+} vc_rec_conn;
+
+/* Spikes of one step ("per-block spike bookkeeping"): one 64-bit spike mask
+ * per 64-neuron tile, and the unit conductance increments of each spiking
+ * neuron stored at slot tile*64 + rank, rank = number of lower spiking lanes
+ * of the tile (so a tile's slots list its spikes in ascending neuron order,
+ * E before I).  og/oh are [VC_MAX_NG][nV1]. */
+typedef struct {
+    uint64_t *mask;             /* [nV1/64] */
+    float *og, *oh;             /* [VC_MAX_NG][nV1] */
+} vc_spikes;
+
+/* One simulation step on all V1 neurons: apply the recurrent increments of
+ * `prev` (the previous step's spikes), decay, LGN + background input, RK2
+ * update, spike time, refractory handling, compaction into `next`.
+ * out_sInfo[nV1] receives 1 + tsp/dt for a spike and v otherwise;
+ * out_dep[nV1] (may be NULL) receives depC; out_gE_prev (may be NULL)
+ * receives, as [ngE + ngI][nV1] g values, the conductances at the end of the
+ * previous step (after its recurrent increments), i.e. the reference's gE/gI
+ * output of that step. */
+int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+               const vc_ff_conn *ff, const vc_rec_conn *rec,
+               const vc_spikes *prev, const vc_spikes *next,
+               const float *lgnSInfo, uint32_t step,
+               float *out_sInfo, float *out_dep, float *out_g_prev);
+
+/* Apply the recurrent increments of `prev` to gE/hE/gI/hI without stepping
+ * (used after the last step, before reading or saving the state). */
+int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                    const vc_rec_conn *rec, const vc_spikes *prev);
+
+/* Initial V1 state: v, w, tBack, g/h as given; clears a spike buffer. */
+int vc_spikes_clear(void *stream, uint32_t nV1, const vc_spikes *sp);
+
+/* ----------------------------------------------------------------- LGN -- */
+
+/* Per-cell LGN parameters in the reference's SoA order [region][nLGN]
+ * (src/LGN_props.cuh:14-100), region 0 = centre, 1 = surround. Angles in
+ * radians; the stimulus is flat (flat_retina = true). */
+typedef struct {
+    uint32_t nLGN, nSample1D, nKernelSample, kernelSampleInterval,
+             iKernelSampleT0, maxSample1D;
+    float dt, nsig, normViewDistance, x0, y0;
+    const float *polar, *ecc, *rw, *rh, *orient, *k;        /* [2][nLGN] */
+    const float *tauR, *tauD, *delay, *ratio, *nR, *nD;     /* [2][nLGN] */
+    const float *covariant;                                 /* [nLGN]    */
+} vc_lgn_setup;
+
+/* Temporal weights TW[nLGN][2][nKernelSample], shared spatial weights
+ * SW[nSample1D^2], sample coordinates SCx/SCy[nLGN][2][nSample1D^2] (texture
+ * coordinates normalised to [0,1]), and maxConvol[nLGN]. */
+int vc_lgn_precompute(void *stream, const vc_lgn_setup *q, float *TW, float *SW,
+                      float *SCx, float *SCy, float *maxConvol);
+
+/* Spatial response of every cell to one frame: R[nLGN][2] = sum over samples
+ * of SW * clip(I/mean - 1, +-1), with the mean taken over the surround
+ * samples (src/discrete_input_convol.cu:1055-1069, 1143-1211).  frame is
+ * [3][height][width] (L, M, S) float32; coneType is [2][nLGN]. */
+int vc_lgn_frame_response(void *stream, const float *frame, uint32_t width,
+                          uint32_t height, uint32_t nLGN, uint32_t nSample1D,
+                          const float *SCx, const float *SCy, const float *SW,
+                          const uint32_t *coneType, float *R);
+
+typedef struct {
+    uint32_t nLGN, nKernelSample, kernelSampleInterval, iKernelSampleT0;
+    uint32_t nRetrace, ntPerFrame, maxFrame;
+    float dt, convolRatio, frRatio, tRef;
+    uint32_t seed_lo, seed_hi, trial;
+} vc_lgn_params;
+
+typedef struct {
+    const float *TW;                 /* [nLGN][2][nKernelSample]   */
+    const float *kC, *kS;            /* [nLGN] centre / surround k */
+    const float *maxConvol;          /* [nLGN]                     */
+    const float *c50, *sharpness, *logA, *logB;   /* [nLGN]        */
+    float *leftTimeRate, *lastNegLogRand;         /* [nLGN] state  */
+} vc_lgn_state;
+
+/* Steps it0 .. it0+nsteps-1 of stages a (temporal part) and b for every LGN
+ * cell.  Rring is [maxFrame][nLGN][2]: the spatial responses of frame f are
+ * in slot f mod maxFrame (pre-stimulus frames f < 0 included).  Writes
+ * sInfo[nsteps][nLGN] (nsp + mean tsp/dt) and, if fr != NULL,
+ * fr[nsteps][nLGN] in Hz. */
+int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
+                 const float *Rring, uint32_t it0, uint32_t nsteps,
+                 float *sInfo, float *fr);
+
+/* LGN Poisson state initialisation (logRand_init, src/coredynamics.cu:148-177):
+ * leftTimeRate = 0, lastNegLogRand = -log(u) with u a Philox draw. */
+int vc_lgn_init(void *stream, const vc_lgn_params *p, float *leftTimeRate,
+                float *lastNegLogRand);
+
+const char *vc_last_error(void);
+int vc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VC_ABI_H */

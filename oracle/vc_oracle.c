@@ -1,0 +1,816 @@
+/*
+ * vc_oracle.c -- CPU restatement of the patchV1 simulation step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is compiled
+ * into oracle/libvc_oracle.so and called only by tests/, by the smoke check
+ * and by the CPU-baseline leg of bench.py.  The product (libvc_hip.so and the
+ * Python driver) never links, loads or calls it.
+ *
+ * It restates, in plain C and binary32 arithmetic with no FP contraction,
+ * the reference's equations in the reference's own data layouts:
+ *   - LGN precompute     src/discrete_input_convol.cu:8-33 (kernels),
+ *                        :230-305 (temporal weights), :307-425 (spatial
+ *                        weights, sample coordinates), :746-881 (maxConvol)
+ *   - LGN frame response src/discrete_input_convol.cu:35-76, 1003-1069,
+ *                        1143-1211 (mean over surround samples, contrast,
+ *                        spatial reduction)
+ *   - LGN temporal sum,  src/discrete_input_convol.cu:1212-1340,
+ *     rate and spikes    :1637-1821, :1587-1635; src/LGN_props.cuh:94-100
+ *   - V1 update          src/coredynamics.cu:1235-1629 (non-learning path),
+ *                        src/coredynamics.cuh:27-118 (AdEx), src/condShape.h:13-41
+ *   - spike compaction   src/coredynamics.cu:1994-2046 (lists in ascending
+ *                        order: the reference's atomicAdd_block order is
+ *                        unspecified, ascending is one of its outcomes)
+ *   - recurrent gather   src/coredynamics.cu:2609-2922, dense conMat
+ *                        [nblock][maxNear][bs pre][bs post]
+ * Reductions follow the reference's block_reduce tree (32-lane shuffle-down
+ * halving per warp, then halving over the warp partials,
+ * src/util/cuda_util.cuh:22-127).
+ * exp/log on the per-step path use include/vc_math.h (the same functions
+ * the GPU uses); the one-time precompute uses the C library's expf, logf,
+ * lgammaf, cosf, sinf as the reference uses CUDA's.
+ *
+ * Parity pins (tests/test_oracle.py): closed forms of the conductance ODE,
+ * Heun amplification for constant conductances, spike-time interpolation,
+ * depC steady state, brute-force tiny networks, logistic identities
+ * f(0)=spont f(1)=1, Poisson time rescaling, TW against gamma pdfs, SW sum,
+ * uniform-frame zero contrast.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../include/vc_math.h"
+#include "../include/vc_abi.h"
+
+#define ORC_VE 0.0f     /* src/CONST.h:7-9 */
+#define ORC_VI (-75.0f)
+#define ORC_VL (-65.0f)
+#define ORC_SQRT2 1.4142135623730951   /* src/CONST.h:32, a double */
+
+/* ---------------------------------------------------------------------- */
+/* reference block_reduce over nThreads (a power of two, >= 32); x[i] for  */
+/* i >= n contribute 0.                                                    */
+static float orc_block_reduce(const float *x, int n, int nThreads) {
+    float part[32];
+    int W = nThreads / 32;
+    for (int w = 0; w < W; ++w) {
+        float a[32];
+        for (int l = 0; l < 32; ++l) {
+            int i = w * 32 + l;
+            a[l] = (i < n) ? x[i] : 0.0f;
+        }
+        for (int off = 16; off > 0; off >>= 1)
+            for (int l = 0; l < off; ++l) a[l] = a[l] + a[l + off];
+        part[w] = a[0];
+    }
+    for (int off = W / 2; off > 0; off >>= 1)
+        for (int l = 0; l < off; ++l) part[l] = part[l] + part[l + off];
+    return part[0];
+}
+
+float orc_block_reduce_export(const float *x, int n, int nThreads) {
+    return orc_block_reduce(x, n, nThreads);
+}
+
+/* ---------------------------------------------------------------------- */
+/* ConductanceShape, src/condShape.h:28-41                                 */
+static inline void orc_single_input(float *g, float *h, float f, float ddt,
+                                    float rise, float decay, float dod) {
+    float etr = vc_expf(-ddt / rise);
+    *g = *g + f * dod * (vc_expf(-ddt / decay) - etr);
+    *h = *h + f * etr;
+}
+static inline void orc_decay(float *g, float *h, float ddt, float rise,
+                             float decay, float dod) {
+    float etr = vc_expf(-ddt / rise);
+    float etd = vc_expf(-ddt / decay);
+    *g = *g * etd + *h * dod * (etd - etr);
+    *h = *h * etr;
+}
+
+/* exported for the closed-form tests */
+void orc_cond_single(float *g, float *h, float f, float ddt, float rise,
+                     float decay) {
+    float dod = rise / (decay - rise);
+    orc_single_input(g, h, f, ddt, rise, decay, dod);
+}
+void orc_cond_decay(float *g, float *h, float ddt, float rise, float decay) {
+    float dod = rise / (decay - rise);
+    orc_decay(g, h, ddt, rise, decay, dod);
+}
+
+float orc_expf(float x) { return vc_expf(x); }
+float orc_logf(float x) { return vc_logf(x); }
+void orc_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
+    vc_u32x4 c;
+    for (int i = 0; i < 4; ++i) c.v[i] = ctr[i];
+    c = vc_philox4x32_10(c, k0, k1);
+    for (int i = 0; i < 4; ++i) out[i] = c.v[i];
+}
+
+/* ====================================================================== */
+/* V1                                                                      */
+/* ====================================================================== */
+typedef struct {
+    float v, v0, a0, b0, a1, b1, vR, vThres, tRef, tBack, tsp, C, gL, depC;
+    float w0, w, tau_w, a, b, vT, deltaT;
+    int spiked;
+} orc_adex;
+
+/* AdEx constructor, src/coredynamics.cuh:40-49.  deltaT == 0 is the LIF
+ * extension: the exponential current is absent. */
+static void orc_adex_init(orc_adex *m, const vc_v1_params *p, int t, float w0,
+                          float v0, float tBack, float dep) {
+    m->w0 = w0; m->tau_w = p->tau_w[t]; m->a = p->a[t]; m->b = p->b[t];
+    m->v0 = v0; m->tBack = tBack; m->vR = p->vR[t]; m->vThres = p->vThres[t];
+    m->gL = p->gL[t]; m->C = p->C[t]; m->tRef = p->tRef[t]; m->vT = p->vT[t];
+    m->deltaT = p->deltaT[t];
+    m->spiked = 0;
+    m->v = v0; m->w = w0; m->tsp = 0.0f;
+    float targetV = ORC_VL + (m->vT - ORC_VL) * dep;
+    if (targetV > m->vT) {
+        m->depC = (m->gL + m->a) * (targetV - ORC_VL) - m->gL * m->deltaT;
+    } else if (m->deltaT > 0.0f) {
+        m->depC = (m->gL + m->a) * (targetV - ORC_VL) -
+                  m->gL * m->deltaT * vc_expf((targetV - m->vT) / m->deltaT);
+    } else {
+        m->depC = (m->gL + m->a) * (targetV - ORC_VL) - 0.0f;
+    }
+}
+
+/* src/coredynamics.cuh:75-87 with noise0 = noise1 = 0 */
+static void orc_rk2(orc_adex *m, float dt) {
+    float dTgL = m->deltaT * m->gL;
+    float fk1, fk2;
+    if (m->deltaT > 0.0f)
+        fk1 = (-m->a0 * m->v0 + m->b0 + dTgL * vc_expf((m->v0 - m->vT) / m->deltaT) - m->w0) / m->C;
+    else
+        fk1 = (-m->a0 * m->v0 + m->b0 - m->w0) / m->C;
+    float gk1 = (m->a * (m->v0 - ORC_VL) - m->w0) / m->tau_w;
+    float v1 = m->v0 + dt * fk1;
+    float w1 = m->w0 + dt * gk1;
+    if (m->deltaT > 0.0f)
+        fk2 = (-m->a1 * v1 + m->b1 + dTgL * vc_expf((v1 - m->vT) / m->deltaT) - w1) / m->C;
+    else
+        fk2 = (-m->a1 * v1 + m->b1 - w1) / m->C;
+    float gk2 = (m->a * (v1 - ORC_VL) - w1) / m->tau_w;
+    m->w = m->w0 + dt * (gk1 + gk2) / 2.0f;
+    m->v = m->v0 + dt * (fk1 + fk2) / 2.0f;
+}
+
+static inline int orc_type_of(const vc_v1_params *p, uint32_t local) {
+    for (uint32_t j = 0; j < p->nType; ++j)
+        if (local < p->typeAcc[j]) return (int)j;
+    return (int)p->nType - 1;
+}
+
+/* Background drive (extension): one Philox draw per neuron-step gives the
+ * Poisson count (inversion) and the event time. */
+static inline uint32_t orc_bg_draw(const vc_v1_params *p, int t, uint32_t n,
+                                   uint32_t step, float *frac) {
+    vc_u32x4 r = vc_draw4(p->seed_lo, p->seed_hi, n, step, VC_STREAM_V1_BG, 0,
+                          p->trial);
+    float u = vc_u01_open0(r.v[0]);
+    *frac = vc_u01_open1(r.v[1]);
+    return vc_poisson_inv(u, p->bgLambda[t], p->bgP0[t]);
+}
+
+/* One V1 neuron, src/coredynamics.cu:1296-1629.  Returns sInfo. */
+static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
+                               float *w, float *tBack, float *gFF, float *hFF,
+                               float *gE, float *hE, float *gI, float *hI,
+                               const float *tonicDep, const uint32_t *nLGN,
+                               const float *sLGN, const uint32_t *idLGN,
+                               const float *lgnSInfo, uint32_t step,
+                               float *out_dep) {
+    const uint32_t nV1 = p->nV1;
+    const uint32_t local = n % p->blockSize;
+    const int t = orc_type_of(p, local);
+    const float dt = p->dt;
+    orc_adex m;
+    orc_adex_init(&m, p, t, w[n], v[n], tBack[n], tonicDep[n]);
+
+    float ge[VC_MAX_NG], he[VC_MAX_NG], gi[VC_MAX_NG], hi[VC_MAX_NG];
+    float gE_t1 = 0.0f, gI_t1 = 0.0f;
+    for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+        float g = gE[ig * nV1 + n], h = hE[ig * nV1 + n];
+        ge[ig] = g; he[ig] = h;
+        orc_decay(&g, &h, dt, p->riseE[ig], p->decayE[ig], p->dodE[ig]);
+        gE[ig * nV1 + n] = g; hE[ig * nV1 + n] = h;
+        gE_t1 += g;
+    }
+    for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+        float g = gI[ig * nV1 + n], h = hI[ig * nV1 + n];
+        gi[ig] = g; hi[ig] = h;
+        orc_decay(&g, &h, dt, p->riseI[ig], p->decayI[ig], p->dodI[ig]);
+        gI[ig * nV1 + n] = g; hI[ig * nV1 + n] = h;
+        gI_t1 += g;
+    }
+    float g0[VC_MAX_NG], h0[VC_MAX_NG], g1[VC_MAX_NG], h1[VC_MAX_NG];
+    for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
+        g1[ig] = gFF[ig * nV1 + n]; h1[ig] = hFF[ig * nV1 + n];
+        g0[ig] = g1[ig]; h0[ig] = h1[ig];
+        orc_decay(&g1[ig], &h1[ig], dt, p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+    }
+    const int backing = (m.tBack < dt) && (m.tBack > 0.0f);
+    if (out_dep) out_dep[n] = m.depC;
+    if (backing)
+        for (uint32_t ig = 0; ig < p->ngFF; ++ig)
+            orc_decay(&g0[ig], &h0[ig], m.tBack, p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+
+    const uint32_t mLGN = nLGN[n];
+    for (uint32_t i = 0; i < mLGN; ++i) {
+        uint32_t lid = i * nV1 + n;
+        float f = sLGN[lid];
+        float s = lgnSInfo[idLGN[lid]];
+        uint32_t nsp = (uint32_t)floorf(s);
+        if (nsp > 0) {
+            float tsp = (s - (float)nsp) * dt;
+            if (f > 0.0f) {
+                float ddt = 0.0f;
+                if (backing) ddt = m.tBack - tsp;
+                for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
+                    float str = f * p->pFF[t * p->ngFF + ig];
+                    if (backing && ddt > 0.0f)
+                        orc_single_input(&g0[ig], &h0[ig], str * (float)nsp, ddt,
+                                         p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+                    orc_single_input(&g1[ig], &h1[ig], str * (float)nsp, dt - tsp,
+                                     p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+                }
+            }
+        }
+    }
+    if (p->bgStrength[t] > 0.0f) {
+        float frac;
+        uint32_t nsp = orc_bg_draw(p, t, n, step, &frac);
+        if (nsp > 0) {
+            float tsp = frac * dt;
+            float f = p->bgStrength[t];
+            float ddt = 0.0f;
+            if (backing) ddt = m.tBack - tsp;
+            for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
+                float str = f * p->pBG[t * p->ngFF + ig];
+                if (backing && ddt > 0.0f)
+                    orc_single_input(&g0[ig], &h0[ig], str * (float)nsp, ddt,
+                                     p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+                orc_single_input(&g1[ig], &h1[ig], str * (float)nsp, dt - tsp,
+                                 p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+            }
+        }
+    }
+    float gE_t0 = 0.0f;
+    for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
+        gE_t0 += g0[ig];
+        gE_t1 += g1[ig];
+        gFF[ig * nV1 + n] = g1[ig];
+        hFF[ig * nV1 + n] = h1[ig];
+    }
+    for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+        if (backing) orc_decay(&ge[ig], &he[ig], m.tBack, p->riseE[ig], p->decayE[ig], p->dodE[ig]);
+        gE_t0 += ge[ig];
+    }
+    float gI_t0 = 0.0f;
+    for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+        if (backing) orc_decay(&gi[ig], &hi[ig], m.tBack, p->riseI[ig], p->decayI[ig], p->dodI[ig]);
+        gI_t0 += gi[ig];
+    }
+
+    m.tsp = 0.0f;
+    if (m.tBack < dt) {
+        m.a0 = gE_t0 + gI_t0 + m.gL + 0.0f;
+        m.b0 = gE_t0 * ORC_VE + gI_t0 * ORC_VI + m.gL * ORC_VL + m.depC + 0.0f;
+        m.a1 = gE_t1 + gI_t1 + m.gL + 0.0f;
+        m.b1 = gE_t1 * ORC_VE + gI_t1 * ORC_VI + m.gL * ORC_VL + m.depC + 0.0f;
+        float new_dt = dt - m.tBack;
+        if (backing) {
+            float A = m.a * (m.v0 - ORC_VL);
+            m.w0 = (m.w0 - A) * vc_expf(-m.tBack / m.tau_w) + A;
+        }
+        orc_rk2(&m, new_dt);
+        if (m.v > m.vThres) {
+            float ratio = (m.vThres - m.v0) / (m.v - m.v0);
+            m.tsp = m.tBack + ratio * new_dt;
+            m.w0 += ratio * (m.w - m.w0) + m.b;
+            m.spiked = 1;
+            m.tBack = m.tsp + m.tRef;
+            m.v = m.vR;
+        } else {
+            if (m.tBack > 0.0f) m.tBack = 0.0f;
+        }
+    }
+    if (m.tBack >= dt) {
+        m.v = m.vR;
+        float A = m.a * (m.v - ORC_VL);
+        m.w = (m.w0 - A) * vc_expf(-(dt - m.tsp) / m.tau_w) + A;
+        m.tBack -= dt;
+    }
+    float sInfo = m.spiked ? 1.0f + m.tsp / dt : m.v;
+    v[n] = m.v;
+    if (p->iModel == 1) w[n] = m.w;
+    tBack[n] = m.tBack;
+    return sInfo;
+}
+
+/*
+ * One V1 step in the reference's order: the update kernel (c-FF, d, e) for
+ * every neuron, then recal_G for the spikes of this step.
+ *   ipre [nblock*bs], npre [2*nblock], og/oh [nblock*(ngE*nE + ngI*nI)]:
+ *   the reference's spike lists and unit increments (E list first).
+ *   conMat: dense [nblock][maxNear][bs][bs].
+ * If do_rec is 0 the recurrent increments are not applied (the lists are
+ * still produced).
+ */
+int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
+                float *gFF, float *hFF, float *gE, float *hE, float *gI,
+                float *hI, const float *tonicDep, const uint32_t *nLGN,
+                const float *sLGN, const uint32_t *idLGN,
+                const float *lgnSInfo, uint32_t maxNear, const uint32_t *nNear,
+                const uint32_t *nbr, const float *conMat, uint32_t *ipre,
+                uint32_t *npre, float *og, float *oh, uint32_t step,
+                float *out_sInfo, float *out_dep, int do_rec) {
+    const uint32_t nV1 = p->nV1, bs = p->blockSize, nblock = p->nblock;
+    const uint32_t nE = p->nE, nI = bs - nE;
+    if (nV1 != nblock * bs) return VC_E_INVALID;
+    float *sInfo = out_sInfo;
+    int own = 0;
+    if (!sInfo) { sInfo = (float *)malloc(sizeof(float) * nV1); own = 1; }
+
+#pragma omp parallel for schedule(static)
+    for (uint32_t n = 0; n < nV1; ++n) {
+        sInfo[n] = orc_update_neuron(p, n, v, w, tBack, gFF, hFF, gE, hE, gI,
+                                     hI, tonicDep, nLGN, sLGN, idLGN, lgnSInfo,
+                                     step, out_dep);
+    }
+    /* compaction, src/coredynamics.cu:1994-2046, ascending order */
+    const uint32_t block_ng = p->ngE * nE + p->ngI * nI;
+#pragma omp parallel for schedule(static)
+    for (uint32_t b = 0; b < nblock; ++b) {
+        uint32_t cE = 0, cI = 0;
+        for (uint32_t l = 0; l < bs; ++l) {
+            if (sInfo[b * bs + l] >= 1.0f) { if (l < nE) cE++; else cI++; }
+        }
+        npre[b] = cE; npre[nblock + b] = cI;
+        uint32_t iE = 0, iI = 0;
+        for (uint32_t l = 0; l < bs; ++l) {
+            float s = sInfo[b * bs + l];
+            if (!(s >= 1.0f)) continue;
+            float nsp = floorf(s);
+            float tsp = (s - nsp) * p->dt;
+            uint32_t i = block_ng * b;
+            if (l < nE) {
+                i += iE;
+                for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+                    float lg = 0.0f, lh = 0.0f;
+                    orc_single_input(&lg, &lh, nsp, p->dt - tsp, p->riseE[ig], p->decayE[ig], p->dodE[ig]);
+                    og[ig * cE + i] = lg; oh[ig * cE + i] = lh;
+                }
+                ipre[b * bs + iE] = l;
+                iE++;
+            } else {
+                i += cE * p->ngE + iI;
+                for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+                    float lg = 0.0f, lh = 0.0f;
+                    orc_single_input(&lg, &lh, nsp, p->dt - tsp, p->riseI[ig], p->decayI[ig], p->dodI[ig]);
+                    og[ig * cI + i] = lg; oh[ig * cI + i] = lh;
+                }
+                ipre[b * bs + cE + iI] = l;
+                iI++;
+            }
+        }
+    }
+    if (do_rec) {
+        /* recal_G_mat_nd_fast, src/coredynamics.cu:2721-2916 */
+#pragma omp parallel for schedule(static)
+        for (uint32_t n = 0; n < nV1; ++n) {
+            uint32_t b = n / bs, l = n % bs;
+            int t = orc_type_of(p, l);
+            float lgE[VC_MAX_NG] = {0}, lhE[VC_MAX_NG] = {0};
+            float lgI[VC_MAX_NG] = {0}, lhI[VC_MAX_NG] = {0};
+            for (uint32_t ib = 0; ib < nNear[b]; ++ib) {
+                uint32_t bid = nbr[b * maxNear + ib];
+                uint32_t cE = npre[bid], cI = npre[nblock + bid];
+                uint64_t local_bid = (uint64_t)b * maxNear + ib;
+                for (uint32_t i = 0; i < cE; ++i) {
+                    uint32_t pre = ipre[bid * bs + i];
+                    float strength = conMat[(local_bid * bs + pre) * bs + l];
+                    if (strength > 0.0f) {
+                        for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+                            float str = strength * p->pE[t * p->ngE + ig];
+                            uint32_t id = block_ng * bid + i;
+                            lgE[ig] += str * og[ig * cE + id];
+                            lhE[ig] += str * oh[ig * cE + id];
+                        }
+                    }
+                }
+                for (uint32_t i = 0; i < cI; ++i) {
+                    uint32_t pre = ipre[bid * bs + cE + i];
+                    float strength = conMat[(local_bid * bs + pre) * bs + l];
+                    if (strength > 0.0f) {
+                        for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+                            float str = strength * p->pI[t * p->ngI + ig];
+                            uint32_t id = block_ng * bid + cE * p->ngE + i;
+                            lgI[ig] += str * og[ig * cI + id];
+                            lhI[ig] += str * oh[ig * cI + id];
+                        }
+                    }
+                }
+            }
+            for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+                gE[ig * nV1 + n] += lgE[ig];
+                hE[ig * nV1 + n] += lhE[ig];
+            }
+            for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+                gI[ig * nV1 + n] += lgI[ig];
+                hI[ig * nV1 + n] += lhI[ig];
+            }
+        }
+    }
+    if (own) free(sInfo);
+    return VC_OK;
+}
+
+/* ====================================================================== */
+/* LGN                                                                     */
+/* ====================================================================== */
+
+/* Exact-float bilinear filtering of one channel with border addressing
+ * (0 outside) and normalised coordinates, the semantics of
+ * tex2DLayered with cudaFilterModeLinear / cudaAddressModeBorder
+ * (src/patch.h:95-109) restated in binary32 arithmetic. */
+static inline float orc_texel(const float *img, int W, int H, int i, int j) {
+    if (i < 0 || j < 0 || i >= W || j >= H) return 0.0f;
+    return img[(size_t)j * W + i];
+}
+static float orc_bilinear(const float *img, int W, int H, float x, float y) {
+    float xt = x * (float)W - 0.5f;
+    float yt = y * (float)H - 0.5f;
+    if (!(xt > -2.0f && xt < (float)W + 1.0f && yt > -2.0f && yt < (float)H + 1.0f))
+        return 0.0f;
+    float fx = floorf(xt), fy = floorf(yt);
+    float ax = xt - fx, ay = yt - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    float t00 = orc_texel(img, W, H, i0, j0);
+    float t10 = orc_texel(img, W, H, i0 + 1, j0);
+    float t01 = orc_texel(img, W, H, i0, j0 + 1);
+    float t11 = orc_texel(img, W, H, i0 + 1, j0 + 1);
+    float bx = 1.0f - ax, by = 1.0f - ay;
+    return by * (bx * t00 + ax * t10) + ay * (bx * t01 + ax * t11);
+}
+float orc_bilinear_export(const float *img, int W, int H, float x, float y) {
+    return orc_bilinear(img, W, H, x, y);
+}
+
+/* get_intensity, src/discrete_input_convol.cu:35-76 */
+static float orc_intensity(uint32_t cone, const float *frame, int W, int H,
+                           float x, float y) {
+    size_t np = (size_t)W * H;
+    switch (cone) {
+    case 0: return orc_bilinear(frame, W, H, x, y);
+    case 1: return orc_bilinear(frame + np, W, H, x, y);
+    case 2: return orc_bilinear(frame + 2 * np, W, H, x, y);
+    case 3: return (orc_bilinear(frame, W, H, x, y) + orc_bilinear(frame + np, W, H, x, y)) / 2.0f;
+    case 4: return orc_bilinear(frame, W, H, x, y) * 0.45436904f +
+                   orc_bilinear(frame + np, W, H, x, y) * 0.47353315f +
+                   orc_bilinear(frame + 2 * np, W, H, x, y) * 0.0720978f;
+    default: return 0.0f;
+    }
+}
+
+static inline float orc_clip_contrast(float I, float mean) {
+    float c = (mean > 0.0f) ? I / mean - 1.0f : I;
+    if (fabsf(c) > 1.0f) c = copysignf(1.0f, c);
+    return c;
+}
+
+void orc_lgn_frame_response(const float *frame, uint32_t W, uint32_t H,
+                            uint32_t nLGN, uint32_t nSample1D, const float *SCx,
+                            const float *SCy, const float *SW,
+                            const uint32_t *coneType, float *R) {
+    const int nS = (int)(nSample1D * nSample1D);
+#pragma omp parallel for schedule(static)
+    for (uint32_t c = 0; c < nLGN; ++c) {
+        float *a = (float *)malloc(sizeof(float) * nS * 4);
+        float *bS = a + nS, *fC = a + 2 * nS, *fS = a + 3 * nS;
+        uint32_t tC = coneType[c], tS = coneType[nLGN + c];
+        const float *xC = SCx + ((size_t)c * 2 + 0) * nS, *yC = SCy + ((size_t)c * 2 + 0) * nS;
+        const float *xS = SCx + ((size_t)c * 2 + 1) * nS, *yS = SCy + ((size_t)c * 2 + 1) * nS;
+        for (int s = 0; s < nS; ++s) {
+            a[s] = orc_intensity(tC, frame, (int)W, (int)H, xS[s], yS[s]);
+            bS[s] = orc_intensity(tS, frame, (int)W, (int)H, xS[s], yS[s]);
+        }
+        float meanC = orc_block_reduce(a, nS, nS) / (float)nS;
+        float meanS = orc_block_reduce(bS, nS, nS) / (float)nS;
+        for (int s = 0; s < nS; ++s) {
+            float I = orc_intensity(tC, frame, (int)W, (int)H, xC[s], yC[s]);
+            fC[s] = SW[s] * orc_clip_contrast(I, meanC);
+            fS[s] = SW[s] * orc_clip_contrast(bS[s], meanS);
+        }
+        R[c * 2 + 0] = orc_block_reduce(fC, nS, nS);
+        R[c * 2 + 1] = orc_block_reduce(fS, nS, nS);
+        free(a);
+    }
+}
+
+static inline int64_t orc_floor_div(int64_t a, int64_t b) {
+    int64_t q = a / b;
+    if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
+    return q;
+}
+
+/* src/LGN_props.cuh:94-100; 1.0 is a double literal in the reference */
+static inline float orc_transform(float C50, float K, float A, float B,
+                                  float input) {
+    float X = (float)(1.0 / (double)(1.0f + vc_expf(K * (C50 - input))));
+    return A * X + B;
+}
+
+/* one LGN cell, one step: returns sInfo */
+static float orc_lgn_cell_step(const vc_lgn_params *p, const float *TW,
+                               const float *kC, const float *kS,
+                               const float *maxConvol, const float *c50,
+                               const float *K, const float *A, const float *B,
+                               float *leftTimeRate, float *lastNegLogRand,
+                               const float *Rring, uint32_t c, uint32_t it,
+                               float *fr_out, float *convol_out) {
+    const uint32_t nKS = p->nKernelSample;
+    float prod[2][1024];
+    for (int r = 0; r < 2; ++r) {
+        for (uint32_t i = 0; i < nKS; ++i) {
+            int64_t tau = (int64_t)it + 1 - (int64_t)p->nRetrace +
+                          (int64_t)p->iKernelSampleT0 + (int64_t)i * p->kernelSampleInterval;
+            int64_t f = orc_floor_div(tau, (int64_t)p->ntPerFrame);
+            int64_t slot = f % (int64_t)p->maxFrame;
+            if (slot < 0) slot += p->maxFrame;
+            float Rv = Rring[((size_t)slot * p->nLGN + c) * 2 + r];
+            prod[r][i] = Rv * TW[((size_t)c * 2 + r) * nKS + i];
+        }
+    }
+    /* the reference sums the taps with block_reduce2 over a patch of nSample
+     * threads; nSample >= 64 >= nKS here, so the patch is one tree. */
+    float convolC = 0.0f + orc_block_reduce(prod[0], (int)nKS, 256);
+    float convolS = 0.0f + orc_block_reduce(prod[1], (int)nKS, 256);
+    convolC *= kC[c];
+    convolS *= kS[c];
+    float convol = convolC + convolS;
+    if (convol_out) *convol_out = convol;
+
+    float current = convol * p->convolRatio;
+    float mx = maxConvol[c];
+    if (current < 0.0f) current = 0.0f;
+    float fr = p->frRatio * mx * orc_transform(c50[c], K[c], A[c], B[c], current / mx);
+    if (fr_out) *fr_out = fr;
+
+    /* get_spike, src/discrete_input_convol.cu:1587-1622 */
+    float rate = fr / 1000.0f;
+    float lTR = leftTimeRate[c], lNL = lastNegLogRand[c];
+    float rT = p->dt * rate;
+    float next_rT = lNL - lTR;
+    uint32_t nsp = 0;
+    float tsp = 0.0f;
+    if (next_rT > rT) {
+        lTR += rT;
+    } else {
+        float old_rT = 0.0f;
+        uint32_t k = 0;
+        vc_u32x4 pack;
+        float tr = p->tRef * rate;
+        do {
+            if ((k & 3u) == 0u)
+                pack = vc_draw4(p->seed_lo, p->seed_hi, c, it, VC_STREAM_LGN_SPIKE,
+                                k >> 2, p->trial);
+            float u = vc_u01_open0(pack.v[k & 3u]);
+            k++;
+            float x = -vc_logf(u);
+            lNL = (x > tr) ? x : tr;
+            old_rT = next_rT;
+            next_rT += lNL;
+            nsp++;
+        } while (next_rT <= rT);
+        lTR = rT - old_rT;
+        tsp = old_rT / (rate * (float)nsp);
+    }
+    float sInfo = (float)nsp + tsp / p->dt;
+    if (sInfo > 0.0f) lastNegLogRand[c] = lNL;
+    leftTimeRate[c] = lTR;
+    return sInfo;
+}
+
+void orc_lgn_steps(const vc_lgn_params *p, const float *TW, const float *kC,
+                   const float *kS, const float *maxConvol, const float *c50,
+                   const float *K, const float *A, const float *B,
+                   float *leftTimeRate, float *lastNegLogRand,
+                   const float *Rring, uint32_t it0, uint32_t nsteps,
+                   float *sInfo, float *fr, float *convol) {
+    for (uint32_t s = 0; s < nsteps; ++s) {
+        uint32_t it = it0 + s;
+#pragma omp parallel for schedule(static)
+        for (uint32_t c = 0; c < p->nLGN; ++c) {
+            float frv, cv;
+            float si = orc_lgn_cell_step(p, TW, kC, kS, maxConvol, c50, K, A, B,
+                                         leftTimeRate, lastNegLogRand, Rring, c,
+                                         it, &frv, &cv);
+            sInfo[(size_t)s * p->nLGN + c] = si;
+            if (fr) fr[(size_t)s * p->nLGN + c] = frv;
+            if (convol) convol[(size_t)s * p->nLGN + c] = cv;
+        }
+    }
+}
+
+void orc_lgn_init(const vc_lgn_params *p, float *leftTimeRate,
+                  float *lastNegLogRand) {
+    for (uint32_t c = 0; c < p->nLGN; ++c) {
+        vc_u32x4 r = vc_draw4(p->seed_lo, p->seed_hi, c, 0xffffffffu,
+                              VC_STREAM_LGN_INIT, 0, p->trial);
+        leftTimeRate[c] = 0.0f;
+        lastNegLogRand[c] = -vc_logf(vc_u01_open0(r.v[0]));
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* LGN precompute (setup; C library transcendentals)                       */
+
+/* src/discrete_input_convol.cu:14-33 */
+static float orc_temporal_kernel(float tau, float tauR, float tauD, float ratio,
+                                 float nR, float nD, float lfac1, float lfac2) {
+    float tau1 = tau / tauR;
+    float tau2 = tau / tauD;
+    float A1 = (nR - 1) * logf(tau1);
+    float A2 = (nD - 1) * logf(tau2);
+    float exponent = A1 - tau1 - lfac1;
+    float tpR = expf(exponent) / tauR;
+    exponent = A2 - tau2 - lfac2;
+    float tpD = expf(exponent) / tauD;
+    return tpR - ratio * tpD;
+}
+
+static float orc_spatial_kernel(float x, float y, float rx, float ry) {
+    return expf(-x * x / (rx * rx) - y * y / (ry * ry));
+}
+
+void orc_lgn_precompute(const vc_lgn_setup *q, float *TW, float *SW,
+                        float *SCx, float *SCy, float *maxConvol) {
+    const uint32_t nLGN = q->nLGN, n1D = q->nSample1D, nS = n1D * n1D;
+    const uint32_t nKS = q->nKernelSample;
+    const float kernelSampleDt = (float)q->kernelSampleInterval * q->dt;
+    const float kernelSampleT0 = (float)q->iKernelSampleT0 * q->dt;
+    /* temporal weights, src/discrete_input_convol.cu:230-305 */
+#pragma omp parallel for schedule(static)
+    for (uint32_t c = 0; c < nLGN; ++c) {
+        float *absw = (float *)malloc(sizeof(float) * nS);
+        for (uint32_t r = 0; r < 2; ++r) {
+            uint32_t lid = r * nLGN + c;
+            float lfac1 = lgammaf(q->nR[lid]);
+            float lfac2 = lgammaf(q->nD[lid]);
+            float T0 = kernelSampleT0 - q->delay[lid];
+            float total = 0.0f;
+            float *tw = TW + ((size_t)c * 2 + r) * nKS;
+            uint32_t nPatch = (nKS + nS - 1) / nS;
+            for (uint32_t ip = 0; ip < nPatch; ++ip) {
+                uint32_t nAct = (ip + 1 < nPatch) ? nS : nKS - ip * nS;
+                for (uint32_t tid = 0; tid < nS; ++tid) {
+                    if (tid < nAct) {
+                        uint32_t j = ip * nS + tid;
+                        uint32_t twid = nKS - 1 - j;
+                        float t = (float)twid * kernelSampleDt + T0;
+                        float v = 0.0f;
+                        if (!(t < 0.0f))
+                            v = orc_temporal_kernel(t, q->tauR[lid], q->tauD[lid], q->ratio[lid],
+                                                    q->nR[lid], q->nD[lid], lfac1, lfac2);
+                        tw[j] = v;
+                        absw[tid] = fabsf(v);
+                    } else {
+                        absw[tid] = 0.0f;
+                    }
+                }
+                total += orc_block_reduce(absw, (int)nS, (int)nS);
+            }
+            for (uint32_t j = 0; j < nKS; ++j) tw[j] /= total;
+        }
+        free(absw);
+    }
+    /* shared spatial weights from cell 0 centre, :332-336 */
+    {
+        float rx = q->rw[0], ry = q->rh[0];
+        float wSpan = (float)((double)(q->nsig * rx) / ORC_SQRT2);
+        float dw = 2 * wSpan / (float)n1D;
+        float hSpan = (float)((double)(q->nsig * ry) / ORC_SQRT2);
+        float dh = 2 * hSpan / (float)n1D;
+        float *sw = (float *)malloc(sizeof(float) * nS);
+        for (uint32_t ty = 0; ty < n1D; ++ty)
+            for (uint32_t tx = 0; tx < n1D; ++tx) {
+                float w = (float)((tx + 0.5) * (double)dw - (double)wSpan);
+                float h = (float)((ty + 0.5) * (double)dh - (double)hSpan);
+                sw[ty * n1D + tx] = orc_spatial_kernel(w, h, rx, ry) * 1.0f;
+            }
+        float tot = orc_block_reduce(sw, (int)nS, (int)nS);
+        for (uint32_t s = 0; s < nS; ++s) SW[s] = sw[s] / tot;
+        free(sw);
+    }
+    /* sample coordinates, flat retina, :398-417 */
+#pragma omp parallel for schedule(static)
+    for (uint32_t c = 0; c < nLGN; ++c) {
+        for (uint32_t r = 0; r < 2; ++r) {
+            uint32_t lid = r * nLGN + c;
+            float rx = q->rw[lid], ry = q->rh[lid];
+            float wSpan = (float)((double)(q->nsig * rx) / ORC_SQRT2);
+            float dw = 2 * wSpan / (float)n1D;
+            float hSpan = (float)((double)(q->nsig * ry) / ORC_SQRT2);
+            float dh = 2 * hSpan / (float)n1D;
+            float polar = q->polar[lid], ecc = q->ecc[lid];
+            float coso = cosf(q->orient[lid]), sino = sinf(q->orient[lid]);
+            for (uint32_t ty = 0; ty < n1D; ++ty)
+                for (uint32_t tx = 0; tx < n1D; ++tx) {
+                    float w = (float)((tx + 0.5) * (double)dw - (double)wSpan);
+                    float h = (float)((ty + 0.5) * (double)dh - (double)hSpan);
+                    float x = q->x0 + (ecc * cosf(polar) + w * coso - h * sino) * q->normViewDistance;
+                    float y = q->y0 + (ecc * sinf(polar) + w * sino + h * coso) * q->normViewDistance;
+                    size_t id = ((size_t)c * 2 + r) * nS + ty * n1D + tx;
+                    SCx[id] = x;
+                    SCy[id] = y;
+                }
+        }
+    }
+    /* maxConvol, src/discrete_input_convol.cu:746-881, 1024 threads */
+    const uint32_t m1D = q->maxSample1D, mS = m1D * m1D;
+    const uint32_t nPatch = mS / 1024;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t c = 0; c < nLGN; ++c) {
+        float *swC = (float *)malloc(sizeof(float) * mS * 2);
+        float *swS = swC + mS;
+        float buf[1024];
+        float rxs = q->rw[nLGN + c], rys = q->rh[nLGN + c];
+        float xs = q->polar[nLGN + c], ys = q->ecc[nLGN + c];
+        float wSpan = (float)((double)(q->nsig * rxs) / ORC_SQRT2);
+        float hSpan = (float)((double)(q->nsig * rys) / ORC_SQRT2);
+        float rxc = q->rw[c], ryc = q->rh[c];
+        float R = wSpan > hSpan ? wSpan : hSpan;
+        float xc = q->polar[c], yc = q->ecc[c];
+        wSpan = (float)((double)(q->nsig * rxc) / ORC_SQRT2);
+        hSpan = (float)((double)(q->nsig * ryc) / ORC_SQRT2);
+        float dx = yc * cosf(xc) - ys * cosf(xs);
+        float dy = yc * sinf(xc) - ys * sinf(xs);
+        float r = sqrtf(dx * dx + dy * dy);
+        float cov = q->covariant[c];
+        float orients = q->orient[nLGN + c], orientc = q->orient[c];
+        r += wSpan > hSpan ? wSpan : hSpan;
+        float coss = cosf(orients);
+        R = r > R ? r : R;
+        float sins = sinf(orients);
+        float cosc = cosf(orientc), sinc = sinf(orientc);
+        float ks = q->k[nLGN + c], kc = q->k[c];
+        float ds = 2 * R / (float)m1D;
+        float sumC = 0.0f, sumS = 0.0f;
+        for (uint32_t ip = 0; ip < nPatch; ++ip) {
+            for (uint32_t t = 0; t < 1024; ++t) {
+                uint32_t pid = ip * 1024 + t;
+                uint32_t wi = pid % m1D, hi = pid / m1D;
+                float lx = (float)((wi + 0.5) * (double)ds - (double)R);
+                float ly = (float)((hi + 0.5) * (double)ds - (double)R);
+                float x = cosc * (lx - dx) + sinc * (ly - dy);
+                float y = -sinc * (lx - dx) + cosc * (ly - dy);
+                swC[pid] = orc_spatial_kernel(x, y, rxc, ryc);
+            }
+            sumC += orc_block_reduce(swC + ip * 1024, 1024, 1024);
+            for (uint32_t t = 0; t < 1024; ++t) {
+                uint32_t pid = ip * 1024 + t;
+                uint32_t wi = pid % m1D, hi = pid / m1D;
+                float lx = (float)((wi + 0.5) * (double)ds - (double)R);
+                float ly = (float)((hi + 0.5) * (double)ds - (double)R);
+                float x = coss * lx + sins * ly;
+                float y = -sins * lx + coss * ly;
+                swS[pid] = orc_spatial_kernel(x, y, rxs, rys);
+            }
+            sumS += orc_block_reduce(swS + ip * 1024, 1024, 1024);
+        }
+        for (uint32_t pid = 0; pid < mS; ++pid) { swC[pid] /= sumC; swS[pid] /= sumS; }
+        float convol = 0.0f;
+        for (uint32_t it = 0; it < nKS; ++it) {
+            float tc = TW[(size_t)c * 2 * nKS + it];
+            float ts = TW[((size_t)c * 2 + 1) * nKS + it];
+            for (uint32_t ip = 0; ip < nPatch; ++ip) {
+                for (uint32_t t = 0; t < 1024; ++t) {
+                    uint32_t pid = ip * 1024 + t;
+                    float lc = swC[pid] * tc * kc;
+                    float ls = swS[pid] * ts * ks;
+                    float d;
+                    if (fabsf(lc) > fabsf(ls)) d = fabsf(lc) + ls * copysignf(cov, lc);
+                    else d = fabsf(ls) + lc * copysignf(cov, ls);
+                    buf[t] = d;
+                }
+                convol += orc_block_reduce(buf, 1024, 1024);
+            }
+        }
+        maxConvol[c] = convol;
+        free(swC);
+    }
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    extern int omp_get_max_threads(void);
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

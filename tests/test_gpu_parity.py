@@ -1,0 +1,72 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Integer results (spike counts, spike steps, compaction)
+must be identical; with identical LGN tables the whole per-step path is
+bit-identical by construction (same binary32 operation sequences), which is
+what these tests assert.  The one-time LGN tables use the platform
+transcendentals and are compared with a relative tolerance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def tiny(pkg):
+    from visual_cortex_amd import network
+    return network.build_network(network.spec_tiny())
+
+
+def test_lgn_tables_match_oracle(pkg, tiny):
+    import oracle
+    from visual_cortex_amd.sim import GpuSim
+    _flush("test_lgn_tables_match_oracle: tiny spec")
+    g = GpuSim(tiny)
+    o = oracle.OracleSim(tiny, pkg)
+    for k, gt in (("TW", g.TW), ("SW", g.SW), ("SCx", g.SCx), ("SCy", g.SCy), ("maxConvol", g.maxConvol)):
+        a = gt.cpu().numpy()
+        b = o.tables[k]
+        np.testing.assert_allclose(a, b, rtol=2e-6, atol=2e-7, err_msg=k)
+
+
+def test_frame_response_bitwise(pkg, tiny):
+    import oracle
+    from visual_cortex_amd.sim import GpuSim
+    _flush("test_frame_response_bitwise: tiny spec")
+    g = GpuSim(tiny)
+    tables = dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
+                  SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
+    o = oracle.OracleSim(tiny, pkg, tables=tables)
+    for f in range(3):
+        g._frame_response(g.frames[f], 1)
+        Rg = g.Rring[1].cpu().numpy()
+        Ro = o.frame_response(tiny.frames[f])
+        assert np.array_equal(Rg.view(np.uint32), Ro.view(np.uint32)), f
+
+
+@pytest.mark.parametrize("nsteps", [400])
+def test_full_path_bitwise_tiny(pkg, tiny, nsteps):
+    import oracle
+    from visual_cortex_amd.sim import GpuSim
+    _flush(f"test_full_path_bitwise_tiny: {nsteps} steps")
+    g = GpuSim(tiny)
+    tables = dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
+                  SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
+    o = oracle.OracleSim(tiny, pkg, tables=tables)
+    g.run(nsteps)
+    o.run(nsteps)
+    lg, lo = g.lgn_sinfo(), o.lgn_sinfo()
+    assert np.array_equal(lg.view(np.uint32), lo.view(np.uint32)), "LGN sInfo differs"
+    sg, so = g.sInfo(), o.sInfo()
+    nsp_g, nsp_o = (sg >= 1).sum(), (so >= 1).sum()
+    _flush(f"  V1 spikes gpu={nsp_g} oracle={nsp_o}")
+    assert nsp_o > 0, "no V1 spikes: the test would not exercise the recurrent path"
+    diff = np.argwhere(sg.view(np.uint32) != so.view(np.uint32))
+    assert diff.size == 0, f"first sInfo difference at (step, neuron) {diff[0]}"
+    g.finalize()
+    sgs = g.state_numpy()
+    for k in ("v", "w", "tBack", "gFF", "hFF", "gE", "hE", "gI", "hI"):
+        assert np.array_equal(sgs[k].view(np.uint32), o.st[k].view(np.uint32)), k

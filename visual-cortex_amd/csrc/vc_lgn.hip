@@ -1,0 +1,517 @@
+// vc_lgn.hip -- stages a and b of the simulation step (LGN) on gfx950.
+//
+//  k_lgn_frame_response  once per stimulus frame: one workgroup per LGN cell,
+//      one thread per spatial sample.  Bilinear samples of the frame at the
+//      cell's centre/surround sample coordinates, mean over the surround
+//      samples, clipped contrast, weighted spatial sums R_C, R_S.  The sums
+//      use the reference's block_reduce tree (32-lane halving, then halving
+//      over the warp partials), so R is bitwise the reference formulation's
+//      per-frame value (src/discrete_input_convol.cu:1003-1069, 1143-1211).
+//  k_lgn_steps  for a run of steps: one thread per cell keeps the cell's
+//      temporal weights in registers, forms tap x frame products from the
+//      cached per-frame responses (the separable form: R depends only on the
+//      frame, so it is not recomputed per step as the reference does), sums
+//      them with the same tree, then the logistic rate and the time-rescaled
+//      Poisson spike draw (src/discrete_input_convol.cu:1587-1821).
+//  k_lgn_tw / k_lgn_sw / k_lgn_sc / k_lgn_maxconvol  one-time tables
+//      (store_PM, parvo_maxConvol).
+//
+// MFMA is not used: every contraction here is a per-cell dot product over
+// that cell's own irregular sample set (no operand is shared across cells),
+// i.e. a batched GEMV, not a GEMM.
+#include "vc_common.h"
+#include <math.h>
+
+namespace {
+
+constexpr double kSqrt2 = 1.4142135623730951;  // src/CONST.h:32 (double)
+
+// block_reduce of src/util/cuda_util.cuh:89-101 for blockDim.x threads (a
+// power of two >= 32).  Every thread gets the result.
+__device__ float block_reduce_ref(float x, float *part /* >= 32 floats */) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t W = blockDim.x / 32u;
+    x = vc_tree32(x);
+    __syncthreads();
+    if ((tid & 31u) == 0u) part[tid >> 5] = x;
+    __syncthreads();
+    if (tid == 0) {
+        for (uint32_t off = W / 2; off > 0; off >>= 1)
+            for (uint32_t l = 0; l < off; ++l) part[l] = part[l] + part[l + off];
+    }
+    __syncthreads();
+    float r = part[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ float texel(const float *img, int W, int H, int i, int j) {
+    if (i < 0 || j < 0 || i >= W || j >= H) return 0.0f;
+    return img[(size_t)j * W + i];
+}
+
+// exact-float bilinear with border addressing and normalised coordinates
+__device__ __forceinline__ float bilinear(const float *img, int W, int H, float x, float y) {
+    float xt = x * (float)W - 0.5f;
+    float yt = y * (float)H - 0.5f;
+    if (!(xt > -2.0f && xt < (float)W + 1.0f && yt > -2.0f && yt < (float)H + 1.0f))
+        return 0.0f;
+    float fx = floorf(xt), fy = floorf(yt);
+    float ax = xt - fx, ay = yt - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    float t00 = texel(img, W, H, i0, j0);
+    float t10 = texel(img, W, H, i0 + 1, j0);
+    float t01 = texel(img, W, H, i0, j0 + 1);
+    float t11 = texel(img, W, H, i0 + 1, j0 + 1);
+    float bx = 1.0f - ax, by = 1.0f - ay;
+    return by * (bx * t00 + ax * t10) + ay * (bx * t01 + ax * t11);
+}
+
+// get_intensity, src/discrete_input_convol.cu:35-76
+__device__ float intensity(uint32_t cone, const float *frame, int W, int H, float x, float y) {
+    const size_t np = (size_t)W * H;
+    switch (cone) {
+    case 0: return bilinear(frame, W, H, x, y);
+    case 1: return bilinear(frame + np, W, H, x, y);
+    case 2: return bilinear(frame + 2 * np, W, H, x, y);
+    case 3: return (bilinear(frame, W, H, x, y) + bilinear(frame + np, W, H, x, y)) / 2.0f;
+    case 4: return bilinear(frame, W, H, x, y) * 0.45436904f +
+                   bilinear(frame + np, W, H, x, y) * 0.47353315f +
+                   bilinear(frame + 2 * np, W, H, x, y) * 0.0720978f;
+    default: return 0.0f;
+    }
+}
+
+__device__ __forceinline__ float clip_contrast(float I, float mean) {
+    float c = (mean > 0.0f) ? I / mean - 1.0f : I;
+    if (fabsf(c) > 1.0f) c = copysignf(1.0f, c);
+    return c;
+}
+
+__global__ __launch_bounds__(1024) void k_lgn_frame_response(
+    const float *__restrict__ frame, int W, int H, uint32_t nLGN,
+    const float *__restrict__ SCx, const float *__restrict__ SCy,
+    const float *__restrict__ SW, const uint32_t *__restrict__ coneType,
+    float *__restrict__ R) {
+    __shared__ float part[32];
+    const uint32_t c = blockIdx.x;
+    const uint32_t s = threadIdx.x;
+    const uint32_t nS = blockDim.x;
+    const uint32_t tC = coneType[c], tS = coneType[nLGN + c];
+    const size_t iC = ((size_t)c * 2 + 0) * nS + s;
+    const size_t iS = ((size_t)c * 2 + 1) * nS + s;
+    const float xC = SCx[iC], yC = SCy[iC], xS = SCx[iS], yS = SCy[iS];
+    const float sw = SW[s];
+    const float IC_S = intensity(tC, frame, W, H, xS, yS);
+    const float IS_S = intensity(tS, frame, W, H, xS, yS);
+    const float IC_C = intensity(tC, frame, W, H, xC, yC);
+    const float meanC = block_reduce_ref(IC_S, part) / (float)nS;
+    const float meanS = block_reduce_ref(IS_S, part) / (float)nS;
+    const float fC = sw * clip_contrast(IC_C, meanC);
+    const float fS = sw * clip_contrast(IS_S, meanS);
+    const float rC = block_reduce_ref(fC, part);
+    const float rS = block_reduce_ref(fS, part);
+    if (s == 0) {
+        R[(size_t)c * 2 + 0] = rC;
+        R[(size_t)c * 2 + 1] = rS;
+    }
+}
+
+// in-register 32-element halving tree (the same additions as vc_tree32)
+__device__ __forceinline__ float tree32_regs(float a[32]) {
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) {
+#pragma unroll
+        for (int l = 0; l < off; ++l) a[l] = a[l] + a[l + off];
+    }
+    return a[0];
+}
+
+__device__ __forceinline__ int64_t floor_div(int64_t a, int64_t b) {
+    int64_t q = a / b;
+    if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
+    return q;
+}
+
+constexpr int kMaxTaps = 64;
+constexpr int kMaxFrame = 8;
+
+__device__ __forceinline__ float transform(float C50, float K, float A, float B, float input) {
+    float X = (float)(1.0 / (double)(1.0f + vc_expf(K * (C50 - input))));
+    return A * X + B;
+}
+
+__global__ __launch_bounds__(256) void k_lgn_steps(vc_lgn_params p, vc_lgn_state st,
+                                                   const float *__restrict__ Rring,
+                                                   uint32_t it0, uint32_t nsteps,
+                                                   float *__restrict__ sInfoOut,
+                                                   float *__restrict__ frOut) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nLGN) return;
+    const uint32_t nKS = p.nKernelSample;
+    float tw[2][kMaxTaps];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int i = 0; i < kMaxTaps; ++i)
+            tw[r][i] = (i < (int)nKS) ? st.TW[((size_t)c * 2 + r) * nKS + i] : 0.0f;
+    const float kC = st.kC[c], kS = st.kS[c], mx = st.maxConvol[c];
+    const float c50 = st.c50[c], K = st.sharpness[c], A = st.logA[c], B = st.logB[c];
+    float lTR = st.leftTimeRate[c], lNL = st.lastNegLogRand[c];
+    const int64_t T0 = (int64_t)p.iKernelSampleT0 - (int64_t)p.nRetrace + 1;
+
+    for (uint32_t s = 0; s < nsteps; ++s) {
+        const uint32_t it = it0 + s;
+        const int64_t base = (int64_t)it + T0;
+        const int64_t f0 = floor_div(base, (int64_t)p.ntPerFrame);
+        // responses of the (at most maxFrame) frames under the window
+        float Rf[2][kMaxFrame];
+#pragma unroll
+        for (int d = 0; d < kMaxFrame; ++d) {
+            int64_t slot = (f0 + d) % (int64_t)p.maxFrame;
+            if (slot < 0) slot += p.maxFrame;
+            if (d < (int)p.maxFrame) {
+                Rf[0][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + 0];
+                Rf[1][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + 1];
+            } else {
+                Rf[0][d] = 0.0f;
+                Rf[1][d] = 0.0f;
+            }
+        }
+        float conv[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            float a0[32], a1[32];
+#pragma unroll
+            for (int i = 0; i < kMaxTaps; ++i) {
+                float v = 0.0f;
+                if (i < (int)nKS) {
+                    int64_t tau = base + (int64_t)i * p.kernelSampleInterval;
+                    int d = (int)(floor_div(tau, (int64_t)p.ntPerFrame) - f0);
+                    float Rv = Rf[r][0];
+#pragma unroll
+                    for (int dd = 1; dd < kMaxFrame; ++dd)
+                        if (d == dd) Rv = Rf[r][dd];
+                    v = Rv * tw[r][i];
+                }
+                if (i < 32) a0[i] = v; else a1[i - 32] = v;
+            }
+            float P0 = tree32_regs(a0);
+            float P1 = tree32_regs(a1);
+            // halving over 8 warp partials of which 6 are zero (256-thread block)
+            float sum = ((P0 + 0.0f) + (0.0f + 0.0f)) + ((P1 + 0.0f) + (0.0f + 0.0f));
+            conv[r] = 0.0f + sum;
+        }
+        float convolC = conv[0] * kC;
+        float convolS = conv[1] * kS;
+        float convol = convolC + convolS;
+        float current = convol * p.convolRatio;
+        if (current < 0.0f) current = 0.0f;
+        float fr = p.frRatio * mx * transform(c50, K, A, B, current / mx);
+
+        // get_spike, src/discrete_input_convol.cu:1587-1622
+        float rate = fr / 1000.0f;
+        float rT = p.dt * rate;
+        float next_rT = lNL - lTR;
+        uint32_t nsp = 0;
+        float tsp = 0.0f;
+        float lNLn = lNL;
+        if (next_rT > rT) {
+            lTR += rT;
+        } else {
+            float old_rT = 0.0f;
+            uint32_t kdraw = 0;
+            vc_u32x4 pack;
+            const float tr = p.tRef * rate;
+            do {
+                if ((kdraw & 3u) == 0u)
+                    pack = vc_draw4(p.seed_lo, p.seed_hi, c, it, VC_STREAM_LGN_SPIKE,
+                                    kdraw >> 2, p.trial);
+                float u = vc_u01_open0(pack.v[kdraw & 3u]);
+                kdraw++;
+                float x = -vc_logf(u);
+                lNLn = (x > tr) ? x : tr;
+                old_rT = next_rT;
+                next_rT += lNLn;
+                nsp++;
+            } while (next_rT <= rT);
+            lTR = rT - old_rT;
+            tsp = old_rT / (rate * (float)nsp);
+        }
+        float sInfo = (float)nsp + tsp / p.dt;
+        if (sInfo > 0.0f) lNL = lNLn;
+        sInfoOut[(size_t)s * p.nLGN + c] = sInfo;
+        if (frOut) frOut[(size_t)s * p.nLGN + c] = fr;
+    }
+    st.leftTimeRate[c] = lTR;
+    st.lastNegLogRand[c] = lNL;
+}
+
+__global__ void k_lgn_init(vc_lgn_params p, float *lTR, float *lNL) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nLGN) return;
+    vc_u32x4 r = vc_draw4(p.seed_lo, p.seed_hi, c, 0xffffffffu, VC_STREAM_LGN_INIT, 0u, p.trial);
+    lTR[c] = 0.0f;
+    lNL[c] = -vc_logf(vc_u01_open0(r.v[0]));
+}
+
+// ---------------------------------------------------------------- setup --
+
+// src/discrete_input_convol.cu:14-33
+__device__ float temporal_kernel(float tau, float tauR, float tauD, float ratio, float nR,
+                                 float nD, float lfac1, float lfac2) {
+    float tau1 = tau / tauR;
+    float tau2 = tau / tauD;
+    float A1 = (nR - 1) * logf(tau1);
+    float A2 = (nD - 1) * logf(tau2);
+    float exponent = A1 - tau1 - lfac1;
+    float tpR = expf(exponent) / tauR;
+    exponent = A2 - tau2 - lfac2;
+    float tpD = expf(exponent) / tauD;
+    return tpR - ratio * tpD;
+}
+
+__device__ __forceinline__ float spatial_kernel(float x, float y, float rx, float ry) {
+    return expf(-x * x / (rx * rx) - y * y / (ry * ry));
+}
+
+// store_temporalWeight, one workgroup of nS threads per (cell, region)
+__global__ __launch_bounds__(1024) void k_lgn_tw(vc_lgn_setup q, float *__restrict__ TW) {
+    __shared__ float part[32];
+    const uint32_t c = blockIdx.x, r = blockIdx.y, tid = threadIdx.x, nS = blockDim.x;
+    const uint32_t lid = r * q.nLGN + c, nKS = q.nKernelSample;
+    const float kernelSampleDt = (float)q.kernelSampleInterval * q.dt;
+    const float kernelSampleT0 = (float)q.iKernelSampleT0 * q.dt;
+    const float lfac1 = lgammaf(q.nR[lid]);
+    const float lfac2 = lgammaf(q.nD[lid]);
+    const float T0 = kernelSampleT0 - q.delay[lid];
+    float *tw = TW + ((size_t)c * 2 + r) * nKS;
+    const uint32_t nPatch = (nKS + nS - 1) / nS;
+    float total = 0.0f;
+    for (uint32_t ip = 0; ip < nPatch; ++ip) {
+        const uint32_t nAct = (ip + 1 < nPatch) ? nS : nKS - ip * nS;
+        float a = 0.0f;
+        if (tid < nAct) {
+            uint32_t j = ip * nS + tid;
+            uint32_t twid = nKS - 1 - j;
+            float t = (float)twid * kernelSampleDt + T0;
+            float v = 0.0f;
+            if (!(t < 0.0f))
+                v = temporal_kernel(t, q.tauR[lid], q.tauD[lid], q.ratio[lid], q.nR[lid],
+                                    q.nD[lid], lfac1, lfac2);
+            tw[j] = v;
+            a = fabsf(v);
+        }
+        total += block_reduce_ref(a, part);
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < nKS; j += nS) tw[j] /= total;
+}
+
+// shared spatial weights from cell 0's centre (src/discrete_input_convol.cu:332-336)
+__global__ __launch_bounds__(1024) void k_lgn_sw(vc_lgn_setup q, float *__restrict__ SW) {
+    __shared__ float part[32];
+    const uint32_t n1D = q.nSample1D;
+    const uint32_t tx = threadIdx.x % n1D, ty = threadIdx.x / n1D;
+    const float rx = q.rw[0], ry = q.rh[0];
+    const float wSpan = (float)((double)(q.nsig * rx) / kSqrt2);
+    const float dw = 2 * wSpan / (float)n1D;
+    const float hSpan = (float)((double)(q.nsig * ry) / kSqrt2);
+    const float dh = 2 * hSpan / (float)n1D;
+    const float w = (float)((tx + 0.5) * (double)dw - (double)wSpan);
+    const float h = (float)((ty + 0.5) * (double)dh - (double)hSpan);
+    const float sw = spatial_kernel(w, h, rx, ry) * 1.0f;
+    const float tot = block_reduce_ref(sw, part);
+    SW[threadIdx.x] = sw / tot;
+}
+
+// sample texture coordinates, flat retina (src/discrete_input_convol.cu:398-417)
+__global__ __launch_bounds__(1024) void k_lgn_sc(vc_lgn_setup q, float *__restrict__ SCx,
+                                                 float *__restrict__ SCy) {
+    const uint32_t c = blockIdx.x, r = blockIdx.y;
+    const uint32_t n1D = q.nSample1D, nS = n1D * n1D;
+    const uint32_t tx = threadIdx.x % n1D, ty = threadIdx.x / n1D;
+    const uint32_t lid = r * q.nLGN + c;
+    const float rx = q.rw[lid], ry = q.rh[lid];
+    const float wSpan = (float)((double)(q.nsig * rx) / kSqrt2);
+    const float dw = 2 * wSpan / (float)n1D;
+    const float hSpan = (float)((double)(q.nsig * ry) / kSqrt2);
+    const float dh = 2 * hSpan / (float)n1D;
+    const float polar = q.polar[lid], ecc = q.ecc[lid];
+    const float coso = cosf(q.orient[lid]), sino = sinf(q.orient[lid]);
+    const float w = (float)((tx + 0.5) * (double)dw - (double)wSpan);
+    const float h = (float)((ty + 0.5) * (double)dh - (double)hSpan);
+    const float x = q.x0 + (ecc * cosf(polar) + w * coso - h * sino) * q.normViewDistance;
+    const float y = q.y0 + (ecc * sinf(polar) + w * sino + h * coso) * q.normViewDistance;
+    const size_t id = ((size_t)c * 2 + r) * nS + threadIdx.x;
+    SCx[id] = x;
+    SCy[id] = y;
+}
+
+// parvo_maxConvol, src/discrete_input_convol.cu:746-881 (1024 threads per cell)
+__global__ __launch_bounds__(1024) void k_lgn_maxconvol(vc_lgn_setup q, const float *__restrict__ TW,
+                                                        float *__restrict__ maxConvol) {
+    extern __shared__ float sw[];          // [2][mS]
+    __shared__ float part[32];
+    const uint32_t c = blockIdx.x, tid = threadIdx.x, nLGN = q.nLGN, nKS = q.nKernelSample;
+    const uint32_t m1D = q.maxSample1D, mS = m1D * m1D, nPatch = mS / 1024;
+    float *swC = sw, *swS = sw + mS;
+    const float rxs = q.rw[nLGN + c], rys = q.rh[nLGN + c];
+    const float xs = q.polar[nLGN + c], ys = q.ecc[nLGN + c];
+    float wSpan = (float)((double)(q.nsig * rxs) / kSqrt2);
+    float hSpan = (float)((double)(q.nsig * rys) / kSqrt2);
+    const float rxc = q.rw[c], ryc = q.rh[c];
+    float R = wSpan > hSpan ? wSpan : hSpan;
+    const float xc = q.polar[c], yc = q.ecc[c];
+    wSpan = (float)((double)(q.nsig * rxc) / kSqrt2);
+    hSpan = (float)((double)(q.nsig * ryc) / kSqrt2);
+    const float dx = yc * cosf(xc) - ys * cosf(xs);
+    const float dy = yc * sinf(xc) - ys * sinf(xs);
+    float r = sqrtf(dx * dx + dy * dy);
+    const float cov = q.covariant[c];
+    const float orients = q.orient[nLGN + c], orientc = q.orient[c];
+    r += wSpan > hSpan ? wSpan : hSpan;
+    const float coss = cosf(orients);
+    R = r > R ? r : R;
+    const float sins = sinf(orients);
+    const float cosc = cosf(orientc), sinc = sinf(orientc);
+    const float ks = q.k[nLGN + c], kc = q.k[c];
+    const float ds = 2 * R / (float)m1D;
+    float sumC = 0.0f, sumS = 0.0f;
+    for (uint32_t ip = 0; ip < nPatch; ++ip) {
+        const uint32_t pid = ip * 1024 + tid;
+        const uint32_t wi = pid % m1D, hi = pid / m1D;
+        const float lx = (float)((wi + 0.5) * (double)ds - (double)R);
+        const float ly = (float)((hi + 0.5) * (double)ds - (double)R);
+        float x = cosc * (lx - dx) + sinc * (ly - dy);
+        float y = -sinc * (lx - dx) + cosc * (ly - dy);
+        const float lc = spatial_kernel(x, y, rxc, ryc);
+        swC[pid] = lc;
+        sumC += block_reduce_ref(lc, part);
+        x = coss * lx + sins * ly;
+        y = -sins * lx + coss * ly;
+        const float ls = spatial_kernel(x, y, rxs, rys);
+        swS[pid] = ls;
+        sumS += block_reduce_ref(ls, part);
+    }
+    __syncthreads();
+    for (uint32_t ip = 0; ip < nPatch; ++ip) {
+        const uint32_t pid = ip * 1024 + tid;
+        swC[pid] /= sumC;
+        swS[pid] /= sumS;
+    }
+    __syncthreads();
+    float convol = 0.0f;
+    for (uint32_t it = 0; it < nKS; ++it) {
+        const float tc = TW[(size_t)c * 2 * nKS + it];
+        const float ts = TW[((size_t)c * 2 + 1) * nKS + it];
+        for (uint32_t ip = 0; ip < nPatch; ++ip) {
+            const uint32_t pid = ip * 1024 + tid;
+            const float lc = swC[pid] * tc * kc;
+            const float ls = swS[pid] * ts * ks;
+            float d;
+            if (fabsf(lc) > fabsf(ls)) d = fabsf(lc) + ls * copysignf(cov, lc);
+            else d = fabsf(ls) + lc * copysignf(cov, ls);
+            convol += block_reduce_ref(d, part);
+        }
+    }
+    if (tid == 0) maxConvol[c] = convol;
+}
+
+}  // namespace
+
+extern "C" int vc_lgn_precompute(void *stream, const vc_lgn_setup *q, float *TW, float *SW,
+                                 float *SCx, float *SCy, float *maxConvol) {
+    if (!q || !TW || !SW || !SCx || !SCy || !maxConvol) {
+        vc_set_error("vc_lgn_precompute: null argument");
+        return VC_E_INVALID;
+    }
+    const uint32_t nS = q->nSample1D * q->nSample1D;
+    if (nS < 32 || nS > 1024 || (nS & (nS - 1)) != 0) {
+        vc_set_error("nSpatialSample1D^2 = %u must be a power of two in [32, 1024]", nS);
+        return VC_E_UNSUPPORTED;
+    }
+    const uint32_t mS = q->maxSample1D * q->maxSample1D;
+    if (mS < 1024 || mS % 1024 != 0 || 2 * mS * sizeof(float) > 64 * 1024) {
+        vc_set_error("maxSample1D^2 = %u must be a multiple of 1024 and fit in LDS", mS);
+        return VC_E_UNSUPPORTED;
+    }
+    if (q->nLGN == 0 || q->nKernelSample == 0) {
+        vc_set_error("vc_lgn_precompute: empty LGN");
+        return VC_E_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_lgn_tw, dim3(q->nLGN, 2), dim3(nS), 0, st, *q, TW);
+    VC_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_lgn_sw, dim3(1), dim3(nS), 0, st, *q, SW);
+    VC_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_lgn_sc, dim3(q->nLGN, 2), dim3(nS), 0, st, *q, SCx, SCy);
+    VC_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_lgn_maxconvol, dim3(q->nLGN), dim3(1024), 2 * mS * sizeof(float), st,
+                       *q, TW, maxConvol);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t width,
+                                     uint32_t height, uint32_t nLGN, uint32_t nSample1D,
+                                     const float *SCx, const float *SCy, const float *SW,
+                                     const uint32_t *coneType, float *R) {
+    const uint32_t nS = nSample1D * nSample1D;
+    if (!frame || !SCx || !SCy || !SW || !coneType || !R) {
+        vc_set_error("vc_lgn_frame_response: null argument");
+        return VC_E_INVALID;
+    }
+    if (nS < 32 || nS > 1024 || (nS & (nS - 1)) != 0) {
+        vc_set_error("nSpatialSample1D^2 = %u must be a power of two in [32, 1024]", nS);
+        return VC_E_UNSUPPORTED;
+    }
+    if (width == 0 || height == 0 || nLGN == 0) {
+        vc_set_error("vc_lgn_frame_response: empty frame or LGN");
+        return VC_E_INVALID;
+    }
+    hipLaunchKernelGGL(k_lgn_frame_response, dim3(nLGN), dim3(nS), 0, (hipStream_t)stream,
+                       frame, (int)width, (int)height, nLGN, SCx, SCy, SW, coneType, R);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
+                            const float *Rring, uint32_t it0, uint32_t nsteps, float *sInfo,
+                            float *fr) {
+    if (!p || !s || !Rring || !sInfo) {
+        vc_set_error("vc_lgn_steps: null argument");
+        return VC_E_INVALID;
+    }
+    if (p->nKernelSample == 0 || p->nKernelSample > (uint32_t)kMaxTaps) {
+        vc_set_error("nKernelSample %u unsupported (1..%d)", p->nKernelSample, kMaxTaps);
+        return VC_E_UNSUPPORTED;
+    }
+    if (p->maxFrame == 0 || p->maxFrame > (uint32_t)kMaxFrame || p->ntPerFrame == 0) {
+        vc_set_error("maxFrame %u unsupported (1..%d)", p->maxFrame, kMaxFrame);
+        return VC_E_UNSUPPORTED;
+    }
+    // the window spans nRetrace steps; every tap's frame must be within maxFrame of the first
+    if ((p->nRetrace + p->ntPerFrame - 1) / p->ntPerFrame + 1 > p->maxFrame) {
+        vc_set_error("maxFrame %u too small for a %u-step window", p->maxFrame, p->nRetrace);
+        return VC_E_INVALID;
+    }
+    if (nsteps == 0 || p->nLGN == 0) return VC_OK;
+    hipLaunchKernelGGL(k_lgn_steps, dim3((p->nLGN + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, *p, *s, Rring, it0, nsteps, sInfo, fr);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_lgn_init(void *stream, const vc_lgn_params *p, float *leftTimeRate,
+                           float *lastNegLogRand) {
+    if (!p || !leftTimeRate || !lastNegLogRand) {
+        vc_set_error("vc_lgn_init: null argument");
+        return VC_E_INVALID;
+    }
+    if (p->nLGN == 0) return VC_OK;
+    hipLaunchKernelGGL(k_lgn_init, dim3((p->nLGN + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       *p, leftTimeRate, lastNegLogRand);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}

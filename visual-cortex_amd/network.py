@@ -1,0 +1,447 @@
+"""Model specification and synthetic network construction for the patchV1
+simulation step.
+
+A `ModelSpec` carries the reference's cfg parameters (src/patch.cu:180-374,
+same key names where they exist) plus the sizes of a synthetic workload.
+`build_network(spec)` generates, with numpy and a fixed seed:
+
+  * V1: `nblock` blocks of `blockSize` neurons (E first, src/patch.cu:695-706),
+    block b covering a 1 mm x 1 mm square of a sqrt(nblock)-wide grid;
+  * recurrent synapses: each neuron draws `KE` E and `KI` I partners from its
+    3x3 block neighbourhood without replacement with p ~ exp(-d^2/2 sigma^2)
+    (Efraimidis-Spirakis keys), lognormal strengths;
+  * LGN: ON/OFF cells on a jittered grid, DoG centre/surround parameters and a
+    biphasic difference-of-gamma temporal kernel (the fLGN parameter set,
+    src/patch.cu:2472-2530);
+  * LGN->V1: `m` inputs per neuron by Gaussian distance in the visual field;
+  * a drifting sinusoidal grating (the stimulus.bin layout, grating.py:267-372).
+
+The same Network feeds the GPU path (masked CSR, see include/vc_abi.h) and the
+CPU oracle (the reference's dense conMat).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+DEG2RAD = math.pi / 180.0
+
+
+@dataclass
+class ModelSpec:
+    # sizes
+    nblock: int = 4
+    blockSize: int = 1024
+    nE: int = 768
+    KE: int = 400
+    KI: int = 100
+    m: int = 20
+    nLGN_side: int = 32           # LGN grid is nLGN_side^2 cells
+    sheet_mm: float = 2.0         # V1 sheet side
+    field_deg: float = 3.0        # LGN centre field side
+    width: int = 128              # stimulus pixels
+    height: int = 128
+    stim_deg: float = 4.0         # stimulus extent (range = stim_deg/2)
+    # time
+    dt: float = 0.125
+    nt: int = 8000
+    frameRate: int = 125
+    # V1 neurons, per type [E, I]  (I is LIF: a = b = 0, deltaT = 0)
+    iModel: int = 1
+    vR: tuple = (-58.0, -58.0)
+    vThres: tuple = (-20.0, -50.0)
+    gL: tuple = (2.0, 2.8)
+    C: tuple = (20.0, 15.0)
+    tRef: tuple = (2.0, 1.0)
+    vT: tuple = (-50.0, -50.0)
+    deltaT: tuple = (1.5, 0.0)
+    tau_w: tuple = (200.0, 10.0)
+    a: tuple = (0.0, 0.0)
+    b: tuple = (1.0, 0.0)
+    tonicDep: tuple = (0.0, 0.0)
+    # receptors: FF/E = (AMPA, NMDA), I = (GABA_A,)
+    riseTimeFF: tuple = (1.0, 5.0)
+    decayTimeFF: tuple = (3.0, 80.0)
+    riseTimeE: tuple = (1.0, 5.0)
+    decayTimeE: tuple = (3.0, 80.0)
+    riseTimeI: tuple = (1.0,)
+    decayTimeI: tuple = (5.0,)
+    pFF: tuple = ((0.85, 0.15), (0.85, 0.15))
+    pE: tuple = ((0.8, 0.2), (0.8, 0.2))
+    pI: tuple = ((1.0,), (1.0,))
+    # synaptic scales
+    sRatioLGN: tuple = (1.0, 1.0)
+    sLGN: float = 1.2             # base LGN synapse strength
+    sE: float = 0.25              # E recurrent scale (times lognormal)
+    sI: float = 1.2               # I recurrent scale
+    w_mu: float = -1.0
+    w_sigma: float = 0.8
+    sigma_um: float = 150.0
+    # background Poisson drive (extension)
+    bgRate: float = 2000.0        # Hz
+    bgStrength: tuple = (0.22, 0.22)
+    pBG: tuple = ((1.0, 0.0), (1.0, 0.0))
+    # LGN
+    nSpatialSample1D: int = 16
+    nsig: float = 3.0
+    tau: float = 40.0             # temporal kernel span, ms
+    nKernelSample: int = 40
+    sigC_deg: float = 0.05
+    sigS_deg: float = 0.15
+    kS_over_kC: float = 0.7
+    tauR: float = 2.0
+    tauD: float = 3.0
+    nR: float = 6.0
+    nD: float = 8.0
+    ratio: float = 0.8
+    spont: float = 0.08
+    c50: float = 0.3
+    sharpness: float = 8.0
+    frRatioLGN: float = 60.0
+    convolRatio: float = 1.0
+    ff_sigma_cells: float = 1.5   # LGN->V1 pooling sigma in LGN grid spacings
+    # grating
+    sf_cpd: float = 4.0
+    tf_hz: float = 8.0
+    contrast: float = 0.5
+    ori_deg: float = 0.0
+    phase0: float = 0.0
+    # seeds
+    seed: int = 1234
+    trial: int = 0
+
+    @property
+    def nV1(self) -> int:
+        return self.nblock * self.blockSize
+
+    @property
+    def nLGN(self) -> int:
+        return self.nLGN_side * self.nLGN_side
+
+    @property
+    def nI(self) -> int:
+        return self.blockSize - self.nE
+
+
+def spec_c1(**kw) -> ModelSpec:
+    """Plain forward, small: 4 blocks x 1024, 1024 LGN, 128x128 px, 1 s."""
+    return replace(ModelSpec(), **kw)
+
+
+def spec_c2(**kw) -> ModelSpec:
+    """Plain forward, full patch: 64 blocks, 16384 LGN, m = 30, 800 E + 200 I
+    partners, 256x256 px, 2 s."""
+    base = ModelSpec(nblock=64, KE=800, KI=200, m=30, nLGN_side=128, sheet_mm=8.0,
+                     width=256, height=256, nt=16000, sE=0.125, sI=0.6)
+    return replace(base, **kw)
+
+
+def spec_tiny(**kw) -> ModelSpec:
+    """A small case that the oracle finishes in well under a second per 100
+    steps: 2 blocks of 256 (192 E), 64 LGN cells, 32x32 px."""
+    base = ModelSpec(nblock=2, blockSize=256, nE=192, KE=100, KI=25, m=8, nLGN_side=8,
+                     sheet_mm=1.0, field_deg=2.0, width=32, height=32, nt=400,
+                     sE=0.8, sI=3.0)
+    return replace(base, **kw)
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class Network:
+    spec: ModelSpec
+    # V1
+    pos: np.ndarray                 # [nV1, 2] mm
+    tonicDep: np.ndarray            # [nV1]
+    maxNear: int
+    nNear: np.ndarray               # [nblock] u32
+    nbr: np.ndarray                 # [nblock, maxNear] u32
+    rec_pairs: list                 # per (b, ib): (pre_local u32, post_local u32, w f32), sorted by (pre, post)
+    # LGN -> V1 (transposed ELL)
+    nLGN_of: np.ndarray             # [nV1] u32
+    sLGN: np.ndarray                # [maxLGN, nV1] f32
+    idLGN: np.ndarray               # [maxLGN, nV1] u32
+    # LGN parameters, [2, nLGN] (centre, surround)
+    lgn: dict = field(default_factory=dict)
+    # stimulus
+    frames: np.ndarray | None = None  # [nFrame, 3, H, W] f32
+    init_lms: tuple = (0.5, 0.5, 0.5)
+
+    # ---- recurrent formats -------------------------------------------------
+    def masked_csr(self):
+        """(rowMask u64 [nRow*nSeg], rowOff u32 [nRow*nSeg], w f32 [nnz]) with
+        rows r = (b*maxNear + ib)*bs + pre (see include/vc_abi.h)."""
+        sp = self.spec
+        bs = sp.blockSize
+        nSeg = bs // 64
+        nRow = sp.nblock * self.maxNear * bs
+        rowMask = np.zeros(nRow * nSeg, dtype=np.uint64)
+        counts = np.zeros(nRow * nSeg, dtype=np.int64)
+        ws = []
+        for b in range(sp.nblock):
+            for ib in range(self.maxNear):
+                pre, post, w = self.rec_pairs[b * self.maxNear + ib]
+                if pre.size == 0:
+                    continue
+                row = (b * self.maxNear + ib) * bs + pre.astype(np.int64)
+                key = row * nSeg + (post.astype(np.int64) >> 6)
+                bits = np.left_shift(np.uint64(1), (post & 63).astype(np.uint64))
+                # entries are sorted by (pre, post) -> keys are sorted
+                uk, start = np.unique(key, return_index=True)
+                rowMask[uk] = np.bitwise_or.reduceat(bits, start)
+                counts[uk] = np.diff(np.append(start, key.size))
+                ws.append(w)
+        rowOff = np.zeros(nRow * nSeg, dtype=np.int64)
+        np.cumsum(counts[:-1], out=rowOff[1:])
+        w = np.concatenate(ws).astype(np.float32) if ws else np.zeros(1, np.float32)
+        if rowOff[-1] + counts[-1] >= 2 ** 32:
+            raise ValueError("too many recurrent synapses for 32-bit offsets")
+        return rowMask, rowOff.astype(np.uint32), w
+
+    def dense_conmat(self) -> np.ndarray:
+        """The reference's V1_conMat layout [nblock, maxNear, bs pre, bs post]."""
+        sp = self.spec
+        bs = sp.blockSize
+        M = np.zeros((sp.nblock, self.maxNear, bs, bs), dtype=np.float32)
+        for b in range(sp.nblock):
+            for ib in range(self.maxNear):
+                pre, post, w = self.rec_pairs[b * self.maxNear + ib]
+                M[b, ib, pre, post] = w
+        return M
+
+    def n_rec_synapses(self) -> int:
+        return int(sum(p[0].size for p in self.rec_pairs))
+
+
+# ---------------------------------------------------------------------------
+def _block_grid(nblock: int) -> int:
+    """Number of block columns of the sheet (blocks are laid out row-major on
+    a cols x nblock/cols grid, cols the smallest divisor >= sqrt(nblock))."""
+    cols = int(math.ceil(math.sqrt(nblock) - 1e-9))
+    while nblock % cols:
+        cols += 1
+    return cols
+
+
+def _neighbours(nblock: int):
+    """3x3 neighbourhood of every block, self first then ascending id."""
+    side = _block_grid(nblock)
+    rows = nblock // side
+    lists = []
+    for b in range(nblock):
+        bx, by = b % side, b // side
+        nb = [b]
+        for oy in (-1, 0, 1):
+            for ox in (-1, 0, 1):
+                x, y = bx + ox, by + oy
+                if (ox or oy) and 0 <= x < side and 0 <= y < rows:
+                    nb.append(y * side + x)
+        lists.append([nb[0]] + sorted(nb[1:]))
+    maxNear = max(len(l) for l in lists)
+    nNear = np.array([len(l) for l in lists], dtype=np.uint32)
+    nbr = np.zeros((nblock, maxNear), dtype=np.uint32)
+    for b, l in enumerate(lists):
+        nbr[b, :len(l)] = l
+    return maxNear, nNear, nbr
+
+
+def _weighted_topk(keysrc_w: np.ndarray, k: int, rng: np.random.Generator) -> np.ndarray:
+    """Row-wise weighted sampling of k columns without replacement
+    (Efraimidis & Spirakis 2006: largest log(u)/w)."""
+    u = rng.random(keysrc_w.shape)
+    with np.errstate(divide="ignore"):
+        keys = np.log(u) / keysrc_w
+    keys[~(keysrc_w > 0)] = -np.inf
+    idx = np.argpartition(-keys, k - 1, axis=1)[:, :k]
+    return idx
+
+
+def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
+    rng = np.random.default_rng(spec.seed)
+    bs, nb = spec.blockSize, spec.nblock
+    side = _block_grid(nb)
+    blk_mm = spec.sheet_mm / side
+    # ---- V1 positions
+    pos = np.zeros((spec.nV1, 2))
+    for b in range(nb):
+        bx, by = b % side, b // side
+        pos[b * bs:(b + 1) * bs, 0] = (bx + rng.random(bs)) * blk_mm
+        pos[b * bs:(b + 1) * bs, 1] = (by + rng.random(bs)) * blk_mm
+    local = np.arange(spec.nV1) % bs
+    isE = local < spec.nE
+    tonic = np.where(isE, spec.tonicDep[0], spec.tonicDep[1]).astype(np.float32)
+
+    # ---- recurrent
+    maxNear, nNear, nbr = _neighbours(nb)
+    sig_mm = spec.sigma_um / 1000.0
+    rec_pairs = [None] * (nb * maxNear)
+    for b in range(nb):
+        posts = np.arange(b * bs, (b + 1) * bs)
+        cand = np.concatenate([np.arange(int(j) * bs, (int(j) + 1) * bs) for j in nbr[b, :nNear[b]]])
+        cand_slot = np.repeat(np.arange(nNear[b]), bs)
+        cE = cand[isE[cand]]
+        cI = cand[~isE[cand]]
+        chosen = []
+        for cset, K, scale in ((cE, spec.KE, spec.sE), (cI, spec.KI, spec.sI)):
+            if K == 0:
+                continue
+            d2 = ((pos[posts, None, :] - pos[None, cset, :]) ** 2).sum(-1)
+            wgt = np.exp(-d2 / (2 * sig_mm * sig_mm))
+            wgt[posts[:, None] == cset[None, :]] = 0.0        # no self-connection
+            Kc = min(K, int((wgt > 0).sum(1).min()))
+            sel = _weighted_topk(wgt, Kc, rng)
+            pre_g = cset[sel]                                  # [bs, Kc]
+            strength = (scale * rng.lognormal(spec.w_mu, spec.w_sigma, size=pre_g.shape)).astype(np.float32)
+            post_g = np.broadcast_to(posts[:, None], pre_g.shape)
+            chosen.append((pre_g.ravel(), post_g.ravel(), strength.ravel()))
+        pre_all = np.concatenate([c[0] for c in chosen])
+        post_all = np.concatenate([c[1] for c in chosen])
+        w_all = np.concatenate([c[2] for c in chosen])
+        pre_blk = pre_all // bs
+        for ib in range(maxNear):
+            if ib < nNear[b]:
+                sel = pre_blk == nbr[b, ib]
+                pl = (pre_all[sel] % bs).astype(np.uint32)
+                ql = (post_all[sel] % bs).astype(np.uint32)
+                ww = w_all[sel]
+                order = np.lexsort((ql, pl))
+                rec_pairs[b * maxNear + ib] = (pl[order], ql[order], ww[order])
+            else:
+                z = np.zeros(0, np.uint32)
+                rec_pairs[b * maxNear + ib] = (z, z, np.zeros(0, np.float32))
+        del cand_slot
+
+    # ---- LGN cells on a jittered grid, checkerboard ON/OFF
+    ns = spec.nLGN_side
+    spacing = spec.field_deg / ns
+    gx, gy = np.meshgrid(np.arange(ns), np.arange(ns))
+    gx = gx.ravel(); gy = gy.ravel()
+    lx = (gx + 0.5) * spacing - spec.field_deg / 2 + (rng.random(ns * ns) - 0.5) * 0.6 * spacing
+    ly = (gy + 0.5) * spacing - spec.field_deg / 2 + (rng.random(ns * ns) - 0.5) * 0.6 * spacing
+    on = ((gx + gy) % 2 == 0)
+    nL = ns * ns
+    ecc = np.sqrt(lx * lx + ly * ly) * DEG2RAD
+    polar = np.arctan2(ly, lx)
+    rc = spec.sigC_deg * math.sqrt(2) * DEG2RAD
+    rs = spec.sigS_deg * math.sqrt(2) * DEG2RAD
+    kC = np.where(on, 1.0, -1.0)
+    kS = -spec.kS_over_kC * kC
+    two = lambda a, b_: np.stack([np.broadcast_to(a, (nL,)), np.broadcast_to(b_, (nL,))]).astype(np.float32)
+    spont = spec.spont
+    K = spec.sharpness
+    c50 = spec.c50
+    e_kc50 = math.exp(K * c50)
+    e_kc50_k = math.exp(K * c50 - K)
+    logA = (1 - spont) * (1 + e_kc50) * (1 + e_kc50_k) / (e_kc50 - e_kc50_k)   # src/LGN_props.h:163-169
+    logB = 1 - logA / (1 + e_kc50_k)
+    lgn = dict(
+        polar=two(polar, polar), ecc=two(ecc, ecc), rw=two(rc, rs), rh=two(rc, rs),
+        orient=two(0.0, 0.0), k=two(kC, kS),
+        tauR=two(spec.tauR, spec.tauR), tauD=two(spec.tauD, spec.tauD),
+        delay=two(0.0, 0.0), ratio=two(spec.ratio, spec.ratio),
+        nR=two(spec.nR, spec.nR), nD=two(spec.nD, spec.nD),
+        covariant=np.ones(nL, np.float32),
+        coneType=np.full((2, nL), 3, dtype=np.uint32),            # (L+M)/2 both regions
+        c50=np.full(nL, c50, np.float32), sharpness=np.full(nL, K, np.float32),
+        logA=np.full(nL, logA, np.float32), logB=np.full(nL, logB, np.float32),
+        xdeg=lx.astype(np.float32), ydeg=ly.astype(np.float32), on=on,
+    )
+
+    # ---- LGN -> V1 by Gaussian distance in the visual field
+    from scipy.spatial import cKDTree
+    vx = (pos[:, 0] / spec.sheet_mm - 0.5) * spec.field_deg * 0.9
+    vy = (pos[:, 1] / spec.sheet_mm - 0.5) * spec.field_deg * 0.9
+    tree = cKDTree(np.stack([lx, ly], 1))
+    kq = min(nL, max(4 * spec.m, 64))
+    dist, nn = tree.query(np.stack([vx, vy], 1), k=kq)
+    sig = spec.ff_sigma_cells * spacing
+    wq = np.exp(-dist ** 2 / (2 * sig * sig))
+    sel = _weighted_topk(wq, spec.m, rng)
+    ids = np.take_along_axis(nn, sel, 1)                        # [nV1, m]
+    ids.sort(axis=1)
+    sR = np.where(isE, spec.sRatioLGN[0], spec.sRatioLGN[1])
+    s_ff = (spec.sLGN * sR)[:, None] * (0.75 + 0.5 * rng.random(ids.shape))
+    net = Network(
+        spec=spec, pos=pos, tonicDep=tonic, maxNear=maxNear, nNear=nNear, nbr=nbr,
+        rec_pairs=rec_pairs,
+        nLGN_of=np.full(spec.nV1, spec.m, dtype=np.uint32),
+        sLGN=np.ascontiguousarray(s_ff.T.astype(np.float32)),
+        idLGN=np.ascontiguousarray(ids.T.astype(np.uint32)),
+        lgn=lgn,
+    )
+    if with_frames:
+        net.frames = grating_frames(spec)
+    return net
+
+
+def n_frames(spec: ModelSpec) -> int:
+    steps_per_frame = round(1000.0 / spec.dt) // spec.frameRate
+    return (spec.nt + steps_per_frame - 1) // steps_per_frame + 1
+
+
+def grating_frames(spec: ModelSpec, nframe: int | None = None) -> np.ndarray:
+    """Drifting sinusoidal grating, L = M = S = 0.5 (1 + c sin(...)), frames
+    [nFrame, 3, H, W] (the stimulus.bin frame layout, grating.py:267-372)."""
+    nf = n_frames(spec) if nframe is None else nframe
+    W, H = spec.width, spec.height
+    xs = ((np.arange(W) + 0.5) / W - 0.5) * spec.stim_deg
+    ys = ((np.arange(H) + 0.5) / H - 0.5) * spec.stim_deg
+    X, Y = np.meshgrid(xs, ys)
+    th = spec.ori_deg * DEG2RAD
+    u = X * math.cos(th) + Y * math.sin(th)
+    t = np.arange(nf) / spec.frameRate
+    ph = 2 * math.pi * (spec.sf_cpd * u[None] - spec.tf_hz * t[:, None, None]) + spec.phase0
+    lum = (0.5 * (1.0 + spec.contrast * np.sin(ph))).astype(np.float32)
+    return np.ascontiguousarray(np.repeat(lum[:, None], 3, axis=1))
+
+
+# ---------------------------------------------------------------------------
+def timing(spec: ModelSpec):
+    """Kernel-sampling and frame bookkeeping (src/patch.cu:4949-5046)."""
+    stepRate = round(1000.0 / spec.dt)
+    if abs(stepRate * spec.dt - 1000.0) > 1e-9:
+        raise ValueError("1000/dt must be an integer (src/patch.cu stepRate)")
+    if stepRate % spec.frameRate != 0:
+        raise NotImplementedError("stepRate must be a multiple of frameRate (denorm = 1)")
+    ntPerFrame = stepRate // spec.frameRate
+    nRetrace = int(round(spec.tau / spec.dt))
+    interval = nRetrace // spec.nKernelSample
+    iT0 = interval // 2 if interval % 2 == 0 else 0
+    maxFrame = (nRetrace + ntPerFrame - 1) // ntPerFrame + 1
+    return dict(stepRate=stepRate, ntPerFrame=ntPerFrame, nRetrace=nRetrace,
+                kernelSampleInterval=interval, iKernelSampleT0=iT0, maxFrame=maxFrame)
+
+
+def max_sample_1d(net: Network) -> int:
+    """Sample grid of parvo_maxConvol (src/patch.cu:5266-5300): round(mean
+    surround/centre span ratio) * nSpatialSample1D, raised to a multiple of 32
+    so that the 1024-thread patches tile it (the reference asserts this)."""
+    sp = net.spec
+    L = net.lgn
+    nL = sp.nLGN
+    rwc, rhc, rws, rhs = L["rw"][0], L["rh"][0], L["rw"][1], L["rh"][1]
+    dx = L["ecc"][0] * np.cos(L["polar"][0]) - L["ecc"][1] * np.cos(L["polar"][1])
+    dy = L["ecc"][0] * np.sin(L["polar"][0]) - L["ecc"][1] * np.sin(L["polar"][1])
+    r = np.sqrt(dx * dx + dy * dy)
+    R = np.maximum(abs(sp.nsig) * rws / math.sqrt(2), abs(sp.nsig) * rhs / math.sqrt(2))
+    wS = abs(sp.nsig) * rwc / math.sqrt(2)
+    hS = abs(sp.nsig) * rhc / math.sqrt(2)
+    r = r + np.maximum(wS, hS)
+    R = np.maximum(r, R)
+    ratio = np.sqrt(R * R / wS / hS)
+    m1 = int(round(float(ratio.mean()))) * sp.nSpatialSample1D
+    if m1 * m1 < 1024:
+        return 32
+    return int(math.ceil(m1 / 32.0) * 32)
+
+
+def view_geometry(spec: ModelSpec):
+    """Flat retina (src/patch.cu:1630-1640): x0 = y0 = 0.5 and the scale that
+    maps radians to normalised texture coordinates."""
+    stim_range = spec.stim_deg / 2.0
+    extent = 2.0 * (stim_range + 0.0)
+    max_ecc = stim_range
+    normEcc = max_ecc / extent
+    nvd = normEcc / max_ecc / DEG2RAD
+    return 0.5, 0.5, float(nvd)

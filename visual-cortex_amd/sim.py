@@ -1,0 +1,185 @@
+"""Python driver of the MI355X simulation step: owns every buffer as a torch
+tensor on the GPU and calls libvc_hip.so through its C ABI.
+
+The step order follows the reference's host loop (src/patch.cu:5966-6546):
+a new stimulus frame enters every ntPerFrame steps; the LGN stage for a run of
+steps only depends on the stimulus, so it is computed one frame period at a
+time (vc_lgn_steps) before the V1 steps of that period; each V1 step
+(vc_v1_step) applies the previous step's recurrent increments, updates every
+neuron and compacts its spikes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import abi
+from .abi import check, ptr
+from .network import Network, max_sample_1d, timing, view_geometry
+from .params import initial_state, lgn_params_struct, v1_params_struct
+
+
+class GpuSim:
+    """One simulation of one network on one GPU."""
+
+    def __init__(self, net: Network, device: str = "cuda", record: bool = True,
+                 state: dict | None = None):
+        self.lib = abi.load_library()
+        self.net = net
+        sp = net.spec
+        self.dev = torch.device(device)
+        self.stream = torch.cuda.current_stream(self.dev)
+        self.tm = timing(sp)
+        d = self.dev
+        T = lambda a, dt=None: torch.as_tensor(np.ascontiguousarray(a), device=d) if dt is None \
+            else torch.as_tensor(np.ascontiguousarray(a).astype(dt), device=d)
+        # ---- V1 parameters and state
+        self.p = v1_params_struct(net)
+        st = initial_state(net) if state is None else state
+        self.state = {k: T(v) for k, v in st.items()}
+        self.tonic = T(net.tonicDep, np.float32)
+        self.s = abi.V1State(*(ptr(self.state[k]) for k in
+                               ("v", "w", "tBack", "gFF", "hFF", "gE", "hE", "gI", "hI")),
+                             ptr(self.tonic))
+        # ---- FF synapses
+        self.nLGN_of = T(net.nLGN_of.astype(np.uint32).view(np.int32))
+        self.sLGN = T(net.sLGN, np.float32)
+        self.idLGN = T(net.idLGN.astype(np.uint32).view(np.int32))
+        self.ff = abi.FFConn(ptr(self.nLGN_of), ptr(self.sLGN), ptr(self.idLGN), net.sLGN.shape[0])
+        # ---- recurrent synapses (masked CSR)
+        rowMask, rowOff, w = net.masked_csr()
+        self.rowMask = T(rowMask.view(np.int64))
+        self.rowOff = T(rowOff.view(np.int32))
+        self.wrec = T(w, np.float32)
+        self.nNear = T(net.nNear.astype(np.uint32).view(np.int32))
+        self.nbr = T(net.nbr.astype(np.uint32).view(np.int32))
+        self.rec = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
+                               ptr(self.rowOff), ptr(self.wrec))
+        # ---- spike buffers (double buffered)
+        nT = sp.nV1 // 64
+        self.masks = [torch.zeros(nT, dtype=torch.int64, device=d) for _ in range(2)]
+        self.og = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
+        self.oh = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
+        self.spk = [abi.Spikes(ptr(self.masks[i]), ptr(self.og[i]), ptr(self.oh[i])) for i in range(2)]
+        # ---- LGN
+        self._setup_lgn()
+        self.it = 0
+        self.cur = 0                 # index of the spike buffer holding the last step's spikes
+        self.record = record
+        self.sInfo_hist: list[torch.Tensor] = []
+        self.lgn_hist: list[torch.Tensor] = []
+        self.sInfo_step = torch.zeros(sp.nV1, dtype=torch.float32, device=d)
+
+    # ------------------------------------------------------------------ LGN
+    def _setup_lgn(self):
+        net, sp, d = self.net, self.net.spec, self.dev
+        L = net.lgn
+        self.lgn_t = {k: torch.as_tensor(np.ascontiguousarray(L[k]), device=d)
+                      for k in ("polar", "ecc", "rw", "rh", "orient", "k", "tauR", "tauD",
+                                "delay", "ratio", "nR", "nD", "covariant", "c50", "sharpness",
+                                "logA", "logB")}
+        self.coneType = torch.as_tensor(L["coneType"].view(np.int32), device=d)
+        x0, y0, nvd = view_geometry(sp)
+        tm = self.tm
+        q = abi.LgnSetup()
+        q.nLGN, q.nSample1D, q.nKernelSample = sp.nLGN, sp.nSpatialSample1D, sp.nKernelSample
+        q.kernelSampleInterval, q.iKernelSampleT0 = tm["kernelSampleInterval"], tm["iKernelSampleT0"]
+        q.maxSample1D = max_sample_1d(net)
+        q.dt, q.nsig, q.normViewDistance, q.x0, q.y0 = sp.dt, sp.nsig, nvd, x0, y0
+        for k in ("polar", "ecc", "rw", "rh", "orient", "k", "tauR", "tauD", "delay", "ratio",
+                  "nR", "nD", "covariant"):
+            setattr(q, k, ptr(self.lgn_t[k]))
+        self.lgn_setup = q
+        nS = sp.nSpatialSample1D ** 2
+        f32 = dict(dtype=torch.float32, device=d)
+        self.TW = torch.zeros(sp.nLGN, 2, sp.nKernelSample, **f32)
+        self.SW = torch.zeros(nS, **f32)
+        self.SCx = torch.zeros(sp.nLGN, 2, nS, **f32)
+        self.SCy = torch.zeros(sp.nLGN, 2, nS, **f32)
+        self.maxConvol = torch.zeros(sp.nLGN, **f32)
+        check(self.lib.vc_lgn_precompute(None, C.byref(q), ptr(self.TW), ptr(self.SW),
+                                         ptr(self.SCx), ptr(self.SCy), ptr(self.maxConvol)),
+              "vc_lgn_precompute")
+        self.kC = self.lgn_t["k"][0].contiguous()
+        self.kS = self.lgn_t["k"][1].contiguous()
+        self.lTR = torch.zeros(sp.nLGN, **f32)
+        self.lNL = torch.zeros(sp.nLGN, **f32)
+        self.lp = lgn_params_struct(net)
+        check(self.lib.vc_lgn_init(None, C.byref(self.lp), ptr(self.lTR), ptr(self.lNL)), "vc_lgn_init")
+        self.ls = abi.LgnState(ptr(self.TW), ptr(self.kC), ptr(self.kS), ptr(self.maxConvol),
+                               ptr(self.lgn_t["c50"]), ptr(self.lgn_t["sharpness"]),
+                               ptr(self.lgn_t["logA"]), ptr(self.lgn_t["logB"]),
+                               ptr(self.lTR), ptr(self.lNL))
+        # stimulus resident in HBM; pre-stimulus frame of the initial luminance
+        self.frames = torch.as_tensor(net.frames, device=d)
+        init = np.empty((3, sp.height, sp.width), np.float32)
+        for c in range(3):
+            init[c] = net.init_lms[c]
+        self.init_frame = torch.as_tensor(init, device=d)
+        self.Rring = torch.zeros(tm["maxFrame"], sp.nLGN, 2, **f32)
+        self._frame_response(self.init_frame, 0)
+        for s in range(1, tm["maxFrame"]):
+            self.Rring[s].copy_(self.Rring[0])
+        self.lgn_sInfo = torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32)
+        self.lgn_fr = torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32)
+
+    def _frame_response(self, frame: torch.Tensor, slot: int):
+        sp = self.net.spec
+        check(self.lib.vc_lgn_frame_response(None, ptr(frame), sp.width, sp.height, sp.nLGN,
+                                             sp.nSpatialSample1D, ptr(self.SCx), ptr(self.SCy),
+                                             ptr(self.SW), ptr(self.coneType), ptr(self.Rring[slot])),
+              "vc_lgn_frame_response")
+
+    # ------------------------------------------------------------------ steps
+    def run(self, nsteps: int, keep_fr: bool = False):
+        tm = self.tm
+        npf = tm["ntPerFrame"]
+        nF = self.frames.shape[0]
+        done = 0
+        while done < nsteps:
+            it = self.it
+            k_in = it % npf
+            if k_in == 0:
+                fidx = it // npf
+                self._frame_response(self.frames[fidx % nF], fidx % tm["maxFrame"])
+                check(self.lib.vc_lgn_steps(None, C.byref(self.lp), C.byref(self.ls), ptr(self.Rring),
+                                            it, npf, ptr(self.lgn_sInfo), ptr(self.lgn_fr)),
+                      "vc_lgn_steps")
+                if self.record:
+                    self.lgn_hist.append(self.lgn_sInfo.clone())
+            out = torch.empty(self.net.spec.nV1, dtype=torch.float32, device=self.dev) \
+                if self.record else self.sInfo_step
+            nxt = 1 - self.cur
+            check(self.lib.vc_v1_step(None, C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                                      C.byref(self.rec), C.byref(self.spk[self.cur]),
+                                      C.byref(self.spk[nxt]), ptr(self.lgn_sInfo[k_in]), it,
+                                      ptr(out), None, None), "vc_v1_step")
+            if self.record:
+                self.sInfo_hist.append(out)
+            self.cur = nxt
+            self.it += 1
+            done += 1
+
+    def finalize(self):
+        """Apply the last step's pending recurrent increments (the reference
+        applies them inside the same step, in recal_G)."""
+        check(self.lib.vc_v1_rec_apply(None, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
+                                       C.byref(self.spk[self.cur])), "vc_v1_rec_apply")
+        # the increments are now in the state: clear the pending list
+        check(self.lib.vc_spikes_clear(None, self.net.spec.nV1, C.byref(self.spk[self.cur])),
+              "vc_spikes_clear")
+
+    def sInfo(self) -> np.ndarray:
+        torch.cuda.synchronize()
+        return torch.stack(self.sInfo_hist).cpu().numpy()
+
+    def lgn_sinfo(self) -> np.ndarray:
+        torch.cuda.synchronize()
+        return torch.cat(self.lgn_hist).cpu().numpy()
+
+    def state_numpy(self) -> dict:
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in self.state.items()}
