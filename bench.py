@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark of the patchV1 simulation step on one MI355X.
+
+Workload: the full-patch plain forward -- 64 blocks x 1024 = 65,536 V1
+neurons (E AdEx, I LIF), 16,384 LGN cells, 30 LGN + 800 E + 200 I synapses
+per neuron, 256x256-px drifting grating at 125 Hz, dt = 0.125 ms.  One
+"step" is one dt of the whole path (LGN stage + V1 stages) with all inputs
+resident in HBM.  Synthetic, seeded network and stimulus.
+
+Prints one JSON line: neuron-steps/s over the timed steps, the roofline of
+the dominant kernel (the fused V1 step) from HIP events over the timed
+region, and a CPU baseline (the oracle, timed on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    name = "visual_cortex_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    d = ROOT / "visual-cortex_amd"
+    spec = importlib.util.spec_from_file_location(name, d / "__init__.py",
+                                                  submodule_search_locations=[str(d)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def v1_bytes_per_neuron_step(spec, rho: float) -> float:
+    """Algorithmic HBM bytes per V1 neuron-step of the fused V1 kernel:
+    13 persistent state floats read + written, tonicDep read, sInfo written
+    (4*(2*13 + 2) = 112), m LGN synapses (weight + index, 8 B each), the LGN
+    sInfo table amortised over the neurons (4*nLGN/nV1), and the event-driven
+    recurrent term 8*rho*K (K = outgoing synapses per neuron).  The
+    background drive is generated in-kernel (0 B)."""
+    K = spec.KE + spec.KI
+    return 112.0 + 8.0 * spec.m + 4.0 * spec.nLGN / spec.nV1 + 8.0 * rho * K
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16000)
+    ap.add_argument("--warmup", type=int, default=640)
+    ap.add_argument("--cpu-steps", type=int, default=800,
+                    help="steps of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the "
+              "multi-GPU sharding of this path is a design (see DESIGN.md), not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_package()
+    import numpy as np
+    import torch
+    from visual_cortex_amd import network
+    from visual_cortex_amd.sim import GpuSim
+
+    spec = network.spec_c2() if args.config == "c2" else network.spec_c1()
+    K, W = args.steps, args.warmup
+    spec = network.replace(spec, nt=K + W) if hasattr(network, "replace") else spec
+    t0 = time.time()
+    net = network.build_network(spec)
+    log(f"[bench] network {args.config}: nV1={spec.nV1} nLGN={spec.nLGN} "
+        f"rec synapses={net.n_rec_synapses()} built in {time.time() - t0:.1f}s")
+
+    sim = GpuSim(net, record=False)
+    sim.run(W)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup {W} steps done")
+
+    out = torch.empty(K, spec.nV1, dtype=torch.float32, device="cuda")
+    events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run(K, out=out, events=events)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    log(f"[bench] timed {K} steps in {elapsed:.3f}s")
+
+    spikes = int((out >= 1.0).sum().item())
+    rho = spikes / float(K * spec.nV1)
+    ev_ms = sum(a.elapsed_time(b) for a, b, n in events)
+    ev_n = sum(n for a, b, n in events)
+    v1_launch_s = ev_ms / 1e3 / ev_n
+    bpn = v1_bytes_per_neuron_step(spec, rho)
+    bytes_per_launch = bpn * spec.nV1
+    achieved = bytes_per_launch / v1_launch_s / 1e9
+    value = spec.nV1 * K / elapsed
+    isE = (np.arange(spec.nV1) % spec.blockSize) < spec.nE
+    per_n = (out >= 1.0).sum(0).cpu().numpy()
+    T = K * spec.dt / 1000.0
+    log(f"[bench] rates: E {per_n[isE].mean() / T:.2f} Hz, I {per_n[~isE].mean() / T:.2f} Hz; "
+        f"rho={rho:.3e}; V1 launch {v1_launch_s * 1e6:.2f} us; {bpn:.1f} B/neuron-step")
+    del out
+
+    cpu = None
+    if args.cpu_steps > 0:
+        cpu = cpu_baseline(pkg, net, sim, args.cpu_steps)
+
+    line = {
+        "metric": "V1 neuron-steps/s at 64 blocks dt=0.125ms",
+        "value": value,
+        "unit": "neuron-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded network, drifting-grating stimulus)",
+        "config": {"workload": f"plain forward, {'full patch' if args.config == 'c2' else 'small'}: "
+                               f"{spec.nblock} blocks x {spec.blockSize} neurons, {spec.nLGN} LGN, "
+                               f"m={spec.m}, K={spec.KE}+{spec.KI}, {spec.width}x{spec.height} px, "
+                               f"dt={spec.dt} ms",
+                   "nV1": spec.nV1, "nLGN": spec.nLGN, "dt_ms": spec.dt,
+                   "spikes_per_neuron_step": rho},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_v1_step", "bytes_per_launch": bytes_per_launch,
+                     "launch_us": v1_launch_s * 1e6},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(pkg, net, sim, nsteps):
+    """The oracle (C restatement of the reference's step equations, OpenMP)
+    on the first `nsteps` steps of the same network from the same initial
+    state, LGN tables taken from the GPU so that only the step loop is timed."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    tables = dict(TW=sim.TW.cpu().numpy(), SW=sim.SW.cpu().numpy(), SCx=sim.SCx.cpu().numpy(),
+                  SCy=sim.SCy.cpu().numpy(), maxConvol=sim.maxConvol.cpu().numpy())
+    t0 = time.time()
+    o = oracle.OracleSim(net, pkg, tables=tables)
+    log(f"[bench] cpu baseline set-up {time.time() - t0:.1f}s (dense conMat)")
+    t0 = time.perf_counter()
+    o.run(nsteps, record=False)
+    el = time.perf_counter() - t0
+    sp = net.spec
+    log(f"[bench] cpu baseline {nsteps} steps in {el:.2f}s")
+    return {"value": sp.nV1 * nsteps / el, "unit": "neuron-steps/s", "cores": o.o.num_threads(),
+            "kind": "port",
+            "sample": f"first {nsteps} steps ({nsteps * sp.dt:g} ms) of the same workload, "
+                      f"same network and initial state, OpenMP over neurons/cells"}
+
+
+if __name__ == "__main__":
+    main()

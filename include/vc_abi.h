@@ -125,15 +125,26 @@ typedef struct {
  * `prev` (the previous step's spikes), decay, LGN + background input, RK2
  * update, spike time, refractory handling, compaction into `next`.
  * out_sInfo[nV1] receives 1 + tsp/dt for a spike and v otherwise;
- * out_dep[nV1] (may be NULL) receives depC; out_gE_prev (may be NULL)
- * receives, as [ngE + ngI][nV1] g values, the conductances at the end of the
- * previous step (after its recurrent increments), i.e. the reference's gE/gI
- * output of that step. */
+ * out_dep[nV1] (may be NULL) receives depC; out_g_prev (may be NULL)
+ * receives [2*(ngE+ngI)][nV1] = gE[ngE], gI[ngI], hE[ngE], hI[ngI]: the
+ * cortical conductances at the end of the previous step after its recurrent
+ * increments, i.e. the reference's gE/gI/hE/hI output of that step. */
 int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                const vc_ff_conn *ff, const vc_rec_conn *rec,
                const vc_spikes *prev, const vc_spikes *next,
                const float *lgnSInfo, uint32_t step,
                float *out_sInfo, float *out_dep, float *out_g_prev);
+
+/* `nsteps` consecutive vc_v1_step launches from the native side: step i
+ * reads the spikes of spk2[cur ^ (i&1)], writes spk2[cur ^ ((i+1)&1)], takes
+ * its LGN input from lgnSInfo + i*lgnStride and writes out_sInfo +
+ * i*outStride (strides in floats; 0 reuses one row).  After the call the
+ * last step's spikes are in spk2[cur ^ (nsteps&1)].  (The reference's host
+ * loop, src/patch.cu:5966-6913, launched one step at a time.) */
+int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+              const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
+              uint32_t cur, const float *lgnSInfo, uint64_t lgnStride, uint32_t it0,
+              uint32_t nsteps, float *out_sInfo, uint64_t outStride);
 
 /* Apply the recurrent increments of `prev` to gE/hE/gI/hI without stepping
  * (used after the last step, before reading or saving the state). */
@@ -188,10 +199,11 @@ typedef struct {
 } vc_lgn_state;
 
 /* Steps it0 .. it0+nsteps-1 of stages a (temporal part) and b for every LGN
- * cell.  Rring is [maxFrame][nLGN][2]: the spatial responses of frame f are
+ * cell; the steps must lie within one frame period (no frame enters after
+ * it0).  Rring is [maxFrame][nLGN][2]: the spatial responses of frame f are
  * in slot f mod maxFrame (pre-stimulus frames f < 0 included).  Writes
- * sInfo[nsteps][nLGN] (nsp + mean tsp/dt) and, if fr != NULL,
- * fr[nsteps][nLGN] in Hz. */
+ * sInfo[nsteps][nLGN] (nsp + mean tsp/dt) and fr[nsteps][nLGN] (Hz; required,
+ * it is also the hand-over buffer between the rate and spike kernels). */
 int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
                  const float *Rring, uint32_t it0, uint32_t nsteps,
                  float *sInfo, float *fr);

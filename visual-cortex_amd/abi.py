@@ -101,6 +101,9 @@ SIGNATURES = {
     "vc_v1_step": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
                    C.POINTER(RecConn), C.POINTER(Spikes), C.POINTER(Spikes), vp, u32,
                    vp, vp, vp],
+    "vc_v1_run": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
+                  C.POINTER(RecConn), C.POINTER(Spikes), u32, vp, C.c_uint64, u32, u32, vp,
+                  C.c_uint64],
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],

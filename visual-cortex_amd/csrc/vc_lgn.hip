@@ -7,12 +7,13 @@
 //      use the reference's block_reduce tree (32-lane halving, then halving
 //      over the warp partials), so R is bitwise the reference formulation's
 //      per-frame value (src/discrete_input_convol.cu:1003-1069, 1143-1211).
-//  k_lgn_steps  for a run of steps: one thread per cell keeps the cell's
-//      temporal weights in registers, forms tap x frame products from the
-//      cached per-frame responses (the separable form: R depends only on the
-//      frame, so it is not recomputed per step as the reference does), sums
-//      them with the same tree, then the logistic rate and the time-rescaled
-//      Poisson spike draw (src/discrete_input_convol.cu:1587-1821).
+//  k_lgn_rate  for a run of steps within one frame period: one wavefront
+//      per cell, lane = step, forms tap x frame products from the cached
+//      per-frame responses (the separable form: R depends only on the frame,
+//      so it is not recomputed per step as the reference does), sums them
+//      with the same tree and applies the logistic rate.
+//  k_lgn_spike  the time-rescaled Poisson spike draw, one thread per cell,
+//      sequential over the steps (src/discrete_input_convol.cu:1587-1821).
 //  k_lgn_tw / k_lgn_sw / k_lgn_sc / k_lgn_maxconvol  one-time tables
 //      (store_PM, parvo_maxConvol).
 //
@@ -141,77 +142,85 @@ __device__ __forceinline__ float transform(float C50, float K, float A, float B,
     return A * X + B;
 }
 
-__global__ __launch_bounds__(256) void k_lgn_steps(vc_lgn_params p, vc_lgn_state st,
-                                                   const float *__restrict__ Rring,
+// Stage a (temporal part) and the rate of stage b for every (cell, step):
+// one wavefront per cell, lane = step, so the cell's temporal weights and
+// frame responses are wave-uniform (scalar loads); the tap products and the
+// reference's tap-sum tree are evaluated in registers.
+__global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state st,
+                                                  const float *__restrict__ Rring,
+                                                  uint32_t it0, uint32_t nsteps,
+                                                  float *__restrict__ frOut) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wpc = (nsteps + 63u) / 64u;               // wavefronts per cell
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(g >> 6);
+    const uint32_t c = wave / wpc;
+    if (c >= p.nLGN) return;
+    const uint32_t s = (wave - c * wpc) * 64u + (g & 63u);
+    const uint32_t nKS = p.nKernelSample;
+    const uint32_t it = it0 + (s < nsteps ? s : 0u);
+    const int64_t base = (int64_t)it + (int64_t)p.iKernelSampleT0 - (int64_t)p.nRetrace + 1;
+    // responses of the frames under the first step's window, wave-uniform
+    const int64_t fw = floor_div((int64_t)(it0 + (wave - c * wpc) * 64u) + (int64_t)p.iKernelSampleT0 -
+                                     (int64_t)p.nRetrace + 1, (int64_t)p.ntPerFrame);
+    float Rf[2][kMaxFrame + 1];
+#pragma unroll
+    for (int d = 0; d <= kMaxFrame; ++d) {
+        int64_t slot = (fw + d) % (int64_t)p.maxFrame;
+        if (slot < 0) slot += p.maxFrame;
+        Rf[0][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + 0];
+        Rf[1][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + 1];
+    }
+    float conv[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        float a0[32], a1[32];
+#pragma unroll
+        for (int i = 0; i < kMaxTaps; ++i) {
+            float v = 0.0f;
+            if (i < (int)nKS) {
+                const int64_t tau = base + (int64_t)i * p.kernelSampleInterval;
+                const int d = (int)(floor_div(tau, (int64_t)p.ntPerFrame) - fw);
+                float Rv = Rf[r][0];
+#pragma unroll
+                for (int dd = 1; dd <= kMaxFrame; ++dd)
+                    if (d == dd) Rv = Rf[r][dd];
+                v = Rv * st.TW[((size_t)c * 2 + r) * nKS + i];
+            }
+            if (i < 32) a0[i] = v; else a1[i - 32] = v;
+        }
+        const float P0 = tree32_regs(a0);
+        const float P1 = tree32_regs(a1);
+        // halving over the 8 warp partials of a 256-thread block, 6 of them zero
+        const float sum = ((P0 + 0.0f) + (0.0f + 0.0f)) + ((P1 + 0.0f) + (0.0f + 0.0f));
+        conv[r] = 0.0f + sum;
+    }
+    const float convolC = conv[0] * st.kC[c];
+    const float convolS = conv[1] * st.kS[c];
+    const float convol = convolC + convolS;
+    float current = convol * p.convolRatio;
+    if (current < 0.0f) current = 0.0f;
+    const float mx = st.maxConvol[c];
+    const float fr = p.frRatio * mx * transform(st.c50[c], st.sharpness[c], st.logA[c], st.logB[c],
+                                                current / mx);
+    if (s < nsteps) frOut[(size_t)s * p.nLGN + c] = fr;
+}
+
+// Stage b, the time-rescaled Poisson draw (get_spike,
+// src/discrete_input_convol.cu:1587-1622): sequential in time per cell.
+__global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state st,
                                                    uint32_t it0, uint32_t nsteps,
-                                                   float *__restrict__ sInfoOut,
-                                                   float *__restrict__ frOut) {
+                                                   const float *__restrict__ frIn,
+                                                   float *__restrict__ sInfoOut) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= p.nLGN) return;
-    const uint32_t nKS = p.nKernelSample;
-    float tw[2][kMaxTaps];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int i = 0; i < kMaxTaps; ++i)
-            tw[r][i] = (i < (int)nKS) ? st.TW[((size_t)c * 2 + r) * nKS + i] : 0.0f;
-    const float kC = st.kC[c], kS = st.kS[c], mx = st.maxConvol[c];
-    const float c50 = st.c50[c], K = st.sharpness[c], A = st.logA[c], B = st.logB[c];
     float lTR = st.leftTimeRate[c], lNL = st.lastNegLogRand[c];
-    const int64_t T0 = (int64_t)p.iKernelSampleT0 - (int64_t)p.nRetrace + 1;
-
+    float frn = frIn[c];
     for (uint32_t s = 0; s < nsteps; ++s) {
         const uint32_t it = it0 + s;
-        const int64_t base = (int64_t)it + T0;
-        const int64_t f0 = floor_div(base, (int64_t)p.ntPerFrame);
-        // responses of the (at most maxFrame) frames under the window
-        float Rf[2][kMaxFrame];
-#pragma unroll
-        for (int d = 0; d < kMaxFrame; ++d) {
-            int64_t slot = (f0 + d) % (int64_t)p.maxFrame;
-            if (slot < 0) slot += p.maxFrame;
-            if (d < (int)p.maxFrame) {
-                Rf[0][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + 0];
-                Rf[1][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + 1];
-            } else {
-                Rf[0][d] = 0.0f;
-                Rf[1][d] = 0.0f;
-            }
-        }
-        float conv[2];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            float a0[32], a1[32];
-#pragma unroll
-            for (int i = 0; i < kMaxTaps; ++i) {
-                float v = 0.0f;
-                if (i < (int)nKS) {
-                    int64_t tau = base + (int64_t)i * p.kernelSampleInterval;
-                    int d = (int)(floor_div(tau, (int64_t)p.ntPerFrame) - f0);
-                    float Rv = Rf[r][0];
-#pragma unroll
-                    for (int dd = 1; dd < kMaxFrame; ++dd)
-                        if (d == dd) Rv = Rf[r][dd];
-                    v = Rv * tw[r][i];
-                }
-                if (i < 32) a0[i] = v; else a1[i - 32] = v;
-            }
-            float P0 = tree32_regs(a0);
-            float P1 = tree32_regs(a1);
-            // halving over 8 warp partials of which 6 are zero (256-thread block)
-            float sum = ((P0 + 0.0f) + (0.0f + 0.0f)) + ((P1 + 0.0f) + (0.0f + 0.0f));
-            conv[r] = 0.0f + sum;
-        }
-        float convolC = conv[0] * kC;
-        float convolS = conv[1] * kS;
-        float convol = convolC + convolS;
-        float current = convol * p.convolRatio;
-        if (current < 0.0f) current = 0.0f;
-        float fr = p.frRatio * mx * transform(c50, K, A, B, current / mx);
-
-        // get_spike, src/discrete_input_convol.cu:1587-1622
-        float rate = fr / 1000.0f;
-        float rT = p.dt * rate;
+        const float fr = frn;
+        if (s + 1 < nsteps) frn = frIn[(size_t)(s + 1) * p.nLGN + c];
+        const float rate = fr / 1000.0f;
+        const float rT = p.dt * rate;
         float next_rT = lNL - lTR;
         uint32_t nsp = 0;
         float tsp = 0.0f;
@@ -227,9 +236,9 @@ __global__ __launch_bounds__(256) void k_lgn_steps(vc_lgn_params p, vc_lgn_state
                 if ((kdraw & 3u) == 0u)
                     pack = vc_draw4(p.seed_lo, p.seed_hi, c, it, VC_STREAM_LGN_SPIKE,
                                     kdraw >> 2, p.trial);
-                float u = vc_u01_open0(pack.v[kdraw & 3u]);
+                const float u = vc_u01_open0(pack.v[kdraw & 3u]);
                 kdraw++;
-                float x = -vc_logf(u);
+                const float x = -vc_logf(u);
                 lNLn = (x > tr) ? x : tr;
                 old_rT = next_rT;
                 next_rT += lNLn;
@@ -238,10 +247,9 @@ __global__ __launch_bounds__(256) void k_lgn_steps(vc_lgn_params p, vc_lgn_state
             lTR = rT - old_rT;
             tsp = old_rT / (rate * (float)nsp);
         }
-        float sInfo = (float)nsp + tsp / p.dt;
+        const float sInfo = (float)nsp + tsp / p.dt;
         if (sInfo > 0.0f) lNL = lNLn;
         sInfoOut[(size_t)s * p.nLGN + c] = sInfo;
-        if (frOut) frOut[(size_t)s * p.nLGN + c] = fr;
     }
     st.leftTimeRate[c] = lTR;
     st.lastNegLogRand[c] = lNL;
@@ -496,9 +504,21 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
         vc_set_error("maxFrame %u too small for a %u-step window", p->maxFrame, p->nRetrace);
         return VC_E_INVALID;
     }
+    if (!fr) {
+        vc_set_error("vc_lgn_steps: fr (the [nsteps][nLGN] rate buffer) is required");
+        return VC_E_INVALID;
+    }
+    if (nsteps > p->ntPerFrame - it0 % p->ntPerFrame) {
+        vc_set_error("vc_lgn_steps: steps %u..%u cross a frame boundary", it0, it0 + nsteps - 1);
+        return VC_E_INVALID;
+    }
     if (nsteps == 0 || p->nLGN == 0) return VC_OK;
-    hipLaunchKernelGGL(k_lgn_steps, dim3((p->nLGN + 255) / 256), dim3(256), 0,
-                       (hipStream_t)stream, *p, *s, Rring, it0, nsteps, sInfo, fr);
+    const uint64_t waves = (uint64_t)p->nLGN * ((nsteps + 63u) / 64u);
+    hipLaunchKernelGGL(k_lgn_rate, dim3((uint32_t)((waves * 64 + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, *p, *s, Rring, it0, nsteps, fr);
+    VC_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_lgn_spike, dim3((p->nLGN + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, *p, *s, it0, nsteps, fr, sInfo);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

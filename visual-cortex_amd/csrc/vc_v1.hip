@@ -1,36 +1,40 @@
 // vc_v1.hip -- stages c, d, e of the V1 simulation step on gfx950.
 //
-// One thread per V1 neuron, workgroups of up to 256 threads (4 wavefronts)
-// inside one V1 block, so every wavefront is one 64-neuron tile.
-//
-//   c-rec  recurrent gather of the previous step's spikes: the workgroup
-//          stages the spike masks of its neighbour blocks in LDS; each
-//          wavefront lists the spikes (neighbour order, ascending neuron id)
-//          into an LDS batch, fetches one masked-CSR row segment per spike
-//          with one vector load (lane k <- spike k), then walks the batch in
-//          order, each lane loading only the weights whose mask bit it owns
-//          (rank via mbcnt).  Sums are accumulated in the reference order
-//          (src/coredynamics.cu:2721-2838).
-//   c-FF   LGN input, ELL/transposed like the reference (src/coredynamics.cu
-//          :1457-1513), loads issued 8 synapses at a time.
-//   d      decay, Heun RK2 AdEx/LIF, spike time, refractory back-up
-//          (src/coredynamics.cu:1367-1589, src/coredynamics.cuh:27-118).
-//   e      ballot compaction: one spike mask per tile and the unit increments
-//          at slot tile*64 + rank (src/coredynamics.cu:1994-2046).
+// One thread per V1 neuron, workgroups of up to 256 threads inside one V1
+// block, so every wavefront is one 64-neuron tile.  The kernel is latency
+// bound at the full-patch size (one wavefront per SIMD), so it is organised
+// around few dependent memory round trips:
+//   round 1: state, the LGN synapse indices/strengths of the neuron, and the
+//            spike masks of the 3x3 neighbour blocks (one 64-bit mask per
+//            tile, held lane-parallel: lane l <- tile entry l);
+//   round 2: the LGN spike words, and for the spikes of the neighbourhood
+//            (listed in neighbour order, ascending neuron id, by a ballot
+//            scan over the non-empty tiles) one masked-CSR row-segment header
+//            per spike (lane k <- spike k);
+//   round 3: the synaptic weights each lane owns (rank via mbcnt).
+// Sums are accumulated in the reference order: recurrent E and I sums over
+// neighbour blocks then ascending presynaptic id (src/coredynamics.cu
+// :2721-2838), LGN synapses in list order (src/coredynamics.cu:1466-1502).
+//   stage d: decay, Heun RK2 AdEx/LIF, spike time, refractory back-up
+//            (src/coredynamics.cu:1367-1589, src/coredynamics.cuh:27-118);
+//   stage e: ballot compaction -- one spike mask per tile and the unit
+//            increments at slot tile*64 + rank (src/coredynamics.cu:1994-2046).
 #include "vc_common.h"
 
 namespace {
 
-constexpr int kWG = 256;          // max workgroup size
-constexpr int kMaxNearTiles = 512; // neighbour blocks x tiles per block staged in LDS
+constexpr int kWG = 256;           // max workgroup size
+constexpr int kMaskRegs = 8;       // neighbour-tile masks held per lane: 8*64 entries
 constexpr int kBatch = 64;
+constexpr int kFF = 32;            // LGN synapses loaded per round
+constexpr int kW = 16;             // weight loads in flight per lane
 
 // per-launch constants derived on the host from vc_v1_params
 struct V1Const {
     float etrFF[VC_MAX_NG], etdFF[VC_MAX_NG];
     float etrE[VC_MAX_NG], etdE[VC_MAX_NG];
     float etrI[VC_MAX_NG], etdI[VC_MAX_NG];
-    uint32_t nSeg;                // blockSize / 64
+    uint32_t nSeg;                 // blockSize / 64
 };
 
 __device__ __forceinline__ void single_input(float &g, float &h, float f, float ddt,
@@ -59,147 +63,136 @@ __device__ __forceinline__ int type_of(const vc_v1_params &p, uint32_t local) {
     return t;
 }
 
-__device__ __forceinline__ uint64_t emask_of(uint32_t tile, uint32_t nE) {
-    uint32_t lo = tile * 64u;
-    if (nE >= lo + 64u) return ~0ull;
-    if (nE <= lo) return 0ull;
-    return (1ull << (nE - lo)) - 1ull;
-}
-
-// Recurrent increments of the spikes in `prev` for the calling neuron.
-// Must be called by all threads of the workgroup (barriers inside).
-__device__ __forceinline__ void rec_gather(const vc_v1_params &p, const V1Const &k,
-                                           const vc_rec_conn &rec, const vc_spikes &prev,
-                                           uint32_t b, uint32_t local, int t,
-                                           float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
-                                           float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
-    __shared__ uint64_t smask[kMaxNearTiles];
-    __shared__ uint32_t sbid[64];
-    __shared__ uint32_t brow[kWG / 64][2 * kBatch];
-    __shared__ uint32_t bslot[kWG / 64][2 * kBatch];
-
-    const uint32_t nTile = k.nSeg;
-    const uint32_t nb = rec.nNear[b];
-    const uint32_t lane = vc_lane();
-    const uint32_t wv = threadIdx.x >> 6;
-    const uint32_t myseg = local >> 6;
-    const uint32_t bs = p.blockSize;
-
-    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
-        sbid[i] = rec.nbr[b * rec.maxNear + i];
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nb * nTile; i += blockDim.x) {
-        uint32_t ib = i / nTile, tt = i % nTile;
-        smask[i] = prev.mask[sbid[ib] * nTile + tt];
+// Adds the recurrent increments of a batch of K <= 64 spikes listed in
+// brow/bslot (this wavefront's LDS rows) to the calling lane's sums.
+__device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_conn &rec,
+                                            const vc_spikes &prev, uint32_t K, uint32_t lane,
+                                            uint32_t nSeg, uint32_t myseg, int t,
+                                            const uint32_t *brow, const uint32_t *bslot,
+                                            float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
+                                            float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
+    // lane kk fetches the row-segment header and the increments of spike kk
+    uint64_t rm = 0;
+    uint32_t off = 0, isI = 0;
+    float og0 = 0.f, oh0 = 0.f, og1 = 0.f, oh1 = 0.f;
+    if (lane < K) {
+        const uint32_t row = brow[lane];
+        const uint32_t sw = bslot[lane];
+        isI = sw >> 31;
+        const uint32_t slot = sw & 0x7fffffffu;
+        const size_t hid = (size_t)row * nSeg + myseg;
+        rm = rec.rowMask[hid];
+        off = rec.rowOff[hid];
+        og0 = prev.og[slot];
+        oh0 = prev.oh[slot];
+        og1 = prev.og[p.nV1 + slot];
+        oh1 = prev.oh[p.nV1 + slot];
     }
-    __syncthreads();
-
-    uint32_t cnt = 0;   // entries in this wavefront's batch (uniform)
-
-    auto flush = [&](uint32_t K) {
-        // lane kk loads row segment header and increments of spike kk
-        uint64_t rm = 0;
-        uint32_t off = 0;
-        float og0 = 0.f, oh0 = 0.f, og1 = 0.f, oh1 = 0.f;
-        uint32_t isI = 0;
-        if (lane < K) {
-            uint32_t row = brow[wv][lane];
-            uint32_t sw = bslot[wv][lane];
-            isI = sw >> 31;
-            uint32_t slot = sw & 0x7fffffffu;
-            size_t hid = (size_t)row * nTile + myseg;
-            rm = rec.rowMask[hid];
-            off = rec.rowOff[hid];
-            og0 = prev.og[slot];
-            oh0 = prev.oh[slot];
-            og1 = prev.og[p.nV1 + slot];
-            oh1 = prev.oh[p.nV1 + slot];
-        }
-        for (uint32_t k0 = 0; k0 < K; k0 += 8) {
-            float wv8[8];
+    const float pE0 = p.pE[t * p.ngE + 0], pE1 = p.pE[t * p.ngE + (p.ngE > 1 ? 1 : 0)];
+    const float pI0 = p.pI[t * p.ngI + 0], pI1 = p.pI[t * p.ngI + (p.ngI > 1 ? 1 : 0)];
+    for (uint32_t k0 = 0; k0 < K; k0 += kW) {
+        float wv[kW];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                uint32_t kk = k0 + j;
-                wv8[j] = 0.0f;
-                if (kk < K) {
-                    const uint64_t rmk = vc_readlane_u64(rm, kk);
-                    const uint32_t offk = vc_readlane_u32(off, kk);
-                    const uint32_t idx = offk + vc_mbcnt(rmk);
-                    if ((rmk >> lane) & 1ull) wv8[j] = rec.w[idx];
-                }
+        for (int j = 0; j < kW; ++j) {
+            const uint32_t kk = k0 + j;
+            wv[j] = 0.0f;
+            if (kk < K) {
+                const uint64_t rmk = vc_readlane_u64(rm, kk);
+                const uint32_t idx = vc_readlane_u32(off, kk) + vc_mbcnt(rmk);
+                if ((rmk >> lane) & 1ull) wv[j] = rec.w[idx];
             }
+        }
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                uint32_t kk = k0 + j;
-                if (kk < K) {
-                    // wave-uniform values of spike kk, read before any divergence
-                    const uint64_t rmk = vc_readlane_u64(rm, kk);
-                    const float g0 = vc_readlane_f32(og0, kk), h0 = vc_readlane_f32(oh0, kk);
-                    const float g1 = vc_readlane_f32(og1, kk), h1 = vc_readlane_f32(oh1, kk);
-                    const uint32_t inh = vc_readlane_u32(isI, kk);
-                    if ((rmk >> lane) & 1ull) {
-                        float strength = wv8[j];
-                        if (inh == 0u) {
-                            float str = strength * p.pE[t * p.ngE + 0];
-                            lgE[0] += str * g0;
-                            lhE[0] += str * h0;
-                            if (p.ngE > 1) {
-                                str = strength * p.pE[t * p.ngE + 1];
-                                lgE[1] += str * g1;
-                                lhE[1] += str * h1;
-                            }
-                        } else {
-                            float str = strength * p.pI[t * p.ngI + 0];
-                            lgI[0] += str * g0;
-                            lhI[0] += str * h0;
-                            if (p.ngI > 1) {
-                                str = strength * p.pI[t * p.ngI + 1];
-                                lgI[1] += str * g1;
-                                lhI[1] += str * h1;
-                            }
+        for (int j = 0; j < kW; ++j) {
+            const uint32_t kk = k0 + j;
+            if (kk < K) {
+                const uint64_t rmk = vc_readlane_u64(rm, kk);
+                const float g0 = vc_readlane_f32(og0, kk), h0 = vc_readlane_f32(oh0, kk);
+                const float g1 = vc_readlane_f32(og1, kk), h1 = vc_readlane_f32(oh1, kk);
+                const uint32_t inh = vc_readlane_u32(isI, kk);
+                if ((rmk >> lane) & 1ull) {
+                    const float strength = wv[j];
+                    if (inh == 0u) {
+                        float str = strength * pE0;
+                        lgE[0] += str * g0;
+                        lhE[0] += str * h0;
+                        if (p.ngE > 1) {
+                            str = strength * pE1;
+                            lgE[1] += str * g1;
+                            lhE[1] += str * h1;
+                        }
+                    } else {
+                        float str = strength * pI0;
+                        lgI[0] += str * g0;
+                        lhI[0] += str * h0;
+                        if (p.ngI > 1) {
+                            str = strength * pI1;
+                            lgI[1] += str * g1;
+                            lhI[1] += str * h1;
                         }
                     }
                 }
             }
         }
-    };
+    }
+}
 
-    for (uint32_t ib = 0; ib < nb; ++ib) {
-        const uint32_t bid = sbid[ib];
-        const uint32_t rowbase = (b * rec.maxNear + ib) * bs;
-        for (uint32_t part = 0; part < 2; ++part) {          // E spikes, then I
-            for (uint32_t tt = 0; tt < nTile; ++tt) {
-                const uint64_t m = smask[ib * nTile + tt];
-                if (m == 0ull) continue;
-                const uint64_t em = emask_of(tt, p.nE);
-                const uint64_t mm = part == 0 ? (m & em) : (m & ~em);
-                if (mm == 0ull) continue;
-                const uint32_t c = (uint32_t)__builtin_popcountll(mm);
-                if ((mm >> lane) & 1ull) {
-                    uint32_t pos = cnt + vc_mbcnt(mm);
-                    uint32_t rank = vc_mbcnt(m);
-                    brow[wv][pos] = rowbase + tt * 64u + lane;
-                    bslot[wv][pos] = (bid * bs + tt * 64u + rank) | (part << 31);
+// Recurrent increments of the spikes of `prev` for the calling neuron.
+// mk[c] holds, in lane l, the spike mask of neighbour-tile entry c*64 + l
+// (entry e = ib*nSeg + tt); bidv[c] the neighbour block id of that entry.
+__device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_conn &rec,
+                                           const vc_spikes &prev, uint32_t b, uint32_t myseg,
+                                           uint32_t nSeg, uint32_t nEnt, int t,
+                                           const uint64_t mk[kMaskRegs], const uint32_t bidv[kMaskRegs],
+                                           uint32_t *brow, uint32_t *bslot,
+                                           float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
+                                           float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
+    const uint32_t lane = vc_lane();
+    const uint32_t bs = p.blockSize;
+    uint32_t cnt = 0;   // entries in the batch (wave-uniform)
+#pragma unroll
+    for (int c = 0; c < kMaskRegs; ++c) {
+        if ((uint32_t)c * 64u >= nEnt) break;
+        uint64_t nz = __ballot(mk[c] != 0ull);
+        while (nz) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(nz);
+            nz &= nz - 1ull;
+            const uint32_t e = (uint32_t)c * 64u + j;
+            const uint64_t m = vc_readlane_u64(mk[c], j);
+            const uint32_t bid = vc_readlane_u32(bidv[c], j);
+            const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
+            if ((m >> lane) & 1ull) {
+                const uint32_t rank = vc_mbcnt(m);
+                const uint32_t pos = cnt + rank;
+                const uint32_t pre = tt * 64u + lane;
+                brow[pos] = (b * rec.maxNear + ib) * bs + pre;
+                bslot[pos] = (bid * bs + tt * 64u + rank) | ((pre >= p.nE ? 1u : 0u) << 31);
+            }
+            cnt += (uint32_t)__builtin_popcountll(m);
+            if (cnt >= (uint32_t)kBatch) {
+                __builtin_amdgcn_wave_barrier();
+                flush_batch(p, rec, prev, kBatch, lane, nSeg, myseg, t, brow, bslot,
+                            lgE, lhE, lgI, lhI);
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t rest = cnt - kBatch;
+                uint32_t r0 = 0, r1 = 0;
+                if (lane < rest) {
+                    r0 = brow[kBatch + lane];
+                    r1 = bslot[kBatch + lane];
                 }
-                cnt += c;
-                if (cnt >= (uint32_t)kBatch) {
-                    __builtin_amdgcn_wave_barrier();
-                    flush(kBatch);
-                    __builtin_amdgcn_wave_barrier();
-                    uint32_t rest = cnt - kBatch;
-                    if (lane < rest) {
-                        brow[wv][lane] = brow[wv][kBatch + lane];
-                        bslot[wv][lane] = bslot[wv][kBatch + lane];
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    cnt = rest;
+                __builtin_amdgcn_wave_barrier();
+                if (lane < rest) {
+                    brow[lane] = r0;
+                    bslot[lane] = r1;
                 }
+                __builtin_amdgcn_wave_barrier();
+                cnt = rest;
             }
         }
     }
     if (cnt > 0) {
         __builtin_amdgcn_wave_barrier();
-        flush(cnt);
+        flush_batch(p, rec, prev, cnt, lane, nSeg, myseg, t, brow, bslot, lgE, lhE, lgI, lhI);
     }
 }
 
@@ -211,29 +204,80 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
                                                  uint32_t step, float *__restrict__ out_sInfo,
                                                  float *__restrict__ out_dep,
                                                  float *__restrict__ out_g_prev) {
+    __shared__ uint32_t sbrow[kWG / 64][2 * kBatch];
+    __shared__ uint32_t sbslot[kWG / 64][2 * kBatch];
     const uint32_t nV1 = p.nV1;
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t b = n / p.blockSize;
     const uint32_t local = n - b * p.blockSize;
+    const uint32_t lane = vc_lane();
+    const uint32_t wv = threadIdx.x >> 6;
     const int t = type_of(p, local);
     const float dt = p.dt;
+    const uint32_t nSeg = k.nSeg;
+    const uint32_t myseg = local >> 6;
 
-    float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
-    float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
-    rec_gather(p, k, rec, prev, b, local, t, lgE, lhE, lgI, lhI);
-
-    // conductances at the end of the previous step (reference: after recal_G)
+    // ---- round 1: neighbour spike masks (lane-parallel), LGN synapses, state
+    const uint32_t nb = rec.nNear[b];
+    const uint32_t nEnt = nb * nSeg;
+    uint64_t mk[kMaskRegs];
+    uint32_t bidv[kMaskRegs];
+#pragma unroll
+    for (int c = 0; c < kMaskRegs; ++c) {
+        const uint32_t e = (uint32_t)c * 64u + lane;
+        mk[c] = 0ull;
+        bidv[c] = 0u;
+        if (e < nEnt) {
+            const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
+            const uint32_t bid = rec.nbr[b * rec.maxNear + ib];
+            bidv[c] = bid;
+            mk[c] = prev.mask[bid * nSeg + tt];
+        }
+    }
+    const uint32_t mLGN = kUpdate ? ff.nLGN[n] : 0u;
+    float fv[kFF];
+    uint32_t fid[kFF];
+#pragma unroll
+    for (int j = 0; j < kFF; ++j) {
+        fv[j] = 0.0f;
+        fid[j] = 0u;
+        if ((uint32_t)j < mLGN) {
+            const uint32_t lid = (uint32_t)j * nV1 + n;
+            fv[j] = ff.sLGN[lid];
+            fid[j] = ff.idLGN[lid];
+        }
+    }
     float ge[VC_MAX_NG], he[VC_MAX_NG], gi[VC_MAX_NG], hi[VC_MAX_NG];
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        ge[ig] = he[ig] = gi[ig] = hi[ig] = 0.0f;
         if (ig < p.ngE) {
-            ge[ig] = s.gE[ig * nV1 + n] + lgE[ig];
-            he[ig] = s.hE[ig * nV1 + n] + lhE[ig];
+            ge[ig] = s.gE[ig * nV1 + n];
+            he[ig] = s.hE[ig * nV1 + n];
         }
         if (ig < p.ngI) {
-            gi[ig] = s.gI[ig * nV1 + n] + lgI[ig];
-            hi[ig] = s.hI[ig * nV1 + n] + lhI[ig];
+            gi[ig] = s.gI[ig * nV1 + n];
+            hi[ig] = s.hI[ig * nV1 + n];
         }
+    }
+
+    // ---- round 2: LGN spike words, then the recurrent gather
+    float sv[kFF];
+#pragma unroll
+    for (int j = 0; j < kFF; ++j) sv[j] = ((uint32_t)j < mLGN) ? lgnSInfo[fid[j]] : 0.0f;
+
+    float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
+    float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
+    rec_gather(p, rec, prev, b, myseg, nSeg, nEnt, t, mk, bidv, sbrow[wv], sbslot[wv],
+               lgE, lhE, lgI, lhI);
+
+    // conductances at the end of the previous step (reference: after recal_G)
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        ge[ig] = ge[ig] + lgE[ig];
+        he[ig] = he[ig] + lhE[ig];
+        gi[ig] = gi[ig] + lgI[ig];
+        hi[ig] = hi[ig] + lhI[ig];
     }
     if (out_g_prev) {
         const uint32_t ng = p.ngE + p.ngI;
@@ -302,6 +346,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     float g0[VC_MAX_NG], h0[VC_MAX_NG], g1[VC_MAX_NG], h1[VC_MAX_NG];
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        g0[ig] = h0[ig] = g1[ig] = h1[ig] = 0.0f;
         if (ig < p.ngFF) {
             g1[ig] = s.gFF[ig * nV1 + n];
             h1[ig] = s.hFF[ig * nV1 + n];
@@ -318,63 +363,62 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
             if (ig < p.ngFF) decay_var(g0[ig], h0[ig], tBack, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
     }
 
-    // ---- stage c-FF: LGN synapses (transposed ELL), 8 loads in flight
-    const uint32_t mLGN = ff.nLGN[n];
-    for (uint32_t i0 = 0; i0 < mLGN; i0 += 8) {
-        float fv[8], sv[8];
+    // ---- stage c-FF: LGN synapses in list order
+    auto ff_input = [&](float f, float sI) {
+        const uint32_t nsp = (uint32_t)floorf(sI);
+        if (nsp > 0u) {
+            const float tsp = (sI - (float)nsp) * dt;
+            if (f > 0.0f) {
+                float ddt = 0.0f;
+                if (backing) ddt = tBack - tsp;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            uint32_t i = i0 + j;
-            fv[j] = 0.0f;
-            sv[j] = 0.0f;
-            if (i < mLGN) {
-                uint32_t lid = i * nV1 + n;
-                fv[j] = ff.sLGN[lid];
-                sv[j] = lgnSInfo[ff.idLGN[lid]];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if (i0 + j < mLGN) {
-                float sI = sv[j];
-                uint32_t nsp = (uint32_t)floorf(sI);
-                if (nsp > 0u) {
-                    float tsp = (sI - (float)nsp) * dt;
-                    float f = fv[j];
-                    if (f > 0.0f) {
-                        float ddt = 0.0f;
-                        if (backing) ddt = tBack - tsp;
-#pragma unroll
-                        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-                            if (ig < p.ngFF) {
-                                float str = f * p.pFF[t * p.ngFF + ig];
-                                if (backing && ddt > 0.0f)
-                                    single_input(g0[ig], h0[ig], str * (float)nsp, ddt,
-                                                 p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
-                                single_input(g1[ig], h1[ig], str * (float)nsp, dt - tsp,
-                                             p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
-                            }
-                        }
+                for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+                    if (ig < p.ngFF) {
+                        const float str = f * p.pFF[t * p.ngFF + ig];
+                        if (backing && ddt > 0.0f)
+                            single_input(g0[ig], h0[ig], str * (float)nsp, ddt,
+                                         p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
+                        single_input(g1[ig], h1[ig], str * (float)nsp, dt - tsp,
+                                     p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
                     }
                 }
             }
         }
+    };
+#pragma unroll
+    for (int j = 0; j < kFF; ++j)
+        if ((uint32_t)j < mLGN) ff_input(fv[j], sv[j]);
+    for (uint32_t i0 = kFF; i0 < mLGN; i0 += 8) {       // lists longer than kFF
+        float fw[8], sw[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            fw[j] = 0.0f;
+            sw[j] = 0.0f;
+            if (i0 + j < mLGN) {
+                const uint32_t lid = (i0 + j) * nV1 + n;
+                fw[j] = ff.sLGN[lid];
+                sw[j] = lgnSInfo[ff.idLGN[lid]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (i0 + j < mLGN) ff_input(fw[j], sw[j]);
     }
     // background Poisson drive (extension)
     if (p.bgStrength[t] > 0.0f) {
-        vc_u32x4 r = vc_draw4(p.seed_lo, p.seed_hi, n, step, VC_STREAM_V1_BG, 0u, p.trial);
-        float u = vc_u01_open0(r.v[0]);
-        float frac = vc_u01_open1(r.v[1]);
-        uint32_t nsp = vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
+        const vc_u32x4 r = vc_draw4(p.seed_lo, p.seed_hi, n, step, VC_STREAM_V1_BG, 0u, p.trial);
+        const float u = vc_u01_open0(r.v[0]);
+        const float frac = vc_u01_open1(r.v[1]);
+        const uint32_t nsp = vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
         if (nsp > 0u) {
-            float tsp = frac * dt;
-            float f = p.bgStrength[t];
+            const float tsp = frac * dt;
+            const float f = p.bgStrength[t];
             float ddt = 0.0f;
             if (backing) ddt = tBack - tsp;
 #pragma unroll
             for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
                 if (ig < p.ngFF) {
-                    float str = f * p.pBG[t * p.ngFF + ig];
+                    const float str = f * p.pBG[t * p.ngFF + ig];
                     if (backing && ddt > 0.0f)
                         single_input(g0[ig], h0[ig], str * (float)nsp, ddt,
                                      p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
@@ -420,7 +464,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         const float b1 = gE_t1 * VC_VE + gI_t1 * VC_VI + gL * VC_VL + depC + 0.0f;
         const float new_dt = dt - tBack;
         if (backing) {
-            float A = a * (v0 - VC_VL);
+            const float A = a * (v0 - VC_VL);
             w0 = (w0 - A) * vc_expf(-tBack / tau_w) + A;
         }
         const float dTgL = deltaT * gL;
@@ -440,7 +484,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         w = w0 + new_dt * (gk1 + gk2) / 2.0f;
         v = v0 + new_dt * (fk1 + fk2) / 2.0f;
         if (v > vThres) {
-            float ratio = (vThres - v0) / (v - v0);
+            const float ratio = (vThres - v0) / (v - v0);
             tsp = tBack + ratio * new_dt;
             w0 += ratio * (w - w0) + bb;
             spiked = true;
@@ -452,7 +496,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     }
     if (tBack >= dt) {
         v = vR;
-        float A = a * (v - VC_VL);
+        const float A = a * (v - VC_VL);
         w = (w0 - A) * vc_expf(-(dt - tsp) / tau_w) + A;
         tBack -= dt;
     }
@@ -466,7 +510,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     const bool exist = sInfo >= 1.0f;
     const uint64_t m = __ballot(exist);
     const uint32_t tile = n >> 6;
-    if (vc_lane() == 0) next.mask[tile] = m;
+    if (lane == 0) next.mask[tile] = m;
     if (exist) {
         const float nsp = floorf(sInfo);
         const float tsp2 = (sInfo - nsp) * dt;
@@ -528,10 +572,14 @@ int check_params(const vc_v1_params *p, const vc_rec_conn *rec) {
         }
         if (p->deltaT[t] < 0.0f) { vc_set_error("deltaT < 0"); return VC_E_INVALID; }
     }
+    if ((uint64_t)p->nblock * (rec ? rec->maxNear : 1) * p->blockSize >= (1ull << 32)) {
+        vc_set_error("nblock * maxNear * blockSize must be < 2^32");
+        return VC_E_UNSUPPORTED;
+    }
     if (rec) {
-        if (rec->maxNear * (p->blockSize / 64) > (uint32_t)kMaxNearTiles || rec->maxNear > 64) {
-            vc_set_error("maxNear %u x tiles %u exceeds the LDS staging limit %d",
-                         rec->maxNear, p->blockSize / 64, kMaxNearTiles);
+        if (rec->maxNear * (p->blockSize / 64) > (uint32_t)(kMaskRegs * 64)) {
+            vc_set_error("maxNear %u x tiles %u exceeds the %d neighbour tiles held per lane",
+                         rec->maxNear, p->blockSize / 64, kMaskRegs * 64);
             return VC_E_UNSUPPORTED;
         }
     }
@@ -580,6 +628,33 @@ extern "C" int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state
     hipLaunchKernelGGL(k_v1_step<true>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
                        *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, step, out_sInfo, out_dep,
                        out_g_prev);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                         const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
+                         uint32_t cur, const float *lgnSInfo, uint64_t lgnStride, uint32_t it0,
+                         uint32_t nsteps, float *out_sInfo, uint64_t outStride) {
+    int rc = check_params(p, rec);
+    if (rc) return rc;
+    if (!s || !ff || !rec || !spk2 || !lgnSInfo || !out_sInfo || cur > 1) {
+        vc_set_error("vc_v1_run: bad argument");
+        return VC_E_INVALID;
+    }
+    if (spk2[0].mask == spk2[1].mask) {
+        vc_set_error("vc_v1_run: the two spike buffers must differ");
+        return VC_E_INVALID;
+    }
+    V1Const k = make_const(p);
+    const uint32_t wg = wg_size(p->blockSize);
+    for (uint32_t i = 0; i < nsteps; ++i) {
+        const uint32_t nx = 1u - cur;
+        hipLaunchKernelGGL(k_v1_step<true>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
+                           *p, k, *s, *ff, *rec, spk2[cur], spk2[nx], lgnSInfo + i * lgnStride,
+                           it0 + i, out_sInfo + i * outStride, nullptr, nullptr);
+        cur = nx;
+    }
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -74,14 +74,14 @@ class ModelSpec:
     # synaptic scales
     sRatioLGN: tuple = (1.0, 1.0)
     sLGN: float = 1.2             # base LGN synapse strength
-    sE: float = 0.25              # E recurrent scale (times lognormal)
-    sI: float = 1.2               # I recurrent scale
+    sE: float = 0.12              # E recurrent scale (times lognormal)
+    sI: float = 2.0               # I recurrent scale
     w_mu: float = -1.0
     w_sigma: float = 0.8
     sigma_um: float = 150.0
     # background Poisson drive (extension)
     bgRate: float = 2000.0        # Hz
-    bgStrength: tuple = (0.22, 0.22)
+    bgStrength: tuple = (0.08, 0.12)
     pBG: tuple = ((1.0, 0.0), (1.0, 0.0))
     # LGN
     nSpatialSample1D: int = 16
@@ -96,9 +96,9 @@ class ModelSpec:
     nR: float = 6.0
     nD: float = 8.0
     ratio: float = 0.8
-    spont: float = 0.08
-    c50: float = 0.3
-    sharpness: float = 8.0
+    spont: float = 0.1
+    c50: float = 0.15
+    sharpness: float = 10.0
     frRatioLGN: float = 60.0
     convolRatio: float = 1.0
     ff_sigma_cells: float = 1.5   # LGN->V1 pooling sigma in LGN grid spacings
@@ -134,7 +134,7 @@ def spec_c2(**kw) -> ModelSpec:
     """Plain forward, full patch: 64 blocks, 16384 LGN, m = 30, 800 E + 200 I
     partners, 256x256 px, 2 s."""
     base = ModelSpec(nblock=64, KE=800, KI=200, m=30, nLGN_side=128, sheet_mm=8.0,
-                     width=256, height=256, nt=16000, sE=0.125, sI=0.6)
+                     width=256, height=256, nt=16000, sE=0.06, sI=1.0)
     return replace(base, **kw)
 
 
@@ -143,7 +143,7 @@ def spec_tiny(**kw) -> ModelSpec:
     steps: 2 blocks of 256 (192 E), 64 LGN cells, 32x32 px."""
     base = ModelSpec(nblock=2, blockSize=256, nE=192, KE=100, KI=25, m=8, nLGN_side=8,
                      sheet_mm=1.0, field_deg=2.0, width=32, height=32, nt=400,
-                     sE=0.8, sI=3.0)
+                     sE=0.8, sI=3.0, bgStrength=(0.15, 0.15))
     return replace(base, **kw)
 
 
@@ -295,6 +295,8 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
             strength = (scale * rng.lognormal(spec.w_mu, spec.w_sigma, size=pre_g.shape)).astype(np.float32)
             post_g = np.broadcast_to(posts[:, None], pre_g.shape)
             chosen.append((pre_g.ravel(), post_g.ravel(), strength.ravel()))
+        if not chosen:
+            chosen.append((np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, np.float32)))
         pre_all = np.concatenate([c[0] for c in chosen])
         post_all = np.concatenate([c[1] for c in chosen])
         w_all = np.concatenate([c[2] for c in chosen])
@@ -357,8 +359,11 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
     dist, nn = tree.query(np.stack([vx, vy], 1), k=kq)
     sig = spec.ff_sigma_cells * spacing
     wq = np.exp(-dist ** 2 / (2 * sig * sig))
-    sel = _weighted_topk(wq, spec.m, rng)
-    ids = np.take_along_axis(nn, sel, 1)                        # [nV1, m]
+    if spec.m > 0:
+        sel = _weighted_topk(wq, spec.m, rng)
+        ids = np.take_along_axis(nn, sel, 1)                    # [nV1, m]
+    else:
+        ids = np.zeros((spec.nV1, 1), dtype=np.int64)
     ids.sort(axis=1)
     sR = np.where(isE, spec.sRatioLGN[0], spec.sRatioLGN[1])
     s_ff = (spec.sLGN * sR)[:, None] * (0.75 + 0.5 * rng.random(ids.shape))

@@ -64,6 +64,7 @@ class GpuSim:
         self.og = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
         self.oh = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
         self.spk = [abi.Spikes(ptr(self.masks[i]), ptr(self.og[i]), ptr(self.oh[i])) for i in range(2)]
+        self.spk2 = (abi.Spikes * 2)(*self.spk)
         # ---- LGN
         self._setup_lgn()
         self.it = 0
@@ -134,34 +135,56 @@ class GpuSim:
               "vc_lgn_frame_response")
 
     # ------------------------------------------------------------------ steps
-    def run(self, nsteps: int, keep_fr: bool = False):
+    def _lgn_period(self, it: int):
+        """Frame entering at step `it` (a multiple of ntPerFrame) and the LGN
+        steps of the following frame period."""
         tm = self.tm
         npf = tm["ntPerFrame"]
-        nF = self.frames.shape[0]
+        fidx = it // npf
+        self._frame_response(self.frames[fidx % self.frames.shape[0]], fidx % tm["maxFrame"])
+        check(self.lib.vc_lgn_steps(None, C.byref(self.lp), C.byref(self.ls), ptr(self.Rring),
+                                    it, npf, ptr(self.lgn_sInfo), ptr(self.lgn_fr)),
+              "vc_lgn_steps")
+
+    def run(self, nsteps: int, out: torch.Tensor | None = None, events: list | None = None):
+        """Advance `nsteps` steps.  V1 sInfo of step i goes to out[i] if `out`
+        ([nsteps, nV1] float32 on the device) is given, else to the history
+        when recording, else to a scratch row.  If `events` is a list, a
+        (start, end, n) triple of timing events bracketing each run of n V1
+        step launches is appended (on the launch stream)."""
+        sp = self.net.spec
+        npf = self.tm["ntPerFrame"]
+        if out is None and self.record:
+            out = torch.empty(nsteps, sp.nV1, dtype=torch.float32, device=self.dev)
+            self.sInfo_hist.append(out)
         done = 0
+        lgn_base = ptr(self.lgn_sInfo)
         while done < nsteps:
             it = self.it
             k_in = it % npf
             if k_in == 0:
-                fidx = it // npf
-                self._frame_response(self.frames[fidx % nF], fidx % tm["maxFrame"])
-                check(self.lib.vc_lgn_steps(None, C.byref(self.lp), C.byref(self.ls), ptr(self.Rring),
-                                            it, npf, ptr(self.lgn_sInfo), ptr(self.lgn_fr)),
-                      "vc_lgn_steps")
+                self._lgn_period(it)
                 if self.record:
                     self.lgn_hist.append(self.lgn_sInfo.clone())
-            out = torch.empty(self.net.spec.nV1, dtype=torch.float32, device=self.dev) \
-                if self.record else self.sInfo_step
-            nxt = 1 - self.cur
-            check(self.lib.vc_v1_step(None, C.byref(self.p), C.byref(self.s), C.byref(self.ff),
-                                      C.byref(self.rec), C.byref(self.spk[self.cur]),
-                                      C.byref(self.spk[nxt]), ptr(self.lgn_sInfo[k_in]), it,
-                                      ptr(out), None, None), "vc_v1_step")
-            if self.record:
-                self.sInfo_hist.append(out)
-            self.cur = nxt
-            self.it += 1
-            done += 1
+            n = min(npf - k_in, nsteps - done)
+            if out is not None:
+                optr, ostride = ptr(out) + done * sp.nV1 * 4, sp.nV1
+            else:
+                optr, ostride = ptr(self.sInfo_step), 0
+            if events is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            check(self.lib.vc_v1_run(None, C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                                     C.byref(self.rec), self.spk2, self.cur,
+                                     lgn_base + k_in * sp.nLGN * 4, sp.nLGN, it, n, optr, ostride),
+                  "vc_v1_run")
+            if events is not None:
+                e1 = torch.cuda.Event(enable_timing=True)
+                e1.record()
+                events.append((e0, e1, n))
+            self.cur ^= (n & 1)
+            self.it += n
+            done += n
 
     def finalize(self):
         """Apply the last step's pending recurrent increments (the reference
@@ -174,7 +197,7 @@ class GpuSim:
 
     def sInfo(self) -> np.ndarray:
         torch.cuda.synchronize()
-        return torch.stack(self.sInfo_hist).cpu().numpy()
+        return torch.cat(self.sInfo_hist).cpu().numpy()
 
     def lgn_sinfo(self) -> np.ndarray:
         torch.cuda.synchronize()
