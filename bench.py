@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=800,
                     help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 256 MiB device copy (x4) for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the "
@@ -112,6 +114,15 @@ def main():
     log(f"[bench] rates: E {per_n[isE].mean() / T:.2f} Hz, I {per_n[~isE].mean() / T:.2f} Hz; "
         f"rho={rho:.3e}; V1 launch {v1_launch_s * 1e6:.2f} us; {bpn:.1f} B/neuron-step")
     del out
+
+    if args.copy_calib:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 4 x {src.numel() * 4} bytes")
+        del src, dst
 
     cpu = None
     if args.cpu_steps > 0:

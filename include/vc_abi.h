@@ -94,6 +94,20 @@ typedef struct {
     uint32_t maxLGN;
 } vc_ff_conn;
 
+/* Event-driven LGN->V1 delivery (for maxLGN <= 32): the reverse of
+ * vc_ff_conn, revTgt[revOff[c] .. revOff[c+1]) = (n << 5) | i for every
+ * synapse i of neuron n fed by LGN cell c.  When cell c spikes in step s of
+ * a frame period, vc_lgn_steps sets bit i of mask[s*nV1 + n]; vc_v1_step then
+ * visits only the set bits of its neuron's word, in synapse order (the
+ * reference visits every synapse and skips those with no spike,
+ * src/coredynamics.cu:1466-1473), and clears the word for the next period. */
+typedef struct {
+    const uint32_t *revOff;    /* [nLGN + 1] */
+    const uint32_t *revTgt;    /* [revOff[nLGN]] */
+    uint32_t *mask;            /* [ntPerFrame][nV1], zero between uses */
+    uint32_t nV1;
+} vc_ff_events;
+
 /* V1->V1 near-neighbour synapses in "masked CSR": the reference's dense
  * V1_conMat[nblock][maxNear][blockSize pre][blockSize post]
  * (src/patch.cu:3441-3461) with only its strictly positive entries kept.
@@ -124,6 +138,9 @@ typedef struct {
 /* One simulation step on all V1 neurons: apply the recurrent increments of
  * `prev` (the previous step's spikes), decay, LGN + background input, RK2
  * update, spike time, refractory handling, compaction into `next`.
+ * ffMask (may be NULL) is this step's row of vc_ff_events.mask: when given,
+ * only the synapses whose bit is set are visited and the row is cleared;
+ * when NULL every synapse's LGN cell is read.
  * out_sInfo[nV1] receives 1 + tsp/dt for a spike and v otherwise;
  * out_dep[nV1] (may be NULL) receives depC; out_g_prev (may be NULL)
  * receives [2*(ngE+ngI)][nV1] = gE[ngE], gI[ngI], hE[ngE], hI[ngI]: the
@@ -132,18 +149,20 @@ typedef struct {
 int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                const vc_ff_conn *ff, const vc_rec_conn *rec,
                const vc_spikes *prev, const vc_spikes *next,
-               const float *lgnSInfo, uint32_t step,
+               const float *lgnSInfo, uint32_t *ffMask, uint32_t step,
                float *out_sInfo, float *out_dep, float *out_g_prev);
 
 /* `nsteps` consecutive vc_v1_step launches from the native side: step i
  * reads the spikes of spk2[cur ^ (i&1)], writes spk2[cur ^ ((i+1)&1)], takes
- * its LGN input from lgnSInfo + i*lgnStride and writes out_sInfo +
+ * its LGN input from lgnSInfo + i*lgnStride (and ffMask + i*maskStride
+ * when ffMask is not NULL) and writes out_sInfo +
  * i*outStride (strides in floats; 0 reuses one row).  After the call the
  * last step's spikes are in spk2[cur ^ (nsteps&1)].  (The reference's host
  * loop, src/patch.cu:5966-6913, launched one step at a time.) */
 int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
               const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
-              uint32_t cur, const float *lgnSInfo, uint64_t lgnStride, uint32_t it0,
+              uint32_t cur, const float *lgnSInfo, uint64_t lgnStride,
+              uint32_t *ffMask, uint64_t maskStride, uint32_t it0,
               uint32_t nsteps, float *out_sInfo, uint64_t outStride);
 
 /* Apply the recurrent increments of `prev` to gE/hE/gI/hI without stepping
@@ -203,10 +222,12 @@ typedef struct {
  * it0).  Rring is [maxFrame][nLGN][2]: the spatial responses of frame f are
  * in slot f mod maxFrame (pre-stimulus frames f < 0 included).  Writes
  * sInfo[nsteps][nLGN] (nsp + mean tsp/dt) and fr[nsteps][nLGN] (Hz; required,
- * it is also the hand-over buffer between the rate and spike kernels). */
+ * it is also the hand-over buffer between the rate and spike kernels).  If ev
+ * is not NULL, every spike of step it0+s is also delivered as bits of
+ * ev->mask[s*nV1 + n] (see vc_ff_events). */
 int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
                  const float *Rring, uint32_t it0, uint32_t nsteps,
-                 float *sInfo, float *fr);
+                 float *sInfo, float *fr, const vc_ff_events *ev);
 
 /* LGN Poisson state initialisation (logRand_init, src/coredynamics.cu:148-177):
  * leftTimeRate = 0, lastNegLogRand = -log(u) with u a Philox draw. */

@@ -47,26 +47,69 @@ def test_frame_response_bitwise(pkg, tiny):
         assert np.array_equal(Rg.view(np.uint32), Ro.view(np.uint32)), f
 
 
-@pytest.mark.parametrize("nsteps", [400])
-def test_full_path_bitwise_tiny(pkg, tiny, nsteps):
+CASES = {
+    # name: (spec factory kwargs, steps)
+    "tiny": (dict(), 400),
+    # synchronous start under strong tonic drive: hundreds of spikes per
+    # neighbourhood in one step (several 64-spike batch flushes)
+    "burst": (dict(tonicDep=(2.0, 2.0), sE=1.5), 200),
+    # E/I boundary inside a 64-neuron tile (mixed tiles), 3 blocks
+    "mixed_tile": (dict(nE=200, nblock=3, bgStrength=(0.4, 0.4)), 300),
+    # one block of 64 neurons, one tile
+    "one_tile": (dict(nblock=1, blockSize=64, nE=48, KE=20, KI=5, bgStrength=(0.4, 0.4)), 300),
+}
+
+
+@pytest.mark.parametrize("case,events", [(c, True) for c in CASES] + [("tiny", False), ("burst", False)])
+def test_full_path_bitwise(pkg, case, events):
     import oracle
+    from visual_cortex_amd import network
     from visual_cortex_amd.sim import GpuSim
-    _flush(f"test_full_path_bitwise_tiny: {nsteps} steps")
-    g = GpuSim(tiny)
+    kw, nsteps = CASES[case]
+    _flush(f"test_full_path_bitwise[{case}, events={events}]: {kw} {nsteps} steps")
+    net = network.build_network(network.spec_tiny(**kw))
+    state = None
+    if case == "burst":
+        from visual_cortex_amd import params
+        state = params.initial_state(net)
+        state["v"][:] = -52.0
+    g = GpuSim(net, state=state, ff_events=events)
     tables = dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
                   SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
-    o = oracle.OracleSim(tiny, pkg, tables=tables)
+    o = oracle.OracleSim(net, pkg, tables=tables, state=state)
     g.run(nsteps)
     o.run(nsteps)
     lg, lo = g.lgn_sinfo(), o.lgn_sinfo()
     assert np.array_equal(lg.view(np.uint32), lo.view(np.uint32)), "LGN sInfo differs"
     sg, so = g.sInfo(), o.sInfo()
     nsp_g, nsp_o = (sg >= 1).sum(), (so >= 1).sum()
-    _flush(f"  V1 spikes gpu={nsp_g} oracle={nsp_o}")
+    per_step = (so >= 1).sum(1)
+    _flush(f"  V1 spikes gpu={nsp_g} oracle={nsp_o}, max per step {per_step.max()}")
     assert nsp_o > 0, "no V1 spikes: the test would not exercise the recurrent path"
+    if case == "burst":
+        assert per_step.max() > 64
     diff = np.argwhere(sg.view(np.uint32) != so.view(np.uint32))
     assert diff.size == 0, f"first sInfo difference at (step, neuron) {diff[0]}"
     g.finalize()
     sgs = g.state_numpy()
     for k in ("v", "w", "tBack", "gFF", "hFF", "gE", "hE", "gI", "hI"):
         assert np.array_equal(sgs[k].view(np.uint32), o.st[k].view(np.uint32)), k
+
+
+def test_c1_first_200_steps_bitwise(pkg):
+    """The small plain-forward workload (4 blocks x 1024, 1024 LGN cells,
+    128x128 px) for its first 200 steps."""
+    import oracle
+    from visual_cortex_amd import network
+    from visual_cortex_amd.sim import GpuSim
+    _flush("test_c1_first_200_steps_bitwise")
+    net = network.build_network(network.spec_c1())
+    g = GpuSim(net)
+    tables = dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
+                  SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
+    o = oracle.OracleSim(net, pkg, tables=tables)
+    g.run(200)
+    o.run(200)
+    sg, so = g.sInfo(), o.sInfo()
+    assert (so >= 1).sum() > 0
+    assert np.array_equal(sg.view(np.uint32), so.view(np.uint32))

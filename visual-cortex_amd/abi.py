@@ -61,6 +61,10 @@ class FFConn(C.Structure):
     _fields_ = [("nLGN", vp), ("sLGN", vp), ("idLGN", vp), ("maxLGN", u32)]
 
 
+class FFEvents(C.Structure):
+    _fields_ = [("revOff", vp), ("revTgt", vp), ("mask", vp), ("nV1", u32)]
+
+
 class RecConn(C.Structure):
     _fields_ = [("maxNear", u32), ("nNear", vp), ("nbr", vp), ("rowMask", vp),
                 ("rowOff", vp), ("w", vp)]
@@ -99,17 +103,18 @@ class LgnState(C.Structure):
 # exported symbols (name -> argtypes); tests check that the .so exports each
 SIGNATURES = {
     "vc_v1_step": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
-                   C.POINTER(RecConn), C.POINTER(Spikes), C.POINTER(Spikes), vp, u32,
+                   C.POINTER(RecConn), C.POINTER(Spikes), C.POINTER(Spikes), vp, vp, u32,
                    vp, vp, vp],
     "vc_v1_run": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
-                  C.POINTER(RecConn), C.POINTER(Spikes), u32, vp, C.c_uint64, u32, u32, vp,
-                  C.c_uint64],
+                  C.POINTER(RecConn), C.POINTER(Spikes), u32, vp, C.c_uint64, vp, C.c_uint64,
+                  u32, u32, vp, C.c_uint64],
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],
     "vc_lgn_precompute": [vp, C.POINTER(LgnSetup), vp, vp, vp, vp, vp],
     "vc_lgn_frame_response": [vp, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp],
-    "vc_lgn_steps": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp, vp],
+    "vc_lgn_steps": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp, vp,
+                     C.POINTER(FFEvents)],
     "vc_lgn_init": [vp, C.POINTER(LgnParams), vp, vp],
     "vc_last_error": [],
     "vc_version": [],

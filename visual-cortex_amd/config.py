@@ -1,0 +1,181 @@
+"""The reference simulator's configuration interface: `patch_fast -c <cfg>
+[-s seed] [-p print]` with a Boost program_options config file
+(src/patch.cu:172-405).
+
+Semantics kept from Boost program_options as the reference uses it:
+  * `key = value` lines, `#` starts a comment, blank lines ignored,
+    `[section]` prefixes the following keys with `section.`;
+  * vector options take comma-separated values and a repeated key appends
+    (e.g. two `pFF = a, b` lines give [a, b, a, b]; src/patchV1.cfg:127-128);
+  * the command line is stored first, so it wins over the file;
+  * file options are prefixed with inputFolder / resourceFolder /
+    outputFolder unless they start with `!` (src/patch.cu:407-586), and the
+    suffix options become `-<suffix>.bin` (src/patch.cu:587-605).
+"""
+from __future__ import annotations
+
+from dataclasses import replace
+from pathlib import Path
+
+# file option -> (default name, folder option, suffix option)
+FILES = {
+    "fStimulus": ("stimulus.bin", "resourceFolder", None),
+    "fV1_allpos": ("V1_allpos", "resourceFolder", "res_suffix"),
+    "fLGN_vpos": ("LGN_vpos", "resourceFolder", "res_suffix"),
+    "fConnectome_cfg": ("connectome_cfg", "inputFolder", "conV1_suffix"),
+    "fV1_conMat": ("V1_conMat", "inputFolder", "conV1_suffix"),
+    "fNeighborBlock": ("neighborBlock", "inputFolder", "conV1_suffix"),
+    "fLGN_V1_ID": ("LGN_V1_idList", "inputFolder", "conLGN_suffix"),
+    "fLGN_V1_s": ("LGN_V1_sList", "inputFolder", "conLGN_suffix"),
+    "fLGN": ("LGN", "outputFolder", "output_suffix"),
+    "fRawData": ("rawData", "outputFolder", "output_suffix"),
+    "fLGN_fr": ("LGN_fr", "outputFolder", "output_suffix"),
+    "fLGN_sp": ("LGN_sp", "outputFolder", "output_suffix"),
+}
+
+# options that hold vectors (everything else is a scalar; the last value wins)
+VECTOR_KEYS = {
+    "sRatioLGN", "sRatioV1", "synPerConFF", "synPerCon", "gapRatio", "riseTimeFF", "riseTimeE",
+    "riseTimeI", "decayTimeFF", "decayTimeE", "decayTimeI", "v0", "w0", "gFF0", "gE0", "gI0",
+    "pFF", "pE", "pI", "vR", "vThres", "gL", "C", "tRef", "vT", "deltaT", "tau_w", "a", "b",
+    "nTypeHierarchy", "spE0", "spI0", "noisyDep", "tonicDep", "minTonicRatio", "synFailFF",
+    "synFail", "boostOri", "iOri", "bgStrength", "pBG",
+}
+
+
+def parse_cfg_text(text: str, into: dict | None = None) -> dict:
+    """Parse cfg text into {key: [str, ...]} (repeated keys append)."""
+    out = {} if into is None else into
+    section = ""
+    for raw in text.splitlines():
+        line = raw.split("#", 1)[0].strip()
+        if not line:
+            continue
+        if line.startswith("[") and line.endswith("]"):
+            section = line[1:-1].strip() + "."
+            continue
+        if "=" not in line:
+            raise ValueError(f"cfg line without '=': {raw!r}")
+        k, v = line.split("=", 1)
+        k = section + k.strip()
+        vals = [x.strip() for x in v.split(",")] if k in VECTOR_KEYS else [v.strip()]
+        out.setdefault(k, []).extend(x for x in vals if x != "")
+    return out
+
+
+class Config:
+    """Command-line options first (they win), then the cfg file."""
+
+    def __init__(self, cfg_text: str = "", cmdline: dict | None = None):
+        self.file = parse_cfg_text(cfg_text)
+        self.cmd = {k: [str(v)] for k, v in (cmdline or {}).items()}
+
+    @classmethod
+    def from_file(cls, path, cmdline: dict | None = None):
+        return cls(Path(path).read_text(), cmdline)
+
+    def has(self, key):
+        return key in self.cmd or key in self.file
+
+    def raw(self, key):
+        if key in self.cmd:
+            return self.cmd[key]
+        return self.file.get(key)
+
+    def str(self, key, default=None):
+        r = self.raw(key)
+        return default if r is None else r[-1]
+
+    def float(self, key, default=None):
+        r = self.raw(key)
+        return default if r is None else float(r[-1])
+
+    def int(self, key, default=None):
+        r = self.raw(key)
+        return default if r is None else int(float(r[-1]))
+
+    def bool(self, key, default=None):
+        r = self.raw(key)
+        if r is None:
+            return default
+        s = r[-1].strip().lower()
+        if s in ("1", "true", "yes", "on"):
+            return True
+        if s in ("0", "false", "no", "off"):
+            return False
+        raise ValueError(f"{key}: not a bool: {r[-1]!r}")
+
+    def floats(self, key, default=None):
+        r = self.raw(key)
+        return default if r is None else [float(x) for x in r]
+
+    def path(self, key) -> str:
+        """Resolved path of a file option (folder prefix, '!' rule, suffix)."""
+        name, folder_key, suffix_key = FILES[key]
+        name = self.str(key, name)
+        if name.startswith("!"):
+            base = name[1:]
+        else:
+            base = self.str(folder_key, "") + name
+        if suffix_key is None:
+            return base
+        suf = self.str(suffix_key, "")
+        return base + ("-" + suf if suf else "") + ".bin"
+
+
+def _pairs(vals, n_types, per):
+    """A flat [type][per] list -> tuple of per-type tuples."""
+    if vals is None:
+        return None
+    if len(vals) < n_types * per:
+        raise ValueError(f"expected {n_types}x{per} values, got {len(vals)}")
+    return tuple(tuple(vals[t * per:(t + 1) * per]) for t in range(n_types))
+
+
+def spec_from_config(cfg: Config, base=None):
+    """ModelSpec from the cfg keys of src/patch.cu:180-374 (keys the model
+    uses); everything not given keeps the base spec's value."""
+    from . import network
+    sp = base if base is not None else network.ModelSpec()
+    kw = {}
+    for key in ("dt", "nsig", "tau", "convolRatio", "frRatioLGN"):
+        if cfg.has(key):
+            kw[key] = cfg.float(key)
+    for key in ("nt", "nKernelSample", "nSpatialSample1D", "frameRate", "iModel"):
+        if cfg.has(key):
+            kw[key] = cfg.int(key)
+    if cfg.has("seed"):
+        kw["seed"] = cfg.int("seed")
+    for key in ("vR", "vThres", "gL", "C", "tRef", "vT", "deltaT", "tau_w", "a", "b",
+                "tonicDep", "sRatioLGN", "bgStrength"):
+        v = cfg.floats(key)
+        if v is not None:
+            kw[key] = tuple(v[:2]) if len(v) >= 2 else (v[0], v[0])
+    for key in ("riseTimeFF", "decayTimeFF", "riseTimeE", "decayTimeE", "riseTimeI", "decayTimeI"):
+        v = cfg.floats(key)
+        if v is not None:
+            kw[key] = tuple(v)
+    ngFF = len(kw.get("riseTimeFF", sp.riseTimeFF))
+    ngE = len(kw.get("riseTimeE", sp.riseTimeE))
+    ngI = len(kw.get("riseTimeI", sp.riseTimeI))
+    for key, ng in (("pFF", ngFF), ("pE", ngE), ("pI", ngI), ("pBG", ngFF)):
+        v = cfg.floats(key)
+        if v is not None:
+            kw[key] = _pairs(v, 2, ng)
+    if cfg.has("bgRate"):
+        kw["bgRate"] = cfg.float("bgRate")
+    for key in ("noisyDep", "synFailFF", "synFail", "spE0", "spI0"):
+        v = cfg.floats(key)
+        if v is not None and any(x != 0 for x in v):
+            raise NotImplementedError(f"{key} != 0 is not built (the configs of this path use 0)")
+    if cfg.has("noDelay") and cfg.int("noDelay") == 0:
+        raise NotImplementedError("noDelay = 0 (axonal delays, recal_G_mat) is not built")
+    if cfg.bool("InhGap", False):
+        raise NotImplementedError("InhGap (gap junctions) is not built")
+    if cfg.int("learning", 0) != 0:
+        raise NotImplementedError("learning != 0 (plasticity) is not built")
+    if cfg.has("flat_retina") and not cfg.bool("flat_retina"):
+        raise NotImplementedError("only flat_retina = true is built")
+    if cfg.bool("virtual_LGN", False):
+        raise NotImplementedError("virtual_LGN is not built")
+    return replace(sp, **kw)

@@ -162,14 +162,19 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
     // responses of the frames under the first step's window, wave-uniform
     const int64_t fw = floor_div((int64_t)(it0 + (wave - c * wpc) * 64u) + (int64_t)p.iKernelSampleT0 -
                                      (int64_t)p.nRetrace + 1, (int64_t)p.ntPerFrame);
-    float Rf[2][kMaxFrame + 1];
-#pragma unroll
-    for (int d = 0; d <= kMaxFrame; ++d) {
+    // frame responses of this cell under the wavefront's windows, in LDS so
+    // that each lane can pick its tap's frame with one read
+    __shared__ float sR[256 / 64][2][kMaxFrame + 1];
+    const uint32_t wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
+    if (ln < 2u * (kMaxFrame + 1)) {
+        const uint32_t r = ln / (kMaxFrame + 1), d = ln % (kMaxFrame + 1);
         int64_t slot = (fw + d) % (int64_t)p.maxFrame;
         if (slot < 0) slot += p.maxFrame;
-        Rf[0][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + 0];
-        Rf[1][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + 1];
+        sR[wv][r][d] = Rring[((size_t)slot * p.nLGN + c) * 2 + r];
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
     float conv[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
@@ -180,11 +185,7 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
             if (i < (int)nKS) {
                 const int64_t tau = base + (int64_t)i * p.kernelSampleInterval;
                 const int d = (int)(floor_div(tau, (int64_t)p.ntPerFrame) - fw);
-                float Rv = Rf[r][0];
-#pragma unroll
-                for (int dd = 1; dd <= kMaxFrame; ++dd)
-                    if (d == dd) Rv = Rf[r][dd];
-                v = Rv * st.TW[((size_t)c * 2 + r) * nKS + i];
+                v = sR[wv][r][d] * st.TW[((size_t)c * 2 + r) * nKS + i];
             }
             if (i < 32) a0[i] = v; else a1[i - 32] = v;
         }
@@ -210,49 +211,74 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
 __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state st,
                                                    uint32_t it0, uint32_t nsteps,
                                                    const float *__restrict__ frIn,
-                                                   float *__restrict__ sInfoOut) {
+                                                   float *__restrict__ sInfoOut,
+                                                   vc_ff_events ev) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= p.nLGN) return;
-    float lTR = st.leftTimeRate[c], lNL = st.lastNegLogRand[c];
-    float frn = frIn[c];
+    const bool valid = c < p.nLGN;            // every lane runs the step loop (ballots)
+    const uint32_t lane = vc_lane();
+    const uint32_t cbase = c - lane;           // cell of lane 0 of this wavefront
+    float lTR = valid ? st.leftTimeRate[c] : 0.0f;
+    float lNL = valid ? st.lastNegLogRand[c] : 1.0f;
+    float frn = valid ? frIn[c] : 0.0f;
     for (uint32_t s = 0; s < nsteps; ++s) {
         const uint32_t it = it0 + s;
         const float fr = frn;
-        if (s + 1 < nsteps) frn = frIn[(size_t)(s + 1) * p.nLGN + c];
-        const float rate = fr / 1000.0f;
-        const float rT = p.dt * rate;
-        float next_rT = lNL - lTR;
-        uint32_t nsp = 0;
-        float tsp = 0.0f;
-        float lNLn = lNL;
-        if (next_rT > rT) {
-            lTR += rT;
-        } else {
-            float old_rT = 0.0f;
-            uint32_t kdraw = 0;
-            vc_u32x4 pack;
-            const float tr = p.tRef * rate;
-            do {
-                if ((kdraw & 3u) == 0u)
-                    pack = vc_draw4(p.seed_lo, p.seed_hi, c, it, VC_STREAM_LGN_SPIKE,
-                                    kdraw >> 2, p.trial);
-                const float u = vc_u01_open0(pack.v[kdraw & 3u]);
-                kdraw++;
-                const float x = -vc_logf(u);
-                lNLn = (x > tr) ? x : tr;
-                old_rT = next_rT;
-                next_rT += lNLn;
-                nsp++;
-            } while (next_rT <= rT);
-            lTR = rT - old_rT;
-            tsp = old_rT / (rate * (float)nsp);
+        if (valid && s + 1 < nsteps) frn = frIn[(size_t)(s + 1) * p.nLGN + c];
+        float sInfo = 0.0f;
+        if (valid) {
+            const float rate = fr / 1000.0f;
+            const float rT = p.dt * rate;
+            float next_rT = lNL - lTR;
+            uint32_t nsp = 0;
+            float tsp = 0.0f;
+            float lNLn = lNL;
+            if (next_rT > rT) {
+                lTR += rT;
+            } else {
+                float old_rT = 0.0f;
+                uint32_t kdraw = 0;
+                vc_u32x4 pack;
+                const float tr = p.tRef * rate;
+                do {
+                    if ((kdraw & 3u) == 0u)
+                        pack = vc_draw4(p.seed_lo, p.seed_hi, c, it, VC_STREAM_LGN_SPIKE,
+                                        kdraw >> 2, p.trial);
+                    const float u = vc_u01_open0(pack.v[kdraw & 3u]);
+                    kdraw++;
+                    const float x = -vc_logf(u);
+                    lNLn = (x > tr) ? x : tr;
+                    old_rT = next_rT;
+                    next_rT += lNLn;
+                    nsp++;
+                } while (next_rT <= rT);
+                lTR = rT - old_rT;
+                tsp = old_rT / (rate * (float)nsp);
+            }
+            sInfo = (float)nsp + tsp / p.dt;
+            if (sInfo > 0.0f) lNL = lNLn;
+            sInfoOut[(size_t)s * p.nLGN + c] = sInfo;
         }
-        const float sInfo = (float)nsp + tsp / p.dt;
-        if (sInfo > 0.0f) lNL = lNLn;
-        sInfoOut[(size_t)s * p.nLGN + c] = sInfo;
+        // deliver this step's spikes to the synapse bits of the V1 neurons they
+        // feed: the wavefront walks its spiking cells, lanes over targets
+        if (ev.mask != nullptr) {
+            uint64_t sm = __ballot(valid && sInfo >= 1.0f);
+            uint32_t *row = ev.mask + (size_t)s * ev.nV1;
+            while (sm) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(sm);
+                sm &= sm - 1ull;
+                const uint32_t cl = cbase + l;
+                const uint32_t k0 = ev.revOff[cl], k1 = ev.revOff[cl + 1];
+                for (uint32_t k = k0 + lane; k < k1; k += 64u) {
+                    const uint32_t tg = ev.revTgt[k];
+                    atomicOr(row + (tg >> 5), 1u << (tg & 31u));
+                }
+            }
+        }
     }
-    st.leftTimeRate[c] = lTR;
-    st.lastNegLogRand[c] = lNL;
+    if (valid) {
+        st.leftTimeRate[c] = lTR;
+        st.lastNegLogRand[c] = lNL;
+    }
 }
 
 __global__ void k_lgn_init(vc_lgn_params p, float *lTR, float *lNL) {
@@ -486,7 +512,7 @@ extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t 
 
 extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
                             const float *Rring, uint32_t it0, uint32_t nsteps, float *sInfo,
-                            float *fr) {
+                            float *fr, const vc_ff_events *ev) {
     if (!p || !s || !Rring || !sInfo) {
         vc_set_error("vc_lgn_steps: null argument");
         return VC_E_INVALID;
@@ -497,6 +523,12 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
     }
     if (p->maxFrame == 0 || p->maxFrame > (uint32_t)kMaxFrame || p->ntPerFrame == 0) {
         vc_set_error("maxFrame %u unsupported (1..%d)", p->maxFrame, kMaxFrame);
+        return VC_E_UNSUPPORTED;
+    }
+    // one wavefront covers 64 steps: its taps span at most kMaxFrame + 1 frames
+    if ((63u + p->nRetrace) / p->ntPerFrame + 1u > (uint32_t)kMaxFrame) {
+        vc_set_error("a %u-step window over %u-step frames spans too many frames", p->nRetrace,
+                     p->ntPerFrame);
         return VC_E_UNSUPPORTED;
     }
     // the window spans nRetrace steps; every tap's frame must be within maxFrame of the first
@@ -517,8 +549,16 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
     hipLaunchKernelGGL(k_lgn_rate, dim3((uint32_t)((waves * 64 + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, *p, *s, Rring, it0, nsteps, fr);
     VC_HIP_CHECK(hipGetLastError());
+    vc_ff_events e{};
+    if (ev) {
+        if (!ev->revOff || !ev->revTgt || !ev->mask || ev->nV1 == 0) {
+            vc_set_error("vc_lgn_steps: incomplete vc_ff_events");
+            return VC_E_INVALID;
+        }
+        e = *ev;
+    }
     hipLaunchKernelGGL(k_lgn_spike, dim3((p->nLGN + 255) / 256), dim3(256), 0,
-                       (hipStream_t)stream, *p, *s, it0, nsteps, fr, sInfo);
+                       (hipStream_t)stream, *p, *s, it0, nsteps, fr, sInfo, e);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -24,10 +24,11 @@
 namespace {
 
 constexpr int kWG = 256;           // max workgroup size
-constexpr int kMaskRegs = 8;       // neighbour-tile masks held per lane: 8*64 entries
+constexpr int kMaxEnt = 512;       // neighbour-tile entries staged in LDS per wavefront
 constexpr int kBatch = 64;
 constexpr int kFF = 32;            // LGN synapses loaded per round
-constexpr int kW = 16;             // weight loads in flight per lane
+constexpr int kW = 8;              // weight loads in flight per lane
+constexpr int kEv = 4;             // LGN spike events prefetched per neuron
 
 // per-launch constants derived on the host from vc_v1_params
 struct V1Const {
@@ -54,6 +55,38 @@ __device__ __forceinline__ void decay_const(float &g, float &h, float etr, float
                                             float dod) {
     g = g * etd + h * dod * (etd - etr);
     h = h * etr;
+}
+
+// LGN (or background) conductance input of one synapse, src/coredynamics.cu
+// :1474-1500: nsp spikes at tsp into (g1, h1) over dt - tsp and, when the
+// neuron is backing up from refractoriness, into (g0, h0) over tBack - tsp.
+// Not inlined: it runs only for synapses whose cell spiked, and one copy of
+// its code serves every synapse slot.
+struct FFAcc {
+    float g0[VC_MAX_NG], h0[VC_MAX_NG], g1[VC_MAX_NG], h1[VC_MAX_NG];
+};
+struct FFShape {
+    float p[VC_MAX_NG], rise[VC_MAX_NG], decay[VC_MAX_NG], dod[VC_MAX_NG];
+    uint32_t ng;
+};
+__device__ __noinline__ FFAcc ff_add(FFAcc a, const FFShape sh, float f, float nsp, float tsp,
+                                     float dt, float tBack, bool backing) {
+    if (f > 0.0f) {
+        float ddt = 0.0f;
+        if (backing) ddt = tBack - tsp;
+#pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+            if (ig < sh.ng) {
+                const float str = f * sh.p[ig];
+                if (backing && ddt > 0.0f)
+                    single_input(a.g0[ig], a.h0[ig], str * nsp, ddt, sh.rise[ig], sh.decay[ig],
+                                 sh.dod[ig]);
+                single_input(a.g1[ig], a.h1[ig], str * nsp, dt - tsp, sh.rise[ig], sh.decay[ig],
+                             sh.dod[ig]);
+            }
+        }
+    }
+    return a;
 }
 
 __device__ __forceinline__ int type_of(const vc_v1_params &p, uint32_t local) {
@@ -138,28 +171,37 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
 }
 
 // Recurrent increments of the spikes of `prev` for the calling neuron.
-// mk[c] holds, in lane l, the spike mask of neighbour-tile entry c*64 + l
-// (entry e = ib*nSeg + tt); bidv[c] the neighbour block id of that entry.
+// smk/sbid (this wavefront's LDS rows) hold the spike mask and block id of
+// neighbour-tile entry e = ib*nSeg + tt, e < nEnt.  Entries are scanned in
+// order with a ballot over 64 entries at a time; the spikes of non-empty
+// tiles are appended to the batch, which is flushed from one place.
 __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_conn &rec,
-                                           const vc_spikes &prev, uint32_t b, uint32_t myseg,
-                                           uint32_t nSeg, uint32_t nEnt, int t,
-                                           const uint64_t mk[kMaskRegs], const uint32_t bidv[kMaskRegs],
-                                           uint32_t *brow, uint32_t *bslot,
-                                           float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
-                                           float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
+                                        const vc_spikes &prev, uint32_t b, uint32_t myseg,
+                                        uint32_t nSeg, uint32_t nEnt, int t,
+                                        const uint64_t *smk, const uint32_t *sbid,
+                                        uint32_t *brow, uint32_t *bslot,
+                                        float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
+                                        float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
     const uint32_t lane = vc_lane();
     const uint32_t bs = p.blockSize;
-    uint32_t cnt = 0;   // entries in the batch (wave-uniform)
-#pragma unroll
-    for (int c = 0; c < kMaskRegs; ++c) {
-        if ((uint32_t)c * 64u >= nEnt) break;
-        uint64_t nz = __ballot(mk[c] != 0ull);
-        while (nz) {
+    const uint32_t nChunk = (nEnt + 63u) / 64u;
+    uint32_t cnt = 0;                     // entries in the batch (wave-uniform)
+    uint32_t c = 0;                       // next chunk of 64 entries to scan
+    uint64_t nz = 0;                      // non-empty entries left in chunk c-1
+    while (true) {
+        // fill the batch up to 64 spikes or until the entries are exhausted
+        while (cnt < (uint32_t)kBatch) {
+            while (nz == 0ull && c < nChunk) {
+                const uint32_t e = c * 64u + lane;
+                nz = __ballot(e < nEnt && smk[e] != 0ull);
+                ++c;
+            }
+            if (nz == 0ull) break;
             const uint32_t j = (uint32_t)__builtin_ctzll(nz);
             nz &= nz - 1ull;
-            const uint32_t e = (uint32_t)c * 64u + j;
-            const uint64_t m = vc_readlane_u64(mk[c], j);
-            const uint32_t bid = vc_readlane_u32(bidv[c], j);
+            const uint32_t e = (c - 1u) * 64u + j;
+            const uint64_t m = smk[e];
+            const uint32_t bid = sbid[e];
             const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
             if ((m >> lane) & 1ull) {
                 const uint32_t rank = vc_mbcnt(m);
@@ -169,30 +211,25 @@ __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_c
                 bslot[pos] = (bid * bs + tt * 64u + rank) | ((pre >= p.nE ? 1u : 0u) << 31);
             }
             cnt += (uint32_t)__builtin_popcountll(m);
-            if (cnt >= (uint32_t)kBatch) {
-                __builtin_amdgcn_wave_barrier();
-                flush_batch(p, rec, prev, kBatch, lane, nSeg, myseg, t, brow, bslot,
-                            lgE, lhE, lgI, lhI);
-                __builtin_amdgcn_wave_barrier();
-                const uint32_t rest = cnt - kBatch;
-                uint32_t r0 = 0, r1 = 0;
-                if (lane < rest) {
-                    r0 = brow[kBatch + lane];
-                    r1 = bslot[kBatch + lane];
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (lane < rest) {
-                    brow[lane] = r0;
-                    bslot[lane] = r1;
-                }
-                __builtin_amdgcn_wave_barrier();
-                cnt = rest;
-            }
         }
-    }
-    if (cnt > 0) {
+        if (cnt == 0u) break;
+        const uint32_t K = cnt < (uint32_t)kBatch ? cnt : (uint32_t)kBatch;
         __builtin_amdgcn_wave_barrier();
-        flush_batch(p, rec, prev, cnt, lane, nSeg, myseg, t, brow, bslot, lgE, lhE, lgI, lhI);
+        flush_batch(p, rec, prev, K, lane, nSeg, myseg, t, brow, bslot, lgE, lhE, lgI, lhI);
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t rest = cnt - K;
+        uint32_t r0 = 0, r1 = 0;
+        if (lane < rest) {
+            r0 = brow[K + lane];
+            r1 = bslot[K + lane];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < rest) {
+            brow[lane] = r0;
+            bslot[lane] = r1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        cnt = rest;
     }
 }
 
@@ -201,11 +238,14 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
                                                  vc_ff_conn ff, vc_rec_conn rec,
                                                  vc_spikes prev, vc_spikes next,
                                                  const float *__restrict__ lgnSInfo,
+                                                 uint32_t *__restrict__ ffMask,
                                                  uint32_t step, float *__restrict__ out_sInfo,
                                                  float *__restrict__ out_dep,
                                                  float *__restrict__ out_g_prev) {
     __shared__ uint32_t sbrow[kWG / 64][2 * kBatch];
     __shared__ uint32_t sbslot[kWG / 64][2 * kBatch];
+    __shared__ uint64_t smk[kWG / 64][kMaxEnt];
+    __shared__ uint32_t sbid[kWG / 64][kMaxEnt];
     const uint32_t nV1 = p.nV1;
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t b = n / p.blockSize;
@@ -220,21 +260,15 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     // ---- round 1: neighbour spike masks (lane-parallel), LGN synapses, state
     const uint32_t nb = rec.nNear[b];
     const uint32_t nEnt = nb * nSeg;
-    uint64_t mk[kMaskRegs];
-    uint32_t bidv[kMaskRegs];
-#pragma unroll
-    for (int c = 0; c < kMaskRegs; ++c) {
-        const uint32_t e = (uint32_t)c * 64u + lane;
-        mk[c] = 0ull;
-        bidv[c] = 0u;
-        if (e < nEnt) {
-            const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
-            const uint32_t bid = rec.nbr[b * rec.maxNear + ib];
-            bidv[c] = bid;
-            mk[c] = prev.mask[bid * nSeg + tt];
-        }
+    for (uint32_t e = lane; e < nEnt; e += 64u) {
+        const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
+        const uint32_t bid = rec.nbr[b * rec.maxNear + ib];
+        smk[wv][e] = prev.mask[bid * nSeg + tt];
+        sbid[wv][e] = bid;
     }
-    const uint32_t mLGN = kUpdate ? ff.nLGN[n] : 0u;
+    const bool events = kUpdate && ffMask != nullptr;
+    const uint32_t mLGN = (kUpdate && !events) ? ff.nLGN[n] : 0u;
+    const uint32_t ffBits = events ? ffMask[n] : 0u;
     float fv[kFF];
     uint32_t fid[kFF];
 #pragma unroll
@@ -265,10 +299,32 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     float sv[kFF];
 #pragma unroll
     for (int j = 0; j < kFF; ++j) sv[j] = ((uint32_t)j < mLGN) ? lgnSInfo[fid[j]] : 0.0f;
+    // event path: the first kEv synapses that received a spike this step
+    uint32_t evj[kEv];
+    float evf[kEv];
+    uint32_t evid[kEv];
+    {
+        uint32_t bits = ffBits;
+#pragma unroll
+        for (int k = 0; k < kEv; ++k) {
+            evj[k] = 0xffffffffu;
+            evf[k] = 0.0f;
+            evid[k] = 0u;
+            if (bits) {
+                const uint32_t j = (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
+                const uint32_t lid = j * nV1 + n;
+                evj[k] = j;
+                evf[k] = ff.sLGN[lid];
+                evid[k] = ff.idLGN[lid];
+            }
+        }
+    }
 
     float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
     float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
-    rec_gather(p, rec, prev, b, myseg, nSeg, nEnt, t, mk, bidv, sbrow[wv], sbslot[wv],
+    __builtin_amdgcn_wave_barrier();
+    rec_gather(p, rec, prev, b, myseg, nSeg, nEnt, t, smk[wv], sbid[wv], sbrow[wv], sbslot[wv],
                lgE, lhE, lgI, lhI);
 
     // conductances at the end of the previous step (reference: after recal_G)
@@ -363,70 +419,85 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
             if (ig < p.ngFF) decay_var(g0[ig], h0[ig], tBack, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
     }
 
-    // ---- stage c-FF: LGN synapses in list order
-    auto ff_input = [&](float f, float sI) {
-        const uint32_t nsp = (uint32_t)floorf(sI);
-        if (nsp > 0u) {
-            const float tsp = (sI - (float)nsp) * dt;
-            if (f > 0.0f) {
-                float ddt = 0.0f;
-                if (backing) ddt = tBack - tsp;
+    // ---- stage c-FF: LGN synapses in list order; only synapses whose LGN
+    // cell spiked (sInfo >= 1) do work (reference: nsp_FF > 0).
+    FFAcc acc;
+    FFShape shFF, shBG;
 #pragma unroll
-                for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-                    if (ig < p.ngFF) {
-                        const float str = f * p.pFF[t * p.ngFF + ig];
-                        if (backing && ddt > 0.0f)
-                            single_input(g0[ig], h0[ig], str * (float)nsp, ddt,
-                                         p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
-                        single_input(g1[ig], h1[ig], str * (float)nsp, dt - tsp,
-                                     p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
-                    }
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        acc.g0[ig] = g0[ig]; acc.h0[ig] = h0[ig]; acc.g1[ig] = g1[ig]; acc.h1[ig] = h1[ig];
+        const uint32_t pi = t * p.ngFF + (ig < p.ngFF ? ig : 0u);
+        shFF.p[ig] = p.pFF[pi];
+        shBG.p[ig] = p.pBG[pi];
+        shFF.rise[ig] = shBG.rise[ig] = p.riseFF[ig];
+        shFF.decay[ig] = shBG.decay[ig] = p.decayFF[ig];
+        shFF.dod[ig] = shBG.dod[ig] = p.dodFF[ig];
+    }
+    shFF.ng = shBG.ng = p.ngFF;
+    if (events && __ballot(ffBits != 0u) != 0ull) {
+        // synapses whose LGN cell spiked, in synapse order
+        float evs[kEv];
+#pragma unroll
+        for (int k = 0; k < kEv; ++k) evs[k] = (evj[k] != 0xffffffffu) ? lgnSInfo[evid[k]] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kEv; ++k) {
+            if (evj[k] != 0xffffffffu) {
+                const float sI = evs[k];
+                const float nsp = (float)(uint32_t)floorf(sI);
+                acc = ff_add(acc, shFF, evf[k], nsp, (sI - nsp) * dt, dt, tBack, backing);
+            }
+        }
+        uint32_t rest = ffBits;
+#pragma unroll
+        for (int k = 0; k < kEv; ++k) rest &= rest - 1u;    // drop the prefetched bits
+        while (rest) {
+            const uint32_t j = (uint32_t)__builtin_ctz(rest);
+            rest &= rest - 1u;
+            const uint32_t lid = j * nV1 + n;
+            const float sI = lgnSInfo[ff.idLGN[lid]];
+            const float nsp = (float)(uint32_t)floorf(sI);
+            acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
+        }
+        if (ffBits) ffMask[n] = 0u;
+    }
+    {
+        uint32_t spk = 0;
+#pragma unroll
+        for (int j = 0; j < kFF; ++j) spk |= (sv[j] >= 1.0f ? 1u : 0u) << j;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) spk |= (uint32_t)__shfl_xor((int)spk, off);
+        spk = __builtin_amdgcn_readfirstlane(spk);     // slots that spiked for any lane
+#pragma unroll
+        for (int j = 0; j < kFF; ++j) {
+            if ((spk >> j) & 1u) {
+                const float sI = sv[j];
+                if (sI >= 1.0f) {
+                    const float nsp = (float)(uint32_t)floorf(sI);
+                    acc = ff_add(acc, shFF, fv[j], nsp, (sI - nsp) * dt, dt, tBack, backing);
                 }
             }
         }
-    };
-#pragma unroll
-    for (int j = 0; j < kFF; ++j)
-        if ((uint32_t)j < mLGN) ff_input(fv[j], sv[j]);
-    for (uint32_t i0 = kFF; i0 < mLGN; i0 += 8) {       // lists longer than kFF
-        float fw[8], sw[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            fw[j] = 0.0f;
-            sw[j] = 0.0f;
-            if (i0 + j < mLGN) {
-                const uint32_t lid = (i0 + j) * nV1 + n;
-                fw[j] = ff.sLGN[lid];
-                sw[j] = lgnSInfo[ff.idLGN[lid]];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (i0 + j < mLGN) ff_input(fw[j], sw[j]);
     }
-    // background Poisson drive (extension)
+    for (uint32_t i = kFF; i < mLGN; ++i) {            // lists longer than kFF
+        const uint32_t lid = i * nV1 + n;
+        const float sI = lgnSInfo[ff.idLGN[lid]];
+        if (sI >= 1.0f) {
+            const float nsp = (float)(uint32_t)floorf(sI);
+            acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
+        }
+    }
+    // background Poisson drive (extension): one more input after the LGN list
     if (p.bgStrength[t] > 0.0f) {
         const vc_u32x4 r = vc_draw4(p.seed_lo, p.seed_hi, n, step, VC_STREAM_V1_BG, 0u, p.trial);
         const float u = vc_u01_open0(r.v[0]);
         const float frac = vc_u01_open1(r.v[1]);
         const uint32_t nsp = vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
-        if (nsp > 0u) {
-            const float tsp = frac * dt;
-            const float f = p.bgStrength[t];
-            float ddt = 0.0f;
-            if (backing) ddt = tBack - tsp;
+        if (nsp > 0u)
+            acc = ff_add(acc, shBG, p.bgStrength[t], (float)nsp, frac * dt, dt, tBack, backing);
+    }
 #pragma unroll
-            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-                if (ig < p.ngFF) {
-                    const float str = f * p.pBG[t * p.ngFF + ig];
-                    if (backing && ddt > 0.0f)
-                        single_input(g0[ig], h0[ig], str * (float)nsp, ddt,
-                                     p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
-                    single_input(g1[ig], h1[ig], str * (float)nsp, dt - tsp,
-                                 p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
-                }
-            }
-        }
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        g0[ig] = acc.g0[ig]; h0[ig] = acc.h0[ig]; g1[ig] = acc.g1[ig]; h1[ig] = acc.h1[ig];
     }
     float gE_t0 = 0.0f;
 #pragma unroll
@@ -577,9 +648,9 @@ int check_params(const vc_v1_params *p, const vc_rec_conn *rec) {
         return VC_E_UNSUPPORTED;
     }
     if (rec) {
-        if (rec->maxNear * (p->blockSize / 64) > (uint32_t)(kMaskRegs * 64)) {
-            vc_set_error("maxNear %u x tiles %u exceeds the %d neighbour tiles held per lane",
-                         rec->maxNear, p->blockSize / 64, kMaskRegs * 64);
+        if (rec->maxNear * (p->blockSize / 64) > (uint32_t)kMaxEnt) {
+            vc_set_error("maxNear %u x tiles %u exceeds the %d neighbour tiles staged in LDS",
+                         rec->maxNear, p->blockSize / 64, kMaxEnt);
             return VC_E_UNSUPPORTED;
         }
     }
@@ -611,8 +682,8 @@ uint32_t wg_size(uint32_t blockSize) {
 extern "C" int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                           const vc_ff_conn *ff, const vc_rec_conn *rec,
                           const vc_spikes *prev, const vc_spikes *next,
-                          const float *lgnSInfo, uint32_t step, float *out_sInfo,
-                          float *out_dep, float *out_g_prev) {
+                          const float *lgnSInfo, uint32_t *ffMask, uint32_t step,
+                          float *out_sInfo, float *out_dep, float *out_g_prev) {
     int rc = check_params(p, rec);
     if (rc) return rc;
     if (!s || !ff || !rec || !prev || !next || !lgnSInfo || !out_sInfo) {
@@ -623,18 +694,23 @@ extern "C" int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state
         vc_set_error("vc_v1_step: prev and next spike buffers must differ");
         return VC_E_INVALID;
     }
+    if (ffMask && ff->maxLGN > 32u) {
+        vc_set_error("vc_v1_step: event-driven LGN input needs maxLGN <= 32 (got %u)", ff->maxLGN);
+        return VC_E_INVALID;
+    }
     V1Const k = make_const(p);
     uint32_t wg = wg_size(p->blockSize);
     hipLaunchKernelGGL(k_v1_step<true>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
-                       *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, step, out_sInfo, out_dep,
-                       out_g_prev);
+                       *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask, step, out_sInfo,
+                       out_dep, out_g_prev);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
 
 extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                          const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
-                         uint32_t cur, const float *lgnSInfo, uint64_t lgnStride, uint32_t it0,
+                         uint32_t cur, const float *lgnSInfo, uint64_t lgnStride,
+                         uint32_t *ffMask, uint64_t maskStride, uint32_t it0,
                          uint32_t nsteps, float *out_sInfo, uint64_t outStride) {
     int rc = check_params(p, rec);
     if (rc) return rc;
@@ -646,13 +722,18 @@ extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state 
         vc_set_error("vc_v1_run: the two spike buffers must differ");
         return VC_E_INVALID;
     }
+    if (ffMask && ff->maxLGN > 32u) {
+        vc_set_error("vc_v1_run: event-driven LGN input needs maxLGN <= 32 (got %u)", ff->maxLGN);
+        return VC_E_INVALID;
+    }
     V1Const k = make_const(p);
     const uint32_t wg = wg_size(p->blockSize);
     for (uint32_t i = 0; i < nsteps; ++i) {
         const uint32_t nx = 1u - cur;
         hipLaunchKernelGGL(k_v1_step<true>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
                            *p, k, *s, *ff, *rec, spk2[cur], spk2[nx], lgnSInfo + i * lgnStride,
-                           it0 + i, out_sInfo + i * outStride, nullptr, nullptr);
+                           ffMask ? ffMask + i * maskStride : nullptr, it0 + i,
+                           out_sInfo + i * outStride, nullptr, nullptr);
         cur = nx;
     }
     VC_HIP_CHECK(hipGetLastError());
@@ -672,7 +753,8 @@ extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_
     vc_ff_conn ff{};
     vc_spikes nx{};
     hipLaunchKernelGGL(k_v1_step<false>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
-                       *p, k, *s, ff, *rec, *prev, nx, nullptr, 0u, nullptr, nullptr, nullptr);
+                       *p, k, *s, ff, *rec, *prev, nx, nullptr, nullptr, 0u, nullptr, nullptr,
+                       nullptr);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -1,0 +1,351 @@
+"""The reference simulator's binary input and output files (little-endian;
+Size/PosInt = u32, Float = f32), read and written so that this package is a
+drop-in for `patch_fast`'s file interface.
+
+Inputs (readers + writers, the writers let a synthetic Network be saved in
+the reference's formats):
+  stimulus.bin            src/patch.cu:1393-1470, frames read at :5985-5993
+  V1_allpos<res>.bin      src/patch.cu:2754-2790
+  connectome_cfg<conV1>   src/patch.cu:634-645
+  V1_conMat<conV1>.bin    src/patch.cu:3441-3461, 3649-3687 (dense, streamed
+                          block by block into the masked-CSR rows)
+  neighborBlock<conV1>    src/patch.cu:3481-3512
+  LGN_V1_idList / sList   src/util/util.h:161-185, src/patch.h:30-77
+  fLGN (old LGN setup)    src/patch.cu:2472-2530
+Outputs:
+  rawData<out>.bin        header src/patch.cu:4688-4716; per step sInfo
+                          (:6139-6164), the dep/w/v/gFF/hFF block, then
+                          gE/gI/hE/hI and gap (src/stats.cu:38-97); read back
+                          with the semantics of src/readPatchOutput.py:291-351
+  LGN_fr<out>.bin         src/patch.cu:4666-4686, per step at :6346
+  LGN_sp<out>.bin         per step sInfo[nLGN] (src/patch.cu:6422-6424)
+"""
+from __future__ import annotations
+
+import io
+import struct
+from pathlib import Path
+
+import numpy as np
+
+
+def _u32(f, n=1):
+    a = np.fromfile(f, dtype="<u4", count=n)
+    if a.size != n:
+        raise ValueError(f"{getattr(f, 'name', 'file')}: truncated")
+    return a if n > 1 else int(a[0])
+
+
+def _f32(f, n):
+    a = np.fromfile(f, dtype="<f4", count=n)
+    if a.size != n:
+        raise ValueError(f"{getattr(f, 'name', 'file')}: truncated")
+    return a
+
+
+# ------------------------------------------------------------------ stimulus
+def write_stimulus(path, frames: np.ndarray, init_lms=(0.5, 0.5, 0.5), buffer_deg=0.0,
+                   range_deg=2.0, neye=1):
+    """frames [nFrame, 3, H, W] float32 (L, M, S)."""
+    nF, c, H, W = frames.shape
+    assert c == 3
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", -1))
+        f.write(struct.pack("<3I", nF, H, W))
+        f.write(struct.pack("<3f", *init_lms))
+        f.write(struct.pack("<2f", buffer_deg, range_deg))
+        f.write(struct.pack("<I", neye))
+        f.write(np.ascontiguousarray(frames, dtype="<f4").tobytes())
+
+
+def read_stimulus(path):
+    """Returns dict(frames [nFrame,3,H,W], init_lms, buffer_deg, range_deg, neye)."""
+    with open(path, "rb") as f:
+        inputType = struct.unpack("<i", f.read(4))[0]
+        if inputType != -1:
+            raise ValueError(f"{path}: inputType {inputType} (only -1, an LMS movie, is supported)")
+        nF, H, W = struct.unpack("<3I", f.read(12))
+        init = struct.unpack("<3f", f.read(12))
+        buf, rng = struct.unpack("<2f", f.read(8))
+        neye = struct.unpack("<I", f.read(4))[0]
+        frames = _f32(f, nF * 3 * H * W).reshape(nF, 3, H, W)
+    return dict(frames=frames, init_lms=init, buffer_deg=buf, range_deg=rng, neye=neye)
+
+
+# ------------------------------------------------------------------ V1 geometry
+def write_v1_allpos(path, nblock, blockSize, pos_mm: np.ndarray):
+    x0, y0 = float(pos_mm[:, 0].min()), float(pos_mm[:, 1].min())
+    xs, ys = float(np.ptp(pos_mm[:, 0])), float(np.ptp(pos_mm[:, 1]))
+    with open(path, "wb") as f:
+        f.write(struct.pack("<3I", nblock, blockSize, 2))
+        f.write(struct.pack("<4d", x0, xs, y0, ys))
+        f.write(np.ascontiguousarray(pos_mm[:, 0], dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(pos_mm[:, 1], dtype="<f8").tobytes())
+
+
+def read_v1_allpos(path):
+    with open(path, "rb") as f:
+        nblock, blockSize, posDim = struct.unpack("<3I", f.read(12))
+        if posDim > 2:
+            raise ValueError("3-D positions are not supported (src/patch.cu:2762)")
+        struct.unpack("<4d", f.read(32))
+        n = nblock * blockSize
+        x = np.fromfile(f, "<f8", n)
+        y = np.fromfile(f, "<f8", n)
+    return nblock, blockSize, np.stack([x, y], 1)
+
+
+def write_connectome_cfg(path, typeAcc, nTypeI):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<2I", len(typeAcc), nTypeI))
+        f.write(np.asarray(typeAcc, "<u4").tobytes())
+
+
+def read_connectome_cfg(path):
+    with open(path, "rb") as f:
+        nType, nTypeI = struct.unpack("<2I", f.read(8))
+        typeAcc = _u32(f, nType) if nType > 1 else np.array([_u32(f, 1)], np.uint32)
+    return int(nType), int(nTypeI), np.atleast_1d(typeAcc).astype(np.uint32)
+
+
+# ------------------------------------------------------------------ connectome
+def write_neighbor_block(path, nNear, nbr_lists):
+    """nbr_lists[b] = all neighbour ids of block b, the first nNear[b] near."""
+    with open(path, "wb") as f:
+        f.write(np.asarray(nNear, "<u4").tobytes())
+        f.write(np.asarray([len(l) for l in nbr_lists], "<u4").tobytes())
+        for l in nbr_lists:
+            f.write(np.asarray(l, "<u4").tobytes())
+
+
+def read_neighbor_block(path, nblock):
+    with open(path, "rb") as f:
+        nNear = np.fromfile(f, "<u4", nblock)
+        nAll = np.fromfile(f, "<u4", nblock)
+        lists = [np.fromfile(f, "<u4", int(k)) for k in nAll]
+    return nNear.astype(np.uint32), lists
+
+
+def write_conmat(path, net):
+    """Dense V1_conMat [nblock][nearNeighborBlock][bs][bs] of a Network, one
+    block at a time."""
+    sp = net.spec
+    bs = sp.blockSize
+    with open(path, "wb") as f:
+        f.write(struct.pack("<I", net.maxNear))
+        for b in range(sp.nblock):
+            M = np.zeros((net.maxNear, bs, bs), "<f4")
+            for ib in range(net.maxNear):
+                pre, post, w = net.rec_pairs[b * net.maxNear + ib]
+                M[ib, pre, post] = w
+            f.write(M.tobytes())
+
+
+def read_conmat_pairs(path, nblock, blockSize):
+    """Stream V1_conMat block by block into per-(block, neighbour slot) sorted
+    (pre, post, w) triples of the strictly positive entries (the entries the
+    reference's recal_G uses, src/coredynamics.cu:2758)."""
+    bs = blockSize
+    pairs = []
+    with open(path, "rb") as f:
+        maxNear = _u32(f, 1)
+        for b in range(nblock):
+            M = np.fromfile(f, "<f4", maxNear * bs * bs)
+            if M.size != maxNear * bs * bs:
+                raise ValueError(f"{path}: truncated at block {b}")
+            M = M.reshape(maxNear, bs, bs)
+            for ib in range(maxNear):
+                pre, post = np.nonzero(M[ib] > 0)
+                pairs.append((pre.astype(np.uint32), post.astype(np.uint32),
+                              M[ib, pre, post].astype(np.float32)))
+    return int(maxNear), pairs
+
+
+# ------------------------------------------------------------------ LGN -> V1
+def write_list_of_lists(path, lists, dtype):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<I", len(lists)))
+        for l in lists:
+            a = np.asarray(l, dtype)
+            f.write(struct.pack("<I", a.size))
+            f.write(a.tobytes())
+
+
+def read_list_of_lists(path, dtype):
+    out = []
+    with open(path, "rb") as f:
+        n = _u32(f, 1)
+        for _ in range(n):
+            k = _u32(f, 1)
+            out.append(np.fromfile(f, dtype, k) if k else np.zeros(0, dtype))
+    return out
+
+
+def write_lgn_v1(id_path, s_path, ids_lists, s_lists):
+    write_list_of_lists(id_path, ids_lists, "<u4")
+    maxList = max((len(l) for l in s_lists), default=0)
+    with open(s_path, "wb") as f:
+        f.write(struct.pack("<2I", len(s_lists), maxList))
+        for l in s_lists:
+            a = np.asarray(l, "<f4")
+            f.write(struct.pack("<I", a.size))
+            f.write(a.tobytes())
+
+
+def read_lgn_v1(id_path, s_path, sRatioLGN, typeAcc):
+    """-> (nLGN_of [nV1], sLGN [maxList][nV1], idLGN [maxList][nV1]): the
+    transposed layout of read_LGN (src/patch.h:30-77), strengths scaled by
+    sRatioLGN of the neuron's type."""
+    ids = read_list_of_lists(id_path, "<u4")
+    with open(s_path, "rb") as f:
+        nList, maxList = struct.unpack("<2I", f.read(8))
+        s_lists = []
+        for _ in range(nList):
+            k = _u32(f, 1)
+            s_lists.append(np.fromfile(f, "<f4", k) if k else np.zeros(0, "<f4"))
+    if len(ids) != nList:
+        raise ValueError("LGN_V1_idList and LGN_V1_sList list counts differ")
+    bs = int(typeAcc[-1])
+    nV1 = nList
+    mx = max(1, maxList)
+    sLGN = np.zeros((mx, nV1), np.float32)
+    idLGN = np.zeros((mx, nV1), np.uint32)
+    nof = np.zeros(nV1, np.uint32)
+    for i in range(nV1):
+        k = len(s_lists[i])
+        if len(ids[i]) != k:
+            raise ValueError(f"neuron {i}: id and strength lists differ in length")
+        t = int(np.searchsorted(typeAcc, i % bs, side="right"))
+        sLGN[:k, i] = s_lists[i] * np.float32(sRatioLGN[t])
+        idLGN[:k, i] = ids[i]
+        nof[i] = k
+    return nof, sLGN, idLGN
+
+
+# ------------------------------------------------------------------ fLGN
+_FLGN_2N = ("polar", "ecc", "rw", "rh", "orient", "k", "ratio", "tauR", "tauD", "nR", "nD", "delay")
+
+
+def write_flgn(path, lgn: dict, lgn_type=None):
+    nL = lgn["polar"].shape[1]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<I", nL))
+        t = np.zeros(nL, "<u4") if lgn_type is None else np.asarray(lgn_type, "<u4")
+        f.write(t.tobytes())
+        for k in _FLGN_2N:
+            f.write(np.ascontiguousarray(lgn[k], "<f4").tobytes())
+        for k in ("spont", "c50", "sharpness"):
+            f.write(np.ascontiguousarray(lgn[k], "<f4").tobytes())
+        f.write(np.ascontiguousarray(lgn["coneType"], "<u4").tobytes())
+        f.write(np.ascontiguousarray(lgn["covariant"], "<f4").tobytes())
+
+
+def read_flgn(path):
+    with open(path, "rb") as f:
+        nL = _u32(f, 1)
+        out = {"type": np.fromfile(f, "<u4", nL)}
+        for k in _FLGN_2N:
+            out[k] = _f32(f, 2 * nL).reshape(2, nL)
+        for k in ("spont", "c50", "sharpness"):
+            out[k] = _f32(f, nL)
+        out["coneType"] = np.fromfile(f, "<u4", 2 * nL).reshape(2, nL)
+        out["covariant"] = _f32(f, nL)
+    return out
+
+
+# ------------------------------------------------------------------ outputs
+class RawDataWriter:
+    """rawData<suffix>.bin: header (dt f4, nt u4, nV1 u4, iModel i4, mI u4,
+    hWrite u4, ngFF u4, ngE u4, ngI u4), then per step
+    sInfo[nV1], dep[nV1], w[nV1] (iModel 1), v[nV1], gFF[ngFF][nV1],
+    hFF[ngFF][nV1] (hWrite), gE[ngE][nV1], gI[ngI][nV1], hE, hI (hWrite),
+    gap[mI]."""
+
+    def __init__(self, path, dt, nt, nV1, iModel, mI, hWrite, ngFF, ngE, ngI):
+        self.f = open(path, "wb")
+        self.f.write(struct.pack("<fIIiIIIII", dt, nt, nV1, iModel, mI, int(hWrite), ngFF, ngE, ngI))
+        self.nV1, self.iModel, self.mI, self.hWrite = nV1, iModel, mI, hWrite
+
+    def write_step(self, sInfo, dep, w, v, gFF, hFF, gE, gI, hE, hI, gap=None):
+        parts = [sInfo, dep]
+        if self.iModel == 1:
+            parts.append(w)
+        parts += [v, gFF]
+        if self.hWrite:
+            parts.append(hFF)
+        parts += [gE, gI]
+        if self.hWrite:
+            parts += [hE, hI]
+        if self.mI:
+            parts.append(np.zeros(self.mI, np.float32) if gap is None else gap)
+        for a in parts:
+            self.f.write(np.ascontiguousarray(a, "<f4").tobytes())
+
+    def close(self):
+        self.f.close()
+
+
+def read_raw_data(path):
+    """-> dict(header..., sInfo [nt, nV1], and the per-step traces)."""
+    with open(path, "rb") as f:
+        dt, nt, nV1, iModel, mI, hW, ngFF, ngE, ngI = struct.unpack("<fIIiIIIII", f.read(36))
+        per = nV1 * (1 + (3 if iModel == 1 else 2) + (ngE + ngI + ngFF) * (1 + hW)) + mI
+        data = np.fromfile(f, "<f4", per * nt)
+    if data.size != per * nt:
+        raise ValueError(f"{path}: truncated")
+    data = data.reshape(nt, per)
+    out = dict(dt=dt, nt=nt, nV1=nV1, iModel=iModel, mI=mI, hWrite=hW, ngFF=ngFF, ngE=ngE, ngI=ngI)
+    o = 0
+
+    def take(k):
+        nonlocal o
+        a = data[:, o:o + k * nV1].reshape(nt, k, nV1) if k else None
+        o += k * nV1
+        return a
+    out["sInfo"] = take(1)[:, 0]
+    out["dep"] = take(1)[:, 0]
+    out["w"] = take(1)[:, 0] if iModel == 1 else None
+    out["v"] = take(1)[:, 0]
+    out["gFF"] = take(ngFF)
+    out["hFF"] = take(ngFF) if hW else None
+    out["gE"] = take(ngE)
+    out["gI"] = take(ngI)
+    out["hE"] = take(ngE) if hW else None
+    out["hI"] = take(ngI) if hW else None
+    out["gap"] = data[:, o:o + mI]
+    return out
+
+
+def spikes_from_sinfo(sInfo: np.ndarray, dt: float):
+    """Spike times per neuron from a [nt, nV1] sInfo array, the decoding of
+    readSpike (src/readPatchOutput.py:314-340): sInfo >= 1 is nsp + tsp/dt;
+    several spikes in one step are spread around the mean time."""
+    nt, nV1 = sInfo.shape
+    out = [[] for _ in range(nV1)]
+    its, js = np.nonzero(sInfo >= 1)
+    for it, j in zip(its, js):
+        s = float(sInfo[it, j])
+        nsp = int(np.floor(s))
+        tsp = s - nsp
+        if nsp > 1:
+            dtsp = tsp / nsp if 1 - tsp > 0.5 else (1 - tsp) / nsp
+            t0 = tsp - (nsp // 2) * dtsp
+            out[j].extend(((it + t0 + k * dtsp) * dt) for k in range(nsp))
+        else:
+            out[j].append((it + tsp) * dt)
+    return [np.asarray(x) for x in out]
+
+
+class StepWriter:
+    """LGN_fr (header nt, nLGN) or LGN_sp (no header) per-step float rows."""
+
+    def __init__(self, path, nt=None, n=None):
+        self.f = open(path, "wb")
+        if nt is not None:
+            self.f.write(struct.pack("<2I", nt, n))
+
+    def write(self, rows: np.ndarray):
+        self.f.write(np.ascontiguousarray(rows, "<f4").tobytes())
+
+    def close(self):
+        self.f.close()

@@ -210,6 +210,24 @@ class Network:
                 M[b, ib, pre, post] = w
         return M
 
+    def ff_reverse(self):
+        """(revOff u32 [nLGN+1], revTgt u32): for every LGN cell the synapses
+        it feeds, encoded (neuron << 5) | slot (see vc_ff_events)."""
+        sp = self.spec
+        mx = self.sLGN.shape[0]
+        if mx > 32:
+            raise ValueError("event-driven LGN delivery needs <= 32 LGN inputs per neuron")
+        slot = np.arange(mx)[:, None]
+        valid = slot < self.nLGN_of[None, :].astype(np.int64)
+        cells = self.idLGN[valid].astype(np.int64)
+        tgt = ((np.broadcast_to(np.arange(sp.nV1)[None, :], valid.shape)[valid].astype(np.int64) << 5)
+               | np.broadcast_to(slot, valid.shape)[valid])
+        order = np.argsort(cells, kind="stable")
+        cnt = np.bincount(cells, minlength=sp.nLGN)
+        off = np.zeros(sp.nLGN + 1, np.int64)
+        np.cumsum(cnt, out=off[1:])
+        return off.astype(np.uint32), tgt[order].astype(np.uint32)
+
     def n_rec_synapses(self) -> int:
         return int(sum(p[0].size for p in self.rec_pairs))
 

@@ -26,7 +26,7 @@ class GpuSim:
     """One simulation of one network on one GPU."""
 
     def __init__(self, net: Network, device: str = "cuda", record: bool = True,
-                 state: dict | None = None):
+                 state: dict | None = None, ff_events: bool | None = None):
         self.lib = abi.load_library()
         self.net = net
         sp = net.spec
@@ -49,6 +49,19 @@ class GpuSim:
         self.sLGN = T(net.sLGN, np.float32)
         self.idLGN = T(net.idLGN.astype(np.uint32).view(np.int32))
         self.ff = abi.FFConn(ptr(self.nLGN_of), ptr(self.sLGN), ptr(self.idLGN), net.sLGN.shape[0])
+        # ---- event-driven LGN delivery (default when <= 32 inputs per neuron)
+        if ff_events is None:
+            ff_events = net.sLGN.shape[0] <= 32
+        self.ev = None
+        if ff_events:
+            off, tgt = net.ff_reverse()
+            self.revOff = T(off.view(np.int32))
+            self.revTgt = T(tgt.view(np.int32) if tgt.size else np.zeros(1, np.int32))
+            self.ffMask2 = [torch.zeros(self.tm["ntPerFrame"], sp.nV1, dtype=torch.int32, device=d)
+                            for _ in range(2)]
+            self.ev2 = [abi.FFEvents(ptr(self.revOff), ptr(self.revTgt), ptr(m), sp.nV1)
+                        for m in self.ffMask2]
+            self.ev = self.ev2[0]
         # ---- recurrent synapses (masked CSR)
         rowMask, rowOff, w = net.masked_csr()
         self.rowMask = T(rowMask.view(np.int64))
@@ -65,6 +78,12 @@ class GpuSim:
         self.oh = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
         self.spk = [abi.Spikes(ptr(self.masks[i]), ptr(self.og[i]), ptr(self.oh[i])) for i in range(2)]
         self.spk2 = (abi.Spikes * 2)(*self.spk)
+        # ---- streams (default priority) and the LGN pipeline state
+        self.s_v1 = torch.cuda.current_stream(d)
+        self.s_lgn = torch.cuda.Stream(device=d)
+        self.lgn_done = [None, None]     # event: LGN period in buffer i computed
+        self.v1_done = [None, None]      # event: V1 steps reading buffer i finished
+        self.lgn_next = 0                # next frame period whose LGN stage is due
         # ---- LGN
         self._setup_lgn()
         self.it = 0
@@ -73,6 +92,7 @@ class GpuSim:
         self.sInfo_hist: list[torch.Tensor] = []
         self.lgn_hist: list[torch.Tensor] = []
         self.sInfo_step = torch.zeros(sp.nV1, dtype=torch.float32, device=d)
+        self.s_lgn.wait_stream(self.s_v1)          # set-up work precedes the pipeline
 
     # ------------------------------------------------------------------ LGN
     def _setup_lgn(self):
@@ -121,11 +141,15 @@ class GpuSim:
             init[c] = net.init_lms[c]
         self.init_frame = torch.as_tensor(init, device=d)
         self.Rring = torch.zeros(tm["maxFrame"], sp.nLGN, 2, **f32)
-        self._frame_response(self.init_frame, 0)
+        self._frame_response(self.init_frame, 0)          # on the current stream
         for s in range(1, tm["maxFrame"]):
             self.Rring[s].copy_(self.Rring[0])
-        self.lgn_sInfo = torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32)
-        self.lgn_fr = torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32)
+        # double buffers: the LGN stage of frame period k+1 runs on its own
+        # stream while the V1 steps of period k run
+        self.lgn_sInfo2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
+        self.lgn_fr2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
+        self.lgn_sInfo = self.lgn_sInfo2[0]
+        self.lgn_fr = self.lgn_fr2[0]
 
     def _frame_response(self, frame: torch.Tensor, slot: int):
         sp = self.net.spec
@@ -135,56 +159,86 @@ class GpuSim:
               "vc_lgn_frame_response")
 
     # ------------------------------------------------------------------ steps
-    def _lgn_period(self, it: int):
-        """Frame entering at step `it` (a multiple of ntPerFrame) and the LGN
-        steps of the following frame period."""
+    def _hs(self, stream) -> C.c_void_p:
+        return C.c_void_p(stream.cuda_stream)
+
+    def _lgn_period(self, k: int):
+        """Issue, on the LGN stream, the frame entering at period k and the LGN
+        steps of period k into buffer k % 2."""
         tm = self.tm
         npf = tm["ntPerFrame"]
-        fidx = it // npf
-        self._frame_response(self.frames[fidx % self.frames.shape[0]], fidx % tm["maxFrame"])
-        check(self.lib.vc_lgn_steps(None, C.byref(self.lp), C.byref(self.ls), ptr(self.Rring),
-                                    it, npf, ptr(self.lgn_sInfo), ptr(self.lgn_fr)),
+        it = k * npf
+        buf = k % 2
+        st = self.s_lgn
+        if self.v1_done[buf] is not None:
+            st.wait_event(self.v1_done[buf])           # V1 of period k-2 released it
+        fidx = k
+        sp = self.net.spec
+        frame = self.frames[fidx % self.frames.shape[0]]
+        check(self.lib.vc_lgn_frame_response(self._hs(st), ptr(frame), sp.width, sp.height,
+                                             sp.nLGN, sp.nSpatialSample1D, ptr(self.SCx),
+                                             ptr(self.SCy), ptr(self.SW), ptr(self.coneType),
+                                             ptr(self.Rring[fidx % tm["maxFrame"]])),
+              "vc_lgn_frame_response")
+        ev = C.byref(self.ev2[buf]) if self.ev is not None else None
+        check(self.lib.vc_lgn_steps(self._hs(st), C.byref(self.lp), C.byref(self.ls),
+                                    ptr(self.Rring), it, npf, ptr(self.lgn_sInfo2[buf]),
+                                    ptr(self.lgn_fr2[buf]), ev),
               "vc_lgn_steps")
+        e = torch.cuda.Event()
+        e.record(st)
+        self.lgn_done[buf] = e
 
     def run(self, nsteps: int, out: torch.Tensor | None = None, events: list | None = None):
         """Advance `nsteps` steps.  V1 sInfo of step i goes to out[i] if `out`
         ([nsteps, nV1] float32 on the device) is given, else to the history
         when recording, else to a scratch row.  If `events` is a list, a
         (start, end, n) triple of timing events bracketing each run of n V1
-        step launches is appended (on the launch stream)."""
+        step launches is appended (on the V1 stream).  The LGN stage runs one
+        frame period ahead on its own stream."""
         sp = self.net.spec
         npf = self.tm["ntPerFrame"]
         if out is None and self.record:
             out = torch.empty(nsteps, sp.nV1, dtype=torch.float32, device=self.dev)
             self.sInfo_hist.append(out)
         done = 0
-        lgn_base = ptr(self.lgn_sInfo)
+        sv = self.s_v1
         while done < nsteps:
             it = self.it
+            k = it // npf
             k_in = it % npf
-            if k_in == 0:
-                self._lgn_period(it)
-                if self.record:
-                    self.lgn_hist.append(self.lgn_sInfo.clone())
+            while self.lgn_next <= k + 1:              # this period and the next
+                self._lgn_period(self.lgn_next)
+                self.lgn_next += 1
+            buf = k % 2
+            sv.wait_event(self.lgn_done[buf])
+            if k_in == 0 and self.record:
+                self.lgn_hist.append(self.lgn_sInfo2[buf].clone())
             n = min(npf - k_in, nsteps - done)
             if out is not None:
                 optr, ostride = ptr(out) + done * sp.nV1 * 4, sp.nV1
             else:
                 optr, ostride = ptr(self.sInfo_step), 0
+            mptr = ptr(self.ffMask2[buf]) + k_in * sp.nV1 * 4 if self.ev is not None else None
             if events is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
-                e0.record()
-            check(self.lib.vc_v1_run(None, C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                e0.record(sv)
+            check(self.lib.vc_v1_run(self._hs(sv), C.byref(self.p), C.byref(self.s), C.byref(self.ff),
                                      C.byref(self.rec), self.spk2, self.cur,
-                                     lgn_base + k_in * sp.nLGN * 4, sp.nLGN, it, n, optr, ostride),
+                                     ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, sp.nLGN,
+                                     mptr, sp.nV1, it, n, optr, ostride),
                   "vc_v1_run")
             if events is not None:
                 e1 = torch.cuda.Event(enable_timing=True)
-                e1.record()
+                e1.record(sv)
                 events.append((e0, e1, n))
             self.cur ^= (n & 1)
             self.it += n
             done += n
+            if self.it % npf == 0:
+                e = torch.cuda.Event()
+                e.record(sv)
+                self.v1_done[buf] = e
 
     def finalize(self):
         """Apply the last step's pending recurrent increments (the reference
