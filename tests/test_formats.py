@@ -1,0 +1,99 @@
+"""The reference's file formats and cfg semantics: round trips, the cfg
+parser's Boost program_options rules, and a synthetic network saved in the
+reference's input files and loaded back through a cfg."""
+import numpy as np
+import pytest
+
+
+def test_cfg_semantics(pkg):
+    from visual_cortex_amd import config
+    text = """
+    # comment
+    dt = 0.125 # in ms
+    vThres = -20.0, -50.0
+    pFF = 0.9, 0.1
+    pFF = 0.8, 0.2
+    inputFolder = /in/
+    conV1_suffix = abc
+    fV1_conMat = !/abs/conmat.bin
+    [sec]
+    x = 3
+    """
+    c = config.Config(text, {"dt": 0.25})
+    assert c.float("dt") == 0.25                                 # command line wins
+    assert c.floats("vThres") == [-20.0, -50.0]
+    assert c.floats("pFF") == [0.9, 0.1, 0.8, 0.2]                # repeated keys append
+    assert c.int("sec.x") == 3
+    assert c.path("fNeighborBlock") == "/in/neighborBlock-abc.bin"
+    assert c.path("fV1_conMat") == "/abs/conmat.bin-abc.bin"       # '!' drops the folder only
+    sp = config.spec_from_config(c)
+    assert sp.pFF == ((0.9, 0.1), (0.8, 0.2)) and sp.dt == 0.25
+    with pytest.raises(NotImplementedError):
+        config.spec_from_config(config.Config("InhGap = true"))
+    with pytest.raises(NotImplementedError):
+        config.spec_from_config(config.Config("noisyDep = 0.1, 0"))
+
+
+def test_stimulus_and_lists_round_trip(pkg, tmp_path):
+    from visual_cortex_amd import formats as F
+    fr = np.random.default_rng(0).random((3, 3, 5, 7)).astype(np.float32)
+    F.write_stimulus(tmp_path / "s.bin", fr, (0.1, 0.2, 0.3), 0.5, 2.0)
+    st = F.read_stimulus(tmp_path / "s.bin")
+    assert np.array_equal(st["frames"], fr) and st["range_deg"] == 2.0 and st["buffer_deg"] == 0.5
+    lists = [np.array([1, 2, 3], np.uint32), np.zeros(0, np.uint32), np.array([7], np.uint32)]
+    F.write_list_of_lists(tmp_path / "l.bin", lists, "<u4")
+    back = F.read_list_of_lists(tmp_path / "l.bin", "<u4")
+    assert all(np.array_equal(a, b) for a, b in zip(lists, back))
+
+
+def test_raw_data_round_trip_and_spike_decoding(pkg, tmp_path):
+    from visual_cortex_amd import formats as F
+    nV1, nt = 8, 3
+    w = F.RawDataWriter(tmp_path / "r.bin", 0.125, nt, nV1, 1, 2, True, 2, 2, 1)
+    rng = np.random.default_rng(1)
+    recs = []
+    for it in range(nt):
+        sI = np.full(nV1, -60.0, np.float32)
+        sI[it] = 1.25                                       # a spike at 1/4 of the step
+        parts = [sI] + [rng.random(nV1).astype(np.float32) for _ in range(3)] + \
+                [rng.random((2, nV1)).astype(np.float32) for _ in range(2)] + \
+                [rng.random((2, nV1)).astype(np.float32), rng.random((1, nV1)).astype(np.float32),
+                 rng.random((2, nV1)).astype(np.float32), rng.random((1, nV1)).astype(np.float32)]
+        w.write_step(*parts)
+        recs.append(parts)
+    w.close()
+    d = F.read_raw_data(tmp_path / "r.bin")
+    assert d["nt"] == nt and d["mI"] == 2
+    for it in range(nt):
+        assert np.array_equal(d["sInfo"][it], recs[it][0])
+        assert np.array_equal(d["v"][it], recs[it][3])
+        assert np.array_equal(d["gI"][it], recs[it][7])
+        assert np.array_equal(d["hI"][it], recs[it][9])
+    sp = F.spikes_from_sinfo(d["sInfo"], 0.125)
+    assert np.allclose(sp[1], [(1 + 0.25) * 0.125])
+    # two spikes in one step are spread around the mean time (readSpike)
+    two = F.spikes_from_sinfo(np.array([[2.5]], np.float32), 1.0)[0]
+    assert len(two) == 2
+
+
+def test_network_files_round_trip(pkg, tmp_path):
+    """A synthetic network written in the reference's input files and read
+    back through a cfg gives identical arrays (masked CSR, LGN->V1, LGN
+    parameters, stimulus)."""
+    from visual_cortex_amd import config, loader, network
+    net = network.build_network(network.spec_tiny(nt=128))
+    cfg_path = loader.save_network(net, tmp_path / "in")
+    back = loader.load_network(config.Config.from_file(cfg_path))
+    a, b = net.masked_csr(), back.masked_csr()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert np.array_equal(back.nbr, net.nbr) and np.array_equal(back.nNear, net.nNear)
+    np.testing.assert_array_equal(back.idLGN, net.idLGN)
+    np.testing.assert_allclose(back.sLGN, net.sLGN, rtol=1e-6)
+    for k in ("polar", "ecc", "rw", "rh", "k", "tauR", "nD", "coneType"):
+        assert np.array_equal(back.lgn[k], net.lgn[k]), k
+    assert np.array_equal(back.lgn["logA"], net.lgn["logA"])
+    assert np.array_equal(back.lgn["logB"], net.lgn["logB"])
+    assert np.array_equal(back.frames, net.frames)
+    assert back.spec.nLGN == net.spec.nLGN and back.spec.nE == net.spec.nE
+    assert back.spec.vThres == net.spec.vThres and back.spec.pE == net.spec.pE

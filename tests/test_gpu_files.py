@@ -1,0 +1,33 @@
+"""The file-level drop-in on the GPU: a network saved in the reference's
+input files and run through the cfg-driven CLI produces the same spikes as
+the same network run directly, and its rawData output decodes with the
+reference reader's semantics."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cli_from_files_matches_direct_run(pkg, tmp_path):
+    from visual_cortex_amd import cli, formats, loader, network
+    from visual_cortex_amd.sim import GpuSim
+    print("test_cli_from_files_matches_direct_run: tiny, 192 steps", flush=True)
+    net = network.build_network(network.spec_tiny(nt=192, bgStrength=(0.4, 0.4)))
+    cfg = loader.save_network(net, tmp_path / "run")
+    with open(cfg, "a") as f:
+        f.write("rawData = true\nsaveLGN_fr = true\ngetLGN_sp = true\nhWrite = true\n")
+    assert cli.main(["-c", str(cfg)]) == 0
+    raw = formats.read_raw_data(tmp_path / "run" / "rawData-vc.bin")
+    assert raw["nt"] == 192 and raw["nV1"] == net.spec.nV1
+    g = GpuSim(net)
+    g.run(192)
+    s_direct = g.sInfo()
+    assert np.array_equal(raw["sInfo"].view(np.uint32), s_direct.view(np.uint32))
+    g.finalize()
+    st = g.state_numpy()
+    assert np.array_equal(raw["v"][-1], st["v"])
+    assert np.array_equal(raw["gE"][-1], st["gE"]) and np.array_equal(raw["hI"][-1], st["hI"])
+    lsp = np.fromfile(tmp_path / "run" / "LGN_sp-vc.bin", "<f4").reshape(192, -1)
+    assert np.array_equal(lsp.view(np.uint32), g.lgn_sinfo()[:192].view(np.uint32))
+    spikes = formats.spikes_from_sinfo(raw["sInfo"], net.spec.dt)
+    assert sum(len(x) for x in spikes) == int((s_direct >= 1).sum()) > 0

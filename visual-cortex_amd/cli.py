@@ -1,0 +1,58 @@
+"""`patch_fast -c <cfg> [-s seed] [-p print]` for the MI355X simulation step
+(src/patch.cu:172-177).  Runs nt steps of the network the cfg's input files
+describe and writes rawData / LGN_fr / LGN_sp like the reference when the cfg
+asks for them (rawData, saveLGN_fr, getLGN_sp; hWrite for the h traces).
+
+    python visual-cortex_amd/cli.py -c patchV1.cfg [-s seed]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import sys
+import time
+from pathlib import Path
+
+
+def _pkg():
+    name = "visual_cortex_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    d = Path(__file__).resolve().parent
+    spec = importlib.util.spec_from_file_location(name, d / "__init__.py",
+                                                  submodule_search_locations=[str(d)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="patch_fast")
+    ap.add_argument("-c", "--cfg", required=True)
+    ap.add_argument("-s", "--seed", type=int)
+    ap.add_argument("-p", "--print", dest="print_log", default="false")
+    a = ap.parse_args(argv)
+    _pkg()
+    from visual_cortex_amd import config, loader
+    from visual_cortex_amd.sim import GpuSim
+    cmd = {} if a.seed is None else {"seed": a.seed}
+    cfg = config.Config.from_file(a.cfg, cmd)
+    t0 = time.time()
+    net = loader.load_network(cfg)
+    sp = net.spec
+    print(f"{sp.nV1} V1 neurons in {sp.nblock} blocks, {sp.nLGN} LGN cells, "
+          f"loaded in {time.time() - t0:.1f}s", flush=True)
+    sim = GpuSim(net, record=False)
+    outputs = dict(rawData=cfg.bool("rawData", True), lgn_fr=cfg.bool("saveLGN_fr", True),
+                   lgn_sp=cfg.bool("getLGN_sp", False), hWrite=cfg.bool("hWrite", True))
+    paths = dict(rawData=cfg.path("fRawData"), lgn_fr=cfg.path("fLGN_fr"),
+                 lgn_sp=cfg.path("fLGN_sp"))
+    t0 = time.time()
+    sim.run_with_outputs(sp.nt, paths, outputs)
+    print(f"simulated {sp.nt} steps ({sp.nt * sp.dt:g} ms) in {time.time() - t0:.1f}s", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -111,6 +111,9 @@ class ModelSpec:
     # seeds
     seed: int = 1234
     trial: int = 0
+    # set when the network comes from the reference's input files
+    nLGN_override: int = 0
+    stim_buffer_deg: float = 0.0
 
     @property
     def nV1(self) -> int:
@@ -118,7 +121,7 @@ class ModelSpec:
 
     @property
     def nLGN(self) -> int:
-        return self.nLGN_side * self.nLGN_side
+        return self.nLGN_override if self.nLGN_override else self.nLGN_side * self.nLGN_side
 
     @property
     def nI(self) -> int:
@@ -348,13 +351,10 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
     kC = np.where(on, 1.0, -1.0)
     kS = -spec.kS_over_kC * kC
     two = lambda a, b_: np.stack([np.broadcast_to(a, (nL,)), np.broadcast_to(b_, (nL,))]).astype(np.float32)
-    spont = spec.spont
-    K = spec.sharpness
-    c50 = spec.c50
-    e_kc50 = math.exp(K * c50)
-    e_kc50_k = math.exp(K * c50 - K)
-    logA = (1 - spont) * (1 + e_kc50) * (1 + e_kc50_k) / (e_kc50 - e_kc50_k)   # src/LGN_props.h:163-169
-    logB = 1 - logA / (1 + e_kc50_k)
+    spont = np.full(nL, spec.spont, np.float32)
+    c50 = np.full(nL, spec.c50, np.float32)
+    K = np.full(nL, spec.sharpness, np.float32)
+    logA, logB = logistic_ab(spont, c50, K)
     lgn = dict(
         polar=two(polar, polar), ecc=two(ecc, ecc), rw=two(rc, rs), rh=two(rc, rs),
         orient=two(0.0, 0.0), k=two(kC, kS),
@@ -363,8 +363,7 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
         nR=two(spec.nR, spec.nR), nD=two(spec.nD, spec.nD),
         covariant=np.ones(nL, np.float32),
         coneType=np.full((2, nL), 3, dtype=np.uint32),            # (L+M)/2 both regions
-        c50=np.full(nL, c50, np.float32), sharpness=np.full(nL, K, np.float32),
-        logA=np.full(nL, logA, np.float32), logB=np.full(nL, logB, np.float32),
+        c50=c50, sharpness=K, spont=spont, logA=logA, logB=logB,
         xdeg=lx.astype(np.float32), ydeg=ly.astype(np.float32), on=on,
     )
 
@@ -396,6 +395,17 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
     if with_frames:
         net.frames = grating_frames(spec)
     return net
+
+
+def logistic_ab(spont, c50, K):
+    """The precached logistic coefficients a, b of src/LGN_props.h:163-169
+    (f(0) = spont, f(1) = 1), in float64 from the float32 parameters."""
+    spont, c50, K = (np.asarray(x, np.float32).astype(np.float64) for x in (spont, c50, K))
+    e1 = np.exp(K * c50)
+    e2 = np.exp(K * c50 - K)
+    a = (1 - spont) * (1 + e1) * (1 + e2) / (e1 - e2)
+    b = 1 - a / (1 + e2)
+    return a.astype(np.float32), b.astype(np.float32)
 
 
 def n_frames(spec: ModelSpec) -> int:
@@ -463,7 +473,7 @@ def view_geometry(spec: ModelSpec):
     """Flat retina (src/patch.cu:1630-1640): x0 = y0 = 0.5 and the scale that
     maps radians to normalised texture coordinates."""
     stim_range = spec.stim_deg / 2.0
-    extent = 2.0 * (stim_range + 0.0)
+    extent = 2.0 * (stim_range + spec.stim_buffer_deg)
     max_ecc = stim_range
     normEcc = max_ecc / extent
     nvd = normEcc / max_ecc / DEG2RAD

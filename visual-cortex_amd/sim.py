@@ -240,6 +240,76 @@ class GpuSim:
                 e.record(sv)
                 self.v1_done[buf] = e
 
+    def run_with_outputs(self, nsteps: int, paths: dict, outputs: dict):
+        """Run `nsteps` steps one launch at a time and write the reference's
+        output files: rawData (sInfo, dep, w, v, gFF, hFF, gE, gI, hE, hI, gap
+        per step, src/patch.cu:4688-4716, 6139-6164, src/stats.cu:38-97),
+        LGN_fr and LGN_sp.  The cortical g/h of step t are known after the
+        recurrent increments of its spikes, i.e. at step t+1 (or finalize)."""
+        from . import formats as F
+        sp = self.net.spec
+        nV1, npf = sp.nV1, self.tm["ntPerFrame"]
+        ngFF, ngE, ngI = self.p.ngFF, self.p.ngE, self.p.ngI
+        mI = sp.nblock * sp.nI
+        hW = bool(outputs.get("hWrite", True))
+        raw = F.RawDataWriter(paths["rawData"], sp.dt, nsteps, nV1, sp.iModel, mI, hW, ngFF, ngE,
+                              ngI) if outputs.get("rawData") else None
+        fr_w = F.StepWriter(paths["lgn_fr"], nsteps, sp.nLGN) if outputs.get("lgn_fr") else None
+        sp_w = F.StepWriter(paths["lgn_sp"]) if outputs.get("lgn_sp") else None
+        dev = self.dev
+        sInfo = torch.empty(nV1, dtype=torch.float32, device=dev)
+        dep = torch.empty(nV1, dtype=torch.float32, device=dev)
+        gprev = torch.empty(2 * (ngE + ngI), nV1, dtype=torch.float32, device=dev)
+        held = None
+        st = self.state
+
+        def g_parts(g):
+            return g[:ngE], g[ngE:ngE + ngI], g[ngE + ngI:2 * ngE + ngI], g[2 * ngE + ngI:]
+
+        for i in range(nsteps):
+            it = self.it
+            k, k_in = divmod(it, npf)
+            while self.lgn_next <= k + 1:
+                self._lgn_period(self.lgn_next)
+                self.lgn_next += 1
+            buf = k % 2
+            self.s_v1.wait_event(self.lgn_done[buf])
+            if k_in == 0:
+                n_here = min(npf, nsteps - i)
+                if fr_w:
+                    fr_w.write(self.lgn_fr2[buf][:n_here].cpu().numpy())
+                if sp_w:
+                    sp_w.write(self.lgn_sInfo2[buf][:n_here].cpu().numpy())
+            nxt = 1 - self.cur
+            mptr = ptr(self.ffMask2[buf]) + k_in * nV1 * 4 if self.ev is not None else None
+            check(self.lib.vc_v1_step(self._hs(self.s_v1), C.byref(self.p), C.byref(self.s),
+                                      C.byref(self.ff), C.byref(self.rec),
+                                      C.byref(self.spk[self.cur]), C.byref(self.spk[nxt]),
+                                      ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, mptr, it,
+                                      ptr(sInfo), ptr(dep), ptr(gprev)), "vc_v1_step")
+            self.cur = nxt
+            self.it += 1
+            if raw:
+                if held is not None:
+                    gE, gI, hE, hI = (x.cpu().numpy() for x in g_parts(gprev))
+                    raw.write_step(*held, gE, gI, hE, hI)
+                held = (sInfo.cpu().numpy(), dep.cpu().numpy(), st["w"].cpu().numpy(),
+                        st["v"].cpu().numpy(), st["gFF"][:ngFF].cpu().numpy(),
+                        st["hFF"][:ngFF].cpu().numpy())
+            if self.it % npf == 0:
+                e = torch.cuda.Event()
+                e.record(self.s_v1)
+                self.v1_done[buf] = e
+        if raw:
+            self.finalize()
+            torch.cuda.synchronize()
+            raw.write_step(*held, st["gE"].cpu().numpy(), st["gI"].cpu().numpy(),
+                           st["hE"].cpu().numpy(), st["hI"].cpu().numpy())
+            raw.close()
+        for w in (fr_w, sp_w):
+            if w:
+                w.close()
+
     def finalize(self):
         """Apply the last step's pending recurrent increments (the reference
         applies them inside the same step, in recal_G)."""
