@@ -60,7 +60,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=640)
     ap.add_argument("--cpu-steps", type=int, default=800,
                     help="steps of the CPU-baseline sample (0 = skip)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--trials", type=int, default=32, help="c3: number of trials")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (x4) for HBM counter calibration")
     args = ap.parse_args()
@@ -76,6 +77,8 @@ def main():
     from visual_cortex_amd import network
     from visual_cortex_amd.sim import GpuSim
 
+    if args.config == "c3":
+        return bench_batched(args)
     spec = network.spec_c2() if args.config == "c2" else network.spec_c1()
     K, W = args.steps, args.warmup
     spec = network.replace(spec, nt=K + W) if hasattr(network, "replace") else spec
@@ -152,6 +155,62 @@ def main():
                      "kernel": "k_v1_step", "bytes_per_launch": bytes_per_launch,
                      "launch_us": v1_launch_s * 1e6},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_batched(args):
+    """Batched trials: `--trials` independent trials (8-96 blocks, 8
+    orientations, distinct seeds, the small plain-forward distributions),
+    500 ms each, in one launch per step, against the same trials run one
+    after the other (trials/s, and the ratio)."""
+    load_package()
+    import torch
+    from visual_cortex_amd import batch
+    from visual_cortex_amd.sim import GpuSim
+    K, W = args.steps, args.warmup
+    specs = batch.c3_specs(ntrial=args.trials, nt=K + W)
+    t0 = time.time()
+    nets = [batch.build_network_fast(s, device="cuda") for s in specs]
+    bnet = batch.batch_networks(nets)
+    log(f"[bench] {len(nets)} trials, blocks {[s.nblock for s in specs]}, nV1={bnet.spec.nV1}, "
+        f"built in {time.time() - t0:.1f}s")
+    sim = GpuSim(bnet, record=False)
+    sim.run(W)
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    sim.run(K, events=events)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    launch = sum(a.elapsed_time(b) for a, b, n in events) / 1e3 / sum(n for a, b, n in events)
+    log(f"[bench] batched: {K} steps in {tb:.3f}s ({tb / K * 1e6:.1f} us/step, V1 launch {launch * 1e6:.1f} us)")
+    del sim
+    ts = 0.0
+    for n in nets:
+        s1 = GpuSim(n, record=False)
+        s1.run(W)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s1.run(K)
+        torch.cuda.synchronize()
+        ts += time.perf_counter() - t0
+        del s1
+    log(f"[bench] serial: {len(nets)} trials x {K} steps in {ts:.3f}s")
+    sp = bnet.spec
+    line = {
+        "metric": "batched trials/s",
+        "value": len(nets) * (K * sp.dt / 500.0) / tb,
+        "unit": "trials/s (500 ms trials)",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": tb * 1e3 / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded networks, drifting gratings at 8 orientations)",
+        "config": {"workload": f"{len(nets)} batched trials, 8-96 blocks x 1024 neurons, "
+                               f"400 E + 100 I partners, 20 LGN inputs, dt={sp.dt} ms",
+                   "nV1_total": sp.nV1, "blocks": [s.nblock for s in specs]},
+        "serial_trials_per_s": len(nets) * (K * sp.dt / 500.0) / ts,
+        "speedup_vs_serial": ts / tb,
+        "neuron_steps_per_s": sp.nV1 * K / tb,
     }
     print(json.dumps(line), flush=True)
 

@@ -74,6 +74,14 @@ typedef struct {
     float bgStrength[VC_MAX_NTYPE], bgLambda[VC_MAX_NTYPE], bgP0[VC_MAX_NTYPE];
     float pBG[VC_MAX_NTYPE * VC_MAX_NG];
     uint32_t seed_lo, seed_hi, trial;         /* Philox key and trial word       */
+    /* Batched trials (NULL = one trial): blocks of several independent
+     * trials are concatenated; blockTrial[b] is the trial of block b,
+     * trialV1Off[t] its first neuron, trialSeed[2t..2t+1] its Philox key.
+     * A trial's draws use its own key and local neuron ids, so a trial gives
+     * the same result batched as alone. */
+    const uint32_t *blockTrial;
+    const uint32_t *trialV1Off;
+    const uint32_t *trialSeed;
 } vc_v1_params;
 
 /* Persistent V1 state, all [..][nV1] float32, receptor-major. */
@@ -196,17 +204,25 @@ int vc_lgn_precompute(void *stream, const vc_lgn_setup *q, float *TW, float *SW,
 /* Spatial response of every cell to one frame: R[nLGN][2] = sum over samples
  * of SW * clip(I/mean - 1, +-1), with the mean taken over the surround
  * samples (src/discrete_input_convol.cu:1055-1069, 1143-1211).  frame is
- * [3][height][width] (L, M, S) float32; coneType is [2][nLGN]. */
+ * [3][height][width] (L, M, S) float32; coneType is [2][nLGN].  With batched
+ * trials (cellTrial != NULL) cell c reads frame + cellTrial[c] *
+ * trialFrameStride (floats), its trial's copy of the frame. */
 int vc_lgn_frame_response(void *stream, const float *frame, uint32_t width,
                           uint32_t height, uint32_t nLGN, uint32_t nSample1D,
                           const float *SCx, const float *SCy, const float *SW,
-                          const uint32_t *coneType, float *R);
+                          const uint32_t *coneType, const uint32_t *cellTrial,
+                          uint64_t trialFrameStride, float *R);
 
 typedef struct {
     uint32_t nLGN, nKernelSample, kernelSampleInterval, iKernelSampleT0;
     uint32_t nRetrace, ntPerFrame, maxFrame;
     float dt, convolRatio, frRatio, tRef;
     uint32_t seed_lo, seed_hi, trial;
+    /* Batched trials (NULL = one trial): cellTrial[c], trialLgnOff[t] first
+     * cell of trial t, trialSeed[2t..2t+1] its Philox key. */
+    const uint32_t *cellTrial;
+    const uint32_t *trialLgnOff;
+    const uint32_t *trialSeed;
 } vc_lgn_params;
 
 typedef struct {

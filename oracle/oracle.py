@@ -34,7 +34,8 @@ class Oracle:
         L.orc_v1_step.argtypes = [P(abi_mod.V1Params)] + [vp] * 14 + [u32, vp, vp, vp, vp, vp, vp, vp,
                                                                       u32, vp, vp, i32]
         L.orc_v1_step.restype = i32
-        L.orc_lgn_frame_response.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp]
+        L.orc_lgn_frame_response.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp,
+                                             C.c_uint64, vp]
         L.orc_lgn_steps.argtypes = [P(abi_mod.LgnParams)] + [vp] * 11 + [u32, u32, vp, vp, vp]
         L.orc_lgn_init.argtypes = [P(abi_mod.LgnParams), vp, vp]
         L.orc_lgn_precompute.argtypes = [P(abi_mod.LgnSetup), vp, vp, vp, vp, vp]
@@ -119,6 +120,17 @@ class OracleSim:
             tables = self.precompute()
         self.tables = {k: np.ascontiguousarray(v, dtype=np.float32) for k, v in tables.items()}
         self.lp = params.lgn_params_struct(net)
+        self.batch = getattr(net, "batch", None)
+        if self.batch is not None:                  # host copies of the trial tables
+            B = self.batch
+            self.bt = {k: np.ascontiguousarray(getattr(B, k), dtype=np.uint32)
+                       for k in ("blockTrial", "trialV1Off", "cellTrial", "trialLgnOff", "trialSeed")}
+            self.p.blockTrial = self.bt["blockTrial"].ctypes.data
+            self.p.trialV1Off = self.bt["trialV1Off"].ctypes.data
+            self.p.trialSeed = self.bt["trialSeed"].ctypes.data
+            self.lp.cellTrial = self.bt["cellTrial"].ctypes.data
+            self.lp.trialLgnOff = self.bt["trialLgnOff"].ctypes.data
+            self.lp.trialSeed = self.bt["trialSeed"].ctypes.data
         self.lTR = np.zeros(sp.nLGN, np.float32)
         self.lNL = np.zeros(sp.nLGN, np.float32)
         self.o.L.orc_lgn_init(C.byref(self.lp), _fp(self.lTR), _fp(self.lNL))
@@ -161,14 +173,17 @@ class OracleSim:
         self.o.L.orc_lgn_precompute(C.byref(q), _fp(TW), _fp(SW), _fp(SCx), _fp(SCy), _fp(mc))
         return dict(TW=TW, SW=SW, SCx=SCx, SCy=SCy, maxConvol=mc)
 
-    def frame_response(self, frame):
+    def frame_response(self, frame, per_trial: bool = False):
+        """frame: [3][H][W], or with per_trial [ntrial][3][H][W] (batched)."""
         sp = self.net.spec
         frame = np.ascontiguousarray(frame, dtype=np.float32)
         R = np.zeros((sp.nLGN, 2), np.float32)
         t = self.tables
+        ct = _fp(self.bt["cellTrial"]) if per_trial else None
+        stride = int(frame[0].size) if per_trial else 0
         self.o.L.orc_lgn_frame_response(_fp(frame), sp.width, sp.height, sp.nLGN,
                                         sp.nSpatialSample1D, _fp(t["SCx"]), _fp(t["SCy"]),
-                                        _fp(t["SW"]), _fp(self.lgn["coneType"]), _fp(R))
+                                        _fp(t["SW"]), _fp(self.lgn["coneType"]), ct, stride, _fp(R))
         return R
 
     def run(self, nsteps: int, record: bool = True):
@@ -184,7 +199,12 @@ class OracleSim:
             k_in = it % npf
             if k_in == 0:
                 fidx = it // npf
-                self.Rring[fidx % tm["maxFrame"]] = self.frame_response(frames[fidx % frames.shape[0]])
+                if self.batch is not None:                   # frames [ntrial][nF][3][H][W]
+                    self.Rring[fidx % tm["maxFrame"]] = self.frame_response(
+                        frames[:, fidx % frames.shape[1]], per_trial=True)
+                else:
+                    self.Rring[fidx % tm["maxFrame"]] = self.frame_response(
+                        frames[fidx % frames.shape[0]])
                 self.o.L.orc_lgn_steps(C.byref(self.lp), _fp(t["TW"]), _fp(kC), _fp(kS),
                                        _fp(t["maxConvol"]), _fp(self.lgn["c50"]),
                                        _fp(self.lgn["sharpness"]), _fp(self.lgn["logA"]),

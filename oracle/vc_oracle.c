@@ -169,8 +169,14 @@ static inline int orc_type_of(const vc_v1_params *p, uint32_t local) {
  * Poisson count (inversion) and the event time. */
 static inline uint32_t orc_bg_draw(const vc_v1_params *p, int t, uint32_t n,
                                    uint32_t step, float *frac) {
-    vc_u32x4 r = vc_draw4(p->seed_lo, p->seed_hi, n, step, VC_STREAM_V1_BG, 0,
-                          p->trial);
+    uint32_t klo = p->seed_lo, khi = p->seed_hi, ent = n;
+    if (p->blockTrial) {                 /* batched trials: the trial's key, local id */
+        uint32_t tr = p->blockTrial[n / p->blockSize];
+        klo = p->trialSeed[2 * tr];
+        khi = p->trialSeed[2 * tr + 1];
+        ent = n - p->trialV1Off[tr];
+    }
+    vc_u32x4 r = vc_draw4(klo, khi, ent, step, VC_STREAM_V1_BG, 0, p->trial);
     float u = vc_u01_open0(r.v[0]);
     *frac = vc_u01_open1(r.v[1]);
     return vc_poisson_inv(u, p->bgLambda[t], p->bgP0[t]);
@@ -483,13 +489,15 @@ static inline float orc_clip_contrast(float I, float mean) {
     return c;
 }
 
-void orc_lgn_frame_response(const float *frame, uint32_t W, uint32_t H,
+void orc_lgn_frame_response(const float *frame0, uint32_t W, uint32_t H,
                             uint32_t nLGN, uint32_t nSample1D, const float *SCx,
                             const float *SCy, const float *SW,
-                            const uint32_t *coneType, float *R) {
+                            const uint32_t *coneType, const uint32_t *cellTrial,
+                            uint64_t trialFrameStride, float *R) {
     const int nS = (int)(nSample1D * nSample1D);
 #pragma omp parallel for schedule(static)
     for (uint32_t c = 0; c < nLGN; ++c) {
+        const float *frame = cellTrial ? frame0 + (size_t)cellTrial[c] * trialFrameStride : frame0;
         float *a = (float *)malloc(sizeof(float) * nS * 4);
         float *bS = a + nS, *fC = a + 2 * nS, *fS = a + 3 * nS;
         uint32_t tC = coneType[c], tS = coneType[nLGN + c];
@@ -575,10 +583,16 @@ static float orc_lgn_cell_step(const vc_lgn_params *p, const float *TW,
         uint32_t k = 0;
         vc_u32x4 pack;
         float tr = p->tRef * rate;
+        uint32_t klo = p->seed_lo, khi = p->seed_hi, ent = c;
+        if (p->cellTrial) {
+            uint32_t t = p->cellTrial[c];
+            klo = p->trialSeed[2 * t];
+            khi = p->trialSeed[2 * t + 1];
+            ent = c - p->trialLgnOff[t];
+        }
         do {
             if ((k & 3u) == 0u)
-                pack = vc_draw4(p->seed_lo, p->seed_hi, c, it, VC_STREAM_LGN_SPIKE,
-                                k >> 2, p->trial);
+                pack = vc_draw4(klo, khi, ent, it, VC_STREAM_LGN_SPIKE, k >> 2, p->trial);
             float u = vc_u01_open0(pack.v[k & 3u]);
             k++;
             float x = -vc_logf(u);
@@ -620,8 +634,14 @@ void orc_lgn_steps(const vc_lgn_params *p, const float *TW, const float *kC,
 void orc_lgn_init(const vc_lgn_params *p, float *leftTimeRate,
                   float *lastNegLogRand) {
     for (uint32_t c = 0; c < p->nLGN; ++c) {
-        vc_u32x4 r = vc_draw4(p->seed_lo, p->seed_hi, c, 0xffffffffu,
-                              VC_STREAM_LGN_INIT, 0, p->trial);
+        uint32_t klo = p->seed_lo, khi = p->seed_hi, ent = c;
+        if (p->cellTrial) {
+            uint32_t t = p->cellTrial[c];
+            klo = p->trialSeed[2 * t];
+            khi = p->trialSeed[2 * t + 1];
+            ent = c - p->trialLgnOff[t];
+        }
+        vc_u32x4 r = vc_draw4(klo, khi, ent, 0xffffffffu, VC_STREAM_LGN_INIT, 0, p->trial);
         leftTimeRate[c] = 0.0f;
         lastNegLogRand[c] = -vc_logf(vc_u01_open0(r.v[0]));
     }

@@ -49,6 +49,7 @@ class V1Params(C.Structure):
         ("bgP0", _arr(f32, VC_MAX_NTYPE)),
         ("pBG", _arr(f32, VC_MAX_NTYPE * VC_MAX_NG)),
         ("seed_lo", u32), ("seed_hi", u32), ("trial", u32),
+        ("blockTrial", vp), ("trialV1Off", vp), ("trialSeed", vp),
     ]
 
 
@@ -91,6 +92,7 @@ class LgnParams(C.Structure):
         ("iKernelSampleT0", u32), ("nRetrace", u32), ("ntPerFrame", u32), ("maxFrame", u32),
         ("dt", f32), ("convolRatio", f32), ("frRatio", f32), ("tRef", f32),
         ("seed_lo", u32), ("seed_hi", u32), ("trial", u32),
+        ("cellTrial", vp), ("trialLgnOff", vp), ("trialSeed", vp),
     ]
 
 
@@ -112,7 +114,7 @@ SIGNATURES = {
                         C.POINTER(Spikes)],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],
     "vc_lgn_precompute": [vp, C.POINTER(LgnSetup), vp, vp, vp, vp, vp],
-    "vc_lgn_frame_response": [vp, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp],
+    "vc_lgn_frame_response": [vp, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, C.c_uint64, vp],
     "vc_lgn_steps": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp, vp,
                      C.POINTER(FFEvents)],
     "vc_lgn_init": [vp, C.POINTER(LgnParams), vp, vp],

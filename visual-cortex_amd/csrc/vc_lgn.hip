@@ -90,12 +90,14 @@ __device__ __forceinline__ float clip_contrast(float I, float mean) {
 }
 
 __global__ __launch_bounds__(1024) void k_lgn_frame_response(
-    const float *__restrict__ frame, int W, int H, uint32_t nLGN,
+    const float *__restrict__ frame0, int W, int H, uint32_t nLGN,
     const float *__restrict__ SCx, const float *__restrict__ SCy,
     const float *__restrict__ SW, const uint32_t *__restrict__ coneType,
+    const uint32_t *__restrict__ cellTrial, uint64_t trialFrameStride,
     float *__restrict__ R) {
     __shared__ float part[32];
     const uint32_t c = blockIdx.x;
+    const float *frame = cellTrial ? frame0 + (size_t)cellTrial[c] * trialFrameStride : frame0;
     const uint32_t s = threadIdx.x;
     const uint32_t nS = blockDim.x;
     const uint32_t tC = coneType[c], tS = coneType[nLGN + c];
@@ -219,6 +221,13 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
     const uint32_t cbase = c - lane;           // cell of lane 0 of this wavefront
     float lTR = valid ? st.leftTimeRate[c] : 0.0f;
     float lNL = valid ? st.lastNegLogRand[c] : 1.0f;
+    uint32_t klo = p.seed_lo, khi = p.seed_hi, ent = c;     // Philox key, entity id
+    if (valid && p.cellTrial != nullptr) {
+        const uint32_t tr = p.cellTrial[c];
+        klo = p.trialSeed[2 * tr];
+        khi = p.trialSeed[2 * tr + 1];
+        ent = c - p.trialLgnOff[tr];
+    }
     float frn = valid ? frIn[c] : 0.0f;
     for (uint32_t s = 0; s < nsteps; ++s) {
         const uint32_t it = it0 + s;
@@ -241,8 +250,8 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
                 const float tr = p.tRef * rate;
                 do {
                     if ((kdraw & 3u) == 0u)
-                        pack = vc_draw4(p.seed_lo, p.seed_hi, c, it, VC_STREAM_LGN_SPIKE,
-                                        kdraw >> 2, p.trial);
+                        pack = vc_draw4(klo, khi, ent, it, VC_STREAM_LGN_SPIKE, kdraw >> 2,
+                                        p.trial);
                     const float u = vc_u01_open0(pack.v[kdraw & 3u]);
                     kdraw++;
                     const float x = -vc_logf(u);
@@ -284,7 +293,14 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
 __global__ void k_lgn_init(vc_lgn_params p, float *lTR, float *lNL) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= p.nLGN) return;
-    vc_u32x4 r = vc_draw4(p.seed_lo, p.seed_hi, c, 0xffffffffu, VC_STREAM_LGN_INIT, 0u, p.trial);
+    uint32_t klo = p.seed_lo, khi = p.seed_hi, ent = c;
+    if (p.cellTrial != nullptr) {
+        const uint32_t tr = p.cellTrial[c];
+        klo = p.trialSeed[2 * tr];
+        khi = p.trialSeed[2 * tr + 1];
+        ent = c - p.trialLgnOff[tr];
+    }
+    vc_u32x4 r = vc_draw4(klo, khi, ent, 0xffffffffu, VC_STREAM_LGN_INIT, 0u, p.trial);
     lTR[c] = 0.0f;
     lNL[c] = -vc_logf(vc_u01_open0(r.v[0]));
 }
@@ -490,7 +506,8 @@ extern "C" int vc_lgn_precompute(void *stream, const vc_lgn_setup *q, float *TW,
 extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t width,
                                      uint32_t height, uint32_t nLGN, uint32_t nSample1D,
                                      const float *SCx, const float *SCy, const float *SW,
-                                     const uint32_t *coneType, float *R) {
+                                     const uint32_t *coneType, const uint32_t *cellTrial,
+                                     uint64_t trialFrameStride, float *R) {
     const uint32_t nS = nSample1D * nSample1D;
     if (!frame || !SCx || !SCy || !SW || !coneType || !R) {
         vc_set_error("vc_lgn_frame_response: null argument");
@@ -505,7 +522,8 @@ extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t 
         return VC_E_INVALID;
     }
     hipLaunchKernelGGL(k_lgn_frame_response, dim3(nLGN), dim3(nS), 0, (hipStream_t)stream,
-                       frame, (int)width, (int)height, nLGN, SCx, SCy, SW, coneType, R);
+                       frame, (int)width, (int)height, nLGN, SCx, SCy, SW, coneType, cellTrial,
+                       trialFrameStride, R);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

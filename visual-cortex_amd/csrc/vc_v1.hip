@@ -233,7 +233,7 @@ __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_c
     }
 }
 
-template <bool kUpdate>
+template <bool kUpdate, bool kEvents>
 __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v1_state s,
                                                  vc_ff_conn ff, vc_rec_conn rec,
                                                  vc_spikes prev, vc_spikes next,
@@ -244,8 +244,8 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
                                                  float *__restrict__ out_g_prev) {
     __shared__ uint32_t sbrow[kWG / 64][2 * kBatch];
     __shared__ uint32_t sbslot[kWG / 64][2 * kBatch];
-    __shared__ uint64_t smk[kWG / 64][kMaxEnt];
-    __shared__ uint32_t sbid[kWG / 64][kMaxEnt];
+    __shared__ uint64_t smk[kMaxEnt];          // neighbour-tile masks of this block (all
+    __shared__ uint32_t sbid[kMaxEnt];         // wavefronts of a workgroup share the block)
     const uint32_t nV1 = p.nV1;
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t b = n / p.blockSize;
@@ -260,19 +260,20 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     // ---- round 1: neighbour spike masks (lane-parallel), LGN synapses, state
     const uint32_t nb = rec.nNear[b];
     const uint32_t nEnt = nb * nSeg;
-    for (uint32_t e = lane; e < nEnt; e += 64u) {
+    for (uint32_t e = threadIdx.x; e < nEnt; e += blockDim.x) {
         const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
         const uint32_t bid = rec.nbr[b * rec.maxNear + ib];
-        smk[wv][e] = prev.mask[bid * nSeg + tt];
-        sbid[wv][e] = bid;
+        smk[e] = prev.mask[bid * nSeg + tt];
+        sbid[e] = bid;
     }
-    const bool events = kUpdate && ffMask != nullptr;
+    constexpr bool events = kUpdate && kEvents;
+    constexpr int nFF = (kUpdate && !kEvents) ? kFF : 1;    // ELL synapses held in registers
     const uint32_t mLGN = (kUpdate && !events) ? ff.nLGN[n] : 0u;
     const uint32_t ffBits = events ? ffMask[n] : 0u;
-    float fv[kFF];
-    uint32_t fid[kFF];
+    float fv[nFF];
+    uint32_t fid[nFF];
 #pragma unroll
-    for (int j = 0; j < kFF; ++j) {
+    for (int j = 0; j < nFF; ++j) {
         fv[j] = 0.0f;
         fid[j] = 0u;
         if ((uint32_t)j < mLGN) {
@@ -296,17 +297,18 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     }
 
     // ---- round 2: LGN spike words, then the recurrent gather
-    float sv[kFF];
+    float sv[nFF];
 #pragma unroll
-    for (int j = 0; j < kFF; ++j) sv[j] = ((uint32_t)j < mLGN) ? lgnSInfo[fid[j]] : 0.0f;
+    for (int j = 0; j < nFF; ++j) sv[j] = ((uint32_t)j < mLGN) ? lgnSInfo[fid[j]] : 0.0f;
     // event path: the first kEv synapses that received a spike this step
-    uint32_t evj[kEv];
-    float evf[kEv];
-    uint32_t evid[kEv];
+    constexpr int nEv = events ? kEv : 1;
+    uint32_t evj[nEv];
+    float evf[nEv];
+    uint32_t evid[nEv];
     {
         uint32_t bits = ffBits;
 #pragma unroll
-        for (int k = 0; k < kEv; ++k) {
+        for (int k = 0; k < nEv; ++k) {
             evj[k] = 0xffffffffu;
             evf[k] = 0.0f;
             evid[k] = 0u;
@@ -323,8 +325,8 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
 
     float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
     float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
-    __builtin_amdgcn_wave_barrier();
-    rec_gather(p, rec, prev, b, myseg, nSeg, nEnt, t, smk[wv], sbid[wv], sbrow[wv], sbslot[wv],
+    __syncthreads();
+    rec_gather(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, sbrow[wv], sbslot[wv],
                lgE, lhE, lgI, lhI);
 
     // conductances at the end of the previous step (reference: after recal_G)
@@ -436,11 +438,11 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     shFF.ng = shBG.ng = p.ngFF;
     if (events && __ballot(ffBits != 0u) != 0ull) {
         // synapses whose LGN cell spiked, in synapse order
-        float evs[kEv];
+        float evs[nEv];
 #pragma unroll
-        for (int k = 0; k < kEv; ++k) evs[k] = (evj[k] != 0xffffffffu) ? lgnSInfo[evid[k]] : 0.0f;
+        for (int k = 0; k < nEv; ++k) evs[k] = (evj[k] != 0xffffffffu) ? lgnSInfo[evid[k]] : 0.0f;
 #pragma unroll
-        for (int k = 0; k < kEv; ++k) {
+        for (int k = 0; k < nEv; ++k) {
             if (evj[k] != 0xffffffffu) {
                 const float sI = evs[k];
                 const float nsp = (float)(uint32_t)floorf(sI);
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         }
         uint32_t rest = ffBits;
 #pragma unroll
-        for (int k = 0; k < kEv; ++k) rest &= rest - 1u;    // drop the prefetched bits
+        for (int k = 0; k < nEv; ++k) rest &= rest - 1u;    // drop the prefetched bits
         while (rest) {
             const uint32_t j = (uint32_t)__builtin_ctz(rest);
             rest &= rest - 1u;
@@ -460,15 +462,15 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         }
         if (ffBits) ffMask[n] = 0u;
     }
-    {
+    if (!events) {
         uint32_t spk = 0;
 #pragma unroll
-        for (int j = 0; j < kFF; ++j) spk |= (sv[j] >= 1.0f ? 1u : 0u) << j;
+        for (int j = 0; j < nFF; ++j) spk |= (sv[j] >= 1.0f ? 1u : 0u) << j;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) spk |= (uint32_t)__shfl_xor((int)spk, off);
         spk = __builtin_amdgcn_readfirstlane(spk);     // slots that spiked for any lane
 #pragma unroll
-        for (int j = 0; j < kFF; ++j) {
+        for (int j = 0; j < nFF; ++j) {
             if ((spk >> j) & 1u) {
                 const float sI = sv[j];
                 if (sI >= 1.0f) {
@@ -478,7 +480,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
             }
         }
     }
-    for (uint32_t i = kFF; i < mLGN; ++i) {            // lists longer than kFF
+    for (uint32_t i = nFF; i < mLGN; ++i) {            // lists longer than kFF
         const uint32_t lid = i * nV1 + n;
         const float sI = lgnSInfo[ff.idLGN[lid]];
         if (sI >= 1.0f) {
@@ -488,7 +490,14 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     }
     // background Poisson drive (extension): one more input after the LGN list
     if (p.bgStrength[t] > 0.0f) {
-        const vc_u32x4 r = vc_draw4(p.seed_lo, p.seed_hi, n, step, VC_STREAM_V1_BG, 0u, p.trial);
+        uint32_t klo = p.seed_lo, khi = p.seed_hi, ent = n;
+        if (p.blockTrial != nullptr) {
+            const uint32_t tr = p.blockTrial[b];
+            klo = p.trialSeed[2 * tr];
+            khi = p.trialSeed[2 * tr + 1];
+            ent = n - p.trialV1Off[tr];
+        }
+        const vc_u32x4 r = vc_draw4(klo, khi, ent, step, VC_STREAM_V1_BG, 0u, p.trial);
         const float u = vc_u01_open0(r.v[0]);
         const float frac = vc_u01_open1(r.v[1]);
         const uint32_t nsp = vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
@@ -700,9 +709,14 @@ extern "C" int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state
     }
     V1Const k = make_const(p);
     uint32_t wg = wg_size(p->blockSize);
-    hipLaunchKernelGGL(k_v1_step<true>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
-                       *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask, step, out_sInfo,
-                       out_dep, out_g_prev);
+    if (ffMask)
+        hipLaunchKernelGGL((k_v1_step<true, true>), dim3(p->nV1 / wg), dim3(wg), 0,
+                           (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
+                           step, out_sInfo, out_dep, out_g_prev);
+    else
+        hipLaunchKernelGGL((k_v1_step<true, false>), dim3(p->nV1 / wg), dim3(wg), 0,
+                           (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
+                           step, out_sInfo, out_dep, out_g_prev);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
@@ -730,10 +744,16 @@ extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state 
     const uint32_t wg = wg_size(p->blockSize);
     for (uint32_t i = 0; i < nsteps; ++i) {
         const uint32_t nx = 1u - cur;
-        hipLaunchKernelGGL(k_v1_step<true>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
-                           *p, k, *s, *ff, *rec, spk2[cur], spk2[nx], lgnSInfo + i * lgnStride,
-                           ffMask ? ffMask + i * maskStride : nullptr, it0 + i,
-                           out_sInfo + i * outStride, nullptr, nullptr);
+        if (ffMask)
+            hipLaunchKernelGGL((k_v1_step<true, true>), dim3(p->nV1 / wg), dim3(wg), 0,
+                               (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur], spk2[nx],
+                               lgnSInfo + i * lgnStride, ffMask + i * maskStride, it0 + i,
+                               out_sInfo + i * outStride, nullptr, nullptr);
+        else
+            hipLaunchKernelGGL((k_v1_step<true, false>), dim3(p->nV1 / wg), dim3(wg), 0,
+                               (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur], spk2[nx],
+                               lgnSInfo + i * lgnStride, nullptr, it0 + i,
+                               out_sInfo + i * outStride, nullptr, nullptr);
         cur = nx;
     }
     VC_HIP_CHECK(hipGetLastError());
@@ -752,7 +772,7 @@ extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_
     uint32_t wg = wg_size(p->blockSize);
     vc_ff_conn ff{};
     vc_spikes nx{};
-    hipLaunchKernelGGL(k_v1_step<false>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL((k_v1_step<false, false>), dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
                        *p, k, *s, ff, *rec, *prev, nx, nullptr, nullptr, 0u, nullptr, nullptr,
                        nullptr);
     VC_HIP_CHECK(hipGetLastError());

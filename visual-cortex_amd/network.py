@@ -168,13 +168,40 @@ class Network:
     # LGN parameters, [2, nLGN] (centre, surround)
     lgn: dict = field(default_factory=dict)
     # stimulus
-    frames: np.ndarray | None = None  # [nFrame, 3, H, W] f32
+    frames: np.ndarray | None = None  # [nFrame, 3, H, W] f32 ([ntrial, ...] when batched)
     init_lms: tuple = (0.5, 0.5, 0.5)
+    # recurrent synapses already in masked-CSR form (rowMask, rowOff, w), as
+    # produced by the device generator; used instead of rec_pairs when set
+    csr: tuple | None = None
+    batch: object | None = None         # batch.BatchInfo for batched trials
+    trial_specs: list | None = None
 
     # ---- recurrent formats -------------------------------------------------
+    def masked_csr_torch(self, device="cuda"):
+        """The masked CSR as torch tensors (int64 rowMask bits, int64 rowOff,
+        float32 w) on `device`."""
+        import torch
+        if self.csr is not None:
+            rm, ro, w = self.csr
+            return (torch.as_tensor(rm, device=device), torch.as_tensor(ro, device=device).to(torch.int64),
+                    torch.as_tensor(w, device=device))
+        rm, ro, w = self.masked_csr()
+        return (torch.as_tensor(rm.view(np.int64), device=device),
+                torch.as_tensor(ro.astype(np.int64), device=device), torch.as_tensor(w, device=device))
+
+    def n_rec_synapses_csr(self) -> int:
+        if self.csr is not None:
+            return int(self.csr[2].numel() if hasattr(self.csr[2], "numel") else self.csr[2].size)
+        return self.n_rec_synapses()
+
     def masked_csr(self):
         """(rowMask u64 [nRow*nSeg], rowOff u32 [nRow*nSeg], w f32 [nnz]) with
         rows r = (b*maxNear + ib)*bs + pre (see include/vc_abi.h)."""
+        if self.csr is not None:
+            rm, ro, w = self.csr
+            to_np = lambda x: x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
+            return (to_np(rm).view(np.uint64), to_np(ro).astype(np.uint32),
+                    to_np(w).astype(np.float32))
         sp = self.spec
         bs = sp.blockSize
         nSeg = bs // 64
@@ -206,6 +233,18 @@ class Network:
         """The reference's V1_conMat layout [nblock, maxNear, bs pre, bs post]."""
         sp = self.spec
         bs = sp.blockSize
+        if self.csr is not None:                      # expand the masked CSR
+            rm, ro, w = self.masked_csr()
+            nSeg = bs // 64
+            M = np.zeros((sp.nblock * self.maxNear * bs, bs), dtype=np.float32)
+            bits = ((rm[:, None] >> np.arange(64, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
+            rank = np.cumsum(bits, axis=1) - 1
+            idx = ro.astype(np.int64)[:, None] + rank
+            seg = np.arange(rm.size)
+            row, sg = seg // nSeg, seg % nSeg
+            r_i, l_i = np.nonzero(bits)
+            M[row[r_i], sg[r_i] * 64 + l_i] = w[idx[r_i, l_i]]
+            return M.reshape(sp.nblock, self.maxNear, bs, bs)
         M = np.zeros((sp.nblock, self.maxNear, bs, bs), dtype=np.float32)
         for b in range(sp.nblock):
             for ib in range(self.maxNear):
@@ -237,18 +276,15 @@ class Network:
 
 # ---------------------------------------------------------------------------
 def _block_grid(nblock: int) -> int:
-    """Number of block columns of the sheet (blocks are laid out row-major on
-    a cols x nblock/cols grid, cols the smallest divisor >= sqrt(nblock))."""
-    cols = int(math.ceil(math.sqrt(nblock) - 1e-9))
-    while nblock % cols:
-        cols += 1
-    return cols
+    """Number of block columns of the sheet: blocks are laid out row-major on
+    a ceil(sqrt(nblock))-wide grid whose last row may be partial."""
+    return max(1, int(math.ceil(math.sqrt(nblock) - 1e-9)))
 
 
 def _neighbours(nblock: int):
     """3x3 neighbourhood of every block, self first then ascending id."""
     side = _block_grid(nblock)
-    rows = nblock // side
+    rows = (nblock + side - 1) // side
     lists = []
     for b in range(nblock):
         bx, by = b % side, b // side
@@ -256,7 +292,7 @@ def _neighbours(nblock: int):
         for oy in (-1, 0, 1):
             for ox in (-1, 0, 1):
                 x, y = bx + ox, by + oy
-                if (ox or oy) and 0 <= x < side and 0 <= y < rows:
+                if (ox or oy) and 0 <= x < side and 0 <= y < rows and y * side + x < nblock:
                     nb.append(y * side + x)
         lists.append([nb[0]] + sorted(nb[1:]))
     maxNear = max(len(l) for l in lists)
@@ -276,6 +312,36 @@ def _weighted_topk(keysrc_w: np.ndarray, k: int, rng: np.random.Generator) -> np
     keys[~(keysrc_w > 0)] = -np.inf
     idx = np.argpartition(-keys, k - 1, axis=1)[:, :k]
     return idx
+
+
+def lgn_parameters(spec: ModelSpec, lx, ly, on) -> dict:
+    """Per-cell LGN parameters [2, nLGN] (centre, surround) of the fLGN set
+    (src/patch.cu:2472-2530) for cells at (lx, ly) degrees, ON where `on`:
+    DoG radii r = sigma*sqrt(2) (the reference's radius parameter), surround
+    gain -kS/kC, the biphasic difference-of-gamma temporal kernel, and the
+    logistic nonlinearity with its precached coefficients."""
+    nL = len(lx)
+    ecc = np.sqrt(lx * lx + ly * ly) * DEG2RAD
+    polar = np.arctan2(ly, lx)
+    rc = spec.sigC_deg * math.sqrt(2) * DEG2RAD
+    rs = spec.sigS_deg * math.sqrt(2) * DEG2RAD
+    kC = np.where(on, 1.0, -1.0)
+    kS = -spec.kS_over_kC * kC
+    two = lambda a, b_: np.stack([np.broadcast_to(a, (nL,)), np.broadcast_to(b_, (nL,))]).astype(np.float32)
+    spont = np.full(nL, spec.spont, np.float32)
+    c50 = np.full(nL, spec.c50, np.float32)
+    K = np.full(nL, spec.sharpness, np.float32)
+    logA, logB = logistic_ab(spont, c50, K)
+    return dict(
+        polar=two(polar, polar), ecc=two(ecc, ecc), rw=two(rc, rs), rh=two(rc, rs),
+        orient=two(0.0, 0.0), k=two(kC, kS),
+        tauR=two(spec.tauR, spec.tauR), tauD=two(spec.tauD, spec.tauD),
+        delay=two(0.0, 0.0), ratio=two(spec.ratio, spec.ratio),
+        nR=two(spec.nR, spec.nR), nD=two(spec.nD, spec.nD),
+        covariant=np.ones(nL, np.float32),
+        coneType=np.full((2, nL), 3, dtype=np.uint32),            # (L+M)/2 both regions
+        c50=c50, sharpness=K, spont=spont, logA=logA, logB=logB,
+    )
 
 
 def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
@@ -344,28 +410,7 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
     ly = (gy + 0.5) * spacing - spec.field_deg / 2 + (rng.random(ns * ns) - 0.5) * 0.6 * spacing
     on = ((gx + gy) % 2 == 0)
     nL = ns * ns
-    ecc = np.sqrt(lx * lx + ly * ly) * DEG2RAD
-    polar = np.arctan2(ly, lx)
-    rc = spec.sigC_deg * math.sqrt(2) * DEG2RAD
-    rs = spec.sigS_deg * math.sqrt(2) * DEG2RAD
-    kC = np.where(on, 1.0, -1.0)
-    kS = -spec.kS_over_kC * kC
-    two = lambda a, b_: np.stack([np.broadcast_to(a, (nL,)), np.broadcast_to(b_, (nL,))]).astype(np.float32)
-    spont = np.full(nL, spec.spont, np.float32)
-    c50 = np.full(nL, spec.c50, np.float32)
-    K = np.full(nL, spec.sharpness, np.float32)
-    logA, logB = logistic_ab(spont, c50, K)
-    lgn = dict(
-        polar=two(polar, polar), ecc=two(ecc, ecc), rw=two(rc, rs), rh=two(rc, rs),
-        orient=two(0.0, 0.0), k=two(kC, kS),
-        tauR=two(spec.tauR, spec.tauR), tauD=two(spec.tauD, spec.tauD),
-        delay=two(0.0, 0.0), ratio=two(spec.ratio, spec.ratio),
-        nR=two(spec.nR, spec.nR), nD=two(spec.nD, spec.nD),
-        covariant=np.ones(nL, np.float32),
-        coneType=np.full((2, nL), 3, dtype=np.uint32),            # (L+M)/2 both regions
-        c50=c50, sharpness=K, spont=spont, logA=logA, logB=logB,
-        xdeg=lx.astype(np.float32), ydeg=ly.astype(np.float32), on=on,
-    )
+    lgn = lgn_parameters(spec, lx, ly, on)
 
     # ---- LGN -> V1 by Gaussian distance in the visual field
     from scipy.spatial import cKDTree

@@ -74,8 +74,17 @@ def lgn_params_struct(net: Network) -> abi.LgnParams:
 
 
 def initial_state(net: Network) -> dict:
-    """Initial V1 state (host arrays): v uniform in [vR, vT), w, tBack, g = 0."""
-    sp = net.spec
+    """Initial V1 state (host arrays): v uniform in [vR, vT), w, tBack, g = 0.
+    A batched network concatenates its trials' own initial states."""
+    if getattr(net, "batch", None) is not None:
+        parts = []
+        for sp_t in net.trial_specs:
+            parts.append(_initial_state_spec(sp_t))
+        return {k: np.concatenate([q[k] for q in parts], axis=-1) for k in parts[0]}
+    return _initial_state_spec(net.spec)
+
+
+def _initial_state_spec(sp) -> dict:
     rng = np.random.default_rng(sp.seed + 7)
     local = np.arange(sp.nV1) % sp.blockSize
     t = (local >= sp.nE).astype(int)

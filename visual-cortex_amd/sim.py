@@ -63,10 +63,19 @@ class GpuSim:
                         for m in self.ffMask2]
             self.ev = self.ev2[0]
         # ---- recurrent synapses (masked CSR)
-        rowMask, rowOff, w = net.masked_csr()
-        self.rowMask = T(rowMask.view(np.int64))
-        self.rowOff = T(rowOff.view(np.int32))
-        self.wrec = T(w, np.float32)
+        rm, ro, w = net.masked_csr_torch(d)
+        self.rowMask = rm.contiguous()
+        self.rowOff = ro.to(torch.int32).contiguous()          # u32 offsets, reinterpreted
+        self.wrec = w.float().contiguous() if w.numel() else torch.zeros(1, device=d)
+        # ---- batched trials: per-trial Philox keys and local ids
+        self.batch = getattr(net, "batch", None)
+        if self.batch is not None:
+            B = self.batch
+            self.bt = {k: T(np.ascontiguousarray(getattr(B, k)).astype(np.uint32).view(np.int32))
+                       for k in ("blockTrial", "trialV1Off", "cellTrial", "trialLgnOff", "trialSeed")}
+            self.p.blockTrial = ptr(self.bt["blockTrial"])
+            self.p.trialV1Off = ptr(self.bt["trialV1Off"])
+            self.p.trialSeed = ptr(self.bt["trialSeed"])
         self.nNear = T(net.nNear.astype(np.uint32).view(np.int32))
         self.nbr = T(net.nbr.astype(np.uint32).view(np.int32))
         self.rec = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
@@ -129,6 +138,10 @@ class GpuSim:
         self.lTR = torch.zeros(sp.nLGN, **f32)
         self.lNL = torch.zeros(sp.nLGN, **f32)
         self.lp = lgn_params_struct(net)
+        if getattr(net, "batch", None) is not None:
+            self.lp.cellTrial = ptr(self.bt["cellTrial"])
+            self.lp.trialLgnOff = ptr(self.bt["trialLgnOff"])
+            self.lp.trialSeed = ptr(self.bt["trialSeed"])
         check(self.lib.vc_lgn_init(None, C.byref(self.lp), ptr(self.lTR), ptr(self.lNL)), "vc_lgn_init")
         self.ls = abi.LgnState(ptr(self.TW), ptr(self.kC), ptr(self.kS), ptr(self.maxConvol),
                                ptr(self.lgn_t["c50"]), ptr(self.lgn_t["sharpness"]),
@@ -136,6 +149,12 @@ class GpuSim:
                                ptr(self.lTR), ptr(self.lNL))
         # stimulus resident in HBM; pre-stimulus frame of the initial luminance
         self.frames = torch.as_tensor(net.frames, device=d)
+        if getattr(net, "batch", None) is not None:      # [ntrial][nF][3][H][W]
+            self.nF = self.frames.shape[1]
+            self.trialFrameStride = int(self.frames[0].numel())
+        else:
+            self.nF = self.frames.shape[0]
+            self.trialFrameStride = 0
         init = np.empty((3, sp.height, sp.width), np.float32)
         for c in range(3):
             init[c] = net.init_lms[c]
@@ -155,7 +174,8 @@ class GpuSim:
         sp = self.net.spec
         check(self.lib.vc_lgn_frame_response(None, ptr(frame), sp.width, sp.height, sp.nLGN,
                                              sp.nSpatialSample1D, ptr(self.SCx), ptr(self.SCy),
-                                             ptr(self.SW), ptr(self.coneType), ptr(self.Rring[slot])),
+                                             ptr(self.SW), ptr(self.coneType), None, 0,
+                                             ptr(self.Rring[slot])),
               "vc_lgn_frame_response")
 
     # ------------------------------------------------------------------ steps
@@ -174,10 +194,16 @@ class GpuSim:
             st.wait_event(self.v1_done[buf])           # V1 of period k-2 released it
         fidx = k
         sp = self.net.spec
-        frame = self.frames[fidx % self.frames.shape[0]]
+        if self.trialFrameStride:
+            frame = self.frames[0, fidx % self.nF]
+            ctr = ptr(self.bt["cellTrial"])
+        else:
+            frame = self.frames[fidx % self.nF]
+            ctr = None
         check(self.lib.vc_lgn_frame_response(self._hs(st), ptr(frame), sp.width, sp.height,
                                              sp.nLGN, sp.nSpatialSample1D, ptr(self.SCx),
                                              ptr(self.SCy), ptr(self.SW), ptr(self.coneType),
+                                             ctr, self.trialFrameStride,
                                              ptr(self.Rring[fidx % tm["maxFrame"]])),
               "vc_lgn_frame_response")
         ev = C.byref(self.ev2[buf]) if self.ev is not None else None
