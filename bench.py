@@ -60,14 +60,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=640)
     ap.add_argument("--cpu-steps", type=int, default=800,
                     help="steps of the CPU-baseline sample (0 = skip)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--reps", type=int, default=5, help="c4: timed forward/backward repetitions")
     ap.add_argument("--trials", type=int, default=32, help="c3: number of trials")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (x4) for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the "
-              "multi-GPU sharding of this path is a design (see DESIGN.md), not built.",
+              "multi-GPU sharding of this path (by V1 blocks, or by trial) is not built.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -79,6 +80,8 @@ def main():
 
     if args.config == "c3":
         return bench_batched(args)
+    if args.config == "c4":
+        return bench_lgn_adjoint(args)
     spec = network.spec_c2() if args.config == "c2" else network.spec_c1()
     K, W = args.steps, args.warmup
     spec = network.replace(spec, nt=K + W) if hasattr(network, "replace") else spec
@@ -211,6 +214,68 @@ def bench_batched(args):
         "serial_trials_per_s": len(nets) * (K * sp.dt / 500.0) / ts,
         "speedup_vs_serial": ts / tb,
         "neuron_steps_per_s": sp.nV1 * K / tb,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_lgn_adjoint(args):
+    """Differentiation of the LGN stage: 16,384 cells, 500 ms (4000 steps,
+    63 frames) of 256x256 white-noise frames (mean 0.5, sigma 0.3), loss =
+    mean squared error of the rates against the rates under a drifting
+    grating.  Times the forward (frame responses + rates) and the backward
+    (rate adjoint + frame adjoint + temporal-parameter adjoint) with HIP
+    events on the stream they run on; reports backward / forward."""
+    load_package()
+    import torch
+    from visual_cortex_amd import network as N
+    from visual_cortex_amd.lgn_grad import LgnStage
+    spec = N.spec_c2()
+    T = 4000
+    t0 = time.time()
+    lgn = N.lgn_layer(spec)
+    st = LgnStage(spec, lgn, T)
+    frames = torch.as_tensor(N.white_noise_frames(spec, st.nF, seed=11), device="cuda")
+    with torch.no_grad():
+        tgt = st.rates(torch.as_tensor(N.grating_frames(spec, st.nF), device="cuda"), st.theta0)
+    theta = st.theta0.clone()
+    log(f"[bench] LGN adjoint: nLGN={spec.nLGN} {spec.width}x{spec.height} T={T} nF={st.nF}, "
+        f"set up in {time.time() - t0:.1f}s")
+    stream = torch.cuda.current_stream()
+
+    def once(timing=None):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        fr, R, th = st.forward(frames, theta)
+        e[1].record(stream)
+        gFr = 2.0 * (fr - tgt) / fr.numel()
+        dF, dTh = st.backward(frames, R, th, gFr)
+        e[2].record(stream)
+        if timing is not None:
+            timing.append(e)
+        return dF, dTh
+
+    once()
+    torch.cuda.synchronize()
+    tim = []
+    t0 = time.perf_counter()
+    for _ in range(args.reps):
+        once(tim)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.reps
+    fwd = sum(a.elapsed_time(b) for a, b, c in tim) / len(tim) / 1e3
+    bwd = sum(b.elapsed_time(c) for a, b, c in tim) / len(tim) / 1e3
+    log(f"[bench] forward {fwd * 1e3:.2f} ms, backward {bwd * 1e3:.2f} ms, wall/rep {wall * 1e3:.2f} ms")
+    line = {
+        "metric": "LGN adjoint/fwd time",
+        "value": bwd / fwd,
+        "unit": "backward time / forward time",
+        "n_gpus": 1, "steps": T, "warmup": 1, "ms_per_step": (fwd + bwd) * 1e3 / T,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (white-noise frames sigma 0.3, grating-driven target rates)",
+        "config": {"workload": f"LGN stage differentiation: {spec.nLGN} cells, {st.nF} frames of "
+                               f"{spec.width}x{spec.height} px, {T} steps, 6 parameters per cell",
+                   "reps": args.reps},
+        "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
     }
     print(json.dumps(line), flush=True)
 

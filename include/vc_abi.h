@@ -213,6 +213,17 @@ int vc_lgn_frame_response(void *stream, const float *frame, uint32_t width,
                           const uint32_t *coneType, const uint32_t *cellTrial,
                           uint64_t trialFrameStride, float *R);
 
+/* The same for nFrames frames [nFrames][3][height][width] in one launch:
+ * R[nFrames][nLGN][2]. */
+int vc_lgn_frame_responses(void *stream, const float *frames, uint32_t nFrames,
+                           uint32_t width, uint32_t height, uint32_t nLGN,
+                           uint32_t nSample1D, const float *SCx, const float *SCy,
+                           const float *SW, const uint32_t *coneType, float *R);
+
+/* Temporal weights alone (the TW part of vc_lgn_precompute), for callers that
+ * change the temporal parameters without re-deriving maxConvol. */
+int vc_lgn_temporal_weights(void *stream, const vc_lgn_setup *q, float *TW);
+
 typedef struct {
     uint32_t nLGN, nKernelSample, kernelSampleInterval, iKernelSampleT0;
     uint32_t nRetrace, ntPerFrame, maxFrame;
@@ -244,6 +255,43 @@ typedef struct {
 int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
                  const float *Rring, uint32_t it0, uint32_t nsteps,
                  float *sInfo, float *fr, const vc_ff_events *ev);
+
+/* The rate part of vc_lgn_steps alone (no spike draw), for any run of steps
+ * and any maxFrame: fr[nsteps][nLGN] (Hz).  R is read at slot f mod maxFrame
+ * for frame f; a caller holding every frame of a run passes maxFrame = the
+ * number of frames held. */
+int vc_lgn_rates(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
+                 const float *R, uint32_t it0, uint32_t nsteps, float *fr);
+
+/* ---- reverse mode of the LGN rates (no counterpart in the reference; the
+ * forward it differentiates is vc_lgn_frame_responses + vc_lgn_rates, i.e.
+ * LGN_convol_parvo and the rate of LGN_nonlinear,
+ * src/discrete_input_convol.cu:1003-1340, 1587-1622).  maxConvol is held
+ * constant.  Every output is ACCUMULATED into (the caller zeroes it first). */
+
+/* Given gFr[nsteps][nLGN] = dL/dfr of vc_lgn_rates(p, s, R, it0, nsteps):
+ *   dR[maxFrame][nLGN][2]   += dL/dR     (same slots as R; maxFrame must hold
+ *                                         every frame the run touches)
+ *   dTW[nLGN][2][nKS]       += dL/dTW
+ *   dK[nLGN][2]             += dL/d(kC, kS) */
+int vc_lgn_rates_adjoint(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
+                         const float *R, uint32_t it0, uint32_t nsteps, const float *gFr,
+                         float *dR, float *dTW, float *dK);
+
+/* Given dR[nFrames][nLGN][2] for the frames of vc_lgn_frame_responses:
+ * dFrames[nFrames][3][height][width] += dL/dframes (float atomics: the order
+ * of the additions, hence the last bits, varies from run to run). */
+int vc_lgn_frame_responses_adjoint(void *stream, const float *frames, uint32_t nFrames,
+                                   uint32_t width, uint32_t height, uint32_t nLGN,
+                                   uint32_t nSample1D, const float *SCx, const float *SCy,
+                                   const float *SW, const uint32_t *coneType,
+                                   const float *dR, float *dFrames);
+
+/* Given dTW[nLGN][2][nKS] for the weights of vc_lgn_temporal_weights(q):
+ * dTauR[2][nLGN], dTauD[2][nLGN] = dL/d(tauR, tauD) (written, not
+ * accumulated), through the normalisation by sum |TW|. */
+int vc_lgn_temporal_adjoint(void *stream, const vc_lgn_setup *q, const float *dTW,
+                            float *dTauR, float *dTauD);
 
 /* LGN Poisson state initialisation (logRand_init, src/coredynamics.cu:148-177):
  * leftTimeRate = 0, lastNegLogRand = -log(u) with u a Philox draw. */

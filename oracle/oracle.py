@@ -234,3 +234,127 @@ class OracleSim:
 
     def lgn_sinfo(self):
         return np.concatenate(self.lgn_hist)
+
+
+# ---------------------------------------------------------------------------
+# float64 LGN rate loss and gradient (orc_lgn64_*, oracle/vc_oracle_grad.c)
+
+class _OrcLgn64(C.Structure):
+    _fields_ = ([(n, C.c_uint32) for n in ("nLGN", "nS", "nKS", "interval", "iT0", "nRetrace",
+                                          "npf", "nF", "T", "W", "H")]
+                + [(n, C.c_float) for n in ("dt", "convolRatio", "frRatio")]
+                + [(n, C.c_void_p) for n in ("SCx", "SCy", "SW", "coneType", "delay", "ratio",
+                                            "nR", "nD", "maxConvol", "c50", "sharpness",
+                                            "logA", "logB", "initFrame")])
+
+
+def lgn_tables(o: Oracle, spec, lgn: dict, network_mod) -> dict:
+    """orc_lgn_precompute for an LGN layer given as (spec, lgn dict)."""
+    from types import SimpleNamespace
+    tm = network_mod.timing(spec)
+    x0, y0, nvd = network_mod.view_geometry(spec)
+    q = o.abi.LgnSetup()
+    q.nLGN, q.nSample1D, q.nKernelSample = spec.nLGN, spec.nSpatialSample1D, spec.nKernelSample
+    q.kernelSampleInterval, q.iKernelSampleT0 = tm["kernelSampleInterval"], tm["iKernelSampleT0"]
+    q.maxSample1D = network_mod.max_sample_1d(SimpleNamespace(spec=spec, lgn=lgn))
+    q.dt, q.nsig, q.normViewDistance, q.x0, q.y0 = spec.dt, spec.nsig, nvd, x0, y0
+    keep = []
+    for k in ("polar", "ecc", "rw", "rh", "orient", "k", "tauR", "tauD", "delay", "ratio",
+              "nR", "nD", "covariant"):
+        a = np.ascontiguousarray(lgn[k], dtype=np.float32)
+        keep.append(a)
+        setattr(q, k, a.ctypes.data)
+    nS = spec.nSpatialSample1D ** 2
+    TW = np.zeros((spec.nLGN, 2, spec.nKernelSample), np.float32)
+    SW = np.zeros(nS, np.float32)
+    SCx = np.zeros((spec.nLGN, 2, nS), np.float32)
+    SCy = np.zeros((spec.nLGN, 2, nS), np.float32)
+    mc = np.zeros(spec.nLGN, np.float32)
+    o.L.orc_lgn_precompute(C.byref(q), _fp(TW), _fp(SW), _fp(SCx), _fp(SCy), _fp(mc))
+    return dict(TW=TW, SW=SW, SCx=SCx, SCy=SCy, maxConvol=mc)
+
+
+class LgnGrad64:
+    """The LGN rates of `nsteps` steps as a float64 function of (frames,
+    theta), theta[c] = (kC, kS, tauR_C, tauD_C, tauR_S, tauD_S), with the
+    loss mean((fr - target)^2) and its gradient."""
+
+    def __init__(self, o: Oracle, spec, lgn: dict, tables: dict, nsteps: int, network_mod,
+                 init_lms=(0.5, 0.5, 0.5)):
+        tm = network_mod.timing(spec)
+        npf = tm["ntPerFrame"]
+        self.nF = -(-nsteps // npf)
+        self.spec, self.T = spec, nsteps
+        self.L = o.L
+        vp = C.c_void_p
+        self.L.orc_lgn64_loss.argtypes = [C.POINTER(_OrcLgn64), vp, vp, vp, vp, C.c_int,
+                                          C.POINTER(C.c_long)]
+        self.L.orc_lgn64_loss.restype = C.c_double
+        self.L.orc_lgn64_grad.argtypes = [C.POINTER(_OrcLgn64), vp, vp, vp, vp, vp]
+        self.L.orc_lgn64_rates.argtypes = [C.POINTER(_OrcLgn64), vp, vp, vp]
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        self.keep = dict(
+            SCx=f32(tables["SCx"]), SCy=f32(tables["SCy"]), SW=f32(tables["SW"]),
+            maxConvol=f32(tables["maxConvol"]),
+            coneType=np.ascontiguousarray(lgn["coneType"], dtype=np.uint32),
+            **{k: f32(lgn[k]) for k in ("delay", "ratio", "nR", "nD", "c50", "sharpness",
+                                        "logA", "logB")})
+        init = np.empty((3, spec.height, spec.width), np.float64)
+        for c in range(3):
+            init[c] = np.float32(init_lms[c])
+        self.keep["initFrame"] = init
+        m = _OrcLgn64()
+        m.nLGN, m.nS, m.nKS = spec.nLGN, spec.nSpatialSample1D ** 2, spec.nKernelSample
+        m.interval, m.iT0 = tm["kernelSampleInterval"], tm["iKernelSampleT0"]
+        m.nRetrace, m.npf, m.nF, m.T = tm["nRetrace"], npf, self.nF, nsteps
+        m.W, m.H = spec.width, spec.height
+        m.dt, m.convolRatio, m.frRatio = spec.dt, spec.convolRatio, spec.frRatioLGN
+        for k, a in self.keep.items():
+            setattr(m, k, a.ctypes.data)
+        self.m = m
+        self.theta0 = np.stack([lgn["k"][0], lgn["k"][1], lgn["tauR"][0], lgn["tauD"][0],
+                                lgn["tauR"][1], lgn["tauD"][1]], 1).astype(np.float32)
+
+    def _frames(self, frames):
+        f = np.ascontiguousarray(frames[:self.nF], dtype=np.float64)
+        assert f.shape[0] == self.nF, "need ceil(T / ntPerFrame) frames"
+        return f
+
+    def loss(self, frames, theta, target, cells=None):
+        f = self._frames(frames)
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        tg = np.ascontiguousarray(target, dtype=np.float32)
+        cl = None if cells is None else np.ascontiguousarray(cells, dtype=np.int32)
+        pat = C.c_long(0)
+        v = self.L.orc_lgn64_loss(C.byref(self.m), _fp(f), _fp(th), _fp(tg), _fp(cl),
+                                  0 if cl is None else cl.size, C.byref(pat))
+        return v, pat.value
+
+    def rates(self, frames, theta):
+        f = self._frames(frames)
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        fr = np.zeros((self.T, self.spec.nLGN), np.float64)
+        self.L.orc_lgn64_rates(C.byref(self.m), _fp(f), _fp(th), _fp(fr))
+        return fr
+
+    def grad(self, frames, theta, target):
+        f = self._frames(frames)
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        tg = np.ascontiguousarray(target, dtype=np.float32)
+        df = np.zeros_like(f)
+        dth = np.zeros_like(th)
+        self.L.orc_lgn64_grad(C.byref(self.m), _fp(f), _fp(th), _fp(tg), _fp(df), _fp(dth))
+        return df, dth
+
+    def cells_touching(self, px, py, ch):
+        """Cells any of whose samples read pixel (px, py) of channel ch."""
+        sp = self.spec
+        W, H = sp.width, sp.height
+        xt = self.keep["SCx"].astype(np.float64) * W - 0.5
+        yt = self.keep["SCy"].astype(np.float64) * H - 0.5
+        i0 = np.floor(xt); j0 = np.floor(yt)
+        hit = ((i0 == px) | (i0 + 1 == px)) & ((j0 == py) | (j0 + 1 == py))
+        cone = self.keep["coneType"]                      # [2][nLGN]
+        uses = {0: (0, 3, 4), 1: (1, 3, 4), 2: (2, 4)}[ch]
+        ok = np.isin(cone, uses).any(0)
+        return np.nonzero(hit.any(axis=(1, 2)) & ok)[0].astype(np.int32)

@@ -118,6 +118,14 @@ SIGNATURES = {
     "vc_lgn_steps": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp, vp,
                      C.POINTER(FFEvents)],
     "vc_lgn_init": [vp, C.POINTER(LgnParams), vp, vp],
+    "vc_lgn_frame_responses": [vp, vp, u32, u32, u32, u32, u32, vp, vp, vp, vp, vp],
+    "vc_lgn_temporal_weights": [vp, C.POINTER(LgnSetup), vp],
+    "vc_lgn_rates": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp],
+    "vc_lgn_rates_adjoint": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp,
+                             vp, vp, vp],
+    "vc_lgn_frame_responses_adjoint": [vp, vp, u32, u32, u32, u32, u32, vp, vp, vp, vp, vp,
+                                       vp],
+    "vc_lgn_temporal_adjoint": [vp, C.POINTER(LgnSetup), vp, vp, vp],
     "vc_last_error": [],
     "vc_version": [],
 }

@@ -20,83 +20,21 @@
 // MFMA is not used: every contraction here is a per-cell dot product over
 // that cell's own irregular sample set (no operand is shared across cells),
 // i.e. a batched GEMV, not a GEMM.
-#include "vc_common.h"
-#include <math.h>
+#include "vc_lgn_device.h"
 
 namespace {
-
-constexpr double kSqrt2 = 1.4142135623730951;  // src/CONST.h:32 (double)
-
-// block_reduce of src/util/cuda_util.cuh:89-101 for blockDim.x threads (a
-// power of two >= 32).  Every thread gets the result.
-__device__ float block_reduce_ref(float x, float *part /* >= 32 floats */) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t W = blockDim.x / 32u;
-    x = vc_tree32(x);
-    __syncthreads();
-    if ((tid & 31u) == 0u) part[tid >> 5] = x;
-    __syncthreads();
-    if (tid == 0) {
-        for (uint32_t off = W / 2; off > 0; off >>= 1)
-            for (uint32_t l = 0; l < off; ++l) part[l] = part[l] + part[l + off];
-    }
-    __syncthreads();
-    float r = part[0];
-    __syncthreads();
-    return r;
-}
-
-__device__ __forceinline__ float texel(const float *img, int W, int H, int i, int j) {
-    if (i < 0 || j < 0 || i >= W || j >= H) return 0.0f;
-    return img[(size_t)j * W + i];
-}
-
-// exact-float bilinear with border addressing and normalised coordinates
-__device__ __forceinline__ float bilinear(const float *img, int W, int H, float x, float y) {
-    float xt = x * (float)W - 0.5f;
-    float yt = y * (float)H - 0.5f;
-    if (!(xt > -2.0f && xt < (float)W + 1.0f && yt > -2.0f && yt < (float)H + 1.0f))
-        return 0.0f;
-    float fx = floorf(xt), fy = floorf(yt);
-    float ax = xt - fx, ay = yt - fy;
-    int i0 = (int)fx, j0 = (int)fy;
-    float t00 = texel(img, W, H, i0, j0);
-    float t10 = texel(img, W, H, i0 + 1, j0);
-    float t01 = texel(img, W, H, i0, j0 + 1);
-    float t11 = texel(img, W, H, i0 + 1, j0 + 1);
-    float bx = 1.0f - ax, by = 1.0f - ay;
-    return by * (bx * t00 + ax * t10) + ay * (bx * t01 + ax * t11);
-}
-
-// get_intensity, src/discrete_input_convol.cu:35-76
-__device__ float intensity(uint32_t cone, const float *frame, int W, int H, float x, float y) {
-    const size_t np = (size_t)W * H;
-    switch (cone) {
-    case 0: return bilinear(frame, W, H, x, y);
-    case 1: return bilinear(frame + np, W, H, x, y);
-    case 2: return bilinear(frame + 2 * np, W, H, x, y);
-    case 3: return (bilinear(frame, W, H, x, y) + bilinear(frame + np, W, H, x, y)) / 2.0f;
-    case 4: return bilinear(frame, W, H, x, y) * 0.45436904f +
-                   bilinear(frame + np, W, H, x, y) * 0.47353315f +
-                   bilinear(frame + 2 * np, W, H, x, y) * 0.0720978f;
-    default: return 0.0f;
-    }
-}
-
-__device__ __forceinline__ float clip_contrast(float I, float mean) {
-    float c = (mean > 0.0f) ? I / mean - 1.0f : I;
-    if (fabsf(c) > 1.0f) c = copysignf(1.0f, c);
-    return c;
-}
 
 __global__ __launch_bounds__(1024) void k_lgn_frame_response(
     const float *__restrict__ frame0, int W, int H, uint32_t nLGN,
     const float *__restrict__ SCx, const float *__restrict__ SCy,
     const float *__restrict__ SW, const uint32_t *__restrict__ coneType,
     const uint32_t *__restrict__ cellTrial, uint64_t trialFrameStride,
-    float *__restrict__ R) {
+    uint64_t frameStride, float *__restrict__ R) {
     __shared__ float part[32];
     const uint32_t c = blockIdx.x;
+    // blockIdx.y: frame of a batch of frames frameStride floats apart
+    frame0 += (size_t)blockIdx.y * frameStride;
+    R += (size_t)blockIdx.y * nLGN * 2;
     const float *frame = cellTrial ? frame0 + (size_t)cellTrial[c] * trialFrameStride : frame0;
     const uint32_t s = threadIdx.x;
     const uint32_t nS = blockDim.x;
@@ -130,19 +68,8 @@ __device__ __forceinline__ float tree32_regs(float a[32]) {
     return a[0];
 }
 
-__device__ __forceinline__ int64_t floor_div(int64_t a, int64_t b) {
-    int64_t q = a / b;
-    if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
-    return q;
-}
-
-constexpr int kMaxTaps = 64;
 constexpr int kMaxFrame = 8;
 
-__device__ __forceinline__ float transform(float C50, float K, float A, float B, float input) {
-    float X = (float)(1.0 / (double)(1.0f + vc_expf(K * (C50 - input))));
-    return A * X + B;
-}
 
 // Stage a (temporal part) and the rate of stage b for every (cell, step):
 // one wavefront per cell, lane = step, so the cell's temporal weights and
@@ -307,23 +234,6 @@ __global__ void k_lgn_init(vc_lgn_params p, float *lTR, float *lNL) {
 
 // ---------------------------------------------------------------- setup --
 
-// src/discrete_input_convol.cu:14-33
-__device__ float temporal_kernel(float tau, float tauR, float tauD, float ratio, float nR,
-                                 float nD, float lfac1, float lfac2) {
-    float tau1 = tau / tauR;
-    float tau2 = tau / tauD;
-    float A1 = (nR - 1) * logf(tau1);
-    float A2 = (nD - 1) * logf(tau2);
-    float exponent = A1 - tau1 - lfac1;
-    float tpR = expf(exponent) / tauR;
-    exponent = A2 - tau2 - lfac2;
-    float tpD = expf(exponent) / tauD;
-    return tpR - ratio * tpD;
-}
-
-__device__ __forceinline__ float spatial_kernel(float x, float y, float rx, float ry) {
-    return expf(-x * x / (rx * rx) - y * y / (ry * ry));
-}
 
 // store_temporalWeight, one workgroup of nS threads per (cell, region)
 __global__ __launch_bounds__(1024) void k_lgn_tw(vc_lgn_setup q, float *__restrict__ TW) {
@@ -523,7 +433,33 @@ extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t 
     }
     hipLaunchKernelGGL(k_lgn_frame_response, dim3(nLGN), dim3(nS), 0, (hipStream_t)stream,
                        frame, (int)width, (int)height, nLGN, SCx, SCy, SW, coneType, cellTrial,
-                       trialFrameStride, R);
+                       trialFrameStride, (uint64_t)0, R);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_lgn_frame_responses(void *stream, const float *frames, uint32_t nFrames,
+                                      uint32_t width, uint32_t height, uint32_t nLGN,
+                                      uint32_t nSample1D, const float *SCx, const float *SCy,
+                                      const float *SW, const uint32_t *coneType, float *R) {
+    const uint32_t nS = nSample1D * nSample1D;
+    if (!frames || !SCx || !SCy || !SW || !coneType || !R) {
+        vc_set_error("vc_lgn_frame_responses: null argument");
+        return VC_E_INVALID;
+    }
+    if (nS < 32 || nS > 1024 || (nS & (nS - 1)) != 0) {
+        vc_set_error("nSpatialSample1D^2 = %u must be a power of two in [32, 1024]", nS);
+        return VC_E_UNSUPPORTED;
+    }
+    if (width == 0 || height == 0 || nLGN == 0 || nFrames > 65535u) {
+        vc_set_error("vc_lgn_frame_responses: empty frame or LGN, or more than 65535 frames");
+        return VC_E_INVALID;
+    }
+    if (nFrames == 0) return VC_OK;
+    const uint64_t fs = 3ull * width * height;
+    hipLaunchKernelGGL(k_lgn_frame_response, dim3(nLGN, nFrames), dim3(nS), 0,
+                       (hipStream_t)stream, frames, (int)width, (int)height, nLGN, SCx, SCy, SW,
+                       coneType, (const uint32_t *)nullptr, (uint64_t)0, fs, R);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
@@ -577,6 +513,57 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
     }
     hipLaunchKernelGGL(k_lgn_spike, dim3((p->nLGN + 255) / 256), dim3(256), 0,
                        (hipStream_t)stream, *p, *s, it0, nsteps, fr, sInfo, e);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_lgn_rates(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
+                            const float *R, uint32_t it0, uint32_t nsteps, float *fr) {
+    if (!p || !s || !R || !fr) {
+        vc_set_error("vc_lgn_rates: null argument");
+        return VC_E_INVALID;
+    }
+    if (p->nKernelSample == 0 || p->nKernelSample > (uint32_t)kMaxTaps) {
+        vc_set_error("nKernelSample %u unsupported (1..%d)", p->nKernelSample, kMaxTaps);
+        return VC_E_UNSUPPORTED;
+    }
+    if (p->maxFrame == 0 || p->ntPerFrame == 0) {
+        vc_set_error("vc_lgn_rates: maxFrame and ntPerFrame must be positive");
+        return VC_E_INVALID;
+    }
+    if ((63u + p->nRetrace) / p->ntPerFrame + 1u > (uint32_t)kMaxFrame) {
+        vc_set_error("a %u-step window over %u-step frames spans too many frames", p->nRetrace,
+                     p->ntPerFrame);
+        return VC_E_UNSUPPORTED;
+    }
+    if (nsteps == 0 || p->nLGN == 0) return VC_OK;
+    const uint64_t waves = (uint64_t)p->nLGN * ((nsteps + 63u) / 64u);
+    if (waves * 64 / 256 + 1 > 0x7fffffffull) {
+        vc_set_error("vc_lgn_rates: %u steps x %u cells is too large for one launch", nsteps,
+                     p->nLGN);
+        return VC_E_INVALID;
+    }
+    hipLaunchKernelGGL(k_lgn_rate, dim3((uint32_t)((waves * 64 + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, *p, *s, R, it0, nsteps, fr);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_lgn_temporal_weights(void *stream, const vc_lgn_setup *q, float *TW) {
+    if (!q || !TW) {
+        vc_set_error("vc_lgn_temporal_weights: null argument");
+        return VC_E_INVALID;
+    }
+    const uint32_t nS = q->nSample1D * q->nSample1D;
+    if (nS < 32 || nS > 1024 || (nS & (nS - 1)) != 0) {
+        vc_set_error("nSpatialSample1D^2 = %u must be a power of two in [32, 1024]", nS);
+        return VC_E_UNSUPPORTED;
+    }
+    if (q->nLGN == 0 || q->nKernelSample == 0) {
+        vc_set_error("vc_lgn_temporal_weights: empty LGN");
+        return VC_E_INVALID;
+    }
+    hipLaunchKernelGGL(k_lgn_tw, dim3(q->nLGN, 2), dim3(nS), 0, (hipStream_t)stream, *q, TW);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -442,6 +442,29 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
     return net
 
 
+def lgn_layer(spec: ModelSpec, seed: int | None = None) -> dict:
+    """The LGN cells alone (no V1): the jittered nLGN_side^2 grid with
+    checkerboard ON/OFF of build_network, drawn from its own generator."""
+    rng = np.random.default_rng(spec.seed if seed is None else seed)
+    ns = spec.nLGN_side
+    spacing = spec.field_deg / ns
+    gx, gy = np.meshgrid(np.arange(ns), np.arange(ns))
+    gx = gx.ravel(); gy = gy.ravel()
+    lx = (gx + 0.5) * spacing - spec.field_deg / 2 + (rng.random(ns * ns) - 0.5) * 0.6 * spacing
+    ly = (gy + 0.5) * spacing - spec.field_deg / 2 + (rng.random(ns * ns) - 0.5) * 0.6 * spacing
+    on = ((gx + gy) % 2 == 0)
+    return lgn_parameters(spec, lx, ly, on)
+
+
+def white_noise_frames(spec: ModelSpec, nframe: int, sigma: float = 0.3, mean: float = 0.5,
+                       seed: int = 0) -> np.ndarray:
+    """Gaussian white-noise frames [nFrame, 3, H, W] float32: mean + sigma N(0, 1)
+    per pixel and channel (no clipping, so every pixel is a smooth variable)."""
+    rng = np.random.default_rng(seed)
+    f = mean + sigma * rng.standard_normal((nframe, 3, spec.height, spec.width))
+    return np.ascontiguousarray(f.astype(np.float32))
+
+
 def logistic_ab(spont, c50, K):
     """The precached logistic coefficients a, b of src/LGN_props.h:163-169
     (f(0) = spont, f(1) = 1), in float64 from the float32 parameters."""
