@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=640)
     ap.add_argument("--cpu-steps", type=int, default=800,
                     help="steps of the CPU-baseline sample (0 = skip)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--reps", type=int, default=5, help="c4: timed forward/backward repetitions")
     ap.add_argument("--trials", type=int, default=32, help="c3: number of trials")
     ap.add_argument("--copy-calib", action="store_true",
@@ -82,6 +82,8 @@ def main():
         return bench_batched(args)
     if args.config == "c4":
         return bench_lgn_adjoint(args)
+    if args.config == "c5":
+        return bench_v1_adjoint(args)
     spec = network.spec_c2() if args.config == "c2" else network.spec_c1()
     K, W = args.steps, args.warmup
     spec = network.replace(spec, nt=K + W) if hasattr(network, "replace") else spec
@@ -276,6 +278,76 @@ def bench_lgn_adjoint(args):
                                f"{spec.width}x{spec.height} px, {T} steps, 6 parameters per cell",
                    "reps": args.reps},
         "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_v1_adjoint(args):
+    """Differentiation of the V1 dynamics: 16 blocks = 16,384 conductance-LIF
+    neurons, 20 LGN + 400 E + 100 I synapses per neuron, 250 ms (2000
+    steps), checkpoints every 100 steps; loss = squared error of smoothed
+    spike counts in 25 ms bins against a run with perturbed recurrent
+    weights.  Times the forward (2000 steps, checkpoints kept) and the
+    backward (segment recompute with snapshots + reverse sweep) with HIP
+    events on their stream; reports backward / forward."""
+    load_package()
+    import torch
+    from visual_cortex_amd import batch, network as N
+    from visual_cortex_amd.v1_grad import V1Run, soft_bin_counts, soft_count_loss
+    spec = N.spec_c5()
+    T = spec.nt
+    t0 = time.time()
+    net = batch.build_network_fast(spec, device="cuda")
+    run = V1Run(net, T, ckpt=100)
+    sL, w = run.sim.sLGN.clone(), run.sim.wrec.clone()
+    g = torch.Generator(device="cuda").manual_seed(3)
+    with torch.no_grad():
+        wp = w * torch.exp(0.2 * torch.randn(w.shape, generator=g, device="cuda"))
+        tgt = soft_bin_counts(run.forward(sL, wp), spec.dt, 200, 1.0)
+    log(f"[bench] V1 adjoint: nV1={spec.nV1} synapses={w.numel()} + {sL.numel()} T={T}, "
+        f"set up in {time.time() - t0:.1f}s")
+    stream = torch.cuda.current_stream()
+
+    def once(timing=None):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        sInfo = run.forward(sL, w)
+        e[1].record(stream)
+        x = sInfo.detach().requires_grad_(True)
+        loss = soft_count_loss(x, tgt, spec.dt, 200, 1.0)
+        gOut, = torch.autograd.grad(loss, x)
+        e[2].record(stream)
+        dsl, dw = run.backward(gOut)
+        e2 = torch.cuda.Event(enable_timing=True)
+        e2.record(stream)
+        if timing is not None:
+            timing.append((e[0], e[1], e[2], e2))
+        return sInfo, dsl, dw
+
+    sInfo, _, _ = once()
+    torch.cuda.synchronize()
+    rho = float((sInfo >= 1).float().mean())
+    tim = []
+    for _ in range(args.reps):
+        once(tim)
+    torch.cuda.synchronize()
+    fwd = sum(a.elapsed_time(b) for a, b, c, d in tim) / len(tim) / 1e3
+    bwd = sum(c.elapsed_time(d) for a, b, c, d in tim) / len(tim) / 1e3
+    log(f"[bench] forward {fwd * 1e3:.2f} ms ({spec.nV1 * T / fwd:.3e} neuron-steps/s), "
+        f"backward {bwd * 1e3:.2f} ms; spikes/neuron-step {rho:.2e} "
+        f"({rho / (spec.dt / 1000):.1f} Hz)")
+    line = {
+        "metric": "V1 adjoint/fwd time",
+        "value": bwd / fwd,
+        "unit": "backward time / forward time",
+        "n_gpus": 1, "steps": T, "warmup": 1, "ms_per_step": (fwd + bwd) * 1e3 / T,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded LIF network, drifting grating, perturbed-weight target)",
+        "config": {"workload": f"V1 spike-time adjoint: {spec.nblock} blocks x {spec.blockSize} "
+                               f"LIF neurons, {spec.m} LGN + {spec.KE}+{spec.KI} recurrent "
+                               f"synapses, {T} steps, checkpoint every 100",
+                   "reps": args.reps, "weights": int(w.numel() + sL.numel())},
+        "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3, "spikes_per_neuron_step": rho,
     }
     print(json.dumps(line), flush=True)
 

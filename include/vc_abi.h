@@ -181,6 +181,58 @@ int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,
 /* Initial V1 state: v, w, tBack, g/h as given; clears a spike buffer. */
 int vc_spikes_clear(void *stream, uint32_t nV1, const vc_spikes *sp);
 
+/* ---- reverse mode of the V1 step (no counterpart in the reference; the
+ * forward it differentiates is vc_v1_step, i.e. compute_V_collect_spike_
+ * learnFF_fast + recal_G_mat_nd_fast, src/coredynamics.cu:1235-1629,
+ * 2609-2922, for the conductance LIF model). */
+
+/* Rows of a step snapshot, each [nV1]: the state at the start of a step with
+ * the cortical conductances after the step's recurrent increments. */
+#define VC_SNAP_ROWS 14
+#define VC_SNAP_V     0
+#define VC_SNAP_TBACK 1
+#define VC_SNAP_GFF   2     /* + ig, ig < VC_MAX_NG */
+#define VC_SNAP_HFF   4
+#define VC_SNAP_GE    6
+#define VC_SNAP_HE    8
+#define VC_SNAP_GI   10
+#define VC_SNAP_HI   12
+
+/* vc_v1_step that also writes the step's snapshot to out_snap
+ * [VC_SNAP_ROWS][nV1] (rows past the receptor counts are left untouched). */
+int vc_v1_step_snapshot(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                        const vc_ff_conn *ff, const vc_rec_conn *rec,
+                        const vc_spikes *prev, const vc_spikes *next,
+                        const float *lgnSInfo, uint32_t *ffMask, uint32_t step,
+                        float *out_sInfo, float *out_snap);
+
+/* Inputs and accumulators of the reverse sweep over steps it0 .. it0+n-1. */
+typedef struct {
+    const float *snap;          /* [n][VC_SNAP_ROWS][nV1] snapshots of the steps    */
+    const vc_spikes *spk;       /* [n + 1]: spk[k] = spikes of step it0+k-1 (the
+                                   input of step it0+k), as vc_v1_step wrote them */
+    const float *sInfo;         /* [n][nV1] the forward's output of each step       */
+    uint64_t lgnStride;         /* floats between the LGN rows of two steps        */
+    const float *lgnSInfo;      /* [n][lgnStride] LGN output of each step          */
+    const float *gOut;          /* [n][nV1] dL/dsInfo of each step                 */
+    float *lam;                 /* [VC_SNAP_ROWS][nV1] dL/d(state): in, after step
+                                   it0+n-1 (zero at the end of the run); out, at the
+                                   start of step it0 (before its increments)        */
+    float *lamTsp;              /* [2][nV1] dL/d(spike time) accumulators, slot
+                                   step & 1; zero before the first call of a run    */
+    float *dsLGN;               /* [maxLGN][nV1] += dL/d(sLGN)                      */
+    float *dw;                  /* [nnz] += dL/d(w), the order of vc_rec_conn.w     */
+} vc_v1_adjoint;
+
+/* Reverse sweep over steps it0+n-1 down to it0 for the conductance LIF
+ * model (iModel 0, a = 0, deltaT = 0 for every type; VC_E_UNSUPPORTED
+ * otherwise), at the fixed spike pattern of the forward: spike times are
+ * differentiated through their interpolation, the refractory back-up and the
+ * conductance increments they cause.  s supplies tonicDep and w. */
+int vc_v1_adjoint_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                      const vc_ff_conn *ff, const vc_rec_conn *rec,
+                      const vc_v1_adjoint *a, uint32_t it0, uint32_t n);
+
 /* ----------------------------------------------------------------- LGN -- */
 
 /* Per-cell LGN parameters in the reference's SoA order [region][nLGN]

@@ -358,3 +358,88 @@ class LgnGrad64:
         uses = {0: (0, 3, 4), 1: (1, 3, 4), 2: (2, 4)}[ch]
         ok = np.isin(cone, uses).any(0)
         return np.nonzero(hit.any(axis=(1, 2)) & ok)[0].astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+# float64 V1 LIF forward and tape gradient (orc_v1_lif64, oracle/vc_oracle_v1grad.c)
+
+class V1Grad64:
+    """The V1 steps of a Network (conductance LIF) in float64 as a function of
+    its weights, with the reverse-mode tape gradient.  lgn [T][nLGN] is the
+    LGN output of every step (an input); init is the state in snapshot rows."""
+
+    def __init__(self, o: Oracle, net, params_mod, lgn: np.ndarray, init: np.ndarray):
+        self.net, self.L = net, o.L
+        sp = net.spec
+        self.p = params_mod.v1_params_struct(net)
+        vp, u32 = C.c_void_p, C.c_uint32
+        self.L.orc_v1_lif64.argtypes = ([C.POINTER(o.abi.V1Params), u32, vp, vp, vp, u32, vp, vp,
+                                         vp, vp, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp])
+        self.L.orc_v1_lif64.restype = C.c_int
+        self.lgn = np.ascontiguousarray(lgn, dtype=np.float32)
+        self.T = self.lgn.shape[0]
+        self.init = np.ascontiguousarray(init, dtype=np.float32)
+        self.tonic = np.ascontiguousarray(net.tonicDep, dtype=np.float32)
+        self.wstate = np.zeros(sp.nV1, np.float32)
+        self.nLGN_of = np.ascontiguousarray(net.nLGN_of, dtype=np.uint32)
+        self.idLGN = np.ascontiguousarray(net.idLGN, dtype=np.uint32)
+        self.nNear = np.ascontiguousarray(net.nNear, dtype=np.uint32)
+        self.nbr = np.ascontiguousarray(net.nbr, dtype=np.uint32)
+        self.conMat = np.ascontiguousarray(net.dense_conmat(), dtype=np.float64)
+        self.sLGN = np.ascontiguousarray(net.sLGN, dtype=np.float64)
+
+    def run(self, sLGN=None, conMat=None, pattern=None, gOut=None):
+        """sInfo [T][nV1] (float64); with gOut also (dsLGN, dConMat)."""
+        sp = self.net.spec
+        sL = self.sLGN if sLGN is None else np.ascontiguousarray(sLGN, dtype=np.float64)
+        cm = self.conMat if conMat is None else np.ascontiguousarray(conMat, dtype=np.float64)
+        pat = None if pattern is None else np.ascontiguousarray(pattern, dtype=np.uint8)
+        out = np.zeros((self.T, sp.nV1), np.float64)
+        g = None if gOut is None else np.ascontiguousarray(gOut, dtype=np.float64)
+        dsl = np.zeros_like(sL) if g is not None else None
+        dcm = np.zeros_like(cm) if g is not None else None
+        rc = self.L.orc_v1_lif64(C.byref(self.p), self.T, _fp(self.init), _fp(self.tonic),
+                                 _fp(self.wstate), sL.shape[0], _fp(self.nLGN_of), _fp(self.idLGN),
+                                 _fp(sL), _fp(self.lgn), self.lgn.shape[1], self.net.maxNear,
+                                 _fp(self.nNear), _fp(self.nbr), _fp(cm), _fp(pat), _fp(out),
+                                 _fp(g), _fp(dsl), _fp(dcm))
+        if rc != 0:
+            raise RuntimeError(f"orc_v1_lif64 returned {rc}")
+        return out if g is None else (out, dsl, dcm)
+
+
+def soft_bin_counts64(sInfo, dt, bin_steps, tau_s):
+    """Spike counts per 25 ms-style bin, each spike weighted by a logistic-
+    smoothed bin indicator sigma((t-lo)/tau_s) - sigma((t-hi)/tau_s) over its
+    own bin and the two next to it; float64 twin of the torch loss.  Returns
+    (counts [nBins][nV1], d counts / d sInfo as a function of the residual)."""
+    T, N = sInfo.shape
+    nb = T // bin_steps
+    counts = np.zeros((nb, N))
+    it, n = np.nonzero(sInfo >= 1.0)
+    t = it * dt + (sInfo[it, n] - 1.0) * dt
+    b0 = it // bin_steps
+    parts = []
+    for db in (-1, 0, 1):
+        b = b0 + db
+        ok = (b >= 0) & (b < nb)
+        lo, hi = b * bin_steps * dt, (b + 1) * bin_steps * dt
+        s_lo = 1.0 / (1.0 + np.exp(-(t - lo) / tau_s))
+        s_hi = 1.0 / (1.0 + np.exp(-(t - hi) / tau_s))
+        k = s_lo - s_hi
+        dk = (s_lo * (1 - s_lo) - s_hi * (1 - s_hi)) / tau_s       # d k / d t
+        np.add.at(counts, (b[ok], n[ok]), k[ok])
+        parts.append((b, ok, dk))
+    return counts, (it, n, parts)
+
+
+def soft_count_loss64(sInfo, target, dt, bin_steps, tau_s):
+    """mean((counts - target)^2) and d loss / d sInfo [T][nV1] (float64)."""
+    counts, (it, n, parts) = soft_bin_counts64(sInfo, dt, bin_steps, tau_s)
+    r = 2.0 * (counts - target) / counts.size
+    g = np.zeros_like(sInfo)
+    for b, ok, dk in parts:
+        gg = np.zeros(it.size)
+        gg[ok] = r[b[ok], n[ok]] * dk[ok] * dt                    # dt/dsInfo = dt
+        np.add.at(g, (it, n), gg)
+    return float(np.mean((counts - target) ** 2)), g

@@ -15,6 +15,7 @@ LIB_PATH = PKG_DIR / "libvc_hip.so"
 
 VC_MAX_NTYPE = 2
 VC_MAX_NG = 2
+VC_SNAP_ROWS = 14          # snapshot rows: v, tBack, gFF[2], hFF[2], gE[2], hE[2], gI[2], hI[2]
 
 f32 = C.c_float
 u32 = C.c_uint32
@@ -102,6 +103,12 @@ class LgnState(C.Structure):
                 ("leftTimeRate", vp), ("lastNegLogRand", vp)]
 
 
+class V1Adjoint(C.Structure):
+    _fields_ = [("snap", vp), ("spk", vp), ("sInfo", vp), ("lgnStride", C.c_uint64),
+                ("lgnSInfo", vp), ("gOut", vp), ("lam", vp), ("lamTsp", vp), ("dsLGN", vp),
+                ("dw", vp)]
+
+
 # exported symbols (name -> argtypes); tests check that the .so exports each
 SIGNATURES = {
     "vc_v1_step": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
@@ -113,6 +120,11 @@ SIGNATURES = {
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],
+    "vc_v1_step_snapshot": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
+                            C.POINTER(RecConn), C.POINTER(Spikes), C.POINTER(Spikes), vp, vp,
+                            u32, vp, vp],
+    "vc_v1_adjoint_run": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
+                          C.POINTER(RecConn), C.POINTER(V1Adjoint), u32, u32],
     "vc_lgn_precompute": [vp, C.POINTER(LgnSetup), vp, vp, vp, vp, vp],
     "vc_lgn_frame_response": [vp, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, C.c_uint64, vp],
     "vc_lgn_steps": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp, vp,

@@ -19,7 +19,7 @@
 //            (src/coredynamics.cu:1367-1589, src/coredynamics.cuh:27-118);
 //   stage e: ballot compaction -- one spike mask per tile and the unit
 //            increments at slot tile*64 + rank (src/coredynamics.cu:1994-2046).
-#include "vc_common.h"
+#include "vc_v1_device.h"
 
 namespace {
 
@@ -30,32 +30,7 @@ constexpr int kFF = 32;            // LGN synapses loaded per round
 constexpr int kW = 8;              // weight loads in flight per lane
 constexpr int kEv = 4;             // LGN spike events prefetched per neuron
 
-// per-launch constants derived on the host from vc_v1_params
-struct V1Const {
-    float etrFF[VC_MAX_NG], etdFF[VC_MAX_NG];
-    float etrE[VC_MAX_NG], etdE[VC_MAX_NG];
-    float etrI[VC_MAX_NG], etdI[VC_MAX_NG];
-    uint32_t nSeg;                 // blockSize / 64
-};
 
-__device__ __forceinline__ void single_input(float &g, float &h, float f, float ddt,
-                                             float rise, float decay, float dod) {
-    float etr = vc_expf(-ddt / rise);
-    g = g + f * dod * (vc_expf(-ddt / decay) - etr);
-    h = h + f * etr;
-}
-__device__ __forceinline__ void decay_var(float &g, float &h, float ddt, float rise,
-                                          float decay, float dod) {
-    float etr = vc_expf(-ddt / rise);
-    float etd = vc_expf(-ddt / decay);
-    g = g * etd + h * dod * (etd - etr);
-    h = h * etr;
-}
-__device__ __forceinline__ void decay_const(float &g, float &h, float etr, float etd,
-                                            float dod) {
-    g = g * etd + h * dod * (etd - etr);
-    h = h * etr;
-}
 
 // LGN (or background) conductance input of one synapse, src/coredynamics.cu
 // :1474-1500: nsp spikes at tsp into (g1, h1) over dt - tsp and, when the
@@ -89,12 +64,6 @@ __device__ __noinline__ FFAcc ff_add(FFAcc a, const FFShape sh, float f, float n
     return a;
 }
 
-__device__ __forceinline__ int type_of(const vc_v1_params &p, uint32_t local) {
-    int t = (int)p.nType - 1;
-    for (int j = (int)p.nType - 1; j >= 0; --j)
-        if (local < p.typeAcc[j]) t = j;
-    return t;
-}
 
 // Adds the recurrent increments of a batch of K <= 64 spikes listed in
 // brow/bslot (this wavefront's LDS rows) to the calling lane's sums.
@@ -241,7 +210,8 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
                                                  uint32_t *__restrict__ ffMask,
                                                  uint32_t step, float *__restrict__ out_sInfo,
                                                  float *__restrict__ out_dep,
-                                                 float *__restrict__ out_g_prev) {
+                                                 float *__restrict__ out_g_prev,
+                                                 float *__restrict__ out_snap) {
     __shared__ uint32_t sbrow[kWG / 64][2 * kBatch];
     __shared__ uint32_t sbslot[kWG / 64][2 * kBatch];
     __shared__ uint64_t smk[kMaxEnt];          // neighbour-tile masks of this block (all
@@ -336,6 +306,22 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         he[ig] = he[ig] + lhE[ig];
         gi[ig] = gi[ig] + lgI[ig];
         hi[ig] = hi[ig] + lhI[ig];
+    }
+    if (out_snap) {                 // state at the start of the step, cortical g/h incremented
+        out_snap[VC_SNAP_V * nV1 + n] = s.v[n];
+        out_snap[VC_SNAP_TBACK * nV1 + n] = s.tBack[n];
+        for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+            out_snap[(VC_SNAP_GFF + ig) * nV1 + n] = s.gFF[ig * nV1 + n];
+            out_snap[(VC_SNAP_HFF + ig) * nV1 + n] = s.hFF[ig * nV1 + n];
+        }
+        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+            out_snap[(VC_SNAP_GE + ig) * nV1 + n] = ge[ig];
+            out_snap[(VC_SNAP_HE + ig) * nV1 + n] = he[ig];
+        }
+        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+            out_snap[(VC_SNAP_GI + ig) * nV1 + n] = gi[ig];
+            out_snap[(VC_SNAP_HI + ig) * nV1 + n] = hi[ig];
+        }
     }
     if (out_g_prev) {
         const uint32_t ng = p.ngE + p.ngI;
@@ -666,25 +652,7 @@ int check_params(const vc_v1_params *p, const vc_rec_conn *rec) {
     return VC_OK;
 }
 
-V1Const make_const(const vc_v1_params *p) {
-    V1Const k;
-    for (int ig = 0; ig < VC_MAX_NG; ++ig) {
-        k.etrFF[ig] = vc_expf(-p->dt / p->riseFF[ig]);
-        k.etdFF[ig] = vc_expf(-p->dt / p->decayFF[ig]);
-        k.etrE[ig] = vc_expf(-p->dt / p->riseE[ig]);
-        k.etdE[ig] = vc_expf(-p->dt / p->decayE[ig]);
-        k.etrI[ig] = vc_expf(-p->dt / p->riseI[ig]);
-        k.etdI[ig] = vc_expf(-p->dt / p->decayI[ig]);
-    }
-    k.nSeg = p->blockSize / 64;
-    return k;
-}
 
-uint32_t wg_size(uint32_t blockSize) {
-    if (blockSize % 256 == 0) return 256;
-    if (blockSize % 128 == 0) return 128;
-    return 64;
-}
 
 }  // namespace
 
@@ -712,11 +680,44 @@ extern "C" int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state
     if (ffMask)
         hipLaunchKernelGGL((k_v1_step<true, true>), dim3(p->nV1 / wg), dim3(wg), 0,
                            (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
-                           step, out_sInfo, out_dep, out_g_prev);
+                           step, out_sInfo, out_dep, out_g_prev, nullptr);
     else
         hipLaunchKernelGGL((k_v1_step<true, false>), dim3(p->nV1 / wg), dim3(wg), 0,
                            (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
-                           step, out_sInfo, out_dep, out_g_prev);
+                           step, out_sInfo, out_dep, out_g_prev, nullptr);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_v1_step_snapshot(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                                   const vc_ff_conn *ff, const vc_rec_conn *rec,
+                                   const vc_spikes *prev, const vc_spikes *next,
+                                   const float *lgnSInfo, uint32_t *ffMask, uint32_t step,
+                                   float *out_sInfo, float *out_snap) {
+    int rc = check_params(p, rec);
+    if (rc) return rc;
+    if (!s || !ff || !rec || !prev || !next || !lgnSInfo || !out_sInfo || !out_snap) {
+        vc_set_error("vc_v1_step_snapshot: null argument");
+        return VC_E_INVALID;
+    }
+    if (prev->mask == next->mask) {
+        vc_set_error("vc_v1_step_snapshot: prev and next spike buffers must differ");
+        return VC_E_INVALID;
+    }
+    if (ffMask && ff->maxLGN > 32u) {
+        vc_set_error("vc_v1_step_snapshot: event-driven LGN input needs maxLGN <= 32");
+        return VC_E_INVALID;
+    }
+    V1Const k = make_const(p);
+    uint32_t wg = wg_size(p->blockSize);
+    if (ffMask)
+        hipLaunchKernelGGL((k_v1_step<true, true>), dim3(p->nV1 / wg), dim3(wg), 0,
+                           (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
+                           step, out_sInfo, nullptr, nullptr, out_snap);
+    else
+        hipLaunchKernelGGL((k_v1_step<true, false>), dim3(p->nV1 / wg), dim3(wg), 0,
+                           (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
+                           step, out_sInfo, nullptr, nullptr, out_snap);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
@@ -748,12 +749,12 @@ extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state 
             hipLaunchKernelGGL((k_v1_step<true, true>), dim3(p->nV1 / wg), dim3(wg), 0,
                                (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur], spk2[nx],
                                lgnSInfo + i * lgnStride, ffMask + i * maskStride, it0 + i,
-                               out_sInfo + i * outStride, nullptr, nullptr);
+                               out_sInfo + i * outStride, nullptr, nullptr, nullptr);
         else
             hipLaunchKernelGGL((k_v1_step<true, false>), dim3(p->nV1 / wg), dim3(wg), 0,
                                (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur], spk2[nx],
                                lgnSInfo + i * lgnStride, nullptr, it0 + i,
-                               out_sInfo + i * outStride, nullptr, nullptr);
+                               out_sInfo + i * outStride, nullptr, nullptr, nullptr);
         cur = nx;
     }
     VC_HIP_CHECK(hipGetLastError());
@@ -774,7 +775,7 @@ extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_
     vc_spikes nx{};
     hipLaunchKernelGGL((k_v1_step<false, false>), dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
                        *p, k, *s, ff, *rec, *prev, nx, nullptr, nullptr, 0u, nullptr, nullptr,
-                       nullptr);
+                       nullptr, nullptr);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

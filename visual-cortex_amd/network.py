@@ -141,6 +141,21 @@ def spec_c2(**kw) -> ModelSpec:
     return replace(base, **kw)
 
 
+def spec_c5(**kw) -> ModelSpec:
+    """Differentiation of the V1 dynamics: 16 blocks = 16,384 conductance-LIF
+    neurons (iModel 0, deltaT = a = b = 0 for both types), 20 LGN + 400 E +
+    100 I synapses per neuron, 4096 LGN cells, 250 ms (2000 steps)."""
+    base = ModelSpec(nblock=16, sheet_mm=4.0, nLGN_side=64, nt=2000, iModel=0,
+                     vThres=(-50.0, -50.0), deltaT=(0.0, 0.0), a=(0.0, 0.0), b=(0.0, 0.0))
+    return replace(base, **kw)
+
+
+def lif(spec: ModelSpec) -> ModelSpec:
+    """The conductance-LIF variant of a spec (the model the V1 adjoint covers)."""
+    return replace(spec, iModel=0, vThres=(-50.0, -50.0), deltaT=(0.0, 0.0), a=(0.0, 0.0),
+                   b=(0.0, 0.0))
+
+
 def spec_tiny(**kw) -> ModelSpec:
     """A small case that the oracle finishes in well under a second per 100
     steps: 2 blocks of 256 (192 E), 64 LGN cells, 32x32 px."""
@@ -269,6 +284,21 @@ class Network:
         off = np.zeros(sp.nLGN + 1, np.int64)
         np.cumsum(cnt, out=off[1:])
         return off.astype(np.uint32), tgt[order].astype(np.uint32)
+
+    def csr_dense_index(self) -> np.ndarray:
+        """For every value of the masked CSR (in its order), the flat index of
+        the same synapse in the dense conMat [nblock, maxNear, bs, bs]."""
+        bs = self.spec.blockSize
+        nSeg = bs // 64
+        rm, ro, w = self.masked_csr()
+        bits = ((rm[:, None] >> np.arange(64, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
+        rank = np.cumsum(bits, axis=1) - 1
+        seg_i, lane_i = np.nonzero(bits)
+        idx = ro.astype(np.int64)[seg_i] + rank[seg_i, lane_i]
+        row, sg = seg_i // nSeg, seg_i % nSeg
+        out = np.zeros(idx.size, np.int64)
+        out[idx] = row.astype(np.int64) * bs + sg * 64 + lane_i
+        return out
 
     def n_rec_synapses(self) -> int:
         return int(sum(p[0].size for p in self.rec_pairs))
