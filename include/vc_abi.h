@@ -131,6 +131,12 @@ typedef struct {
     const uint64_t *rowMask;    /* [nblock*maxNear*blockSize][blockSize/64]     */
     const uint32_t *rowOff;     /* [nblock*maxNear*blockSize][blockSize/64]     */
     const float *w;             /* kept strengths                                */
+    /* Optional (NULL = not used): the same strengths in the dense V1_conMat
+     * layout [nblock][maxNear][blockSize][blockSize], zeros included.  When
+     * given, the forward kernels read a spike's 64 strengths for a wavefront
+     * straight from its row (one round trip fewer than rowMask/rowOff/w);
+     * the results are identical.  The adjoint always uses w. */
+    const float *wDense;
 } vc_rec_conn;
 
 /* Spikes of one step ("per-block spike bookkeeping"): one 64-bit spike mask
@@ -172,6 +178,27 @@ int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
               uint32_t cur, const float *lgnSInfo, uint64_t lgnStride,
               uint32_t *ffMask, uint64_t maskStride, uint32_t it0,
               uint32_t nsteps, float *out_sInfo, uint64_t outStride);
+
+/* vc_v1_run in ONE cooperative launch: the neurons' state stays in registers
+ * across the steps and each workgroup waits only for its neighbour blocks'
+ * previous step.  `scratch` (64-byte aligned, vc_v1_persistent_scratch_bytes
+ * long, cleared by the call) holds the tagged per-tile spike records the
+ * workgroups exchange.  Results are identical to
+ * vc_v1_run; the last step's spikes are left in spk2 as vc_v1_run leaves them.  Every wait is
+ * bounded: on a time-out *status (caller-zeroed, sticky) becomes nonzero
+ * and the results of the call are invalid.  Returns VC_E_UNSUPPORTED when
+ * the grid (nV1/256 workgroups) cannot be co-resident on the device; use
+ * vc_v1_run then.  No counterpart in the reference (its host loop launches
+ * each step, src/patch.cu:5966-6913). */
+int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                         const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
+                         uint32_t cur, const float *lgnSInfo, uint64_t lgnStride,
+                         uint32_t *ffMask, uint64_t maskStride, uint32_t it0,
+                         uint32_t nsteps, float *out_sInfo, uint64_t outStride,
+                         void *scratch, uint32_t *status);
+
+/* Bytes of the scratch buffer of vc_v1_run_persistent. */
+uint64_t vc_v1_persistent_scratch_bytes(const vc_v1_params *p);
 
 /* Apply the recurrent increments of `prev` to gE/hE/gI/hI without stepping
  * (used after the last step, before reading or saving the state). */

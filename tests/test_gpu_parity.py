@@ -60,25 +60,32 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("case,events", [(c, True) for c in CASES] + [("tiny", False), ("burst", False)])
-def test_full_path_bitwise(pkg, case, events):
+@pytest.mark.parametrize("case,events,persistent,dense",
+                         [(c, True, True, True) for c in CASES]
+                         + [(c, True, False, False) for c in CASES]
+                         + [("tiny", True, True, False), ("burst", True, False, True),
+                            ("tiny", False, True, True), ("burst", False, True, False),
+                            ("tiny", False, False, False), ("burst", False, False, True)])
+def test_full_path_bitwise(pkg, case, events, persistent, dense):
     import oracle
     from visual_cortex_amd import network
     from visual_cortex_amd.sim import GpuSim
     kw, nsteps = CASES[case]
-    _flush(f"test_full_path_bitwise[{case}, events={events}]: {kw} {nsteps} steps")
+    _flush(f"test_full_path_bitwise[{case}, events={events}, persistent={persistent}, "
+           f"dense={dense}]: {kw} {nsteps} steps")
     net = network.build_network(network.spec_tiny(**kw))
     state = None
     if case == "burst":
         from visual_cortex_amd import params
         state = params.initial_state(net)
         state["v"][:] = -52.0
-    g = GpuSim(net, state=state, ff_events=events)
+    g = GpuSim(net, state=state, ff_events=events, persistent=persistent, dense_weights=dense)
     tables = dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
                   SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
     o = oracle.OracleSim(net, pkg, tables=tables, state=state)
     g.run(nsteps)
     o.run(nsteps)
+    assert g.persistent == persistent          # the requested V1 launch kind ran
     lg, lo = g.lgn_sinfo(), o.lgn_sinfo()
     assert np.array_equal(lg.view(np.uint32), lo.view(np.uint32)), "LGN sInfo differs"
     sg, so = g.sInfo(), o.sInfo()
@@ -110,6 +117,7 @@ def test_c1_first_200_steps_bitwise(pkg):
     o = oracle.OracleSim(net, pkg, tables=tables)
     g.run(200)
     o.run(200)
+    assert g.persistent
     sg, so = g.sInfo(), o.sInfo()
     assert (so >= 1).sum() > 0
     assert np.array_equal(sg.view(np.uint32), so.view(np.uint32))

@@ -15,6 +15,7 @@ LIB_PATH = PKG_DIR / "libvc_hip.so"
 
 VC_MAX_NTYPE = 2
 VC_MAX_NG = 2
+VC_OK, VC_E_INVALID, VC_E_UNSUPPORTED, VC_E_HIP = 0, -1, -2, -3
 VC_SNAP_ROWS = 14          # snapshot rows: v, tBack, gFF[2], hFF[2], gE[2], hE[2], gI[2], hI[2]
 
 f32 = C.c_float
@@ -69,7 +70,7 @@ class FFEvents(C.Structure):
 
 class RecConn(C.Structure):
     _fields_ = [("maxNear", u32), ("nNear", vp), ("nbr", vp), ("rowMask", vp),
-                ("rowOff", vp), ("w", vp)]
+                ("rowOff", vp), ("w", vp), ("wDense", vp)]
 
 
 class Spikes(C.Structure):
@@ -117,6 +118,10 @@ SIGNATURES = {
     "vc_v1_run": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
                   C.POINTER(RecConn), C.POINTER(Spikes), u32, vp, C.c_uint64, vp, C.c_uint64,
                   u32, u32, vp, C.c_uint64],
+    "vc_v1_run_persistent": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
+                             C.POINTER(RecConn), C.POINTER(Spikes), u32, vp, C.c_uint64, vp,
+                             C.c_uint64, u32, u32, vp, C.c_uint64, vp, vp],
+    "vc_v1_persistent_scratch_bytes": [C.POINTER(V1Params)],
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],
@@ -166,7 +171,8 @@ def load_library(require_gpu: bool = True):
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = C.c_char_p if name == "vc_last_error" else C.c_int
+        fn.restype = (C.c_char_p if name == "vc_last_error" else
+                      C.c_uint64 if name.endswith("_bytes") else C.c_int)
     _LIB = lib
     return lib
 

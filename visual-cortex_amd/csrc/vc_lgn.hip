@@ -104,19 +104,30 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
+    // frame of tap i: floor((base + i*interval) / ntPerFrame), walked
+    // incrementally from tap 0 (quotient dd relative to fw, remainder rr)
+    const int64_t q0 = floor_div(base, (int64_t)p.ntPerFrame);
+    const uint32_t rr0 = (uint32_t)(base - q0 * (int64_t)p.ntPerFrame);
+    const int dd0 = (int)(q0 - fw);
+    const uint32_t npf = p.ntPerFrame;
+    const uint32_t dq = p.kernelSampleInterval / npf, dr = p.kernelSampleInterval % npf;
     float conv[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         float a0[32], a1[32];
+        uint32_t rr = rr0;
+        int dd = dd0;
 #pragma unroll
         for (int i = 0; i < kMaxTaps; ++i) {
             float v = 0.0f;
-            if (i < (int)nKS) {
-                const int64_t tau = base + (int64_t)i * p.kernelSampleInterval;
-                const int d = (int)(floor_div(tau, (int64_t)p.ntPerFrame) - fw);
-                v = sR[wv][r][d] * st.TW[((size_t)c * 2 + r) * nKS + i];
-            }
+            if (i < (int)nKS) v = sR[wv][r][dd] * st.TW[((size_t)c * 2 + r) * nKS + i];
             if (i < 32) a0[i] = v; else a1[i - 32] = v;
+            rr += dr;
+            dd += (int)dq;
+            if (rr >= npf) {
+                rr -= npf;
+                dd += 1;
+            }
         }
         const float P0 = tree32_regs(a0);
         const float P1 = tree32_regs(a1);

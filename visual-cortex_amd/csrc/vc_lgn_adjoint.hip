@@ -80,13 +80,23 @@ __global__ __launch_bounds__(kAdjWG) void k_lgn_rate_adj(
     const float C50 = st.c50[c], K = st.sharpness[c], A = st.logA[c];
     const float dfr = p.frRatio * A * K * p.convolRatio;
     float dkC = 0.0f, dkS = 0.0f;
+    const uint32_t dq = (uint32_t)(itv / npf), dr = (uint32_t)(itv % npf);
     for (uint32_t s = tid; s < T; s += blockDim.x) {
         const int64_t u = u0 + s;
+        // frame of tap i, floor((u + i itv) / npf) - Fmin, walked incrementally
+        const int64_t q0 = floor_div(u, npf);
+        uint32_t rr = (uint32_t)(u - q0 * npf);
+        uint32_t f = (uint32_t)(q0 - Fmin);
         float convC = 0.0f, convS = 0.0f;
         for (uint32_t i = 0; i < nKS; ++i) {
-            const uint32_t f = (uint32_t)(floor_div(u + (int64_t)i * itv, npf) - Fmin);
             convC += sR[0][f] * sTW[0][i];
             convS += sR[1][f] * sTW[1][i];
+            rr += dr;
+            f += dq;
+            if (rr >= (uint32_t)npf) {
+                rr -= (uint32_t)npf;
+                f += 1;
+            }
         }
         const float cur = (convC * kC + convS * kS) * p.convolRatio;
         float g = 0.0f;

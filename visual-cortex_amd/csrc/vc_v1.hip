@@ -27,7 +27,8 @@ constexpr int kWG = 256;           // max workgroup size
 constexpr int kMaxEnt = 512;       // neighbour-tile entries staged in LDS per wavefront
 constexpr int kBatch = 64;
 constexpr int kFF = 32;            // LGN synapses loaded per round
-constexpr int kW = 8;              // weight loads in flight per lane
+constexpr int kW = 8;              // weight loads in flight per lane (masked CSR)
+constexpr int kWD = 32;            // weight loads in flight per lane (dense rows)
 constexpr int kEv = 4;             // LGN spike events prefetched per neuron
 
 
@@ -65,33 +66,176 @@ __device__ __noinline__ FFAcc ff_add(FFAcc a, const FFShape sh, float f, float n
 }
 
 
+// Loads and stores of the spike buffers.  Within one launch of the
+// persistent kernel they are exchanged between workgroups that may sit on
+// different XCDs (each with its own L2): kCoh selects agent-coherent
+// accesses (relaxed agent-scope atomics, which bypass or write through the
+// XCD's L2), otherwise ordinary ones (the data was written by an earlier
+// launch).
+template <bool kCoh, typename T>
+__device__ __forceinline__ T ld_spk(const T *p) {
+    if (kCoh) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+template <bool kCoh, typename T>
+__device__ __forceinline__ void st_spk(T *p, T v) {
+    if (kCoh) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+// Spike records of the persistent kernel.  Every 64-bit word carries a
+// 32-bit value and, in its high half, the tag of the step that wrote it, so a
+// reader needs no flag: it polls the words it needs until their tag is the
+// step it expects (each word is written and read whole).  Per 64-neuron tile
+// and step: a record of kRecW words, [0]/[1] the low/high half of the tile's
+// spike mask and [2 + 4r + j] (r < kRecSpk) the increments og0, oh0, og1, oh1
+// (j = 0..3) of its spike of rank r; spikes of rank >= kRecSpk use the
+// overflow words [(tile*64 + rank)*4 + j].
+constexpr int kRecSpk = 3;
+constexpr int kRecW = 16;                    // 64-bit words per record
+constexpr uint32_t kSpinMax = 1u << 20;      // bound of every wait
+
+__device__ __forceinline__ uint64_t tagged(uint32_t v, uint32_t tag) {
+    return ((uint64_t)tag << 32) | v;
+}
+// The values of N tagged words p[0..N-1] once each carries `tag`: all N
+// loads are issued together, then only the words still carrying another tag
+// are loaded again (bounded wait; on a time-out *status is set and the stale
+// values returned).
+template <int N>
+__device__ __forceinline__ void ld_tagged(const uint64_t *p, uint32_t tag, uint32_t *status,
+                                          uint32_t out[N]) {
+    uint64_t v[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = ld_spk<true>(p + j);
+    for (uint32_t spin = 0;; ++spin) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < N; ++j) ok = ok && (uint32_t)(v[j] >> 32) == tag;
+        if (ok) break;
+        if (spin >= kSpinMax || ((spin & 255u) == 255u && ld_spk<true>(status) != 0u)) {
+            atomicOr(status, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if ((uint32_t)(v[j] >> 32) != tag) v[j] = ld_spk<true>(p + j);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = (uint32_t)v[j];
+}
+
+// Adds one spike's increments (strength w) to the calling lane's sums in the
+// reference's order.
+__device__ __forceinline__ void add_spike(const vc_v1_params &p, float strength, uint32_t inh,
+                                          float pE0, float pE1, float pI0, float pI1, float g0,
+                                          float h0, float g1, float h1, float lgE[VC_MAX_NG],
+                                          float lhE[VC_MAX_NG], float lgI[VC_MAX_NG],
+                                          float lhI[VC_MAX_NG]) {
+    if (inh == 0u) {
+        float str = strength * pE0;
+        lgE[0] += str * g0;
+        lhE[0] += str * h0;
+        if (p.ngE > 1) {
+            str = strength * pE1;
+            lgE[1] += str * g1;
+            lhE[1] += str * h1;
+        }
+    } else {
+        float str = strength * pI0;
+        lgI[0] += str * g0;
+        lhI[0] += str * h0;
+        if (p.ngI > 1) {
+            str = strength * pI1;
+            lgI[1] += str * g1;
+            lhI[1] += str * h1;
+        }
+    }
+}
+
+// Spike records being read (persistent kernel): the tagged words and the tag.
+struct RecIn {
+    const uint64_t *words;        // scratch base
+    uint32_t rec, ovf;            // word offsets of the record and overflow buffers read
+    uint32_t tag;
+    uint32_t *status;
+};
+
 // Adds the recurrent increments of a batch of K <= 64 spikes listed in
-// brow/bslot (this wavefront's LDS rows) to the calling lane's sums.
+// brow/bslot (this wavefront's LDS rows) to the calling lane's sums.  With
+// kRec the increments of spike k are the four tagged words at ri.words +
+// bsrc[k]; otherwise they are read from prev.og/oh.
+template <bool kCoh, bool kRec>
 __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_conn &rec,
                                             const vc_spikes &prev, uint32_t K, uint32_t lane,
                                             uint32_t nSeg, uint32_t myseg, int t,
                                             const uint32_t *brow, const uint32_t *bslot,
+                                            const uint32_t *bsrc, const RecIn &ri,
                                             float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
                                             float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
     // lane kk fetches the row-segment header and the increments of spike kk
+    const bool dense = rec.wDense != nullptr;
     uint64_t rm = 0;
-    uint32_t off = 0, isI = 0;
+    uint32_t off = 0, isI = 0, row = 0, slot = 0;
     float og0 = 0.f, oh0 = 0.f, og1 = 0.f, oh1 = 0.f;
     if (lane < K) {
-        const uint32_t row = brow[lane];
+        row = brow[lane];
         const uint32_t sw = bslot[lane];
         isI = sw >> 31;
-        const uint32_t slot = sw & 0x7fffffffu;
-        const size_t hid = (size_t)row * nSeg + myseg;
-        rm = rec.rowMask[hid];
-        off = rec.rowOff[hid];
-        og0 = prev.og[slot];
-        oh0 = prev.oh[slot];
-        og1 = prev.og[p.nV1 + slot];
-        oh1 = prev.oh[p.nV1 + slot];
+        slot = sw & 0x7fffffffu;
+        if (!dense) {
+            const size_t hid = (size_t)row * nSeg + myseg;
+            rm = rec.rowMask[hid];
+            off = rec.rowOff[hid];
+        }
     }
+    // the increments of spike `lane` (after the dense weights are in flight:
+    // with kRec they may have to wait for their tags)
+    auto load_increments = [&]() {
+        if (lane >= K) return;
+        if (kRec) {
+            uint32_t u[4];
+            ld_tagged<4>(ri.words + bsrc[lane], ri.tag, ri.status, u);
+            og0 = __uint_as_float(u[0]);
+            oh0 = __uint_as_float(u[1]);
+            og1 = __uint_as_float(u[2]);
+            oh1 = __uint_as_float(u[3]);
+        } else {
+            og0 = ld_spk<kCoh>(prev.og + slot);
+            oh0 = ld_spk<kCoh>(prev.oh + slot);
+            og1 = ld_spk<kCoh>(prev.og + p.nV1 + slot);
+            oh1 = ld_spk<kCoh>(prev.oh + p.nV1 + slot);
+        }
+    };
     const float pE0 = p.pE[t * p.ngE + 0], pE1 = p.pE[t * p.ngE + (p.ngE > 1 ? 1 : 0)];
     const float pI0 = p.pI[t * p.ngI + 0], pI1 = p.pI[t * p.ngI + (p.ngI > 1 ? 1 : 0)];
+    if (dense) {
+        // a spike's strengths for this wavefront are one coalesced 256-byte
+        // row segment; absent synapses hold +0 and add +0 to the sums
+        const float *wrow = rec.wDense + (size_t)myseg * 64u + lane;
+        const size_t bs = (size_t)nSeg * 64u;
+        for (uint32_t k0 = 0; k0 < K; k0 += kWD) {
+            float wv[kWD];
+#pragma unroll
+            for (int j = 0; j < kWD; ++j) {
+                const uint32_t kk = k0 + j;
+                wv[j] = kk < K ? wrow[(size_t)vc_readlane_u32(row, kk) * bs] : 0.0f;
+            }
+            if (k0 == 0) load_increments();
+#pragma unroll
+            for (int j = 0; j < kWD; ++j) {
+                const uint32_t kk = k0 + j;
+                if (kk < K)
+                    add_spike(p, wv[j], vc_readlane_u32(isI, kk), pE0, pE1, pI0, pI1,
+                              vc_readlane_f32(og0, kk), vc_readlane_f32(oh0, kk),
+                              vc_readlane_f32(og1, kk), vc_readlane_f32(oh1, kk), lgE, lhE, lgI,
+                              lhI);
+            }
+        }
+        return;
+    }
+    load_increments();
     for (uint32_t k0 = 0; k0 < K; k0 += kW) {
         float wv[kW];
 #pragma unroll
@@ -107,313 +251,327 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
 #pragma unroll
         for (int j = 0; j < kW; ++j) {
             const uint32_t kk = k0 + j;
-            if (kk < K) {
-                const uint64_t rmk = vc_readlane_u64(rm, kk);
-                const float g0 = vc_readlane_f32(og0, kk), h0 = vc_readlane_f32(oh0, kk);
-                const float g1 = vc_readlane_f32(og1, kk), h1 = vc_readlane_f32(oh1, kk);
-                const uint32_t inh = vc_readlane_u32(isI, kk);
-                if ((rmk >> lane) & 1ull) {
-                    const float strength = wv[j];
-                    if (inh == 0u) {
-                        float str = strength * pE0;
-                        lgE[0] += str * g0;
-                        lhE[0] += str * h0;
-                        if (p.ngE > 1) {
-                            str = strength * pE1;
-                            lgE[1] += str * g1;
-                            lhE[1] += str * h1;
-                        }
-                    } else {
-                        float str = strength * pI0;
-                        lgI[0] += str * g0;
-                        lhI[0] += str * h0;
-                        if (p.ngI > 1) {
-                            str = strength * pI1;
-                            lgI[1] += str * g1;
-                            lhI[1] += str * h1;
-                        }
-                    }
-                }
-            }
+            if (kk < K && ((vc_readlane_u64(rm, kk) >> lane) & 1ull))
+                add_spike(p, wv[j], vc_readlane_u32(isI, kk), pE0, pE1, pI0, pI1,
+                          vc_readlane_f32(og0, kk), vc_readlane_f32(oh0, kk),
+                          vc_readlane_f32(og1, kk), vc_readlane_f32(oh1, kk), lgE, lhE, lgI, lhI);
         }
     }
 }
 
+// LDS of the workgroup's spike list (see rec_gather).
+constexpr int kList = 512;                   // spikes listed per pass
+struct GatherLds {
+    uint32_t off[kMaxEnt];                   // list position of entry e's first spike
+    uint32_t wsum[kWG / 64];
+    uint32_t row[kList], slot[kList], src[kList];
+};
+
 // Recurrent increments of the spikes of `prev` for the calling neuron.
-// smk/sbid (this wavefront's LDS rows) hold the spike mask and block id of
-// neighbour-tile entry e = ib*nSeg + tt, e < nEnt.  Entries are scanned in
-// order with a ballot over 64 entries at a time; the spikes of non-empty
-// tiles are appended to the batch, which is flushed from one place.
+// smk/sbid (LDS) hold the spike mask and block id of neighbour-tile entry
+// e = ib*nSeg + tt, e < nEnt.  The workgroup lists the spikes once, in entry
+// order and then ascending neuron id (popcounts, a block-wide prefix sum,
+// then every thread writes its entries' spikes), and every wavefront adds
+// the listed spikes' increments for its own 64 posts, 64 spikes per batch.
+// Lists longer than kList are done in several passes.  Every thread of the
+// workgroup must call it.
+template <bool kCoh, bool kRec>
 __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_conn &rec,
                                         const vc_spikes &prev, uint32_t b, uint32_t myseg,
                                         uint32_t nSeg, uint32_t nEnt, int t,
                                         const uint64_t *smk, const uint32_t *sbid,
-                                        uint32_t *brow, uint32_t *bslot,
+                                        const RecIn &ri, GatherLds &g,
                                         float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
                                         float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
-    const uint32_t lane = vc_lane();
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    const uint32_t nw = blockDim.x >> 6;
     const uint32_t bs = p.blockSize;
-    const uint32_t nChunk = (nEnt + 63u) / 64u;
-    uint32_t cnt = 0;                     // entries in the batch (wave-uniform)
-    uint32_t c = 0;                       // next chunk of 64 entries to scan
-    uint64_t nz = 0;                      // non-empty entries left in chunk c-1
-    while (true) {
-        // fill the batch up to 64 spikes or until the entries are exhausted
-        while (cnt < (uint32_t)kBatch) {
-            while (nz == 0ull && c < nChunk) {
-                const uint32_t e = c * 64u + lane;
-                nz = __ballot(e < nEnt && smk[e] != 0ull);
-                ++c;
+    // each thread owns a run of `per` consecutive entries
+    const uint32_t per = (nEnt + blockDim.x - 1u) / blockDim.x;
+    const uint32_t eb = tid * per;
+    uint32_t mine = 0;
+    for (uint32_t i = 0; i < per; ++i)
+        if (eb + i < nEnt) mine += (uint32_t)__builtin_popcountll(smk[eb + i]);
+    uint32_t incl = mine;                                   // inclusive scan in the wavefront
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63u) g.wsum[wv] = incl;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (uint32_t w = 0; w < nw; ++w) {
+        base += w < wv ? g.wsum[w] : 0u;
+        total += g.wsum[w];
+    }
+    {
+        uint32_t o = base + incl - mine;
+        for (uint32_t i = 0; i < per; ++i)
+            if (eb + i < nEnt) {
+                g.off[eb + i] = o;
+                o += (uint32_t)__builtin_popcountll(smk[eb + i]);
             }
-            if (nz == 0ull) break;
-            const uint32_t j = (uint32_t)__builtin_ctzll(nz);
-            nz &= nz - 1ull;
-            const uint32_t e = (c - 1u) * 64u + j;
-            const uint64_t m = smk[e];
+    }
+    for (uint32_t L0 = 0; L0 < total; L0 += (uint32_t)kList) {
+        // list the spikes at positions [L0, L0 + kList)
+        for (uint32_t i = 0; i < per; ++i) {
+            const uint32_t e = eb + i;
+            if (e >= nEnt) break;
+            uint64_t m = smk[e];
+            const uint32_t o = g.off[e];
+            if (m == 0ull || o >= L0 + (uint32_t)kList || o + (uint32_t)__builtin_popcountll(m) <= L0)
+                continue;
             const uint32_t bid = sbid[e];
             const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
-            if ((m >> lane) & 1ull) {
-                const uint32_t rank = vc_mbcnt(m);
-                const uint32_t pos = cnt + rank;
-                const uint32_t pre = tt * 64u + lane;
-                brow[pos] = (b * rec.maxNear + ib) * bs + pre;
-                bslot[pos] = (bid * bs + tt * 64u + rank) | ((pre >= p.nE ? 1u : 0u) << 31);
+            for (uint32_t rank = 0; m != 0ull; ++rank) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1ull;
+                const uint32_t pos = o + rank;
+                if (pos < L0 || pos >= L0 + (uint32_t)kList) continue;
+                const uint32_t pre = tt * 64u + l;
+                g.row[pos - L0] = (b * rec.maxNear + ib) * bs + pre;
+                g.slot[pos - L0] = (bid * bs + tt * 64u + rank) | ((pre >= p.nE ? 1u : 0u) << 31);
+                if (kRec) {
+                    const uint32_t tile = bid * nSeg + tt;
+                    g.src[pos - L0] = rank < (uint32_t)kRecSpk
+                                          ? ri.rec + tile * kRecW + 2u + 4u * rank
+                                          : ri.ovf + (tile * 64u + rank) * 4u;
+                }
             }
-            cnt += (uint32_t)__builtin_popcountll(m);
         }
-        if (cnt == 0u) break;
-        const uint32_t K = cnt < (uint32_t)kBatch ? cnt : (uint32_t)kBatch;
-        __builtin_amdgcn_wave_barrier();
-        flush_batch(p, rec, prev, K, lane, nSeg, myseg, t, brow, bslot, lgE, lhE, lgI, lhI);
-        __builtin_amdgcn_wave_barrier();
-        const uint32_t rest = cnt - K;
-        uint32_t r0 = 0, r1 = 0;
-        if (lane < rest) {
-            r0 = brow[K + lane];
-            r1 = bslot[K + lane];
+        __syncthreads();
+        const uint32_t K = (total - L0) < (uint32_t)kList ? (total - L0) : (uint32_t)kList;
+        for (uint32_t k0 = 0; k0 < K; k0 += (uint32_t)kBatch) {
+            const uint32_t kb = (K - k0) < (uint32_t)kBatch ? (K - k0) : (uint32_t)kBatch;
+            flush_batch<kCoh, kRec>(p, rec, prev, kb, lane, nSeg, myseg, t, g.row + k0,
+                                    g.slot + k0, g.src + k0, ri, lgE, lhE, lgI, lhI);
         }
-        __builtin_amdgcn_wave_barrier();
-        if (lane < rest) {
-            brow[lane] = r0;
-            bslot[lane] = r1;
-        }
-        __builtin_amdgcn_wave_barrier();
-        cnt = rest;
+        __syncthreads();                    // the list is rewritten by the next pass
     }
 }
 
-template <bool kUpdate, bool kEvents>
-__global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v1_state s,
-                                                 vc_ff_conn ff, vc_rec_conn rec,
-                                                 vc_spikes prev, vc_spikes next,
-                                                 const float *__restrict__ lgnSInfo,
-                                                 uint32_t *__restrict__ ffMask,
-                                                 uint32_t step, float *__restrict__ out_sInfo,
-                                                 float *__restrict__ out_dep,
-                                                 float *__restrict__ out_g_prev,
-                                                 float *__restrict__ out_snap) {
-    __shared__ uint32_t sbrow[kWG / 64][2 * kBatch];
-    __shared__ uint32_t sbslot[kWG / 64][2 * kBatch];
-    __shared__ uint64_t smk[kMaxEnt];          // neighbour-tile masks of this block (all
-    __shared__ uint32_t sbid[kMaxEnt];         // wavefronts of a workgroup share the block)
-    const uint32_t nV1 = p.nV1;
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t b = n / p.blockSize;
-    const uint32_t local = n - b * p.blockSize;
-    const uint32_t lane = vc_lane();
-    const uint32_t wv = threadIdx.x >> 6;
-    const int t = type_of(p, local);
-    const float dt = p.dt;
-    const uint32_t nSeg = k.nSeg;
-    const uint32_t myseg = local >> 6;
+// ---------------------------------------------------- per-neuron pieces --
+// Shared by the one-step kernel and the persistent multi-step kernel.
 
-    // ---- round 1: neighbour spike masks (lane-parallel), LGN synapses, state
-    const uint32_t nb = rec.nNear[b];
-    const uint32_t nEnt = nb * nSeg;
-    for (uint32_t e = threadIdx.x; e < nEnt; e += blockDim.x) {
-        const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
-        const uint32_t bid = rec.nbr[b * rec.maxNear + ib];
-        smk[e] = prev.mask[bid * nSeg + tt];
-        sbid[e] = bid;
-    }
-    constexpr bool events = kUpdate && kEvents;
-    constexpr int nFF = (kUpdate && !kEvents) ? kFF : 1;    // ELL synapses held in registers
-    const uint32_t mLGN = (kUpdate && !events) ? ff.nLGN[n] : 0u;
-    const uint32_t ffBits = events ? ffMask[n] : 0u;
-    float fv[nFF];
-    uint32_t fid[nFF];
-#pragma unroll
-    for (int j = 0; j < nFF; ++j) {
-        fv[j] = 0.0f;
-        fid[j] = 0u;
-        if ((uint32_t)j < mLGN) {
-            const uint32_t lid = (uint32_t)j * nV1 + n;
-            fv[j] = ff.sLGN[lid];
-            fid[j] = ff.idLGN[lid];
-        }
-    }
-    float ge[VC_MAX_NG], he[VC_MAX_NG], gi[VC_MAX_NG], hi[VC_MAX_NG];
+struct NState {                      // one neuron's state between steps
+    float v, w, tBack;
+    float gFF[VC_MAX_NG], hFF[VC_MAX_NG];
+    float gE[VC_MAX_NG], hE[VC_MAX_NG], gI[VC_MAX_NG], hI[VC_MAX_NG];
+};
+
+__device__ __forceinline__ void load_cortical(const vc_v1_params &p, const vc_v1_state &s,
+                                              uint32_t n, NState &x) {
+    const uint32_t nV1 = p.nV1;
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-        ge[ig] = he[ig] = gi[ig] = hi[ig] = 0.0f;
+        x.gE[ig] = x.hE[ig] = x.gI[ig] = x.hI[ig] = 0.0f;
         if (ig < p.ngE) {
-            ge[ig] = s.gE[ig * nV1 + n];
-            he[ig] = s.hE[ig * nV1 + n];
+            x.gE[ig] = s.gE[ig * nV1 + n];
+            x.hE[ig] = s.hE[ig * nV1 + n];
         }
         if (ig < p.ngI) {
-            gi[ig] = s.gI[ig * nV1 + n];
-            hi[ig] = s.hI[ig * nV1 + n];
+            x.gI[ig] = s.gI[ig * nV1 + n];
+            x.hI[ig] = s.hI[ig * nV1 + n];
         }
     }
+}
 
-    // ---- round 2: LGN spike words, then the recurrent gather
-    float sv[nFF];
+__device__ __forceinline__ void load_rest(const vc_v1_params &p, const vc_v1_state &s,
+                                          uint32_t n, NState &x) {
+    const uint32_t nV1 = p.nV1;
+    x.v = s.v[n];
+    x.w = s.w[n];
+    x.tBack = s.tBack[n];
 #pragma unroll
-    for (int j = 0; j < nFF; ++j) sv[j] = ((uint32_t)j < mLGN) ? lgnSInfo[fid[j]] : 0.0f;
-    // event path: the first kEv synapses that received a spike this step
-    constexpr int nEv = events ? kEv : 1;
-    uint32_t evj[nEv];
-    float evf[nEv];
-    uint32_t evid[nEv];
-    {
-        uint32_t bits = ffBits;
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        x.gFF[ig] = ig < p.ngFF ? s.gFF[ig * nV1 + n] : 0.0f;
+        x.hFF[ig] = ig < p.ngFF ? s.hFF[ig * nV1 + n] : 0.0f;
+    }
+}
+
+__device__ __forceinline__ void store_state(const vc_v1_params &p, const vc_v1_state &s,
+                                            uint32_t n, const NState &x) {
+    const uint32_t nV1 = p.nV1;
+    s.v[n] = x.v;
+    if (p.iModel == 1) s.w[n] = x.w;
+    s.tBack[n] = x.tBack;
+    for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+        s.gFF[ig * nV1 + n] = x.gFF[ig];
+        s.hFF[ig * nV1 + n] = x.hFF[ig];
+    }
+    for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+        s.gE[ig * nV1 + n] = x.gE[ig];
+        s.hE[ig * nV1 + n] = x.hE[ig];
+    }
+    for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+        s.gI[ig * nV1 + n] = x.gI[ig];
+        s.hI[ig * nV1 + n] = x.hI[ig];
+    }
+}
+
+// tonic drive of the AdEx constructor, src/coredynamics.cuh:40-49
+__device__ __forceinline__ float dep_current(const vc_v1_params &p, int t, float dep) {
+    const float gL = p.gL[t], vT = p.vT[t], deltaT = p.deltaT[t], a = p.a[t];
+    const float targetV = VC_VL + (vT - VC_VL) * dep;
+    if (targetV > vT) return (gL + a) * (targetV - VC_VL) - gL * deltaT;
+    if (deltaT > 0.0f)
+        return (gL + a) * (targetV - VC_VL) - gL * deltaT * vc_expf((targetV - vT) / deltaT);
+    return (gL + a) * (targetV - VC_VL) - 0.0f;
+}
+
+// LGN input of one neuron, loaded in two rounds so that other loads overlap.
+// Event path: the neuron's word of synapse bits whose cell spiked; ELL path:
+// the first kFF synapses (strength, cell) and their cells' spike words.
+template <bool kEvents> struct FFPre;
+template <> struct FFPre<true> {
+    uint32_t bits;
+    uint32_t evj[kEv], evid[kEv];
+    float evf[kEv];
+};
+template <> struct FFPre<false> {
+    uint32_t m;
+    float fv[kFF], sv[kFF];
+    uint32_t fid[kFF];
+};
+
+__device__ __forceinline__ void ff_round1(FFPre<true> &f, const vc_ff_conn &, const uint32_t *ffMask,
+                                          uint32_t n, uint32_t) {
+    f.bits = ffMask[n];
+}
+__device__ __forceinline__ void ff_round1(FFPre<false> &f, const vc_ff_conn &ff, const uint32_t *,
+                                          uint32_t n, uint32_t nV1) {
+    f.m = ff.nLGN[n];
 #pragma unroll
-        for (int k = 0; k < nEv; ++k) {
-            evj[k] = 0xffffffffu;
-            evf[k] = 0.0f;
-            evid[k] = 0u;
-            if (bits) {
-                const uint32_t j = (uint32_t)__builtin_ctz(bits);
-                bits &= bits - 1u;
-                const uint32_t lid = j * nV1 + n;
-                evj[k] = j;
-                evf[k] = ff.sLGN[lid];
-                evid[k] = ff.idLGN[lid];
+    for (int j = 0; j < kFF; ++j) {
+        f.fv[j] = 0.0f;
+        f.fid[j] = 0u;
+        if ((uint32_t)j < f.m) {
+            const uint32_t lid = (uint32_t)j * nV1 + n;
+            f.fv[j] = ff.sLGN[lid];
+            f.fid[j] = ff.idLGN[lid];
+        }
+    }
+}
+// events: the first kEv synapses that received a spike this step
+__device__ __forceinline__ void ff_round2(FFPre<true> &f, const vc_ff_conn &ff, const float *,
+                                          uint32_t n, uint32_t nV1) {
+    uint32_t bits = f.bits;
+#pragma unroll
+    for (int k = 0; k < kEv; ++k) {
+        f.evj[k] = 0xffffffffu;
+        f.evf[k] = 0.0f;
+        f.evid[k] = 0u;
+        if (bits) {
+            const uint32_t j = (uint32_t)__builtin_ctz(bits);
+            bits &= bits - 1u;
+            const uint32_t lid = j * nV1 + n;
+            f.evj[k] = j;
+            f.evf[k] = ff.sLGN[lid];
+            f.evid[k] = ff.idLGN[lid];
+        }
+    }
+}
+__device__ __forceinline__ void ff_round2(FFPre<false> &f, const vc_ff_conn &, const float *lgnSInfo,
+                                          uint32_t, uint32_t) {
+#pragma unroll
+    for (int j = 0; j < kFF; ++j) f.sv[j] = ((uint32_t)j < f.m) ? lgnSInfo[f.fid[j]] : 0.0f;
+}
+
+// LGN synapses in list order; only synapses whose cell spiked (sInfo >= 1)
+// do work (reference: nsp_FF > 0).
+__device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, FFPre<true> &f,
+                                          const vc_ff_conn &ff, const float *lgnSInfo,
+                                          uint32_t *ffMask, uint32_t n, uint32_t nV1, float dt,
+                                          float tBack, bool backing) {
+    if (__ballot(f.bits != 0u) == 0ull) return;
+    float evs[kEv];
+#pragma unroll
+    for (int k = 0; k < kEv; ++k) evs[k] = (f.evj[k] != 0xffffffffu) ? lgnSInfo[f.evid[k]] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kEv; ++k) {
+        if (f.evj[k] != 0xffffffffu) {
+            const float sI = evs[k];
+            const float nsp = (float)(uint32_t)floorf(sI);
+            acc = ff_add(acc, shFF, f.evf[k], nsp, (sI - nsp) * dt, dt, tBack, backing);
+        }
+    }
+    uint32_t rest = f.bits;
+#pragma unroll
+    for (int k = 0; k < kEv; ++k) rest &= rest - 1u;        // drop the prefetched bits
+    while (rest) {
+        const uint32_t j = (uint32_t)__builtin_ctz(rest);
+        rest &= rest - 1u;
+        const uint32_t lid = j * nV1 + n;
+        const float sI = lgnSInfo[ff.idLGN[lid]];
+        const float nsp = (float)(uint32_t)floorf(sI);
+        acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
+    }
+    if (f.bits) ffMask[n] = 0u;
+}
+__device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, FFPre<false> &f,
+                                          const vc_ff_conn &ff, const float *lgnSInfo, uint32_t *,
+                                          uint32_t n, uint32_t nV1, float dt, float tBack,
+                                          bool backing) {
+    uint32_t spk = 0;
+#pragma unroll
+    for (int j = 0; j < kFF; ++j) spk |= (f.sv[j] >= 1.0f ? 1u : 0u) << j;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) spk |= (uint32_t)__shfl_xor((int)spk, off);
+    spk = __builtin_amdgcn_readfirstlane(spk);           // slots that spiked for any lane
+#pragma unroll
+    for (int j = 0; j < kFF; ++j) {
+        if ((spk >> j) & 1u) {
+            const float sI = f.sv[j];
+            if (sI >= 1.0f) {
+                const float nsp = (float)(uint32_t)floorf(sI);
+                acc = ff_add(acc, shFF, f.fv[j], nsp, (sI - nsp) * dt, dt, tBack, backing);
             }
         }
     }
+    for (uint32_t i = kFF; i < f.m; ++i) {                // lists longer than kFF
+        const uint32_t lid = i * nV1 + n;
+        const float sI = lgnSInfo[ff.idLGN[lid]];
+        if (sI >= 1.0f) {
+            const float nsp = (float)(uint32_t)floorf(sI);
+            acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
+        }
+    }
+}
 
-    float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
-    float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
-    __syncthreads();
-    rec_gather(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, sbrow[wv], sbslot[wv],
-               lgE, lhE, lgI, lhI);
-
-    // conductances at the end of the previous step (reference: after recal_G)
+// Stage c-FF of one neuron: the LGN and background inputs into the FF
+// receptors at the end of the step (g1, h1) and, when backing up from
+// refractoriness (0 < tBack < dt), at tBack (g0, h0).  Depends on the
+// neuron's own state only, not on the other neurons' spikes.
+template <bool kEvents>
+__device__ __forceinline__ FFAcc ff_stage(const vc_v1_params &p, const V1Const &k,
+                                          const vc_ff_conn &ff, int t, uint32_t n, uint32_t b,
+                                          uint32_t step, const NState &x, FFPre<kEvents> &f,
+                                          const float *lgnSInfo, uint32_t *ffMask) {
+    const float dt = p.dt;
+    const float tBack = x.tBack;
+    FFAcc acc;
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-        ge[ig] = ge[ig] + lgE[ig];
-        he[ig] = he[ig] + lhE[ig];
-        gi[ig] = gi[ig] + lgI[ig];
-        hi[ig] = hi[ig] + lhI[ig];
-    }
-    if (out_snap) {                 // state at the start of the step, cortical g/h incremented
-        out_snap[VC_SNAP_V * nV1 + n] = s.v[n];
-        out_snap[VC_SNAP_TBACK * nV1 + n] = s.tBack[n];
-        for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
-            out_snap[(VC_SNAP_GFF + ig) * nV1 + n] = s.gFF[ig * nV1 + n];
-            out_snap[(VC_SNAP_HFF + ig) * nV1 + n] = s.hFF[ig * nV1 + n];
-        }
-        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
-            out_snap[(VC_SNAP_GE + ig) * nV1 + n] = ge[ig];
-            out_snap[(VC_SNAP_HE + ig) * nV1 + n] = he[ig];
-        }
-        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
-            out_snap[(VC_SNAP_GI + ig) * nV1 + n] = gi[ig];
-            out_snap[(VC_SNAP_HI + ig) * nV1 + n] = hi[ig];
-        }
-    }
-    if (out_g_prev) {
-        const uint32_t ng = p.ngE + p.ngI;
-        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
-            out_g_prev[ig * nV1 + n] = ge[ig];
-            out_g_prev[(ng + ig) * nV1 + n] = he[ig];
-        }
-        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
-            out_g_prev[(p.ngE + ig) * nV1 + n] = gi[ig];
-            out_g_prev[(ng + p.ngE + ig) * nV1 + n] = hi[ig];
-        }
-    }
-    if (!kUpdate) {
-        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
-            s.gE[ig * nV1 + n] = ge[ig];
-            s.hE[ig * nV1 + n] = he[ig];
-        }
-        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
-            s.gI[ig * nV1 + n] = gi[ig];
-            s.hI[ig * nV1 + n] = hi[ig];
-        }
-        return;
-    }
-
-    // ---- stage d: model set-up (AdEx constructor, src/coredynamics.cuh:40-49)
-    const float vR = p.vR[t], vThres = p.vThres[t], gL = p.gL[t], C = p.C[t];
-    const float tRef = p.tRef[t], vT = p.vT[t], deltaT = p.deltaT[t];
-    const float tau_w = p.tau_w[t], a = p.a[t], bb = p.b[t];
-    float v0 = s.v[n];
-    float w0 = s.w[n];
-    float tBack = s.tBack[n];
-    const float dep = s.tonicDep[n];
-    float depC;
-    {
-        float targetV = VC_VL + (vT - VC_VL) * dep;
-        if (targetV > vT) {
-            depC = (gL + a) * (targetV - VC_VL) - gL * deltaT;
-        } else if (deltaT > 0.0f) {
-            depC = (gL + a) * (targetV - VC_VL) - gL * deltaT * vc_expf((targetV - vT) / deltaT);
-        } else {
-            depC = (gL + a) * (targetV - VC_VL) - 0.0f;
-        }
-    }
-
-    float gE_t1 = 0.0f, gI_t1 = 0.0f;
-#pragma unroll
-    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-        if (ig < p.ngE) {
-            float g = ge[ig], h = he[ig];
-            decay_const(g, h, k.etrE[ig], k.etdE[ig], p.dodE[ig]);
-            s.gE[ig * nV1 + n] = g;
-            s.hE[ig * nV1 + n] = h;
-            gE_t1 += g;
-        }
-    }
-#pragma unroll
-    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-        if (ig < p.ngI) {
-            float g = gi[ig], h = hi[ig];
-            decay_const(g, h, k.etrI[ig], k.etdI[ig], p.dodI[ig]);
-            s.gI[ig * nV1 + n] = g;
-            s.hI[ig * nV1 + n] = h;
-            gI_t1 += g;
-        }
-    }
-    float g0[VC_MAX_NG], h0[VC_MAX_NG], g1[VC_MAX_NG], h1[VC_MAX_NG];
-#pragma unroll
-    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-        g0[ig] = h0[ig] = g1[ig] = h1[ig] = 0.0f;
+        acc.g0[ig] = acc.h0[ig] = acc.g1[ig] = acc.h1[ig] = 0.0f;
         if (ig < p.ngFF) {
-            g1[ig] = s.gFF[ig * nV1 + n];
-            h1[ig] = s.hFF[ig * nV1 + n];
-            g0[ig] = g1[ig];
-            h0[ig] = h1[ig];
-            decay_const(g1[ig], h1[ig], k.etrFF[ig], k.etdFF[ig], p.dodFF[ig]);
+            acc.g1[ig] = x.gFF[ig];
+            acc.h1[ig] = x.hFF[ig];
+            acc.g0[ig] = acc.g1[ig];
+            acc.h0[ig] = acc.h1[ig];
+            decay_const(acc.g1[ig], acc.h1[ig], k.etrFF[ig], k.etdFF[ig], p.dodFF[ig]);
         }
     }
     const bool backing = (tBack < dt) && (tBack > 0.0f);
-    if (out_dep) out_dep[n] = depC;
     if (backing) {
 #pragma unroll
         for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
-            if (ig < p.ngFF) decay_var(g0[ig], h0[ig], tBack, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
+            if (ig < p.ngFF)
+                decay_var(acc.g0[ig], acc.h0[ig], tBack, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
     }
-
-    // ---- stage c-FF: LGN synapses in list order; only synapses whose LGN
-    // cell spiked (sInfo >= 1) do work (reference: nsp_FF > 0).
-    FFAcc acc;
     FFShape shFF, shBG;
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-        acc.g0[ig] = g0[ig]; acc.h0[ig] = h0[ig]; acc.g1[ig] = g1[ig]; acc.h1[ig] = h1[ig];
         const uint32_t pi = t * p.ngFF + (ig < p.ngFF ? ig : 0u);
         shFF.p[ig] = p.pFF[pi];
         shBG.p[ig] = p.pBG[pi];
@@ -422,58 +580,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         shFF.dod[ig] = shBG.dod[ig] = p.dodFF[ig];
     }
     shFF.ng = shBG.ng = p.ngFF;
-    if (events && __ballot(ffBits != 0u) != 0ull) {
-        // synapses whose LGN cell spiked, in synapse order
-        float evs[nEv];
-#pragma unroll
-        for (int k = 0; k < nEv; ++k) evs[k] = (evj[k] != 0xffffffffu) ? lgnSInfo[evid[k]] : 0.0f;
-#pragma unroll
-        for (int k = 0; k < nEv; ++k) {
-            if (evj[k] != 0xffffffffu) {
-                const float sI = evs[k];
-                const float nsp = (float)(uint32_t)floorf(sI);
-                acc = ff_add(acc, shFF, evf[k], nsp, (sI - nsp) * dt, dt, tBack, backing);
-            }
-        }
-        uint32_t rest = ffBits;
-#pragma unroll
-        for (int k = 0; k < nEv; ++k) rest &= rest - 1u;    // drop the prefetched bits
-        while (rest) {
-            const uint32_t j = (uint32_t)__builtin_ctz(rest);
-            rest &= rest - 1u;
-            const uint32_t lid = j * nV1 + n;
-            const float sI = lgnSInfo[ff.idLGN[lid]];
-            const float nsp = (float)(uint32_t)floorf(sI);
-            acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
-        }
-        if (ffBits) ffMask[n] = 0u;
-    }
-    if (!events) {
-        uint32_t spk = 0;
-#pragma unroll
-        for (int j = 0; j < nFF; ++j) spk |= (sv[j] >= 1.0f ? 1u : 0u) << j;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) spk |= (uint32_t)__shfl_xor((int)spk, off);
-        spk = __builtin_amdgcn_readfirstlane(spk);     // slots that spiked for any lane
-#pragma unroll
-        for (int j = 0; j < nFF; ++j) {
-            if ((spk >> j) & 1u) {
-                const float sI = sv[j];
-                if (sI >= 1.0f) {
-                    const float nsp = (float)(uint32_t)floorf(sI);
-                    acc = ff_add(acc, shFF, fv[j], nsp, (sI - nsp) * dt, dt, tBack, backing);
-                }
-            }
-        }
-    }
-    for (uint32_t i = nFF; i < mLGN; ++i) {            // lists longer than kFF
-        const uint32_t lid = i * nV1 + n;
-        const float sI = lgnSInfo[ff.idLGN[lid]];
-        if (sI >= 1.0f) {
-            const float nsp = (float)(uint32_t)floorf(sI);
-            acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
-        }
-    }
+    ff_inputs(acc, shFF, f, ff, lgnSInfo, ffMask, n, p.nV1, dt, tBack, backing);
     // background Poisson drive (extension): one more input after the LGN list
     if (p.bgStrength[t] > 0.0f) {
         uint32_t klo = p.seed_lo, khi = p.seed_hi, ent = n;
@@ -490,37 +597,80 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         if (nsp > 0u)
             acc = ff_add(acc, shBG, p.bgStrength[t], (float)nsp, frac * dt, dt, tBack, backing);
     }
+    return acc;
+}
+
+// Stage d of one neuron: x holds the state at the start of the step with the
+// cortical g/h already incremented by the previous step's spikes, acc the FF
+// receptors of ff_stage; on return x holds the state after the step.
+// Returns sInfo.
+__device__ __forceinline__ float membrane_stage(const vc_v1_params &p, const V1Const &k, int t,
+                                                float depC, NState &x, const FFAcc &acc) {
+    const float dt = p.dt;
+    const float vR = p.vR[t], vThres = p.vThres[t], gL = p.gL[t], C = p.C[t];
+    const float tRef = p.tRef[t], vT = p.vT[t], deltaT = p.deltaT[t];
+    const float tau_w = p.tau_w[t], a = p.a[t], bb = p.b[t];
+    const float v0 = x.v;
+    float w0 = x.w;
+    float tBack = x.tBack;
+    const bool backing = (tBack < dt) && (tBack > 0.0f);
+    // cortical conductances decayed to the end of the step (the next state)
+    float geN[VC_MAX_NG], heN[VC_MAX_NG], giN[VC_MAX_NG], hiN[VC_MAX_NG];
+    float gE_t1 = 0.0f, gI_t1 = 0.0f;
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-        g0[ig] = acc.g0[ig]; h0[ig] = acc.h0[ig]; g1[ig] = acc.g1[ig]; h1[ig] = acc.h1[ig];
+        float g = x.gE[ig], h = x.hE[ig];
+        if (ig < p.ngE) {
+            decay_const(g, h, k.etrE[ig], k.etdE[ig], p.dodE[ig]);
+            gE_t1 += g;
+        }
+        geN[ig] = g;
+        heN[ig] = h;
+    }
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        float g = x.gI[ig], h = x.hI[ig];
+        if (ig < p.ngI) {
+            decay_const(g, h, k.etrI[ig], k.etdI[ig], p.dodI[ig]);
+            gI_t1 += g;
+        }
+        giN[ig] = g;
+        hiN[ig] = h;
     }
     float gE_t0 = 0.0f;
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
         if (ig < p.ngFF) {
-            gE_t0 += g0[ig];
-            gE_t1 += g1[ig];
-            s.gFF[ig * nV1 + n] = g1[ig];
-            s.hFF[ig * nV1 + n] = h1[ig];
+            gE_t0 += acc.g0[ig];
+            gE_t1 += acc.g1[ig];
+            x.gFF[ig] = acc.g1[ig];
+            x.hFF[ig] = acc.h1[ig];
         }
     }
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
         if (ig < p.ngE) {
-            if (backing) decay_var(ge[ig], he[ig], tBack, p.riseE[ig], p.decayE[ig], p.dodE[ig]);
-            gE_t0 += ge[ig];
+            if (backing) decay_var(x.gE[ig], x.hE[ig], tBack, p.riseE[ig], p.decayE[ig], p.dodE[ig]);
+            gE_t0 += x.gE[ig];
         }
     }
     float gI_t0 = 0.0f;
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
         if (ig < p.ngI) {
-            if (backing) decay_var(gi[ig], hi[ig], tBack, p.riseI[ig], p.decayI[ig], p.dodI[ig]);
-            gI_t0 += gi[ig];
+            if (backing) decay_var(x.gI[ig], x.hI[ig], tBack, p.riseI[ig], p.decayI[ig], p.dodI[ig]);
+            gI_t0 += x.gI[ig];
         }
     }
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        x.gE[ig] = geN[ig];
+        x.hE[ig] = heN[ig];
+        x.gI[ig] = giN[ig];
+        x.hI[ig] = hiN[ig];
+    }
 
-    // ---- stage d: Heun RK2 (src/coredynamics.cu:1534-1589)
+    // ---- Heun RK2 (src/coredynamics.cu:1534-1589)
     float v = v0, w = w0, tsp = 0.0f;
     bool spiked = false;
     if (tBack < dt) {
@@ -566,43 +716,289 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         w = (w0 - A) * vc_expf(-(dt - tsp) / tau_w) + A;
         tBack -= dt;
     }
-    const float sInfo = spiked ? 1.0f + tsp / dt : v;
-    s.v[n] = v;
-    if (p.iModel == 1) s.w[n] = w;
-    s.tBack[n] = tBack;
-    out_sInfo[n] = sInfo;
+    x.v = v;
+    if (p.iModel == 1) x.w = w;
+    x.tBack = tBack;
+    return spiked ? 1.0f + tsp / dt : v;
+}
 
-    // ---- stage e: ballot compaction (src/coredynamics.cu:1994-2046)
+// Spike records being written (persistent kernel).
+struct RecOut {
+    uint64_t *words;              // scratch base
+    uint32_t rec, ovf;            // word offsets of the record and overflow buffers written
+    uint32_t tag;
+};
+
+// Stage e: ballot compaction -- one spike mask per tile and the unit
+// increments at slot tile*64 + rank (src/coredynamics.cu:1994-2046).  With
+// kRec (persistent kernel) the mask and the increments also go to the tile's
+// tagged record words.
+template <bool kRec>
+__device__ __forceinline__ void compact(const vc_v1_params &p, const vc_spikes &next,
+                                        const RecOut &ro, uint32_t n, uint32_t local,
+                                        uint32_t lane, float sInfo) {
+    const uint32_t nV1 = p.nV1;
+    const float dt = p.dt;
     const bool exist = sInfo >= 1.0f;
     const uint64_t m = __ballot(exist);
     const uint32_t tile = n >> 6;
-    if (lane == 0) next.mask[tile] = m;
+    if (lane == 0) {
+        next.mask[tile] = m;
+        if (kRec) {
+            uint64_t *r = ro.words + ro.rec + (size_t)tile * kRecW;
+            st_spk<true>(r, tagged((uint32_t)m, ro.tag));
+            st_spk<true>(r + 1, tagged((uint32_t)(m >> 32), ro.tag));
+        }
+    }
     if (exist) {
         const float nsp = floorf(sInfo);
         const float tsp2 = (sInfo - nsp) * dt;
-        const uint32_t slot = tile * 64u + vc_mbcnt(m);
-        if (local < p.nE) {
+        const uint32_t rank = vc_mbcnt(m);
+        const uint32_t slot = tile * 64u + rank;
+        const bool isE = local < p.nE;
+        const uint32_t ng = isE ? p.ngE : p.ngI;
+        float lg[VC_MAX_NG], lh[VC_MAX_NG];
 #pragma unroll
-            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-                if (ig < p.ngE) {
-                    float lg = 0.0f, lh = 0.0f;
-                    single_input(lg, lh, nsp, dt - tsp2, p.riseE[ig], p.decayE[ig], p.dodE[ig]);
-                    next.og[ig * nV1 + slot] = lg;
-                    next.oh[ig * nV1 + slot] = lh;
-                }
-            }
-        } else {
-#pragma unroll
-            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-                if (ig < p.ngI) {
-                    float lg = 0.0f, lh = 0.0f;
-                    single_input(lg, lh, nsp, dt - tsp2, p.riseI[ig], p.decayI[ig], p.dodI[ig]);
-                    next.og[ig * nV1 + slot] = lg;
-                    next.oh[ig * nV1 + slot] = lh;
-                }
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+            lg[ig] = lh[ig] = 0.0f;
+            if (ig < ng) {
+                if (isE)
+                    single_input(lg[ig], lh[ig], nsp, dt - tsp2, p.riseE[ig], p.decayE[ig],
+                                 p.dodE[ig]);
+                else
+                    single_input(lg[ig], lh[ig], nsp, dt - tsp2, p.riseI[ig], p.decayI[ig],
+                                 p.dodI[ig]);
+                next.og[ig * nV1 + slot] = lg[ig];
+                next.oh[ig * nV1 + slot] = lh[ig];
             }
         }
+        if (kRec) {
+            uint64_t *w = ro.words + (rank < (uint32_t)kRecSpk
+                                          ? ro.rec + (size_t)tile * kRecW + 2u + 4u * rank
+                                          : ro.ovf + ((size_t)tile * 64u + rank) * 4u);
+            st_spk<true>(w + 0, tagged(__float_as_uint(lg[0]), ro.tag));
+            st_spk<true>(w + 1, tagged(__float_as_uint(lh[0]), ro.tag));
+            st_spk<true>(w + 2, tagged(__float_as_uint(lg[1]), ro.tag));
+            st_spk<true>(w + 3, tagged(__float_as_uint(lh[1]), ro.tag));
+        }
     }
+}
+
+// ------------------------------------------------------ one-step kernel --
+template <bool kUpdate, bool kEvents>
+__global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v1_state s,
+                                                 vc_ff_conn ff, vc_rec_conn rec,
+                                                 vc_spikes prev, vc_spikes next,
+                                                 const float *__restrict__ lgnSInfo,
+                                                 uint32_t *__restrict__ ffMask,
+                                                 uint32_t step, float *__restrict__ out_sInfo,
+                                                 float *__restrict__ out_dep,
+                                                 float *__restrict__ out_g_prev,
+                                                 float *__restrict__ out_snap) {
+    __shared__ GatherLds glds;
+    __shared__ uint64_t smk[kMaxEnt];          // neighbour-tile masks of this block (all
+    __shared__ uint32_t sbid[kMaxEnt];         // wavefronts of a workgroup share the block)
+    const uint32_t nV1 = p.nV1;
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = n / p.blockSize;
+    const uint32_t local = n - b * p.blockSize;
+    const uint32_t lane = vc_lane();
+    const uint32_t wv = threadIdx.x >> 6;
+    const int t = type_of(p, local);
+    const uint32_t nSeg = k.nSeg;
+    const uint32_t myseg = local >> 6;
+
+    // ---- round 1: neighbour spike masks (lane-parallel), LGN synapses, state
+    const uint32_t nb = rec.nNear[b];
+    const uint32_t nEnt = nb * nSeg;
+    for (uint32_t e = threadIdx.x; e < nEnt; e += blockDim.x) {
+        const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
+        const uint32_t bid = rec.nbr[b * rec.maxNear + ib];
+        smk[e] = prev.mask[bid * nSeg + tt];
+        sbid[e] = bid;
+    }
+    FFPre<kEvents> f;
+    if (kUpdate) ff_round1(f, ff, ffMask, n, nV1);
+    NState x;
+    load_cortical(p, s, n, x);
+    // ---- round 2: LGN spike words, then the recurrent gather
+    if (kUpdate) ff_round2(f, ff, lgnSInfo, n, nV1);
+    float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
+    float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
+    __syncthreads();
+    const RecIn noRec{nullptr, 0u, 0u, 0u, nullptr};
+    rec_gather<false, false>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, noRec, glds,
+                             lgE, lhE, lgI, lhI);
+    // conductances at the end of the previous step (reference: after recal_G)
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        x.gE[ig] = x.gE[ig] + lgE[ig];
+        x.hE[ig] = x.hE[ig] + lhE[ig];
+        x.gI[ig] = x.gI[ig] + lgI[ig];
+        x.hI[ig] = x.hI[ig] + lhI[ig];
+    }
+    if (out_snap) {                 // state at the start of the step, cortical g/h incremented
+        out_snap[VC_SNAP_V * nV1 + n] = s.v[n];
+        out_snap[VC_SNAP_TBACK * nV1 + n] = s.tBack[n];
+        for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+            out_snap[(VC_SNAP_GFF + ig) * nV1 + n] = s.gFF[ig * nV1 + n];
+            out_snap[(VC_SNAP_HFF + ig) * nV1 + n] = s.hFF[ig * nV1 + n];
+        }
+        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+            out_snap[(VC_SNAP_GE + ig) * nV1 + n] = x.gE[ig];
+            out_snap[(VC_SNAP_HE + ig) * nV1 + n] = x.hE[ig];
+        }
+        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+            out_snap[(VC_SNAP_GI + ig) * nV1 + n] = x.gI[ig];
+            out_snap[(VC_SNAP_HI + ig) * nV1 + n] = x.hI[ig];
+        }
+    }
+    if (out_g_prev) {
+        const uint32_t ng = p.ngE + p.ngI;
+        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+            out_g_prev[ig * nV1 + n] = x.gE[ig];
+            out_g_prev[(ng + ig) * nV1 + n] = x.hE[ig];
+        }
+        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+            out_g_prev[(p.ngE + ig) * nV1 + n] = x.gI[ig];
+            out_g_prev[(ng + p.ngE + ig) * nV1 + n] = x.hI[ig];
+        }
+    }
+    if (!kUpdate) {
+        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+            s.gE[ig * nV1 + n] = x.gE[ig];
+            s.hE[ig * nV1 + n] = x.hE[ig];
+        }
+        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+            s.gI[ig * nV1 + n] = x.gI[ig];
+            s.hI[ig * nV1 + n] = x.hI[ig];
+        }
+        return;
+    }
+    load_rest(p, s, n, x);
+    const float depC = dep_current(p, t, s.tonicDep[n]);
+    if (out_dep) out_dep[n] = depC;
+    const FFAcc acc = ff_stage<kEvents>(p, k, ff, t, n, b, step, x, f, lgnSInfo, ffMask);
+    const float sInfo = membrane_stage(p, k, t, depC, x, acc);
+    store_state(p, s, n, x);
+    out_sInfo[n] = sInfo;
+    const RecOut noOut{nullptr, 0u, 0u, 0u};
+    compact<false>(p, next, noOut, n, local, lane, sInfo);
+}
+
+// ----------------------------------------------- persistent multi-step --
+// Runs nsteps steps in one launch; every workgroup (a quarter, half or whole
+// V1 block) keeps its neurons' state in registers.  Spikes travel between
+// workgroups as tagged records (see kRecSpk) in `words`: the records of step
+// s go to buffer (s+1)&1 with tag s+2, a prologue writing those of the step
+// before it0 (from sa) as s = -1.  A step reads its neighbours' records of
+// the previous step, waiting on each word's tag; it never waits on anything
+// else.  A buffer is rewritten (step s+2) only after every neighbour has
+// published step s+1, which it does after reading the buffer in step s+1.
+// The vc_spikes buffers sa/sb receive every step's spikes as vc_v1_run
+// leaves them.  Every wait is bounded: on a time-out *status is set and the
+// kernel runs to its end (results invalid, no hang).  All workgroups must be
+// co-resident (cooperative launch); `words` is zero on entry.
+template <bool kEvents>
+__global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_v1_state s,
+                                                  vc_ff_conn ff, vc_rec_conn rec,
+                                                  vc_spikes sa, vc_spikes sb,
+                                                  const float *__restrict__ lgnSInfo,
+                                                  uint64_t lgnStride, uint32_t *__restrict__ ffMask,
+                                                  uint64_t maskStride, uint32_t it0, uint32_t nsteps,
+                                                  float *__restrict__ out_sInfo, uint64_t outStride,
+                                                  uint64_t *words, uint32_t *status) {
+    __shared__ GatherLds glds;
+    __shared__ uint64_t smk[kMaxEnt];
+    __shared__ uint32_t sbid[kMaxEnt];
+    const uint32_t nV1 = p.nV1;
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = n / p.blockSize;
+    const uint32_t local = n - b * p.blockSize;
+    const uint32_t lane = vc_lane();
+    const uint32_t wv = threadIdx.x >> 6;
+    const int t = type_of(p, local);
+    const uint32_t nSeg = k.nSeg;
+    const uint32_t myseg = local >> 6;
+    const uint32_t nb = rec.nNear[b];
+    const uint32_t nEnt = nb * nSeg;
+    const uint32_t nTiles = nV1 / 64u;
+    const uint32_t recW = nTiles * kRecW;                     // words of one record buffer
+    const uint32_t ovfW = nTiles * 64u * 4u;                  // words of one overflow buffer
+    auto rec_of = [&](uint32_t buf) { return buf * recW; };
+    auto ovf_of = [&](uint32_t buf) { return 2u * recW + buf * ovfW; };
+    for (uint32_t e = threadIdx.x; e < nEnt; e += blockDim.x)
+        sbid[e] = rec.nbr[b * rec.maxNear + e / nSeg];
+    NState x;
+    load_cortical(p, s, n, x);
+    load_rest(p, s, n, x);
+    const float depC = dep_current(p, t, s.tonicDep[n]);
+    // prologue: the spikes of the step before it0 (sa) as records of step -1
+    {
+        const uint32_t tile = n >> 6;
+        const uint64_t m = sa.mask[tile];
+        const RecOut ro{words, rec_of(0), ovf_of(0), 1u};
+        uint64_t *r = words + ro.rec + (size_t)tile * kRecW;
+        if (lane == 0) {
+            st_spk<true>(r, tagged((uint32_t)m, ro.tag));
+            st_spk<true>(r + 1, tagged((uint32_t)(m >> 32), ro.tag));
+        }
+        if ((m >> lane) & 1ull) {
+            const uint32_t rank = vc_mbcnt(m);
+            const uint32_t slot = tile * 64u + rank;
+            uint64_t *w = words + (rank < (uint32_t)kRecSpk
+                                       ? ro.rec + (size_t)tile * kRecW + 2u + 4u * rank
+                                       : ro.ovf + ((size_t)tile * 64u + rank) * 4u);
+            st_spk<true>(w + 0, tagged(__float_as_uint(sa.og[slot]), ro.tag));
+            st_spk<true>(w + 1, tagged(__float_as_uint(sa.oh[slot]), ro.tag));
+            st_spk<true>(w + 2, tagged(__float_as_uint(sa.og[nV1 + slot]), ro.tag));
+            st_spk<true>(w + 3, tagged(__float_as_uint(sa.oh[nV1 + slot]), ro.tag));
+        }
+    }
+    for (uint32_t q = 0; q < nsteps; ++q) {
+        const vc_spikes &prev = (q & 1u) ? sb : sa;
+        const vc_spikes &next = (q & 1u) ? sa : sb;
+        const float *lgn = lgnSInfo + q * lgnStride;
+        uint32_t *fm = kEvents ? ffMask + q * maskStride : nullptr;
+        // the LGN / background part does not depend on the neighbours: it
+        // runs before their spikes are awaited
+        FFPre<kEvents> f;
+        ff_round1(f, ff, fm, n, nV1);
+        ff_round2(f, ff, lgn, n, nV1);
+        const FFAcc acc = ff_stage<kEvents>(p, k, ff, t, n, b, it0 + q, x, f, lgn, fm);
+        // the neighbours' spike masks of the previous step (tag q+1)
+        const RecIn ri{words, rec_of(q & 1u), ovf_of(q & 1u), q + 1u, status};
+        __syncthreads();                      // every wavefront is done with smk
+        // one entry (both mask words) per thread, issued together
+        for (uint32_t e = threadIdx.x; e < nEnt; e += blockDim.x) {
+            const uint32_t tile = sbid[e] * nSeg + (e - (e / nSeg) * nSeg);
+            uint32_t u[2];
+            ld_tagged<2>(words + ri.rec + (size_t)tile * kRecW, ri.tag, status, u);
+            smk[e] = ((uint64_t)u[1] << 32) | u[0];
+        }
+        float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
+        float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
+        __syncthreads();
+        rec_gather<true, true>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, ri, glds, lgE,
+                               lhE, lgI, lhI);
+#pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+            x.gE[ig] = x.gE[ig] + lgE[ig];
+            x.hE[ig] = x.hE[ig] + lhE[ig];
+            x.gI[ig] = x.gI[ig] + lgI[ig];
+            x.hI[ig] = x.hI[ig] + lhI[ig];
+        }
+        const float sInfo = membrane_stage(p, k, t, depC, x, acc);
+        out_sInfo[q * outStride + n] = sInfo;
+        const RecOut ro{words, rec_of((q + 1u) & 1u), ovf_of((q + 1u) & 1u), q + 2u};
+        compact<true>(p, next, ro, n, local, lane, sInfo);
+    }
+    store_state(p, s, n, x);
+}
+
+__global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
 __global__ void k_clear_masks(uint64_t *mask, uint32_t nTiles) {
@@ -758,6 +1154,75 @@ extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state 
         cur = nx;
     }
     VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" uint64_t vc_v1_persistent_scratch_bytes(const vc_v1_params *p) {
+    if (!p) return 0;
+    const uint64_t nTiles = p->nV1 / 64u;
+    return 2ull * nTiles * (kRecW + 64u * 4u) * sizeof(uint64_t);
+}
+
+extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                                    const vc_ff_conn *ff, const vc_rec_conn *rec,
+                                    const vc_spikes *spk2, uint32_t cur, const float *lgnSInfo,
+                                    uint64_t lgnStride, uint32_t *ffMask, uint64_t maskStride,
+                                    uint32_t it0, uint32_t nsteps, float *out_sInfo,
+                                    uint64_t outStride, void *scratch, uint32_t *status) {
+    int rc = check_params(p, rec);
+    if (rc) return rc;
+    if (!s || !ff || !rec || !spk2 || !lgnSInfo || !out_sInfo || !scratch || !status || cur > 1) {
+        vc_set_error("vc_v1_run_persistent: bad argument");
+        return VC_E_INVALID;
+    }
+    if (spk2[0].mask == spk2[1].mask) {
+        vc_set_error("vc_v1_run_persistent: the two spike buffers must differ");
+        return VC_E_INVALID;
+    }
+    if (ffMask && ff->maxLGN > 32u) {
+        vc_set_error("vc_v1_run_persistent: event-driven LGN input needs maxLGN <= 32");
+        return VC_E_INVALID;
+    }
+    if (rec->maxNear > 64u) {
+        vc_set_error("vc_v1_run_persistent: more than 64 neighbour blocks");
+        return VC_E_UNSUPPORTED;
+    }
+    if (nsteps == 0) return VC_OK;
+    const uint32_t wg = wg_size(p->blockSize);
+    const uint32_t grid = p->nV1 / wg;
+    const void *fn = ffMask ? (const void *)k_v1_multi<true> : (const void *)k_v1_multi<false>;
+    int dev = 0, ncu = 0, per = 0;
+    VC_HIP_CHECK(hipGetDevice(&dev));
+    VC_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    VC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, (int)wg, 0));
+    if ((uint64_t)per * (uint64_t)ncu < grid) {
+        vc_set_error("vc_v1_run_persistent: %u workgroups cannot be co-resident (%d per CU x %d "
+                     "CUs); use vc_v1_run", grid, per, ncu);
+        return VC_E_UNSUPPORTED;
+    }
+    if (((uintptr_t)scratch & 63u) != 0) {
+        vc_set_error("vc_v1_run_persistent: scratch must be 64-byte aligned");
+        return VC_E_INVALID;
+    }
+    uint64_t *words = (uint64_t *)scratch;
+    const uint64_t nWords32 = vc_v1_persistent_scratch_bytes(p) / 4u;
+    if (nWords32 >= (1ull << 32) || 2ull * (p->nV1 / 64u) * (kRecW + 256u) >= (1ull << 32)) {
+        vc_set_error("vc_v1_run_persistent: nV1 %u too large for 32-bit record offsets", p->nV1);
+        return VC_E_UNSUPPORTED;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nWords32 + 255) / 256)), dim3(256), 0, st,
+                       (uint32_t *)scratch, (uint32_t)nWords32, 0u);
+    VC_HIP_CHECK(hipGetLastError());
+    vc_v1_params pv = *p;
+    V1Const k = make_const(p);
+    vc_v1_state sv = *s;
+    vc_ff_conn fv = *ff;
+    vc_rec_conn rv = *rec;
+    vc_spikes sa = spk2[cur], sb = spk2[1u - cur];
+    void *args[] = {&pv, &k, &sv, &fv, &rv, &sa, &sb, (void *)&lgnSInfo, &lgnStride, &ffMask,
+                    &maskStride, &it0, &nsteps, &out_sInfo, &outStride, &words, &status};
+    VC_HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(wg), args, 0, st));
     return VC_OK;
 }
 

@@ -26,7 +26,8 @@ class GpuSim:
     """One simulation of one network on one GPU."""
 
     def __init__(self, net: Network, device: str = "cuda", record: bool = True,
-                 state: dict | None = None, ff_events: bool | None = None):
+                 state: dict | None = None, ff_events: bool | None = None,
+                 persistent: bool = True, dense_weights: bool | None = None):
         self.lib = abi.load_library()
         self.net = net
         sp = net.spec
@@ -78,8 +79,14 @@ class GpuSim:
             self.p.trialSeed = ptr(self.bt["trialSeed"])
         self.nNear = T(net.nNear.astype(np.uint32).view(np.int32))
         self.nbr = T(net.nbr.astype(np.uint32).view(np.int32))
+        # optional dense V1_conMat copy of the strengths (one gather round trip
+        # fewer); by default when it takes at most 4 GiB
+        dense_bytes = sp.nblock * net.maxNear * sp.blockSize * sp.blockSize * 4
+        if dense_weights is None:
+            dense_weights = dense_bytes <= (4 << 30)
+        self.wDense = self._dense_weights() if dense_weights else None
         self.rec = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
-                               ptr(self.rowOff), ptr(self.wrec))
+                               ptr(self.rowOff), ptr(self.wrec), ptr(self.wDense))
         # ---- spike buffers (double buffered)
         nT = sp.nV1 // 64
         self.masks = [torch.zeros(nT, dtype=torch.int64, device=d) for _ in range(2)]
@@ -87,6 +94,11 @@ class GpuSim:
         self.oh = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
         self.spk = [abi.Spikes(ptr(self.masks[i]), ptr(self.og[i]), ptr(self.oh[i])) for i in range(2)]
         self.spk2 = (abi.Spikes * 2)(*self.spk)
+        # persistent multi-step launches (per-block step counters, time-out flag)
+        self.persistent = persistent
+        nbytes = int(self.lib.vc_v1_persistent_scratch_bytes(C.byref(self.p)))
+        self.scratch = torch.zeros(nbytes // 4, dtype=torch.int32, device=d)
+        self.status = torch.zeros(1, dtype=torch.int32, device=d)
         # ---- streams (default priority) and the LGN pipeline state
         self.s_v1 = torch.cuda.current_stream(d)
         self.s_lgn = torch.cuda.Stream(device=d)
@@ -102,6 +114,24 @@ class GpuSim:
         self.lgn_hist: list[torch.Tensor] = []
         self.sInfo_step = torch.zeros(sp.nV1, dtype=torch.float32, device=d)
         self.s_lgn.wait_stream(self.s_v1)          # set-up work precedes the pipeline
+
+    def _dense_weights(self) -> torch.Tensor:
+        """The masked CSR expanded on the device to the dense layout
+        [nblock][maxNear][bs pre][bs post] (flat)."""
+        rm, ro, w = self.rowMask, self.rowOff.to(torch.int64) & 0xffffffff, self.wrec
+        nS = rm.numel()
+        dense = torch.zeros(nS * 64, dtype=torch.float32, device=self.dev)
+        lanes = torch.arange(64, device=self.dev, dtype=torch.int64)
+        ch = 1 << 20
+        for a in range(0, nS, ch):
+            b = min(nS, a + ch)
+            bits = ((rm[a:b, None] >> lanes[None, :]) & 1).bool()
+            seg, lane = torch.nonzero(bits, as_tuple=True)          # CSR order
+            if seg.numel() == 0:
+                continue
+            o0 = int(ro[a])
+            dense[(seg + a) * 64 + lane] = w[o0:o0 + seg.numel()]
+        return dense
 
     # ------------------------------------------------------------------ LGN
     def _setup_lgn(self):
@@ -249,11 +279,19 @@ class GpuSim:
             if events is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e0.record(sv)
-            check(self.lib.vc_v1_run(self._hs(sv), C.byref(self.p), C.byref(self.s), C.byref(self.ff),
-                                     C.byref(self.rec), self.spk2, self.cur,
-                                     ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, sp.nLGN,
-                                     mptr, sp.nV1, it, n, optr, ostride),
-                  "vc_v1_run")
+            args = (self._hs(sv), C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                    C.byref(self.rec), self.spk2, self.cur,
+                    ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, sp.nLGN, mptr, sp.nV1, it, n,
+                    optr, ostride)
+            rc = abi.VC_E_UNSUPPORTED
+            if self.persistent:
+                rc = self.lib.vc_v1_run_persistent(*args, ptr(self.scratch), ptr(self.status))
+                if rc == abi.VC_E_UNSUPPORTED:          # grid cannot be co-resident
+                    self.persistent = False
+                else:
+                    check(rc, "vc_v1_run_persistent")
+            if not self.persistent:
+                check(self.lib.vc_v1_run(*args), "vc_v1_run")
             if events is not None:
                 e1 = torch.cuda.Event(enable_timing=True)
                 e1.record(sv)
@@ -345,14 +383,21 @@ class GpuSim:
         check(self.lib.vc_spikes_clear(None, self.net.spec.nV1, C.byref(self.spk[self.cur])),
               "vc_spikes_clear")
 
-    def sInfo(self) -> np.ndarray:
+    def check(self):
+        """Wait for the device and raise if a persistent launch timed out
+        waiting for a neighbour block (its results are then invalid)."""
         torch.cuda.synchronize()
+        if int(self.status.item()) != 0:
+            raise abi.VCError("persistent V1 launch timed out waiting for neighbour blocks")
+
+    def sInfo(self) -> np.ndarray:
+        self.check()
         return torch.cat(self.sInfo_hist).cpu().numpy()
 
     def lgn_sinfo(self) -> np.ndarray:
-        torch.cuda.synchronize()
+        self.check()
         return torch.cat(self.lgn_hist).cpu().numpy()
 
     def state_numpy(self) -> dict:
-        torch.cuda.synchronize()
+        self.check()
         return {k: v.cpu().numpy() for k, v in self.state.items()}

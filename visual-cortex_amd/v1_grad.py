@@ -42,7 +42,7 @@ class V1Run:
         if sp.iModel != 0 or any(x != 0 for x in sp.deltaT) or any(x != 0 for x in sp.a):
             raise ValueError("the V1 adjoint covers the conductance LIF model "
                              "(iModel 0, deltaT = a = 0): use network.lif(spec)")
-        self.sim = GpuSim(net, device=device, record=False, ff_events=False)
+        self.sim = GpuSim(net, device=device, record=False, ff_events=False, dense_weights=False)
         self.net, self.T, self.ckpt = net, int(nsteps), int(ckpt)
         self.lib = self.sim.lib
         sim = self.sim
