@@ -24,13 +24,39 @@
 
 namespace {
 
+// block_reduce_ref of two values at once: the same tree for each, one set
+// of barriers.
+__device__ void block_reduce_ref2(float &x, float &y, float *part /* >= 64 floats */) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t W = blockDim.x / 32u;
+    x = vc_tree32(x);
+    y = vc_tree32(y);
+    __syncthreads();
+    if ((tid & 31u) == 0u) {
+        part[tid >> 5] = x;
+        part[32 + (tid >> 5)] = y;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (uint32_t off = W / 2; off > 0; off >>= 1)
+            for (uint32_t l = 0; l < off; ++l) {
+                part[l] = part[l] + part[l + off];
+                part[32 + l] = part[32 + l] + part[32 + l + off];
+            }
+    }
+    __syncthreads();
+    x = part[0];
+    y = part[32];
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(1024) void k_lgn_frame_response(
     const float *__restrict__ frame0, int W, int H, uint32_t nLGN,
     const float *__restrict__ SCx, const float *__restrict__ SCy,
     const float *__restrict__ SW, const uint32_t *__restrict__ coneType,
     const uint32_t *__restrict__ cellTrial, uint64_t trialFrameStride,
     uint64_t frameStride, float *__restrict__ R) {
-    __shared__ float part[32];
+    __shared__ float part[64];
     const uint32_t c = blockIdx.x;
     // blockIdx.y: frame of a batch of frames frameStride floats apart
     frame0 += (size_t)blockIdx.y * frameStride;
@@ -46,12 +72,13 @@ __global__ __launch_bounds__(1024) void k_lgn_frame_response(
     const float IC_S = intensity(tC, frame, W, H, xS, yS);
     const float IS_S = intensity(tS, frame, W, H, xS, yS);
     const float IC_C = intensity(tC, frame, W, H, xC, yC);
-    const float meanC = block_reduce_ref(IC_S, part) / (float)nS;
-    const float meanS = block_reduce_ref(IS_S, part) / (float)nS;
-    const float fC = sw * clip_contrast(IC_C, meanC);
-    const float fS = sw * clip_contrast(IS_S, meanS);
-    const float rC = block_reduce_ref(fC, part);
-    const float rS = block_reduce_ref(fS, part);
+    float sumC = IC_S, sumS = IS_S;
+    block_reduce_ref2(sumC, sumS, part);
+    const float meanC = sumC / (float)nS;
+    const float meanS = sumS / (float)nS;
+    float rC = sw * clip_contrast(IC_C, meanC);
+    float rS = sw * clip_contrast(IS_S, meanS);
+    block_reduce_ref2(rC, rS, part);
     if (s == 0) {
         R[(size_t)c * 2 + 0] = rC;
         R[(size_t)c * 2 + 1] = rS;
@@ -69,6 +96,7 @@ __device__ __forceinline__ float tree32_regs(float a[32]) {
 }
 
 constexpr int kMaxFrame = 8;
+constexpr int kWin = 8;                     // rate window of k_lgn_spike
 
 
 // Stage a (temporal part) and the rate of stage b for every (cell, step):
@@ -112,8 +140,8 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
     const uint32_t npf = p.ntPerFrame;
     const uint32_t dq = p.kernelSampleInterval / npf, dr = p.kernelSampleInterval % npf;
     float conv[2];
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
+#pragma unroll 1
+    for (int r = 0; r < 2; ++r) {              // one region at a time: half the registers
         float a0[32], a1[32];
         uint32_t rr = rr0;
         int dd = dd0;
@@ -166,11 +194,8 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
         khi = p.trialSeed[2 * tr + 1];
         ent = c - p.trialLgnOff[tr];
     }
-    float frn = valid ? frIn[c] : 0.0f;
-    for (uint32_t s = 0; s < nsteps; ++s) {
+    auto step_body = [&](uint32_t s, float fr) {
         const uint32_t it = it0 + s;
-        const float fr = frn;
-        if (valid && s + 1 < nsteps) frn = frIn[(size_t)(s + 1) * p.nLGN + c];
         float sInfo = 0.0f;
         if (valid) {
             const float rate = fr / 1000.0f;
@@ -221,6 +246,24 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
                 }
             }
         }
+    };
+    // the rates arrive kWin steps at a time, the next window loaded while the
+    // current one is drawn
+    float cur[kWin], nxt[kWin];
+#pragma unroll
+    for (uint32_t j = 0; j < (uint32_t)kWin; ++j)
+        cur[j] = (valid && j < nsteps) ? frIn[(size_t)j * p.nLGN + c] : 0.0f;
+    for (uint32_t s0 = 0; s0 < nsteps; s0 += (uint32_t)kWin) {
+#pragma unroll
+        for (uint32_t j = 0; j < (uint32_t)kWin; ++j) {
+            const uint32_t sn = s0 + (uint32_t)kWin + j;
+            nxt[j] = (valid && sn < nsteps) ? frIn[(size_t)sn * p.nLGN + c] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < (uint32_t)kWin; ++j)
+            if (s0 + j < nsteps) step_body(s0 + j, cur[j]);
+#pragma unroll
+        for (uint32_t j = 0; j < (uint32_t)kWin; ++j) cur[j] = nxt[j];
     }
     if (valid) {
         st.leftTimeRate[c] = lTR;

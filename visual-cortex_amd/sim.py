@@ -80,10 +80,8 @@ class GpuSim:
         self.nNear = T(net.nNear.astype(np.uint32).view(np.int32))
         self.nbr = T(net.nbr.astype(np.uint32).view(np.int32))
         # optional dense V1_conMat copy of the strengths (one gather round trip
-        # fewer); by default when it takes at most 4 GiB
-        dense_bytes = sp.nblock * net.maxNear * sp.blockSize * sp.blockSize * 4
-        if dense_weights is None:
-            dense_weights = dense_bytes <= (4 << 30)
+        # fewer, but random rows of a multi-GiB matrix): off by default, the
+        # masked CSR measured faster at the full-patch size
         self.wDense = self._dense_weights() if dense_weights else None
         self.rec = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
                                ptr(self.rowOff), ptr(self.wrec), ptr(self.wDense))
