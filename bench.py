@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16000)
     ap.add_argument("--warmup", type=int, default=640)
-    ap.add_argument("--cpu-steps", type=int, default=800,
+    ap.add_argument("--cpu-steps", type=int, default=8000,
                     help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--reps", type=int, default=5, help="c4: timed forward/backward repetitions")
@@ -107,20 +107,26 @@ def main():
     elapsed = t1 - t0
     log(f"[bench] timed {K} steps in {elapsed:.3f}s")
 
+    sim.check()
     spikes = int((out >= 1.0).sum().item())
     rho = spikes / float(K * spec.nV1)
+    # HIP events on the V1 stream bracket every V1 launch (a persistent launch
+    # covers one frame period, a one-step launch one step)
     ev_ms = sum(a.elapsed_time(b) for a, b, n in events)
     ev_n = sum(n for a, b, n in events)
-    v1_launch_s = ev_ms / 1e3 / ev_n
+    steps_per_launch = ev_n / len(events) if sim.persistent else 1.0
+    v1_step_s = ev_ms / 1e3 / ev_n
     bpn = v1_bytes_per_neuron_step(spec, rho)
-    bytes_per_launch = bpn * spec.nV1
-    achieved = bytes_per_launch / v1_launch_s / 1e9
+    bytes_per_launch = bpn * spec.nV1 * steps_per_launch
+    launch_s = v1_step_s * steps_per_launch
+    achieved = bytes_per_launch / launch_s / 1e9
     value = spec.nV1 * K / elapsed
     isE = (np.arange(spec.nV1) % spec.blockSize) < spec.nE
     per_n = (out >= 1.0).sum(0).cpu().numpy()
     T = K * spec.dt / 1000.0
     log(f"[bench] rates: E {per_n[isE].mean() / T:.2f} Hz, I {per_n[~isE].mean() / T:.2f} Hz; "
-        f"rho={rho:.3e}; V1 launch {v1_launch_s * 1e6:.2f} us; {bpn:.1f} B/neuron-step")
+        f"rho={rho:.3e}; V1 {v1_step_s * 1e6:.2f} us/step ({launch_s * 1e6:.1f} us per launch of "
+        f"{steps_per_launch:.0f} steps); {bpn:.1f} B/neuron-step")
     del out
 
     if args.copy_calib:
@@ -157,8 +163,10 @@ def main():
                    "spikes_per_neuron_step": rho},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_v1_step", "bytes_per_launch": bytes_per_launch,
-                     "launch_us": v1_launch_s * 1e6},
+                     "kernel": "k_v1_multi" if sim.persistent else "k_v1_step",
+                     "steps_per_launch": steps_per_launch,
+                     "bytes_per_launch": bytes_per_launch, "launch_us": launch_s * 1e6,
+                     "step_us": v1_step_s * 1e6},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

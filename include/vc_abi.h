@@ -179,11 +179,13 @@ int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
               uint32_t *ffMask, uint64_t maskStride, uint32_t it0,
               uint32_t nsteps, float *out_sInfo, uint64_t outStride);
 
-/* vc_v1_run in ONE cooperative launch: the neurons' state stays in registers
+/* vc_v1_run in ONE launch: the neurons' state stays in registers
  * across the steps and each workgroup waits only for its neighbour blocks'
  * previous step.  `scratch` (64-byte aligned, vc_v1_persistent_scratch_bytes
- * long, cleared by the call) holds the tagged per-tile spike records the
- * workgroups exchange.  Results are identical to
+ * long) holds the tagged per-tile spike records the workgroups exchange;
+ * the records carry `epoch`, which must differ between calls that share a
+ * scratch buffer: pass 1, 2, ..., 65535, 1, ... (the call with epoch 1
+ * clears the buffer).  Results are identical to
  * vc_v1_run; the last step's spikes are left in spk2 as vc_v1_run leaves them.  Every wait is
  * bounded: on a time-out *status (caller-zeroed, sticky) becomes nonzero
  * and the results of the call are invalid.  Returns VC_E_UNSUPPORTED when
@@ -195,7 +197,7 @@ int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const vc_v1_state 
                          uint32_t cur, const float *lgnSInfo, uint64_t lgnStride,
                          uint32_t *ffMask, uint64_t maskStride, uint32_t it0,
                          uint32_t nsteps, float *out_sInfo, uint64_t outStride,
-                         void *scratch, uint32_t *status);
+                         void *scratch, uint32_t epoch, uint32_t *status);
 
 /* Bytes of the scratch buffer of vc_v1_run_persistent. */
 uint64_t vc_v1_persistent_scratch_bytes(const vc_v1_params *p);

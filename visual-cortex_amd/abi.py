@@ -120,7 +120,7 @@ SIGNATURES = {
                   u32, u32, vp, C.c_uint64],
     "vc_v1_run_persistent": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
                              C.POINTER(RecConn), C.POINTER(Spikes), u32, vp, C.c_uint64, vp,
-                             C.c_uint64, u32, u32, vp, C.c_uint64, vp, vp],
+                             C.c_uint64, u32, u32, vp, C.c_uint64, vp, u32, vp],
     "vc_v1_persistent_scratch_bytes": [C.POINTER(V1Params)],
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],

@@ -895,10 +895,11 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
 // the previous step, waiting on each word's tag; it never waits on anything
 // else.  A buffer is rewritten (step s+2) only after every neighbour has
 // published step s+1, which it does after reading the buffer in step s+1.
-// The vc_spikes buffers sa/sb receive every step's spikes as vc_v1_run
-// leaves them.  Every wait is bounded: on a time-out *status is set and the
-// kernel runs to its end (results invalid, no hang).  All workgroups must be
-// co-resident (cooperative launch); `words` is zero on entry.
+// Tags are epoch*65536 + s+2 with a per-launch epoch, so words left by
+// earlier launches never match.  The vc_spikes buffers sa/sb receive every
+// step's spikes as vc_v1_run leaves them.  Every wait is bounded: on a
+// time-out *status is set and the kernel runs to its end (results invalid,
+// no hang).  The grid must fit on the device at once.
 template <bool kEvents>
 __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_v1_state s,
                                                   vc_ff_conn ff, vc_rec_conn rec,
@@ -907,7 +908,8 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
                                                   uint64_t lgnStride, uint32_t *__restrict__ ffMask,
                                                   uint64_t maskStride, uint32_t it0, uint32_t nsteps,
                                                   float *__restrict__ out_sInfo, uint64_t outStride,
-                                                  uint64_t *words, uint32_t *status) {
+                                                  uint64_t *words, uint32_t epoch,
+                                                  uint32_t *status) {
     __shared__ GatherLds glds;
     __shared__ uint64_t smk[kMaxEnt];
     __shared__ uint32_t sbid[kMaxEnt];
@@ -937,7 +939,7 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
     {
         const uint32_t tile = n >> 6;
         const uint64_t m = sa.mask[tile];
-        const RecOut ro{words, rec_of(0), ovf_of(0), 1u};
+        const RecOut ro{words, rec_of(0), ovf_of(0), (epoch << 16) + 1u};
         uint64_t *r = words + ro.rec + (size_t)tile * kRecW;
         if (lane == 0) {
             st_spk<true>(r, tagged((uint32_t)m, ro.tag));
@@ -967,7 +969,7 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
         ff_round2(f, ff, lgn, n, nV1);
         const FFAcc acc = ff_stage<kEvents>(p, k, ff, t, n, b, it0 + q, x, f, lgn, fm);
         // the neighbours' spike masks of the previous step (tag q+1)
-        const RecIn ri{words, rec_of(q & 1u), ovf_of(q & 1u), q + 1u, status};
+        const RecIn ri{words, rec_of(q & 1u), ovf_of(q & 1u), (epoch << 16) + q + 1u, status};
         __syncthreads();                      // every wavefront is done with smk
         // one entry (both mask words) per thread, issued together
         for (uint32_t e = threadIdx.x; e < nEnt; e += blockDim.x) {
@@ -990,7 +992,8 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
         }
         const float sInfo = membrane_stage(p, k, t, depC, x, acc);
         out_sInfo[q * outStride + n] = sInfo;
-        const RecOut ro{words, rec_of((q + 1u) & 1u), ovf_of((q + 1u) & 1u), q + 2u};
+        const RecOut ro{words, rec_of((q + 1u) & 1u), ovf_of((q + 1u) & 1u),
+                        (epoch << 16) + q + 2u};
         compact<true>(p, next, ro, n, local, lane, sInfo);
     }
     store_state(p, s, n, x);
@@ -1168,7 +1171,8 @@ extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const v
                                     const vc_spikes *spk2, uint32_t cur, const float *lgnSInfo,
                                     uint64_t lgnStride, uint32_t *ffMask, uint64_t maskStride,
                                     uint32_t it0, uint32_t nsteps, float *out_sInfo,
-                                    uint64_t outStride, void *scratch, uint32_t *status) {
+                                    uint64_t outStride, void *scratch, uint32_t epoch,
+                                    uint32_t *status) {
     int rc = check_params(p, rec);
     if (rc) return rc;
     if (!s || !ff || !rec || !spk2 || !lgnSInfo || !out_sInfo || !scratch || !status || cur > 1) {
@@ -1186,6 +1190,11 @@ extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const v
     if (rec->maxNear > 64u) {
         vc_set_error("vc_v1_run_persistent: more than 64 neighbour blocks");
         return VC_E_UNSUPPORTED;
+    }
+    if (epoch == 0 || epoch > 0xffffu || nsteps > 0xfffdu) {
+        vc_set_error("vc_v1_run_persistent: epoch %u (1..65535) or %u steps (<= 65533) out of "
+                     "range", epoch, nsteps);
+        return VC_E_INVALID;
     }
     if (nsteps == 0) return VC_OK;
     const uint32_t wg = wg_size(p->blockSize);
@@ -1211,18 +1220,25 @@ extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const v
         return VC_E_UNSUPPORTED;
     }
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nWords32 + 255) / 256)), dim3(256), 0, st,
-                       (uint32_t *)scratch, (uint32_t)nWords32, 0u);
+    if (epoch == 1) {               // a new cycle of epochs: no tag of an earlier one may match
+        hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nWords32 + 255) / 256)), dim3(256), 0,
+                           st, (uint32_t *)scratch, (uint32_t)nWords32, 0u);
+        VC_HIP_CHECK(hipGetLastError());
+    }
+    // An ordinary launch: every workgroup fits on the device at once (checked
+    // above), the only waits are of these workgroups on one another and are
+    // bounded, so kernels of other streams can at most delay some of them.
+    // (A cooperative launch would also serialise the host with the device.)
+    const V1Const k = make_const(p);
+    if (ffMask)
+        hipLaunchKernelGGL(k_v1_multi<true>, dim3(grid), dim3(wg), 0, st, *p, k, *s, *ff, *rec,
+                           spk2[cur], spk2[1u - cur], lgnSInfo, lgnStride, ffMask, maskStride,
+                           it0, nsteps, out_sInfo, outStride, words, epoch, status);
+    else
+        hipLaunchKernelGGL(k_v1_multi<false>, dim3(grid), dim3(wg), 0, st, *p, k, *s, *ff, *rec,
+                           spk2[cur], spk2[1u - cur], lgnSInfo, lgnStride, ffMask, maskStride,
+                           it0, nsteps, out_sInfo, outStride, words, epoch, status);
     VC_HIP_CHECK(hipGetLastError());
-    vc_v1_params pv = *p;
-    V1Const k = make_const(p);
-    vc_v1_state sv = *s;
-    vc_ff_conn fv = *ff;
-    vc_rec_conn rv = *rec;
-    vc_spikes sa = spk2[cur], sb = spk2[1u - cur];
-    void *args[] = {&pv, &k, &sv, &fv, &rv, &sa, &sb, (void *)&lgnSInfo, &lgnStride, &ffMask,
-                    &maskStride, &it0, &nsteps, &out_sInfo, &outStride, &words, &status};
-    VC_HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(wg), args, 0, st));
     return VC_OK;
 }
 

@@ -96,6 +96,7 @@ class GpuSim:
         self.persistent = persistent
         nbytes = int(self.lib.vc_v1_persistent_scratch_bytes(C.byref(self.p)))
         self.scratch = torch.zeros(nbytes // 4, dtype=torch.int32, device=d)
+        self.epoch = 0                   # of the last persistent launch (1..65535, cycling)
         self.status = torch.zeros(1, dtype=torch.int32, device=d)
         # ---- streams (default priority) and the LGN pipeline state
         self.s_v1 = torch.cuda.current_stream(d)
@@ -283,7 +284,9 @@ class GpuSim:
                     optr, ostride)
             rc = abi.VC_E_UNSUPPORTED
             if self.persistent:
-                rc = self.lib.vc_v1_run_persistent(*args, ptr(self.scratch), ptr(self.status))
+                self.epoch = self.epoch % 65535 + 1
+                rc = self.lib.vc_v1_run_persistent(*args, ptr(self.scratch), self.epoch,
+                                                   ptr(self.status))
                 if rc == abi.VC_E_UNSUPPORTED:          # grid cannot be co-resident
                     self.persistent = False
                 else:
