@@ -17,6 +17,7 @@ import argparse
 import importlib.util
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -51,6 +52,20 @@ def v1_bytes_per_neuron_step(spec, rho: float) -> float:
     background drive is generated in-kernel (0 B)."""
     K = spec.KE + spec.KI
     return 112.0 + 8.0 * spec.m + 4.0 * spec.nLGN / spec.nV1 + 8.0 * rho * K
+
+
+def pmc_traffic(kernel: str, workload: str):
+    """Physical HBM bytes per launch of `kernel` on `workload`, from the
+    committed rocprofv3 counter summary profiles/<kernel>_<workload>_pmc.md
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter
+    reads for a plain device copy of known size); None without one."""
+    path = ROOT / "profiles" / f"{kernel}_{workload}_pmc.md"
+    try:
+        text = path.read_text()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)\s*$", text, re.M)
+    return float(m.group(1)) if m else None
 
 
 def main():
@@ -142,6 +157,7 @@ def main():
     if args.cpu_steps > 0:
         cpu = cpu_baseline(pkg, net, sim, args.cpu_steps)
 
+    kernel = "k_v1_multi" if sim.persistent else "k_v1_step"
     line = {
         "metric": "V1 neuron-steps/s at 64 blocks dt=0.125ms",
         "value": value,
@@ -162,8 +178,9 @@ def main():
                    "nV1": spec.nV1, "nLGN": spec.nLGN, "dt_ms": spec.dt,
                    "spikes_per_neuron_step": rho},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_v1_multi" if sim.persistent else "k_v1_step",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc_traffic(kernel, "full_patch") if args.config == "c2" else None,
+                     "kernel": kernel,
                      "steps_per_launch": steps_per_launch,
                      "bytes_per_launch": bytes_per_launch, "launch_us": launch_s * 1e6,
                      "step_us": v1_step_s * 1e6},

@@ -36,8 +36,7 @@ constexpr int kEv = 4;             // LGN spike events prefetched per neuron
 // LGN (or background) conductance input of one synapse, src/coredynamics.cu
 // :1474-1500: nsp spikes at tsp into (g1, h1) over dt - tsp and, when the
 // neuron is backing up from refractoriness, into (g0, h0) over tBack - tsp.
-// Not inlined: it runs only for synapses whose cell spiked, and one copy of
-// its code serves every synapse slot.
+// pm[ig] is the receptor split of the input's kind (LGN or background).
 struct FFAcc {
     float g0[VC_MAX_NG], h0[VC_MAX_NG], g1[VC_MAX_NG], h1[VC_MAX_NG];
 };
@@ -45,15 +44,16 @@ struct FFShape {
     float p[VC_MAX_NG], rise[VC_MAX_NG], decay[VC_MAX_NG], dod[VC_MAX_NG];
     uint32_t ng;
 };
-__device__ __noinline__ FFAcc ff_add(FFAcc a, const FFShape sh, float f, float nsp, float tsp,
-                                     float dt, float tBack, bool backing) {
+__device__ __forceinline__ void ff_apply(FFAcc &a, const FFShape &sh, const float pm[VC_MAX_NG],
+                                         float f, float nsp, float tsp, float dt, float tBack,
+                                         bool backing) {
     if (f > 0.0f) {
         float ddt = 0.0f;
         if (backing) ddt = tBack - tsp;
 #pragma unroll
         for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
             if (ig < sh.ng) {
-                const float str = f * sh.p[ig];
+                const float str = f * pm[ig];
                 if (backing && ddt > 0.0f)
                     single_input(a.g0[ig], a.h0[ig], str * nsp, ddt, sh.rise[ig], sh.decay[ig],
                                  sh.dod[ig]);
@@ -62,9 +62,14 @@ __device__ __noinline__ FFAcc ff_add(FFAcc a, const FFShape sh, float f, float n
             }
         }
     }
+}
+// The same as a call, for the ELL path whose unrolled synapse slots would
+// otherwise each hold a copy of the body.
+__device__ __noinline__ FFAcc ff_add(FFAcc a, const FFShape sh, float f, float nsp, float tsp,
+                                     float dt, float tBack, bool backing) {
+    ff_apply(a, sh, sh.p, f, nsp, tsp, dt, tBack, backing);
     return a;
 }
-
 
 // Loads and stores of the spike buffers.  Within one launch of the
 // persistent kernel they are exchanged between workgroups that may sit on
@@ -133,24 +138,24 @@ __device__ __forceinline__ void add_spike(const vc_v1_params &p, float strength,
                                           float h0, float g1, float h1, float lgE[VC_MAX_NG],
                                           float lhE[VC_MAX_NG], float lgI[VC_MAX_NG],
                                           float lhI[VC_MAX_NG]) {
-    if (inh == 0u) {
-        float str = strength * pE0;
-        lgE[0] += str * g0;
-        lhE[0] += str * h0;
-        if (p.ngE > 1) {
-            str = strength * pE1;
-            lgE[1] += str * g1;
-            lhE[1] += str * h1;
-        }
-    } else {
-        float str = strength * pI0;
-        lgI[0] += str * g0;
-        lhI[0] += str * h0;
-        if (p.ngI > 1) {
-            str = strength * pI1;
-            lgI[1] += str * g1;
-            lhI[1] += str * h1;
-        }
+    // Selects on values, not on which array is updated: a branch between the
+    // E and I sums is otherwise merged into a select between the arrays'
+    // addresses, which puts them in scratch memory.
+    const bool e = inh == 0u;
+    const float str0 = strength * (e ? pE0 : pI0);
+    const float a0 = str0 * g0, b0 = str0 * h0;
+    lgE[0] = e ? lgE[0] + a0 : lgE[0];
+    lhE[0] = e ? lhE[0] + b0 : lhE[0];
+    lgI[0] = e ? lgI[0] : lgI[0] + a0;
+    lhI[0] = e ? lhI[0] : lhI[0] + b0;
+    const bool e1 = e && p.ngE > 1, i1 = !e && p.ngI > 1;
+    if (e1 || i1) {
+        const float str1 = strength * (e ? pE1 : pI1);
+        const float a1 = str1 * g1, b1 = str1 * h1;
+        lgE[1] = e1 ? lgE[1] + a1 : lgE[1];
+        lhE[1] = e1 ? lhE[1] + b1 : lhE[1];
+        lgI[1] = i1 ? lgI[1] + a1 : lgI[1];
+        lhI[1] = i1 ? lhI[1] + b1 : lhI[1];
     }
 }
 
@@ -396,15 +401,18 @@ __device__ __forceinline__ void store_state(const vc_v1_params &p, const vc_v1_s
     s.v[n] = x.v;
     if (p.iModel == 1) s.w[n] = x.w;
     s.tBack[n] = x.tBack;
-    for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+    #pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngFF) {
         s.gFF[ig * nV1 + n] = x.gFF[ig];
         s.hFF[ig * nV1 + n] = x.hFF[ig];
     }
-    for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+    #pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngE) {
         s.gE[ig * nV1 + n] = x.gE[ig];
         s.hE[ig * nV1 + n] = x.hE[ig];
     }
-    for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+    #pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngI) {
         s.gI[ig * nV1 + n] = x.gI[ig];
         s.hI[ig * nV1 + n] = x.hI[ig];
     }
@@ -478,41 +486,71 @@ __device__ __forceinline__ void ff_round2(FFPre<false> &f, const vc_ff_conn &, c
     for (int j = 0; j < kFF; ++j) f.sv[j] = ((uint32_t)j < f.m) ? lgnSInfo[f.fid[j]] : 0.0f;
 }
 
-// LGN synapses in list order; only synapses whose cell spiked (sInfo >= 1)
-// do work (reference: nsp_FF > 0).
-__device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, FFPre<true> &f,
-                                          const vc_ff_conn &ff, const float *lgnSInfo,
-                                          uint32_t *ffMask, uint32_t n, uint32_t nV1, float dt,
-                                          float tBack, bool backing) {
-    if (__ballot(f.bits != 0u) == 0ull) return;
+// Background input of one neuron for a step (extension: 2 kHz Poisson drive
+// applied after the LGN list): nsp events at tsp, nsp = 0 for none.
+struct BGIn {
+    float f, nsp, tsp;
+};
+
+// LGN synapses in list order, then the background input; only synapses
+// whose cell spiked (sInfo >= 1) do work (reference: nsp_FF > 0).  Every
+// input of a lane goes through the one inlined copy of ff_apply in the loop
+// (no call, so no stack frame).
+__device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const FFShape &shBG,
+                                          FFPre<true> &f, const vc_ff_conn &ff,
+                                          const float *lgnSInfo, uint32_t *ffMask, uint32_t n,
+                                          uint32_t nV1, float dt, float tBack, bool backing,
+                                          const BGIn &bg) {
     float evs[kEv];
+    if (__ballot(f.bits != 0u) != 0ull) {
 #pragma unroll
-    for (int k = 0; k < kEv; ++k) evs[k] = (f.evj[k] != 0xffffffffu) ? lgnSInfo[f.evid[k]] : 0.0f;
-#pragma unroll
-    for (int k = 0; k < kEv; ++k) {
-        if (f.evj[k] != 0xffffffffu) {
-            const float sI = evs[k];
-            const float nsp = (float)(uint32_t)floorf(sI);
-            acc = ff_add(acc, shFF, f.evf[k], nsp, (sI - nsp) * dt, dt, tBack, backing);
-        }
+        for (int k = 0; k < kEv; ++k)
+            evs[k] = (f.evj[k] != 0xffffffffu) ? lgnSInfo[f.evid[k]] : 0.0f;
     }
     uint32_t rest = f.bits;
+    uint32_t nPre = 0;                       // prefetched events consumed
+    bool bgLeft = bg.nsp > 0.0f;
+    while (rest != 0u || bgLeft) {
+        float fv, nsp, tsp, pm[VC_MAX_NG];
+        if (rest != 0u) {
+            const uint32_t j = (uint32_t)__builtin_ctz(rest);
+            rest &= rest - 1u;
+            float sI;
+            if (nPre < (uint32_t)kEv) {      // the next prefetched event (ascending j)
+                fv = f.evf[0];
+                sI = evs[0];
 #pragma unroll
-    for (int k = 0; k < kEv; ++k) rest &= rest - 1u;        // drop the prefetched bits
-    while (rest) {
-        const uint32_t j = (uint32_t)__builtin_ctz(rest);
-        rest &= rest - 1u;
-        const uint32_t lid = j * nV1 + n;
-        const float sI = lgnSInfo[ff.idLGN[lid]];
-        const float nsp = (float)(uint32_t)floorf(sI);
-        acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
+                for (int k = 0; k + 1 < kEv; ++k) {
+                    f.evf[k] = f.evf[k + 1];
+                    evs[k] = evs[k + 1];
+                }
+                ++nPre;
+            } else {
+                const uint32_t lid = j * nV1 + n;
+                fv = ff.sLGN[lid];
+                sI = lgnSInfo[ff.idLGN[lid]];
+            }
+            nsp = (float)(uint32_t)floorf(sI);
+            tsp = (sI - nsp) * dt;
+#pragma unroll
+            for (int ig = 0; ig < VC_MAX_NG; ++ig) pm[ig] = shFF.p[ig];
+        } else {
+            fv = bg.f;
+            nsp = bg.nsp;
+            tsp = bg.tsp;
+            bgLeft = false;
+#pragma unroll
+            for (int ig = 0; ig < VC_MAX_NG; ++ig) pm[ig] = shBG.p[ig];
+        }
+        ff_apply(acc, shFF, pm, fv, nsp, tsp, dt, tBack, backing);
     }
     if (f.bits) ffMask[n] = 0u;
 }
-__device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, FFPre<false> &f,
-                                          const vc_ff_conn &ff, const float *lgnSInfo, uint32_t *,
-                                          uint32_t n, uint32_t nV1, float dt, float tBack,
-                                          bool backing) {
+__device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const FFShape &shBG,
+                                          FFPre<false> &f, const vc_ff_conn &ff,
+                                          const float *lgnSInfo, uint32_t *, uint32_t n,
+                                          uint32_t nV1, float dt, float tBack, bool backing,
+                                          const BGIn &bg) {
     uint32_t spk = 0;
 #pragma unroll
     for (int j = 0; j < kFF; ++j) spk |= (f.sv[j] >= 1.0f ? 1u : 0u) << j;
@@ -537,6 +575,7 @@ __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, FFPre
             acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
         }
     }
+    if (bg.nsp > 0.0f) acc = ff_add(acc, shBG, bg.f, bg.nsp, bg.tsp, dt, tBack, backing);
 }
 
 // Stage c-FF of one neuron: the LGN and background inputs into the FF
@@ -580,8 +619,8 @@ __device__ __forceinline__ FFAcc ff_stage(const vc_v1_params &p, const V1Const &
         shFF.dod[ig] = shBG.dod[ig] = p.dodFF[ig];
     }
     shFF.ng = shBG.ng = p.ngFF;
-    ff_inputs(acc, shFF, f, ff, lgnSInfo, ffMask, n, p.nV1, dt, tBack, backing);
     // background Poisson drive (extension): one more input after the LGN list
+    BGIn bg{0.0f, 0.0f, 0.0f};
     if (p.bgStrength[t] > 0.0f) {
         uint32_t klo = p.seed_lo, khi = p.seed_hi, ent = n;
         if (p.blockTrial != nullptr) {
@@ -594,9 +633,9 @@ __device__ __forceinline__ FFAcc ff_stage(const vc_v1_params &p, const V1Const &
         const float u = vc_u01_open0(r.v[0]);
         const float frac = vc_u01_open1(r.v[1]);
         const uint32_t nsp = vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
-        if (nsp > 0u)
-            acc = ff_add(acc, shBG, p.bgStrength[t], (float)nsp, frac * dt, dt, tBack, backing);
+        if (nsp > 0u) bg = BGIn{p.bgStrength[t], (float)nsp, frac * dt};
     }
+    ff_inputs(acc, shFF, shBG, f, ff, lgnSInfo, ffMask, n, p.nV1, dt, tBack, backing, bg);
     return acc;
 }
 
@@ -840,36 +879,43 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     if (out_snap) {                 // state at the start of the step, cortical g/h incremented
         out_snap[VC_SNAP_V * nV1 + n] = s.v[n];
         out_snap[VC_SNAP_TBACK * nV1 + n] = s.tBack[n];
-        for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngFF) {
             out_snap[(VC_SNAP_GFF + ig) * nV1 + n] = s.gFF[ig * nV1 + n];
             out_snap[(VC_SNAP_HFF + ig) * nV1 + n] = s.hFF[ig * nV1 + n];
         }
-        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngE) {
             out_snap[(VC_SNAP_GE + ig) * nV1 + n] = x.gE[ig];
             out_snap[(VC_SNAP_HE + ig) * nV1 + n] = x.hE[ig];
         }
-        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngI) {
             out_snap[(VC_SNAP_GI + ig) * nV1 + n] = x.gI[ig];
             out_snap[(VC_SNAP_HI + ig) * nV1 + n] = x.hI[ig];
         }
     }
     if (out_g_prev) {
         const uint32_t ng = p.ngE + p.ngI;
-        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngE) {
             out_g_prev[ig * nV1 + n] = x.gE[ig];
             out_g_prev[(ng + ig) * nV1 + n] = x.hE[ig];
         }
-        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngI) {
             out_g_prev[(p.ngE + ig) * nV1 + n] = x.gI[ig];
             out_g_prev[(ng + p.ngE + ig) * nV1 + n] = x.hI[ig];
         }
     }
     if (!kUpdate) {
-        for (uint32_t ig = 0; ig < p.ngE; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngE) {
             s.gE[ig * nV1 + n] = x.gE[ig];
             s.hE[ig * nV1 + n] = x.hE[ig];
         }
-        for (uint32_t ig = 0; ig < p.ngI; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngI) {
             s.gI[ig * nV1 + n] = x.gI[ig];
             s.hI[ig * nV1 + n] = x.hI[ig];
         }

@@ -176,7 +176,8 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
         if (!(sI >= 1.0f)) continue;
         const float nsp = (float)(uint32_t)floorf(sI), tsp = (sI - nsp) * dt;
         const float f = ff.sLGN[lid];
-        for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngFF) {
             const float str = f * p.pFF[t * p.ngFF + ig] * nsp;
             const Shape s1 = shape_at(dt - tsp, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
             g1[ig] = g1[ig] + str * s1.A;
@@ -201,7 +202,8 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
         bg.nsp = (float)vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
         bg.tsp = frac * dt;
         if (bg.nsp > 0.0f) {
-            for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+            #pragma unroll
+            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngFF) {
                 const float str = p.bgStrength[t] * p.pBG[t * p.ngFF + ig] * bg.nsp;
                 const Shape s1 = shape_at(dt - bg.tsp, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
                 g1[ig] = g1[ig] + str * s1.A;
@@ -343,7 +345,8 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
         const float nsp = (float)(uint32_t)floorf(sI), tsp = (sI - nsp) * dt;
         const float f = ff.sLGN[lid];
         float df = 0.0f;
-        for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngFF) {
             const float pn = p.pFF[t * p.ngFF + ig] * nsp;
             const Shape s1 = shape_at(dt - tsp, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
             df += pn * (lg1t[ig] * s1.A + lh1[ig] * s1.B);
@@ -356,7 +359,8 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
         dsLGN[lid] += df;
     }
     if (bg.nsp > 0.0f && backing && tB - bg.tsp > 0.0f) {
-        for (uint32_t ig = 0; ig < p.ngFF; ++ig) {
+        #pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngFF) {
             const float str = p.bgStrength[t] * p.pBG[t * p.ngFF + ig] * bg.nsp;
             const Shape s0 = shape_at(tB - bg.tsp, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
             ltB += str * lg0[ig] * s0.dA;
