@@ -207,6 +207,13 @@ uint64_t vc_v1_persistent_scratch_bytes(const vc_v1_params *p);
 int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                     const vc_rec_conn *rec, const vc_spikes *prev);
 
+/* depC[nV1] of every neuron from s->tonicDep, as the step kernels compute it
+ * (the AdEx constructor's tonic drive, src/coredynamics.cuh:40-49); the
+ * reference keeps it as the first array of its V1 state block
+ * (src/patch.cu:4001-4019), which its snapshots store (:6897-6898). */
+int vc_v1_dep_current(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                      float *out_dep);
+
 /* Initial V1 state: v, w, tBack, g/h as given; clears a spike buffer. */
 int vc_spikes_clear(void *stream, uint32_t nV1, const vc_spikes *sp);
 

@@ -8,10 +8,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_cli_from_files_matches_direct_run(pkg, tmp_path):
+def test_cli_from_files_matches_direct_run(pkg, tmp_path, monkeypatch):
     from visual_cortex_amd import cli, formats, loader, network
     from visual_cortex_amd.sim import GpuSim
     print("test_cli_from_files_matches_direct_run: tiny, 192 steps", flush=True)
+    monkeypatch.chdir(tmp_path)                 # the last-step snapshot goes to the cwd
     net = network.build_network(network.spec_tiny(nt=192, bgStrength=(0.4, 0.4)))
     cfg = loader.save_network(net, tmp_path / "run")
     with open(cfg, "a") as f:

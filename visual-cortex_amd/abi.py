@@ -122,6 +122,7 @@ SIGNATURES = {
                              C.POINTER(RecConn), C.POINTER(Spikes), u32, vp, C.c_uint64, vp,
                              C.c_uint64, u32, u32, vp, C.c_uint64, vp, u32, vp],
     "vc_v1_persistent_scratch_bytes": [C.POINTER(V1Params)],
+    "vc_v1_dep_current": [vp, C.POINTER(V1Params), C.POINTER(V1State), vp],
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],

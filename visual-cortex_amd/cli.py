@@ -34,7 +34,7 @@ def main(argv=None) -> int:
     ap.add_argument("-p", "--print", dest="print_log", default="false")
     a = ap.parse_args(argv)
     _pkg()
-    from visual_cortex_amd import config, loader
+    from visual_cortex_amd import config, loader, snapshot
     from visual_cortex_amd.sim import GpuSim
     cmd = {} if a.seed is None else {"seed": a.seed}
     cfg = config.Config.from_file(a.cfg, cmd)
@@ -44,12 +44,33 @@ def main(argv=None) -> int:
     print(f"{sp.nV1} V1 neurons in {sp.nblock} blocks, {sp.nLGN} LGN cells, "
           f"loaded in {time.time() - t0:.1f}s", flush=True)
     sim = GpuSim(net, record=False)
+    snap = config.snapshot_options(cfg)
+    if snap["restore"]:
+        h = snapshot.restore(sim, snap["restore"], as_init=snap["as_init"])
+        print(f"restored {snap['restore']} (step {h['it0']}"
+              f"{', as the initial condition' if snap['as_init'] else ''})", flush=True)
     outputs = dict(rawData=cfg.bool("rawData", True), lgn_fr=cfg.bool("saveLGN_fr", True),
                    lgn_sp=cfg.bool("getLGN_sp", False), hWrite=cfg.bool("hWrite", True))
     paths = dict(rawData=cfg.path("fRawData"), lgn_fr=cfg.path("fLGN_fr"),
                  lgn_sp=cfg.path("fLGN_sp"))
+    npf = sim.tm["ntPerFrame"]
+    last = {"path": None}
+
+    def take_snapshot(g, i):
+        # every snapshotInterval steps and at the last step (src/patch.cu:6375),
+        # at frame-period boundaries: the interval is rounded up to them
+        every = -(-max(snap["interval"], 1) // npf) * npf
+        if g.it % npf != 0 or not ((i + 1) % every == 0 or i == sp.nt - 1):
+            return
+        path = f"snapShot_{int(time.time())}{snap['output_suffix']}"
+        snapshot.save(g, path)
+        if last["path"] and last["path"] != path and snap["delete_previous"]:
+            Path(last["path"]).unlink(missing_ok=True)
+        last["path"] = path
+        print(f"saved a snapshot in {path}", flush=True)
+
     t0 = time.time()
-    sim.run_with_outputs(sp.nt, paths, outputs)
+    sim.run_with_outputs(sp.nt, paths, outputs, on_step=take_snapshot)
     print(f"simulated {sp.nt} steps ({sp.nt * sp.dt:g} ms) in {time.time() - t0:.1f}s", flush=True)
     return 0
 

@@ -123,6 +123,27 @@ class Config:
         return base + ("-" + suf if suf else "") + ".bin"
 
 
+def snapshot_options(cfg: Config) -> dict:
+    """Snapshot / restore options of the reference (src/patch.cu:194, 306-307,
+    340, 439-446, 611-619): snapshotInterval (steps), delPrevSnapshot, the
+    file to restore from (fSnapshot under inputFolder, '!' for a literal
+    path, then '_<snapshot_suffix>.bin') and asInit (use it as the initial
+    condition rather than resume).  A resume needs snapshot_suffix equal to
+    output_suffix, as in the reference."""
+    restore = None
+    if cfg.has("fSnapshot"):
+        name = cfg.str("fSnapshot")
+        base = name[1:] if name.startswith("!") else cfg.str("inputFolder", "") + name
+        restore = base + "_" + cfg.str("snapshot_suffix", "") + ".bin"
+    as_init = cfg.bool("asInit", True)
+    if restore and not as_init and cfg.str("snapshot_suffix", "") != cfg.str("output_suffix", ""):
+        raise ValueError("resuming from a snapshot needs snapshot_suffix == output_suffix")
+    suf = cfg.str("output_suffix", "")
+    return dict(interval=cfg.int("snapshotInterval", 10000),
+                delete_previous=cfg.bool("delPrevSnapshot", True), restore=restore,
+                as_init=as_init, output_suffix=("-" + suf) if suf else "")
+
+
 def _pairs(vals, n_types, per):
     """A flat [type][per] list -> tuple of per-type tuples."""
     if vals is None:

@@ -1050,6 +1050,13 @@ __global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v) {
     if (i < n) a[i] = v;
 }
 
+__global__ void k_dep_current(vc_v1_params p, const float *__restrict__ tonicDep,
+                              float *__restrict__ out) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nV1) return;
+    out[n] = dep_current(p, type_of(p, n % p.blockSize), tonicDep[n]);
+}
+
 __global__ void k_clear_masks(uint64_t *mask, uint32_t nTiles) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nTiles) mask[i] = 0ull;
@@ -1303,6 +1310,18 @@ extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_
     hipLaunchKernelGGL((k_v1_step<false, false>), dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
                        *p, k, *s, ff, *rec, *prev, nx, nullptr, nullptr, 0u, nullptr, nullptr,
                        nullptr, nullptr);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_v1_dep_current(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                                 float *out_dep) {
+    if (!p || !s || !s->tonicDep || !out_dep || p->blockSize == 0) {
+        vc_set_error("vc_v1_dep_current: null argument");
+        return VC_E_INVALID;
+    }
+    hipLaunchKernelGGL(k_dep_current, dim3((p->nV1 + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, *p, s->tonicDep, out_dep);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

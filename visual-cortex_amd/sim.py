@@ -112,6 +112,7 @@ class GpuSim:
         self.sInfo_hist: list[torch.Tensor] = []
         self.lgn_hist: list[torch.Tensor] = []
         self.sInfo_step = torch.zeros(sp.nV1, dtype=torch.float32, device=d)
+        self.sInfo_last = self.sInfo_step          # V1 output of the last step run (a view)
         self.s_lgn.wait_stream(self.s_v1)          # set-up work precedes the pipeline
 
     def _dense_weights(self) -> torch.Tensor:
@@ -166,6 +167,10 @@ class GpuSim:
         self.kS = self.lgn_t["k"][1].contiguous()
         self.lTR = torch.zeros(sp.nLGN, **f32)
         self.lNL = torch.zeros(sp.nLGN, **f32)
+        # (leftTimeRate, lastNegLogRand) at the start of the last two issued
+        # frame periods: the LGN stage runs ahead of V1, so a snapshot at a
+        # period boundary takes the LGN state from here
+        self.lgn_at = [torch.zeros(2, sp.nLGN, **f32) for _ in range(2)]
         self.lp = lgn_params_struct(net)
         if getattr(net, "batch", None) is not None:
             self.lp.cellTrial = ptr(self.bt["cellTrial"])
@@ -235,6 +240,9 @@ class GpuSim:
                                              ctr, self.trialFrameStride,
                                              ptr(self.Rring[fidx % tm["maxFrame"]])),
               "vc_lgn_frame_response")
+        with torch.cuda.stream(st):
+            self.lgn_at[buf][0].copy_(self.lTR)
+            self.lgn_at[buf][1].copy_(self.lNL)
         ev = C.byref(self.ev2[buf]) if self.ev is not None else None
         check(self.lib.vc_lgn_steps(self._hs(st), C.byref(self.lp), C.byref(self.ls),
                                     ptr(self.Rring), it, npf, ptr(self.lgn_sInfo2[buf]),
@@ -272,8 +280,10 @@ class GpuSim:
             n = min(npf - k_in, nsteps - done)
             if out is not None:
                 optr, ostride = ptr(out) + done * sp.nV1 * 4, sp.nV1
+                self.sInfo_last = out[done + min(npf - k_in, nsteps - done) - 1]
             else:
                 optr, ostride = ptr(self.sInfo_step), 0
+                self.sInfo_last = self.sInfo_step
             mptr = ptr(self.ffMask2[buf]) + k_in * sp.nV1 * 4 if self.ev is not None else None
             if events is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
@@ -305,12 +315,14 @@ class GpuSim:
                 e.record(sv)
                 self.v1_done[buf] = e
 
-    def run_with_outputs(self, nsteps: int, paths: dict, outputs: dict):
+    def run_with_outputs(self, nsteps: int, paths: dict, outputs: dict, on_step=None):
         """Run `nsteps` steps one launch at a time and write the reference's
         output files: rawData (sInfo, dep, w, v, gFF, hFF, gE, gI, hE, hI, gap
         per step, src/patch.cu:4688-4716, 6139-6164, src/stats.cu:38-97),
         LGN_fr and LGN_sp.  The cortical g/h of step t are known after the
-        recurrent increments of its spikes, i.e. at step t+1 (or finalize)."""
+        recurrent increments of its spikes, i.e. at step t+1 (or finalize).
+        on_step(self, i), if given, runs after step i (0-based) of the call,
+        e.g. to write a snapshot."""
         from . import formats as F
         sp = self.net.spec
         nV1, npf = sp.nV1, self.tm["ntPerFrame"]
@@ -354,6 +366,7 @@ class GpuSim:
                                       ptr(sInfo), ptr(dep), ptr(gprev)), "vc_v1_step")
             self.cur = nxt
             self.it += 1
+            self.sInfo_last = sInfo
             if raw:
                 if held is not None:
                     gE, gI, hE, hI = (x.cpu().numpy() for x in g_parts(gprev))
@@ -365,6 +378,8 @@ class GpuSim:
                 e = torch.cuda.Event()
                 e.record(self.s_v1)
                 self.v1_done[buf] = e
+            if on_step is not None:
+                on_step(self, i)
         if raw:
             self.finalize()
             torch.cuda.synchronize()
@@ -374,6 +389,48 @@ class GpuSim:
         for w in (fr_w, sp_w):
             if w:
                 w.close()
+
+    def seek(self, it0: int):
+        """Restart the step pipeline at step it0 (a multiple of ntPerFrame):
+        no pending spikes, the frames' spatial responses that the temporal
+        kernels of the period reach back to recomputed from the resident
+        stimulus, and the LGN stage due from period it0 / ntPerFrame.  The V1
+        and LGN state are left as they are (snapshot.restore sets them)."""
+        tm, sp = self.tm, self.net.spec
+        npf, maxF = tm["ntPerFrame"], tm["maxFrame"]
+        if it0 % npf != 0:
+            raise ValueError(f"a run can restart at a frame period boundary only "
+                             f"(step {it0}, {npf} steps per frame)")
+        torch.cuda.synchronize()
+        K = it0 // npf
+        self._frame_response(self.init_frame, 0)
+        for slot in range(1, maxF):
+            self.Rring[slot].copy_(self.Rring[0])
+        for fidx in range(max(0, K - maxF + 1), K):
+            if self.trialFrameStride:
+                frame, ctr = self.frames[0, fidx % self.nF], ptr(self.bt["cellTrial"])
+            else:
+                frame, ctr = self.frames[fidx % self.nF], None
+            check(self.lib.vc_lgn_frame_response(None, ptr(frame), sp.width, sp.height, sp.nLGN,
+                                                 sp.nSpatialSample1D, ptr(self.SCx),
+                                                 ptr(self.SCy), ptr(self.SW), ptr(self.coneType),
+                                                 ctr, self.trialFrameStride,
+                                                 ptr(self.Rring[fidx % maxF])),
+                  "vc_lgn_frame_response")
+        if self.ev is not None:
+            for m in self.ffMask2:
+                m.zero_()
+        for i in range(2):
+            self.masks[i].zero_()
+            self.og[i].zero_()
+            self.oh[i].zero_()
+        self.cur = 0
+        self.it = it0
+        self.lgn_next = K
+        self.lgn_done = [None, None]
+        self.v1_done = [None, None]
+        torch.cuda.synchronize()
+        self.s_lgn.wait_stream(self.s_v1)
 
     def finalize(self):
         """Apply the last step's pending recurrent increments (the reference

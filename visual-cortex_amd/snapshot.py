@@ -1,0 +1,232 @@
+"""Snapshots of a running simulation in the reference's snapshot layout, and
+resuming from them (src/patch.cu:6793-6912 writes, :5489-5684 reads).
+
+File layout, in order (little endian; u32 unless noted):
+
+    it0 (the next step), frameCycle, iFrameHead, oldFrameHead, currentFrame,
+    iFramePhaseHead, iFramePhaseTail, mFramePhaseHead, mFramePhaseTail,
+    iStatus, currentTimeSlot
+    iModel (i32), learning (i32)
+    nV1, nLGN, ngTypeFF, ngTypeE, ngTypeI, nType, trainDepth
+    LGN  leftTimeRate f32[nLGN], lastNegLogRand f32[nLGN], generator[nLGN]
+    V1   rGenCond generator[nV1], rNoisy generator[nV1], tBack f32[nV1]
+    spikeTrain f32[trainDepth][nV1]
+    state block f32: depC[nV1], (w[nV1] if iModel == 1), v[nV1],
+         gFF[ngFF][nV1], hFF[ngFF][nV1], gE[ngE][nV1], gI[ngI][nV1],
+         hE[ngE][nV1], hI[ngI][nV1], gap[nblock*nI]
+
+A generator slot is 48 bytes, the size of the reference's MRG32k3a state.
+The random numbers here come from counter-based Philox streams (the stream
+id, the entity and the step are the counter), so a slot holds that stream's
+identity instead of a state: u32 'PHLX' tag, u32 entity, u32 key low, u32 key
+high, u32 stream id (0xffffffff: none), zeros.  The frame-ring fields are
+this simulator's: currentFrame frames have entered, iFrameHead =
+currentFrame mod maxFrame, the parvo phase fields and iStatus are 0.  With no
+axonal delays trainDepth is 1 and the spike train is the last step's output.
+
+A snapshot is taken at a frame-period boundary (the LGN stage runs a frame
+period ahead of V1).  Before writing, the last step's pending recurrent
+increments are added to the state, as the reference's end-of-step recal_G
+has done when it writes; the continuing run is unchanged by this, and a run
+restored from the file continues bit for bit.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import struct
+
+import numpy as np
+import torch
+
+from . import abi
+from .abi import check, ptr
+
+GEN_BYTES = 48
+TAG = 0x584C4850                  # 'PHLX'
+NO_STREAM = 0xFFFFFFFF
+STREAM_LGN_SPIKE = 1              # the Philox stream ids of the kernels
+STREAM_V1_BG = 3
+HEADER_U32 = 11
+
+
+def generator_slots(n: int, key_lo: int, key_hi: int, stream: int) -> bytes:
+    """n 48-byte generator slots naming Philox stream `stream` of entities 0..n-1."""
+    a = np.zeros((n, GEN_BYTES // 4), np.uint32)
+    a[:, 0] = TAG
+    a[:, 1] = np.arange(n, dtype=np.uint32)
+    a[:, 2] = key_lo
+    a[:, 3] = key_hi
+    a[:, 4] = stream
+    return a.tobytes()
+
+
+def state_block_floats(nV1: int, iModel: int, ngFF: int, ngE: int, ngI: int, mI: int) -> int:
+    return nV1 * (2 + (1 if iModel == 1 else 0) + 2 * ngFF + 2 * (ngE + ngI)) + mI
+
+
+def file_size(nV1, nLGN, iModel, ngFF, ngE, ngI, mI, trainDepth=1) -> int:
+    """Bytes of a snapshot (the reference's section sizes, src/patch.cu:5479, 3902-3907)."""
+    return (4 * HEADER_U32 + 4 * 2 + 4 * 7 + nLGN * (8 + GEN_BYTES) + nV1 * (2 * GEN_BYTES + 4)
+            + 4 * trainDepth * nV1 + 4 * state_block_floats(nV1, iModel, ngFF, ngE, ngI, mI))
+
+
+def write_file(path, h: dict, lgn: dict, v1: dict, spike_train: np.ndarray, block: dict,
+               keys=(0, 0)):
+    """Write a snapshot from host arrays.  h: the header fields; lgn:
+    leftTimeRate, lastNegLogRand; v1: tBack; block: depC, w (iModel 1), v, gFF,
+    hFF, gE, gI, hE, hI, gap."""
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).tobytes()
+    nV1, nLGN = h["nV1"], h["nLGN"]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<11I", h["it0"], h["frameCycle"], h["iFrameHead"], h["oldFrameHead"],
+                            h["currentFrame"], 0, 0, 0, 0, h.get("iStatus", 0),
+                            h.get("currentTimeSlot", 0)))
+        f.write(struct.pack("<2i", h["iModel"], h.get("learning", 0)))
+        f.write(struct.pack("<7I", nV1, nLGN, h["ngTypeFF"], h["ngTypeE"], h["ngTypeI"],
+                            h["nType"], h.get("trainDepth", 1)))
+        f.write(f32(lgn["leftTimeRate"]))
+        f.write(f32(lgn["lastNegLogRand"]))
+        f.write(generator_slots(nLGN, keys[0], keys[1], STREAM_LGN_SPIKE))
+        f.write(generator_slots(nV1, keys[0], keys[1], STREAM_V1_BG))
+        f.write(generator_slots(nV1, keys[0], keys[1], NO_STREAM))
+        f.write(f32(v1["tBack"]))
+        f.write(f32(spike_train))
+        f.write(f32(block["depC"]))
+        if h["iModel"] == 1:
+            f.write(f32(block["w"]))
+        for k in ("v", "gFF", "hFF", "gE", "gI", "hE", "hI", "gap"):
+            f.write(f32(block[k]))
+
+
+def read_file(path) -> dict:
+    """Read a snapshot written by write_file (or by the reference, whose
+    generator slots are then MRG32k3a states, returned as raw bytes)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    o = 0
+
+    def take(fmt):
+        nonlocal o
+        v = struct.unpack_from(fmt, data, o)
+        o += struct.calcsize(fmt)
+        return v
+
+    def floats(n):
+        nonlocal o
+        a = np.frombuffer(data, np.float32, n, o).copy()
+        o += 4 * n
+        return a
+
+    def raw(n):
+        nonlocal o
+        b = data[o:o + n]
+        o += n
+        return b
+
+    names = ("it0", "frameCycle", "iFrameHead", "oldFrameHead", "currentFrame",
+             "iFramePhaseHead", "iFramePhaseTail", "mFramePhaseHead", "mFramePhaseTail",
+             "iStatus", "currentTimeSlot")
+    h = dict(zip(names, take("<11I")))
+    h["iModel"], h["learning"] = take("<2i")
+    if h["learning"]:
+        raise ValueError("snapshots of plastic runs (learning != 0) are not supported")
+    h.update(zip(("nV1", "nLGN", "ngTypeFF", "ngTypeE", "ngTypeI", "nType", "trainDepth"),
+                 take("<7I")))
+    nV1, nLGN = h["nV1"], h["nLGN"]
+    out = {"header": h}
+    out["leftTimeRate"] = floats(nLGN)
+    out["lastNegLogRand"] = floats(nLGN)
+    out["lgn_generators"] = raw(nLGN * GEN_BYTES)
+    out["rGenCond"] = raw(nV1 * GEN_BYTES)
+    out["rNoisy"] = raw(nV1 * GEN_BYTES)
+    out["tBack"] = floats(nV1)
+    out["spikeTrain"] = floats(h["trainDepth"] * nV1).reshape(h["trainDepth"], nV1)
+    ngFF, ngE, ngI = h["ngTypeFF"], h["ngTypeE"], h["ngTypeI"]
+    rest = (len(data) - o) // 4
+    mI = rest - state_block_floats(nV1, h["iModel"], ngFF, ngE, ngI, 0)
+    if mI < 0 or (len(data) - o) % 4:
+        raise ValueError(f"{path}: truncated snapshot ({len(data)} bytes)")
+    out["depC"] = floats(nV1)
+    if h["iModel"] == 1:
+        out["w"] = floats(nV1)
+    out["v"] = floats(nV1)
+    for k, ng in (("gFF", ngFF), ("hFF", ngFF), ("gE", ngE), ("gI", ngI), ("hE", ngE),
+                  ("hI", ngI)):
+        out[k] = floats(ng * nV1).reshape(ng, nV1)
+    out["gap"] = floats(mI)
+    assert o == len(data)
+    return out
+
+
+def save(sim, path) -> dict:
+    """Snapshot of `sim` (a GpuSim at a frame-period boundary) to `path`;
+    returns the header."""
+    net, sp, tm = sim.net, sim.net.spec, sim.tm
+    npf, maxF = tm["ntPerFrame"], tm["maxFrame"]
+    if getattr(net, "batch", None) is not None:
+        raise ValueError("snapshots of batched trials are not supported")
+    if sim.it % npf != 0:
+        raise ValueError(f"snapshot at step {sim.it}: snapshots are taken at frame-period "
+                         f"boundaries ({npf} steps)")
+    K = sim.it // npf
+    sim.finalize()                          # pending increments into the state
+    torch.cuda.synchronize()
+    if K == 0:
+        lgn = torch.stack([sim.lTR, sim.lNL])
+    else:
+        lgn = sim.lgn_at[K % 2]             # the LGN state at the start of period K
+    dep = torch.empty(sp.nV1, dtype=torch.float32, device=sim.dev)
+    check(sim.lib.vc_v1_dep_current(None, C.byref(sim.p), C.byref(sim.s), ptr(dep)),
+          "vc_v1_dep_current")
+    sim.check()
+    st = {k: v.cpu().numpy() for k, v in sim.state.items()}
+    ngFF, ngE, ngI = int(sim.p.ngFF), int(sim.p.ngE), int(sim.p.ngI)
+    h = dict(it0=sim.it, frameCycle=K // maxF, iFrameHead=K % maxF,
+             oldFrameHead=(K - 1) % maxF, currentFrame=K, iModel=sp.iModel, learning=0,
+             nV1=sp.nV1, nLGN=sp.nLGN, ngTypeFF=ngFF, ngTypeE=ngE, ngTypeI=ngI,
+             nType=int(sim.p.nType), trainDepth=1)
+    lg = lgn.cpu().numpy()
+    block = dict(depC=dep.cpu().numpy(), w=st["w"], v=st["v"], gFF=st["gFF"][:ngFF],
+                 hFF=st["hFF"][:ngFF], gE=st["gE"][:ngE], gI=st["gI"][:ngI],
+                 hE=st["hE"][:ngE], hI=st["hI"][:ngI],
+                 gap=np.zeros(sp.nblock * sp.nI, np.float32))
+    write_file(path, h, dict(leftTimeRate=lg[0], lastNegLogRand=lg[1]), dict(tBack=st["tBack"]),
+               sim.sInfo_last.cpu().numpy()[None], block,
+               keys=(int(sim.p.seed_lo), int(sim.p.seed_hi)))
+    return h
+
+
+def restore(sim, path, as_init: bool = False) -> dict:
+    """Load the snapshot at `path` into `sim` (a GpuSim of the same network).
+    as_init (the reference's asInit): take the state as the initial
+    condition and start at step 0; otherwise resume at the snapshot's step."""
+    net, sp = sim.net, sim.net.spec
+    d = read_file(path)
+    h = d["header"]
+    ngFF, ngE, ngI = int(sim.p.ngFF), int(sim.p.ngE), int(sim.p.ngI)
+    mine = (sp.iModel, sp.nV1, sp.nLGN, ngFF, ngE, ngI)
+    theirs = (h["iModel"], h["nV1"], h["nLGN"], h["ngTypeFF"], h["ngTypeE"], h["ngTypeI"])
+    if mine != theirs:
+        raise ValueError(f"snapshot {path} is of another network: (iModel, nV1, nLGN, ngFF, "
+                         f"ngE, ngI) = {theirs}, this one {mine}")
+    dep = torch.empty(sp.nV1, dtype=torch.float32, device=sim.dev)
+    check(sim.lib.vc_v1_dep_current(None, C.byref(sim.p), C.byref(sim.s), ptr(dep)),
+          "vc_v1_dep_current")
+    if not np.array_equal(dep.cpu().numpy().view(np.uint32), d["depC"].view(np.uint32)):
+        raise ValueError(f"snapshot {path}: its depC differs from this network's tonic drive")
+    sim.seek(0 if as_init else h["it0"])
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=sim.dev)
+    st = sim.state
+    st["v"].copy_(T(d["v"]))
+    st["tBack"].copy_(T(d["tBack"]))
+    if h["iModel"] == 1:
+        st["w"].copy_(T(d["w"]))
+    for k, ng in (("gFF", ngFF), ("hFF", ngFF), ("gE", ngE), ("gI", ngI), ("hE", ngE),
+                  ("hI", ngI)):
+        st[k][:ng].copy_(T(d[k]))
+    sim.lTR.copy_(T(d["leftTimeRate"]))
+    sim.lNL.copy_(T(d["lastNegLogRand"]))
+    sim.sInfo_step.copy_(T(d["spikeTrain"][-1]))
+    sim.sInfo_last = sim.sInfo_step
+    torch.cuda.synchronize()
+    return h
