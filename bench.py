@@ -205,7 +205,10 @@ def bench_batched(args):
     bnet = batch.batch_networks(nets)
     log(f"[bench] {len(nets)} trials, blocks {[s.nblock for s in specs]}, nV1={bnet.spec.nV1}, "
         f"built in {time.time() - t0:.1f}s")
-    sim = GpuSim(bnet, record=False)
+    t0 = time.time()
+    sim = GpuSim(bnet, record=False, tune=True)
+    log(f"[bench] autotuned one-step launch: stepWG {sim.shape[0]}, gatherTile {sim.shape[1]} "
+        f"({time.time() - t0:.1f}s)")
     sim.run(W)
     torch.cuda.synchronize()
     events = []
@@ -215,6 +218,7 @@ def bench_batched(args):
     tb = time.perf_counter() - t0
     launch = sum(a.elapsed_time(b) for a, b, n in events) / 1e3 / sum(n for a, b, n in events)
     log(f"[bench] batched: {K} steps in {tb:.3f}s ({tb / K * 1e6:.1f} us/step, V1 launch {launch * 1e6:.1f} us)")
+    sim_shape = sim.shape
     del sim
     ts = 0.0
     for n in nets:
@@ -237,7 +241,8 @@ def bench_batched(args):
         "data": "synthetic (seeded networks, drifting gratings at 8 orientations)",
         "config": {"workload": f"{len(nets)} batched trials, 8-96 blocks x 1024 neurons, "
                                f"400 E + 100 I partners, 20 LGN inputs, dt={sp.dt} ms",
-                   "nV1_total": sp.nV1, "blocks": [s.nblock for s in specs]},
+                   "nV1_total": sp.nV1, "blocks": [s.nblock for s in specs],
+                   "stepWG": sim_shape[0], "gatherTile": sim_shape[1]},
         "serial_trials_per_s": len(nets) * (K * sp.dt / 500.0) / ts,
         "speedup_vs_serial": ts / tb,
         "neuron_steps_per_s": sp.nV1 * K / tb,

@@ -82,6 +82,12 @@ typedef struct {
     const uint32_t *blockTrial;
     const uint32_t *trialV1Off;
     const uint32_t *trialSeed;
+    /* Launch shape of the one-step kernels (vc_v1_step, vc_v1_run,
+     * vc_v1_step_snapshot); 0 = default.  stepWG: threads per workgroup (64,
+     * 128 or 256, dividing blockSize); gatherTile: masked-CSR strengths in
+     * flight per lane in the recurrent gather (4, 8 or 16).  Results do not
+     * depend on them (visual-cortex_amd/tune.py picks them by timing). */
+    uint32_t stepWG, gatherTile;
 } vc_v1_params;
 
 /* Persistent V1 state, all [..][nV1] float32, receptor-major. */

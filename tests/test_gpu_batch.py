@@ -38,3 +38,27 @@ def test_batched_equals_serial_and_oracle(pkg):
     o.run(192)
     assert np.array_equal(o.sInfo().view(np.uint32), sb.view(np.uint32))
     assert np.array_equal(o.lgn_sinfo().view(np.uint32), g.lgn_sinfo().view(np.uint32))
+
+
+def test_autotuned_shapes_are_bitwise_equal(pkg):
+    """Every one-step launch shape the autotuner may pick gives the same
+    results bit for bit (workgroup size and gather tile change only the
+    schedule), and the autotuner returns one of them."""
+    import numpy as np
+    from visual_cortex_amd import network, tune
+    from visual_cortex_amd.sim import GpuSim
+    print("test_autotuned_shapes_are_bitwise_equal: tiny, 192 steps", flush=True)
+    net = network.build_network(network.spec_tiny(nt=256, bgStrength=(0.4, 0.4)))
+    ref = GpuSim(net, persistent=False)
+    ref.run(192)
+    s_ref = ref.sInfo()
+    assert (s_ref >= 1).sum() > 0
+    for wg, t in tune.candidates(net.spec.blockSize):
+        g = GpuSim(net, persistent=False)
+        g.p.stepWG, g.p.gatherTile = wg, t
+        g.run(192)
+        assert np.array_equal(g.sInfo().view(np.uint32), s_ref.view(np.uint32)), (wg, t)
+    tuned = GpuSim(net, persistent=False, tune=True)
+    assert tuned.shape in tune.candidates(net.spec.blockSize)
+    tuned.run(192)
+    assert np.array_equal(tuned.sInfo().view(np.uint32), s_ref.view(np.uint32))

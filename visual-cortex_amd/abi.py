@@ -52,6 +52,7 @@ class V1Params(C.Structure):
         ("pBG", _arr(f32, VC_MAX_NTYPE * VC_MAX_NG)),
         ("seed_lo", u32), ("seed_hi", u32), ("trial", u32),
         ("blockTrial", vp), ("trialV1Off", vp), ("trialSeed", vp),
+        ("stepWG", u32), ("gatherTile", u32),
     ]
 
 

@@ -27,8 +27,14 @@ constexpr int kWG = 256;           // max workgroup size
 constexpr int kMaxEnt = 512;       // neighbour-tile entries staged in LDS per wavefront
 constexpr int kBatch = 64;
 constexpr int kFF = 32;            // LGN synapses loaded per round
-constexpr int kW = 8;              // weight loads in flight per lane (masked CSR)
-constexpr int kWD = 32;            // weight loads in flight per lane (dense rows)
+// The recurrent gather's weight layout and tile, a template parameter kG of
+// the kernels: 0 = dense rows (kWD strengths in flight per lane), else the
+// masked CSR with kG strengths in flight per lane (4, 8 or 16).  A compile-
+// time choice: the dense path's 32 loads in flight would otherwise set the
+// register count of every kernel.
+constexpr int kWD = 32;
+constexpr uint32_t kTileStep = 4;  // default kG of the one-step kernel
+constexpr uint32_t kTileMulti = 8; // kG of the persistent kernel
 constexpr int kEv = 4;             // LGN spike events prefetched per neuron
 
 
@@ -62,13 +68,6 @@ __device__ __forceinline__ void ff_apply(FFAcc &a, const FFShape &sh, const floa
             }
         }
     }
-}
-// The same as a call, for the ELL path whose unrolled synapse slots would
-// otherwise each hold a copy of the body.
-__device__ __noinline__ FFAcc ff_add(FFAcc a, const FFShape sh, float f, float nsp, float tsp,
-                                     float dt, float tBack, bool backing) {
-    ff_apply(a, sh, sh.p, f, nsp, tsp, dt, tBack, backing);
-    return a;
 }
 
 // Loads and stores of the spike buffers.  Within one launch of the
@@ -171,7 +170,7 @@ struct RecIn {
 // brow/bslot (this wavefront's LDS rows) to the calling lane's sums.  With
 // kRec the increments of spike k are the four tagged words at ri.words +
 // bsrc[k]; otherwise they are read from prev.og/oh.
-template <bool kCoh, bool kRec>
+template <bool kCoh, bool kRec, int kG>
 __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_conn &rec,
                                             const vc_spikes &prev, uint32_t K, uint32_t lane,
                                             uint32_t nSeg, uint32_t myseg, int t,
@@ -180,7 +179,8 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
                                             float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
                                             float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
     // lane kk fetches the row-segment header and the increments of spike kk
-    const bool dense = rec.wDense != nullptr;
+    constexpr bool dense = kG == 0;
+    constexpr int kW = kG > 0 ? kG : 1;
     uint64_t rm = 0;
     uint32_t off = 0, isI = 0, row = 0, slot = 0;
     float og0 = 0.f, oh0 = 0.f, og1 = 0.f, oh1 = 0.f;
@@ -215,7 +215,7 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
     };
     const float pE0 = p.pE[t * p.ngE + 0], pE1 = p.pE[t * p.ngE + (p.ngE > 1 ? 1 : 0)];
     const float pI0 = p.pI[t * p.ngI + 0], pI1 = p.pI[t * p.ngI + (p.ngI > 1 ? 1 : 0)];
-    if (dense) {
+    if constexpr (dense) {
         // a spike's strengths for this wavefront are one coalesced 256-byte
         // row segment; absent synapses hold +0 and add +0 to the sums
         const float *wrow = rec.wDense + (size_t)myseg * 64u + lane;
@@ -280,7 +280,7 @@ struct GatherLds {
 // the listed spikes' increments for its own 64 posts, 64 spikes per batch.
 // Lists longer than kList are done in several passes.  Every thread of the
 // workgroup must call it.
-template <bool kCoh, bool kRec>
+template <bool kCoh, bool kRec, int kG>
 __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_conn &rec,
                                         const vc_spikes &prev, uint32_t b, uint32_t myseg,
                                         uint32_t nSeg, uint32_t nEnt, int t,
@@ -349,7 +349,7 @@ __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_c
         const uint32_t K = (total - L0) < (uint32_t)kList ? (total - L0) : (uint32_t)kList;
         for (uint32_t k0 = 0; k0 < K; k0 += (uint32_t)kBatch) {
             const uint32_t kb = (K - k0) < (uint32_t)kBatch ? (K - k0) : (uint32_t)kBatch;
-            flush_batch<kCoh, kRec>(p, rec, prev, kb, lane, nSeg, myseg, t, g.row + k0,
+            flush_batch<kCoh, kRec, kG>(p, rec, prev, kb, lane, nSeg, myseg, t, g.row + k0,
                                     g.slot + k0, g.src + k0, ri, lgE, lhE, lgI, lhI);
         }
         __syncthreads();                    // the list is rewritten by the next pass
@@ -557,25 +557,50 @@ __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) spk |= (uint32_t)__shfl_xor((int)spk, off);
     spk = __builtin_amdgcn_readfirstlane(spk);           // slots that spiked for any lane
+    // one inlined input body: the slots that spiked for some lane (in list
+    // order, a wave-uniform index into the registers), then the synapses
+    // past kFF, then the background input
+    uint32_t j = 0;
+    uint32_t i = kFF;
+    bool bgLeft = bg.nsp > 0.0f;
+    for (;;) {
+        float fv = 0.0f, sI = 0.0f, pm[VC_MAX_NG];
+        bool isBG = false;
+        if (spk != 0u) {
+            j = (uint32_t)__builtin_ctz(spk);
+            spk &= spk - 1u;
 #pragma unroll
-    for (int j = 0; j < kFF; ++j) {
-        if ((spk >> j) & 1u) {
-            const float sI = f.sv[j];
-            if (sI >= 1.0f) {
-                const float nsp = (float)(uint32_t)floorf(sI);
-                acc = ff_add(acc, shFF, f.fv[j], nsp, (sI - nsp) * dt, dt, tBack, backing);
+            for (int k = 0; k < kFF; ++k) {      // j is wave-uniform: scalar compares
+                if ((uint32_t)k == j) {
+                    fv = f.fv[k];
+                    sI = f.sv[k];
+                }
             }
+        } else if (i < f.m) {                            // lists longer than kFF
+            const uint32_t lid = i * nV1 + n;
+            ++i;
+            sI = lgnSInfo[ff.idLGN[lid]];
+            fv = ff.sLGN[lid];
+        } else if (bgLeft) {
+            bgLeft = false;
+            isBG = true;
+        } else {
+            break;                                       // this lane is done
         }
-    }
-    for (uint32_t i = kFF; i < f.m; ++i) {                // lists longer than kFF
-        const uint32_t lid = i * nV1 + n;
-        const float sI = lgnSInfo[ff.idLGN[lid]];
-        if (sI >= 1.0f) {
-            const float nsp = (float)(uint32_t)floorf(sI);
-            acc = ff_add(acc, shFF, ff.sLGN[lid], nsp, (sI - nsp) * dt, dt, tBack, backing);
+        float nsp, tsp;
+        if (isBG) {
+            fv = bg.f;
+            nsp = bg.nsp;
+            tsp = bg.tsp;
+        } else {
+            if (!(sI >= 1.0f)) fv = 0.0f;                // this lane's cell did not spike
+            nsp = (float)(uint32_t)floorf(sI);
+            tsp = (sI - nsp) * dt;
         }
+#pragma unroll
+        for (int ig = 0; ig < VC_MAX_NG; ++ig) pm[ig] = isBG ? shBG.p[ig] : shFF.p[ig];
+        ff_apply(acc, shFF, pm, fv, nsp, tsp, dt, tBack, backing);
     }
-    if (bg.nsp > 0.0f) acc = ff_add(acc, shBG, bg.f, bg.nsp, bg.tsp, dt, tBack, backing);
 }
 
 // Stage c-FF of one neuron: the LGN and background inputs into the FF
@@ -824,7 +849,7 @@ __device__ __forceinline__ void compact(const vc_v1_params &p, const vc_spikes &
 }
 
 // ------------------------------------------------------ one-step kernel --
-template <bool kUpdate, bool kEvents>
+template <bool kUpdate, bool kEvents, int kG>
 __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v1_state s,
                                                  vc_ff_conn ff, vc_rec_conn rec,
                                                  vc_spikes prev, vc_spikes next,
@@ -866,7 +891,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
     __syncthreads();
     const RecIn noRec{nullptr, 0u, 0u, 0u, nullptr};
-    rec_gather<false, false>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, noRec, glds,
+    rec_gather<false, false, kG>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, noRec, glds,
                              lgE, lhE, lgI, lhI);
     // conductances at the end of the previous step (reference: after recal_G)
 #pragma unroll
@@ -946,7 +971,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
 // step's spikes as vc_v1_run leaves them.  Every wait is bounded: on a
 // time-out *status is set and the kernel runs to its end (results invalid,
 // no hang).  The grid must fit on the device at once.
-template <bool kEvents>
+template <bool kEvents, int kG>
 __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_v1_state s,
                                                   vc_ff_conn ff, vc_rec_conn rec,
                                                   vc_spikes sa, vc_spikes sb,
@@ -1027,7 +1052,7 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
         float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
         float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
         __syncthreads();
-        rec_gather<true, true>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, ri, glds, lgE,
+        rec_gather<true, true, kG>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, ri, glds, lgE,
                                lhE, lgI, lhI);
 #pragma unroll
         for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
@@ -1060,6 +1085,46 @@ __global__ void k_dep_current(vc_v1_params p, const float *__restrict__ tonicDep
 __global__ void k_clear_masks(uint64_t *mask, uint32_t nTiles) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nTiles) mask[i] = 0ull;
+}
+
+// Launch of the one-step kernel with gather variant g (0 = dense rows, else
+// the masked-CSR tile) and wg threads per workgroup.
+template <bool kUpdate, bool kEvents>
+void launch_step(uint32_t g, uint32_t wg, hipStream_t st, const vc_v1_params &p,
+                 const V1Const &k, const vc_v1_state &s, const vc_ff_conn &ff,
+                 const vc_rec_conn &rec, const vc_spikes &prev, const vc_spikes &next,
+                 const float *lgn, uint32_t *fm, uint32_t step, float *o_sInfo, float *o_dep,
+                 float *o_g, float *o_snap) {
+    const dim3 grid(p.nV1 / wg), block(wg);
+#define VC_STEP(G)                                                                              \
+    hipLaunchKernelGGL((k_v1_step<kUpdate, kEvents, G>), grid, block, 0, st, p, k, s, ff, rec,  \
+                       prev, next, lgn, fm, step, o_sInfo, o_dep, o_g, o_snap)
+    switch (g) {
+    case 0: VC_STEP(0); break;
+    case 4: VC_STEP(4); break;
+    case 16: VC_STEP(16); break;
+    default: VC_STEP(8); break;
+    }
+#undef VC_STEP
+}
+
+// The launch shape of the one-step kernel: the gather variant (0 when the
+// dense rows are given, else p->gatherTile or the default tile) and the
+// workgroup size (p->stepWG or the largest of 256/128/64 dividing blockSize).
+int step_shape(const vc_v1_params *p, const vc_rec_conn *rec, uint32_t *g, uint32_t *wg) {
+    const uint32_t t = p->gatherTile ? p->gatherTile : kTileStep;
+    if (t != 4 && t != 8 && t != 16) {
+        vc_set_error("gatherTile %u must be 0 (default), 4, 8 or 16", p->gatherTile);
+        return VC_E_INVALID;
+    }
+    *g = rec->wDense ? 0u : t;
+    *wg = p->stepWG ? p->stepWG : wg_size(p->blockSize);
+    if (*wg % 64 != 0 || *wg > (uint32_t)kWG || p->blockSize % *wg != 0) {
+        vc_set_error("stepWG %u must be 64, 128 or 256 and divide blockSize %u", p->stepWG,
+                     p->blockSize);
+        return VC_E_INVALID;
+    }
+    return VC_OK;
 }
 
 int check_params(const vc_v1_params *p, const vc_rec_conn *rec) {
@@ -1128,15 +1193,14 @@ extern "C" int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state
         return VC_E_INVALID;
     }
     V1Const k = make_const(p);
-    uint32_t wg = wg_size(p->blockSize);
+    uint32_t g, wg;
+    if ((rc = step_shape(p, rec, &g, &wg))) return rc;
     if (ffMask)
-        hipLaunchKernelGGL((k_v1_step<true, true>), dim3(p->nV1 / wg), dim3(wg), 0,
-                           (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
-                           step, out_sInfo, out_dep, out_g_prev, nullptr);
+        launch_step<true, true>(g, wg, (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next,
+                                lgnSInfo, ffMask, step, out_sInfo, out_dep, out_g_prev, nullptr);
     else
-        hipLaunchKernelGGL((k_v1_step<true, false>), dim3(p->nV1 / wg), dim3(wg), 0,
-                           (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
-                           step, out_sInfo, out_dep, out_g_prev, nullptr);
+        launch_step<true, false>(g, wg, (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next,
+                                 lgnSInfo, ffMask, step, out_sInfo, out_dep, out_g_prev, nullptr);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
@@ -1161,15 +1225,14 @@ extern "C" int vc_v1_step_snapshot(void *stream, const vc_v1_params *p, const vc
         return VC_E_INVALID;
     }
     V1Const k = make_const(p);
-    uint32_t wg = wg_size(p->blockSize);
+    uint32_t g, wg;
+    if ((rc = step_shape(p, rec, &g, &wg))) return rc;
     if (ffMask)
-        hipLaunchKernelGGL((k_v1_step<true, true>), dim3(p->nV1 / wg), dim3(wg), 0,
-                           (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
-                           step, out_sInfo, nullptr, nullptr, out_snap);
+        launch_step<true, true>(g, wg, (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next,
+                                lgnSInfo, ffMask, step, out_sInfo, nullptr, nullptr, out_snap);
     else
-        hipLaunchKernelGGL((k_v1_step<true, false>), dim3(p->nV1 / wg), dim3(wg), 0,
-                           (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next, lgnSInfo, ffMask,
-                           step, out_sInfo, nullptr, nullptr, out_snap);
+        launch_step<true, false>(g, wg, (hipStream_t)stream, *p, k, *s, *ff, *rec, *prev, *next,
+                                 lgnSInfo, ffMask, step, out_sInfo, nullptr, nullptr, out_snap);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
@@ -1194,19 +1257,18 @@ extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state 
         return VC_E_INVALID;
     }
     V1Const k = make_const(p);
-    const uint32_t wg = wg_size(p->blockSize);
+    uint32_t g, wg;
+    if ((rc = step_shape(p, rec, &g, &wg))) return rc;
     for (uint32_t i = 0; i < nsteps; ++i) {
         const uint32_t nx = 1u - cur;
         if (ffMask)
-            hipLaunchKernelGGL((k_v1_step<true, true>), dim3(p->nV1 / wg), dim3(wg), 0,
-                               (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur], spk2[nx],
-                               lgnSInfo + i * lgnStride, ffMask + i * maskStride, it0 + i,
-                               out_sInfo + i * outStride, nullptr, nullptr, nullptr);
+            launch_step<true, true>(g, wg, (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur],
+                                    spk2[nx], lgnSInfo + i * lgnStride, ffMask + i * maskStride,
+                                    it0 + i, out_sInfo + i * outStride, nullptr, nullptr, nullptr);
         else
-            hipLaunchKernelGGL((k_v1_step<true, false>), dim3(p->nV1 / wg), dim3(wg), 0,
-                               (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur], spk2[nx],
-                               lgnSInfo + i * lgnStride, nullptr, it0 + i,
-                               out_sInfo + i * outStride, nullptr, nullptr, nullptr);
+            launch_step<true, false>(g, wg, (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur],
+                                     spk2[nx], lgnSInfo + i * lgnStride, nullptr, it0 + i,
+                                     out_sInfo + i * outStride, nullptr, nullptr, nullptr);
         cur = nx;
     }
     VC_HIP_CHECK(hipGetLastError());
@@ -1252,7 +1314,12 @@ extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const v
     if (nsteps == 0) return VC_OK;
     const uint32_t wg = wg_size(p->blockSize);
     const uint32_t grid = p->nV1 / wg;
-    const void *fn = ffMask ? (const void *)k_v1_multi<true> : (const void *)k_v1_multi<false>;
+    constexpr int kT = (int)kTileMulti;
+    const bool dense = rec->wDense != nullptr;
+    const void *fn = ffMask ? (dense ? (const void *)k_v1_multi<true, 0>
+                                     : (const void *)k_v1_multi<true, kT>)
+                            : (dense ? (const void *)k_v1_multi<false, 0>
+                                     : (const void *)k_v1_multi<false, kT>);
     int dev = 0, ncu = 0, per = 0;
     VC_HIP_CHECK(hipGetDevice(&dev));
     VC_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1283,14 +1350,16 @@ extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const v
     // bounded, so kernels of other streams can at most delay some of them.
     // (A cooperative launch would also serialise the host with the device.)
     const V1Const k = make_const(p);
-    if (ffMask)
-        hipLaunchKernelGGL(k_v1_multi<true>, dim3(grid), dim3(wg), 0, st, *p, k, *s, *ff, *rec,
-                           spk2[cur], spk2[1u - cur], lgnSInfo, lgnStride, ffMask, maskStride,
-                           it0, nsteps, out_sInfo, outStride, words, epoch, status);
-    else
-        hipLaunchKernelGGL(k_v1_multi<false>, dim3(grid), dim3(wg), 0, st, *p, k, *s, *ff, *rec,
-                           spk2[cur], spk2[1u - cur], lgnSInfo, lgnStride, ffMask, maskStride,
-                           it0, nsteps, out_sInfo, outStride, words, epoch, status);
+#define VC_MULTI(E, G)                                                                          \
+    hipLaunchKernelGGL((k_v1_multi<E, G>), dim3(grid), dim3(wg), 0, st, *p, k, *s, *ff, *rec,    \
+                       spk2[cur], spk2[1u - cur], lgnSInfo, lgnStride, ffMask, maskStride, it0,  \
+                       nsteps, out_sInfo, outStride, words, epoch, status)
+    if (ffMask) {
+        if (dense) VC_MULTI(true, 0); else VC_MULTI(true, kT);
+    } else {
+        if (dense) VC_MULTI(false, 0); else VC_MULTI(false, kT);
+    }
+#undef VC_MULTI
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
@@ -1307,9 +1376,14 @@ extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_
     uint32_t wg = wg_size(p->blockSize);
     vc_ff_conn ff{};
     vc_spikes nx{};
-    hipLaunchKernelGGL((k_v1_step<false, false>), dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream,
-                       *p, k, *s, ff, *rec, *prev, nx, nullptr, nullptr, 0u, nullptr, nullptr,
-                       nullptr, nullptr);
+    if (rec->wDense)
+        hipLaunchKernelGGL((k_v1_step<false, false, 0>), dim3(p->nV1 / wg), dim3(wg), 0,
+                           (hipStream_t)stream, *p, k, *s, ff, *rec, *prev, nx, nullptr, nullptr,
+                           0u, nullptr, nullptr, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_v1_step<false, false, (int)kTileMulti>), dim3(p->nV1 / wg),
+                           dim3(wg), 0, (hipStream_t)stream, *p, k, *s, ff, *rec, *prev, nx,
+                           nullptr, nullptr, 0u, nullptr, nullptr, nullptr, nullptr);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -33,13 +33,8 @@ namespace {
 constexpr int kWG = 256;
 constexpr int kMaxEnt = 512;
 constexpr int kBatch = 64;
-constexpr int kW = 8;
-
-__device__ __forceinline__ float wave_sum(float x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    return x;
-}
+constexpr int kW = 32;             // strengths (and gradients) loaded per group
+constexpr int kFF = 32;            // LGN synapses of a neuron held in registers
 
 // Contribution of an input of unit size at lag ddt (single_input) and the
 // derivatives with respect to ddt: A = dod (Ed - Er), B = Er.
@@ -78,6 +73,10 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
     __shared__ uint32_t sbpre[kWG / 64][kBatch];
     __shared__ uint64_t smk[kMaxEnt];
     __shared__ uint32_t sbid[kMaxEnt];
+    // per wavefront, each lane's spike-time contribution to the kW spikes of
+    // a group, transposed for the per-spike sums (row stride 65: no bank
+    // conflicts when lane j reads row j)
+    __shared__ float sred[kWG / 64][kW][65];
     const uint32_t nV1 = p.nV1;
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t b = n / p.blockSize;
@@ -169,13 +168,43 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
         g0[ig] = g;
     }
     const uint32_t mLGN = ff.nLGN[n];
-    // LGN inputs in list order, then the background drive
-    for (uint32_t j = 0; j < mLGN; ++j) {
-        const uint32_t lid = j * nV1 + n;
-        const float sI = lgnSInfo[ff.idLGN[lid]];
-        if (!(sI >= 1.0f)) continue;
+    // The neuron's first kFF LGN synapses: strengths, cells and the cells'
+    // spike words, loaded together (two round trips).  Only synapses whose
+    // cell spiked contribute; they are visited in list order through a
+    // wave-uniform slot index (the slots that spiked for any lane), so the
+    // input arithmetic has one copy.  Synapses past kFF take a per-lane loop.
+    float fv[kFF], sv[kFF];
+    {
+        uint32_t fid[kFF];
+#pragma unroll
+        for (int j = 0; j < kFF; ++j) {
+            fv[j] = 0.0f;
+            fid[j] = 0u;
+            if ((uint32_t)j < mLGN) {
+                fv[j] = ff.sLGN[(uint32_t)j * nV1 + n];
+                fid[j] = ff.idLGN[(uint32_t)j * nV1 + n];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kFF; ++j) sv[j] = (uint32_t)j < mLGN ? lgnSInfo[fid[j]] : 0.0f;
+    }
+    uint32_t spkAny = 0;
+#pragma unroll
+    for (int j = 0; j < kFF; ++j) spkAny |= (sv[j] >= 1.0f ? 1u : 0u) << j;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) spkAny |= (uint32_t)__shfl_xor((int)spkAny, off);
+    spkAny = __builtin_amdgcn_readfirstlane(spkAny);
+    // the value of slot j (wave-uniform) of a register array
+    auto slot = [](const float (&a)[kFF], uint32_t j) {
+        float x = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kFF; ++q)
+            if ((uint32_t)q == j) x = a[q];
+        return x;
+    };
+    // LGN input (f, sI) of the neuron into g1 / g0
+    auto ff_fwd = [&](float f, float sI) {
         const float nsp = (float)(uint32_t)floorf(sI), tsp = (sI - nsp) * dt;
-        const float f = ff.sLGN[lid];
         #pragma unroll
         for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngFF) {
             const float str = f * p.pFF[t * p.ngFF + ig] * nsp;
@@ -186,6 +215,18 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
                 g0[ig] = g0[ig] + str * s0.A;
             }
         }
+    };
+    // LGN inputs in list order, then the background drive
+    for (uint32_t any = spkAny; any != 0u;) {
+        const uint32_t j = (uint32_t)__builtin_ctz(any);
+        any &= any - 1u;
+        const float sI = slot(sv, j), f = slot(fv, j);
+        if (sI >= 1.0f) ff_fwd(f, sI);
+    }
+    for (uint32_t j = kFF; j < mLGN; ++j) {
+        const uint32_t lid = j * nV1 + n;
+        const float sI = lgnSInfo[ff.idLGN[lid]];
+        if (sI >= 1.0f) ff_fwd(ff.sLGN[lid], sI);
     }
     FFIn bg{0.0f, 0.0f};
     if (p.bgStrength[t] > 0.0f) {
@@ -338,12 +379,8 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
         lhF[ig] = ig < p.ngFF ? lhf : 0.0f;
     }
     // inputs: strength gradients and the tBack dependence of the backed-up part
-    for (uint32_t j = 0; j < mLGN; ++j) {
-        const uint32_t lid = j * nV1 + n;
-        const float sI = lgnSInfo[ff.idLGN[lid]];
-        if (!(sI >= 1.0f)) continue;
+    auto ff_rev = [&](float f, float sI) {
         const float nsp = (float)(uint32_t)floorf(sI), tsp = (sI - nsp) * dt;
-        const float f = ff.sLGN[lid];
         float df = 0.0f;
         #pragma unroll
         for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) if (ig < p.ngFF) {
@@ -356,7 +393,18 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
                 ltB += f * pn * lg0[ig] * s0.dA;
             }
         }
-        dsLGN[lid] += df;
+        return df;
+    };
+    for (uint32_t any = spkAny; any != 0u;) {
+        const uint32_t j = (uint32_t)__builtin_ctz(any);
+        any &= any - 1u;
+        const float sI = slot(sv, j), f = slot(fv, j);
+        if (sI >= 1.0f) dsLGN[j * nV1 + n] += ff_rev(f, sI);
+    }
+    for (uint32_t j = kFF; j < mLGN; ++j) {
+        const uint32_t lid = j * nV1 + n;
+        const float sI = lgnSInfo[ff.idLGN[lid]];
+        if (sI >= 1.0f) dsLGN[lid] += ff_rev(ff.sLGN[lid], sI);
     }
     if (bg.nsp > 0.0f && backing && tB - bg.tsp > 0.0f) {
         #pragma unroll
@@ -390,12 +438,18 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
     // the increments of an E (I) spike, per receptor: p * (lam_g og + lam_h oh)
     // and their spike-time derivatives og' = (og + dod oh)/decay - dod oh/rise,
     // oh' = oh/rise
-    float pE[VC_MAX_NG], pI[VC_MAX_NG];
+    float pE[VC_MAX_NG], pI[VC_MAX_NG], irE[VC_MAX_NG], idE[VC_MAX_NG], irI[VC_MAX_NG],
+        idI[VC_MAX_NG];
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
         pE[ig] = ig < p.ngE ? p.pE[t * p.ngE + ig] : 0.0f;
         pI[ig] = ig < p.ngI ? p.pI[t * p.ngI + ig] : 0.0f;
+        irE[ig] = 1.0f / p.riseE[ig];
+        idE[ig] = 1.0f / p.decayE[ig];
+        irI[ig] = 1.0f / p.riseI[ig];
+        idI[ig] = 1.0f / p.decayI[ig];
     }
+    float (*red)[65] = sred[wv];
     while (true) {
         uint32_t cnt = 0;
         while (cnt < (uint32_t)kBatch) {
@@ -427,7 +481,7 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
         __builtin_amdgcn_wave_barrier();
         // lane kk holds the header of spike kk
         uint64_t rm = 0;
-        uint32_t off = 0, isI = 0, preId = 0;
+        uint32_t off = 0, isI = 0;
         float og0 = 0.f, oh0 = 0.f, og1 = 0.f, oh1 = 0.f;
         if (lane < cnt) {
             const uint32_t row = brow[lane], sw = bslot[lane];
@@ -440,7 +494,6 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
             oh0 = prev.oh[slot];
             og1 = prev.og[nV1 + slot];
             oh1 = prev.oh[nV1 + slot];
-            preId = bpre[lane];
         }
         __builtin_amdgcn_wave_barrier();
         for (uint32_t k0 = 0; k0 < cnt; k0 += kW) {
@@ -464,36 +517,42 @@ __global__ __launch_bounds__(kWG) void k_v1_adj(
 #pragma unroll
             for (int jj = 0; jj < kW; ++jj) {
                 const uint32_t kk = k0 + jj;
-                if (kk >= cnt) break;
-                const uint64_t rmk = vc_readlane_u64(rm, kk);
-                const float g[2] = {vc_readlane_f32(og0, kk), vc_readlane_f32(og1, kk)};
-                const float h[2] = {vc_readlane_f32(oh0, kk), vc_readlane_f32(oh1, kk)};
-                const uint32_t inh = vc_readlane_u32(isI, kk);
-                float contrib = 0.0f, dts = 0.0f;
-                if ((rmk >> lane) & 1ull) {
+                float dts = 0.0f;
+                if (kk < cnt) {
+                    const uint64_t rmk = vc_readlane_u64(rm, kk);
+                    const float g[2] = {vc_readlane_f32(og0, kk), vc_readlane_f32(og1, kk)};
+                    const float h[2] = {vc_readlane_f32(oh0, kk), vc_readlane_f32(oh1, kk)};
+                    const bool e = vc_readlane_u32(isI, kk) == 0u;
+                    float contrib = 0.0f;
+                    if ((rmk >> lane) & 1ull) {
 #pragma unroll
-                    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-                        float pp, lg, lhh, rise, decay, dod;
-                        if (inh == 0u) {
-                            pp = pE[ig]; lg = L.ge[ig]; lhh = L.he[ig];
-                            rise = p.riseE[ig]; decay = p.decayE[ig]; dod = p.dodE[ig];
-                        } else {
-                            pp = pI[ig]; lg = L.gi[ig]; lhh = L.hi[ig];
-                            rise = p.riseI[ig]; decay = p.decayI[ig]; dod = p.dodI[ig];
+                        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+                            const float pp = e ? pE[ig] : pI[ig];
+                            const float lg = e ? L.ge[ig] : L.gi[ig];
+                            const float lhh = e ? L.he[ig] : L.hi[ig];
+                            const float ir = e ? irE[ig] : irI[ig];
+                            const float id = e ? idE[ig] : idI[ig];
+                            const float dod = e ? p.dodE[ig] : p.dodI[ig];
+                            if (pp != 0.0f) {
+                                contrib += pp * (lg * g[ig] + lhh * h[ig]);
+                                const float dg = (g[ig] + dod * h[ig]) * id - dod * h[ig] * ir;
+                                const float dh = h[ig] * ir;
+                                dts += pp * (lg * dg + lhh * dh);
+                            }
                         }
-                        if (pp != 0.0f) {
-                            contrib += pp * (lg * g[ig] + lhh * h[ig]);
-                            const float dg = (g[ig] + dod * h[ig]) / decay - dod * h[ig] / rise;
-                            const float dh = h[ig] / rise;
-                            dts += pp * (lg * dg + lhh * dh);
-                        }
+                        dw[ix[jj]] = dv_[jj] + contrib;
                     }
-                    dw[ix[jj]] = dv_[jj] + contrib;
                 }
-                const float tot = wave_sum(wv_[jj] * dts);
-                if (lane == 0 && tot != 0.0f)
-                    atomicAdd(lamTspPrev + vc_readlane_u32(preId, kk), tot);
+                red[jj][lane] = wv_[jj] * dts;
             }
+            // per spike, the sum over the wavefront's posts: lane j sums row j
+            __builtin_amdgcn_wave_barrier();
+            if (lane < (uint32_t)kW && k0 + lane < cnt) {
+                float tot = 0.0f;
+                for (int l = 0; l < 64; ++l) tot += red[lane][l];
+                if (tot != 0.0f) atomicAdd(lamTspPrev + bpre[k0 + lane], tot);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
         __builtin_amdgcn_wave_barrier();
     }

@@ -27,7 +27,8 @@ class GpuSim:
 
     def __init__(self, net: Network, device: str = "cuda", record: bool = True,
                  state: dict | None = None, ff_events: bool | None = None,
-                 persistent: bool = True, dense_weights: bool | None = None):
+                 persistent: bool = True, dense_weights: bool | None = None,
+                 tune: bool = False):
         self.lib = abi.load_library()
         self.net = net
         sp = net.spec
@@ -114,6 +115,12 @@ class GpuSim:
         self.sInfo_step = torch.zeros(sp.nV1, dtype=torch.float32, device=d)
         self.sInfo_last = self.sInfo_step          # V1 output of the last step run (a view)
         self.s_lgn.wait_stream(self.s_v1)          # set-up work precedes the pipeline
+        # launch shape of the one-step kernel (runs whose grid cannot be
+        # resident at once): timed per workload when asked, else the default
+        self.shape = None
+        if tune:
+            from .tune import tune_step
+            self.shape = tune_step(self)
 
     def _dense_weights(self) -> torch.Tensor:
         """The masked CSR expanded on the device to the dense layout
