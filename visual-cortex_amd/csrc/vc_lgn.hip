@@ -85,6 +85,109 @@ __global__ __launch_bounds__(1024) void k_lgn_frame_response(
     }
 }
 
+// The same response with one wavefront per cell (nS a multiple of 64): lane
+// l holds samples l + 64k, k < kPer.  The reference's block reduction
+// (src/discrete_input_convol.cu:1055-1069, :1143-1211; emulated by
+// block_reduce_ref2) sums each warp of 32 consecutive samples with a 32-lane
+// tree -- here the half-wave trees of register k give warps 2k and 2k+1 --
+// and then halves over the nS/32 warp partials, here in registers.  The
+// additions are the same, so the results are identical; there is no barrier.
+template <int kPer>
+__global__ __launch_bounds__(256) void k_lgn_frame_response_w(
+    const float *__restrict__ frame0, int W, int H, uint32_t nLGN,
+    const float *__restrict__ SCx, const float *__restrict__ SCy,
+    const float *__restrict__ SW, const uint32_t *__restrict__ coneType,
+    const uint32_t *__restrict__ cellTrial, uint64_t trialFrameStride,
+    uint64_t frameStride, float *__restrict__ R) {
+    constexpr uint32_t nS = 64u * kPer, nW = 2u * kPer;
+    const uint32_t c = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                      (threadIdx.x >> 6));
+    if (c >= nLGN) return;                    // whole wavefronts
+    const uint32_t lane = threadIdx.x & 63u;
+    frame0 += (size_t)blockIdx.y * frameStride;
+    R += (size_t)blockIdx.y * nLGN * 2;
+    const float *frame = cellTrial ? frame0 + (size_t)cellTrial[c] * trialFrameStride : frame0;
+    const uint32_t tC = coneType[c], tS = coneType[nLGN + c];
+    float icS[kPer], isS[kPer], icC[kPer], sw[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t smp = lane + 64u * k;
+        const size_t iC = ((size_t)c * 2 + 0) * nS + smp;
+        const size_t iS = ((size_t)c * 2 + 1) * nS + smp;
+        const float xC = SCx[iC], yC = SCy[iC], xS = SCx[iS], yS = SCy[iS];
+        sw[k] = SW[smp];
+        icS[k] = intensity(tC, frame, W, H, xS, yS);
+        isS[k] = intensity(tS, frame, W, H, xS, yS);
+        icC[k] = intensity(tC, frame, W, H, xC, yC);
+    }
+    // warp partials, then the halving over them (as block_reduce_ref2)
+    auto reduce = [&](const float (&x)[kPer], const float (&y)[kPer], float &sx, float &sy) {
+        float px[nW], py[nW];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const float a = vc_tree32(x[k]), b = vc_tree32(y[k]);
+            px[2 * k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 0));
+            px[2 * k + 1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 32));
+            py[2 * k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 0));
+            py[2 * k + 1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 32));
+        }
+#pragma unroll
+        for (uint32_t off = nW / 2; off > 0; off >>= 1) {
+#pragma unroll
+            for (uint32_t l = 0; l < off; ++l) {
+                px[l] = px[l] + px[l + off];
+                py[l] = py[l] + py[l + off];
+            }
+        }
+        sx = px[0];
+        sy = py[0];
+    };
+    float sumC, sumS;
+    reduce(icS, isS, sumC, sumS);
+    const float meanC = sumC / (float)nS;
+    const float meanS = sumS / (float)nS;
+    float rC[kPer], rS[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        rC[k] = sw[k] * clip_contrast(icC[k], meanC);
+        rS[k] = sw[k] * clip_contrast(isS[k], meanS);
+    }
+    float outC, outS;
+    reduce(rC, rS, outC, outS);
+    if (lane == 0) {
+        R[(size_t)c * 2 + 0] = outC;
+        R[(size_t)c * 2 + 1] = outS;
+    }
+}
+
+// Launch of the frame response for nFrames frames frameStride floats apart:
+// one wavefront per cell when nS is a multiple of 64, else one workgroup.
+hipError_t launch_frame_response(hipStream_t st, uint32_t nS, uint32_t nFrames,
+                                 const float *frame, uint32_t width, uint32_t height,
+                                 uint32_t nLGN, const float *SCx, const float *SCy,
+                                 const float *SW, const uint32_t *coneType,
+                                 const uint32_t *cellTrial, uint64_t trialFrameStride,
+                                 uint64_t frameStride, float *R) {
+    const dim3 grid((nLGN + 3u) / 4u, nFrames), block(256);
+#define VC_FR(K)                                                                            \
+    hipLaunchKernelGGL((k_lgn_frame_response_w<K>), grid, block, 0, st, frame, (int)width,  \
+                       (int)height, nLGN, SCx, SCy, SW, coneType, cellTrial,               \
+                       trialFrameStride, frameStride, R)
+    switch (nS) {
+    case 64: VC_FR(1); break;
+    case 128: VC_FR(2); break;
+    case 256: VC_FR(4); break;
+    case 512: VC_FR(8); break;
+    case 1024: VC_FR(16); break;
+    default:
+        hipLaunchKernelGGL(k_lgn_frame_response, dim3(nLGN, nFrames), dim3(nS), 0, st, frame,
+                           (int)width, (int)height, nLGN, SCx, SCy, SW, coneType, cellTrial,
+                           trialFrameStride, frameStride, R);
+    }
+#undef VC_FR
+    return hipGetLastError();
+}
+
 // in-register 32-element halving tree (the same additions as vc_tree32)
 __device__ __forceinline__ float tree32_regs(float a[32]) {
 #pragma unroll
@@ -145,10 +248,13 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
         float a0[32], a1[32];
         uint32_t rr = rr0;
         int dd = dd0;
+        // the cell's temporal weights of region r: lane i holds tap i, read
+        // back per tap as a scalar operand
+        const int tw = __float_as_int(ln < nKS ? st.TW[((size_t)c * 2 + r) * nKS + ln] : 0.0f);
 #pragma unroll
         for (int i = 0; i < kMaxTaps; ++i) {
             float v = 0.0f;
-            if (i < (int)nKS) v = sR[wv][r][dd] * st.TW[((size_t)c * 2 + r) * nKS + i];
+            if (i < (int)nKS) v = sR[wv][r][dd] * __int_as_float(__builtin_amdgcn_readlane(tw, i));
             if (i < 32) a0[i] = v; else a1[i - 32] = v;
             rr += dr;
             dd += (int)dq;
@@ -485,10 +591,9 @@ extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t 
         vc_set_error("vc_lgn_frame_response: empty frame or LGN");
         return VC_E_INVALID;
     }
-    hipLaunchKernelGGL(k_lgn_frame_response, dim3(nLGN), dim3(nS), 0, (hipStream_t)stream,
-                       frame, (int)width, (int)height, nLGN, SCx, SCy, SW, coneType, cellTrial,
-                       trialFrameStride, (uint64_t)0, R);
-    VC_HIP_CHECK(hipGetLastError());
+    VC_HIP_CHECK(launch_frame_response((hipStream_t)stream, nS, 1u, frame, width, height, nLGN,
+                                       SCx, SCy, SW, coneType, cellTrial, trialFrameStride,
+                                       (uint64_t)0, R));
     return VC_OK;
 }
 
@@ -511,10 +616,9 @@ extern "C" int vc_lgn_frame_responses(void *stream, const float *frames, uint32_
     }
     if (nFrames == 0) return VC_OK;
     const uint64_t fs = 3ull * width * height;
-    hipLaunchKernelGGL(k_lgn_frame_response, dim3(nLGN, nFrames), dim3(nS), 0,
-                       (hipStream_t)stream, frames, (int)width, (int)height, nLGN, SCx, SCy, SW,
-                       coneType, (const uint32_t *)nullptr, (uint64_t)0, fs, R);
-    VC_HIP_CHECK(hipGetLastError());
+    VC_HIP_CHECK(launch_frame_response((hipStream_t)stream, nS, nFrames, frames, width, height,
+                                       nLGN, SCx, SCy, SW, coneType, nullptr, (uint64_t)0, fs,
+                                       R));
     return VC_OK;
 }
 
