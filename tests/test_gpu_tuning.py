@@ -1,0 +1,36 @@
+"""Orientation tuning, GPU against the oracle: the north star's rate and
+tuning criterion (per-neuron rates and orientation tuning within 2%), on a
+small network driven by gratings at 8 orientations.  With bit-identical
+runs the rates, gOSI and OSI agree exactly."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NORI, NSTEPS = 8, 320
+
+
+def test_tuning_matches_oracle(pkg):
+    import oracle
+    from visual_cortex_amd import analysis as A, network
+    from visual_cortex_amd.sim import GpuSim
+    print(f"test_tuning_matches_oracle: tiny, {NORI} orientations x {NSTEPS} steps", flush=True)
+    rg, ro = [], []
+    for k in range(NORI):
+        net = network.build_network(network.spec_tiny(nt=NSTEPS, bgStrength=(0.4, 0.4),
+                                                      ori_deg=180.0 * k / NORI))
+        g = GpuSim(net)
+        tables = dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
+                      SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
+        o = oracle.OracleSim(net, pkg, tables=tables)
+        g.run(NSTEPS)
+        o.run(NSTEPS)
+        rg.append(A.rates(g.sInfo(), net.spec.dt))
+        ro.append(A.rates(o.sInfo(), net.spec.dt))
+    fg, fo = A.tuning_curves(rg), A.tuning_curves(ro)
+    assert fg.sum() > 0
+    np.testing.assert_array_equal(fg, fo)
+    np.testing.assert_array_equal(A.gosi(fg), A.gosi(fo))
+    np.testing.assert_array_equal(A.osi(A.align(fg)), A.osi(A.align(fo)))
+    # the stimulus orientation matters: rates differ across orientations
+    assert (fg.max(axis=0) > fg.min(axis=0)).any()
