@@ -26,6 +26,10 @@
  *                            step's spikes, src/coredynamics.cu:2609-2922
  *   vc_v1_rec_apply        recal_G_mat_nd_fast on its own (the pending
  *                            recurrent increments of the last step)
+ *   vc_v1_rec_delay        recal_G_mat (axonal delays, noDelay = 0)
+ *                            src/coredynamics.cu:2062-2309
+ *   vc_v1_rec_distances    the V1_delayMat distances of genCon
+ *                            src/preprocess/connect.cu:424-466
  * The data-structure layouts are documented field by field below.
  */
 #ifndef VC_ABI_H
@@ -222,6 +226,36 @@ int vc_v1_dep_current(void *stream, const vc_v1_params *p, const vc_v1_state *s,
 
 /* Initial V1 state: v, w, tBack, g/h as given; clears a spike buffer. */
 int vc_spikes_clear(void *stream, uint32_t nV1, const vc_spikes *sp);
+
+/* ---- axonal delays (noDelay = 0) ----------------------------------------
+ * The reference's recal_G_mat, src/coredynamics.cu:2062-2309 (launched at
+ * src/patch.cu:6518-6527 in place of recal_G_mat_nd_fast): each spike reaches
+ * its post neuron distance/speedOfThought ms after it was fired.
+ *
+ * vc_v1_rec_distances: out_dist[i] = the pre-post distance (mm) of masked-CSR
+ * strength w[i], from the double positions pos[nV1][2]: x = (float)(x_pre -
+ * x_post), d = sqrtf(x*x + y*y) (src/preprocess/connect.cu:424-428, 466: the
+ * reference's V1_delayMat, kept here for the connected pairs only). */
+int vc_v1_rec_distances(void *stream, const vc_v1_params *p, const vc_rec_conn *rec,
+                        const double *pos, float *out_dist);
+
+/* vc_v1_rec_delay: adds to s->gE/hE/gI/hI the recurrent input that arrives
+ * during the step whose outputs are in slot `cur` of the spike-train ring:
+ *   train[depth][nV1]        each step's out_sInfo (1 + tsp/dt for a spike, v
+ *                            otherwise), step t in slot t % depth (the
+ *                            reference's spikeTrain / currentTimeSlot);
+ *   trainMask[depth][nV1/64] each step's spike masks (vc_spikes.mask);
+ *   dist                     vc_v1_rec_distances' output; speed in mm/ms.
+ * depth must exceed ceil(max(dist)/speed/dt) (the reference's trainDepth =
+ * that + 1, src/patch.cu:3813); a synapse whose delay does not fit is skipped
+ * and sets *status (may be NULL) to 1.  Use it with vc_v1_step given an empty
+ * `prev` (no undelayed increments): step t = vc_v1_step writing slot t % depth,
+ * then vc_v1_rec_delay(cur = t % depth), as the reference runs compute_V then
+ * recal_G_mat.  Masked CSR only (rec->wDense is ignored). */
+int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                    const vc_rec_conn *rec, const float *dist, float speed,
+                    const float *train, const uint64_t *trainMask, uint32_t depth,
+                    uint32_t cur, uint32_t *status);
 
 /* ---- reverse mode of the V1 step (no counterpart in the reference; the
  * forward it differentiates is vc_v1_step, i.e. compute_V_collect_spike_

@@ -34,6 +34,9 @@ class Oracle:
         L.orc_v1_step.argtypes = [P(abi_mod.V1Params)] + [vp] * 14 + [u32, vp, vp, vp, vp, vp, vp, vp,
                                                                       u32, vp, vp, i32]
         L.orc_v1_step.restype = i32
+        L.orc_v1_rec_delay.argtypes = [P(abi_mod.V1Params)] + [vp] * 4 + [u32] + [vp] * 4 + \
+            [f32, vp, u32, u32]
+        L.orc_v1_rec_delay.restype = i32
         L.orc_lgn_frame_response.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp,
                                              C.c_uint64, vp]
         L.orc_lgn_steps.argtypes = [P(abi_mod.LgnParams)] + [vp] * 11 + [u32, u32, vp, vp, vp]
@@ -131,6 +134,14 @@ class OracleSim:
             self.lp.cellTrial = self.bt["cellTrial"].ctypes.data
             self.lp.trialLgnOff = self.bt["trialLgnOff"].ctypes.data
             self.lp.trialSeed = self.bt["trialSeed"].ctypes.data
+        # axonal delays (noDelay 0): ring of the last `depth` step outputs
+        self.delay = getattr(sp, "noDelay", 1) == 0
+        if self.delay:
+            self.delayMat = np.ascontiguousarray(net.dense_delaymat())
+            nnz = net.n_rec_synapses_csr()
+            dmax = float(net.rec_distances()[:nnz].max()) if nnz else 0.0
+            self.depth = network.train_depth(dmax, sp.speedOfThought, sp.dt)
+            self.train = np.zeros((self.depth, sp.nV1), np.float32)
         self.lTR = np.zeros(sp.nLGN, np.float32)
         self.lNL = np.zeros(sp.nLGN, np.float32)
         self.o.L.orc_lgn_init(C.byref(self.lp), _fp(self.lTR), _fp(self.lNL))
@@ -222,9 +233,19 @@ class OracleSim:
                                       _fp(self.nLGN_of), _fp(self.sLGN), _fp(self.idLGN), _fp(lgn_row),
                                       self.net.maxNear, _fp(self.nNear), _fp(self.nbr),
                                       _fp(self.conMat), _fp(self.ipre), _fp(self.npre), _fp(self.og),
-                                      _fp(self.oh), it, _fp(out), None, 1)
+                                      _fp(self.oh), it, _fp(out), None, 0 if self.delay else 1)
             if rc != 0:
                 raise RuntimeError(f"orc_v1_step returned {rc}")
+            if self.delay:                  # recal_G_mat after compute_V, same step
+                slot = it % self.depth
+                self.train[slot] = out
+                rc = self.o.L.orc_v1_rec_delay(C.byref(self.p), _fp(st["gE"]), _fp(st["hE"]),
+                                               _fp(st["gI"]), _fp(st["hI"]), self.net.maxNear,
+                                               _fp(self.nNear), _fp(self.nbr), _fp(self.conMat),
+                                               _fp(self.delayMat), sp.speedOfThought,
+                                               _fp(self.train), self.depth, slot)
+                if rc != 0:
+                    raise RuntimeError(f"orc_v1_rec_delay returned {rc}")
             if record:
                 self.sInfo_hist.append(out)
             self.it += 1

@@ -23,6 +23,8 @@
  *                        unspecified, ascending is one of its outcomes)
  *   - recurrent gather   src/coredynamics.cu:2609-2922, dense conMat
  *                        [nblock][maxNear][bs pre][bs post]
+ *   - delayed recurrent  src/coredynamics.cu:2062-2309 (recal_G_mat, axonal
+ *     gather             delays from a ring of past step outputs)
  * Reductions follow the reference's block_reduce tree (32-lane shuffle-down
  * halving per warp, then halving over the warp partials,
  * src/util/cuda_util.cuh:22-127).
@@ -434,6 +436,88 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
     }
     if (own) free(sInfo);
     return VC_OK;
+}
+
+/*
+ * Recurrent input with axonal delays: recal_G_mat, src/coredynamics.cu
+ * :2062-2309 (noDelay = 0; synFail 0, no learning, no gap junctions), in its
+ * own loop order: for every post neuron n, neighbour blocks ib, every
+ * presynaptic neuron i with conMat != 0, ring slots j = 0, 1 of the delay,
+ * receptors ig; the sums are added to gE/hE/gI/hI at the end (:2286-2303).
+ *   conMat, delayMat: dense [nblock][maxNear][bs pre][bs post] (strength,
+ *   distance in mm); train: [depth][nV1] step outputs (sInfo), the step
+ *   whose spikes are newest in slot cur.
+ * Returns VC_E_INVALID if a connected pair's delay does not fit the ring.
+ */
+int orc_v1_rec_delay(const vc_v1_params *p, float *gE, float *hE, float *gI,
+                     float *hI, uint32_t maxNear, const uint32_t *nNear,
+                     const uint32_t *nbr, const float *conMat,
+                     const float *delayMat, float speed, const float *train,
+                     uint32_t depth, uint32_t cur) {
+    const uint32_t nV1 = p->nV1, bs = p->blockSize;
+    const float dt = p->dt;
+    int bad = 0;
+    if (nV1 != p->nblock * bs || depth == 0 || cur >= depth) return VC_E_INVALID;
+#pragma omp parallel for schedule(static) reduction(|:bad)
+    for (uint32_t n = 0; n < nV1; ++n) {
+        uint32_t b = n / bs, l = n % bs;
+        int t = orc_type_of(p, l);
+        float lgE[VC_MAX_NG] = {0}, lhE[VC_MAX_NG] = {0};
+        float lgI[VC_MAX_NG] = {0}, lhI[VC_MAX_NG] = {0};
+        for (uint32_t ib = 0; ib < nNear[b]; ++ib) {
+            uint32_t bid = nbr[b * maxNear + ib];
+            uint64_t local_bid = (uint64_t)b * maxNear + ib;
+            for (uint32_t i = 0; i < bs; ++i) {
+                uint32_t ipre = bid * bs + i;
+                uint64_t mid = (local_bid * bs + i) * bs + l;
+                float strength = conMat[mid];
+                if (strength == 0.0f) continue;
+                int inh = i >= p->nE;
+                float time2post = delayMat[mid] / speed;
+                uint32_t it2post = (uint32_t)ceilf(time2post / dt);
+                time2post = (float)it2post * dt - time2post;
+                if (it2post >= depth) { bad = 1; continue; }
+                int j0 = (int)cur - (int)it2post;
+                if (j0 < 0) j0 += (int)depth;
+                for (uint32_t j = 0; j < 2; ++j) {
+                    uint64_t isp = (uint64_t)nV1 * ((j0 + j) % depth) + ipre;
+                    float dtsp = train[isp];
+                    if (dtsp >= 1.0f) {
+                        uint32_t nsp = (uint32_t)floorf(dtsp);
+                        dtsp = (dtsp - (float)nsp + (float)j) * dt - time2post;
+                        if (dtsp < dt && dtsp >= 0.0f) {
+                            float f = strength;
+                            if (f > 0.0f) {
+                                dtsp = dt - dtsp;
+                                if (!inh) {
+                                    for (uint32_t ig = 0; ig < p->ngE; ++ig)
+                                        orc_single_input(&lgE[ig], &lhE[ig],
+                                                         (float)nsp * f * p->pE[t * p->ngE + ig],
+                                                         dtsp, p->riseE[ig], p->decayE[ig],
+                                                         p->dodE[ig]);
+                                } else {
+                                    for (uint32_t ig = 0; ig < p->ngI; ++ig)
+                                        orc_single_input(&lgI[ig], &lhI[ig],
+                                                         (float)nsp * f * p->pI[t * p->ngI + ig],
+                                                         dtsp, p->riseI[ig], p->decayI[ig],
+                                                         p->dodI[ig]);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+            gE[ig * nV1 + n] += lgE[ig];
+            hE[ig * nV1 + n] += lhE[ig];
+        }
+        for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+            gI[ig * nV1 + n] += lgI[ig];
+            hI[ig * nV1 + n] += lhI[ig];
+        }
+    }
+    return bad ? VC_E_INVALID : VC_OK;
 }
 
 /* ====================================================================== */

@@ -1,0 +1,214 @@
+"""Pins of the oracle's delayed recurrent input (orc_v1_rec_delay, the
+reference's recal_G_mat, src/coredynamics.cu:2062-2309) against what the
+mathematics fixes: every spike fired at (step k + tsp)*dt reaches a post
+neuron d/speed later and, if it lands in the current step, adds nsp*w*p of
+the conductance shape over the rest of the step -- computed here in float64
+from absolute arrival times over the whole ring, not from the reference's
+two-slot arithmetic; an infinitely fast spike gives the undelayed increments
+of recal_G_mat_nd_fast; the ring's length does not change the result; the
+distances are the positions' Euclidean ones."""
+import ctypes as C
+import math
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from test_oracle import flat_state, hand_network, osim
+
+
+def fp(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def delay_call(sim, net, st, train, depth, cur, speed, delayMat=None):
+    sp = net.spec
+    D = net.dense_delaymat() if delayMat is None else delayMat
+    D = np.ascontiguousarray(D, dtype=np.float32)
+    train = np.ascontiguousarray(train, dtype=np.float32)
+    return sim.o.L.orc_v1_rec_delay(C.byref(sim.p), fp(st["gE"]), fp(st["hE"]), fp(st["gI"]),
+                                    fp(st["hI"]), net.maxNear, fp(sim.nNear), fp(sim.nbr),
+                                    fp(sim.conMat), fp(D), speed, fp(train), depth, cur)
+
+
+def synapses(net):
+    """(post, pre, w, d) of every recurrent synapse, global ids, float64."""
+    sp = net.spec
+    bs = sp.blockSize
+    post, pre, w = [], [], []
+    for b in range(sp.nblock):
+        for ib in range(int(net.nNear[b])):
+            pl, ql, ww = net.rec_pairs[b * net.maxNear + ib]
+            pre.append(int(net.nbr[b, ib]) * bs + pl.astype(np.int64))
+            post.append(b * bs + ql.astype(np.int64))
+            w.append(ww.astype(np.float64))
+    post, pre, w = map(np.concatenate, (post, pre, w))
+    d = np.hypot(net.pos[pre, 0] - net.pos[post, 0], net.pos[pre, 1] - net.pos[post, 1])
+    return post, pre, w, d
+
+
+def random_ring(rng, depth, nV1, p_spike=0.08):
+    train = rng.uniform(-70.0, -52.0, (depth, nV1)).astype(np.float32)
+    spk = rng.random((depth, nV1)) < p_spike
+    nsp = np.where(rng.random((depth, nV1)) < 0.1, 2.0, 1.0)
+    train[spk] = (nsp + rng.random((depth, nV1)))[spk].astype(np.float32)
+    return train
+
+
+def event_sum(net, train, depth, cur, speed):
+    """float64 increments of every arrival inside the current step."""
+    sp = net.spec
+    dt = sp.dt
+    post, pre, w, d = synapses(net)
+    ngE, ngI = len(sp.riseTimeE), len(sp.riseTimeI)
+    out = {k: np.zeros((n, sp.nV1)) for k, n in (("gE", ngE), ("hE", ngE), ("gI", ngI),
+                                                  ("hI", ngI))}
+    bs = sp.blockSize
+    inh = (pre % bs) >= sp.nE
+    tpost = ((post % bs) >= sp.nE).astype(int)
+    for k in range(depth):
+        sv = train[k, pre].astype(np.float64)
+        fired = sv >= 1.0
+        nsp = np.floor(sv)
+        age = (cur - k) % depth
+        arrival = (sv - nsp - age) * dt + d / speed
+        ok = fired & (arrival >= 0.0) & (arrival < dt)
+        r = dt - arrival
+        for isI, rise, decay, p, gk, hk in ((False, sp.riseTimeE, sp.decayTimeE, sp.pE, "gE", "hE"),
+                                            (True, sp.riseTimeI, sp.decayTimeI, sp.pI, "gI", "hI")):
+            sel = ok & (inh == isI)
+            for ig in range(len(rise)):
+                ip = np.array([p[0][ig], p[1][ig]])[tpost[sel]]
+                A = nsp[sel] * w[sel] * ip
+                rr = r[sel]
+                dod = rise[ig] / (decay[ig] - rise[ig])
+                np.add.at(out[gk][ig], post[sel], A * dod * (np.exp(-rr / decay[ig]) -
+                                                             np.exp(-rr / rise[ig])))
+                np.add.at(out[hk][ig], post[sel], A * np.exp(-rr / rise[ig]))
+    return out
+
+
+def zero_state(net):
+    from visual_cortex_amd import params
+    st = params.initial_state(net)
+    for k in ("gE", "hE", "gI", "hI"):
+        st[k][:] = 0.0
+    return {k: np.ascontiguousarray(v, dtype=np.float32) for k, v in st.items()}
+
+
+@pytest.mark.parametrize("speed,cur", [(0.25, 3), (0.5, 0), (1.0, 1)])
+def test_delayed_input_equals_arrival_time_sum(pkg, speed, cur):
+    from visual_cortex_amd import network
+    net = network.build_network(network.spec_tiny(noDelay=0, speedOfThought=speed),
+                                with_frames=False)
+    sp = net.spec
+    nnz = net.n_rec_synapses_csr()
+    depth = network.train_depth(float(net.rec_distances()[:nnz].max()), speed, sp.dt)
+    assert depth >= 4                                # delays of several steps
+    cur = cur % depth
+    rng = np.random.default_rng(11 + cur)
+    train = random_ring(rng, depth, sp.nV1)
+    sim = osim(pkg, net)
+    st = zero_state(net)
+    assert delay_call(sim, net, st, train, depth, cur, speed) == 0
+    ref = event_sum(net, train, depth, cur, speed)
+    hits = 0
+    for k in ("gE", "hE", "gI", "hI"):
+        n = ref[k].shape[0]
+        got = st[k][:n].astype(np.float64)
+        np.testing.assert_allclose(got, ref[k], rtol=3e-5, atol=1e-7 * max(1.0, ref[k].max()))
+        hits += int((ref[k] > 0).sum())
+    assert hits > 100
+
+
+def test_ring_length_does_not_matter(pkg):
+    """The same spike history in a longer ring (other slot positions) gives
+    bit-identical increments."""
+    from visual_cortex_amd import network
+    speed = 0.5
+    net = network.build_network(network.spec_tiny(noDelay=0, speedOfThought=speed),
+                                with_frames=False)
+    sp = net.spec
+    nnz = net.n_rec_synapses_csr()
+    depth = network.train_depth(float(net.rec_distances()[:nnz].max()), speed, sp.dt)
+    rng = np.random.default_rng(3)
+    hist = random_ring(rng, depth, sp.nV1)            # hist[a] = output of a steps ago
+    sim = osim(pkg, net)
+    outs = []
+    for D, cur in ((depth, depth - 1), (depth, 2 % depth), (depth + 5, 4)):
+        train = np.zeros((D, sp.nV1), np.float32)
+        for a in range(depth):
+            train[(cur - a) % D] = hist[a]
+        st = zero_state(net)
+        assert delay_call(sim, net, st, train, D, cur, speed) == 0
+        outs.append(st)
+    for st in outs[1:]:
+        for k in ("gE", "hE", "gI", "hI"):
+            assert np.array_equal(st[k].view(np.uint32), outs[0][k].view(np.uint32))
+    # a ring too short for the longest delay is refused
+    st = zero_state(net)
+    assert delay_call(sim, net, st, hist[:1], 1, 0, speed) != 0
+
+
+def test_infinite_speed_gives_undelayed_increments(pkg):
+    """At speed -> infinity every spike arrives in the step it was fired in
+    (it2post = 1, slot j = 1), so the delayed sums equal the increments
+    recal_G_mat_nd_fast gives the same spikes (float32 association only)."""
+    rng = np.random.default_rng(5)
+    bs, nb = 64, 2
+    net0 = hand_network(pkg, nblock=nb, bs=bs, nE=48)
+    M = np.where(rng.random((nb, net0.maxNear, bs, bs)) < 0.3,
+                 rng.lognormal(-1, 0.8, (nb, net0.maxNear, bs, bs)), 0).astype(np.float32)
+    kw = dict(conmat=M, tonicDep=(3.0, 3.0), bgStrength=(0.0, 0.0), deltaT=(0.0, 0.0),
+              a=(0.0, 0.0), b=(0.0, 0.0), vThres=(-50.0, -50.0))
+    net = hand_network(pkg, nblock=nb, bs=bs, nE=48, **kw)
+    sp = net.spec
+    # undelayed: one step with and without the recurrent increments
+    a = osim(pkg, net, flat_state(net, -50.05))
+    a.run(1)
+    s = a.sInfo()[0]
+    loc = np.arange(sp.nV1) % bs
+    assert (s[loc < 48] >= 1).sum() > 20 and (s[loc >= 48] >= 1).sum() > 10
+    b = osim(pkg, net, flat_state(net, -50.05))
+    out = np.zeros(sp.nV1, np.float32)
+    lgn_row = np.zeros(sp.nLGN, np.float32)
+    t = b
+    b.o.L.orc_v1_step(C.byref(t.p), fp(t.st["v"]), fp(t.st["w"]), fp(t.st["tBack"]),
+                      fp(t.st["gFF"]), fp(t.st["hFF"]), fp(t.st["gE"]), fp(t.st["hE"]),
+                      fp(t.st["gI"]), fp(t.st["hI"]), fp(t.tonic), fp(t.nLGN_of), fp(t.sLGN),
+                      fp(t.idLGN), fp(lgn_row), net.maxNear, fp(t.nNear), fp(t.nbr),
+                      fp(t.conMat), fp(t.ipre), fp(t.npre), fp(t.og), fp(t.oh), 0, fp(out),
+                      None, 0)
+    assert np.array_equal(out.view(np.uint32), s.view(np.uint32))
+    # delayed at speed 1e9 mm/ms from the same spikes
+    c = osim(pkg, net)
+    st = zero_state(net)
+    train = np.stack([np.full(sp.nV1, -60.0, np.float32), s])
+    assert delay_call(c, net, st, train, 2, 1, 1e9) == 0
+    for k in ("gE", "hE", "gI", "hI"):
+        n = len(sp.riseTimeE) if k.endswith("E") else len(sp.riseTimeI)
+        und = a.st[k][:n].astype(np.float64) - b.st[k][:n].astype(np.float64)
+        np.testing.assert_allclose(st[k][:n], und, rtol=2e-5,
+                                   atol=2e-6 * max(1e-30, np.abs(und).max()))
+        assert np.abs(und).max() > 0, k
+
+
+def test_distances_are_euclidean(pkg):
+    from visual_cortex_amd import network
+    net = network.build_network(network.spec_tiny(), with_frames=False)
+    bs = net.spec.blockSize
+    D = net.dense_delaymat()
+    post, pre, w, d = synapses(net)
+    b = post // bs
+    ib = np.array([int(np.nonzero(net.nbr[bb] == pp // bs)[0][0]) for bb, pp in zip(b, pre)])
+    got = D[b, ib, pre % bs, post % bs].astype(np.float64)
+    np.testing.assert_allclose(got, d, rtol=1e-6, atol=1e-9)
+    assert np.count_nonzero(D) == net.n_rec_synapses() - np.count_nonzero(d == 0)
+
+
+def test_train_depth_closed_form():
+    from visual_cortex_amd import network
+    assert network.train_depth(0.5, 1.0, 0.125) == 5          # 4 whole steps + 1
+    assert network.train_depth(0.51, 1.0, 0.125) == 6
+    assert network.train_depth(0.0, 1.0, 0.125) == 1
+    assert network.train_depth(1.0, 0.25, 0.1) == 41

@@ -164,12 +164,13 @@ def test_recurrent_sum_brute_force_and_lists(pkg):
     net = hand_network(pkg, nblock=nb, bs=bs, nE=48, conmat=M, tonicDep=(3.0, 3.0),
                        bgStrength=(0.0, 0.0), deltaT=(0.0, 0.0), a=(0.0, 0.0), b=(0.0, 0.0),
                        vThres=(-50.0, -50.0))
-    st = flat_state(net, -50.5)
+    st = flat_state(net, -50.05)
     sim = osim(pkg, net, st)
     sim.run(1)                                   # everybody crosses in step 0
     s = sim.sInfo()[0]
     spk = s >= 1
-    assert spk.sum() > 20
+    loc = np.arange(s.size) % bs
+    assert spk[loc < 48].sum() > 20 and spk[loc >= 48].sum() > 10
     sp = net.spec
     npre = sim.npre
     for b in range(nb):
@@ -178,7 +179,7 @@ def test_recurrent_sum_brute_force_and_lists(pkg):
         assert npre[b] == len(E) and npre[nb + b] == len(I)
         assert np.array_equal(sim.ipre[b * bs: b * bs + len(E) + len(I)], np.concatenate([E, I]))
     # brute force increments: decayed state (do_rec=0) vs applied (oracle state)
-    sim2 = osim(pkg, net, flat_state(net, -50.5))
+    sim2 = osim(pkg, net, flat_state(net, -50.05))
     o = sim2.o
     out = np.zeros(sp.nV1, np.float32)
     lgn_row = np.zeros(sp.nLGN, np.float32)
@@ -213,6 +214,7 @@ def test_recurrent_sum_brute_force_and_lists(pkg):
                 else:
                     r_ = sp.riseTimeI[0]
                     ref_hI[post] += w * sp.pI[tpost][0] * math.exp(-(dt - tsp) / r_)
+    assert ref_gE.max() > 0 and ref_hI.max() > 0
     np.testing.assert_allclose(dgE, ref_gE, rtol=2e-5, atol=2e-7)
     np.testing.assert_allclose(dhI[0], ref_hI, rtol=2e-5, atol=2e-7)
 

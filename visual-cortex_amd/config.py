@@ -189,8 +189,10 @@ def spec_from_config(cfg: Config, base=None):
         v = cfg.floats(key)
         if v is not None and any(x != 0 for x in v):
             raise NotImplementedError(f"{key} != 0 is not built (the configs of this path use 0)")
-    if cfg.has("noDelay") and cfg.int("noDelay") == 0:
-        raise NotImplementedError("noDelay = 0 (axonal delays, recal_G_mat) is not built")
+    if cfg.has("noDelay"):
+        kw["noDelay"] = cfg.int("noDelay")
+    if cfg.has("speedOfThought"):
+        kw["speedOfThought"] = cfg.float("speedOfThought")
     if cfg.bool("InhGap", False):
         raise NotImplementedError("InhGap (gap junctions) is not built")
     if cfg.int("learning", 0) != 0:

@@ -79,6 +79,10 @@ class ModelSpec:
     w_mu: float = -1.0
     w_sigma: float = 0.8
     sigma_um: float = 150.0
+    # axonal delays of the near-neighbour synapses (src/patch.cu:184, 236):
+    # noDelay 0 delays every spike by distance / speedOfThought (mm/ms)
+    noDelay: int = 1
+    speedOfThought: float = 1.0
     # background Poisson drive (extension)
     bgRate: float = 2000.0        # Hz
     bgStrength: tuple = (0.08, 0.12)
@@ -302,6 +306,49 @@ class Network:
 
     def n_rec_synapses(self) -> int:
         return int(sum(p[0].size for p in self.rec_pairs))
+
+    # ---- axonal delays ---------------------------------------------------------
+    def rec_distances(self) -> np.ndarray:
+        """Pre-post distance (mm, float32) of every strength of the masked
+        CSR, in its order: x = (float)(x_pre - x_post) from the double
+        positions, d = sqrtf(x*x + y*y) (src/preprocess/connect.cu:424-428,
+        466; the reference records it in V1_delayMat).  vc_v1_rec_distances
+        computes the same values on the device."""
+        bs = self.spec.blockSize
+        nSeg = bs // 64
+        rm, ro, w = self.masked_csr()
+        bits = ((rm[:, None] >> np.arange(64, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
+        rank = np.cumsum(bits, axis=1) - 1
+        seg_i, lane_i = np.nonzero(bits)
+        idx = ro.astype(np.int64)[seg_i] + rank[seg_i, lane_i]
+        row, sg = seg_i // nSeg, seg_i % nSeg
+        b = row // (self.maxNear * bs)
+        ib = (row // bs) % self.maxNear
+        pre = self.nbr[b, ib].astype(np.int64) * bs + row % bs
+        post = b * bs + sg * 64 + lane_i
+        pos = np.asarray(self.pos, dtype=np.float64)
+        x = (pos[pre, 0] - pos[post, 0]).astype(np.float32)
+        y = (pos[pre, 1] - pos[post, 1]).astype(np.float32)
+        d = np.zeros(max(int(w.size), 1), np.float32)
+        d[idx] = np.sqrt(x * x + y * y)
+        return d
+
+    def dense_delaymat(self) -> np.ndarray:
+        """The distances in the reference's V1_delayMat layout [nblock,
+        maxNear, bs pre, bs post] (zero where there is no synapse)."""
+        bs = self.spec.blockSize
+        M = np.zeros(self.spec.nblock * self.maxNear * bs * bs, np.float32)
+        n = self.n_rec_synapses_csr()
+        if n:
+            M[self.csr_dense_index()] = self.rec_distances()[:n]
+        return M.reshape(self.spec.nblock, self.maxNear, bs, bs)
+
+
+def train_depth(max_distance: float, speed: float, dt: float) -> int:
+    """Steps of spike history the delays need: ceil((maxDistance /
+    speedOfThought) / dt) + 1 in float32 (src/patch.cu:3813)."""
+    f = np.float32
+    return int(np.ceil((f(max_distance) / f(speed)) / f(dt))) + 1
 
 
 # ---------------------------------------------------------------------------

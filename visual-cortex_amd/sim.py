@@ -86,6 +86,12 @@ class GpuSim:
         self.wDense = self._dense_weights() if dense_weights else None
         self.rec = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
                                ptr(self.rowOff), ptr(self.wrec), ptr(self.wDense))
+        # ---- axonal delays (noDelay 0): spike-train ring of the last `depth`
+        # step outputs and spike masks; every step is one vc_v1_step (no
+        # undelayed increments) followed by vc_v1_rec_delay
+        self.delay = getattr(sp, "noDelay", 1) == 0
+        if self.delay:
+            self._setup_delay()
         # ---- spike buffers (double buffered)
         nT = sp.nV1 // 64
         self.masks = [torch.zeros(nT, dtype=torch.int64, device=d) for _ in range(2)]
@@ -93,6 +99,15 @@ class GpuSim:
         self.oh = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
         self.spk = [abi.Spikes(ptr(self.masks[i]), ptr(self.og[i]), ptr(self.oh[i])) for i in range(2)]
         self.spk2 = (abi.Spikes * 2)(*self.spk)
+        if self.delay:
+            zero = torch.zeros(nT, dtype=torch.int64, device=d)
+            self.dog = torch.empty(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d)
+            self.doh = torch.empty(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d)
+            self._keep = zero
+            self.spk_none = abi.Spikes(ptr(zero), ptr(self.dog), ptr(self.doh))
+            self.ring_spk = [abi.Spikes(ptr(self.trainMask[k]), ptr(self.dog), ptr(self.doh))
+                             for k in range(self.depth)]
+            persistent = False
         # persistent multi-step launches (per-block step counters, time-out flag)
         self.persistent = persistent
         nbytes = int(self.lib.vc_v1_persistent_scratch_bytes(C.byref(self.p)))
@@ -121,6 +136,46 @@ class GpuSim:
         if tune:
             from .tune import tune_step
             self.shape = tune_step(self)
+
+    def _setup_delay(self):
+        from .network import train_depth
+        net, sp, d = self.net, self.net.spec, self.dev
+        if net.pos is None:
+            raise ValueError("axonal delays need the V1 positions (Network.pos)")
+        pos = torch.as_tensor(np.ascontiguousarray(net.pos, dtype=np.float64), device=d)
+        nnz = net.n_rec_synapses_csr()
+        self.dist = torch.zeros(max(nnz, 1), dtype=torch.float32, device=d)
+        rec0 = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
+                           ptr(self.rowOff), ptr(self.wrec), None)
+        check(self.lib.vc_v1_rec_distances(None, C.byref(self.p), C.byref(rec0), ptr(pos),
+                                           ptr(self.dist)), "vc_v1_rec_distances")
+        dmax = float(self.dist[:nnz].max().item()) if nnz else 0.0
+        self.speed = float(sp.speedOfThought)
+        self.depth = train_depth(dmax, self.speed, sp.dt)
+        self.train = torch.zeros(self.depth, sp.nV1, dtype=torch.float32, device=d)
+        self.trainMask = torch.zeros(self.depth, sp.nV1 // 64, dtype=torch.int64, device=d)
+        self.slot_off = 0            # ring slot of step it is (it + slot_off) % depth
+
+    def _delay_step(self, buf: int, k_in: int, it: int, sInfo=None, dep=None, gprev=None):
+        """Step `it` with axonal delays: the update writes ring slot `slot`,
+        then the delayed recurrent input arriving during the step is added
+        (the reference's compute_V, then recal_G_mat).  Returns the slot."""
+        sp = self.net.spec
+        slot = (it + self.slot_off) % self.depth
+        h = self._hs(self.s_v1)
+        mptr = ptr(self.ffMask2[buf]) + k_in * sp.nV1 * 4 if self.ev is not None else None
+        check(self.lib.vc_v1_step(h, C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                                  C.byref(self.rec), C.byref(self.spk_none),
+                                  C.byref(self.ring_spk[slot]),
+                                  ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, mptr, it,
+                                  ptr(self.train[slot]), ptr(dep), ptr(gprev)), "vc_v1_step")
+        check(self.lib.vc_v1_rec_delay(h, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
+                                       ptr(self.dist), self.speed, ptr(self.train),
+                                       ptr(self.trainMask), self.depth, slot, ptr(self.status)),
+              "vc_v1_rec_delay")
+        if sInfo is not None:
+            sInfo.copy_(self.train[slot])
+        return slot
 
     def _dense_weights(self) -> torch.Tensor:
         """The masked CSR expanded on the device to the dense layout
@@ -295,6 +350,22 @@ class GpuSim:
             if events is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e0.record(sv)
+            if self.delay:
+                for i in range(n):
+                    row = out[done + i] if out is not None else None
+                    slot = self._delay_step(buf, k_in + i, it + i, sInfo=row)
+                self.sInfo_last = out[done + n - 1] if out is not None else self.train[slot]
+                if events is not None:
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e1.record(sv)
+                    events.append((e0, e1, n))
+                self.it += n
+                done += n
+                if self.it % npf == 0:
+                    e = torch.cuda.Event()
+                    e.record(sv)
+                    self.v1_done[buf] = e
+                continue
             args = (self._hs(sv), C.byref(self.p), C.byref(self.s), C.byref(self.ff),
                     C.byref(self.rec), self.spk2, self.cur,
                     ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, sp.nLGN, mptr, sp.nV1, it, n,
@@ -364,14 +435,17 @@ class GpuSim:
                     fr_w.write(self.lgn_fr2[buf][:n_here].cpu().numpy())
                 if sp_w:
                     sp_w.write(self.lgn_sInfo2[buf][:n_here].cpu().numpy())
-            nxt = 1 - self.cur
-            mptr = ptr(self.ffMask2[buf]) + k_in * nV1 * 4 if self.ev is not None else None
-            check(self.lib.vc_v1_step(self._hs(self.s_v1), C.byref(self.p), C.byref(self.s),
-                                      C.byref(self.ff), C.byref(self.rec),
-                                      C.byref(self.spk[self.cur]), C.byref(self.spk[nxt]),
-                                      ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, mptr, it,
-                                      ptr(sInfo), ptr(dep), ptr(gprev)), "vc_v1_step")
-            self.cur = nxt
+            if self.delay:
+                self._delay_step(buf, k_in, it, sInfo=sInfo, dep=dep, gprev=gprev)
+            else:
+                nxt = 1 - self.cur
+                mptr = ptr(self.ffMask2[buf]) + k_in * nV1 * 4 if self.ev is not None else None
+                check(self.lib.vc_v1_step(self._hs(self.s_v1), C.byref(self.p), C.byref(self.s),
+                                          C.byref(self.ff), C.byref(self.rec),
+                                          C.byref(self.spk[self.cur]), C.byref(self.spk[nxt]),
+                                          ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, mptr,
+                                          it, ptr(sInfo), ptr(dep), ptr(gprev)), "vc_v1_step")
+                self.cur = nxt
             self.it += 1
             self.sInfo_last = sInfo
             if raw:
@@ -441,7 +515,10 @@ class GpuSim:
 
     def finalize(self):
         """Apply the last step's pending recurrent increments (the reference
-        applies them inside the same step, in recal_G)."""
+        applies them inside the same step, in recal_G).  With axonal delays
+        nothing is pending: each step ends with its delayed input."""
+        if self.delay:
+            return
         check(self.lib.vc_v1_rec_apply(None, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
                                        C.byref(self.spk[self.cur])), "vc_v1_rec_apply")
         # the increments are now in the state: clear the pending list
@@ -453,6 +530,8 @@ class GpuSim:
         waiting for a neighbour block (its results are then invalid)."""
         torch.cuda.synchronize()
         if int(self.status.item()) != 0:
+            if self.delay:
+                raise abi.VCError("a synapse's delay does not fit the spike-train ring")
             raise abi.VCError("persistent V1 launch timed out waiting for neighbour blocks")
 
     def sInfo(self) -> np.ndarray:

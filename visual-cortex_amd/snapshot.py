@@ -22,7 +22,10 @@ identity instead of a state: u32 'PHLX' tag, u32 entity, u32 key low, u32 key
 high, u32 stream id (0xffffffff: none), zeros.  The frame-ring fields are
 this simulator's: currentFrame frames have entered, iFrameHead =
 currentFrame mod maxFrame, the parvo phase fields and iStatus are 0.  With no
-axonal delays trainDepth is 1 and the spike train is the last step's output.
+axonal delays trainDepth is 1 and the spike train is the last step's output;
+with delays it is the whole ring and currentTimeSlot the slot of step it0
+(a ring of another length is cut or padded keeping the newest steps, as
+src/patch.cu:5580-5660 does).
 
 A snapshot is taken at a frame-period boundary (the LGN stage runs a frame
 period ahead of V1).  Before writing, the last step's pending recurrent
@@ -185,13 +188,18 @@ def save(sim, path) -> dict:
              oldFrameHead=(K - 1) % maxF, currentFrame=K, iModel=sp.iModel, learning=0,
              nV1=sp.nV1, nLGN=sp.nLGN, ngTypeFF=ngFF, ngTypeE=ngE, ngTypeI=ngI,
              nType=int(sim.p.nType), trainDepth=1)
+    if getattr(sim, "delay", False):       # the whole ring; the next step's slot
+        h.update(trainDepth=sim.depth, currentTimeSlot=(sim.it + sim.slot_off) % sim.depth)
+        train = sim.train.cpu().numpy()
+    else:
+        train = sim.sInfo_last.cpu().numpy()[None]
     lg = lgn.cpu().numpy()
     block = dict(depC=dep.cpu().numpy(), w=st["w"], v=st["v"], gFF=st["gFF"][:ngFF],
                  hFF=st["hFF"][:ngFF], gE=st["gE"][:ngE], gI=st["gI"][:ngI],
                  hE=st["hE"][:ngE], hI=st["hI"][:ngI],
                  gap=np.zeros(sp.nblock * sp.nI, np.float32))
     write_file(path, h, dict(leftTimeRate=lg[0], lastNegLogRand=lg[1]), dict(tBack=st["tBack"]),
-               sim.sInfo_last.cpu().numpy()[None], block,
+               train, block,
                keys=(int(sim.p.seed_lo), int(sim.p.seed_hi)))
     return h
 
@@ -226,7 +234,27 @@ def restore(sim, path, as_init: bool = False) -> dict:
         st[k][:ng].copy_(T(d[k]))
     sim.lTR.copy_(T(d["leftTimeRate"]))
     sim.lNL.copy_(T(d["lastNegLogRand"]))
-    sim.sInfo_step.copy_(T(d["spikeTrain"][-1]))
+    ring = d["spikeTrain"]
+    if getattr(sim, "delay", False):
+        it0 = 0 if as_init else h["it0"]
+        Df, D = int(h["trainDepth"]), sim.depth
+        cts = int(h.get("currentTimeSlot", 0)) % Df   # the file's slot of step it0
+        out = np.zeros((D, sp.nV1), np.float32)
+        if Df == D:
+            sim.slot_off = (cts - it0) % D
+            out[:] = ring
+        else:                                  # another ring length: keep the newest steps
+            sim.slot_off = 0
+            for a in range(1, min(Df, D) + 1):
+                out[(it0 - a) % D] = ring[(cts - a) % Df]
+        bits = np.packbits((out >= 1.0).reshape(D, sp.nV1 // 64, 64), axis=-1,
+                           bitorder="little")
+        sim.train.copy_(T(out))
+        sim.trainMask.copy_(T(bits.reshape(D, -1).view(np.int64)))
+        last = out[(it0 + sim.slot_off - 1) % D]
+    else:
+        last = ring[-1]
+    sim.sInfo_step.copy_(T(last))
     sim.sInfo_last = sim.sInfo_step
     torch.cuda.synchronize()
     return h

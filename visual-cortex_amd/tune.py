@@ -53,6 +53,8 @@ def tune_step(sim, steps: int = 64, reps: int = 2, verbose: bool = False) -> tup
         return wg, t
     if sim.wDense is not None:                   # dense rows: no tile to choose
         return int(sim.p.stepWG), 0
+    if getattr(sim, "delay", False):             # the gather runs in vc_v1_rec_delay
+        return int(sim.p.stepWG), int(sim.p.gatherTile)
     sp = sim.net.spec
     npf = sim.tm["ntPerFrame"]
     steps = min(steps, npf)
