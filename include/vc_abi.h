@@ -28,6 +28,7 @@
  *                            recurrent increments of the last step)
  *   vc_v1_rec_delay        recal_G_mat (axonal delays, noDelay = 0)
  *                            src/coredynamics.cu:2062-2309
+ *   vc_v1_rec_delay_table  its per-synapse delay arithmetic, :2173-2186
  *   vc_v1_rec_distances    the V1_delayMat distances of genCon
  *                            src/preprocess/connect.cu:424-466
  * The data-structure layouts are documented field by field below.
@@ -239,21 +240,30 @@ int vc_spikes_clear(void *stream, uint32_t nV1, const vc_spikes *sp);
 int vc_v1_rec_distances(void *stream, const vc_v1_params *p, const vc_rec_conn *rec,
                         const double *pos, float *out_dist);
 
+/* vc_v1_rec_delay_table: out_delays[i] = rem | (uint64_t)it2post << 32 with
+ * time2post = dist[i]/speed, it2post = ceil(time2post/dt), rem (float bits) =
+ * it2post*dt - time2post: the reference's per-step arithmetic of
+ * src/coredynamics.cu:2173-2186, done once (speed in mm/ms, dt in ms). */
+int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *dist, float speed,
+                          float dt, uint64_t *out_delays);
+
 /* vc_v1_rec_delay: adds to s->gE/hE/gI/hI the recurrent input that arrives
  * during the step whose outputs are in slot `cur` of the spike-train ring:
  *   train[depth][nV1]        each step's out_sInfo (1 + tsp/dt for a spike, v
  *                            otherwise), step t in slot t % depth (the
  *                            reference's spikeTrain / currentTimeSlot);
  *   trainMask[depth][nV1/64] each step's spike masks (vc_spikes.mask);
- *   dist                     vc_v1_rec_distances' output; speed in mm/ms.
- * depth must exceed ceil(max(dist)/speed/dt) (the reference's trainDepth =
- * that + 1, src/patch.cu:3813); a synapse whose delay does not fit is skipped
- * and sets *status (may be NULL) to 1.  Use it with vc_v1_step given an empty
- * `prev` (no undelayed increments): step t = vc_v1_step writing slot t % depth,
- * then vc_v1_rec_delay(cur = t % depth), as the reference runs compute_V then
- * recal_G_mat.  Masked CSR only (rec->wDense is ignored). */
+ *   delays                   vc_v1_rec_delay_table's output for the masked
+ *                            CSR's strengths (in w's order).
+ * depth must exceed the largest it2post, i.e. be at least the reference's
+ * trainDepth = ceil(maxDistance/speed/dt) + 1 (src/patch.cu:3813); a synapse
+ * whose delay does not fit is skipped and sets *status (may be NULL) to 1.
+ * Use it with vc_v1_step given an empty `prev` (no undelayed increments):
+ * step t = vc_v1_step writing slot t % depth, then vc_v1_rec_delay(cur =
+ * t % depth), as the reference runs compute_V then recal_G_mat.  Masked CSR
+ * only (rec->wDense is ignored); maxNear * blockSize/64 <= 512. */
 int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
-                    const vc_rec_conn *rec, const float *dist, float speed,
+                    const vc_rec_conn *rec, const uint64_t *delays,
                     const float *train, const uint64_t *trainMask, uint32_t depth,
                     uint32_t cur, uint32_t *status);
 

@@ -40,9 +40,18 @@ def test_device_distances_match_host(pkg):
     dh = net.rec_distances()[:n]
     assert np.array_equal(dg.view(np.uint32), dh.view(np.uint32))
     assert g.depth == g.train.shape[0] >= 4
+    # the per-synapse delay table: float32 arithmetic of src/coredynamics.cu:2173-2186
+    f = np.float32
+    t2p = dh / f(net.spec.speedOfThought)
+    it2post = np.ceil(t2p / f(net.spec.dt)).astype(np.uint64)
+    rem = it2post.astype(np.float32) * f(net.spec.dt) - t2p
+    tab = g.delays[:n].cpu().numpy().view(np.uint64)
+    assert np.array_equal(tab >> np.uint64(32), it2post)
+    assert np.array_equal((tab & np.uint64(0xffffffff)).astype(np.uint32), rem.view(np.uint32))
+    assert int(it2post.max()) == g.depth - 1
 
 
-@pytest.mark.parametrize("speed", [0.25, 1.0])
+@pytest.mark.parametrize("speed", [0.125, 0.25, 1.0])   # rings longer and shorter than 32
 def test_rec_delay_kernel_bitwise(pkg, speed):
     import oracle
     from visual_cortex_amd import abi
@@ -71,7 +80,7 @@ def test_rec_delay_kernel_bitwise(pkg, speed):
         tm = torch.as_tensor(bits, device="cuda")
         status = torch.zeros(1, dtype=torch.int32, device="cuda")
         check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec),
-                                    ptr(g.dist), speed, ptr(tr), ptr(tm), D, cur, ptr(status)),
+                                    ptr(g.delays), ptr(tr), ptr(tm), D, cur, ptr(status)),
               "vc_v1_rec_delay")
         torch.cuda.synchronize()
         assert int(status.item()) == 0
@@ -89,8 +98,9 @@ def test_rec_delay_kernel_bitwise(pkg, speed):
     tr = torch.ones(1, sp.nV1, device="cuda") * 1.5
     tm = torch.full((1, sp.nV1 // 64), -1, dtype=torch.int64, device="cuda")
     status = torch.zeros(1, dtype=torch.int32, device="cuda")
-    check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec), ptr(g.dist),
-                                speed, ptr(tr), ptr(tm), 1, 0, ptr(status)), "vc_v1_rec_delay")
+    check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec),
+                                ptr(g.delays), ptr(tr), ptr(tm), 1, 0, ptr(status)),
+          "vc_v1_rec_delay")
     torch.cuda.synchronize()
     assert int(status.item()) == 1
     assert float(dev["gE"].abs().max()) == 0.0
@@ -99,6 +109,7 @@ def test_rec_delay_kernel_bitwise(pkg, speed):
 DCASES = {
     # name: (spec kwargs, steps)
     "slow": (dict(speedOfThought=0.25), 256),
+    "very_slow": (dict(speedOfThought=0.125), 256),
     "fast": (dict(speedOfThought=2.0), 256),
     "burst": (dict(speedOfThought=0.5, tonicDep=(2.0, 2.0), sE=1.5), 160),
     "mixed_tile": (dict(speedOfThought=0.5, nE=200, nblock=3), 200),
