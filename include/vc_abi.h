@@ -240,12 +240,14 @@ int vc_spikes_clear(void *stream, uint32_t nV1, const vc_spikes *sp);
 int vc_v1_rec_distances(void *stream, const vc_v1_params *p, const vc_rec_conn *rec,
                         const double *pos, float *out_dist);
 
-/* vc_v1_rec_delay_table: out_delays[i] = rem | (uint64_t)it2post << 32 with
- * time2post = dist[i]/speed, it2post = ceil(time2post/dt), rem (float bits) =
- * it2post*dt - time2post: the reference's per-step arithmetic of
- * src/coredynamics.cu:2173-2186, done once (speed in mm/ms, dt in ms). */
-int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *dist, float speed,
-                          float dt, uint64_t *out_delays);
+/* vc_v1_rec_delay_table: one 16-byte record per masked-CSR strength (in w's
+ * order), out_syn[4i..4i+3] = {w[i] bits, rem bits, it2post, 0} with
+ * time2post = dist[i]/speed, it2post = ceil(time2post/dt), rem = it2post*dt -
+ * time2post: the reference's per-step arithmetic of src/coredynamics.cu
+ * :2173-2186, done once (speed in mm/ms, dt in ms).  out_syn is 16-byte
+ * aligned. */
+int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *w, const float *dist,
+                          float speed, float dt, uint32_t *out_syn);
 
 /* vc_v1_rec_delay: adds to s->gE/hE/gI/hI the recurrent input that arrives
  * during the step whose outputs are in slot `cur` of the spike-train ring:
@@ -253,8 +255,8 @@ int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *dist, float spe
  *                            otherwise), step t in slot t % depth (the
  *                            reference's spikeTrain / currentTimeSlot);
  *   trainMask[depth][nV1/64] each step's spike masks (vc_spikes.mask);
- *   delays                   vc_v1_rec_delay_table's output for the masked
- *                            CSR's strengths (in w's order).
+ *   syn                      vc_v1_rec_delay_table's records (rec->w itself
+ *                            is not read).
  * depth must exceed the largest it2post, i.e. be at least the reference's
  * trainDepth = ceil(maxDistance/speed/dt) + 1 (src/patch.cu:3813); a synapse
  * whose delay does not fit is skipped and sets *status (may be NULL) to 1.
@@ -263,7 +265,7 @@ int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *dist, float spe
  * t % depth), as the reference runs compute_V then recal_G_mat.  Masked CSR
  * only (rec->wDense is ignored); maxNear * blockSize/64 <= 512. */
 int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
-                    const vc_rec_conn *rec, const uint64_t *delays,
+                    const vc_rec_conn *rec, const uint32_t *syn,
                     const float *train, const uint64_t *trainMask, uint32_t depth,
                     uint32_t cur, uint32_t *status);
 

@@ -45,9 +45,11 @@ def test_device_distances_match_host(pkg):
     t2p = dh / f(net.spec.speedOfThought)
     it2post = np.ceil(t2p / f(net.spec.dt)).astype(np.uint64)
     rem = it2post.astype(np.float32) * f(net.spec.dt) - t2p
-    tab = g.delays[:n].cpu().numpy().view(np.uint64)
-    assert np.array_equal(tab >> np.uint64(32), it2post)
-    assert np.array_equal((tab & np.uint64(0xffffffff)).astype(np.uint32), rem.view(np.uint32))
+    tab = g.syn[:n].cpu().numpy().view(np.uint32)
+    w = net.masked_csr()[2][:n]
+    assert np.array_equal(tab[:, 0], w.view(np.uint32))
+    assert np.array_equal(tab[:, 1], rem.view(np.uint32))
+    assert np.array_equal(tab[:, 2].astype(np.uint64), it2post)
     assert int(it2post.max()) == g.depth - 1
 
 
@@ -80,7 +82,7 @@ def test_rec_delay_kernel_bitwise(pkg, speed):
         tm = torch.as_tensor(bits, device="cuda")
         status = torch.zeros(1, dtype=torch.int32, device="cuda")
         check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec),
-                                    ptr(g.delays), ptr(tr), ptr(tm), D, cur, ptr(status)),
+                                    ptr(g.syn), ptr(tr), ptr(tm), D, cur, ptr(status)),
               "vc_v1_rec_delay")
         torch.cuda.synchronize()
         assert int(status.item()) == 0
@@ -99,7 +101,7 @@ def test_rec_delay_kernel_bitwise(pkg, speed):
     tm = torch.full((1, sp.nV1 // 64), -1, dtype=torch.int64, device="cuda")
     status = torch.zeros(1, dtype=torch.int32, device="cuda")
     check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec),
-                                ptr(g.delays), ptr(tr), ptr(tm), 1, 0, ptr(status)),
+                                ptr(g.syn), ptr(tr), ptr(tm), 1, 0, ptr(status)),
           "vc_v1_rec_delay")
     torch.cuda.synchronize()
     assert int(status.item()) == 1

@@ -128,7 +128,7 @@ SIGNATURES = {
                         C.POINTER(Spikes)],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],
     "vc_v1_rec_distances": [vp, C.POINTER(V1Params), C.POINTER(RecConn), vp, vp],
-    "vc_v1_rec_delay_table": [vp, C.c_uint64, vp, C.c_float, C.c_float, vp],
+    "vc_v1_rec_delay_table": [vp, C.c_uint64, vp, vp, C.c_float, C.c_float, vp],
     "vc_v1_rec_delay": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn), vp,
                         vp, vp, u32, u32, vp],
     "vc_v1_step_snapshot": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),

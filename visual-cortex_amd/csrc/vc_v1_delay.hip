@@ -17,7 +17,9 @@
 namespace {
 
 constexpr int kMaxEntD = 512;      // neighbour tiles of one block (maxNear * blockSize/64)
-constexpr int kGD = 32;            // spikers whose strengths are loaded together
+// spikers whose synapse records are loaded together: 32 when the LDS allows
+// (staging the records takes kGD*3 KiB per workgroup), else 16
+constexpr uint32_t kLdsMax = 160u * 1024u;
 constexpr int kMaxDepthLds = 32;   // rings up to this long are staged in LDS
 constexpr int kStv = kMaxDepthLds + 1;  // LDS row stride of one spiker's ring values
 
@@ -52,19 +54,20 @@ __global__ __launch_bounds__(256) void k_rec_distances(uint32_t blockSize, uint3
     dist[idx] = sqrtf(x * x + y * y);
 }
 
-// Per-synapse delay in steps and the remainder the reference derives from
-// the distance every step (src/coredynamics.cu:2173-2186):
-// time2post = d / speed, it2post = ceil(time2post / dt), rem = it2post*dt -
-// time2post; packed as rem's bits | it2post << 32.
-__global__ __launch_bounds__(256) void k_rec_delay_table(uint64_t n, const float *__restrict__ dist,
+// One 16-byte record per synapse: the strength, and the delay in steps and
+// remainder the reference derives from the distance every step
+// (src/coredynamics.cu:2173-2186): time2post = d / speed, it2post =
+// ceil(time2post / dt), rem = it2post*dt - time2post; {w, rem, it2post, 0}.
+__global__ __launch_bounds__(256) void k_rec_delay_table(uint64_t n, const float *__restrict__ w,
+                                                         const float *__restrict__ dist,
                                                          float speed, float dt,
-                                                         uint64_t *__restrict__ out) {
+                                                         uint4 *__restrict__ out) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float time2post = dist[i] / speed;
     const uint32_t it2post = (uint32_t)ceilf(time2post / dt);
     time2post = (float)it2post * dt - time2post;
-    out[i] = ((uint64_t)it2post << 32) | __float_as_uint(time2post);
+    out[i] = make_uint4(__float_as_uint(w[i]), __float_as_uint(time2post), it2post, 0u);
 }
 
 // One post neuron per thread, workgroups (up to 256 threads) inside one V1
@@ -83,7 +86,7 @@ __global__ __launch_bounds__(256) void k_rec_delay_table(uint64_t n, const float
 template <int kGD_>
 __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s,
                                                    vc_rec_conn rec,
-                                                   const uint64_t *__restrict__ delays,
+                                                   const uint4 *__restrict__ syn,
                                                    const float *__restrict__ train,
                                                    const uint64_t *__restrict__ tmask,
                                                    uint32_t depth, uint32_t cur,
@@ -91,8 +94,10 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
     __shared__ uint64_t su[kMaxEntD];
     __shared__ uint32_t soff[kMaxEntD + 1];
     __shared__ uint32_t sbid[kMaxEntD];
-    __shared__ uint16_t items[kMaxEntD * 64];
+    extern __shared__ uint16_t items[];                    // [maxNear*blockSize] (dynamic)
     __shared__ float stv[4][64 * kStv];
+    __shared__ float gw[4][kGD_][64], gr[4][kGD_][64];     // per wavefront: strengths,
+    __shared__ uint32_t gi[4][kGD_][64];                   // remainders, steps of kGD spikers
     const uint32_t nV1 = p.nV1, bs = p.blockSize, nSeg = bs >> 6;
     const uint32_t nTiles = nV1 >> 6;
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
         }
     }
     __syncthreads();
-    const uint32_t total = soff[nEnt];
+    const uint32_t total = __builtin_amdgcn_readfirstlane(soff[nEnt]);
 
     float ipE[VC_MAX_NG], ipI[VC_MAX_NG];
 #pragma unroll
@@ -156,7 +161,6 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
     float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
     bool late = false;
     const bool staged = depth <= (uint32_t)kMaxDepthLds;
-    const uint2 *delays2 = reinterpret_cast<const uint2 *>(delays);   // {rem bits, it2post}
     float *sv = stv[wv];
 
     // ---- 3. the spikers, 64 at a time (every wavefront of the workgroup
@@ -195,10 +199,12 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
         }
         __syncthreads();
         for (uint32_t g0 = 0; g0 < nIt; g0 += kGD_) {
-            uint64_t mine = 0ull;                      // bit g: lane has a synapse from spiker g
+            // (a) strengths and delays of kGD spikers in flight together
+            uint32_t mine = 0u;                        // bit g: lane has a synapse from spiker g
+            static_assert(kGD_ <= 32, "mine is a 32-bit mask");
             float wv_[kGD_], rem[kGD_];
             uint32_t i2p[kGD_];
-#pragma clang loop unroll(full)
+#pragma unroll
             for (int g = 0; g < kGD_; ++g) {
                 wv_[g] = rem[g] = 0.0f;
                 i2p[g] = 0u;
@@ -207,31 +213,48 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
                     const uint64_t m = vc_readlane_u64(hm, kk);
                     if ((m >> lane) & 1ull) {
                         const uint32_t idx = vc_readlane_u32(ho, kk) + vc_mbcnt(m);
-                        wv_[g] = rec.w[idx];
-                        const uint2 dl = delays2[idx];
-                        rem[g] = __uint_as_float(dl.x);
-                        i2p[g] = dl.y;
-                        mine |= 1ull << g;
+                        const uint4 r = syn[idx];
+                        wv_[g] = __uint_as_float(r.x);
+                        rem[g] = __uint_as_float(r.y);
+                        i2p[g] = r.z;
+                        mine |= 1u << g;
                     }
                 }
             }
-#pragma clang loop unroll(full)
+            // (b) the synapses one of whose two slots holds a spike, staged in LDS
+            uint32_t act = 0u;
+#pragma unroll
             for (int g = 0; g < kGD_; ++g) {
-                if (!((mine >> g) & 1ull)) continue;
-                const uint32_t kk = g0 + g;
-                // src/coredynamics.cu:2173-2201 (it2post and the remainder precomputed)
+                if (!((mine >> g) & 1u)) continue;
                 const uint32_t it2post = i2p[g];
-                const float time2post = rem[g];
                 if (it2post >= depth) {               // ring too short for this delay
                     late = true;
                     continue;
                 }
                 if (staged) {   // slot j = 0 is it2post steps old, j = 1 one step newer
-                    const uint32_t a = vc_readlane_u32(am, kk);
+                    const uint32_t a = vc_readlane_u32(am, g0 + g);
                     const uint32_t need = it2post ? (a >> (it2post - 1u)) & 3u
                                                   : (a & 1u) | ((a >> (depth - 1u)) & 1u);
-                    if (!need) continue;                  // neither slot holds a spike
+                    if (!need) continue;
                 }
+                act |= 1u << g;
+                gw[wv][g][lane] = wv_[g];
+                gr[wv][g][lane] = rem[g];
+                gi[wv][g][lane] = it2post;
+            }
+            uint32_t wact = 0u;                        // spikers some lane has to add
+#pragma unroll
+            for (int g = 0; g < kGD_; ++g)
+                if (__ballot((act >> g) & 1u)) wact |= 1u << g;
+            // (c) one code path for every spiker, in order (src/coredynamics.cu:2173-2235)
+            while (wact) {
+                const uint32_t g = (uint32_t)__builtin_ctz(wact);
+                wact &= wact - 1u;
+                if (!((act >> g) & 1u)) continue;
+                const uint32_t kk = g0 + g;
+                const float f = gw[wv][g][lane];
+                const float time2post = gr[wv][g][lane];
+                const uint32_t it2post = gi[wv][g][lane];
                 const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
                 const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
                 float tv0, tv1;
@@ -243,30 +266,39 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
                     tv0 = train[(size_t)j0 * nV1 + gp];
                     tv1 = train[(size_t)j1 * nV1 + gp];
                 }
+                // the presynaptic type picks the shape by value (one code path)
                 const bool inh = vc_readlane_u32(lpre, kk) >= p.nE;
+                const uint32_t ng = inh ? p.ngI : p.ngE;
+                float ip[VC_MAX_NG], rise[VC_MAX_NG], decay[VC_MAX_NG], dod[VC_MAX_NG];
+#pragma unroll
+                for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+                    ip[ig] = inh ? ipI[ig] : ipE[ig];
+                    rise[ig] = inh ? p.riseI[ig] : p.riseE[ig];
+                    decay[ig] = inh ? p.decayI[ig] : p.decayE[ig];
+                    dod[ig] = inh ? p.dodI[ig] : p.dodE[ig];
+                }
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     float dtsp = j ? tv1 : tv0;
-                    if (dtsp >= 1.0f) {                // src/coredynamics.cu:2213-2235
+                    if (dtsp >= 1.0f) {
                         const float nsp = floorf(dtsp);
                         dtsp = (dtsp - nsp + (float)j) * dt - time2post;
-                        if (dtsp < dt && dtsp >= 0.0f) {
-                            const float f = wv_[g];
-                            if (f > 0.0f) {
-                                dtsp = dt - dtsp;
-                                const float a = nsp * f;
-                                if (!inh) {
+                        if (dtsp < dt && dtsp >= 0.0f && f > 0.0f) {
+                            const float ddt = dt - dtsp;
+                            const float a = nsp * f;
 #pragma unroll
-                                    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
-                                        if (ig < p.ngE)
-                                            single_input(lgE[ig], lhE[ig], a * ipE[ig], dtsp,
-                                                         p.riseE[ig], p.decayE[ig], p.dodE[ig]);
-                                } else {
-#pragma unroll
-                                    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
-                                        if (ig < p.ngI)
-                                            single_input(lgI[ig], lhI[ig], a * ipI[ig], dtsp,
-                                                         p.riseI[ig], p.decayI[ig], p.dodI[ig]);
+                            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+                                if (ig < ng) {
+                                    // single_input's two increments; added to the E or the
+                                    // I sums by value (+0 leaves the other sum unchanged)
+                                    const float fi = a * ip[ig];
+                                    const float etr = vc_expf(-ddt / rise[ig]);
+                                    const float tg = fi * dod[ig] * (vc_expf(-ddt / decay[ig]) - etr);
+                                    const float th = fi * etr;
+                                    lgE[ig] = lgE[ig] + (inh ? 0.0f : tg);
+                                    lhE[ig] = lhE[ig] + (inh ? 0.0f : th);
+                                    lgI[ig] = lgI[ig] + (inh ? tg : 0.0f);
+                                    lhI[ig] = lhI[ig] + (inh ? th : 0.0f);
                                 }
                             }
                         }
@@ -288,6 +320,13 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
             s.hI[ig * nV1 + n] = s.hI[ig * nV1 + n] + lhI[ig];
         }
     }
+}
+
+// Static LDS of k_rec_delay<kGD_> (its declarations above), bytes.
+template <int kGD_>
+constexpr size_t lds_static() {
+    return sizeof(uint64_t) * kMaxEntD + sizeof(uint32_t) * (2 * kMaxEntD + 1) +
+           sizeof(float) * 4 * 64 * kStv + 3 * sizeof(float) * 4 * kGD_ * 64;
 }
 
 bool rec_ok(const vc_v1_params *p, const vc_rec_conn *rec, const char *who) {
@@ -333,11 +372,15 @@ extern "C" int vc_v1_rec_distances(void *stream, const vc_v1_params *p, const vc
     return VC_OK;
 }
 
-extern "C" int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *dist, float speed,
-                                     float dt, uint64_t *out_delays) {
+extern "C" int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *w, const float *dist,
+                                     float speed, float dt, uint32_t *out_syn) {
     if (n == 0) return VC_OK;
-    if (!dist || !out_delays || !(speed > 0.0f) || !(dt > 0.0f)) {
-        vc_set_error("vc_v1_rec_delay_table: need dist, out_delays, speed > 0 and dt > 0");
+    if (!w || !dist || !out_syn || !(speed > 0.0f) || !(dt > 0.0f)) {
+        vc_set_error("vc_v1_rec_delay_table: need w, dist, out_syn, speed > 0 and dt > 0");
+        return VC_E_INVALID;
+    }
+    if (((uintptr_t)out_syn & 15u) != 0) {
+        vc_set_error("vc_v1_rec_delay_table: out_syn must be 16-byte aligned");
         return VC_E_INVALID;
     }
     const uint64_t grid = (n + 255) / 256;
@@ -347,18 +390,22 @@ extern "C" int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *dist
         return VC_E_INVALID;
     }
     hipLaunchKernelGGL(k_rec_delay_table, dim3((uint32_t)grid), dim3(256), 0,
-                       (hipStream_t)stream, n, dist, speed, dt, out_delays);
+                       (hipStream_t)stream, n, w, dist, speed, dt, (uint4 *)out_syn);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
 
 extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
-                               const vc_rec_conn *rec, const uint64_t *delays,
+                               const vc_rec_conn *rec, const uint32_t *syn,
                                const float *train, const uint64_t *trainMask, uint32_t depth,
                                uint32_t cur, uint32_t *status) {
     if (!rec_ok(p, rec, "vc_v1_rec_delay")) return VC_E_INVALID;
-    if (!s || !s->gE || !s->hE || !s->gI || !s->hI || !delays || !train || !trainMask) {
+    if (!s || !s->gE || !s->hE || !s->gI || !s->hI || !syn || !train || !trainMask) {
         vc_set_error("vc_v1_rec_delay: null argument");
+        return VC_E_INVALID;
+    }
+    if (((uintptr_t)syn & 15u) != 0) {
+        vc_set_error("vc_v1_rec_delay: syn must be 16-byte aligned");
         return VC_E_INVALID;
     }
     if (depth == 0 || cur >= depth) {
@@ -370,8 +417,14 @@ extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_
         return VC_E_INVALID;
     }
     const uint32_t wg = wg_size(p->blockSize);
-    hipLaunchKernelGGL(k_rec_delay<kGD>, dim3(p->nV1 / wg), dim3(wg), 0, (hipStream_t)stream, *p,
-                       *s, *rec, delays, train, trainMask, depth, cur, status);
+    const size_t dyn = (size_t)rec->maxNear * p->blockSize * sizeof(uint16_t);
+    const uint4 *sy = (const uint4 *)syn;
+    if (dyn + lds_static<32>() + 1024u <= kLdsMax)     // margin for alignment padding
+        hipLaunchKernelGGL(k_rec_delay<32>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
+                           *p, *s, *rec, sy, train, trainMask, depth, cur, status);
+    else
+        hipLaunchKernelGGL(k_rec_delay<16>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
+                           *p, *s, *rec, sy, train, trainMask, depth, cur, status);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -151,9 +151,11 @@ class GpuSim:
                                            ptr(self.dist)), "vc_v1_rec_distances")
         dmax = float(self.dist[:nnz].max().item()) if nnz else 0.0
         self.speed = float(sp.speedOfThought)
-        self.delays = torch.zeros(max(nnz, 1), dtype=torch.int64, device=d)
-        check(self.lib.vc_v1_rec_delay_table(None, nnz, ptr(self.dist), self.speed, sp.dt,
-                                             ptr(self.delays)), "vc_v1_rec_delay_table")
+        # {w, rem, it2post, 0} per synapse (16-byte records)
+        self.syn = torch.zeros(max(nnz, 1), 4, dtype=torch.int32, device=d)
+        check(self.lib.vc_v1_rec_delay_table(None, nnz, ptr(self.wrec), ptr(self.dist),
+                                             self.speed, sp.dt, ptr(self.syn)),
+              "vc_v1_rec_delay_table")
         self.depth = train_depth(dmax, self.speed, sp.dt)
         self.train = torch.zeros(self.depth, sp.nV1, dtype=torch.float32, device=d)
         self.trainMask = torch.zeros(self.depth, sp.nV1 // 64, dtype=torch.int64, device=d)
@@ -173,7 +175,7 @@ class GpuSim:
                                   ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, mptr, it,
                                   ptr(self.train[slot]), ptr(dep), ptr(gprev)), "vc_v1_step")
         check(self.lib.vc_v1_rec_delay(h, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
-                                       ptr(self.delays), ptr(self.train), ptr(self.trainMask),
+                                       ptr(self.syn), ptr(self.train), ptr(self.trainMask),
                                        self.depth, slot, ptr(self.status)),
               "vc_v1_rec_delay")
         if sInfo is not None:
