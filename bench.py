@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--reps", type=int, default=5, help="c4: timed forward/backward repetitions")
     ap.add_argument("--trials", type=int, default=32, help="c3: number of trials")
+    ap.add_argument("--delay", type=float, default=0.0,
+                    help="c1/c2 with axonal delays (noDelay 0) at this speedOfThought, mm/ms")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (x4) for HBM counter calibration")
     args = ap.parse_args()
@@ -100,6 +102,8 @@ def main():
     if args.config == "c5":
         return bench_v1_adjoint(args)
     spec = network.spec_c2() if args.config == "c2" else network.spec_c1()
+    if args.delay > 0:
+        spec = network.replace(spec, noDelay=0, speedOfThought=args.delay)
     K, W = args.steps, args.warmup
     spec = network.replace(spec, nt=K + W) if hasattr(network, "replace") else spec
     t0 = time.time()
@@ -155,9 +159,12 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0:
-        cpu = cpu_baseline(pkg, net, sim, args.cpu_steps)
+        # the oracle's delayed gather scans the dense matrix: a short sample
+        cpu = cpu_baseline(pkg, net, sim, min(args.cpu_steps, 48) if sim.delay else args.cpu_steps)
 
     kernel = "k_v1_multi" if sim.persistent else "k_v1_step"
+    if sim.delay:
+        kernel = "k_v1_step+k_rec_delay"
     line = {
         "metric": "V1 neuron-steps/s at 64 blocks dt=0.125ms",
         "value": value,
@@ -174,12 +181,15 @@ def main():
         "config": {"workload": f"plain forward, {'full patch' if args.config == 'c2' else 'small'}: "
                                f"{spec.nblock} blocks x {spec.blockSize} neurons, {spec.nLGN} LGN, "
                                f"m={spec.m}, K={spec.KE}+{spec.KI}, {spec.width}x{spec.height} px, "
-                               f"dt={spec.dt} ms",
+                               f"dt={spec.dt} ms" + (f", axonal delays at {spec.speedOfThought} "
+                                                      f"mm/ms (ring of {sim.depth} steps)"
+                                                      if sim.delay else ""),
                    "nV1": spec.nV1, "nLGN": spec.nLGN, "dt_ms": spec.dt,
                    "spikes_per_neuron_step": rho},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": pmc_traffic(kernel, "full_patch") if args.config == "c2" else None,
+                     "traffic": (pmc_traffic(kernel, "full_patch")
+                                 if args.config == "c2" and not sim.delay else None),
                      "kernel": kernel,
                      "steps_per_launch": steps_per_launch,
                      "bytes_per_launch": bytes_per_launch, "launch_us": launch_s * 1e6,
