@@ -32,6 +32,9 @@ def test_cfg_semantics(pkg):
         config.spec_from_config(config.Config("InhGap = true"))
     with pytest.raises(NotImplementedError):
         config.spec_from_config(config.Config("noisyDep = 0.1, 0"))
+    sp = config.spec_from_config(config.Config("noDelay = 0\nspeedOfThought = 0.5"))
+    assert sp.noDelay == 0 and sp.speedOfThought == 0.5
+    assert config.spec_from_config(config.Config("")).noDelay == 1      # the reference's default
 
 
 def test_stimulus_and_lists_round_trip(pkg, tmp_path):
