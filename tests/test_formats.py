@@ -100,3 +100,24 @@ def test_network_files_round_trip(pkg, tmp_path):
     assert np.array_equal(back.frames, net.frames)
     assert back.spec.nLGN == net.spec.nLGN and back.spec.nE == net.spec.nE
     assert back.spec.vThres == net.spec.vThres and back.spec.pE == net.spec.pE
+
+
+def test_delaymat_round_trip(pkg, tmp_path):
+    """With noDelay = 0 the V1_delayMat file is written with every pair's
+    distance and read back for the connected pairs: the same float32 values
+    the positions give, in the masked CSR's order."""
+    from visual_cortex_amd import config, loader, network
+    net = network.build_network(network.spec_tiny(nt=64, noDelay=0, speedOfThought=0.5),
+                                with_frames=True)
+    cfg_path = loader.save_network(net, tmp_path / "in")
+    back = loader.load_network(config.Config.from_file(cfg_path))
+    assert back.spec.noDelay == 0 and back.spec.speedOfThought == 0.5
+    assert back.rec_dist is not None
+    n = net.n_rec_synapses_csr()
+    a, b = net.rec_distances()[:n], back.rec_distances()[:n]
+    assert a.size == n and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    sp = net.spec
+    raw = np.fromfile(tmp_path / "in" / "V1_delayMat-vc.bin", "<f4", offset=4)
+    assert raw.size == sp.nblock * net.maxNear * sp.blockSize ** 2
+    D = raw.reshape(sp.nblock, net.maxNear, sp.blockSize, sp.blockSize)
+    assert np.all(D[:, :, np.arange(sp.blockSize), np.arange(sp.blockSize)][:, 0] == 0)  # self

@@ -32,3 +32,27 @@ def test_cli_from_files_matches_direct_run(pkg, tmp_path, monkeypatch):
     assert np.array_equal(lsp.view(np.uint32), g.lgn_sinfo()[:192].view(np.uint32))
     spikes = formats.spikes_from_sinfo(raw["sInfo"], net.spec.dt)
     assert sum(len(x) for x in spikes) == int((s_direct >= 1).sum()) > 0
+
+
+def test_cli_delayed_run_from_files(pkg, tmp_path, monkeypatch):
+    """noDelay = 0 from the files (distances from V1_delayMat) gives the same
+    spikes as the direct delayed run (distances computed on the device)."""
+    from visual_cortex_amd import cli, formats, loader, network
+    from visual_cortex_amd.sim import GpuSim
+    print("test_cli_delayed_run_from_files: tiny, 192 steps, speed 0.5", flush=True)
+    monkeypatch.chdir(tmp_path)
+    net = network.build_network(network.spec_tiny(nt=192, bgStrength=(0.4, 0.4), noDelay=0,
+                                                  speedOfThought=0.5))
+    cfg = loader.save_network(net, tmp_path / "run")
+    with open(cfg, "a") as f:
+        f.write("rawData = true\nhWrite = true\n")
+    assert cli.main(["-c", str(cfg)]) == 0
+    raw = formats.read_raw_data(tmp_path / "run" / "rawData-vc.bin")
+    g = GpuSim(net)
+    assert g.delay
+    g.run(192)
+    s_direct = g.sInfo()
+    assert int((s_direct >= 1).sum()) > 100
+    assert np.array_equal(raw["sInfo"].view(np.uint32), s_direct.view(np.uint32))
+    st = g.state_numpy()
+    assert np.array_equal(raw["gE"][-1], st["gE"]) and np.array_equal(raw["hI"][-1], st["hI"])

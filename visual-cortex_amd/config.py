@@ -24,6 +24,7 @@ FILES = {
     "fLGN_vpos": ("LGN_vpos", "resourceFolder", "res_suffix"),
     "fConnectome_cfg": ("connectome_cfg", "inputFolder", "conV1_suffix"),
     "fV1_conMat": ("V1_conMat", "inputFolder", "conV1_suffix"),
+    "fV1_delayMat": ("V1_delayMat", "inputFolder", "conV1_suffix"),   # src/patch.cu:352, 467-470
     "fNeighborBlock": ("neighborBlock", "inputFolder", "conV1_suffix"),
     "fLGN_V1_ID": ("LGN_V1_idList", "inputFolder", "conLGN_suffix"),
     "fLGN_V1_s": ("LGN_V1_sList", "inputFolder", "conLGN_suffix"),

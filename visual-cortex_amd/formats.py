@@ -9,6 +9,9 @@ the reference's formats):
   connectome_cfg<conV1>   src/patch.cu:634-645
   V1_conMat<conV1>.bin    src/patch.cu:3441-3461, 3649-3687 (dense, streamed
                           block by block into the masked-CSR rows)
+  V1_delayMat<conV1>.bin  src/patch.cu:3450-3461, 3683-3684 (the same layout:
+                          pre-post distance in mm of every pair, written by
+                          src/preprocess/genCon.cu:511-516, 911)
   neighborBlock<conV1>    src/patch.cu:3481-3512
   LGN_V1_idList / sList   src/util/util.h:161-185, src/patch.h:30-77
   fLGN (old LGN setup)    src/patch.cu:2472-2530
@@ -159,6 +162,58 @@ def read_conmat_pairs(path, nblock, blockSize):
                 pairs.append((pre.astype(np.uint32), post.astype(np.uint32),
                               M[ib, pre, post].astype(np.float32)))
     return int(maxNear), pairs
+
+
+def pair_distances(net, b, ib, pre, post):
+    """Distances (mm, float32) of pre-post pairs of block b's neighbour slot
+    ib: x = (float)(x_pre - x_post) from the double positions, d = sqrtf(x*x +
+    y*y) (src/preprocess/connect.cu:424-428)."""
+    bs = net.spec.blockSize
+    pos = np.asarray(net.pos, dtype=np.float64)
+    gpre = int(net.nbr[b, ib]) * bs + np.asarray(pre, np.int64)
+    gpost = b * bs + np.asarray(post, np.int64)
+    x = (pos[gpre, 0] - pos[gpost, 0]).astype(np.float32)
+    y = (pos[gpre, 1] - pos[gpost, 1]).astype(np.float32)
+    return np.sqrt(x * x + y * y)
+
+
+def write_delaymat(path, net):
+    """Dense V1_delayMat [nblock][nearNeighborBlock][bs][bs] of a Network:
+    the distance of every pair of a block and its near neighbours (the
+    reference records it whether or not the pair is connected), zeros for
+    neighbour slots past nNear."""
+    sp = net.spec
+    bs = sp.blockSize
+    grid = np.arange(bs)
+    pre, post = np.repeat(grid, bs), np.tile(grid, bs)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<I", net.maxNear))
+        for b in range(sp.nblock):
+            M = np.zeros((net.maxNear, bs * bs), "<f4")
+            for ib in range(int(net.nNear[b])):
+                M[ib] = pair_distances(net, b, ib, pre, post)
+            f.write(M.tobytes())
+
+
+def read_delaymat_pairs(path, nblock, blockSize, pairs):
+    """Stream V1_delayMat block by block and keep the distances of the
+    connected pairs, aligned with read_conmat_pairs' (pre, post, w) lists."""
+    bs = blockSize
+    out = []
+    with open(path, "rb") as f:
+        maxNear = _u32(f, 1)
+        if len(pairs) != nblock * maxNear:
+            raise ValueError(f"{path}: {maxNear} neighbour slots, V1_conMat has "
+                             f"{len(pairs) // max(nblock, 1)}")
+        for b in range(nblock):
+            M = np.fromfile(f, "<f4", maxNear * bs * bs)
+            if M.size != maxNear * bs * bs:
+                raise ValueError(f"{path}: truncated at block {b}")
+            M = M.reshape(maxNear, bs, bs)
+            for ib in range(maxNear):
+                pre, post, _ = pairs[b * maxNear + ib]
+                out.append(M[ib, pre, post].astype(np.float32))
+    return out
 
 
 # ------------------------------------------------------------------ LGN -> V1

@@ -3,7 +3,8 @@ Network in those files (with a .cfg that names them), so that `patch_fast`
 inputs drive this simulator and synthetic workloads can be exported.
 
 Reads (see formats.py for the layouts and reference lines): V1_allpos,
-connectome_cfg, neighborBlock, V1_conMat, LGN_V1_idList / LGN_V1_sList,
+connectome_cfg, neighborBlock, V1_conMat, V1_delayMat (noDelay = 0),
+LGN_V1_idList / LGN_V1_sList,
 fLGN and stimulus.bin.  The neuron and synapse parameters come from the cfg
 keys (config.spec_from_config).
 """
@@ -54,9 +55,12 @@ def load_network(cfg: cfgmod.Config) -> Network:
                    m=int(nof.max()) if nof.size else 0)
     tonic = np.where((np.arange(nblock * bs) % bs) < spec.nE, spec.tonicDep[0],
                      spec.tonicDep[1]).astype(np.float32)
+    rec_dist = None
+    if spec.noDelay == 0:                    # the reference's own distances
+        rec_dist = F.read_delaymat_pairs(cfg.path("fV1_delayMat"), nblock, bs, pairs)
     return Network(spec=spec, pos=pos, tonicDep=tonic, maxNear=maxNear, nNear=nNear, nbr=nbr,
                    rec_pairs=pairs, nLGN_of=nof, sLGN=sLGN, idLGN=idLGN, lgn=lgn,
-                   frames=frames, init_lms=tuple(st["init_lms"]))
+                   frames=frames, init_lms=tuple(st["init_lms"]), rec_dist=rec_dist)
 
 
 def save_network(net: Network, folder, suffix: str = "vc") -> Path:
@@ -72,6 +76,7 @@ def save_network(net: Network, folder, suffix: str = "vc") -> Path:
     lists = [list(net.nbr[b, :net.nNear[b]]) for b in range(sp.nblock)]
     F.write_neighbor_block(folder / f"neighborBlock{suf}", net.nNear, lists)
     F.write_conmat(folder / f"V1_conMat{suf}", net)
+    F.write_delaymat(folder / f"V1_delayMat{suf}", net)
     # LGN_V1 lists hold the un-scaled strengths (read_LGN multiplies by sRatioLGN)
     isE = (np.arange(sp.nV1) % sp.blockSize) < sp.nE
     sR = np.where(isE, sp.sRatioLGN[0], sp.sRatioLGN[1]).astype(np.float32)
@@ -86,7 +91,8 @@ def save_network(net: Network, folder, suffix: str = "vc") -> Path:
     lines = [
         f"inputFolder = {folder}/", f"resourceFolder = {folder}/", f"outputFolder = {folder}/",
         f"res_suffix = {suffix}", f"conV1_suffix = {suffix}", f"conLGN_suffix = {suffix}",
-        f"output_suffix = {suffix}", "flat_retina = true", "noDelay = 1", "learning = 0",
+        f"output_suffix = {suffix}", "flat_retina = true", f"noDelay = {sp.noDelay}",
+        f"speedOfThought = {sp.speedOfThought}", "learning = 0",
         f"seed = {sp.seed}", f"dt = {sp.dt}", f"nt = {sp.nt}", f"frameRate = {sp.frameRate}",
         f"iModel = {sp.iModel}", f"tau = {sp.tau}", f"nKernelSample = {sp.nKernelSample}",
         f"nSpatialSample1D = {sp.nSpatialSample1D}", f"nsig = {sp.nsig}",

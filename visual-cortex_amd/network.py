@@ -193,6 +193,9 @@ class Network:
     # produced by the device generator; used instead of rec_pairs when set
     csr: tuple | None = None
     batch: object | None = None         # batch.BatchInfo for batched trials
+    # distances of the rec_pairs (per (b, ib), aligned), when read from a
+    # V1_delayMat file; otherwise they are computed from pos
+    rec_dist: list | None = None
     trial_specs: list | None = None
 
     # ---- recurrent formats -------------------------------------------------
@@ -313,7 +316,11 @@ class Network:
         CSR, in its order: x = (float)(x_pre - x_post) from the double
         positions, d = sqrtf(x*x + y*y) (src/preprocess/connect.cu:424-428,
         466; the reference records it in V1_delayMat).  vc_v1_rec_distances
-        computes the same values on the device."""
+        computes the same values on the device.  Distances read from a
+        V1_delayMat file (rec_dist) are returned as they are."""
+        if self.rec_dist is not None and self.csr is None:
+            d = np.concatenate([np.asarray(x, np.float32) for x in self.rec_dist])
+            return d if d.size else np.zeros(1, np.float32)
         bs = self.spec.blockSize
         nSeg = bs // 64
         rm, ro, w = self.masked_csr()

@@ -142,13 +142,17 @@ class GpuSim:
         net, sp, d = self.net, self.net.spec, self.dev
         if net.pos is None:
             raise ValueError("axonal delays need the V1 positions (Network.pos)")
-        pos = torch.as_tensor(np.ascontiguousarray(net.pos, dtype=np.float64), device=d)
         nnz = net.n_rec_synapses_csr()
-        self.dist = torch.zeros(max(nnz, 1), dtype=torch.float32, device=d)
-        rec0 = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
-                           ptr(self.rowOff), ptr(self.wrec), None)
-        check(self.lib.vc_v1_rec_distances(None, C.byref(self.p), C.byref(rec0), ptr(pos),
-                                           ptr(self.dist)), "vc_v1_rec_distances")
+        if getattr(net, "rec_dist", None) is not None and net.csr is None:
+            # distances read from the reference's V1_delayMat file
+            self.dist = torch.as_tensor(net.rec_distances(), device=d)
+        else:
+            pos = torch.as_tensor(np.ascontiguousarray(net.pos, dtype=np.float64), device=d)
+            self.dist = torch.zeros(max(nnz, 1), dtype=torch.float32, device=d)
+            rec0 = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
+                               ptr(self.rowOff), ptr(self.wrec), None)
+            check(self.lib.vc_v1_rec_distances(None, C.byref(self.p), C.byref(rec0), ptr(pos),
+                                               ptr(self.dist)), "vc_v1_rec_distances")
         dmax = float(self.dist[:nnz].max().item()) if nnz else 0.0
         self.speed = float(sp.speedOfThought)
         # {w, rem, it2post, 0} per synapse (16-byte records)
