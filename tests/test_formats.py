@@ -121,3 +121,24 @@ def test_delaymat_round_trip(pkg, tmp_path):
     assert raw.size == sp.nblock * net.maxNear * sp.blockSize ** 2
     D = raw.reshape(sp.nblock, net.maxNear, sp.blockSize, sp.blockSize)
     assert np.all(D[:, :, np.arange(sp.blockSize), np.arange(sp.blockSize)][:, 0] == 0)  # self
+
+
+def test_sratio_v1_scales_strengths_by_type_pair(pkg, tmp_path):
+    """sRatioV1 = [EE, EI, IE, II] multiplies each V1 strength by the
+    factor of its (pre type, post type) pair in float32 on reading, and a
+    zero factor removes those synapses."""
+    from visual_cortex_amd import config, loader, network
+    net = network.build_network(network.spec_tiny(nt=64))
+    cfg_path = loader.save_network(net, tmp_path / "in")
+    text = open(cfg_path).read()
+    assert "sRatioV1 = 1.0\n" in text                # repeated keys append: replace the line
+    with open(cfg_path, "w") as f:
+        f.write(text.replace("sRatioV1 = 1.0\n", "sRatioV1 = 0.5, 0.25\nsRatioV1 = 2.0, 0.0\n"))
+    back = loader.load_network(config.Config.from_file(cfg_path))
+    nE = net.spec.nE
+    sr = np.array([0.5, 0.25, 2.0, 0.0], np.float32)
+    for (p0, q0, w0), (p1, q1, w1) in zip(net.rec_pairs, back.rec_pairs):
+        k = (p0 >= nE).astype(int) * 2 + (q0 >= nE).astype(int)
+        keep = k != 3
+        assert np.array_equal(p1, p0[keep]) and np.array_equal(q1, q0[keep])
+        assert np.array_equal(w1.view(np.uint32), (w0 * sr[k])[keep].astype(np.float32).view(np.uint32))

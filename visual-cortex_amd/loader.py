@@ -6,7 +6,8 @@ Reads (see formats.py for the layouts and reference lines): V1_allpos,
 connectome_cfg, neighborBlock, V1_conMat, V1_delayMat (noDelay = 0),
 LGN_V1_idList / LGN_V1_sList,
 fLGN and stimulus.bin.  The neuron and synapse parameters come from the cfg
-keys (config.spec_from_config).
+keys (config.spec_from_config); sRatioV1 scales the V1 strengths on reading
+and sRatioLGN the LGN ones, as the reference does.
 """
 from __future__ import annotations
 
@@ -31,6 +32,7 @@ def load_network(cfg: cfgmod.Config) -> Network:
         raise ValueError("connectome_cfg typeAcc does not end at blockSize")
     nNear, lists = F.read_neighbor_block(cfg.path("fNeighborBlock"), nblock)
     maxNear, pairs = F.read_conmat_pairs(cfg.path("fV1_conMat"), nblock, bs)
+    pairs = scale_v1(pairs, cfg.floats("sRatioV1"), int(typeAcc[0]))
     if int(nNear.max()) > maxNear:
         raise ValueError("neighborBlock has more near neighbours than V1_conMat holds")
     nbr = np.zeros((nblock, maxNear), np.uint32)
@@ -61,6 +63,28 @@ def load_network(cfg: cfgmod.Config) -> Network:
     return Network(spec=spec, pos=pos, tonicDep=tonic, maxNear=maxNear, nNear=nNear, nbr=nbr,
                    rec_pairs=pairs, nLGN_of=nof, sLGN=sLGN, idLGN=idLGN, lgn=lgn,
                    frames=frames, init_lms=tuple(st["init_lms"]), rec_dist=rec_dist)
+
+
+def scale_v1(pairs, sratio, nE: int):
+    """conMat *= sRatioV1[preType*2 + postType] as the reference does on
+    reading (src/patch.cu:3655-3680; one value applies to every type pair,
+    src/patch.cu:725-734), in float32; pairs whose scaled strength is not
+    positive are dropped (recal_G skips zero strengths)."""
+    if sratio is None:
+        return pairs
+    sr = [float(x) for x in sratio]
+    if len(sr) == 1:
+        sr = sr * 4
+    if len(sr) != 4:
+        raise ValueError(f"sRatioV1 has {len(sr)} values, expected 1 or nType*nType = 4")
+    sr = np.asarray(sr, np.float32)
+    out = []
+    for pre, post, w in pairs:
+        k = (pre >= nE).astype(np.int64) * 2 + (post >= nE).astype(np.int64)
+        ws = (w.astype(np.float32) * sr[k]).astype(np.float32)
+        keep = ws > 0
+        out.append((pre[keep], post[keep], ws[keep]))
+    return out
 
 
 def save_network(net: Network, folder, suffix: str = "vc") -> Path:
@@ -97,7 +121,7 @@ def save_network(net: Network, folder, suffix: str = "vc") -> Path:
         f"iModel = {sp.iModel}", f"tau = {sp.tau}", f"nKernelSample = {sp.nKernelSample}",
         f"nSpatialSample1D = {sp.nSpatialSample1D}", f"nsig = {sp.nsig}",
         f"frRatioLGN = {sp.frRatioLGN}", f"convolRatio = {sp.convolRatio}",
-        f"sRatioLGN = {t(sp.sRatioLGN)}", f"bgRate = {sp.bgRate}",
+        f"sRatioLGN = {t(sp.sRatioLGN)}", "sRatioV1 = 1.0", f"bgRate = {sp.bgRate}",
         f"bgStrength = {t(sp.bgStrength)}",
     ]
     for k in ("vR", "vThres", "gL", "C", "tRef", "vT", "deltaT", "tau_w", "a", "b", "tonicDep"):
