@@ -1,6 +1,6 @@
-"""Orientation tuning, GPU against the oracle: the north star's rate and
-tuning criterion (per-neuron rates and orientation tuning within 2%), on a
-small network driven by gratings at 8 orientations.  With bit-identical
+"""Orientation tuning, GPU against the oracle: per-neuron rates and
+orientation tuning (the parity criterion for rates is agreement within 2%),
+on a small network driven by gratings at 8 orientations.  With bit-identical
 runs the rates, gOSI and OSI agree exactly."""
 import numpy as np
 import pytest

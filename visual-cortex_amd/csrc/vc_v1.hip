@@ -4,14 +4,14 @@
 // block, so every wavefront is one 64-neuron tile.  The kernel is latency
 // bound at the full-patch size (one wavefront per SIMD), so it is organised
 // around few dependent memory round trips:
-//   round 1: state, the LGN synapse indices/strengths of the neuron, and the
+//   trip 1: state, the LGN synapse indices/strengths of the neuron, and the
 //            spike masks of the 3x3 neighbour blocks (one 64-bit mask per
 //            tile, held lane-parallel: lane l <- tile entry l);
-//   round 2: the LGN spike words, and for the spikes of the neighbourhood
+//   trip 2: the LGN spike words, and for the spikes of the neighbourhood
 //            (listed in neighbour order, ascending neuron id, by a ballot
 //            scan over the non-empty tiles) one masked-CSR row-segment header
 //            per spike (lane k <- spike k);
-//   round 3: the synaptic weights each lane owns (rank via mbcnt).
+//   trip 3: the synaptic weights each lane owns (rank via mbcnt).
 // Sums are accumulated in the reference order: recurrent E and I sums over
 // neighbour blocks then ascending presynaptic id (src/coredynamics.cu
 // :2721-2838), LGN synapses in list order (src/coredynamics.cu:1466-1502).
@@ -443,11 +443,11 @@ template <> struct FFPre<false> {
     uint32_t fid[kFF];
 };
 
-__device__ __forceinline__ void ff_round1(FFPre<true> &f, const vc_ff_conn &, const uint32_t *ffMask,
+__device__ __forceinline__ void ff_trip1(FFPre<true> &f, const vc_ff_conn &, const uint32_t *ffMask,
                                           uint32_t n, uint32_t) {
     f.bits = ffMask[n];
 }
-__device__ __forceinline__ void ff_round1(FFPre<false> &f, const vc_ff_conn &ff, const uint32_t *,
+__device__ __forceinline__ void ff_trip1(FFPre<false> &f, const vc_ff_conn &ff, const uint32_t *,
                                           uint32_t n, uint32_t nV1) {
     f.m = ff.nLGN[n];
 #pragma unroll
@@ -462,7 +462,7 @@ __device__ __forceinline__ void ff_round1(FFPre<false> &f, const vc_ff_conn &ff,
     }
 }
 // events: the first kEv synapses that received a spike this step
-__device__ __forceinline__ void ff_round2(FFPre<true> &f, const vc_ff_conn &ff, const float *,
+__device__ __forceinline__ void ff_trip2(FFPre<true> &f, const vc_ff_conn &ff, const float *,
                                           uint32_t n, uint32_t nV1) {
     uint32_t bits = f.bits;
 #pragma unroll
@@ -480,7 +480,7 @@ __device__ __forceinline__ void ff_round2(FFPre<true> &f, const vc_ff_conn &ff, 
         }
     }
 }
-__device__ __forceinline__ void ff_round2(FFPre<false> &f, const vc_ff_conn &, const float *lgnSInfo,
+__device__ __forceinline__ void ff_trip2(FFPre<false> &f, const vc_ff_conn &, const float *lgnSInfo,
                                           uint32_t, uint32_t) {
 #pragma unroll
     for (int j = 0; j < kFF; ++j) f.sv[j] = ((uint32_t)j < f.m) ? lgnSInfo[f.fid[j]] : 0.0f;
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     const uint32_t nSeg = k.nSeg;
     const uint32_t myseg = local >> 6;
 
-    // ---- round 1: neighbour spike masks (lane-parallel), LGN synapses, state
+    // ---- memory trip 1: neighbour spike masks (lane-parallel), LGN synapses, state
     const uint32_t nb = rec.nNear[b];
     const uint32_t nEnt = nb * nSeg;
     for (uint32_t e = threadIdx.x; e < nEnt; e += blockDim.x) {
@@ -882,11 +882,11 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         sbid[e] = bid;
     }
     FFPre<kEvents> f;
-    if (kUpdate) ff_round1(f, ff, ffMask, n, nV1);
+    if (kUpdate) ff_trip1(f, ff, ffMask, n, nV1);
     NState x;
     load_cortical(p, s, n, x);
-    // ---- round 2: LGN spike words, then the recurrent gather
-    if (kUpdate) ff_round2(f, ff, lgnSInfo, n, nV1);
+    // ---- memory trip 2: LGN spike words, then the recurrent gather
+    if (kUpdate) ff_trip2(f, ff, lgnSInfo, n, nV1);
     float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
     float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
     __syncthreads();
@@ -1036,8 +1036,8 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
         // the LGN / background part does not depend on the neighbours: it
         // runs before their spikes are awaited
         FFPre<kEvents> f;
-        ff_round1(f, ff, fm, n, nV1);
-        ff_round2(f, ff, lgn, n, nV1);
+        ff_trip1(f, ff, fm, n, nV1);
+        ff_trip2(f, ff, lgn, n, nV1);
         const FFAcc acc = ff_stage<kEvents>(p, k, ff, t, n, b, it0 + q, x, f, lgn, fm);
         // the neighbours' spike masks of the previous step (tag q+1)
         const RecIn ri{words, rec_of(q & 1u), ovf_of(q & 1u), (epoch << 16) + q + 1u, status};
