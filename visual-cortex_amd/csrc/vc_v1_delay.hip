@@ -17,8 +17,8 @@
 namespace {
 
 constexpr int kMaxEntD = 512;      // neighbour tiles of one block (maxNear * blockSize/64)
-// spikers whose synapse records are loaded together: 32 when the LDS allows
-// (staging the records takes kGD*3 KiB per workgroup), else 16
+// spikers whose synapse records are loaded together: 16 when the LDS allows
+// (staging the records and their pairs takes kGD*3.5 KiB per workgroup), else 8
 constexpr uint32_t kLdsMax = 160u * 1024u;
 constexpr int kMaxDepthLds = 32;   // rings up to this long are staged in LDS
 constexpr int kStv = kMaxDepthLds + 1;  // LDS row stride of one spiker's ring values
@@ -98,6 +98,10 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
     __shared__ float stv[4][64 * kStv];
     __shared__ float gw[4][kGD_][64], gr[4][kGD_][64];     // per wavefront: strengths,
     __shared__ uint32_t gi[4][kGD_][64];                   // remainders, steps of kGD spikers
+    __shared__ uint16_t pl[4][kGD_ * 64];                  // (spiker, lane) pairs to evaluate
+    __shared__ float4 tq[4][64][2];                        // their increments, per slot j
+    __shared__ uint32_t tf[4][64];                         // accepted slots, presynaptic type
+    __shared__ uint32_t slp[4][64], sgp[4][64];            // the batch's spikers: local, global id
     const uint32_t nV1 = p.nV1, bs = p.blockSize, nSeg = bs >> 6;
     const uint32_t nTiles = nV1 >> 6;
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,6 +182,8 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
             hm = rec.rowMask[rs];
             ho = rec.rowOff[rs];
             gpre = sbid[ib] * bs + lpre;
+            slp[wv][lane] = lpre;
+            sgp[wv][lane] = gpre;
             if (staged) {
                 am = 0u;
                 for (uint32_t k0 = 0; k0 < depth; k0 += 8u) {
@@ -198,35 +204,38 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
             }
         }
         __syncthreads();
-        for (uint32_t g0 = 0; g0 < nIt; g0 += kGD_) {
-            // (a) strengths and delays of kGD spikers in flight together
-            uint32_t mine = 0u;                        // bit g: lane has a synapse from spiker g
-            static_assert(kGD_ <= 32, "mine is a 32-bit mask");
-            float wv_[kGD_], rem[kGD_];
-            uint32_t i2p[kGD_];
+        // (a) the records of all the batch's spikers in flight together: their
+        // random 16-byte reads are the kernel's longest round trip
+        uint64_t mine = 0ull;                          // bit k: lane has a synapse from spiker k
+        float wv_[64], rem[64];
+        uint32_t i2p[64];
 #pragma unroll
-            for (int g = 0; g < kGD_; ++g) {
-                wv_[g] = rem[g] = 0.0f;
-                i2p[g] = 0u;
-                const uint32_t kk = g0 + g;
-                if (kk < nIt) {
-                    const uint64_t m = vc_readlane_u64(hm, kk);
-                    if ((m >> lane) & 1ull) {
-                        const uint32_t idx = vc_readlane_u32(ho, kk) + vc_mbcnt(m);
-                        const uint4 r = syn[idx];
-                        wv_[g] = __uint_as_float(r.x);
-                        rem[g] = __uint_as_float(r.y);
-                        i2p[g] = r.z;
-                        mine |= 1u << g;
-                    }
+        for (int k = 0; k < 64; ++k) {
+            wv_[k] = rem[k] = 0.0f;
+            i2p[k] = 0u;
+            if ((uint32_t)k < nIt) {
+                const uint64_t m = vc_readlane_u64(hm, k);
+                if ((m >> lane) & 1ull) {
+                    const uint32_t idx = vc_readlane_u32(ho, k) + vc_mbcnt(m);
+                    const uint4 r = syn[idx];
+                    wv_[k] = __uint_as_float(r.x);
+                    rem[k] = __uint_as_float(r.y);
+                    i2p[k] = r.z;
+                    mine |= 1ull << k;
                 }
             }
+        }
+        // then kGD spikers at a time
+#pragma unroll
+        for (int c = 0; c < 64 / kGD_; ++c) {
+            const uint32_t g0 = (uint32_t)c * kGD_;
+            if (g0 >= nIt) break;
             // (b) the synapses one of whose two slots holds a spike, staged in LDS
             uint32_t act = 0u;
 #pragma unroll
             for (int g = 0; g < kGD_; ++g) {
-                if (!((mine >> g) & 1u)) continue;
-                const uint32_t it2post = i2p[g];
+                if (!((mine >> (g0 + g)) & 1ull)) continue;
+                const uint32_t it2post = i2p[g0 + g];
                 if (it2post >= depth) {               // ring too short for this delay
                     late = true;
                     continue;
@@ -238,68 +247,107 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
                     if (!need) continue;
                 }
                 act |= 1u << g;
-                gw[wv][g][lane] = wv_[g];
-                gr[wv][g][lane] = rem[g];
+                gw[wv][g][lane] = wv_[g0 + g];
+                gr[wv][g][lane] = rem[g0 + g];
                 gi[wv][g][lane] = it2post;
             }
-            uint32_t wact = 0u;                        // spikers some lane has to add
+            // (c) the (spiker, lane) pairs to evaluate, listed spiker-major, so
+            // each lane's pairs are in its summation order
+            uint32_t mypos[kGD_];
+            uint32_t P = 0u;
 #pragma unroll
-            for (int g = 0; g < kGD_; ++g)
-                if (__ballot((act >> g) & 1u)) wact |= 1u << g;
-            // (c) one code path for every spiker, in order (src/coredynamics.cu:2173-2235)
-            while (wact) {
-                const uint32_t g = (uint32_t)__builtin_ctz(wact);
-                wact &= wact - 1u;
-                if (!((act >> g) & 1u)) continue;
-                const uint32_t kk = g0 + g;
-                const float f = gw[wv][g][lane];
-                const float time2post = gr[wv][g][lane];
-                const uint32_t it2post = gi[wv][g][lane];
-                const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
-                const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
-                float tv0, tv1;
-                if (staged) {
-                    tv0 = sv[kk * kStv + j0];
-                    tv1 = sv[kk * kStv + j1];
-                } else {
-                    const uint32_t gp = vc_readlane_u32(gpre, kk);
-                    tv0 = train[(size_t)j0 * nV1 + gp];
-                    tv1 = train[(size_t)j1 * nV1 + gp];
-                }
-                // the presynaptic type picks the shape by value (one code path)
-                const bool inh = vc_readlane_u32(lpre, kk) >= p.nE;
-                const uint32_t ng = inh ? p.ngI : p.ngE;
-                float ip[VC_MAX_NG], rise[VC_MAX_NG], decay[VC_MAX_NG], dod[VC_MAX_NG];
+            for (int g = 0; g < kGD_; ++g) {
+                const uint64_t m = __ballot((act >> g) & 1u);
+                mypos[g] = P + vc_mbcnt(m);
+                if ((act >> g) & 1u) pl[wv][mypos[g]] = (uint16_t)(lane | ((uint32_t)g << 6));
+                P += (uint32_t)__popcll(m);
+            }
+            // (d) 64 pairs at a time, one per lane: the increments of src/coredynamics.cu
+            // :2173-2235 (single_input's two terms per receptor, per accepted slot)
+            for (uint32_t pb = 0; pb < P; pb += 64u) {
+                const uint32_t pi = pb + lane;
+                uint32_t fl = 0u;
+                float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+                if (pi < P) {
+                    const uint32_t code = pl[wv][pi];
+                    const uint32_t l = code & 63u, g = code >> 6, kk = g0 + g;
+                    const float f = gw[wv][g][l];
+                    const float time2post = gr[wv][g][l];
+                    const uint32_t it2post = gi[wv][g][l];
+                    const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
+                    const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
+                    float tv0, tv1;
+                    if (staged) {
+                        tv0 = sv[kk * kStv + j0];
+                        tv1 = sv[kk * kStv + j1];
+                    } else {
+                        tv0 = train[(size_t)j0 * nV1 + sgp[wv][kk]];
+                        tv1 = train[(size_t)j1 * nV1 + sgp[wv][kk]];
+                    }
+                    // shapes by the presynaptic type, split by the post neuron's type
+                    const bool inh = slp[wv][kk] >= p.nE;
+                    const int to = type_of(p, myseg * 64u + l);
+                    const uint32_t ng = inh ? p.ngI : p.ngE;
+                    float ip[VC_MAX_NG], rise[VC_MAX_NG], decay[VC_MAX_NG], dod[VC_MAX_NG];
 #pragma unroll
-                for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-                    ip[ig] = inh ? ipI[ig] : ipE[ig];
-                    rise[ig] = inh ? p.riseI[ig] : p.riseE[ig];
-                    decay[ig] = inh ? p.decayI[ig] : p.decayE[ig];
-                    dod[ig] = inh ? p.dodI[ig] : p.dodE[ig];
-                }
+                    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+                        ip[ig] = ig < ng ? (inh ? p.pI[to * p.ngI + ig] : p.pE[to * p.ngE + ig])
+                                         : 0.0f;
+                        rise[ig] = inh ? p.riseI[ig] : p.riseE[ig];
+                        decay[ig] = inh ? p.decayI[ig] : p.decayE[ig];
+                        dod[ig] = inh ? p.dodI[ig] : p.dodE[ig];
+                    }
+                    fl = inh ? 4u : 0u;
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    float dtsp = j ? tv1 : tv0;
-                    if (dtsp >= 1.0f) {
-                        const float nsp = floorf(dtsp);
-                        dtsp = (dtsp - nsp + (float)j) * dt - time2post;
-                        if (dtsp < dt && dtsp >= 0.0f && f > 0.0f) {
-                            const float ddt = dt - dtsp;
-                            const float a = nsp * f;
+                    for (int jj = 0; jj < 2; ++jj) {
+                        float dtsp = jj ? tv1 : tv0;
+                        if (dtsp >= 1.0f) {
+                            const float nsp = floorf(dtsp);
+                            dtsp = (dtsp - nsp + (float)jj) * dt - time2post;
+                            if (dtsp < dt && dtsp >= 0.0f && f > 0.0f) {
+                                const float ddt = dt - dtsp;
+                                const float a = nsp * f;
+                                float tg[VC_MAX_NG], th[VC_MAX_NG];
 #pragma unroll
-                            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
-                                if (ig < ng) {
-                                    // single_input's two increments; added to the E or the
-                                    // I sums by value (+0 leaves the other sum unchanged)
+                                for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
                                     const float fi = a * ip[ig];
                                     const float etr = vc_expf(-ddt / rise[ig]);
-                                    const float tg = fi * dod[ig] * (vc_expf(-ddt / decay[ig]) - etr);
-                                    const float th = fi * etr;
-                                    lgE[ig] = lgE[ig] + (inh ? 0.0f : tg);
-                                    lhE[ig] = lhE[ig] + (inh ? 0.0f : th);
-                                    lgI[ig] = lgI[ig] + (inh ? tg : 0.0f);
-                                    lhI[ig] = lhI[ig] + (inh ? th : 0.0f);
+                                    tg[ig] = fi * dod[ig] * (vc_expf(-ddt / decay[ig]) - etr);
+                                    th[ig] = fi * etr;
                                 }
+                                const float4 q = make_float4(tg[0], th[0], tg[1], th[1]);
+                                if (jj) q1 = q; else q0 = q;
+                                fl |= 1u << jj;
+                            }
+                        }
+                    }
+                }
+                tq[wv][lane][0] = q0;
+                tq[wv][lane][1] = q1;
+                tf[wv][lane] = fl;
+                // (e) every lane adds its pairs of this pass in its order: spiker,
+                // slot j, receptor (the I or E sums by value: +0 leaves the other
+                // sum unchanged)
+#pragma unroll
+                for (int g = 0; g < kGD_; ++g) {
+                    if (!((act >> g) & 1u)) continue;
+                    const uint32_t q = mypos[g] - pb;
+                    if (q >= 64u) continue;
+                    const uint32_t f2 = tf[wv][q];
+                    const bool inh = (f2 >> 2) & 1u;
+                    const uint32_t ng = inh ? p.ngI : p.ngE;
+#pragma unroll
+                    for (int jj = 0; jj < 2; ++jj) {
+                        if (!((f2 >> jj) & 1u)) continue;
+                        const float4 v = tq[wv][q][jj];
+                        const float tg[VC_MAX_NG] = {v.x, v.z}, th[VC_MAX_NG] = {v.y, v.w};
+#pragma unroll
+                        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+                            if (ig < ng) {
+                                lgE[ig] = lgE[ig] + (inh ? 0.0f : tg[ig]);
+                                lhE[ig] = lhE[ig] + (inh ? 0.0f : th[ig]);
+                                lgI[ig] = lgI[ig] + (inh ? tg[ig] : 0.0f);
+                                lhI[ig] = lhI[ig] + (inh ? th[ig] : 0.0f);
                             }
                         }
                     }
@@ -326,7 +374,9 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
 template <int kGD_>
 constexpr size_t lds_static() {
     return sizeof(uint64_t) * kMaxEntD + sizeof(uint32_t) * (2 * kMaxEntD + 1) +
-           sizeof(float) * 4 * 64 * kStv + 3 * sizeof(float) * 4 * kGD_ * 64;
+           sizeof(float) * 4 * 64 * kStv + 3 * sizeof(float) * 4 * kGD_ * 64 +
+           sizeof(uint16_t) * 4 * kGD_ * 64 + sizeof(float) * 4 * 64 * 8 +
+           sizeof(uint32_t) * 4 * 64 * 3;
 }
 
 bool rec_ok(const vc_v1_params *p, const vc_rec_conn *rec, const char *who) {
@@ -419,11 +469,11 @@ extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_
     const uint32_t wg = wg_size(p->blockSize);
     const size_t dyn = (size_t)rec->maxNear * p->blockSize * sizeof(uint16_t);
     const uint4 *sy = (const uint4 *)syn;
-    if (dyn + lds_static<32>() + 1024u <= kLdsMax)     // margin for alignment padding
-        hipLaunchKernelGGL(k_rec_delay<32>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
+    if (dyn + lds_static<16>() + 1024u <= kLdsMax)     // margin for alignment padding
+        hipLaunchKernelGGL(k_rec_delay<16>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
                            *p, *s, *rec, sy, train, trainMask, depth, cur, status);
     else
-        hipLaunchKernelGGL(k_rec_delay<16>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
+        hipLaunchKernelGGL(k_rec_delay<8>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
                            *p, *s, *rec, sy, train, trainMask, depth, cur, status);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
