@@ -187,3 +187,31 @@ def test_delayed_snapshot_resume_bitwise(pkg, tmp_path):
     b.run(2 * npf)
     tail = b.sInfo()
     assert np.array_equal(tail.view(np.uint32), ref[2 * npf:].view(np.uint32))
+
+
+def test_batched_delayed_trials_equal_serial(pkg):
+    """Trials of different sizes with axonal delays batched into one network:
+    each trial's output equals its own delayed run bit for bit (the ring is
+    as long as the longest trial's delays; a longer ring changes nothing)."""
+    from visual_cortex_amd import batch, network
+    from visual_cortex_amd.sim import GpuSim
+    _flush("test_batched_delayed_trials_equal_serial: 3 trials, speed 0.5, 192 steps")
+    specs = [network.spec_tiny(nblock=nb, seed=200 + i, ori_deg=22.5 * i, bgStrength=(0.4, 0.4),
+                               noDelay=0, speedOfThought=0.5)
+             for i, nb in enumerate((1, 3, 2))]
+    nets = [batch.build_network_fast(s, device="cuda") for s in specs]
+    bnet = batch.batch_networks(nets)
+    assert bnet.spec.noDelay == 0
+    g = GpuSim(bnet)
+    assert g.delay
+    g.run(192)
+    sb = g.sInfo()
+    assert int((sb >= 1).sum()) > 100
+    off = 0
+    for t, n in enumerate(nets):
+        gs = GpuSim(n)
+        assert gs.delay and gs.depth <= g.depth
+        gs.run(192)
+        assert np.array_equal(sb[:, off:off + n.spec.nV1].view(np.uint32),
+                              gs.sInfo().view(np.uint32)), t
+        off += n.spec.nV1
