@@ -57,11 +57,13 @@ CASES = {
     "mixed_tile": (dict(nE=200, nblock=3, bgStrength=(0.4, 0.4)), 300),
     # one block of 64 neurons, one tile
     "one_tile": (dict(nblock=1, blockSize=64, nE=48, KE=20, KI=5, bgStrength=(0.4, 0.4)), 300),
+    # a temporal kernel of 32 taps (the rate kernel's generic tap count)
+    "taps32": (dict(nKernelSample=32, bgStrength=(0.4, 0.4)), 200),
 }
 
 
 @pytest.mark.parametrize("case,events,persistent,dense",
-                         [(c, True, True, True) for c in CASES]
+                         [(c, True, True, True) for c in CASES if c != "taps32"]
                          + [(c, True, False, False) for c in CASES]
                          + [("tiny", True, True, False), ("burst", True, False, True),
                             ("tiny", False, True, True), ("burst", False, True, False),

@@ -205,7 +205,9 @@ constexpr int kWin = 8;                     // rate window of k_lgn_spike
 // Stage a (temporal part) and the rate of stage b for every (cell, step):
 // one wavefront per cell, lane = step, so the cell's temporal weights and
 // frame responses are wave-uniform (scalar loads); the tap products and the
-// reference's tap-sum tree are evaluated in registers.
+// reference's tap-sum tree are evaluated in registers.  kNT > 0 fixes the
+// tap count at compile time (the configurations' 40), 0 takes it from p.
+template <int kNT>
 __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state st,
                                                   const float *__restrict__ Rring,
                                                   uint32_t it0, uint32_t nsteps,
@@ -251,17 +253,21 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
         // the cell's temporal weights of region r: lane i holds tap i, read
         // back per tap as a scalar operand
         const int tw = __float_as_int(ln < nKS ? st.TW[((size_t)c * 2 + r) * nKS + ln] : 0.0f);
+        constexpr int kT = kNT > 0 ? kNT : kMaxTaps;   // taps walked (the rest are zeros)
 #pragma unroll
         for (int i = 0; i < kMaxTaps; ++i) {
             float v = 0.0f;
-            if (i < (int)nKS) v = sR[wv][r][dd] * __int_as_float(__builtin_amdgcn_readlane(tw, i));
-            if (i < 32) a0[i] = v; else a1[i - 32] = v;
-            rr += dr;
-            dd += (int)dq;
-            if (rr >= npf) {
-                rr -= npf;
-                dd += 1;
+            if (i < kT) {
+                if (kNT > 0 || i < (int)nKS)
+                    v = sR[wv][r][dd] * __int_as_float(__builtin_amdgcn_readlane(tw, i));
+                rr += dr;
+                dd += (int)dq;
+                if (rr >= npf) {
+                    rr -= npf;
+                    dd += 1;
+                }
             }
+            if (i < 32) a0[i] = v; else a1[i - 32] = v;
         }
         const float P0 = tree32_regs(a0);
         const float P1 = tree32_regs(a1);
@@ -278,6 +284,16 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
     const float fr = p.frRatio * mx * transform(st.c50[c], st.sharpness[c], st.logA[c], st.logB[c],
                                                 current / mx);
     if (s < nsteps) frOut[(size_t)s * p.nLGN + c] = fr;
+}
+
+// k_lgn_rate with the tap count fixed when it is the configurations' 40.
+void launch_rate(hipStream_t st, uint64_t waves, const vc_lgn_params &p, const vc_lgn_state &s,
+                 const float *R, uint32_t it0, uint32_t nsteps, float *fr) {
+    const dim3 grid((uint32_t)((waves * 64 + 255) / 256)), block(256);
+    if (p.nKernelSample == 40)
+        hipLaunchKernelGGL(k_lgn_rate<40>, grid, block, 0, st, p, s, R, it0, nsteps, fr);
+    else
+        hipLaunchKernelGGL(k_lgn_rate<0>, grid, block, 0, st, p, s, R, it0, nsteps, fr);
 }
 
 // Stage b, the time-rescaled Poisson draw (get_spike,
@@ -658,8 +674,7 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
     }
     if (nsteps == 0 || p->nLGN == 0) return VC_OK;
     const uint64_t waves = (uint64_t)p->nLGN * ((nsteps + 63u) / 64u);
-    hipLaunchKernelGGL(k_lgn_rate, dim3((uint32_t)((waves * 64 + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, *p, *s, Rring, it0, nsteps, fr);
+    launch_rate((hipStream_t)stream, waves, *p, *s, Rring, it0, nsteps, fr);
     VC_HIP_CHECK(hipGetLastError());
     vc_ff_events e{};
     if (ev) {
@@ -701,8 +716,7 @@ extern "C" int vc_lgn_rates(void *stream, const vc_lgn_params *p, const vc_lgn_s
                      p->nLGN);
         return VC_E_INVALID;
     }
-    hipLaunchKernelGGL(k_lgn_rate, dim3((uint32_t)((waves * 64 + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, *p, *s, R, it0, nsteps, fr);
+    launch_rate((hipStream_t)stream, waves, *p, *s, R, it0, nsteps, fr);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
