@@ -326,24 +326,27 @@ __global__ __launch_bounds__(256) void k_lgn_frame_adj(
             const uint32_t tC = coneType[c], tS = coneType[nLGN + c];
             const float *xC = SCx + (size_t)c * 2 * nS, *yC = SCy + (size_t)c * 2 * nS;
             const float *xS = xC + nS, *yS = yC + nS;
+            // surround intensities of both cone types (one evaluation when the
+            // types agree), kept for the second pass
             float sumC = 0.0f, sumS = 0.0f;
-            for (uint32_t k = 0; k < spl; ++k) {
-                const uint32_t s = lane + 64u * k;
-                if (s < nS) {
-                    sumC += intensity(tC, frame, W, H, xS[s], yS[s]);
-                    sumS += intensity(tS, frame, W, H, xS[s], yS[s]);
-                }
-            }
-            const float meanC = wave_sum(sumC) * inv_nS, meanS = wave_sum(sumS) * inv_nS;
             float icc[kMaxSPL], iss[kMaxSPL];
-            float tmC = 0.0f, tmS = 0.0f;
 #pragma unroll
             for (int k = 0; k < kMaxSPL; ++k) {
                 icc[k] = iss[k] = 0.0f;
                 const uint32_t s = lane + 64u * k;
                 if ((uint32_t)k < spl && s < nS) {
-                    icc[k] = intensity(tC, frame, W, H, xC[s], yC[s]);
                     iss[k] = intensity(tS, frame, W, H, xS[s], yS[s]);
+                    sumC += tC == tS ? iss[k] : intensity(tC, frame, W, H, xS[s], yS[s]);
+                    sumS += iss[k];
+                }
+            }
+            const float meanC = wave_sum(sumC) * inv_nS, meanS = wave_sum(sumS) * inv_nS;
+            float tmC = 0.0f, tmS = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kMaxSPL; ++k) {
+                const uint32_t s = lane + 64u * k;
+                if ((uint32_t)k < spl && s < nS) {
+                    icc[k] = intensity(tC, frame, W, H, xC[s], yC[s]);
                     float dI, dm;
                     contrast_adj(icc[k], meanC, dI, dm);
                     tmC += dRC * SW[s] * dm;
