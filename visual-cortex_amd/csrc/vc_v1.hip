@@ -981,6 +981,9 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
                                                   float *__restrict__ out_sInfo, uint64_t outStride,
                                                   uint64_t *words, uint32_t epoch,
                                                   uint32_t *status) {
+    // the step chain is the critical path; the LGN stage's waves that share
+    // the SIMDs run a frame period ahead, so they yield issue slots to these
+    __builtin_amdgcn_s_setprio(3);
     __shared__ GatherLds glds;
     __shared__ uint64_t smk[kMaxEnt];
     __shared__ uint32_t sbid[kMaxEnt];
