@@ -31,6 +31,8 @@
  *   vc_v1_rec_delay_table  its per-synapse delay arithmetic, :2173-2186
  *   vc_v1_rec_distances    the V1_delayMat distances of genCon
  *                            src/preprocess/connect.cu:424-466
+ *   vc_v1_rec_far          recal_G_vec (far connections with delays) +
+ *                            sum_G, src/coredynamics.cu:181-372, 2314-2349
  * The data-structure layouts are documented field by field below.
  */
 #ifndef VC_ABI_H
@@ -268,6 +270,32 @@ int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                     const vc_rec_conn *rec, const uint32_t *syn,
                     const float *train, const uint64_t *trainMask, uint32_t depth,
                     uint32_t cur, uint32_t *status);
+
+/* Far (long-range) connections, the reference's V1_vec lists (src/patch.cu
+ * :3308-3352: per post neuron, presynaptic ids, strengths * sRatioV1 and
+ * distances in mm), as a transposed ELL: connection j < nFar[n] of post
+ * neuron n at [j][nV1], in the file's order. */
+typedef struct {
+    uint32_t maxFar;            /* max nFar; 0 = no far connections             */
+    const uint32_t *nFar;       /* [nV1] */
+    const uint32_t *pre;        /* [maxFar][nV1] global presynaptic id          */
+    const float *w;             /* [maxFar][nV1] strength, already * sRatioV1   */
+    const float *dist;          /* [maxFar][nV1] distance, mm                   */
+} vc_far_conn;
+
+/* vc_v1_rec_far: adds to s->gE/hE/gI/hI the far connections' input arriving
+ * during the step in slot `cur` of the ring vc_v1_rec_delay reads (same
+ * train, depth and cur): the reference's recal_G_vec with distance-dependent
+ * delays (noFarDelay 0, src/coredynamics.cu:181-372) followed by sum_G
+ * (:2314-2349).  Call it after vc_v1_rec_delay of the same step, as the
+ * reference adds the far sums after recal_G_mat.  Per connection time2post =
+ * dist/speed, it2post = ceil(time2post/dt); depth must exceed every it2post
+ * (the reference's fTrainDepth = it2post + 1, src/patch.cu:3847).  *status
+ * (may be NULL) |= 1 for a delay that does not fit, |= 2 for a presynaptic
+ * id >= nV1; such connections are skipped. */
+int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                  const vc_far_conn *far, const float *train, uint32_t depth,
+                  uint32_t cur, float speed, uint32_t *status);
 
 /* ---- reverse mode of the V1 step (no counterpart in the reference; the
  * forward it differentiates is vc_v1_step, i.e. compute_V_collect_spike_

@@ -37,6 +37,8 @@ class Oracle:
         L.orc_v1_rec_delay.argtypes = [P(abi_mod.V1Params)] + [vp] * 4 + [u32] + [vp] * 4 + \
             [f32, vp, u32, u32]
         L.orc_v1_rec_delay.restype = i32
+        L.orc_v1_rec_far.argtypes = [P(abi_mod.V1Params)] + [vp] * 8 + [f32, vp, u32, u32]
+        L.orc_v1_rec_far.restype = i32
         L.orc_lgn_frame_response.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp,
                                              C.c_uint64, vp]
         L.orc_lgn_steps.argtypes = [P(abi_mod.LgnParams)] + [vp] * 11 + [u32, u32, vp, vp, vp]
@@ -141,6 +143,13 @@ class OracleSim:
             nnz = net.n_rec_synapses_csr()
             dmax = float(net.rec_distances()[:nnz].max()) if nnz else 0.0
             self.depth = network.train_depth(dmax, sp.speedOfThought, sp.dt)
+            # delayed far connections (recal_G_vec + sum_G) read the same ring
+            self.far_on = net.n_far() > 0 and net.far_delayed()
+            if self.far_on:
+                self.far = [np.ascontiguousarray(x) for x in net.far]
+                fmax = float(self.far[3].max()) if self.far[3].size else 0.0
+                self.depth = max(self.depth,
+                                 network.train_depth(fmax, sp.speedOfThought, sp.dt))
             self.train = np.zeros((self.depth, sp.nV1), np.float32)
         self.lTR = np.zeros(sp.nLGN, np.float32)
         self.lNL = np.zeros(sp.nLGN, np.float32)
@@ -246,6 +255,14 @@ class OracleSim:
                                                _fp(self.train), self.depth, slot)
                 if rc != 0:
                     raise RuntimeError(f"orc_v1_rec_delay returned {rc}")
+                if self.far_on:
+                    off, pre, w, dist = self.far
+                    rc = self.o.L.orc_v1_rec_far(C.byref(self.p), _fp(st["gE"]), _fp(st["hE"]),
+                                                 _fp(st["gI"]), _fp(st["hI"]), _fp(off),
+                                                 _fp(pre), _fp(w), _fp(dist), sp.speedOfThought,
+                                                 _fp(self.train), self.depth, slot)
+                    if rc != 0:
+                        raise RuntimeError(f"orc_v1_rec_far returned {rc}")
             if record:
                 self.sInfo_hist.append(out)
             self.it += 1

@@ -25,6 +25,8 @@
  *                        [nblock][maxNear][bs pre][bs post]
  *   - delayed recurrent  src/coredynamics.cu:2062-2309 (recal_G_mat, axonal
  *     gather             delays from a ring of past step outputs)
+ *   - far connections    src/coredynamics.cu:181-372, 2314-2349 (recal_G_vec
+ *                        with delays, sum_G)
  * Reductions follow the reference's block_reduce tree (32-lane shuffle-down
  * halving per warp, then halving over the warp partials,
  * src/util/cuda_util.cuh:22-127).
@@ -502,6 +504,77 @@ int orc_v1_rec_delay(const vc_v1_params *p, float *gE, float *hE, float *gI,
                                                          dtsp, p->riseI[ig], p->decayI[ig],
                                                          p->dodI[ig]);
                                 }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+            gE[ig * nV1 + n] += lgE[ig];
+            hE[ig * nV1 + n] += lhE[ig];
+        }
+        for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+            gI[ig * nV1 + n] += lgI[ig];
+            hI[ig * nV1 + n] += lhI[ig];
+        }
+    }
+    return bad ? VC_E_INVALID : VC_OK;
+}
+
+/*
+ * Far (long-range) connections with delays: recal_G_vec, src/coredynamics.cu
+ * :181-372 (the noFarDelay = 0 branch; synFail 0), then sum_G (:2314-2349):
+ * for every post neuron n with far inputs, its list in file order, ring
+ * slots k = 0, 1 of the delay, receptors; the sum is added to gE/hE/gI/hI.
+ *   fOff[nV1+1], fPre[], fW[] (strength, already * sRatioV1), fDist[] (mm);
+ *   train: [depth][nV1] step outputs, the newest in slot cur.
+ * The reference keeps one ring per connection, fTrainDepth = ceil(d/speed/dt)
+ * + 1 slots long, written every step (src/patch.cu:3838-3851, 5808); the
+ * slots it reads hold the same steps' outputs as the shared ring here.
+ */
+int orc_v1_rec_far(const vc_v1_params *p, float *gE, float *hE, float *gI,
+                   float *hI, const uint32_t *fOff, const uint32_t *fPre,
+                   const float *fW, const float *fDist, float speed,
+                   const float *train, uint32_t depth, uint32_t cur) {
+    const uint32_t nV1 = p->nV1, bs = p->blockSize;
+    const float dt = p->dt;
+    int bad = 0;
+    if (depth == 0 || cur >= depth) return VC_E_INVALID;
+#pragma omp parallel for schedule(static) reduction(|:bad)
+    for (uint32_t n = 0; n < nV1; ++n) {
+        if (fOff[n + 1] == fOff[n]) continue;          /* sum_G: nVec[id] > 0 */
+        int t = orc_type_of(p, n % bs);
+        float lgE[VC_MAX_NG] = {0}, lhE[VC_MAX_NG] = {0};
+        float lgI[VC_MAX_NG] = {0}, lhI[VC_MAX_NG] = {0};
+        for (uint32_t j = fOff[n]; j < fOff[n + 1]; ++j) {
+            uint32_t ipre = fPre[j], tid = ipre % bs;
+            float strength = fW[j];
+            float time2post = fDist[j] / speed;
+            uint32_t it2post = (uint32_t)ceilf(time2post / dt);
+            time2post = (float)it2post * dt - time2post;
+            if (it2post >= depth) { bad = 1; continue; }
+            int k0 = (int)cur - (int)it2post;
+            if (k0 < 0) k0 += (int)depth;
+            for (uint32_t k = 0; k < 2; ++k) {
+                float dtsp = train[(uint64_t)nV1 * ((k0 + k) % depth) + ipre];
+                if (dtsp >= 1.0f) {
+                    uint32_t nsp = (uint32_t)floorf(dtsp);
+                    dtsp = (dtsp - (float)nsp + (float)k) * dt - time2post;
+                    if (dtsp < dt && dtsp >= 0.0f) {
+                        float f = strength;
+                        if (f > 0.0f) {
+                            dtsp = dt - dtsp;
+                            if (tid < p->nE) {
+                                for (uint32_t ig = 0; ig < p->ngE; ++ig)
+                                    orc_single_input(&lgE[ig], &lhE[ig],
+                                                     (float)nsp * f * p->pE[t * p->ngE + ig], dtsp,
+                                                     p->riseE[ig], p->decayE[ig], p->dodE[ig]);
+                            } else {
+                                for (uint32_t ig = 0; ig < p->ngI; ++ig)
+                                    orc_single_input(&lgI[ig], &lhI[ig],
+                                                     (float)nsp * f * p->pI[t * p->ngI + ig], dtsp,
+                                                     p->riseI[ig], p->decayI[ig], p->dodI[ig]);
                             }
                         }
                     }

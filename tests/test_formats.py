@@ -32,9 +32,12 @@ def test_cfg_semantics(pkg):
         config.spec_from_config(config.Config("InhGap = true"))
     with pytest.raises(NotImplementedError):
         config.spec_from_config(config.Config("noisyDep = 0.1, 0"))
-    sp = config.spec_from_config(config.Config("noDelay = 0\nspeedOfThought = 0.5"))
-    assert sp.noDelay == 0 and sp.speedOfThought == 0.5
-    assert config.spec_from_config(config.Config("")).noDelay == 1      # the reference's default
+    sp = config.spec_from_config(config.Config("noDelay = 0\nnoFarDelay = 0\nspeedOfThought = 0.5"))
+    assert sp.noDelay == 0 and sp.speedOfThought == 0.5 and sp.noFarDelay == 0
+    sp = config.spec_from_config(config.Config(""))
+    assert sp.noDelay == 1 and sp.noFarDelay == 1                       # the reference's defaults
+    with pytest.raises(ValueError):              # noFarDelay defaults to 1: refused with delays
+        config.spec_from_config(config.Config("noDelay = 0"))
 
 
 def test_stimulus_and_lists_round_trip(pkg, tmp_path):
@@ -142,3 +145,37 @@ def test_sratio_v1_scales_strengths_by_type_pair(pkg, tmp_path):
         keep = k != 3
         assert np.array_equal(p1, p0[keep]) and np.array_equal(q1, q0[keep])
         assert np.array_equal(w1.view(np.uint32), (w0 * sr[k])[keep].astype(np.float32).view(np.uint32))
+
+
+def test_v1_vec_round_trip(pkg, tmp_path):
+    """V1_vec written and read back (both connectLongRange forms); sRatioV1
+    scales the strengths by pre and post type on reading; a saved delayed
+    network with far lists loads them back."""
+    from visual_cortex_amd import config, formats, loader, network
+    net = network.build_network(network.spec_tiny(nt=64, nblock=4, noDelay=0,
+                                                  speedOfThought=0.5, nFar=5))
+    sp = net.spec
+    off, pre, w, d = net.far
+    for clr in (0, 1):
+        path = tmp_path / f"V1_vec{clr}.bin"
+        formats.write_v1_vec(path, sp.nV1, net.far, connect_long_range=clr)
+        assert path.stat().st_size == 4 + 4 * sp.nV1 * (1 + clr) + 12 * pre.size
+        o2, p2, w2, d2 = formats.read_v1_vec(path, sp.nV1, sp.blockSize, sp.nE)
+        for a, b in ((off, o2), (pre, p2), (w, w2), (d, d2)):
+            assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+    sr = [0.5, 2.0, 3.0, 0.25]
+    _, _, w3, _ = formats.read_v1_vec(tmp_path / "V1_vec0.bin", sp.nV1, sp.blockSize, sp.nE, sr)
+    post = np.repeat(np.arange(sp.nV1), np.diff(off.astype(np.int64)))
+    k = (pre % sp.blockSize >= sp.nE) * 2 + (post % sp.blockSize >= sp.nE)
+    assert np.array_equal(w3, (w * np.float32(sr)[k]).astype(np.float32))
+    assert len(set(k.tolist())) == 4
+    cfgp = loader.save_network(net, tmp_path / "in")
+    back = loader.load_network(config.Config.from_file(cfgp))
+    assert back.spec.noFarDelay == 0 and back.far_delayed()
+    for a, b in zip(net.far, back.far):
+        assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+    # no far connections: an all-zero nVec file, read back as none
+    net0 = network.build_network(network.spec_tiny(nt=64))
+    cfg0 = loader.save_network(net0, tmp_path / "in0")
+    assert (tmp_path / "in0" / "V1_vec-vc.bin").stat().st_size == 4 + 4 * net0.spec.nV1
+    assert loader.load_network(config.Config.from_file(cfg0)).far is None

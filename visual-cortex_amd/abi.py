@@ -74,6 +74,10 @@ class RecConn(C.Structure):
                 ("rowOff", vp), ("w", vp), ("wDense", vp)]
 
 
+class FarConn(C.Structure):
+    _fields_ = [("maxFar", u32), ("nFar", vp), ("pre", vp), ("w", vp), ("dist", vp)]
+
+
 class Spikes(C.Structure):
     _fields_ = [("mask", vp), ("og", vp), ("oh", vp)]
 
@@ -131,6 +135,8 @@ SIGNATURES = {
     "vc_v1_rec_delay_table": [vp, C.c_uint64, vp, vp, C.c_float, C.c_float, vp],
     "vc_v1_rec_delay": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn), vp,
                         vp, vp, u32, u32, vp],
+    "vc_v1_rec_far": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FarConn), vp,
+                      u32, u32, C.c_float, vp],
     "vc_v1_step_snapshot": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
                             C.POINTER(RecConn), C.POINTER(Spikes), C.POINTER(Spikes), vp, vp,
                             u32, vp, vp],

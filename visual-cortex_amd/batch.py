@@ -168,6 +168,8 @@ def batch_networks(nets: list, device=None) -> N.Network:
                                       nets[0].frames.shape, sp0.noDelay, sp0.speedOfThought):
             raise ValueError("batched trials must share block size, dt, stimulus shape and "
                              "axonal-delay settings")
+        if getattr(n, "far", None) is not None and n.n_far() > 0:
+            raise NotImplementedError("batched trials with far connections are not built")
     if device is None:
         c0 = nets[0].csr
         device = c0[0].device if c0 is not None and hasattr(c0[0], "device") else \

@@ -26,6 +26,7 @@ FILES = {
     "fV1_conMat": ("V1_conMat", "inputFolder", "conV1_suffix"),
     "fV1_delayMat": ("V1_delayMat", "inputFolder", "conV1_suffix"),   # src/patch.cu:352, 467-470
     "fNeighborBlock": ("neighborBlock", "inputFolder", "conV1_suffix"),
+    "fV1_vec": ("V1_vec", "inputFolder", "conV1_suffix"),             # src/patch.cu:355, 482-486
     "fLGN_V1_ID": ("LGN_V1_idList", "inputFolder", "conLGN_suffix"),
     "fLGN_V1_s": ("LGN_V1_sList", "inputFolder", "conLGN_suffix"),
     "fLGN": ("LGN", "outputFolder", "output_suffix"),
@@ -194,6 +195,11 @@ def spec_from_config(cfg: Config, base=None):
         kw["noDelay"] = cfg.int("noDelay")
     if cfg.has("speedOfThought"):
         kw["speedOfThought"] = cfg.float("speedOfThought")
+    # far connections: noFarDelay defaults to 1, and 1 with noDelay 0 is refused,
+    # as the reference does (src/patch.cu:237, 839-842)
+    kw["noFarDelay"] = cfg.int("noFarDelay", 1)
+    if kw["noFarDelay"] and not kw.get("noDelay", sp.noDelay):
+        raise ValueError("noFarDelay = 1 with noDelay = 0: delay or no delay is the question")
     if cfg.bool("InhGap", False):
         raise NotImplementedError("InhGap (gap junctions) is not built")
     if cfg.int("learning", 0) != 0:

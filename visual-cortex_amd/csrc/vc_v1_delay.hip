@@ -11,7 +11,8 @@
 // ascending presynaptic id, slot j = 0 then 1, receptor), so the kernel is
 // organised to keep many loads in flight per wavefront rather than to split
 // a post neuron's sum.  The sums are added to gE/hE/gI/hI at the end, as the
-// reference does.
+// reference does.  The far connections' delayed input (recal_G_vec) reads
+// the same ring.
 #include "vc_v1_device.h"
 
 namespace {
@@ -371,6 +372,84 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
 }
 
 // Static LDS of k_rec_delay<kGD_> (its declarations above), bytes.
+// Far (long-range) connections with delays: the reference's recal_G_vec,
+// src/coredynamics.cu:181-372 (noFarDelay 0, synFail 0), whose sums sum_G
+// (:2314-2349) adds to gE/hE/gI/hI after recal_G_mat.  One thread per post
+// neuron; its list is read in order from the transposed ELL (connection j of
+// neuron n at j*nV1 + n, so a wavefront's record loads are coalesced) and
+// the two ring slots of each connection are gathered from the shared ring,
+// which holds the same steps' outputs as the reference's per-connection
+// rings.  status |= 1 for a delay longer than the ring, |= 2 for a
+// presynaptic id out of range (both skipped).
+__global__ __launch_bounds__(256) void k_rec_far(vc_v1_params p, vc_v1_state s,
+                                                 vc_far_conn fc,
+                                                 const float *__restrict__ train,
+                                                 uint32_t depth, uint32_t cur, float speed,
+                                                 uint32_t *__restrict__ status) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nV1 = p.nV1;
+    if (n >= nV1) return;
+    const uint32_t cnt = fc.nFar[n];
+    if (cnt == 0u) return;                        // sum_G: nVec[id] > 0 only
+    const float dt = p.dt;
+    const int to = type_of(p, n % p.blockSize);
+    float lgE[VC_MAX_NG], lhE[VC_MAX_NG], lgI[VC_MAX_NG], lhI[VC_MAX_NG];
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) lgE[ig] = lhE[ig] = lgI[ig] = lhI[ig] = 0.0f;
+    uint32_t bad = 0u;
+    for (uint32_t j = 0; j < cnt; ++j) {
+        const size_t e = (size_t)j * nV1 + n;
+        const uint32_t ipre = fc.pre[e];
+        const float strength = fc.w[e];
+        float time2post = fc.dist[e] / speed;
+        const uint32_t it2post = (uint32_t)ceilf(time2post / dt);
+        time2post = (float)it2post * dt - time2post;
+        if (ipre >= nV1) { bad |= 2u; continue; }
+        if (it2post >= depth) { bad |= 1u; continue; }
+        const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
+        const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
+        const float tv[2] = {train[(size_t)j0 * nV1 + ipre], train[(size_t)j1 * nV1 + ipre]};
+        const bool inh = (ipre % p.blockSize) >= p.nE;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            float dtsp = tv[k];
+            if (dtsp >= 1.0f) {
+                const float nsp = floorf(dtsp);
+                dtsp = (dtsp - nsp + (float)k) * dt - time2post;
+                if (dtsp < dt && dtsp >= 0.0f && strength > 0.0f) {
+                    const float ddt = dt - dtsp;
+                    const float a = nsp * strength;
+                    if (inh) {
+#pragma unroll
+                        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                            if (ig < p.ngI)
+                                single_input(lgI[ig], lhI[ig], a * p.pI[to * p.ngI + ig], ddt,
+                                             p.riseI[ig], p.decayI[ig], p.dodI[ig]);
+                    } else {
+#pragma unroll
+                        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                            if (ig < p.ngE)
+                                single_input(lgE[ig], lhE[ig], a * p.pE[to * p.ngE + ig], ddt,
+                                             p.riseE[ig], p.decayE[ig], p.dodE[ig]);
+                    }
+                }
+            }
+        }
+    }
+    if (bad && status) atomicOr(status, bad);
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        if (ig < p.ngE) {
+            s.gE[ig * nV1 + n] = s.gE[ig * nV1 + n] + lgE[ig];
+            s.hE[ig * nV1 + n] = s.hE[ig * nV1 + n] + lhE[ig];
+        }
+        if (ig < p.ngI) {
+            s.gI[ig * nV1 + n] = s.gI[ig * nV1 + n] + lgI[ig];
+            s.hI[ig * nV1 + n] = s.hI[ig * nV1 + n] + lhI[ig];
+        }
+    }
+}
+
 template <int kGD_>
 constexpr size_t lds_static() {
     return sizeof(uint64_t) * kMaxEntD + sizeof(uint32_t) * (2 * kMaxEntD + 1) +
@@ -475,6 +554,37 @@ extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_
     else
         hipLaunchKernelGGL(k_rec_delay<8>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
                            *p, *s, *rec, sy, train, trainMask, depth, cur, status);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                             const vc_far_conn *far, const float *train, uint32_t depth,
+                             uint32_t cur, float speed, uint32_t *status) {
+    if (!p || !s || !far || !s->gE || !s->hE || !s->gI || !s->hI) {
+        vc_set_error("vc_v1_rec_far: null argument");
+        return VC_E_INVALID;
+    }
+    if (far->maxFar == 0 || p->nV1 == 0) return VC_OK;
+    if (!far->nFar || !far->pre || !far->w || !far->dist || !train) {
+        vc_set_error("vc_v1_rec_far: null argument (nFar, pre, w, dist and train are required)");
+        return VC_E_INVALID;
+    }
+    if (p->blockSize == 0 || p->nV1 != p->nblock * p->blockSize) {
+        vc_set_error("vc_v1_rec_far: need nV1 = nblock*blockSize");
+        return VC_E_INVALID;
+    }
+    if (depth == 0 || cur >= depth || !(speed > 0.0f) || !(p->dt > 0.0f)) {
+        vc_set_error("vc_v1_rec_far: need cur (%u) < depth (%u), speed > 0 and dt > 0", cur,
+                     depth);
+        return VC_E_INVALID;
+    }
+    if (p->ngE > VC_MAX_NG || p->ngI > VC_MAX_NG || p->nType > VC_MAX_NTYPE) {
+        vc_set_error("vc_v1_rec_far: receptor or type count out of range");
+        return VC_E_INVALID;
+    }
+    hipLaunchKernelGGL(k_rec_far, dim3((p->nV1 + 255u) / 256u), dim3(256), 0,
+                       (hipStream_t)stream, *p, *s, *far, train, depth, cur, speed, status);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

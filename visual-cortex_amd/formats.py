@@ -13,6 +13,8 @@ the reference's formats):
                           pre-post distance in mm of every pair, written by
                           src/preprocess/genCon.cu:511-516, 911)
   neighborBlock<conV1>    src/patch.cu:3481-3512
+  V1_vec<conV1>.bin       src/patch.cu:3308-3352 (far connections: per post
+                          neuron presynaptic ids, strengths, distances)
   LGN_V1_idList / sList   src/util/util.h:161-185, src/patch.h:30-77
   fLGN (old LGN setup)    src/patch.cu:2472-2530
 Outputs:
@@ -217,6 +219,69 @@ def read_delaymat_pairs(path, nblock, blockSize, pairs):
 
 
 # ------------------------------------------------------------------ LGN -> V1
+def write_v1_vec(path, nV1, far=None, connect_long_range=0):
+    """V1_vec: int connectLongRange, nVec[nV1] (u32), a second u32[nV1]
+    when connectLongRange is 1 (skipped by the reader), then for every post
+    neuron with nVec > 0: ids (u32[nVec]), strengths (f32[nVec]) and
+    distances in mm (f32[nVec]).  `far` = (off, pre, w, dist) as
+    Network.far holds them; w is written as given (sRatioV1 applies on
+    reading)."""
+    if far is None:
+        far = (np.zeros(nV1 + 1, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.float32),
+               np.zeros(0, np.float32))
+    off, pre, w, dist = far
+    off = np.asarray(off, np.int64)
+    cnt = np.diff(off).astype("<u4")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", int(connect_long_range)))
+        cnt.tofile(f)
+        if connect_long_range == 1:
+            np.zeros(nV1, "<u4").tofile(f)
+        for n in np.nonzero(cnt)[0]:
+            a, b = off[n], off[n + 1]
+            np.asarray(pre[a:b], "<u4").tofile(f)
+            np.asarray(w[a:b], "<f4").tofile(f)
+            np.asarray(dist[a:b], "<f4").tofile(f)
+
+
+def read_v1_vec(path, nV1, blockSize, nE, sRatioV1=None):
+    """(off u32 [nV1+1], pre u32, w f32, dist f32) from a V1_vec file,
+    strengths times sRatioV1[preType*2 + postType] in float32 as the reference
+    reads them (src/patch.cu:3330-3336; one value applies to all four)."""
+    sr = np.ones(4, np.float32)
+    if sRatioV1 is not None:
+        v = [float(x) for x in sRatioV1]
+        sr = np.asarray(v * 4 if len(v) == 1 else v, np.float32)
+        if sr.size != 4:
+            raise ValueError(f"sRatioV1 has {len(v)} values, expected 1 or 4")
+    with open(path, "rb") as f:
+        hdr = f.read(4)
+        if len(hdr) != 4:
+            raise ValueError(f"{path}: truncated")
+        clr = struct.unpack("<i", hdr)[0]
+        cnt = _u32(f, nV1) if nV1 > 1 else np.atleast_1d(_u32(f, nV1))
+        cnt = np.asarray(cnt, np.int64)
+        if clr == 1:
+            _u32(f, nV1)
+        off = np.zeros(nV1 + 1, np.uint32)
+        off[1:] = np.cumsum(cnt)
+        n = int(off[-1])
+        pre = np.zeros(n, np.uint32)
+        w = np.zeros(n, np.float32)
+        dist = np.zeros(n, np.float32)
+        for i in np.nonzero(cnt)[0]:
+            a, b, c = int(off[i]), int(off[i + 1]), int(cnt[i])
+            pre[a:b] = _u32(f, c) if c > 1 else [_u32(f, 1)]
+            w[a:b] = _f32(f, c)
+            dist[a:b] = _f32(f, c)
+    if n and int(pre.max()) >= nV1:
+        raise ValueError(f"{path}: presynaptic id out of range")
+    post = np.repeat(np.arange(nV1), cnt)
+    k = (pre % blockSize >= nE).astype(np.int64) * 2 + (post % blockSize >= nE).astype(np.int64)
+    w = (w * sr[k]).astype(np.float32)
+    return off, pre, w, dist
+
+
 def write_list_of_lists(path, lists, dtype):
     with open(path, "wb") as f:
         f.write(struct.pack("<I", len(lists)))

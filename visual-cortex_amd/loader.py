@@ -3,7 +3,8 @@ Network in those files (with a .cfg that names them), so that `patch_fast`
 inputs drive this simulator and synthetic workloads can be exported.
 
 Reads (see formats.py for the layouts and reference lines): V1_allpos,
-connectome_cfg, neighborBlock, V1_conMat, V1_delayMat (noDelay = 0),
+connectome_cfg, neighborBlock, V1_conMat, V1_delayMat (noDelay = 0), V1_vec
+(far connections; optional here, the reference requires it),
 LGN_V1_idList / LGN_V1_sList,
 fLGN and stimulus.bin.  The neuron and synapse parameters come from the cfg
 keys (config.spec_from_config); sRatioV1 scales the V1 strengths on reading
@@ -60,9 +61,15 @@ def load_network(cfg: cfgmod.Config) -> Network:
     rec_dist = None
     if spec.noDelay == 0:                    # the reference's own distances
         rec_dist = F.read_delaymat_pairs(cfg.path("fV1_delayMat"), nblock, bs, pairs)
+    far = None
+    vec = Path(cfg.path("fV1_vec"))
+    if vec.exists():
+        far = F.read_v1_vec(vec, nblock * bs, bs, spec.nE, cfg.floats("sRatioV1"))
+        if int(far[0][-1]) == 0:
+            far = None
     return Network(spec=spec, pos=pos, tonicDep=tonic, maxNear=maxNear, nNear=nNear, nbr=nbr,
                    rec_pairs=pairs, nLGN_of=nof, sLGN=sLGN, idLGN=idLGN, lgn=lgn,
-                   frames=frames, init_lms=tuple(st["init_lms"]), rec_dist=rec_dist)
+                   frames=frames, init_lms=tuple(st["init_lms"]), rec_dist=rec_dist, far=far)
 
 
 def scale_v1(pairs, sratio, nE: int):
@@ -101,6 +108,7 @@ def save_network(net: Network, folder, suffix: str = "vc") -> Path:
     F.write_neighbor_block(folder / f"neighborBlock{suf}", net.nNear, lists)
     F.write_conmat(folder / f"V1_conMat{suf}", net)
     F.write_delaymat(folder / f"V1_delayMat{suf}", net)
+    F.write_v1_vec(folder / f"V1_vec{suf}", sp.nV1, getattr(net, "far", None))
     # LGN_V1 lists hold the un-scaled strengths (read_LGN multiplies by sRatioLGN)
     isE = (np.arange(sp.nV1) % sp.blockSize) < sp.nE
     sR = np.where(isE, sp.sRatioLGN[0], sp.sRatioLGN[1]).astype(np.float32)
@@ -117,6 +125,7 @@ def save_network(net: Network, folder, suffix: str = "vc") -> Path:
         f"res_suffix = {suffix}", f"conV1_suffix = {suffix}", f"conLGN_suffix = {suffix}",
         f"output_suffix = {suffix}", "flat_retina = true", f"noDelay = {sp.noDelay}",
         f"speedOfThought = {sp.speedOfThought}", "learning = 0",
+        f"noFarDelay = {sp.noDelay if sp.noFarDelay < 0 else sp.noFarDelay}",
         f"seed = {sp.seed}", f"dt = {sp.dt}", f"nt = {sp.nt}", f"frameRate = {sp.frameRate}",
         f"iModel = {sp.iModel}", f"tau = {sp.tau}", f"nKernelSample = {sp.nKernelSample}",
         f"nSpatialSample1D = {sp.nSpatialSample1D}", f"nsig = {sp.nsig}",

@@ -83,6 +83,14 @@ class ModelSpec:
     # noDelay 0 delays every spike by distance / speedOfThought (mm/ms)
     noDelay: int = 1
     speedOfThought: float = 1.0
+    # far (long-range) connections, the reference's V1_vec lists: delayed by
+    # distance / speedOfThought when noFarDelay is 0; with noFarDelay 1 they
+    # add nothing (src/coredynamics.cu:219-220).  -1: the same as noDelay.
+    noFarDelay: int = -1
+    # synthetic far inputs per post neuron, from blocks that are not its near
+    # neighbours (strength sFar times lognormal)
+    nFar: int = 0
+    sFar: float = 0.05
     # background Poisson drive (extension)
     bgRate: float = 2000.0        # Hz
     bgStrength: tuple = (0.08, 0.12)
@@ -197,6 +205,31 @@ class Network:
     # V1_delayMat file; otherwise they are computed from pos
     rec_dist: list | None = None
     trial_specs: list | None = None
+    # far connections as CSR over post neurons, in the V1_vec file's order:
+    # (off u32 [nV1+1], pre u32, w f32 already * sRatioV1, dist f32 mm)
+    far: tuple | None = None
+
+    def n_far(self) -> int:
+        return 0 if self.far is None else int(self.far[0][-1])
+
+    def far_delayed(self) -> bool:
+        nfd = self.spec.noFarDelay
+        return (self.spec.noDelay if nfd < 0 else nfd) == 0
+
+    def far_ell(self):
+        """(maxFar, nFar u32 [nV1], pre u32, w f32, dist f32 [maxFar, nV1]):
+        the far lists as a transposed ELL, connection j of post n at [j, n]."""
+        off, pre, w, dist = self.far
+        nV1 = self.spec.nV1
+        cnt = np.diff(off.astype(np.int64))
+        mf = max(int(cnt.max()) if cnt.size else 0, 1)
+        P = np.zeros((mf, nV1), np.uint32)
+        W = np.zeros((mf, nV1), np.float32)
+        Dm = np.zeros((mf, nV1), np.float32)
+        post = np.repeat(np.arange(nV1), cnt)
+        j = np.arange(pre.size) - np.repeat(off[:-1].astype(np.int64), cnt)
+        P[j, post], W[j, post], Dm[j, post] = pre, w, dist
+        return mf, cnt.astype(np.uint32), P, W, Dm
 
     # ---- recurrent formats -------------------------------------------------
     def masked_csr_torch(self, device="cuda"):
@@ -521,9 +554,44 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
         idLGN=np.ascontiguousarray(ids.T.astype(np.uint32)),
         lgn=lgn,
     )
+    if spec.nFar > 0:
+        net.far = far_connections(spec, pos, nbr, nNear)
     if with_frames:
         net.frames = grating_frames(spec)
     return net
+
+
+def far_connections(spec: ModelSpec, pos: np.ndarray, nbr, nNear):
+    """Synthetic far lists: spec.nFar presynaptic neurons per post neuron,
+    uniform over the blocks that are not among its block's near neighbours
+    (over every other block when all are near), sorted by id, strength sFar
+    times lognormal; distances as the reference's genCon computes them
+    (float32 of the double coordinate differences, src/preprocess/connect.cu
+    :424-428).  Drawn from a generator of its own, so that the rest of the
+    network does not depend on nFar."""
+    rng = np.random.default_rng(spec.seed + 7919)
+    bs, nb, nV1 = spec.blockSize, spec.nblock, spec.nV1
+    off = np.zeros(nV1 + 1, np.uint32)
+    pres, ws = [], []
+    for b in range(nb):
+        near = set(int(x) for x in nbr[b, :nNear[b]]) | {b}
+        far_b = [j for j in range(nb) if j not in near] or [j for j in range(nb) if j != b]
+        if not far_b:
+            pres.append(np.zeros((bs, 0), np.int64))
+            continue
+        cand = (np.asarray(far_b)[:, None] * bs + np.arange(bs)[None, :]).ravel()
+        k = min(spec.nFar, cand.size)
+        sel = np.argsort(rng.random((bs, cand.size)), axis=1)[:, :k]
+        pres.append(np.sort(cand[sel], axis=1))
+    pre = np.concatenate([p.reshape(-1) for p in pres]).astype(np.int64)
+    cnt = np.concatenate([np.full(bs, p.shape[1]) for p in pres])
+    off[1:] = np.cumsum(cnt)
+    w = (spec.sFar * rng.lognormal(spec.w_mu, spec.w_sigma, pre.size)).astype(np.float32)
+    post = np.repeat(np.arange(nV1), cnt)
+    x = (pos[pre, 0] - pos[post, 0]).astype(np.float32)
+    y = (pos[pre, 1] - pos[post, 1]).astype(np.float32)
+    dist = np.sqrt(x * x + y * y).astype(np.float32)
+    return off, pre.astype(np.uint32), w, dist
 
 
 def lgn_layer(spec: ModelSpec, seed: int | None = None) -> dict:

@@ -90,6 +90,13 @@ class GpuSim:
         # step outputs and spike masks; every step is one vc_v1_step (no
         # undelayed increments) followed by vc_v1_rec_delay
         self.delay = getattr(sp, "noDelay", 1) == 0
+        # far connections: delayed ones read the same ring (vc_v1_rec_far after
+        # vc_v1_rec_delay); with noFarDelay 1 the reference's recal_G_vec adds
+        # nothing (src/coredynamics.cu:219-220), so they are not run
+        self.far_on = net.n_far() > 0 and net.far_delayed()
+        if self.far_on and not self.delay:
+            raise NotImplementedError("delayed far connections with undelayed near ones "
+                                      "(noFarDelay 0, noDelay 1) are not built")
         if self.delay:
             self._setup_delay()
         # ---- spike buffers (double buffered)
@@ -161,6 +168,13 @@ class GpuSim:
                                              self.speed, sp.dt, ptr(self.syn)),
               "vc_v1_rec_delay_table")
         self.depth = train_depth(dmax, self.speed, sp.dt)
+        if self.far_on:
+            mf, cnt, P, W, Dm = net.far_ell()
+            self.far_t = [torch.as_tensor(np.ascontiguousarray(x), device=d)
+                          for x in (cnt, P, W, Dm)]
+            self.far = abi.FarConn(mf, *(ptr(x) for x in self.far_t))
+            fmax = float(net.far[3].max()) if net.far[3].size else 0.0
+            self.depth = max(self.depth, train_depth(fmax, self.speed, sp.dt))
         self.train = torch.zeros(self.depth, sp.nV1, dtype=torch.float32, device=d)
         self.trainMask = torch.zeros(self.depth, sp.nV1 // 64, dtype=torch.int64, device=d)
         self.slot_off = 0            # ring slot of step it is (it + slot_off) % depth
@@ -182,6 +196,10 @@ class GpuSim:
                                        ptr(self.syn), ptr(self.train), ptr(self.trainMask),
                                        self.depth, slot, ptr(self.status)),
               "vc_v1_rec_delay")
+        if self.far_on:
+            check(self.lib.vc_v1_rec_far(h, C.byref(self.p), C.byref(self.s), C.byref(self.far),
+                                         ptr(self.train), self.depth, slot, self.speed,
+                                         ptr(self.status)), "vc_v1_rec_far")
         if sInfo is not None:
             sInfo.copy_(self.train[slot])
         return slot
@@ -540,6 +558,8 @@ class GpuSim:
         torch.cuda.synchronize()
         if int(self.status.item()) != 0:
             if self.delay:
+                if int(self.status.item()) & 2:
+                    raise abi.VCError("a far connection's presynaptic id is out of range")
                 raise abi.VCError("a synapse's delay does not fit the spike-train ring")
             raise abi.VCError("persistent V1 launch timed out waiting for neighbour blocks")
 
