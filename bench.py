@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--trials", type=int, default=32, help="c3: number of trials")
     ap.add_argument("--delay", type=float, default=0.0,
                     help="c1/c2 with axonal delays (noDelay 0) at this speedOfThought, mm/ms")
+    ap.add_argument("--far", type=int, default=0,
+                    help="with --delay: this many far (long-range) inputs per neuron")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (x4) for HBM counter calibration")
     args = ap.parse_args()
@@ -103,7 +105,10 @@ def main():
         return bench_v1_adjoint(args)
     spec = network.spec_c2() if args.config == "c2" else network.spec_c1()
     if args.delay > 0:
-        spec = network.replace(spec, noDelay=0, speedOfThought=args.delay)
+        spec = network.replace(spec, noDelay=0, speedOfThought=args.delay, nFar=args.far)
+    elif args.far:
+        print("--far needs --delay (far connections are delayed by distance)", file=sys.stderr)
+        sys.exit(2)
     K, W = args.steps, args.warmup
     spec = network.replace(spec, nt=K + W) if hasattr(network, "replace") else spec
     t0 = time.time()
@@ -164,7 +169,7 @@ def main():
 
     kernel = "k_v1_multi" if sim.persistent else "k_v1_step"
     if sim.delay:
-        kernel = "k_v1_step+k_rec_delay"
+        kernel = "k_v1_step+k_rec_delay" + ("+k_rec_far" if sim.far_on else "")
     line = {
         "metric": "V1 neuron-steps/s at 64 blocks dt=0.125ms",
         "value": value,
@@ -183,7 +188,9 @@ def main():
                                f"m={spec.m}, K={spec.KE}+{spec.KI}, {spec.width}x{spec.height} px, "
                                f"dt={spec.dt} ms" + (f", axonal delays at {spec.speedOfThought} "
                                                       f"mm/ms (ring of {sim.depth} steps)"
-                                                      if sim.delay else ""),
+                                                      if sim.delay else "")
+                               + (f", {spec.nFar} far inputs per neuron" if sim.delay and
+                                  sim.far_on else ""),
                    "nV1": spec.nV1, "nLGN": spec.nLGN, "dt_ms": spec.dt,
                    "spikes_per_neuron_step": rho},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -281,6 +281,8 @@ typedef struct {
     const uint32_t *pre;        /* [maxFar][nV1] global presynaptic id          */
     const float *w;             /* [maxFar][nV1] strength, already * sRatioV1   */
     const float *dist;          /* [maxFar][nV1] distance, mm                   */
+    const uint64_t *preMask;    /* [nV1/64] bit n%64 of word n/64 set when n is
+                                   the presynaptic neuron of a far connection  */
 } vc_far_conn;
 
 /* vc_v1_rec_far: adds to s->gE/hE/gI/hI the far connections' input arriving
@@ -290,12 +292,15 @@ typedef struct {
  * (:2314-2349).  Call it after vc_v1_rec_delay of the same step, as the
  * reference adds the far sums after recal_G_mat.  Per connection time2post =
  * dist/speed, it2post = ceil(time2post/dt); depth must exceed every it2post
- * (the reference's fTrainDepth = it2post + 1, src/patch.cu:3847).  *status
- * (may be NULL) |= 1 for a delay that does not fit, |= 2 for a presynaptic
- * id >= nV1; such connections are skipped. */
+ * (the reference's fTrainDepth = it2post + 1, src/patch.cu:3847).  As the
+ * reference, it adds nothing in a step in which no far presynaptic neuron
+ * (far->preMask) has train[cur][n] > 1.0 (src/patch.cu:6597-6598, src/stats.cu
+ * :114-126); trainMask[depth][nV1/64] holds each step's spike masks, as for
+ * vc_v1_rec_delay.  *status (may be NULL) |= 1 for a delay that does not
+ * fit, |= 2 for a presynaptic id >= nV1; such connections are skipped. */
 int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_state *s,
-                  const vc_far_conn *far, const float *train, uint32_t depth,
-                  uint32_t cur, float speed, uint32_t *status);
+                  const vc_far_conn *far, const float *train, const uint64_t *trainMask,
+                  uint32_t depth, uint32_t cur, float speed, uint32_t *status);
 
 /* ---- reverse mode of the V1 step (no counterpart in the reference; the
  * forward it differentiates is vc_v1_step, i.e. compute_V_collect_spike_

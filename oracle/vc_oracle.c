@@ -539,8 +539,14 @@ int orc_v1_rec_far(const vc_v1_params *p, float *gE, float *hE, float *gI,
                    const float *train, uint32_t depth, uint32_t cur) {
     const uint32_t nV1 = p->nV1, bs = p->blockSize;
     const float dt = p->dt;
-    int bad = 0;
+    int bad = 0, farSpiked = 0;
     if (depth == 0 || cur >= depth) return VC_E_INVALID;
+    /* fill_fSpikeTrain's test (src/stats.cu:114-126): recal_G_vec and sum_G
+     * run only in a step in which some far presynaptic neuron's output is
+     * above 1.0 (src/patch.cu:6597-6598) */
+    for (uint32_t j = 0; j < fOff[nV1] && !farSpiked; ++j)
+        if (fPre[j] < nV1 && train[(uint64_t)nV1 * cur + fPre[j]] > 1.0f) farSpiked = 1;
+    if (!farSpiked) return VC_OK;
 #pragma omp parallel for schedule(static) reduction(|:bad)
     for (uint32_t n = 0; n < nV1; ++n) {
         if (fOff[n + 1] == fOff[n]) continue;          /* sum_G: nVec[id] > 0 */
@@ -553,7 +559,7 @@ int orc_v1_rec_far(const vc_v1_params *p, float *gE, float *hE, float *gI,
             float time2post = fDist[j] / speed;
             uint32_t it2post = (uint32_t)ceilf(time2post / dt);
             time2post = (float)it2post * dt - time2post;
-            if (it2post >= depth) { bad = 1; continue; }
+            if (ipre >= nV1 || it2post >= depth) { bad = 1; continue; }
             int k0 = (int)cur - (int)it2post;
             if (k0 < 0) k0 += (int)depth;
             for (uint32_t k = 0; k < 2; ++k) {

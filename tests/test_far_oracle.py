@@ -4,9 +4,11 @@ src/coredynamics.cu:181-372, 2314-2349): every spike of a far presynaptic
 neuron reaches its post neuron d/speed after it was fired and, if it lands in
 the current step, adds nsp*w*p of the conductance shape over the rest of the
 step -- summed here in float64 from absolute arrival times, not from the
-two-slot arithmetic; the ring's length does not matter; and far lists that
-hold exactly the near synapses, in recal_G_mat's order, give recal_G_mat's
-sums bit for bit.  Also the V1_vec file format and the synthetic lists."""
+two-slot arithmetic; the ring's length does not matter; far lists that hold
+exactly the near synapses, in recal_G_mat's order, give recal_G_mat's sums
+bit for bit; and nothing is added in a step in which no far presynaptic
+neuron's output is above 1.0 (src/patch.cu:6597-6598).  Also the synthetic
+lists."""
 import ctypes as C
 
 import numpy as np
@@ -187,3 +189,35 @@ def test_synthetic_far_lists():
     from dataclasses import replace
     net.spec = replace(sp, noFarDelay=1)
     assert not net.far_delayed()
+
+
+def test_far_gate_on_current_far_spikes(pkg):
+    """The reference runs recal_G_vec only when a far presynaptic neuron's
+    output in the current step is above 1.0 (fill_fSpikeTrain,
+    src/stats.cu:114-126): with every far presynaptic output of the step
+    below or at 1.0 nothing is added, however many earlier spikes arrive;
+    one such output above 1.0 gives the full arrival-time sum."""
+    net = _far_net(0.5, nFar=4, nblock=9)
+    sp = net.spec
+    depth = far_depth(net)
+    isfar = np.zeros(sp.nV1, bool)
+    isfar[net.far[1].astype(np.int64)] = True
+    assert 0 < isfar.sum() < sp.nV1
+    rng = np.random.default_rng(9)
+    cur = depth // 2
+    train = random_ring(rng, depth, sp.nV1, p_spike=0.2)
+    train[cur] = np.where(isfar, np.float32(1.0), np.float32(1.6))
+    sim = osim(pkg, net)
+    st = zero_state(net)
+    assert far_call(sim, st, net.far, train, depth, cur, 0.5) == 0
+    assert all(float(np.abs(st[k]).max()) == 0.0 for k in ("gE", "hE", "gI", "hI"))
+    ref = far_event_sum(net, net.far, train, depth, cur, 0.5)
+    assert ref["gE"].max() > 0                       # arrivals that the gate drops
+    train[cur][np.nonzero(isfar)[0][0]] = 1.5
+    st = zero_state(net)
+    assert far_call(sim, st, net.far, train, depth, cur, 0.5) == 0
+    ref = far_event_sum(net, net.far, train, depth, cur, 0.5)
+    for k in ("gE", "hE", "gI", "hI"):
+        n = ref[k].shape[0]
+        np.testing.assert_allclose(st[k][:n].astype(np.float64), ref[k], rtol=3e-5,
+                                   atol=1e-7 * max(1.0, ref[k].max()))

@@ -32,6 +32,12 @@ def _net(**kw):
     return network.build_network(network.spec_tiny(**base))
 
 
+def ring_masks(train):
+    D, n = train.shape
+    return np.packbits((train >= 1.0).reshape(D, n // 64, 64), axis=-1,
+                       bitorder="little").reshape(D, -1).view(np.int64)
+
+
 def _state_struct(g, dev):
     from visual_cortex_amd import abi
     from visual_cortex_amd.abi import ptr
@@ -63,9 +69,10 @@ def test_rec_far_kernel_bitwise(pkg, speed):
         dev = {k: torch.as_tensor(v, device="cuda") for k, v in st.items()}
         s = _state_struct(g, dev)
         tr = torch.as_tensor(train, device="cuda")
+        tm = torch.as_tensor(ring_masks(train), device="cuda")
         status = torch.zeros(1, dtype=torch.int32, device="cuda")
-        check(g.lib.vc_v1_rec_far(None, C.byref(g.p), C.byref(s), C.byref(g.far), ptr(tr), D,
-                                  cur, speed, ptr(status)), "vc_v1_rec_far")
+        check(g.lib.vc_v1_rec_far(None, C.byref(g.p), C.byref(s), C.byref(g.far), ptr(tr),
+                                  ptr(tm), D, cur, speed, ptr(status)), "vc_v1_rec_far")
         torch.cuda.synchronize()
         assert int(status.item()) == 0
         hits = 0
@@ -90,9 +97,10 @@ def test_rec_far_status_flags(pkg):
     dev = {k: torch.as_tensor(v, device="cuda") for k, v in st.items()}
     s = _state_struct(g, dev)
     tr = torch.full((1, sp.nV1), 1.5, device="cuda")
+    tm = torch.full((1, sp.nV1 // 64), -1, dtype=torch.int64, device="cuda")
     status = torch.zeros(1, dtype=torch.int32, device="cuda")
-    check(g.lib.vc_v1_rec_far(None, C.byref(g.p), C.byref(s), C.byref(g.far), ptr(tr), 1, 0,
-                              0.25, ptr(status)), "vc_v1_rec_far")
+    check(g.lib.vc_v1_rec_far(None, C.byref(g.p), C.byref(s), C.byref(g.far), ptr(tr), ptr(tm),
+                              1, 0, 0.25, ptr(status)), "vc_v1_rec_far")
     torch.cuda.synchronize()
     assert int(status.item()) == 1
     assert float(dev["gE"].abs().max()) == 0.0 and float(dev["gI"].abs().max()) == 0.0
@@ -102,13 +110,58 @@ def test_rec_far_status_flags(pkg):
     pre = torch.full((1, sp.nV1), sp.nV1 + 5, dtype=torch.int32, device="cuda")
     w = torch.ones(1, sp.nV1, device="cuda")
     dist = torch.zeros(1, sp.nV1, device="cuda")
-    far = abi.FarConn(1, ptr(cnt), ptr(pre), ptr(w), ptr(dist))
+    allpre = torch.full((sp.nV1 // 64,), -1, dtype=torch.int64, device="cuda")
+    far = abi.FarConn(1, ptr(cnt), ptr(pre), ptr(w), ptr(dist), ptr(allpre))
     status.zero_()
-    check(g.lib.vc_v1_rec_far(None, C.byref(g.p), C.byref(s), C.byref(far), ptr(tr), 1, 0,
-                              0.25, ptr(status)), "vc_v1_rec_far")
+    check(g.lib.vc_v1_rec_far(None, C.byref(g.p), C.byref(s), C.byref(far), ptr(tr), ptr(tm),
+                              1, 0, 0.25, ptr(status)), "vc_v1_rec_far")
     torch.cuda.synchronize()
     assert int(status.item()) == 2
     assert float(dev["gE"].abs().max()) == 0.0
+
+
+def test_rec_far_gate(pkg):
+    """No far presynaptic neuron above 1.0 in the current slot (outputs of
+    exactly 1.0, or spikes of other neurons only): nothing is added, though
+    earlier spikes arrive; one far presynaptic spike opens it again -- as
+    the oracle."""
+    import oracle
+    from visual_cortex_amd.abi import check, ptr
+    from visual_cortex_amd.sim import GpuSim
+    _flush("test_rec_far_gate")
+    net = _net(nFar=4, nblock=9)
+    sp = net.spec
+    g = GpuSim(net)
+    o = oracle.OracleSim(net, pkg, tables=_tables(g))
+    D = g.depth
+    isfar = np.unpackbits(net.far_pre_mask().view(np.uint8), bitorder="little").astype(bool)
+    assert 0 < isfar.sum() < sp.nV1
+    rng = np.random.default_rng(77)
+    cur = D // 2
+    base = random_ring(rng, D, sp.nV1, p_spike=0.2)
+    cases = []
+    t1 = base.copy()
+    t1[cur] = np.where(isfar, 1.0, t1[cur]).astype(np.float32)      # exactly 1.0: closed
+    t1[cur][~isfar] = 1.7                                           # non-far spikes only
+    cases.append((t1, False))
+    t2 = t1.copy()
+    t2[cur][np.nonzero(isfar)[0][3]] = 1.25                         # one far spike: open
+    cases.append((t2, True))
+    for train, open_ in cases:
+        st = zero_state(net)
+        ref = {k: v.copy() for k, v in st.items()}
+        assert far_call(o, ref, net.far, train, D, cur, sp.speedOfThought) == 0
+        dev = {k: torch.as_tensor(v, device="cuda") for k, v in st.items()}
+        s = _state_struct(g, dev)
+        tr = torch.as_tensor(train, device="cuda")
+        tm = torch.as_tensor(ring_masks(train), device="cuda")
+        check(g.lib.vc_v1_rec_far(None, C.byref(g.p), C.byref(s), C.byref(g.far), ptr(tr),
+                                  ptr(tm), D, cur, sp.speedOfThought, None), "vc_v1_rec_far")
+        torch.cuda.synchronize()
+        for k in ("gE", "hE", "gI", "hI"):
+            a = dev[k].cpu().numpy()
+            assert np.array_equal(a.view(np.uint32), ref[k].view(np.uint32)), (open_, k)
+        assert (float(np.abs(ref["gE"]).max()) > 0) == open_
 
 
 FCASES = {

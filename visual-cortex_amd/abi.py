@@ -75,7 +75,8 @@ class RecConn(C.Structure):
 
 
 class FarConn(C.Structure):
-    _fields_ = [("maxFar", u32), ("nFar", vp), ("pre", vp), ("w", vp), ("dist", vp)]
+    _fields_ = [("maxFar", u32), ("nFar", vp), ("pre", vp), ("w", vp), ("dist", vp),
+                ("preMask", vp)]
 
 
 class Spikes(C.Structure):
@@ -136,7 +137,7 @@ SIGNATURES = {
     "vc_v1_rec_delay": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn), vp,
                         vp, vp, u32, u32, vp],
     "vc_v1_rec_far": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FarConn), vp,
-                      u32, u32, C.c_float, vp],
+                      vp, u32, u32, C.c_float, vp],
     "vc_v1_step_snapshot": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
                             C.POINTER(RecConn), C.POINTER(Spikes), C.POINTER(Spikes), vp, vp,
                             u32, vp, vp],

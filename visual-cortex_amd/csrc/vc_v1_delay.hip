@@ -379,15 +379,31 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
 // neuron n at j*nV1 + n, so a wavefront's record loads are coalesced) and
 // the two ring slots of each connection are gathered from the shared ring,
 // which holds the same steps' outputs as the reference's per-connection
-// rings.  status |= 1 for a delay longer than the ring, |= 2 for a
-// presynaptic id out of range (both skipped).
+// rings.  As in the reference (src/patch.cu:6597-6598, src/stats.cu:114-126)
+// nothing is added in a step in which no far presynaptic neuron has an
+// output above 1.0: every workgroup tests that itself from the step's spike
+// masks (fc.preMask & trainMask[cur], then the values of the bits set).
+// status |= 1 for a delay longer than the ring, |= 2 for a presynaptic id
+// out of range (both skipped).
 __global__ __launch_bounds__(256) void k_rec_far(vc_v1_params p, vc_v1_state s,
                                                  vc_far_conn fc,
                                                  const float *__restrict__ train,
+                                                 const uint64_t *__restrict__ trainMask,
                                                  uint32_t depth, uint32_t cur, float speed,
                                                  uint32_t *__restrict__ status) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nV1 = p.nV1;
+    const uint32_t nTiles = nV1 / 64u;
+    int fired = 0;
+    for (uint32_t t = threadIdx.x; t < nTiles && !fired; t += blockDim.x) {
+        uint64_t m = trainMask[(size_t)cur * nTiles + t] & fc.preMask[t];
+        while (m) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1ull;
+            if (train[(size_t)cur * nV1 + t * 64u + l] > 1.0f) { fired = 1; break; }
+        }
+    }
+    if (!__syncthreads_or(fired)) return;
     if (n >= nV1) return;
     const uint32_t cnt = fc.nFar[n];
     if (cnt == 0u) return;                        // sum_G: nVec[id] > 0 only
@@ -559,19 +575,22 @@ extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_
 }
 
 extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_state *s,
-                             const vc_far_conn *far, const float *train, uint32_t depth,
-                             uint32_t cur, float speed, uint32_t *status) {
+                             const vc_far_conn *far, const float *train,
+                             const uint64_t *trainMask, uint32_t depth, uint32_t cur,
+                             float speed, uint32_t *status) {
     if (!p || !s || !far || !s->gE || !s->hE || !s->gI || !s->hI) {
         vc_set_error("vc_v1_rec_far: null argument");
         return VC_E_INVALID;
     }
     if (far->maxFar == 0 || p->nV1 == 0) return VC_OK;
-    if (!far->nFar || !far->pre || !far->w || !far->dist || !train) {
-        vc_set_error("vc_v1_rec_far: null argument (nFar, pre, w, dist and train are required)");
+    if (!far->nFar || !far->pre || !far->w || !far->dist || !far->preMask || !train ||
+        !trainMask) {
+        vc_set_error("vc_v1_rec_far: null argument (nFar, pre, w, dist, preMask, train and "
+                     "trainMask are required)");
         return VC_E_INVALID;
     }
-    if (p->blockSize == 0 || p->nV1 != p->nblock * p->blockSize) {
-        vc_set_error("vc_v1_rec_far: need nV1 = nblock*blockSize");
+    if (p->blockSize == 0 || p->blockSize % 64 != 0 || p->nV1 != p->nblock * p->blockSize) {
+        vc_set_error("vc_v1_rec_far: need blockSize a multiple of 64 and nV1 = nblock*blockSize");
         return VC_E_INVALID;
     }
     if (depth == 0 || cur >= depth || !(speed > 0.0f) || !(p->dt > 0.0f)) {
@@ -584,7 +603,8 @@ extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_st
         return VC_E_INVALID;
     }
     hipLaunchKernelGGL(k_rec_far, dim3((p->nV1 + 255u) / 256u), dim3(256), 0,
-                       (hipStream_t)stream, *p, *s, *far, train, depth, cur, speed, status);
+                       (hipStream_t)stream, *p, *s, *far, train, trainMask, depth, cur, speed,
+                       status);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -231,6 +231,14 @@ class Network:
         P[j, post], W[j, post], Dm[j, post] = pre, w, dist
         return mf, cnt.astype(np.uint32), P, W, Dm
 
+    def far_pre_mask(self) -> np.ndarray:
+        """int64 [nV1/64]: bit n%64 of word n/64 set when neuron n is the
+        presynaptic neuron of a far connection."""
+        nV1 = self.spec.nV1
+        b = np.zeros(nV1, bool)
+        b[self.far[1][self.far[1] < nV1].astype(np.int64)] = True
+        return np.packbits(b.reshape(-1, 64), axis=-1, bitorder="little").reshape(-1).view(np.int64)
+
     # ---- recurrent formats -------------------------------------------------
     def masked_csr_torch(self, device="cuda"):
         """The masked CSR as torch tensors (int64 rowMask bits, int64 rowOff,
@@ -581,7 +589,17 @@ def far_connections(spec: ModelSpec, pos: np.ndarray, nbr, nNear):
             continue
         cand = (np.asarray(far_b)[:, None] * bs + np.arange(bs)[None, :]).ravel()
         k = min(spec.nFar, cand.size)
-        sel = np.argsort(rng.random((bs, cand.size)), axis=1)[:, :k]
+        if 4 * k <= cand.size:
+            # k distinct of many: redraw the rows that repeat an index
+            sel = rng.integers(0, cand.size, (bs, k))
+            while k > 1:
+                srt = np.sort(sel, axis=1)
+                dup = (srt[:, 1:] == srt[:, :-1]).any(axis=1)
+                if not dup.any():
+                    break
+                sel[dup] = rng.integers(0, cand.size, (int(dup.sum()), k))
+        else:
+            sel = np.argsort(rng.random((bs, cand.size)), axis=1)[:, :k]
         pres.append(np.sort(cand[sel], axis=1))
     pre = np.concatenate([p.reshape(-1) for p in pres]).astype(np.int64)
     cnt = np.concatenate([np.full(bs, p.shape[1]) for p in pres])

@@ -171,7 +171,7 @@ class GpuSim:
         if self.far_on:
             mf, cnt, P, W, Dm = net.far_ell()
             self.far_t = [torch.as_tensor(np.ascontiguousarray(x), device=d)
-                          for x in (cnt, P, W, Dm)]
+                          for x in (cnt, P, W, Dm, net.far_pre_mask())]
             self.far = abi.FarConn(mf, *(ptr(x) for x in self.far_t))
             fmax = float(net.far[3].max()) if net.far[3].size else 0.0
             self.depth = max(self.depth, train_depth(fmax, self.speed, sp.dt))
@@ -198,8 +198,8 @@ class GpuSim:
               "vc_v1_rec_delay")
         if self.far_on:
             check(self.lib.vc_v1_rec_far(h, C.byref(self.p), C.byref(self.s), C.byref(self.far),
-                                         ptr(self.train), self.depth, slot, self.speed,
-                                         ptr(self.status)), "vc_v1_rec_far")
+                                         ptr(self.train), ptr(self.trainMask), self.depth,
+                                         slot, self.speed, ptr(self.status)), "vc_v1_rec_far")
         if sInfo is not None:
             sInfo.copy_(self.train[slot])
         return slot
