@@ -259,17 +259,20 @@ int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *w, const float 
  *   trainMask[depth][nV1/64] each step's spike masks (vc_spikes.mask);
  *   syn                      vc_v1_rec_delay_table's records (rec->w itself
  *                            is not read).
- * depth must exceed the largest it2post, i.e. be at least the reference's
- * trainDepth = ceil(maxDistance/speed/dt) + 1 (src/patch.cu:3813); a synapse
- * whose delay does not fit is skipped and sets *status (may be NULL) to 1.
- * Use it with vc_v1_step given an empty `prev` (no undelayed increments):
- * step t = vc_v1_step writing slot t % depth, then vc_v1_rec_delay(cur =
- * t % depth), as the reference runs compute_V then recal_G_mat.  Masked CSR
- * only (rec->wDense is ignored); maxNear * blockSize/64 <= 512. */
+ * Only the last `window` slots (ages 0 .. window-1; 0 = depth, at most
+ * depth) are scanned for spikes: window must exceed the largest it2post, i.e.
+ * be at least the reference's trainDepth = ceil(maxDistance/speed/dt) + 1
+ * (src/patch.cu:3813); the ring itself may be longer (the far connections'
+ * delays, vc_v1_rec_far).  A synapse whose delay does not fit is skipped and
+ * sets *status (may be NULL) to 1.  Use it with vc_v1_step given an empty
+ * `prev` (no undelayed increments): step t = vc_v1_step writing slot t %
+ * depth, then vc_v1_rec_delay(cur = t % depth), as the reference runs
+ * compute_V then recal_G_mat.  Masked CSR only (rec->wDense is ignored);
+ * maxNear * blockSize/64 <= 512. */
 int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                     const vc_rec_conn *rec, const uint32_t *syn,
                     const float *train, const uint64_t *trainMask, uint32_t depth,
-                    uint32_t cur, uint32_t *status);
+                    uint32_t cur, uint32_t window, uint32_t *status);
 
 /* Far (long-range) connections, the reference's V1_vec lists (src/patch.cu
  * :3308-3352: per post neuron, presynaptic ids, strengths * sRatioV1 and

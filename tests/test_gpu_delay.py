@@ -82,7 +82,7 @@ def test_rec_delay_kernel_bitwise(pkg, speed):
         tm = torch.as_tensor(bits, device="cuda")
         status = torch.zeros(1, dtype=torch.int32, device="cuda")
         check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec),
-                                    ptr(g.syn), ptr(tr), ptr(tm), D, cur, ptr(status)),
+                                    ptr(g.syn), ptr(tr), ptr(tm), D, cur, 0, ptr(status)),
               "vc_v1_rec_delay")
         torch.cuda.synchronize()
         assert int(status.item()) == 0
@@ -101,7 +101,7 @@ def test_rec_delay_kernel_bitwise(pkg, speed):
     tm = torch.full((1, sp.nV1 // 64), -1, dtype=torch.int64, device="cuda")
     status = torch.zeros(1, dtype=torch.int32, device="cuda")
     check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec),
-                                ptr(g.syn), ptr(tr), ptr(tm), 1, 0, ptr(status)),
+                                ptr(g.syn), ptr(tr), ptr(tm), 1, 0, 0, ptr(status)),
           "vc_v1_rec_delay")
     torch.cuda.synchronize()
     assert int(status.item()) == 1
@@ -215,3 +215,54 @@ def test_batched_delayed_trials_equal_serial(pkg):
         assert np.array_equal(sb[:, off:off + n.spec.nV1].view(np.uint32),
                               gs.sInfo().view(np.uint32)), t
         off += n.spec.nV1
+
+
+@pytest.mark.parametrize("extra", [5, 40])       # window staged in LDS, ring longer than 32
+def test_rec_delay_window_of_a_longer_ring(pkg, extra):
+    """A ring longer than the near delays need (as far connections make it),
+    scanned over the near window only: the same sums as the oracle's dense
+    scan of the whole ring; a window shorter than the longest delay is
+    flagged."""
+    import oracle
+    from visual_cortex_amd import abi
+    from visual_cortex_amd.abi import check, ptr
+    from visual_cortex_amd.sim import GpuSim
+    _flush(f"test_rec_delay_window_of_a_longer_ring: window + {extra} slots")
+    speed = 0.5
+    net = _net(speedOfThought=speed)
+    sp = net.spec
+    g = GpuSim(net)
+    o = oracle.OracleSim(net, pkg, tables=_tables(g))
+    W = g.window
+    assert W == g.depth and W <= 32
+    D = W + extra
+    rng = np.random.default_rng(extra)
+    for cur in (0, W - 1, D - 1):
+        train = random_ring(rng, D, sp.nV1, p_spike=0.1)
+        bits = np.packbits((train >= 1.0).reshape(D, sp.nV1 // 64, 64), axis=-1,
+                           bitorder="little").reshape(D, -1).view(np.int64)
+        st = zero_state(net)
+        for k in ("gE", "hE", "gI", "hI"):
+            st[k][:] = rng.random(st[k].shape).astype(np.float32)
+        ref = {k: v.copy() for k, v in st.items()}
+        assert delay_call(o, net, ref, train, D, cur, speed) == 0
+        for win in (W, 0):                       # the window, and the whole ring
+            dev = {k: torch.as_tensor(v, device="cuda") for k, v in st.items()}
+            s = abi.V1State(*(ptr(dev[k]) for k in ("v", "w", "tBack", "gFF", "hFF", "gE",
+                                                     "hE", "gI", "hI")), ptr(g.tonic))
+            tr = torch.as_tensor(train, device="cuda")
+            tm = torch.as_tensor(bits, device="cuda")
+            status = torch.zeros(1, dtype=torch.int32, device="cuda")
+            check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec),
+                                        ptr(g.syn), ptr(tr), ptr(tm), D, cur, win, ptr(status)),
+                  "vc_v1_rec_delay")
+            torch.cuda.synchronize()
+            assert int(status.item()) == 0
+            for k in ("gE", "hE", "gI", "hI"):
+                a = dev[k].cpu().numpy()
+                assert np.array_equal(a.view(np.uint32), ref[k].view(np.uint32)), (cur, win, k)
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    check(g.lib.vc_v1_rec_delay(None, C.byref(g.p), C.byref(s), C.byref(g.rec), ptr(g.syn),
+                                ptr(tr), ptr(tm), D, 0, W - 1, ptr(status)), "vc_v1_rec_delay")
+    torch.cuda.synchronize()
+    assert int(status.item()) == 1

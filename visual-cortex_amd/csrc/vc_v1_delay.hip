@@ -73,14 +73,15 @@ __global__ __launch_bounds__(256) void k_rec_delay_table(uint64_t n, const float
 
 // One post neuron per thread, workgroups (up to 256 threads) inside one V1
 // block, so every wavefront of a workgroup has the same neighbour tiles.
-//   1. the spike masks of all `depth` ring slots of every neighbour tile are
-//      OR-ed into su[] (the window's spikers);
+//   1. the spike masks of the ring's last `win` slots (the window the near
+//      delays reach; the ring can be longer for the far connections) of every
+//      neighbour tile are OR-ed into su[] (the window's spikers);
 //   2. the window's spikers are listed in visiting order (neighbour block,
 //      ascending presynaptic id) in items[] (a scan of su[]'s popcounts);
 //   3. each wavefront walks the list 64 spikers at a time: lane k loads
-//      spiker k's row-segment header and, for rings of up to kMaxDepthLds
-//      slots, its `depth` ring values into LDS together with a bit mask of
-//      the slots holding a spike (one round trip); then the strengths and
+//      spiker k's row-segment header and, for windows of up to kMaxDepthLds
+//      slots, its `win` ring values into LDS, by age, together with a bit mask
+//      of the ages holding a spike (one round trip); then the strengths and
 //      delays of kGD spikers at a time are loaded (one round trip) and
 //      accumulated in order, a lane reading ring values only when one of
 //      its two slots holds a spike.
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
                                                    const uint4 *__restrict__ syn,
                                                    const float *__restrict__ train,
                                                    const uint64_t *__restrict__ tmask,
-                                                   uint32_t depth, uint32_t cur,
+                                                   uint32_t depth, uint32_t cur, uint32_t win,
                                                    uint32_t *__restrict__ status) {
     __shared__ uint64_t su[kMaxEntD];
     __shared__ uint32_t soff[kMaxEntD + 1];
@@ -120,8 +121,9 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
     for (uint32_t ib = threadIdx.x; ib < nb; ib += blockDim.x)
         sbid[ib] = rec.nbr[b * rec.maxNear + ib];
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < nEnt * depth; q += blockDim.x) {
-        const uint32_t k = q / nEnt, e = q - k * nEnt;
+    for (uint32_t q = threadIdx.x; q < nEnt * win; q += blockDim.x) {
+        const uint32_t a = q / nEnt, e = q - a * nEnt;
+        const uint32_t k = cur >= a ? cur - a : cur + depth - a;
         const uint32_t ib = e / nSeg, tt = e - ib * nSeg;
         const uint64_t m = tmask[(size_t)k * nTiles + sbid[ib] * nSeg + tt];
         if (m) atomicOr((unsigned long long *)&su[e], (unsigned long long)m);
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
     float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
     float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
     bool late = false;
-    const bool staged = depth <= (uint32_t)kMaxDepthLds;
+    const bool staged = win <= (uint32_t)kMaxDepthLds;
     float *sv = stv[wv];
 
     // ---- 3. the spikers, 64 at a time (every wavefront of the workgroup
@@ -185,20 +187,22 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
             gpre = sbid[ib] * bs + lpre;
             slp[wv][lane] = lpre;
             sgp[wv][lane] = gpre;
-            if (staged) {
+            if (staged) {                              // by age a = 0 .. win-1
                 am = 0u;
-                for (uint32_t k0 = 0; k0 < depth; k0 += 8u) {
+                for (uint32_t a0 = 0; a0 < win; a0 += 8u) {
                     float x[8];
 #pragma unroll
-                    for (uint32_t i = 0; i < 8u; ++i)
-                        x[i] = k0 + i < depth ? train[(size_t)(k0 + i) * nV1 + gpre] : 0.0f;
+                    for (uint32_t i = 0; i < 8u; ++i) {
+                        const uint32_t a = a0 + i;
+                        const uint32_t k = cur >= a ? cur - a : cur + depth - a;
+                        x[i] = a < win ? train[(size_t)k * nV1 + gpre] : 0.0f;
+                    }
 #pragma unroll
                     for (uint32_t i = 0; i < 8u; ++i) {
-                        const uint32_t k = k0 + i;
-                        if (k < depth) {
-                            sv[lane * kStv + k] = x[i];
-                            const uint32_t age = cur >= k ? cur - k : cur + depth - k;
-                            am |= (x[i] >= 1.0f ? 1u : 0u) << age;
+                        const uint32_t a = a0 + i;
+                        if (a < win) {
+                            sv[lane * kStv + a] = x[i];
+                            am |= (x[i] >= 1.0f ? 1u : 0u) << a;
                         }
                     }
                 }
@@ -237,14 +241,14 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
             for (int g = 0; g < kGD_; ++g) {
                 if (!((mine >> (g0 + g)) & 1ull)) continue;
                 const uint32_t it2post = i2p[g0 + g];
-                if (it2post >= depth) {               // ring too short for this delay
+                if (it2post >= win) {                 // window too short for this delay
                     late = true;
                     continue;
                 }
                 if (staged) {   // slot j = 0 is it2post steps old, j = 1 one step newer
+                    // (for it2post = 0, slot j = 1 is never accepted: its dtsp is >= dt)
                     const uint32_t a = vc_readlane_u32(am, g0 + g);
-                    const uint32_t need = it2post ? (a >> (it2post - 1u)) & 3u
-                                                  : (a & 1u) | ((a >> (depth - 1u)) & 1u);
+                    const uint32_t need = it2post ? (a >> (it2post - 1u)) & 3u : a & 1u;
                     if (!need) continue;
                 }
                 act |= 1u << g;
@@ -278,9 +282,9 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
                     const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
                     const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
                     float tv0, tv1;
-                    if (staged) {
-                        tv0 = sv[kk * kStv + j0];
-                        tv1 = sv[kk * kStv + j1];
+                    if (staged) {                      // by age: it2post, it2post - 1
+                        tv0 = sv[kk * kStv + it2post];
+                        tv1 = it2post ? sv[kk * kStv + it2post - 1u] : 0.0f;
                     } else {
                         tv0 = train[(size_t)j0 * nV1 + sgp[wv][kk]];
                         tv1 = train[(size_t)j1 * nV1 + sgp[wv][kk]];
@@ -377,8 +381,9 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
 // (:2314-2349) adds to gE/hE/gI/hI after recal_G_mat.  One thread per post
 // neuron; its list is read in order from the transposed ELL (connection j of
 // neuron n at j*nV1 + n, so a wavefront's record loads are coalesced) and
-// the two ring slots of each connection are gathered from the shared ring,
-// which holds the same steps' outputs as the reference's per-connection
+// the two ring slots of each connection are looked up in the shared ring's
+// spike masks (8 bytes per 64 neurons and step, mostly cache hits); the
+// ring holds the same steps' outputs as the reference's per-connection
 // rings.  As in the reference (src/patch.cu:6597-6598, src/stats.cu:114-126)
 // nothing is added in a step in which no far presynaptic neuron has an
 // output above 1.0: every workgroup tests that itself from the step's spike
@@ -413,40 +418,72 @@ __global__ __launch_bounds__(256) void k_rec_far(vc_v1_params p, vc_v1_state s,
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) lgE[ig] = lhE[ig] = lgI[ig] = lhI[ig] = 0.0f;
     uint32_t bad = 0u;
-    for (uint32_t j = 0; j < cnt; ++j) {
-        const size_t e = (size_t)j * nV1 + n;
-        const uint32_t ipre = fc.pre[e];
-        const float strength = fc.w[e];
-        float time2post = fc.dist[e] / speed;
-        const uint32_t it2post = (uint32_t)ceilf(time2post / dt);
-        time2post = (float)it2post * dt - time2post;
-        if (ipre >= nV1) { bad |= 2u; continue; }
-        if (it2post >= depth) { bad |= 1u; continue; }
-        const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
-        const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
-        const float tv[2] = {train[(size_t)j0 * nV1 + ipre], train[(size_t)j1 * nV1 + ipre]};
-        const bool inh = (ipre % p.blockSize) >= p.nE;
+    // kFU connections at a time: their ids and distances, then the two slots'
+    // spike-mask words, all in flight together; strengths and ring values are
+    // read only for the rare connection with a spike in one of its slots (a
+    // slot's value is >= 1 exactly when its mask bit is set)
+    constexpr uint32_t kFU = 4;
+    for (uint32_t j00 = 0; j00 < cnt; j00 += kFU) {
+        uint32_t pr[kFU];
+        float ds[kFU];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            float dtsp = tv[k];
-            if (dtsp >= 1.0f) {
-                const float nsp = floorf(dtsp);
-                dtsp = (dtsp - nsp + (float)k) * dt - time2post;
-                if (dtsp < dt && dtsp >= 0.0f && strength > 0.0f) {
-                    const float ddt = dt - dtsp;
-                    const float a = nsp * strength;
-                    if (inh) {
+        for (uint32_t u = 0; u < kFU; ++u) {
+            const bool ok = j00 + u < cnt;
+            const size_t e = (size_t)(j00 + u) * nV1 + n;
+            pr[u] = ok ? fc.pre[e] : 0u;
+            ds[u] = ok ? fc.dist[e] : 0.0f;
+        }
+        uint32_t s0[kFU], hit[kFU];
+        float rm[kFU];
 #pragma unroll
-                        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
-                            if (ig < p.ngI)
-                                single_input(lgI[ig], lhI[ig], a * p.pI[to * p.ngI + ig], ddt,
-                                             p.riseI[ig], p.decayI[ig], p.dodI[ig]);
-                    } else {
+        for (uint32_t u = 0; u < kFU; ++u) {
+            float time2post = ds[u] / speed;
+            const uint32_t it2post = (uint32_t)ceilf(time2post / dt);
+            rm[u] = (float)it2post * dt - time2post;
+            const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
+            const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
+            s0[u] = j0;
+            hit[u] = 0u;
+            if (j00 + u >= cnt) continue;
+            if (pr[u] >= nV1) { bad |= 2u; continue; }
+            if (it2post >= depth) { bad |= 1u; continue; }
+            const uint32_t tt = pr[u] >> 6, l = pr[u] & 63u;
+            const uint64_t m0 = trainMask[(size_t)j0 * nTiles + tt];
+            const uint64_t m1 = trainMask[(size_t)j1 * nTiles + tt];
+            hit[u] = (uint32_t)((m0 >> l) & 1ull) | ((uint32_t)((m1 >> l) & 1ull) << 1);
+        }
 #pragma unroll
-                        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
-                            if (ig < p.ngE)
-                                single_input(lgE[ig], lhE[ig], a * p.pE[to * p.ngE + ig], ddt,
-                                             p.riseE[ig], p.decayE[ig], p.dodE[ig]);
+        for (uint32_t u = 0; u < kFU; ++u) {
+            if (!hit[u]) continue;
+            const uint32_t ipre = pr[u];
+            const float strength = fc.w[(size_t)(j00 + u) * nV1 + n];
+            const float time2post = rm[u];
+            const uint32_t j0 = s0[u], j1 = j0 + 1u == depth ? 0u : j0 + 1u;
+            const float tv[2] = {(hit[u] & 1u) ? train[(size_t)j0 * nV1 + ipre] : 0.0f,
+                                 (hit[u] & 2u) ? train[(size_t)j1 * nV1 + ipre] : 0.0f};
+            const bool inh = (ipre % p.blockSize) >= p.nE;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                float dtsp = tv[k];
+                if (dtsp >= 1.0f) {
+                    const float nsp = floorf(dtsp);
+                    dtsp = (dtsp - nsp + (float)k) * dt - time2post;
+                    if (dtsp < dt && dtsp >= 0.0f && strength > 0.0f) {
+                        const float ddt = dt - dtsp;
+                        const float a = nsp * strength;
+                        if (inh) {
+#pragma unroll
+                            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                                if (ig < p.ngI)
+                                    single_input(lgI[ig], lhI[ig], a * p.pI[to * p.ngI + ig],
+                                                 ddt, p.riseI[ig], p.decayI[ig], p.dodI[ig]);
+                        } else {
+#pragma unroll
+                            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                                if (ig < p.ngE)
+                                    single_input(lgE[ig], lhE[ig], a * p.pE[to * p.ngE + ig],
+                                                 ddt, p.riseE[ig], p.decayE[ig], p.dodE[ig]);
+                        }
                     }
                 }
             }
@@ -543,7 +580,7 @@ extern "C" int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *w, c
 extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                                const vc_rec_conn *rec, const uint32_t *syn,
                                const float *train, const uint64_t *trainMask, uint32_t depth,
-                               uint32_t cur, uint32_t *status) {
+                               uint32_t cur, uint32_t window, uint32_t *status) {
     if (!rec_ok(p, rec, "vc_v1_rec_delay")) return VC_E_INVALID;
     if (!s || !s->gE || !s->hE || !s->gI || !s->hI || !syn || !train || !trainMask) {
         vc_set_error("vc_v1_rec_delay: null argument");
@@ -553,10 +590,12 @@ extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_
         vc_set_error("vc_v1_rec_delay: syn must be 16-byte aligned");
         return VC_E_INVALID;
     }
-    if (depth == 0 || cur >= depth) {
-        vc_set_error("vc_v1_rec_delay: need cur (%u) < depth (%u)", cur, depth);
+    if (depth == 0 || cur >= depth || window > depth) {
+        vc_set_error("vc_v1_rec_delay: need cur (%u) < depth (%u) and window (%u) <= depth",
+                     cur, depth, window);
         return VC_E_INVALID;
     }
+    const uint32_t win = window ? window : depth;
     if (p->ngE > VC_MAX_NG || p->ngI > VC_MAX_NG || p->nType > VC_MAX_NTYPE) {
         vc_set_error("vc_v1_rec_delay: receptor or type count out of range");
         return VC_E_INVALID;
@@ -566,10 +605,10 @@ extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_
     const uint4 *sy = (const uint4 *)syn;
     if (dyn + lds_static<16>() + 1024u <= kLdsMax)     // margin for alignment padding
         hipLaunchKernelGGL(k_rec_delay<16>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
-                           *p, *s, *rec, sy, train, trainMask, depth, cur, status);
+                           *p, *s, *rec, sy, train, trainMask, depth, cur, win, status);
     else
         hipLaunchKernelGGL(k_rec_delay<8>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
-                           *p, *s, *rec, sy, train, trainMask, depth, cur, status);
+                           *p, *s, *rec, sy, train, trainMask, depth, cur, win, status);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -168,6 +168,7 @@ class GpuSim:
                                              self.speed, sp.dt, ptr(self.syn)),
               "vc_v1_rec_delay_table")
         self.depth = train_depth(dmax, self.speed, sp.dt)
+        self.window = self.depth     # slots the near delays reach
         if self.far_on:
             mf, cnt, P, W, Dm = net.far_ell()
             self.far_t = [torch.as_tensor(np.ascontiguousarray(x), device=d)
@@ -194,7 +195,7 @@ class GpuSim:
                                   ptr(self.train[slot]), ptr(dep), ptr(gprev)), "vc_v1_step")
         check(self.lib.vc_v1_rec_delay(h, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
                                        ptr(self.syn), ptr(self.train), ptr(self.trainMask),
-                                       self.depth, slot, ptr(self.status)),
+                                       self.depth, slot, self.window, ptr(self.status)),
               "vc_v1_rec_delay")
         if self.far_on:
             check(self.lib.vc_v1_rec_far(h, C.byref(self.p), C.byref(self.s), C.byref(self.far),
