@@ -286,6 +286,14 @@ typedef struct {
     const float *dist;          /* [maxFar][nV1] distance, mm                   */
     const uint64_t *preMask;    /* [nV1/64] bit n%64 of word n/64 set when n is
                                    the presynaptic neuron of a far connection  */
+    /* the same connections grouped by presynaptic neuron: connection k of
+     * presynaptic neuron q (outOff[q] <= k < outOff[q+1]) is connection
+     * outJ[k] of post neuron outPost[k]; every pre[j][n] < nV1 */
+    const uint32_t *outOff;     /* [nV1 + 1] */
+    const uint32_t *outPost;    /* [outOff[nV1]] */
+    const uint32_t *outJ;       /* [outOff[nV1]] */
+    uint64_t *hit;              /* [nV1][(maxFar + 63)/64] work bits: zero before
+                                   the first call; every call leaves it zero   */
 } vc_far_conn;
 
 /* vc_v1_rec_far: adds to s->gE/hE/gI/hI the far connections' input arriving
@@ -299,8 +307,12 @@ typedef struct {
  * reference, it adds nothing in a step in which no far presynaptic neuron
  * (far->preMask) has train[cur][n] > 1.0 (src/patch.cu:6597-6598, src/stats.cu
  * :114-126); trainMask[depth][nV1/64] holds each step's spike masks, as for
- * vc_v1_rec_delay.  *status (may be NULL) |= 1 for a delay that does not
- * fit, |= 2 for a presynaptic id >= nV1; such connections are skipped. */
+ * vc_v1_rec_delay.  Only the connections of presynaptic neurons that fired
+ * inside the ring are visited (far->out*), and far->hit is the calls' work
+ * memory (calls on one far->hit must not overlap).  *status (may be NULL)
+ * |= 1 when a spiking presynaptic neuron has a connection whose delay does
+ * not fit the ring, |= 2 for an out* entry out of range (post >= nV1 or
+ * j >= maxFar); such connections are skipped. */
 int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                   const vc_far_conn *far, const float *train, const uint64_t *trainMask,
                   uint32_t depth, uint32_t cur, float speed, uint32_t *status);

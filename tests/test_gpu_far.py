@@ -75,6 +75,7 @@ def test_rec_far_kernel_bitwise(pkg, speed):
                                   ptr(tm), D, cur, speed, ptr(status)), "vc_v1_rec_far")
         torch.cuda.synchronize()
         assert int(status.item()) == 0
+        assert int(g.far_t[-1].abs().max()) == 0             # work bits left clear
         hits = 0
         for k in ("gE", "hE", "gI", "hI"):
             a = dev[k].cpu().numpy()
@@ -84,8 +85,9 @@ def test_rec_far_kernel_bitwise(pkg, speed):
 
 
 def test_rec_far_status_flags(pkg):
-    """A ring shorter than a delay sets bit 0, a presynaptic id >= nV1 bit 1;
-    the connections concerned add nothing."""
+    """A ring shorter than the delays of spiking far presynaptic neurons sets
+    bit 0, an index entry out of range bit 1; the connections concerned add
+    nothing, and the work bits are left clear."""
     from visual_cortex_amd import abi
     from visual_cortex_amd.abi import check, ptr
     from visual_cortex_amd.sim import GpuSim
@@ -104,20 +106,28 @@ def test_rec_far_status_flags(pkg):
     torch.cuda.synchronize()
     assert int(status.item()) == 1
     assert float(dev["gE"].abs().max()) == 0.0 and float(dev["gI"].abs().max()) == 0.0
-    # one connection, to post neuron 0, from an id out of range
-    cnt = torch.zeros(sp.nV1, dtype=torch.int32, device="cuda")
-    cnt[0] = 1
-    pre = torch.full((1, sp.nV1), sp.nV1 + 5, dtype=torch.int32, device="cuda")
+    assert int(g.far_t[-1].abs().max()) == 0
+    # presynaptic neuron 0 with one index entry pointing past the last neuron
+    i32 = dict(dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(sp.nV1, **i32)
+    pre = torch.zeros(1, sp.nV1, **i32)
     w = torch.ones(1, sp.nV1, device="cuda")
     dist = torch.zeros(1, sp.nV1, device="cuda")
-    allpre = torch.full((sp.nV1 // 64,), -1, dtype=torch.int64, device="cuda")
-    far = abi.FarConn(1, ptr(cnt), ptr(pre), ptr(w), ptr(dist), ptr(allpre))
+    pmask = torch.zeros(sp.nV1 // 64, dtype=torch.int64, device="cuda")
+    pmask[0] = 1
+    outOff = torch.ones(sp.nV1 + 1, **i32)
+    outOff[0] = 0
+    outPost = torch.tensor([sp.nV1 + 5], **i32)
+    outJ = torch.zeros(1, **i32)
+    hit = torch.zeros(sp.nV1, dtype=torch.int64, device="cuda")
+    far = abi.FarConn(1, ptr(cnt), ptr(pre), ptr(w), ptr(dist), ptr(pmask), ptr(outOff),
+                      ptr(outPost), ptr(outJ), ptr(hit))
     status.zero_()
     check(g.lib.vc_v1_rec_far(None, C.byref(g.p), C.byref(s), C.byref(far), ptr(tr), ptr(tm),
                               1, 0, 0.25, ptr(status)), "vc_v1_rec_far")
     torch.cuda.synchronize()
     assert int(status.item()) == 2
-    assert float(dev["gE"].abs().max()) == 0.0
+    assert float(dev["gE"].abs().max()) == 0.0 and int(hit.abs().max()) == 0
 
 
 def test_rec_far_gate(pkg):

@@ -76,7 +76,7 @@ class RecConn(C.Structure):
 
 class FarConn(C.Structure):
     _fields_ = [("maxFar", u32), ("nFar", vp), ("pre", vp), ("w", vp), ("dist", vp),
-                ("preMask", vp)]
+                ("preMask", vp), ("outOff", vp), ("outPost", vp), ("outJ", vp), ("hit", vp)]
 
 
 class Spikes(C.Structure):

@@ -377,127 +377,129 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
 
 // Far (long-range) connections with delays: the reference's recal_G_vec,
 // src/coredynamics.cu:181-372 (noFarDelay 0, synFail 0), whose sums sum_G
-// (:2314-2349) adds to gE/hE/gI/hI after recal_G_mat.  A workgroup takes 64
-// post neurons, lane l of each of its 4 wavefronts post neuron l; the lists
-// are a transposed ELL (connection j of neuron n at j*nV1 + n, so every
-// record load of a wavefront is one coalesced 256-byte row).  Per chunk of 64
-// connections, wavefront w looks up connections w, w+4, ..., w+60 of its
-// post neuron in the shared ring's spike masks (8 bytes per 64 neurons and
-// step, mostly cache hits) and marks those with a spike in one of their two
-// slots; wavefront 0 then evaluates the marked connections of its post
-// neuron in list order (they are rare), reading their strengths and ring
-// values.  The ring holds the same steps' outputs as the reference's
-// per-connection rings, and a slot's value is >= 1 exactly when its mask bit
-// is set.  As in the reference (src/patch.cu:6597-6598, src/stats.cu:114-126)
-// nothing is added in a step in which no far presynaptic neuron has an
-// output above 1.0: a workgroup with anything to add tests that from the
-// step's spike masks (fc.preMask & trainMask[cur], then the values of the
-// bits set).  status |= 1 for a delay longer than the ring, |= 2 for a
-// presynaptic id out of range (both skipped).
-__global__ __launch_bounds__(256) void k_rec_far(vc_v1_params p, vc_v1_state s,
-                                                 vc_far_conn fc,
-                                                 const float *__restrict__ train,
-                                                 const uint64_t *__restrict__ trainMask,
-                                                 uint32_t depth, uint32_t cur, float speed,
-                                                 uint32_t *__restrict__ status) {
-    __shared__ uint64_t shit[4][64];
-    __shared__ uint32_t smax;
+// (:2314-2349) adds to gE/hE/gI/hI after recal_G_mat.  Far spikes are rare,
+// so the work is driven by them rather than by the 2-million-entry lists:
+//   k_far_mark: one lane per (age a, 64-neuron tile) of the ring finds the
+//     far presynaptic neurons that fired a steps ago; for each, the
+//     wavefront's lanes go through its connections together, and those whose
+//     delay puts that spike in one of their two slots (it2post = a for slot
+//     j = 0, a + 1 for slot j = 1) get their bit set in the post neuron's
+//     hit words (an OR, so the order of the marks does not matter);
+//   k_far_eval: one thread per post neuron takes its hit words (and clears
+//     them) and evaluates the marked connections in list order, reading their
+//     strengths, distances and ring values, so the per-post sum is the
+//     reference's.  The ring holds the same steps' outputs as the
+//     reference's per-connection rings, and a slot's value is >= 1 exactly
+//     when its mask bit is set.
+// As in the reference (src/patch.cu:6597-6598, src/stats.cu:114-126) nothing
+// is added in a step in which no far presynaptic neuron has an output above
+// 1.0: a workgroup with anything to add tests that from the step's spike
+// masks (fc.preMask & trainMask[cur], then the values of the bits set).
+__global__ __launch_bounds__(256) void k_far_mark(uint32_t nV1, float dt, vc_far_conn fc,
+                                                  const uint64_t *__restrict__ trainMask,
+                                                  uint32_t depth, uint32_t cur, float speed,
+                                                  uint32_t *__restrict__ status) {
+    const uint32_t nTiles = nV1 >> 6;
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = vc_lane();
+    uint32_t a = 0u, t = 0u;
+    uint64_t m = 0ull;
+    if (q < (uint64_t)depth * nTiles) {
+        a = (uint32_t)(q / nTiles);
+        t = (uint32_t)(q - (uint64_t)a * nTiles);
+        const uint32_t k = cur >= a ? cur - a : cur + depth - a;  // the slot of age a
+        m = trainMask[(size_t)k * nTiles + t] & fc.preMask[t];
+    }
+    const uint32_t hw = (fc.maxFar + 63u) >> 6;
+    uint32_t bad = 0u;
+    // the wavefront takes its lanes' spikes one at a time, the 64 lanes
+    // sharing the spiking neuron's connections
+    uint64_t live = __ballot(m != 0ull);
+    while (live) {
+        const uint32_t src = (uint32_t)__builtin_ctzll(live);
+        const uint64_t ms = vc_readlane_u64(m, src);
+        const uint32_t aa = vc_readlane_u32(a, src);
+        const uint32_t pre = vc_readlane_u32(t, src) * 64u + (uint32_t)__builtin_ctzll(ms);
+        if (lane == src) m &= m - 1ull;
+        const uint32_t r1 = fc.outOff[pre + 1];
+        for (uint32_t r = fc.outOff[pre] + lane; r < r1; r += 64u) {
+            const uint32_t post = fc.outPost[r], j = fc.outJ[r];
+            if (post >= nV1 || j >= fc.maxFar) { bad |= 2u; continue; }
+            const float d = fc.dist[(size_t)j * nV1 + post];
+            const uint32_t it2post = (uint32_t)ceilf((d / speed) / dt);
+            if (it2post >= depth) { bad |= 1u; continue; }
+            if (it2post == aa || it2post == aa + 1u)
+                atomicOr((unsigned long long *)&fc.hit[(size_t)post * hw + (j >> 6)],
+                         1ull << (j & 63u));
+        }
+        live = __ballot(m != 0ull);
+    }
+    if (bad && status) atomicOr(status, bad);
+}
+
+__global__ __launch_bounds__(256) void k_far_eval(vc_v1_params p, vc_v1_state s,
+                                                  vc_far_conn fc,
+                                                  const float *__restrict__ train,
+                                                  const uint64_t *__restrict__ trainMask,
+                                                  uint32_t depth, uint32_t cur, float speed) {
     const uint32_t nV1 = p.nV1, nTiles = nV1 >> 6;
-    const uint32_t l = threadIdx.x & 63u, sw = threadIdx.x >> 6;
-    const uint32_t n = blockIdx.x * 64u + l;
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t cnt = n < nV1 ? fc.nFar[n] : 0u;
-    if (threadIdx.x == 0) smax = 0u;
-    __syncthreads();
-    if (sw == 0 && cnt) atomicMax(&smax, cnt);
-    __syncthreads();
-    const uint32_t nCh = (smax + 63u) >> 6;
+    const uint32_t hw = (fc.maxFar + 63u) >> 6;
     const float dt = p.dt;
     const int to = type_of(p, n % p.blockSize);
     float lgE[VC_MAX_NG], lhE[VC_MAX_NG], lgI[VC_MAX_NG], lhI[VC_MAX_NG];
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) lgE[ig] = lhE[ig] = lgI[ig] = lhI[ig] = 0.0f;
-    uint32_t bad = 0u, any = 0u;
-    for (uint32_t c = 0; c < nCh; ++c) {
-        const uint32_t jb = c * 64u;
-        uint64_t hm = 0ull;
-        for (uint32_t h = 0; h < 2u; ++h) {           // 2 rounds of 8 connections per lane
-            uint32_t pr[8];
-            float ds[8];
+    uint32_t any = 0u;
+    for (uint32_t c = 0; n < nV1 && c < hw; ++c) {
+        uint64_t hh = fc.hit[(size_t)n * hw + c];
+        if (!hh) continue;
+        fc.hit[(size_t)n * hw + c] = 0ull;
+        while (hh) {                                  // marked connections in list order
+            const uint32_t j = c * 64u + (uint32_t)__builtin_ctzll(hh);
+            hh &= hh - 1ull;
+            if (j >= cnt) continue;
+            any = 1u;
+            const size_t e = (size_t)j * nV1 + n;
+            const uint32_t ipre = fc.pre[e];
+            const float strength = fc.w[e];
+            float time2post = fc.dist[e] / speed;
+            const uint32_t it2post = (uint32_t)ceilf(time2post / dt);
+            time2post = (float)it2post * dt - time2post;
+            const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
+            const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
+            const float tv[2] = {train[(size_t)j0 * nV1 + ipre], train[(size_t)j1 * nV1 + ipre]};
+            const bool inh = (ipre % p.blockSize) >= p.nE;
 #pragma unroll
-            for (uint32_t u = 0; u < 8u; ++u) {
-                const uint32_t j = jb + sw + 4u * (h * 8u + u);
-                const size_t e = (size_t)j * nV1 + n;
-                pr[u] = j < cnt ? fc.pre[e] : 0u;
-                ds[u] = j < cnt ? fc.dist[e] : 0.0f;
-            }
-            uint64_t m0[8], m1[8];
+            for (int k = 0; k < 2; ++k) {
+                float dtsp = tv[k];
+                if (dtsp >= 1.0f) {
+                    const float nsp = floorf(dtsp);
+                    dtsp = (dtsp - nsp + (float)k) * dt - time2post;
+                    if (dtsp < dt && dtsp >= 0.0f && strength > 0.0f) {
+                        const float ddt = dt - dtsp;
+                        const float a = nsp * strength;
+                        if (inh) {
 #pragma unroll
-            for (uint32_t u = 0; u < 8u; ++u) {
-                const uint32_t j = jb + sw + 4u * (h * 8u + u);
-                m0[u] = m1[u] = 0ull;
-                if (j >= cnt) continue;
-                const uint32_t it2post = (uint32_t)ceilf((ds[u] / speed) / dt);
-                if (pr[u] >= nV1) { bad |= 2u; continue; }
-                if (it2post >= depth) { bad |= 1u; continue; }
-                const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
-                const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
-                m0[u] = trainMask[(size_t)j0 * nTiles + (pr[u] >> 6)];
-                m1[u] = trainMask[(size_t)j1 * nTiles + (pr[u] >> 6)];
-            }
+                            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                                if (ig < p.ngI)
+                                    single_input(lgI[ig], lhI[ig], a * p.pI[to * p.ngI + ig],
+                                                 ddt, p.riseI[ig], p.decayI[ig], p.dodI[ig]);
+                        } else {
 #pragma unroll
-            for (uint32_t u = 0; u < 8u; ++u)
-                hm |= (((m0[u] | m1[u]) >> (pr[u] & 63u)) & 1ull) << (sw + 4u * (h * 8u + u));
-        }
-        shit[sw][l] = hm;
-        __syncthreads();
-        if (sw == 0) {
-            uint64_t hh = shit[0][l] | shit[1][l] | shit[2][l] | shit[3][l];
-            any |= hh != 0ull ? 1u : 0u;
-            while (hh) {                              // marked connections in list order
-                const uint32_t bit = (uint32_t)__builtin_ctzll(hh);
-                hh &= hh - 1ull;
-                const size_t e = (size_t)(jb + bit) * nV1 + n;
-                const uint32_t ipre = fc.pre[e];
-                const float strength = fc.w[e];
-                float time2post = fc.dist[e] / speed;
-                const uint32_t it2post = (uint32_t)ceilf(time2post / dt);
-                time2post = (float)it2post * dt - time2post;
-                const uint32_t j0 = cur >= it2post ? cur - it2post : cur + depth - it2post;
-                const uint32_t j1 = j0 + 1u == depth ? 0u : j0 + 1u;
-                const float tv[2] = {train[(size_t)j0 * nV1 + ipre],
-                                     train[(size_t)j1 * nV1 + ipre]};
-                const bool inh = (ipre % p.blockSize) >= p.nE;
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    float dtsp = tv[k];
-                    if (dtsp >= 1.0f) {
-                        const float nsp = floorf(dtsp);
-                        dtsp = (dtsp - nsp + (float)k) * dt - time2post;
-                        if (dtsp < dt && dtsp >= 0.0f && strength > 0.0f) {
-                            const float ddt = dt - dtsp;
-                            const float a = nsp * strength;
-                            if (inh) {
-#pragma unroll
-                                for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
-                                    if (ig < p.ngI)
-                                        single_input(lgI[ig], lhI[ig], a * p.pI[to * p.ngI + ig],
-                                                     ddt, p.riseI[ig], p.decayI[ig], p.dodI[ig]);
-                            } else {
-#pragma unroll
-                                for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
-                                    if (ig < p.ngE)
-                                        single_input(lgE[ig], lhE[ig], a * p.pE[to * p.ngE + ig],
-                                                     ddt, p.riseE[ig], p.decayE[ig], p.dodE[ig]);
-                            }
+                            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                                if (ig < p.ngE)
+                                    single_input(lgE[ig], lhE[ig], a * p.pE[to * p.ngE + ig],
+                                                 ddt, p.riseE[ig], p.decayE[ig], p.dodE[ig]);
                         }
                     }
                 }
             }
         }
-        __syncthreads();                              // before shit is rewritten
     }
     // the reference's gate: some far presynaptic output of this step above 1.0
-    if (!__syncthreads_or((int)(any | bad))) return;
+    if (!__syncthreads_or((int)any)) return;
     int fired = 0;
     for (uint32_t t = threadIdx.x; t < nTiles && !fired; t += blockDim.x) {
         uint64_t m = trainMask[(size_t)cur * nTiles + t] & fc.preMask[t];
@@ -508,8 +510,7 @@ __global__ __launch_bounds__(256) void k_rec_far(vc_v1_params p, vc_v1_state s,
         }
     }
     if (!__syncthreads_or(fired)) return;
-    if (bad && status) atomicOr(status, bad);
-    if (sw != 0 || cnt == 0u) return;                 // sum_G: nVec[id] > 0 only
+    if (n >= nV1 || cnt == 0u) return;                // sum_G: nVec[id] > 0 only
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
         if (ig < p.ngE) {
@@ -643,9 +644,9 @@ extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_st
         return VC_E_INVALID;
     }
     if (far->maxFar == 0 || p->nV1 == 0) return VC_OK;
-    if (!far->nFar || !far->pre || !far->w || !far->dist || !far->preMask || !train ||
-        !trainMask) {
-        vc_set_error("vc_v1_rec_far: null argument (nFar, pre, w, dist, preMask, train and "
+    if (!far->nFar || !far->pre || !far->w || !far->dist || !far->preMask || !far->outOff ||
+        !far->outPost || !far->outJ || !far->hit || !train || !trainMask) {
+        vc_set_error("vc_v1_rec_far: null argument (every vc_far_conn array, train and "
                      "trainMask are required)");
         return VC_E_INVALID;
     }
@@ -662,9 +663,17 @@ extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_st
         vc_set_error("vc_v1_rec_far: receptor or type count out of range");
         return VC_E_INVALID;
     }
-    hipLaunchKernelGGL(k_rec_far, dim3((p->nV1 + 63u) / 64u), dim3(256), 0,
-                       (hipStream_t)stream, *p, *s, *far, train, trainMask, depth, cur, speed,
+    const uint64_t nq = (uint64_t)depth * (p->nV1 / 64u);
+    if ((nq + 255u) / 256u > 0x7fffffffull) {
+        vc_set_error("vc_v1_rec_far: ring of %u steps is too long for one launch", depth);
+        return VC_E_INVALID;
+    }
+    hipLaunchKernelGGL(k_far_mark, dim3((uint32_t)((nq + 255u) / 256u)), dim3(256), 0,
+                       (hipStream_t)stream, p->nV1, p->dt, *far, trainMask, depth, cur, speed,
                        status);
+    VC_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_far_eval, dim3((p->nV1 + 255u) / 256u), dim3(256), 0,
+                       (hipStream_t)stream, *p, *s, *far, train, trainMask, depth, cur, speed);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

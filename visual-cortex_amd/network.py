@@ -231,6 +231,22 @@ class Network:
         P[j, post], W[j, post], Dm[j, post] = pre, w, dist
         return mf, cnt.astype(np.uint32), P, W, Dm
 
+    def far_out_index(self):
+        """(outOff u32 [nV1+1], outPost u32, outJ u32): the far connections
+        grouped by presynaptic neuron (ascending post neuron, then j), each as
+        (post neuron, index j in its list)."""
+        off, pre = self.far[0].astype(np.int64), self.far[1].astype(np.int64)
+        nV1 = self.spec.nV1
+        if pre.size and int(pre.max()) >= nV1:
+            raise ValueError("a far connection's presynaptic id is out of range")
+        cnt = np.diff(off)
+        post = np.repeat(np.arange(nV1), cnt)
+        j = np.arange(pre.size) - np.repeat(off[:-1], cnt)
+        order = np.argsort(pre, kind="stable")
+        outOff = np.zeros(nV1 + 1, np.uint32)
+        outOff[1:] = np.cumsum(np.bincount(pre, minlength=nV1))
+        return outOff, post[order].astype(np.uint32), j[order].astype(np.uint32)
+
     def far_pre_mask(self) -> np.ndarray:
         """int64 [nV1/64]: bit n%64 of word n/64 set when neuron n is the
         presynaptic neuron of a far connection."""

@@ -172,7 +172,9 @@ class GpuSim:
         if self.far_on:
             mf, cnt, P, W, Dm = net.far_ell()
             self.far_t = [torch.as_tensor(np.ascontiguousarray(x), device=d)
-                          for x in (cnt, P, W, Dm, net.far_pre_mask())]
+                          for x in (cnt, P, W, Dm, net.far_pre_mask(), *net.far_out_index())]
+            self.far_t.append(torch.zeros(sp.nV1 * ((mf + 63) // 64), dtype=torch.int64,
+                                          device=d))
             self.far = abi.FarConn(mf, *(ptr(x) for x in self.far_t))
             fmax = float(net.far[3].max()) if net.far[3].size else 0.0
             self.depth = max(self.depth, train_depth(fmax, self.speed, sp.dt))
