@@ -33,6 +33,8 @@
  *                            src/preprocess/connect.cu:424-466
  *   vc_v1_rec_far          recal_G_vec (far connections with delays) +
  *                            sum_G, src/coredynamics.cu:181-372, 2314-2349
+ *   vc_v1_delay_run        the delayed main loop (step, recal_G_mat,
+ *                            recal_G_vec), src/patch.cu:5966-6700
  * The data-structure layouts are documented field by field below.
  */
 #ifndef VC_ABI_H
@@ -316,6 +318,26 @@ typedef struct {
 int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                   const vc_far_conn *far, const float *train, const uint64_t *trainMask,
                   uint32_t depth, uint32_t cur, float speed, uint32_t *status);
+
+/* vc_v1_delay_run: nsteps steps of the delayed path in one call (the
+ * reference's main loop with noDelay = 0, src/patch.cu:5966-6700: compute_V,
+ * recal_G_mat, then recal_G_vec + sum_G): step i = it0 + i is vc_v1_step
+ * (prev = {zeroMask, og, oh}: nothing pending; next = ring slot (slot0 + i)
+ * % depth, whose row of train receives the step's outputs), then
+ * vc_v1_rec_delay (depth, window), then vc_v1_rec_far when far is not NULL
+ * and has connections.  lgnSInfo + i*lgnStride is step i's LGN output row,
+ * ffMask + i*maskStride its event mask (or NULL); out_sInfo + i*outStride
+ * receives a copy of step i's outputs when out_sInfo is not NULL.  zeroMask
+ * is nV1/64 zero words; og, oh [VC_MAX_NG][nV1] are work rows.  The same
+ * launches as the per-step calls, issued from one host call. */
+int vc_v1_delay_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                    const vc_ff_conn *ff, const vc_rec_conn *rec, const uint32_t *syn,
+                    const vc_far_conn *far, float *train, uint64_t *trainMask,
+                    uint32_t depth, uint32_t window, uint32_t slot0,
+                    const float *lgnSInfo, uint64_t lgnStride, uint32_t *ffMask,
+                    uint64_t maskStride, uint32_t it0, uint32_t nsteps, float speed,
+                    const uint64_t *zeroMask, float *og, float *oh,
+                    float *out_sInfo, uint64_t outStride, uint32_t *status);
 
 /* ---- reverse mode of the V1 step (no counterpart in the reference; the
  * forward it differentiates is vc_v1_step, i.e. compute_V_collect_spike_

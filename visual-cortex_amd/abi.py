@@ -136,6 +136,10 @@ SIGNATURES = {
     "vc_v1_rec_delay_table": [vp, C.c_uint64, vp, vp, C.c_float, C.c_float, vp],
     "vc_v1_rec_delay": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn), vp,
                         vp, vp, u32, u32, u32, vp],
+    "vc_v1_delay_run": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),
+                        C.POINTER(RecConn), vp, C.POINTER(FarConn), vp, vp, u32, u32, u32, vp,
+                        C.c_uint64, vp, C.c_uint64, u32, u32, C.c_float, vp, vp, vp, vp,
+                        C.c_uint64, vp],
     "vc_v1_rec_far": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FarConn), vp,
                       vp, u32, u32, C.c_float, vp],
     "vc_v1_step_snapshot": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(FFConn),

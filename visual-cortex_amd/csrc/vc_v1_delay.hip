@@ -677,3 +677,44 @@ extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_st
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }
+
+extern "C" int vc_v1_delay_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                               const vc_ff_conn *ff, const vc_rec_conn *rec, const uint32_t *syn,
+                               const vc_far_conn *far, float *train, uint64_t *trainMask,
+                               uint32_t depth, uint32_t window, uint32_t slot0,
+                               const float *lgnSInfo, uint64_t lgnStride, uint32_t *ffMask,
+                               uint64_t maskStride, uint32_t it0, uint32_t nsteps, float speed,
+                               const uint64_t *zeroMask, float *og, float *oh,
+                               float *out_sInfo, uint64_t outStride, uint32_t *status) {
+    if (!p || !train || !trainMask || !lgnSInfo || !zeroMask || !og || !oh) {
+        vc_set_error("vc_v1_delay_run: null argument");
+        return VC_E_INVALID;
+    }
+    if (depth == 0 || slot0 >= depth) {
+        vc_set_error("vc_v1_delay_run: need slot0 (%u) < depth (%u)", slot0, depth);
+        return VC_E_INVALID;
+    }
+    const uint32_t nTiles = p->nV1 / 64u;
+    const vc_spikes none = {(uint64_t *)zeroMask, og, oh};
+    for (uint32_t i = 0; i < nsteps; ++i) {
+        const uint32_t slot = (uint32_t)(((uint64_t)slot0 + i) % depth);
+        const vc_spikes ring = {trainMask + (size_t)slot * nTiles, og, oh};
+        float *row = train + (size_t)slot * p->nV1;
+        int rc = vc_v1_step(stream, p, s, ff, rec, &none, &ring, lgnSInfo + i * lgnStride,
+                            ffMask ? ffMask + i * maskStride : nullptr, it0 + i, row, nullptr,
+                            nullptr);
+        if (rc != VC_OK) return rc;
+        rc = vc_v1_rec_delay(stream, p, s, rec, syn, train, trainMask, depth, slot, window,
+                             status);
+        if (rc != VC_OK) return rc;
+        if (far && far->maxFar) {
+            rc = vc_v1_rec_far(stream, p, s, far, train, trainMask, depth, slot, speed, status);
+            if (rc != VC_OK) return rc;
+        }
+        if (out_sInfo)
+            VC_HIP_CHECK(hipMemcpyAsync(out_sInfo + i * outStride, row,
+                                        (size_t)p->nV1 * sizeof(float),
+                                        hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    }
+    return VC_OK;
+}

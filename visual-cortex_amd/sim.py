@@ -381,9 +381,17 @@ class GpuSim:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e0.record(sv)
             if self.delay:
-                for i in range(n):
-                    row = out[done + i] if out is not None else None
-                    slot = self._delay_step(buf, k_in + i, it + i, sInfo=row)
+                # the n steps in one native call (vc_v1_delay_run)
+                slot0 = (it + self.slot_off) % self.depth
+                check(self.lib.vc_v1_delay_run(
+                    self._hs(sv), C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                    C.byref(self.rec), ptr(self.syn), C.byref(self.far) if self.far_on else None,
+                    ptr(self.train), ptr(self.trainMask), self.depth, self.window, slot0,
+                    ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, sp.nLGN, mptr, sp.nV1,
+                    it, n, self.speed, ptr(self._keep), ptr(self.dog), ptr(self.doh),
+                    optr if out is not None else None, sp.nV1, ptr(self.status)),
+                    "vc_v1_delay_run")
+                slot = (it + n - 1 + self.slot_off) % self.depth
                 self.sInfo_last = out[done + n - 1] if out is not None else self.train[slot]
                 if events is not None:
                     e1 = torch.cuda.Event(enable_timing=True)
