@@ -169,7 +169,7 @@ def main():
 
     kernel = "k_v1_multi" if sim.persistent else "k_v1_step"
     if sim.delay:
-        kernel = "k_v1_step+k_rec_delay" + ("+k_rec_far" if sim.far_on else "")
+        kernel = "k_v1_step+k_rec_delay" + ("+k_far_mark+k_far_eval" if sim.far_on else "")
     line = {
         "metric": "V1 neuron-steps/s at 64 blocks dt=0.125ms",
         "value": value,
