@@ -92,7 +92,8 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
                                                    const float *__restrict__ train,
                                                    const uint64_t *__restrict__ tmask,
                                                    uint32_t depth, uint32_t cur, uint32_t win,
-                                                   uint32_t *__restrict__ status) {
+                                                   uint32_t *__restrict__ status,
+                                                   float *__restrict__ copyOut) {
     __shared__ uint64_t su[kMaxEntD];
     __shared__ uint32_t soff[kMaxEntD + 1];
     __shared__ uint32_t sbid[kMaxEntD];
@@ -373,6 +374,7 @@ __global__ __launch_bounds__(256) void k_rec_delay(vc_v1_params p, vc_v1_state s
             s.hI[ig * nV1 + n] = s.hI[ig * nV1 + n] + lhI[ig];
         }
     }
+    if (copyOut) copyOut[n] = train[(size_t)cur * nV1 + n];   // the step's output row
 }
 
 // Far (long-range) connections with delays: the reference's recal_G_vec,
@@ -599,10 +601,11 @@ extern "C" int vc_v1_rec_delay_table(void *stream, uint64_t n, const float *w, c
     return VC_OK;
 }
 
-extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
-                               const vc_rec_conn *rec, const uint32_t *syn,
-                               const float *train, const uint64_t *trainMask, uint32_t depth,
-                               uint32_t cur, uint32_t window, uint32_t *status) {
+// vc_v1_rec_delay, optionally also copying ring row `cur` to copyOut[nV1]
+static int rec_delay_launch(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                            const vc_rec_conn *rec, const uint32_t *syn, const float *train,
+                            const uint64_t *trainMask, uint32_t depth, uint32_t cur,
+                            uint32_t window, uint32_t *status, float *copyOut) {
     if (!rec_ok(p, rec, "vc_v1_rec_delay")) return VC_E_INVALID;
     if (!s || !s->gE || !s->hE || !s->gI || !s->hI || !syn || !train || !trainMask) {
         vc_set_error("vc_v1_rec_delay: null argument");
@@ -627,12 +630,22 @@ extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_
     const uint4 *sy = (const uint4 *)syn;
     if (dyn + lds_static<16>() + 1024u <= kLdsMax)     // margin for alignment padding
         hipLaunchKernelGGL(k_rec_delay<16>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
-                           *p, *s, *rec, sy, train, trainMask, depth, cur, win, status);
+                           *p, *s, *rec, sy, train, trainMask, depth, cur, win, status,
+                           copyOut);
     else
         hipLaunchKernelGGL(k_rec_delay<8>, dim3(p->nV1 / wg), dim3(wg), dyn, (hipStream_t)stream,
-                           *p, *s, *rec, sy, train, trainMask, depth, cur, win, status);
+                           *p, *s, *rec, sy, train, trainMask, depth, cur, win, status,
+                           copyOut);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
+}
+
+extern "C" int vc_v1_rec_delay(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                               const vc_rec_conn *rec, const uint32_t *syn,
+                               const float *train, const uint64_t *trainMask, uint32_t depth,
+                               uint32_t cur, uint32_t window, uint32_t *status) {
+    return rec_delay_launch(stream, p, s, rec, syn, train, trainMask, depth, cur, window,
+                            status, nullptr);
 }
 
 extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_state *s,
@@ -704,17 +717,14 @@ extern "C" int vc_v1_delay_run(void *stream, const vc_v1_params *p, const vc_v1_
                             ffMask ? ffMask + i * maskStride : nullptr, it0 + i, row, nullptr,
                             nullptr);
         if (rc != VC_OK) return rc;
-        rc = vc_v1_rec_delay(stream, p, s, rec, syn, train, trainMask, depth, slot, window,
-                             status);
+        // the delayed gather also copies the step's outputs to out_sInfo
+        rc = rec_delay_launch(stream, p, s, rec, syn, train, trainMask, depth, slot, window,
+                              status, out_sInfo ? out_sInfo + i * outStride : nullptr);
         if (rc != VC_OK) return rc;
         if (far && far->maxFar) {
             rc = vc_v1_rec_far(stream, p, s, far, train, trainMask, depth, slot, speed, status);
             if (rc != VC_OK) return rc;
         }
-        if (out_sInfo)
-            VC_HIP_CHECK(hipMemcpyAsync(out_sInfo + i * outStride, row,
-                                        (size_t)p->nV1 * sizeof(float),
-                                        hipMemcpyDeviceToDevice, (hipStream_t)stream));
     }
     return VC_OK;
 }
