@@ -151,6 +151,13 @@ class OracleSim:
                 self.depth = max(self.depth,
                                  network.train_depth(fmax, sp.speedOfThought, sp.dt))
             self.train = np.zeros((self.depth, sp.nV1), np.float32)
+        # delayed far connections with undelayed near ones: a ring for them alone
+        self.far_only = (not self.delay) and net.n_far() > 0 and net.far_delayed()
+        if self.far_only:
+            self.far = [np.ascontiguousarray(x) for x in net.far]
+            fmax = float(self.far[3].max()) if self.far[3].size else 0.0
+            self.depth = network.train_depth(fmax, sp.speedOfThought, sp.dt)
+            self.train = np.zeros((self.depth, sp.nV1), np.float32)
         self.lTR = np.zeros(sp.nLGN, np.float32)
         self.lNL = np.zeros(sp.nLGN, np.float32)
         self.o.L.orc_lgn_init(C.byref(self.lp), _fp(self.lTR), _fp(self.lNL))
@@ -255,14 +262,16 @@ class OracleSim:
                                                _fp(self.train), self.depth, slot)
                 if rc != 0:
                     raise RuntimeError(f"orc_v1_rec_delay returned {rc}")
-                if self.far_on:
-                    off, pre, w, dist = self.far
-                    rc = self.o.L.orc_v1_rec_far(C.byref(self.p), _fp(st["gE"]), _fp(st["hE"]),
-                                                 _fp(st["gI"]), _fp(st["hI"]), _fp(off),
-                                                 _fp(pre), _fp(w), _fp(dist), sp.speedOfThought,
-                                                 _fp(self.train), self.depth, slot)
-                    if rc != 0:
-                        raise RuntimeError(f"orc_v1_rec_far returned {rc}")
+            if (self.delay and self.far_on) or self.far_only:   # recal_G_vec + sum_G
+                slot = it % self.depth
+                self.train[slot] = out
+                off, pre, w, dist = self.far
+                rc = self.o.L.orc_v1_rec_far(C.byref(self.p), _fp(st["gE"]), _fp(st["hE"]),
+                                             _fp(st["gI"]), _fp(st["hI"]), _fp(off), _fp(pre),
+                                             _fp(w), _fp(dist), sp.speedOfThought,
+                                             _fp(self.train), self.depth, slot)
+                if rc != 0:
+                    raise RuntimeError(f"orc_v1_rec_far returned {rc}")
             if record:
                 self.sInfo_hist.append(out)
             self.it += 1

@@ -38,6 +38,8 @@ def test_cfg_semantics(pkg):
     assert sp.noDelay == 1 and sp.noFarDelay == 1                       # the reference's defaults
     with pytest.raises(ValueError):              # noFarDelay defaults to 1: refused with delays
         config.spec_from_config(config.Config("noDelay = 0"))
+    sp = config.spec_from_config(config.Config("noDelay = 1\nnoFarDelay = 0"))
+    assert sp.noDelay == 1 and sp.noFarDelay == 0                     # far delays only
 
 
 def test_stimulus_and_lists_round_trip(pkg, tmp_path):

@@ -3,8 +3,11 @@ reference's recal_G_vec + sum_G, src/coredynamics.cu:181-372, 2314-2349):
 vc_v1_rec_far against the oracle's orc_v1_rec_far on random spike-train
 rings, and whole delayed simulations with far lists against the oracle's,
 all bit for bit; its status flags; a snapshot of such a run resumes bit for
-bit; a network read back from its files runs the same."""
+bit; a network read back from its files runs the same; far connections with
+undelayed near ones (noDelay 1, noFarDelay 0) equal the oracle too."""
 import ctypes as C
+import pathlib
+import tempfile
 
 import numpy as np
 import pytest
@@ -268,3 +271,39 @@ def test_far_network_from_files(pkg, tmp_path):
     sa = a.sInfo()
     assert int((sa >= 1).sum()) > 100
     assert np.array_equal(sa.view(np.uint32), b.sInfo().view(np.uint32))
+
+
+@pytest.mark.parametrize("speed", [0.25, 2.0])
+def test_far_with_undelayed_near_bitwise(pkg, speed):
+    """noDelay 1 with noFarDelay 0: undelayed near synapses (applied in the
+    step, recal_G_mat_nd) and delayed far connections after them; the whole
+    run equals the oracle's bit for bit, and a snapshot resumes bit for bit."""
+    import oracle
+    from visual_cortex_amd import snapshot
+    from visual_cortex_amd.sim import GpuSim
+    _flush(f"test_far_with_undelayed_near_bitwise: speed {speed}")
+    net = _net(noDelay=1, noFarDelay=0, speedOfThought=speed)
+    g = GpuSim(net)
+    assert g.far_only and not g.delay and not g.persistent
+    o = oracle.OracleSim(net, pkg, tables=_tables(g))
+    assert o.far_only and o.depth == g.depth
+    npf = g.tm["ntPerFrame"]
+    g.run(2 * npf)
+    o.run(4 * npf)
+    with tempfile.TemporaryDirectory() as td:
+        path = pathlib.Path(td) / "snap.bin"
+        snapshot.save(g, path)
+        g.run(2 * npf)
+        b = GpuSim(net)                  # resumed from the snapshot
+        snapshot.restore(b, path)
+        b.run(2 * npf)
+    sg, so = g.sInfo(), o.sInfo()
+    nsp = int((so >= 1).sum())
+    _flush(f"  depth {g.depth}, V1 spikes gpu={int((sg >= 1).sum())} oracle={nsp}")
+    assert nsp > 200
+    diff = np.argwhere(sg.view(np.uint32) != so.view(np.uint32))
+    assert diff.size == 0, f"first sInfo difference at (step, neuron) {diff[0]}"
+    sgs = g.state_numpy()
+    for k in ("v", "w", "tBack", "gFF", "hFF", "gE", "hE", "gI", "hI"):
+        assert np.array_equal(sgs[k].view(np.uint32), o.st[k].view(np.uint32)), k
+    assert np.array_equal(b.sInfo().view(np.uint32), sg[2 * npf:].view(np.uint32))

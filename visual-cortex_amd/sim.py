@@ -94,11 +94,15 @@ class GpuSim:
         # vc_v1_rec_delay); with noFarDelay 1 the reference's recal_G_vec adds
         # nothing (src/coredynamics.cu:219-220), so they are not run
         self.far_on = net.n_far() > 0 and net.far_delayed()
-        if self.far_on and not self.delay:
-            raise NotImplementedError("delayed far connections with undelayed near ones "
-                                      "(noFarDelay 0, noDelay 1) are not built")
+        # delayed far connections with undelayed near ones (noFarDelay 0, noDelay 1):
+        # a ring for the far input only; each step is vc_v1_step, the step's own
+        # recurrent increments (vc_v1_rec_apply, recal_G_mat_nd) and vc_v1_rec_far
+        self.far_only = self.far_on and not self.delay
         if self.delay:
             self._setup_delay()
+        elif self.far_only:
+            self.speed = float(sp.speedOfThought)
+            self._setup_far(1)
         # ---- spike buffers (double buffered)
         nT = sp.nV1 // 64
         self.masks = [torch.zeros(nT, dtype=torch.int64, device=d) for _ in range(2)]
@@ -106,7 +110,7 @@ class GpuSim:
         self.oh = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
         self.spk = [abi.Spikes(ptr(self.masks[i]), ptr(self.og[i]), ptr(self.oh[i])) for i in range(2)]
         self.spk2 = (abi.Spikes * 2)(*self.spk)
-        if self.delay:
+        if self.delay or self.far_only:
             zero = torch.zeros(nT, dtype=torch.int64, device=d)
             self.dog = torch.empty(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d)
             self.doh = torch.empty(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d)
@@ -168,7 +172,15 @@ class GpuSim:
                                              self.speed, sp.dt, ptr(self.syn)),
               "vc_v1_rec_delay_table")
         self.depth = train_depth(dmax, self.speed, sp.dt)
-        self.window = self.depth     # slots the near delays reach
+        self._setup_far(self.depth)
+
+    def _setup_far(self, depth_near: int):
+        """The far lists on the device (when delayed far input is on) and the
+        spike-train ring, as long as the near and the far delays need."""
+        from .network import train_depth
+        net, sp, d = self.net, self.net.spec, self.dev
+        self.depth = depth_near
+        self.window = depth_near     # slots the near delays reach
         if self.far_on:
             mf, cnt, P, W, Dm = net.far_ell()
             self.far_t = [torch.as_tensor(np.ascontiguousarray(x), device=d)
@@ -203,6 +215,29 @@ class GpuSim:
             check(self.lib.vc_v1_rec_far(h, C.byref(self.p), C.byref(self.s), C.byref(self.far),
                                          ptr(self.train), ptr(self.trainMask), self.depth,
                                          slot, self.speed, ptr(self.status)), "vc_v1_rec_far")
+        if sInfo is not None:
+            sInfo.copy_(self.train[slot])
+        return slot
+
+    def _far_only_step(self, buf: int, k_in: int, it: int, sInfo=None, dep=None, gprev=None):
+        """Step `it` with undelayed near and delayed far connections: the update
+        writes ring slot `slot`, the step's own recurrent increments are added
+        (compute_V, then recal_G_mat_nd), then the far input (recal_G_vec +
+        sum_G).  Returns the slot."""
+        sp = self.net.spec
+        slot = (it + self.slot_off) % self.depth
+        h = self._hs(self.s_v1)
+        mptr = ptr(self.ffMask2[buf]) + k_in * sp.nV1 * 4 if self.ev is not None else None
+        check(self.lib.vc_v1_step(h, C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                                  C.byref(self.rec), C.byref(self.spk_none),
+                                  C.byref(self.ring_spk[slot]),
+                                  ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, mptr, it,
+                                  ptr(self.train[slot]), ptr(dep), ptr(gprev)), "vc_v1_step")
+        check(self.lib.vc_v1_rec_apply(h, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
+                                       C.byref(self.ring_spk[slot])), "vc_v1_rec_apply")
+        check(self.lib.vc_v1_rec_far(h, C.byref(self.p), C.byref(self.s), C.byref(self.far),
+                                     ptr(self.train), ptr(self.trainMask), self.depth, slot,
+                                     self.speed, ptr(self.status)), "vc_v1_rec_far")
         if sInfo is not None:
             sInfo.copy_(self.train[slot])
         return slot
@@ -393,6 +428,12 @@ class GpuSim:
                     "vc_v1_delay_run")
                 slot = (it + n - 1 + self.slot_off) % self.depth
                 self.sInfo_last = out[done + n - 1] if out is not None else self.train[slot]
+            elif self.far_only:
+                for i in range(n):
+                    row = out[done + i] if out is not None else None
+                    slot = self._far_only_step(buf, k_in + i, it + i, sInfo=row)
+                self.sInfo_last = out[done + n - 1] if out is not None else self.train[slot]
+            if self.delay or self.far_only:
                 if events is not None:
                     e1 = torch.cuda.Event(enable_timing=True)
                     e1.record(sv)
@@ -475,6 +516,8 @@ class GpuSim:
                     sp_w.write(self.lgn_sInfo2[buf][:n_here].cpu().numpy())
             if self.delay:
                 self._delay_step(buf, k_in, it, sInfo=sInfo, dep=dep, gprev=gprev)
+            elif self.far_only:
+                self._far_only_step(buf, k_in, it, sInfo=sInfo, dep=dep, gprev=gprev)
             else:
                 nxt = 1 - self.cur
                 mptr = ptr(self.ffMask2[buf]) + k_in * nV1 * 4 if self.ev is not None else None
@@ -553,9 +596,10 @@ class GpuSim:
 
     def finalize(self):
         """Apply the last step's pending recurrent increments (the reference
-        applies them inside the same step, in recal_G).  With axonal delays
-        nothing is pending: each step ends with its delayed input."""
-        if self.delay:
+        applies them inside the same step, in recal_G).  With axonal delays,
+        or with far connections, nothing is pending: each step ends with its
+        recurrent input."""
+        if self.delay or self.far_only:
             return
         check(self.lib.vc_v1_rec_apply(None, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
                                        C.byref(self.spk[self.cur])), "vc_v1_rec_apply")
@@ -568,9 +612,9 @@ class GpuSim:
         waiting for a neighbour block (its results are then invalid)."""
         torch.cuda.synchronize()
         if int(self.status.item()) != 0:
-            if self.delay:
+            if self.delay or self.far_only:
                 if int(self.status.item()) & 2:
-                    raise abi.VCError("a far connection's presynaptic id is out of range")
+                    raise abi.VCError("a far connection's index entry is out of range")
                 raise abi.VCError("a synapse's delay does not fit the spike-train ring")
             raise abi.VCError("persistent V1 launch timed out waiting for neighbour blocks")
 

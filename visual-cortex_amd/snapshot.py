@@ -23,7 +23,8 @@ high, u32 stream id (0xffffffff: none), zeros.  The frame-ring fields are
 this simulator's: currentFrame frames have entered, iFrameHead =
 currentFrame mod maxFrame, the parvo phase fields and iStatus are 0.  With no
 axonal delays trainDepth is 1 and the spike train is the last step's output;
-with delays it is the whole ring and currentTimeSlot the slot of step it0
+with delays (or delayed far connections) it is the whole ring and
+currentTimeSlot the slot of step it0
 (a ring of another length is cut or padded keeping the newest steps, as
 src/patch.cu:5580-5660 does).
 
@@ -188,7 +189,8 @@ def save(sim, path) -> dict:
              oldFrameHead=(K - 1) % maxF, currentFrame=K, iModel=sp.iModel, learning=0,
              nV1=sp.nV1, nLGN=sp.nLGN, ngTypeFF=ngFF, ngTypeE=ngE, ngTypeI=ngI,
              nType=int(sim.p.nType), trainDepth=1)
-    if getattr(sim, "delay", False):       # the whole ring; the next step's slot
+    if getattr(sim, "delay", False) or getattr(sim, "far_only", False):
+        # the whole ring (axonal delays, or far connections); the next step's slot
         h.update(trainDepth=sim.depth, currentTimeSlot=(sim.it + sim.slot_off) % sim.depth)
         train = sim.train.cpu().numpy()
     else:
@@ -235,7 +237,7 @@ def restore(sim, path, as_init: bool = False) -> dict:
     sim.lTR.copy_(T(d["leftTimeRate"]))
     sim.lNL.copy_(T(d["lastNegLogRand"]))
     ring = d["spikeTrain"]
-    if getattr(sim, "delay", False):
+    if getattr(sim, "delay", False) or getattr(sim, "far_only", False):
         it0 = 0 if as_init else h["it0"]
         Df, D = int(h["trainDepth"]), sim.depth
         cts = int(h.get("currentTimeSlot", 0)) % Df   # the file's slot of step it0

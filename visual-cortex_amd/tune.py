@@ -53,7 +53,8 @@ def tune_step(sim, steps: int = 64, reps: int = 2, verbose: bool = False) -> tup
         return wg, t
     if sim.wDense is not None:                   # dense rows: no tile to choose
         return int(sim.p.stepWG), 0
-    if getattr(sim, "delay", False):             # the gather runs in vc_v1_rec_delay
+    if getattr(sim, "delay", False) or getattr(sim, "far_only", False):
+        # the gather runs in vc_v1_rec_delay / vc_v1_rec_apply
         return int(sim.p.stepWG), int(sim.p.gatherTile)
     sp = sim.net.spec
     npf = sim.tm["ntPerFrame"]
