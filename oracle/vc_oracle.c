@@ -539,17 +539,28 @@ int orc_v1_rec_far(const vc_v1_params *p, float *gE, float *hE, float *gI,
                    const float *train, uint32_t depth, uint32_t cur) {
     const uint32_t nV1 = p->nV1, bs = p->blockSize;
     const float dt = p->dt;
-    int bad = 0, farSpiked = 0;
+    int bad = 0, anySpiked = 0;
     if (depth == 0 || cur >= depth) return VC_E_INVALID;
     /* fill_fSpikeTrain's test (src/stats.cu:114-126): recal_G_vec and sum_G
      * run only in a step in which some far presynaptic neuron's output is
-     * above 1.0 (src/patch.cu:6597-6598) */
-    for (uint32_t j = 0; j < fOff[nV1] && !farSpiked; ++j)
-        if (fPre[j] < nV1 && train[(uint64_t)nV1 * cur + fPre[j]] > 1.0f) farSpiked = 1;
-    if (!farSpiked) return VC_OK;
+     * above 1.0 (src/patch.cu:6597-6598); batched trials (p->blockTrial) are
+     * separate runs of the reference, so the test is per trial */
+    uint32_t nTrial = 1;
+    if (p->blockTrial)
+        for (uint32_t b = 0; b < p->nblock; ++b)
+            if (p->blockTrial[b] + 1 > nTrial) nTrial = p->blockTrial[b] + 1;
+    unsigned char *farSpiked = (unsigned char *)calloc(nTrial, 1);
+    if (!farSpiked) return VC_E_INVALID;
+    for (uint32_t j = 0; j < fOff[nV1]; ++j)
+        if (fPre[j] < nV1 && train[(uint64_t)nV1 * cur + fPre[j]] > 1.0f) {
+            farSpiked[p->blockTrial ? p->blockTrial[fPre[j] / bs] : 0] = 1;
+            anySpiked = 1;
+        }
+    if (!anySpiked) { free(farSpiked); return VC_OK; }
 #pragma omp parallel for schedule(static) reduction(|:bad)
     for (uint32_t n = 0; n < nV1; ++n) {
         if (fOff[n + 1] == fOff[n]) continue;          /* sum_G: nVec[id] > 0 */
+        if (!farSpiked[p->blockTrial ? p->blockTrial[n / bs] : 0]) continue;
         int t = orc_type_of(p, n % bs);
         float lgE[VC_MAX_NG] = {0}, lhE[VC_MAX_NG] = {0};
         float lgI[VC_MAX_NG] = {0}, lhI[VC_MAX_NG] = {0};
@@ -596,6 +607,7 @@ int orc_v1_rec_far(const vc_v1_params *p, float *gE, float *hE, float *gI,
             hI[ig * nV1 + n] += lhI[ig];
         }
     }
+    free(farSpiked);
     return bad ? VC_E_INVALID : VC_OK;
 }
 

@@ -221,3 +221,29 @@ def test_far_gate_on_current_far_spikes(pkg):
         n = ref[k].shape[0]
         np.testing.assert_allclose(st[k][:n].astype(np.float64), ref[k], rtol=3e-5,
                                    atol=1e-7 * max(1.0, ref[k].max()))
+
+
+def test_batched_far_trials_equal_serial(pkg):
+    """Trials with far lists batched into one network: each trial's output
+    equals its own run bit for bit -- the far gate is per trial, as each
+    trial is a separate run of the reference (trial 2 fires far less, so
+    many steps open the gate of one trial only)."""
+    import oracle
+    from visual_cortex_amd import batch, network
+    specs = [network.spec_tiny(nblock=nb, seed=300 + i, ori_deg=30.0 * i, bgStrength=(0.4, 0.4),
+                               noDelay=0, speedOfThought=0.5, nFar=10)
+             for i, nb in enumerate((4, 2))]
+    nets = [network.build_network(s) for s in specs]
+    b = batch.batch_networks(nets, device="cpu")
+    assert b.n_far() == sum(n.n_far() for n in nets)
+    ob = oracle.OracleSim(b, pkg)
+    ob.run(96)
+    sb = ob.sInfo()
+    off = 0
+    for n in nets:
+        o = oracle.OracleSim(n, pkg)
+        o.run(96)
+        s = o.sInfo()
+        assert int((s >= 1).sum()) > 40
+        assert np.array_equal(sb[:, off:off + n.spec.nV1].view(np.uint32), s.view(np.uint32))
+        off += n.spec.nV1

@@ -307,3 +307,29 @@ def test_far_with_undelayed_near_bitwise(pkg, speed):
     for k in ("v", "w", "tBack", "gFF", "hFF", "gE", "hE", "gI", "hI"):
         assert np.array_equal(sgs[k].view(np.uint32), o.st[k].view(np.uint32)), k
     assert np.array_equal(b.sInfo().view(np.uint32), sg[2 * npf:].view(np.uint32))
+
+
+def test_batched_far_trials_equal_serial(pkg):
+    """Delayed trials with far lists batched into one network: each trial's
+    output equals its own run bit for bit (per-trial far gate)."""
+    from visual_cortex_amd import batch, network
+    from visual_cortex_amd.sim import GpuSim
+    _flush("test_batched_far_trials_equal_serial: 3 trials, speed 0.5, 160 steps")
+    specs = [network.spec_tiny(nblock=nb, seed=310 + i, ori_deg=22.5 * i, bgStrength=(0.4, 0.4),
+                               noDelay=0, speedOfThought=0.5, nFar=10)
+             for i, nb in enumerate((4, 2, 3))]
+    nets = [network.build_network(s) for s in specs]
+    bnet = batch.batch_networks(nets, device="cuda")
+    g = GpuSim(bnet)
+    assert g.delay and g.far_on
+    g.run(160)
+    sb = g.sInfo()
+    off = 0
+    for t, n in enumerate(nets):
+        gs = GpuSim(n)
+        assert gs.far_on and gs.depth <= g.depth
+        gs.run(160)
+        ss = gs.sInfo()
+        assert int((ss >= 1).sum()) > 40, t
+        assert np.array_equal(sb[:, off:off + n.spec.nV1].view(np.uint32), ss.view(np.uint32)), t
+        off += n.spec.nV1

@@ -168,8 +168,6 @@ def batch_networks(nets: list, device=None) -> N.Network:
                                       nets[0].frames.shape, sp0.noDelay, sp0.speedOfThought):
             raise ValueError("batched trials must share block size, dt, stimulus shape and "
                              "axonal-delay settings")
-        if getattr(n, "far", None) is not None and n.n_far() > 0:
-            raise NotImplementedError("batched trials with far connections are not built")
     if device is None:
         c0 = nets[0].csr
         device = c0[0].device if c0 is not None and hasattr(c0[0], "device") else \
@@ -230,6 +228,25 @@ def batch_networks(nets: list, device=None) -> N.Network:
                     sLGN=sLGN, idLGN=idLGN, lgn=lgn,
                     frames=np.stack([n.frames for n in nets]), init_lms=nets[0].init_lms)
     net.csr = (rowMask, rowOff, w)
+    if any(getattr(n, "far", None) is not None and n.n_far() > 0 for n in nets):
+        # far lists: trial t's post neurons and presynaptic ids shifted by its first neuron
+        offs_, pres, wf, df = [np.zeros(1, np.int64)], [], [], []
+        v0 = base_f = 0
+        for n in nets:
+            nv = n.spec.nV1
+            if getattr(n, "far", None) is None:
+                offs_.append(np.full(nv, base_f, np.int64))
+            else:
+                o, pr, ww, dd = n.far
+                offs_.append(o[1:].astype(np.int64) + base_f)
+                pres.append(pr.astype(np.int64) + v0)
+                wf.append(ww)
+                df.append(dd)
+                base_f += int(o[-1])
+            v0 += nv
+        cat = lambda xs, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt)
+        net.far = (np.concatenate(offs_).astype(np.uint32), cat(pres, np.uint32),
+                   cat(wf, np.float32), cat(df, np.float32))
     seeds = np.array([[n.spec.seed & 0xffffffff, (n.spec.seed >> 32) & 0xffffffff] for n in nets],
                      np.uint32).ravel()
     net.batch = BatchInfo(

@@ -501,9 +501,14 @@ __global__ __launch_bounds__(256) void k_far_eval(vc_v1_params p, vc_v1_state s,
         }
     }
     // the reference's gate: some far presynaptic output of this step above 1.0
+    // (of this trial's neurons when trials are batched: a workgroup lies in one
+    // V1 block, and far connections stay inside a trial)
     if (!__syncthreads_or((int)any)) return;
+    const uint32_t myTrial = p.blockTrial ? p.blockTrial[(blockIdx.x * blockDim.x) / p.blockSize]
+                                          : 0u;
     int fired = 0;
     for (uint32_t t = threadIdx.x; t < nTiles && !fired; t += blockDim.x) {
+        if (p.blockTrial && p.blockTrial[(t * 64u) / p.blockSize] != myTrial) continue;
         uint64_t m = trainMask[(size_t)cur * nTiles + t] & fc.preMask[t];
         while (m) {
             const uint32_t b = (uint32_t)__builtin_ctzll(m);
@@ -685,7 +690,8 @@ extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_st
                        (hipStream_t)stream, p->nV1, p->dt, *far, trainMask, depth, cur, speed,
                        status);
     VC_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_far_eval, dim3((p->nV1 + 255u) / 256u), dim3(256), 0,
+    const uint32_t wg = wg_size(p->blockSize);          // workgroups inside one V1 block
+    hipLaunchKernelGGL(k_far_eval, dim3(p->nV1 / wg), dim3(wg), 0,
                        (hipStream_t)stream, *p, *s, *far, train, trainMask, depth, cur, speed);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
