@@ -423,6 +423,15 @@ int vc_lgn_frame_response(void *stream, const float *frame, uint32_t width,
                           const uint32_t *coneType, const uint32_t *cellTrial,
                           uint64_t trialFrameStride, float *R);
 
+/* The same for cells c0 .. c1-1 only (R of the other cells is not written),
+ * so that a caller can spread one frame's spatial work over several calls,
+ * e.g. over the steps of the frame period before the frame enters. */
+int vc_lgn_frame_response_cells(void *stream, const float *frame, uint32_t width,
+                                uint32_t height, uint32_t nLGN, uint32_t c0, uint32_t c1,
+                                uint32_t nSample1D, const float *SCx, const float *SCy,
+                                const float *SW, const uint32_t *coneType,
+                                const uint32_t *cellTrial, uint64_t trialFrameStride, float *R);
+
 /* The same for nFrames frames [nFrames][3][height][width] in one launch:
  * R[nFrames][nLGN][2]. */
 int vc_lgn_frame_responses(void *stream, const float *frames, uint32_t nFrames,

@@ -149,6 +149,8 @@ SIGNATURES = {
                           C.POINTER(RecConn), C.POINTER(V1Adjoint), u32, u32],
     "vc_lgn_precompute": [vp, C.POINTER(LgnSetup), vp, vp, vp, vp, vp],
     "vc_lgn_frame_response": [vp, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, C.c_uint64, vp],
+    "vc_lgn_frame_response_cells": [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp, vp, vp, vp,
+                                    C.c_uint64, vp],
     "vc_lgn_steps": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp, vp,
                      C.POINTER(FFEvents)],
     "vc_lgn_init": [vp, C.POINTER(LgnParams), vp, vp],

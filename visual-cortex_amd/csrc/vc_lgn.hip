@@ -51,13 +51,13 @@ __device__ void block_reduce_ref2(float &x, float &y, float *part /* >= 64 float
 }
 
 __global__ __launch_bounds__(1024) void k_lgn_frame_response(
-    const float *__restrict__ frame0, int W, int H, uint32_t nLGN,
+    const float *__restrict__ frame0, int W, int H, uint32_t nLGN, uint32_t c0,
     const float *__restrict__ SCx, const float *__restrict__ SCy,
     const float *__restrict__ SW, const uint32_t *__restrict__ coneType,
     const uint32_t *__restrict__ cellTrial, uint64_t trialFrameStride,
     uint64_t frameStride, float *__restrict__ R) {
     __shared__ float part[64];
-    const uint32_t c = blockIdx.x;
+    const uint32_t c = c0 + blockIdx.x;
     // blockIdx.y: frame of a batch of frames frameStride floats apart
     frame0 += (size_t)blockIdx.y * frameStride;
     R += (size_t)blockIdx.y * nLGN * 2;
@@ -94,15 +94,15 @@ __global__ __launch_bounds__(1024) void k_lgn_frame_response(
 // additions are the same, so the results are identical; there is no barrier.
 template <int kPer>
 __global__ __launch_bounds__(256) void k_lgn_frame_response_w(
-    const float *__restrict__ frame0, int W, int H, uint32_t nLGN,
+    const float *__restrict__ frame0, int W, int H, uint32_t nLGN, uint32_t c0, uint32_t c1,
     const float *__restrict__ SCx, const float *__restrict__ SCy,
     const float *__restrict__ SW, const uint32_t *__restrict__ coneType,
     const uint32_t *__restrict__ cellTrial, uint64_t trialFrameStride,
     uint64_t frameStride, float *__restrict__ R) {
     constexpr uint32_t nS = 64u * kPer, nW = 2u * kPer;
-    const uint32_t c = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
-                                                      (threadIdx.x >> 6));
-    if (c >= nLGN) return;                    // whole wavefronts
+    const uint32_t c = c0 + __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                           (threadIdx.x >> 6));
+    if (c >= c1) return;                      // whole wavefronts
     const uint32_t lane = threadIdx.x & 63u;
     frame0 += (size_t)blockIdx.y * frameStride;
     R += (size_t)blockIdx.y * nLGN * 2;
@@ -160,18 +160,21 @@ __global__ __launch_bounds__(256) void k_lgn_frame_response_w(
     }
 }
 
-// Launch of the frame response for nFrames frames frameStride floats apart:
-// one wavefront per cell when nS is a multiple of 64, else one workgroup.
+// Launch of the frame response of cells c0 .. c1-1 for nFrames frames
+// frameStride floats apart: one wavefront per cell when nS is a multiple of
+// 64, else one workgroup.
 hipError_t launch_frame_response(hipStream_t st, uint32_t nS, uint32_t nFrames,
                                  const float *frame, uint32_t width, uint32_t height,
-                                 uint32_t nLGN, const float *SCx, const float *SCy,
-                                 const float *SW, const uint32_t *coneType,
+                                 uint32_t nLGN, uint32_t c0, uint32_t c1, const float *SCx,
+                                 const float *SCy, const float *SW, const uint32_t *coneType,
                                  const uint32_t *cellTrial, uint64_t trialFrameStride,
                                  uint64_t frameStride, float *R) {
-    const dim3 grid((nLGN + 3u) / 4u, nFrames), block(256);
+    if (c1 <= c0) return hipSuccess;
+    const uint32_t nc = c1 - c0;
+    const dim3 grid((nc + 3u) / 4u, nFrames), block(256);
 #define VC_FR(K)                                                                            \
     hipLaunchKernelGGL((k_lgn_frame_response_w<K>), grid, block, 0, st, frame, (int)width,  \
-                       (int)height, nLGN, SCx, SCy, SW, coneType, cellTrial,               \
+                       (int)height, nLGN, c0, c1, SCx, SCy, SW, coneType, cellTrial,       \
                        trialFrameStride, frameStride, R)
     switch (nS) {
     case 64: VC_FR(1); break;
@@ -180,8 +183,8 @@ hipError_t launch_frame_response(hipStream_t st, uint32_t nS, uint32_t nFrames,
     case 512: VC_FR(8); break;
     case 1024: VC_FR(16); break;
     default:
-        hipLaunchKernelGGL(k_lgn_frame_response, dim3(nLGN, nFrames), dim3(nS), 0, st, frame,
-                           (int)width, (int)height, nLGN, SCx, SCy, SW, coneType, cellTrial,
+        hipLaunchKernelGGL(k_lgn_frame_response, dim3(nc, nFrames), dim3(nS), 0, st, frame,
+                           (int)width, (int)height, nLGN, c0, SCx, SCy, SW, coneType, cellTrial,
                            trialFrameStride, frameStride, R);
     }
 #undef VC_FR
@@ -589,11 +592,12 @@ extern "C" int vc_lgn_precompute(void *stream, const vc_lgn_setup *q, float *TW,
     return VC_OK;
 }
 
-extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t width,
-                                     uint32_t height, uint32_t nLGN, uint32_t nSample1D,
-                                     const float *SCx, const float *SCy, const float *SW,
-                                     const uint32_t *coneType, const uint32_t *cellTrial,
-                                     uint64_t trialFrameStride, float *R) {
+extern "C" int vc_lgn_frame_response_cells(void *stream, const float *frame, uint32_t width,
+                                           uint32_t height, uint32_t nLGN, uint32_t c0,
+                                           uint32_t c1, uint32_t nSample1D, const float *SCx,
+                                           const float *SCy, const float *SW,
+                                           const uint32_t *coneType, const uint32_t *cellTrial,
+                                           uint64_t trialFrameStride, float *R) {
     const uint32_t nS = nSample1D * nSample1D;
     if (!frame || !SCx || !SCy || !SW || !coneType || !R) {
         vc_set_error("vc_lgn_frame_response: null argument");
@@ -607,10 +611,23 @@ extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t 
         vc_set_error("vc_lgn_frame_response: empty frame or LGN");
         return VC_E_INVALID;
     }
+    if (c0 > c1 || c1 > nLGN) {
+        vc_set_error("vc_lgn_frame_response: cells %u..%u outside 0..%u", c0, c1, nLGN);
+        return VC_E_INVALID;
+    }
     VC_HIP_CHECK(launch_frame_response((hipStream_t)stream, nS, 1u, frame, width, height, nLGN,
-                                       SCx, SCy, SW, coneType, cellTrial, trialFrameStride,
-                                       (uint64_t)0, R));
+                                       c0, c1, SCx, SCy, SW, coneType, cellTrial,
+                                       trialFrameStride, (uint64_t)0, R));
     return VC_OK;
+}
+
+extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t width,
+                                     uint32_t height, uint32_t nLGN, uint32_t nSample1D,
+                                     const float *SCx, const float *SCy, const float *SW,
+                                     const uint32_t *coneType, const uint32_t *cellTrial,
+                                     uint64_t trialFrameStride, float *R) {
+    return vc_lgn_frame_response_cells(stream, frame, width, height, nLGN, 0u, nLGN, nSample1D,
+                                       SCx, SCy, SW, coneType, cellTrial, trialFrameStride, R);
 }
 
 extern "C" int vc_lgn_frame_responses(void *stream, const float *frames, uint32_t nFrames,
@@ -633,8 +650,8 @@ extern "C" int vc_lgn_frame_responses(void *stream, const float *frames, uint32_
     if (nFrames == 0) return VC_OK;
     const uint64_t fs = 3ull * width * height;
     VC_HIP_CHECK(launch_frame_response((hipStream_t)stream, nS, nFrames, frames, width, height,
-                                       nLGN, SCx, SCy, SW, coneType, nullptr, (uint64_t)0, fs,
-                                       R));
+                                       nLGN, 0u, nLGN, SCx, SCy, SW, coneType, nullptr,
+                                       (uint64_t)0, fs, R));
     return VC_OK;
 }
 
@@ -649,9 +666,9 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
         vc_set_error("nKernelSample %u unsupported (1..%d)", p->nKernelSample, kMaxTaps);
         return VC_E_UNSUPPORTED;
     }
-    if (p->maxFrame == 0 || p->maxFrame > (uint32_t)kMaxFrame || p->ntPerFrame == 0) {
-        vc_set_error("maxFrame %u unsupported (1..%d)", p->maxFrame, kMaxFrame);
-        return VC_E_UNSUPPORTED;
+    if (p->maxFrame == 0 || p->ntPerFrame == 0) {
+        vc_set_error("maxFrame %u / ntPerFrame %u must be positive", p->maxFrame, p->ntPerFrame);
+        return VC_E_INVALID;
     }
     // one wavefront covers 64 steps: its taps span at most kMaxFrame + 1 frames
     if ((63u + p->nRetrace) / p->ntPerFrame + 1u > (uint32_t)kMaxFrame) {

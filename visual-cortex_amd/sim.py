@@ -2,11 +2,14 @@
 tensor on the GPU and calls libvc_hip.so through its C ABI.
 
 The step order follows the reference's host loop (src/patch.cu:5966-6546):
-a new stimulus frame enters every ntPerFrame steps; the LGN stage for a run of
-steps only depends on the stimulus, so it is computed one frame period at a
-time (vc_lgn_steps) before the V1 steps of that period; each V1 step
-(vc_v1_step) applies the previous step's recurrent increments, updates every
-neuron and compacts its spikes.
+a new stimulus frame enters every ntPerFrame steps; the LGN stage of a run of
+steps only depends on the stimulus, so it is issued on its own stream for
+exactly the steps of each V1 launch, one launch ahead (vc_lgn_steps), and the
+spatial response of the frame that enters at the next period boundary is
+computed in slices over the steps of the period before it
+(vc_lgn_frame_response_cells), so every step carries the same share of the
+LGN work; each V1 step (vc_v1_step) applies the previous step's recurrent
+increments, updates every neuron and compacts its spikes.
 """
 from __future__ import annotations
 
@@ -124,13 +127,17 @@ class GpuSim:
         nbytes = int(self.lib.vc_v1_persistent_scratch_bytes(C.byref(self.p)))
         self.scratch = torch.zeros(nbytes // 4, dtype=torch.int32, device=d)
         self.epoch = 0                   # of the last persistent launch (1..65535, cycling)
+        self.n_v1_launches = 0           # V1 step kernels dispatched by run()
         self.status = torch.zeros(1, dtype=torch.int32, device=d)
         # ---- streams (default priority) and the LGN pipeline state
         self.s_v1 = torch.cuda.current_stream(d)
         self.s_lgn = torch.cuda.Stream(device=d)
-        self.lgn_done = [None, None]     # event: LGN period in buffer i computed
+        self.lgn_evq = []                # (end step, event) of issued LGN chunks
         self.v1_done = [None, None]      # event: V1 steps reading buffer i finished
-        self.lgn_next = 0                # next frame period whose LGN stage is due
+        self.lgn_to = 0                  # the LGN stage is issued for steps < lgn_to
+        # frames < fr_frame have their spatial response in the ring; frame
+        # fr_frame has its first fr_cells cells
+        self.fr_frame, self.fr_cells = 0, 0
         # ---- LGN
         self._setup_lgn()
         self.it = 0
@@ -320,9 +327,14 @@ class GpuSim:
         for c in range(3):
             init[c] = net.init_lms[c]
         self.init_frame = torch.as_tensor(init, device=d)
-        self.Rring = torch.zeros(tm["maxFrame"], sp.nLGN, 2, **f32)
+        # the ring holds one frame more than the temporal kernel reaches back
+        # over, so the next frame's response can be built while the current
+        # period's steps still read the oldest one
+        self.ringF = tm["maxFrame"] + 1
+        self.lp.maxFrame = self.ringF
+        self.Rring = torch.zeros(self.ringF, sp.nLGN, 2, **f32)
         self._frame_response(self.init_frame, 0)          # on the current stream
-        for s in range(1, tm["maxFrame"]):
+        for s in range(1, self.ringF):
             self.Rring[s].copy_(self.Rring[0])
         # double buffers: the LGN stage of frame period k+1 runs on its own
         # stream while the V1 steps of period k run
@@ -343,68 +355,124 @@ class GpuSim:
     def _hs(self, stream) -> C.c_void_p:
         return C.c_void_p(stream.cuda_stream)
 
-    def _lgn_period(self, k: int):
-        """Issue, on the LGN stream, the frame entering at period k and the LGN
-        steps of period k into buffer k % 2."""
-        tm = self.tm
-        npf = tm["ntPerFrame"]
-        it = k * npf
-        buf = k % 2
-        st = self.s_lgn
-        if self.v1_done[buf] is not None:
-            st.wait_event(self.v1_done[buf])           # V1 of period k-2 released it
-        fidx = k
-        sp = self.net.spec
+    def _frame_of(self, fidx: int):
+        """(frame tensor, cellTrial pointer) of stimulus frame fidx (the file
+        loops at its end, as the reference's reader does)."""
         if self.trialFrameStride:
-            frame = self.frames[0, fidx % self.nF]
-            ctr = ptr(self.bt["cellTrial"])
-        else:
-            frame = self.frames[fidx % self.nF]
-            ctr = None
-        check(self.lib.vc_lgn_frame_response(self._hs(st), ptr(frame), sp.width, sp.height,
-                                             sp.nLGN, sp.nSpatialSample1D, ptr(self.SCx),
-                                             ptr(self.SCy), ptr(self.SW), ptr(self.coneType),
-                                             ctr, self.trialFrameStride,
-                                             ptr(self.Rring[fidx % tm["maxFrame"]])),
-              "vc_lgn_frame_response")
-        with torch.cuda.stream(st):
-            self.lgn_at[buf][0].copy_(self.lTR)
-            self.lgn_at[buf][1].copy_(self.lNL)
-        ev = C.byref(self.ev2[buf]) if self.ev is not None else None
+            return self.frames[0, fidx % self.nF], ptr(self.bt["cellTrial"])
+        return self.frames[fidx % self.nF], None
+
+    def _frame_cells(self, stream, fidx: int, c0: int, c1: int):
+        """Spatial responses of cells c0 .. c1-1 to frame fidx, into its ring slot."""
+        if c1 <= c0:
+            return
+        sp = self.net.spec
+        frame, ctr = self._frame_of(fidx)
+        check(self.lib.vc_lgn_frame_response_cells(
+            self._hs(stream), ptr(frame), sp.width, sp.height, sp.nLGN, c0, c1,
+            sp.nSpatialSample1D, ptr(self.SCx), ptr(self.SCy), ptr(self.SW), ptr(self.coneType),
+            ctr, self.trialFrameStride, ptr(self.Rring[fidx % self.ringF])),
+            "vc_lgn_frame_response_cells")
+
+    def _lgn_chunk(self, a: int, b: int):
+        """Issue, on the LGN stream, the LGN stage of steps a .. b-1 (inside one
+        frame period k, a = lgn_to): the rest of frame k's spatial response if
+        it is not complete, the temporal part and the spike draw of the steps
+        into rows a..b-1 (mod ntPerFrame) of buffer k % 2, and the share
+        (b - a)/ntPerFrame of frame k+1's spatial response."""
+        npf = self.tm["ntPerFrame"]
+        sp = self.net.spec
+        nL = sp.nLGN
+        k, a_in = divmod(a, npf)
+        b_in = b - k * npf
+        if a != self.lgn_to or not 0 < b_in <= npf or b <= a:
+            raise ValueError(f"LGN chunk {a}..{b} out of order (issued up to {self.lgn_to})")
+        st = self.s_lgn
+        buf = k % 2
+        if a_in == 0:
+            if self.v1_done[buf] is not None:
+                st.wait_event(self.v1_done[buf])       # V1 of period k-2 released it
+            with torch.cuda.stream(st):
+                self.lgn_at[buf][0].copy_(self.lTR)
+                self.lgn_at[buf][1].copy_(self.lNL)
+        if self.fr_frame == k:                          # pipeline start: frame k whole
+            self._frame_cells(st, k, self.fr_cells, nL)
+            self.fr_frame, self.fr_cells = k + 1, 0
+        ev = None
+        if self.ev is not None:
+            e2 = self.ev2[buf]
+            ev = C.byref(abi.FFEvents(e2.revOff, e2.revTgt, e2.mask + a_in * sp.nV1 * 4, e2.nV1))
         check(self.lib.vc_lgn_steps(self._hs(st), C.byref(self.lp), C.byref(self.ls),
-                                    ptr(self.Rring), it, npf, ptr(self.lgn_sInfo2[buf]),
-                                    ptr(self.lgn_fr2[buf]), ev),
+                                    ptr(self.Rring), a, b - a,
+                                    ptr(self.lgn_sInfo2[buf]) + a_in * nL * 4,
+                                    ptr(self.lgn_fr2[buf]) + a_in * nL * 4, ev),
               "vc_lgn_steps")
+        c1 = b_in * nL // npf
+        if self.fr_frame == k + 1 and c1 > self.fr_cells:
+            self._frame_cells(st, k + 1, self.fr_cells, c1)
+            self.fr_cells = c1
+            if c1 == nL:
+                self.fr_frame, self.fr_cells = k + 2, 0
         e = torch.cuda.Event()
         e.record(st)
-        self.lgn_done[buf] = e
+        self.lgn_evq.append((b, e))
+        self.lgn_to = b
+
+    def _lgn_period(self, k: int):
+        """Issue the LGN stage of the whole frame period k (it must be next)."""
+        npf = self.tm["ntPerFrame"]
+        self._lgn_chunk(k * npf, (k + 1) * npf)
+
+    def _lgn_issue_to(self, end: int):
+        """Issue the LGN stage up to step `end` in chunks ending at period
+        boundaries."""
+        npf = self.tm["ntPerFrame"]
+        while self.lgn_to < end:
+            a = self.lgn_to
+            self._lgn_chunk(a, min(end, (a // npf + 1) * npf))
+
+    def _lgn_wait(self, stream, end: int):
+        """Make `stream` wait for the LGN chunk that covers step end-1."""
+        while self.lgn_evq and self.lgn_evq[0][0] < end:
+            self.lgn_evq.pop(0)
+        if not self.lgn_evq:
+            raise RuntimeError(f"LGN stage of step {end - 1} not issued")
+        stream.wait_event(self.lgn_evq[0][1])
 
     def run(self, nsteps: int, out: torch.Tensor | None = None, events: list | None = None):
         """Advance `nsteps` steps.  V1 sInfo of step i goes to out[i] if `out`
         ([nsteps, nV1] float32 on the device) is given, else to the history
         when recording, else to a scratch row.  If `events` is a list, a
         (start, end, n) triple of timing events bracketing each run of n V1
-        step launches is appended (on the V1 stream).  The LGN stage runs one
-        frame period ahead on its own stream."""
+        step launches is appended (on the V1 stream).  The LGN stage of each V1
+        launch's steps is issued on its own stream one launch ahead, within
+        the call: the call issues the LGN work of exactly its own steps (when
+        recording, of whole frame periods, which the LGN history keeps)."""
         sp = self.net.spec
         npf = self.tm["ntPerFrame"]
         if out is None and self.record:
             out = torch.empty(nsteps, sp.nV1, dtype=torch.float32, device=self.dev)
             self.sInfo_hist.append(out)
+        end_all = self.it + nsteps
+
+        def lgn_end(a):                # LGN steps a V1 launch starting at a needs
+            e = min(end_all, (a // npf + 1) * npf)
+            return (a // npf + 1) * npf if self.record else e
+
         done = 0
         sv = self.s_v1
         while done < nsteps:
             it = self.it
             k = it // npf
             k_in = it % npf
-            while self.lgn_next <= k + 1:              # this period and the next
-                self._lgn_period(self.lgn_next)
-                self.lgn_next += 1
+            n = min(npf - k_in, nsteps - done)
+            self._lgn_issue_to(lgn_end(it))
+            if it + n < end_all:                       # one launch ahead
+                self._lgn_issue_to(lgn_end(it + n))
             buf = k % 2
-            sv.wait_event(self.lgn_done[buf])
+            self._lgn_wait(sv, it + n if not self.record else (k + 1) * npf)
             if k_in == 0 and self.record:
                 self.lgn_hist.append(self.lgn_sInfo2[buf].clone())
-            n = min(npf - k_in, nsteps - done)
             if out is not None:
                 optr, ostride = ptr(out) + done * sp.nV1 * 4, sp.nV1
                 self.sInfo_last = out[done + min(npf - k_in, nsteps - done) - 1]
@@ -460,6 +528,7 @@ class GpuSim:
                     check(rc, "vc_v1_run_persistent")
             if not self.persistent:
                 check(self.lib.vc_v1_run(*args), "vc_v1_run")
+            self.n_v1_launches += 1 if self.persistent else n
             if events is not None:
                 e1 = torch.cuda.Event(enable_timing=True)
                 e1.record(sv)
@@ -503,11 +572,9 @@ class GpuSim:
         for i in range(nsteps):
             it = self.it
             k, k_in = divmod(it, npf)
-            while self.lgn_next <= k + 1:
-                self._lgn_period(self.lgn_next)
-                self.lgn_next += 1
+            self._lgn_issue_to((k + 2) * npf)           # this period and the next
             buf = k % 2
-            self.s_v1.wait_event(self.lgn_done[buf])
+            self._lgn_wait(self.s_v1, (k + 1) * npf)
             if k_in == 0:
                 n_here = min(npf, nsteps - i)
                 if fr_w:
@@ -566,19 +633,11 @@ class GpuSim:
         torch.cuda.synchronize()
         K = it0 // npf
         self._frame_response(self.init_frame, 0)
-        for slot in range(1, maxF):
+        for slot in range(1, self.ringF):
             self.Rring[slot].copy_(self.Rring[0])
+        cur = torch.cuda.current_stream(self.dev)
         for fidx in range(max(0, K - maxF + 1), K):
-            if self.trialFrameStride:
-                frame, ctr = self.frames[0, fidx % self.nF], ptr(self.bt["cellTrial"])
-            else:
-                frame, ctr = self.frames[fidx % self.nF], None
-            check(self.lib.vc_lgn_frame_response(None, ptr(frame), sp.width, sp.height, sp.nLGN,
-                                                 sp.nSpatialSample1D, ptr(self.SCx),
-                                                 ptr(self.SCy), ptr(self.SW), ptr(self.coneType),
-                                                 ctr, self.trialFrameStride,
-                                                 ptr(self.Rring[fidx % maxF])),
-                  "vc_lgn_frame_response")
+            self._frame_cells(cur, fidx, 0, sp.nLGN)
         if self.ev is not None:
             for m in self.ffMask2:
                 m.zero_()
@@ -588,8 +647,9 @@ class GpuSim:
             self.oh[i].zero_()
         self.cur = 0
         self.it = it0
-        self.lgn_next = K
-        self.lgn_done = [None, None]
+        self.lgn_to = it0
+        self.fr_frame, self.fr_cells = K, 0
+        self.lgn_evq = []
         self.v1_done = [None, None]
         torch.cuda.synchronize()
         self.s_lgn.wait_stream(self.s_v1)

@@ -175,7 +175,8 @@ def save(sim, path) -> dict:
     K = sim.it // npf
     sim.finalize()                          # pending increments into the state
     torch.cuda.synchronize()
-    if K == 0:
+    if sim.lgn_to <= sim.it:
+        # nothing of period K issued yet: the live LGN state is its start
         lgn = torch.stack([sim.lTR, sim.lNL])
     else:
         lgn = sim.lgn_at[K % 2]             # the LGN state at the start of period K

@@ -62,9 +62,8 @@ def tune_step(sim, steps: int = 64, reps: int = 2, verbose: bool = False) -> tup
     lib, d = sim.lib, sim.dev
     # the first frame period's LGN output, then private copies of everything
     # a V1 step writes
-    if sim.lgn_next == 0:
+    if sim.lgn_to == 0:
         sim._lgn_period(0)
-        sim.lgn_next = 1
     torch.cuda.synchronize()
     lgn = sim.lgn_sInfo2[0].clone()
     mask0 = sim.ffMask2[0].clone() if sim.ev is not None else None
