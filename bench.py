@@ -347,7 +347,11 @@ def bench_batched(args) -> dict:
     K = args.trial_steps
     specs = batch.c3_specs(ntrial=args.trials, nt=K)
     t0 = time.time()
-    nets = [batch.build_network_fast(s, device="cuda") for s in specs]
+    nets = []
+    for i, s in enumerate(specs):
+        nets.append(batch.build_network_fast(s, device="cuda"))
+        if i % 8 == 7:
+            log(f"[bench] built {i + 1} of {len(specs)} trial networks ({time.time() - t0:.1f}s)")
     bnet = batch.batch_networks(nets)
     log(f"[bench] {len(nets)} trials, blocks {[s.nblock for s in specs]}, nV1={bnet.spec.nV1}, "
         f"built in {time.time() - t0:.1f}s")
