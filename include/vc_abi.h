@@ -217,6 +217,30 @@ int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const vc_v1_state 
 /* Bytes of the scratch buffer of vc_v1_run_persistent. */
 uint64_t vc_v1_persistent_scratch_bytes(const vc_v1_params *p);
 
+/* vc_v1_run_persistent without a scratch argument: the call allocates
+ * nothing and takes no workspace.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * tagged spike records live in a static device array inside the library,
+ * one per GPU (8.5 MiB, held until the process ends), which limits nV1 to
+ * vc_v1_run_persistent_noalloc_max_v1() = 131072 neurons; larger networks
+ * get VC_E_INVALID.  The library cycles the epoch itself, per GPU (the first
+ * call on a GPU and every 65535th after it clear the array with a kernel).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other host threads) race and their results are invalid;
+ * calls one after the other are safe.  `stream` must belong to the current
+ * device.  Inputs, outputs, *status and the return codes are those of
+ * vc_v1_run_persistent, whose results it reproduces bit for bit. */
+int vc_v1_run_persistent_noalloc(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                                 const vc_ff_conn *ff, const vc_rec_conn *rec,
+                                 const vc_spikes *spk2, uint32_t cur, const float *lgnSInfo,
+                                 uint64_t lgnStride, uint32_t *ffMask, uint64_t maskStride,
+                                 uint32_t it0, uint32_t nsteps, float *out_sInfo,
+                                 uint64_t outStride, uint32_t *status);
+
+/* The largest nV1 vc_v1_run_persistent_noalloc accepts. */
+uint32_t vc_v1_run_persistent_noalloc_max_v1(void);
+
 /* Apply the recurrent increments of `prev` to gE/hE/gI/hI without stepping
  * (used after the last step, before reading or saving the state). */
 int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,

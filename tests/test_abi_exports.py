@@ -9,7 +9,8 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def declared_functions():
     text = (ROOT / "include" / "vc_abi.h").read_text()
-    return sorted(set(re.findall(r"^(?:int|uint64_t|const char \*)\s*(vc_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|uint64_t|uint32_t|const char \*)\s*(vc_\w+)\s*\(", text,
+                                 re.M)))
 
 
 def test_header_declares_entry_points():

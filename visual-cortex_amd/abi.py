@@ -128,6 +128,11 @@ SIGNATURES = {
                              C.POINTER(RecConn), C.POINTER(Spikes), u32, vp, C.c_uint64, vp,
                              C.c_uint64, u32, u32, vp, C.c_uint64, vp, u32, vp],
     "vc_v1_persistent_scratch_bytes": [C.POINTER(V1Params)],
+    "vc_v1_run_persistent_noalloc": [vp, C.POINTER(V1Params), C.POINTER(V1State),
+                                     C.POINTER(FFConn), C.POINTER(RecConn), C.POINTER(Spikes),
+                                     u32, vp, C.c_uint64, vp, C.c_uint64, u32, u32, vp,
+                                     C.c_uint64, vp],
+    "vc_v1_run_persistent_noalloc_max_v1": [],
     "vc_v1_dep_current": [vp, C.POINTER(V1Params), C.POINTER(V1State), vp],
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],
@@ -191,7 +196,8 @@ def load_library(require_gpu: bool = True):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = (C.c_char_p if name == "vc_last_error" else
-                      C.c_uint64 if name.endswith("_bytes") else C.c_int)
+                      C.c_uint64 if name.endswith("_bytes") else
+                      C.c_uint32 if name.endswith("_max_v1") else C.c_int)
     _LIB = lib
     return lib
 

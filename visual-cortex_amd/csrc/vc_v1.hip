@@ -21,6 +21,8 @@
 //            increments at slot tile*64 + rank (src/coredynamics.cu:1994-2046).
 #include "vc_v1_device.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int kWG = 256;           // max workgroup size
@@ -1284,34 +1286,39 @@ extern "C" uint64_t vc_v1_persistent_scratch_bytes(const vc_v1_params *p) {
     return 2ull * nTiles * (kRecW + 64u * 4u) * sizeof(uint64_t);
 }
 
-extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const vc_v1_state *s,
-                                    const vc_ff_conn *ff, const vc_rec_conn *rec,
-                                    const vc_spikes *spk2, uint32_t cur, const float *lgnSInfo,
-                                    uint64_t lgnStride, uint32_t *ffMask, uint64_t maskStride,
-                                    uint32_t it0, uint32_t nsteps, float *out_sInfo,
-                                    uint64_t outStride, void *scratch, uint32_t epoch,
-                                    uint32_t *status) {
+namespace {
+
+// The persistent launch behind vc_v1_run_persistent and its allocation-free
+// variant: `words` is the record buffer, `clearWords32` the number of 32-bit
+// words cleared (by a fill kernel, so that the call may be captured) when
+// epoch is 1.
+int run_persistent(const char *who, void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                   const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
+                   uint32_t cur, const float *lgnSInfo, uint64_t lgnStride, uint32_t *ffMask,
+                   uint64_t maskStride, uint32_t it0, uint32_t nsteps, float *out_sInfo,
+                   uint64_t outStride, uint64_t *words, uint32_t epoch, uint64_t clearWords32,
+                   uint32_t *status) {
     int rc = check_params(p, rec);
     if (rc) return rc;
-    if (!s || !ff || !rec || !spk2 || !lgnSInfo || !out_sInfo || !scratch || !status || cur > 1) {
-        vc_set_error("vc_v1_run_persistent: bad argument");
+    if (!s || !ff || !rec || !spk2 || !lgnSInfo || !out_sInfo || !words || !status || cur > 1) {
+        vc_set_error("%s: bad argument", who);
         return VC_E_INVALID;
     }
     if (spk2[0].mask == spk2[1].mask) {
-        vc_set_error("vc_v1_run_persistent: the two spike buffers must differ");
+        vc_set_error("%s: the two spike buffers must differ", who);
         return VC_E_INVALID;
     }
     if (ffMask && ff->maxLGN > 32u) {
-        vc_set_error("vc_v1_run_persistent: event-driven LGN input needs maxLGN <= 32");
+        vc_set_error("%s: event-driven LGN input needs maxLGN <= 32", who);
         return VC_E_INVALID;
     }
     if (rec->maxNear > 64u) {
-        vc_set_error("vc_v1_run_persistent: more than 64 neighbour blocks");
+        vc_set_error("%s: more than 64 neighbour blocks", who);
         return VC_E_UNSUPPORTED;
     }
     if (epoch == 0 || epoch > 0xffffu || nsteps > 0xfffdu) {
-        vc_set_error("vc_v1_run_persistent: epoch %u (1..65535) or %u steps (<= 65533) out of "
-                     "range", epoch, nsteps);
+        vc_set_error("%s: epoch %u (1..65535) or %u steps (<= 65533) out of range", who, epoch,
+                     nsteps);
         return VC_E_INVALID;
     }
     if (nsteps == 0) return VC_OK;
@@ -1328,24 +1335,24 @@ extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const v
     VC_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     VC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, (int)wg, 0));
     if ((uint64_t)per * (uint64_t)ncu < grid) {
-        vc_set_error("vc_v1_run_persistent: %u workgroups cannot be co-resident (%d per CU x %d "
-                     "CUs); use vc_v1_run", grid, per, ncu);
+        vc_set_error("%s: %u workgroups cannot be co-resident (%d per CU x %d CUs); use "
+                     "vc_v1_run", who, grid, per, ncu);
         return VC_E_UNSUPPORTED;
     }
-    if (((uintptr_t)scratch & 63u) != 0) {
-        vc_set_error("vc_v1_run_persistent: scratch must be 64-byte aligned");
+    if (((uintptr_t)words & 63u) != 0) {
+        vc_set_error("%s: scratch must be 64-byte aligned", who);
         return VC_E_INVALID;
     }
-    uint64_t *words = (uint64_t *)scratch;
     const uint64_t nWords32 = vc_v1_persistent_scratch_bytes(p) / 4u;
-    if (nWords32 >= (1ull << 32) || 2ull * (p->nV1 / 64u) * (kRecW + 256u) >= (1ull << 32)) {
-        vc_set_error("vc_v1_run_persistent: nV1 %u too large for 32-bit record offsets", p->nV1);
+    if (nWords32 >= (1ull << 32) || clearWords32 >= (1ull << 32) ||
+        2ull * (p->nV1 / 64u) * (kRecW + 256u) >= (1ull << 32)) {
+        vc_set_error("%s: nV1 %u too large for 32-bit record offsets", who, p->nV1);
         return VC_E_UNSUPPORTED;
     }
     hipStream_t st = (hipStream_t)stream;
     if (epoch == 1) {               // a new cycle of epochs: no tag of an earlier one may match
-        hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nWords32 + 255) / 256)), dim3(256), 0,
-                           st, (uint32_t *)scratch, (uint32_t)nWords32, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((clearWords32 + 255) / 256)), dim3(256),
+                           0, st, (uint32_t *)words, (uint32_t)clearWords32, 0u);
         VC_HIP_CHECK(hipGetLastError());
     }
     // An ordinary launch: every workgroup fits on the device at once (checked
@@ -1365,6 +1372,83 @@ extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const v
 #undef VC_MULTI
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
+}
+
+// ---- records of the allocation-free persistent launch ----------------------
+// A file-scope device array: every GPU the library runs on holds its own copy
+// until the process ends.  Host code never passes its name to a kernel (on
+// the host it is only a stand-in symbol); it looks up the device address per
+// GPU once with hipGetSymbolAddress and uses that.
+constexpr uint32_t kNoAllocMaxV1 = 131072u;            // 128 blocks of 1024 neurons
+constexpr uint64_t kNoAllocWords = 2ull * (kNoAllocMaxV1 / 64u) * (kRecW + 64u * 4u);
+static __device__ __align__(256) uint64_t noalloc_scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+    uint64_t *words = nullptr;      // device address of noalloc_scratch on this GPU
+    uint32_t epoch = 0;             // of the last launch on this GPU (1..65535, cycling)
+};
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kMaxDevices];
+
+}  // namespace
+
+extern "C" int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                                    const vc_ff_conn *ff, const vc_rec_conn *rec,
+                                    const vc_spikes *spk2, uint32_t cur, const float *lgnSInfo,
+                                    uint64_t lgnStride, uint32_t *ffMask, uint64_t maskStride,
+                                    uint32_t it0, uint32_t nsteps, float *out_sInfo,
+                                    uint64_t outStride, void *scratch, uint32_t epoch,
+                                    uint32_t *status) {
+    if (!p) { vc_set_error("null params"); return VC_E_INVALID; }
+    return run_persistent("vc_v1_run_persistent", stream, p, s, ff, rec, spk2, cur, lgnSInfo,
+                          lgnStride, ffMask, maskStride, it0, nsteps, out_sInfo, outStride,
+                          (uint64_t *)scratch, epoch, vc_v1_persistent_scratch_bytes(p) / 4u,
+                          status);
+}
+
+extern "C" uint32_t vc_v1_run_persistent_noalloc_max_v1(void) { return kNoAllocMaxV1; }
+
+extern "C" int vc_v1_run_persistent_noalloc(void *stream, const vc_v1_params *p,
+                                            const vc_v1_state *s, const vc_ff_conn *ff,
+                                            const vc_rec_conn *rec, const vc_spikes *spk2,
+                                            uint32_t cur, const float *lgnSInfo,
+                                            uint64_t lgnStride, uint32_t *ffMask,
+                                            uint64_t maskStride, uint32_t it0, uint32_t nsteps,
+                                            float *out_sInfo, uint64_t outStride,
+                                            uint32_t *status) {
+    const char *who = "vc_v1_run_persistent_noalloc";
+    if (!p) { vc_set_error("%s: null params", who); return VC_E_INVALID; }
+    if (p->nV1 > kNoAllocMaxV1) {
+        vc_set_error("%s: nV1 %u exceeds the %u neurons its static record buffer holds; use "
+                     "vc_v1_run_persistent with a caller buffer", who, p->nV1, kNoAllocMaxV1);
+        return VC_E_INVALID;
+    }
+    int dev = 0;
+    VC_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        vc_set_error("%s: device ordinal %d beyond %d", who, dev, kMaxDevices);
+        return VC_E_UNSUPPORTED;
+    }
+    uint64_t *words;
+    uint32_t epoch;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        NoAllocDevice &d = g_noalloc[dev];
+        if (!d.words) {
+            void *addr = nullptr;
+            VC_HIP_CHECK(hipGetSymbolAddress(&addr, HIP_SYMBOL(noalloc_scratch)));
+            d.words = (uint64_t *)addr;
+        }
+        d.epoch = d.epoch % 0xffffu + 1u;
+        words = d.words;
+        epoch = d.epoch;
+    }
+    // epoch 1 clears the whole array: a later call with more neurons must not
+    // meet tags that an earlier cycle of epochs left beyond this call's records
+    return run_persistent(who, stream, p, s, ff, rec, spk2, cur, lgnSInfo, lgnStride, ffMask,
+                          maskStride, it0, nsteps, out_sInfo, outStride, words, epoch,
+                          kNoAllocWords * 2u, status);
 }
 
 extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,

@@ -31,7 +31,7 @@ class GpuSim:
     def __init__(self, net: Network, device: str = "cuda", record: bool = True,
                  state: dict | None = None, ff_events: bool | None = None,
                  persistent: bool = True, dense_weights: bool | None = None,
-                 tune: bool = False):
+                 tune: bool = False, noalloc: bool = False):
         self.lib = abi.load_library()
         self.net = net
         sp = net.spec
@@ -128,6 +128,9 @@ class GpuSim:
         self.scratch = torch.zeros(nbytes // 4, dtype=torch.int32, device=d)
         self.epoch = 0                   # of the last persistent launch (1..65535, cycling)
         self.n_v1_launches = 0           # V1 step kernels dispatched by run()
+        # persistent launches through the allocation-free entry point (its
+        # records in the library's static array instead of self.scratch)
+        self.noalloc = noalloc
         self.status = torch.zeros(1, dtype=torch.int32, device=d)
         # ---- streams (default priority) and the LGN pipeline state
         self.s_v1 = torch.cuda.current_stream(d)
@@ -518,7 +521,13 @@ class GpuSim:
                     ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, sp.nLGN, mptr, sp.nV1, it, n,
                     optr, ostride)
             rc = abi.VC_E_UNSUPPORTED
-            if self.persistent:
+            if self.persistent and self.noalloc:
+                rc = self.lib.vc_v1_run_persistent_noalloc(*args, ptr(self.status))
+                if rc == abi.VC_E_UNSUPPORTED:          # grid cannot be co-resident
+                    self.persistent = False
+                else:
+                    check(rc, "vc_v1_run_persistent_noalloc")
+            elif self.persistent:
                 self.epoch = self.epoch % 65535 + 1
                 rc = self.lib.vc_v1_run_persistent(*args, ptr(self.scratch), self.epoch,
                                                    ptr(self.status))
