@@ -173,6 +173,7 @@ class OracleSim:
         self.sInfo_hist = []
         self.lgn_hist = []
         self.fr_hist = []
+        self.convol_hist = []
 
     def precompute(self):
         from importlib import import_module
@@ -213,7 +214,10 @@ class OracleSim:
                                         _fp(t["SW"]), _fp(self.lgn["coneType"]), ct, stride, _fp(R))
         return R
 
-    def run(self, nsteps: int, record: bool = True):
+    def run(self, nsteps: int, record: bool = True, lgn_only: bool = False):
+        """nsteps steps.  lgn_only: the LGN stage alone (no V1 step); the
+        linear filter output (convol, before the rectifier) of every step is
+        kept in convol_hist, one array per frame period."""
         sp, tm = self.net.spec, self.tm
         npf = tm["ntPerFrame"]
         frames = self.net.frames
@@ -232,15 +236,21 @@ class OracleSim:
                 else:
                     self.Rring[fidx % tm["maxFrame"]] = self.frame_response(
                         frames[fidx % frames.shape[0]])
+                cv = np.zeros((npf, sp.nLGN), np.float32) if lgn_only else None
                 self.o.L.orc_lgn_steps(C.byref(self.lp), _fp(t["TW"]), _fp(kC), _fp(kS),
                                        _fp(t["maxConvol"]), _fp(self.lgn["c50"]),
                                        _fp(self.lgn["sharpness"]), _fp(self.lgn["logA"]),
                                        _fp(self.lgn["logB"]), _fp(self.lTR), _fp(self.lNL),
                                        _fp(self.Rring), it, npf, _fp(self.lgn_sInfo),
-                                       _fp(self.lgn_fr), None)
+                                       _fp(self.lgn_fr), _fp(cv))
                 if record:
                     self.lgn_hist.append(self.lgn_sInfo.copy())
                     self.fr_hist.append(self.lgn_fr.copy())
+                if lgn_only:
+                    self.convol_hist.append(cv)
+            if lgn_only:
+                self.it += 1
+                continue
             out = np.zeros(sp.nV1, np.float32)
             lgn_row = np.ascontiguousarray(self.lgn_sInfo[k_in])
             rc = self.o.L.orc_v1_step(C.byref(self.p), _fp(st["v"]), _fp(st["w"]), _fp(st["tBack"]),

@@ -1,0 +1,325 @@
+"""Independent pins of the oracle's LGN stage (CPU only).
+
+Each test fixes one function of the oracle against something that does not
+reuse its code:
+
+* tap -> frame: the reference's own bookkeeping -- the host loop's frame
+  counter, ring head and phase recurrence (src/patch.cu:5396-5422,
+  5966-6069, getTail) and the kernel's per-sample frame
+  (src/discrete_input_convol.cu:1030-1043, 1146-1149) -- restated step by step
+  below, against the frame whose response the oracle actually used at each
+  (step, tap), read off its filter output with one-hot temporal weights;
+* the temporal sum: a stimulus with a single structured frame gives, at every
+  step, the sum of the temporal weights of the taps that the reference
+  bookkeeping puts on that frame, times the frame's response;
+* the spatial response: float64 sums over scipy's bilinear interpolation on
+  frames built so that the mean luminance over the surround coordinates, the
+  contrast clip and the spatial weights each change the result by far more
+  than the tolerance;
+* the sample coordinates: the geometry of the flat-retina sample grid
+  (centre, spacing, orientation, extent) and the float64 formula;
+* maxConvol: a closed form (|kC| + |kS| for same-sign kernels and |cov| = 1)
+  and float64 sums over the reference's sample grid for covariant values in
+  [-1, 1].
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+
+def _net(pkg, **kw):
+    from visual_cortex_amd import network
+    return network.build_network(network.spec_tiny(**kw))
+
+
+# ------------------------------------------------------------ tap -> frame
+def reference_tap_frames(nRetrace, npf, nKS, interval, iT0, nsteps, denorm=1):
+    """[nsteps, nKS]: the stimulus frame (-1: the initial-luminance fill) that
+    the reference's LGN_convol_parvo samples for tap i at step it, from its
+    host bookkeeping and the kernel's frameNow, one patch of samples."""
+    maxFrame = (nRetrace * denorm + npf - 1) // npf + 1          # src/patch.cu:5043-5046
+    content = [-1] * maxFrame                                     # texture prefilled with init
+    currentFrame, iFrameHead, oldFrameHead = 0, 0, None
+    iFramePhaseHead = 0
+    parvoRemain = (nRetrace * denorm) % npf                       # :5399-5401
+    parvoComp = (npf - parvoRemain) % npf
+    iFramePhaseTail = parvoComp
+    out = np.zeros((nsteps, nKS), np.int64)
+    for it in range(nsteps):
+        if (it + 1) * denorm > currentFrame * npf:                # :5969, frame enters
+            content[iFrameHead] = currentFrame
+            currentFrame += 1
+            oldFrameHead = iFrameHead
+            iFrameHead = (iFrameHead + 1) % maxFrame
+        iFramePhaseTail = (iFramePhaseTail + denorm) % npf        # :6051-6052
+        iFramePhaseHead = (iFramePhaseHead + denorm) % npf
+        # getTail (:5413-5422) with the parvo head (:6068)
+        tail = oldFrameHead + 2 if iFramePhaseTail <= parvoComp else oldFrameHead + 1
+        old_currentFrame = tail                                   # kernel argument currentFrame
+        t = iT0                                                   # :1146-1149, first patch
+        for i in range(nKS):
+            frameNow = old_currentFrame + (t * denorm + iFramePhaseTail) // npf
+            out[it, i] = content[frameNow % maxFrame]
+            t += interval
+    return out
+
+
+TIMING_CASES = [
+    # (tau ms, nKernelSample, frameRate Hz)
+    (40.0, 40, 125),    # the configurations: nRetrace 320, interval 8, T0 4, 64 steps per frame
+    (37.5, 30, 125),    # nRetrace 300, not a multiple of 64; interval 10, T0 5
+    (35.0, 40, 100),    # nRetrace 280, interval 7 (odd: T0 0), 80 steps per frame
+    (40.0, 32, 250),    # 32 steps per frame, interval 10, T0 5, 11 frames in the ring
+]
+
+
+def _noise_frames(sp, n, seed):
+    rng = np.random.default_rng(seed)
+    return (0.2 + 0.6 * rng.random((n, 3, sp.height, sp.width))).astype(np.float32)
+
+
+@pytest.mark.parametrize("tau,nks,rate", TIMING_CASES)
+def test_tap_frame_map_follows_reference_bookkeeping(pkg, tau, nks, rate):
+    import oracle
+    from visual_cortex_amd import network
+    net = _net(pkg, tau=tau, nKernelSample=nks, frameRate=rate, nLGN_side=4)
+    sp = net.spec
+    tm = network.timing(sp)
+    npf, nR = tm["ntPerFrame"], tm["nRetrace"]
+    nsteps = nR + 4 * npf + npf // 2
+    nF = nsteps // npf + 3
+    net.frames = _noise_frames(sp, nF, seed=int(tau * 10) + nks)
+    net.lgn["k"] = np.stack([np.ones(sp.nLGN, np.float32), np.zeros(sp.nLGN, np.float32)])
+    ref = reference_tap_frames(nR, npf, nks, tm["kernelSampleInterval"], tm["iKernelSampleT0"],
+                               nsteps)
+    probe = sorted({0, 1, nks // 2, nks - 1})
+    for i in probe:
+        o = oracle.OracleSim(net, pkg)
+        R = {-1: o.frame_response(np.stack([np.full((sp.height, sp.width), v, np.float32)
+                                            for v in net.init_lms]))[:, 0]}
+        # a uniform init frame has no response: use the frames' own responses
+        for f in range(nF):
+            R[f] = o.frame_response(net.frames[f])[:, 0]
+        assert len({R[f].tobytes() for f in R}) == len(R), "frame responses not distinct"
+        TW = np.zeros_like(o.tables["TW"])
+        TW[:, 0, i] = 1.0
+        o.tables["TW"] = TW
+        o.run(nsteps, record=False, lgn_only=True)
+        cv = np.concatenate(o.convol_hist)[:nsteps]
+        lookup = {R[f].tobytes(): f for f in R}
+        used = np.array([lookup.get(cv[it].tobytes(), -99) for it in range(nsteps)])
+        assert (used != -99).all(), "the filter output is not one frame's response"
+        np.testing.assert_array_equal(used, ref[:, i], err_msg=f"tap {i}")
+    # the map is not trivial: several frames per step window, and pre-stimulus taps
+    assert (ref < 0).any() and len(np.unique(ref[-1])) >= 3
+
+
+def test_reference_bookkeeping_edge_rule():
+    """A sample exactly on a frame edge takes the earlier frame (the kernel's
+    comment at src/discrete_input_convol.cu:1147); the restatement shows it:
+    with T0 = 0 and one tap per frame, tap i of step it = k*npf - 1 + nRetrace
+    sits at the start of frame k + i - (nRetrace / npf)."""
+    npf, nR = 8, 16
+    ref = reference_tap_frames(nR, npf, 2, 8, 0, 6 * npf)
+    for it in range(2 * npf, 5 * npf):
+        for i in range(2):
+            s = it + 1 - nR + i * 8          # sample step relative to the stimulus start
+            assert ref[it, i] == (s // npf if s >= 0 else -1)
+
+
+def test_single_frame_impulse_temporal_sum(pkg):
+    """Every frame uniform (no response) except frame F: at each step the
+    filter output is kC * sum of TW over the taps on frame F * R_C(F) +
+    kS * (same, surround)."""
+    import oracle
+    from visual_cortex_amd import network
+    net = _net(pkg, nLGN_side=4)
+    sp = net.spec
+    tm = network.timing(sp)
+    npf, nR, nks = tm["ntPerFrame"], tm["nRetrace"], sp.nKernelSample
+    nsteps = nR + 4 * npf
+    nF = nsteps // npf + 2
+    F = 3
+    frames = np.full((nF, 3, sp.height, sp.width), 0.5, np.float32)
+    frames[F] = _noise_frames(sp, 1, seed=5)[0]
+    net.frames = frames
+    o = oracle.OracleSim(net, pkg)
+    RF = o.frame_response(frames[F]).astype(np.float64)
+    o.run(nsteps, record=False, lgn_only=True)
+    cv = np.concatenate(o.convol_hist)[:nsteps].astype(np.float64)
+    ref = reference_tap_frames(nR, npf, nks, tm["kernelSampleInterval"], tm["iKernelSampleT0"],
+                               nsteps)
+    TW = o.tables["TW"].astype(np.float64)                 # [cell][region][tap]
+    kC, kS = (net.lgn["k"][r].astype(np.float64) for r in (0, 1))
+    on = (ref == F).astype(np.float64)                     # [step][tap]
+    expect = (kC[None, :] * (on @ TW[:, 0, :].T) * RF[None, :, 0] +
+              kS[None, :] * (on @ TW[:, 1, :].T) * RF[None, :, 1])
+    assert np.abs(expect).max() > 1e-3
+    np.testing.assert_allclose(cv, expect, rtol=1e-5, atol=1e-6 * np.abs(expect).max())
+    # outside frame F's window the output is exactly zero
+    assert np.all(cv[on.sum(1) == 0] == 0.0)
+
+
+# ---------------------------------------------------------- spatial response
+def _bilinear64(img, x, y):
+    """Texture fetch with linear filtering, border colour 0 and normalised
+    coordinates (texel centres at (i + 0.5)/W), by scipy's linear spline."""
+    H, W = img.shape
+    return ndimage.map_coordinates(img.astype(np.float64), [y * H - 0.5, x * W - 0.5], order=1,
+                                   mode="grid-constant", cval=0.0)
+
+
+def _response64(frame, xC, yC, xS, yS, SW, mean_over="surround", clip=True):
+    """Float64 parvo spatial response of one cell for cone type (L+M)/2."""
+    I = lambda x, y: 0.5 * (_bilinear64(frame[0], x, y) + _bilinear64(frame[1], x, y))
+    IC, IS = I(xC, yC), I(xS, yS)
+    mean = IS.mean() if mean_over == "surround" else IC.mean()
+    c = lambda v: np.clip(v / mean - 1.0, -1.0, 1.0) if clip else v / mean - 1.0
+    return float((SW * c(IC)).sum()), float((SW * c(IS)).sum())
+
+
+def _disk_frame(sp, cx, cy, r, a, b):
+    H, W = sp.height, sp.width
+    X, Y = np.meshgrid((np.arange(W) + 0.5) / W, (np.arange(H) + 0.5) / H)
+    lum = np.where((X - cx) ** 2 + (Y - cy) ** 2 < r * r, a, b).astype(np.float32)
+    return np.stack([lum, lum, lum])
+
+
+@pytest.mark.parametrize("kind", ["disk", "saturating_spot", "noise"])
+def test_spatial_response_float64(pkg, kind):
+    import oracle
+    net = _net(pkg, width=96, height=96, field_deg=1.5)
+    sp = net.spec
+    o = oracle.OracleSim(net, pkg)
+    SCx, SCy, SW = (o.tables[k].astype(np.float64) for k in ("SCx", "SCy", "SW"))
+    cells = [0, 9, 27, 36, 63]
+    for c in cells:
+        cx, cy = SCx[c, 0].mean(), SCy[c, 0].mean()
+        spanC = np.abs(SCx[c, 0] - cx).max()
+        if kind == "disk":            # bright disk over every centre sample, dark surround
+            frame = _disk_frame(sp, cx, cy, 1.6 * spanC, 0.9, 0.3)
+        elif kind == "saturating_spot":   # I/mean - 1 > 1 inside the spot: clipped
+            frame = _disk_frame(sp, cx, cy, 0.5 * spanC, 5.0, 0.2)
+        else:
+            frame = _noise_frames(sp, 1, seed=c)[0]
+        R = o.frame_response(frame)[c].astype(np.float64)
+        args = (frame, SCx[c, 0], SCy[c, 0], SCx[c, 1], SCy[c, 1], SW)
+        rC, rS = _response64(*args)
+        np.testing.assert_allclose(R, [rC, rS], rtol=2e-5, atol=2e-6, err_msg=f"{kind} cell {c}")
+        if kind == "disk":            # the centre mean would give a very different answer
+            assert abs(_response64(*args, mean_over="centre")[0] - rC) > 0.1
+        if kind == "saturating_spot":
+            assert abs(_response64(*args, clip=False)[0] - rC) > 0.1
+        if kind == "noise":           # the weights matter: uniform weights differ
+            u = np.full_like(SW, 1.0 / SW.size)
+            assert abs(_response64(frame, SCx[c, 0], SCy[c, 0], SCx[c, 1], SCy[c, 1], u)[0]
+                       - rC) > 1e-3
+
+
+# ---------------------------------------------------------- sample coordinates
+def test_sample_coordinates_geometry(pkg):
+    import oracle
+    from visual_cortex_amd import network
+    net = _net(pkg)
+    sp = net.spec
+    rng = np.random.default_rng(17)
+    L = net.lgn
+    n = sp.nLGN
+    for k in ("orient", "rw", "rh"):
+        L[k] = np.ascontiguousarray(L[k].copy())
+    L["orient"][:] = rng.uniform(-math.pi, math.pi, (2, n)).astype(np.float32)
+    L["rh"][:] = (L["rw"] * rng.uniform(0.6, 1.4, (2, n))).astype(np.float32)
+    o = oracle.OracleSim(net, pkg)
+    SCx, SCy = (o.tables[k].astype(np.float64).reshape(n, 2, sp.nSpatialSample1D,
+                                                      sp.nSpatialSample1D)
+                for k in ("SCx", "SCy"))
+    x0, y0, nvd = network.view_geometry(sp)
+    n1 = sp.nSpatialSample1D
+    for c in range(n):
+        for r in range(2):
+            ecc, pol = float(L["ecc"][r, c]), float(L["polar"][r, c])
+            o_, rx, ry = float(L["orient"][r, c]), float(L["rw"][r, c]), float(L["rh"][r, c])
+            X, Y = SCx[c, r], SCy[c, r]                  # [ty][tx]
+            # the grid is centred on the cell's position in the visual field
+            assert X.mean() == pytest.approx(x0 + ecc * math.cos(pol) * nvd, abs=1e-6)
+            assert Y.mean() == pytest.approx(y0 + ecc * math.sin(pol) * nvd, abs=1e-6)
+            # one step along tx is (dw cos o, dw sin o), one along ty (-dh sin o, dh cos o)
+            dw = 2 * sp.nsig * rx / math.sqrt(2) / n1 * nvd
+            dh = 2 * sp.nsig * ry / math.sqrt(2) / n1 * nvd
+            np.testing.assert_allclose(np.diff(X, axis=1), dw * math.cos(o_), atol=2e-7)
+            np.testing.assert_allclose(np.diff(Y, axis=1), dw * math.sin(o_), atol=2e-7)
+            np.testing.assert_allclose(np.diff(X, axis=0), -dh * math.sin(o_), atol=2e-7)
+            np.testing.assert_allclose(np.diff(Y, axis=0), dh * math.cos(o_), atol=2e-7)
+            # the corner sample is half a step inside the +-nsig*r/sqrt(2) span
+            u = -(n1 - 1) / 2.0
+            assert X[0, 0] - X.mean() == pytest.approx(u * (dw * math.cos(o_) - dh * math.sin(o_)),
+                                                       abs=2e-7)
+
+
+# ---------------------------------------------------------- maxConvol
+def _maxconvol_grid64(L, c, nsig, m1):
+    """The reference's parvo_maxConvol sample grid (src/discrete_input_convol.cu
+    :746-881) in float64: the normalised centre and surround Gaussians."""
+    n = L["k"].shape[1]
+    rxs, rys, rxc, ryc = (float(L[k][r, c]) for k, r in (("rw", 1), ("rh", 1), ("rw", 0),
+                                                           ("rh", 0)))
+    R = max(nsig * rxs, nsig * rys) / math.sqrt(2)
+    dx = L["ecc"][0, c] * math.cos(L["polar"][0, c]) - L["ecc"][1, c] * math.cos(L["polar"][1, c])
+    dy = L["ecc"][0, c] * math.sin(L["polar"][0, c]) - L["ecc"][1, c] * math.sin(L["polar"][1, c])
+    r = math.hypot(dx, dy) + max(nsig * rxc, nsig * ryc) / math.sqrt(2)
+    R = max(R, r)
+    ds = 2 * R / m1
+    lx = (np.arange(m1) + 0.5) * ds - R
+    LX, LY = np.meshgrid(lx, lx)
+    oc, os_ = float(L["orient"][0, c]), float(L["orient"][1, c])
+    xc = math.cos(oc) * (LX - dx) + math.sin(oc) * (LY - dy)
+    yc = -math.sin(oc) * (LX - dx) + math.cos(oc) * (LY - dy)
+    gC = np.exp(-xc ** 2 / rxc ** 2 - yc ** 2 / ryc ** 2)
+    xs = math.cos(os_) * LX + math.sin(os_) * LY
+    ys = -math.sin(os_) * LX + math.cos(os_) * LY
+    gS = np.exp(-xs ** 2 / rxs ** 2 - ys ** 2 / rys ** 2)
+    return gC / gC.sum(), gS / gS.sum()
+
+
+@pytest.mark.parametrize("cov", [-1.0, -0.5, 0.0, 0.5, 1.0])
+@pytest.mark.parametrize("same_sign", [False, True])
+def test_maxconvol(pkg, cov, same_sign):
+    """Per tap and grid point the reference adds the larger of |centre| and
+    |surround| plus copysign(cov, larger) * smaller (src/discrete_input_convol.cu
+    :863-867): copysign keeps only |cov|, so the term is
+    max + |cov| * min * sign(centre * surround).  With same-sign kernels and
+    |cov| = 1 that is |centre| + |surround|, whose sum is |kC| + |kS| since
+    each normalised Gaussian and each sum |TW| is 1 -- no grid needed."""
+    import oracle
+    from visual_cortex_amd import network
+    net = _net(pkg, nLGN_side=3)
+    L = net.lgn
+    n = net.spec.nLGN
+    rng = np.random.default_rng(23)
+    L["covariant"] = np.full(n, cov, np.float32)
+    L["orient"] = np.ascontiguousarray(rng.uniform(-1, 1, (2, n)).astype(np.float32))
+    # offset centres on some cells (centre and surround at different places)
+    L["ecc"] = np.ascontiguousarray(L["ecc"].copy())
+    L["ecc"][0, ::2] *= np.float32(1.05)
+    if same_sign:
+        L["k"] = np.ascontiguousarray(np.stack([L["k"][0], 0.7 * L["k"][0]]).astype(np.float32))
+    o = oracle.OracleSim(net, pkg)
+    mc = o.tables["maxConvol"].astype(np.float64)
+    TW = o.tables["TW"].astype(np.float64)
+    m1 = network.max_sample_1d(net)
+    for c in range(n):
+        kc, ks = float(L["k"][0, c]), float(L["k"][1, c])
+        assert (kc * ks > 0) == same_sign
+        if same_sign and abs(cov) == 1.0:
+            assert mc[c] == pytest.approx(abs(kc) + abs(ks), rel=2e-5)
+        gC, gS = _maxconvol_grid64(L, c, net.spec.nsig, m1)
+        lc = TW[c, 0][:, None, None] * kc * gC[None]
+        ls = TW[c, 1][:, None, None] * ks * gS[None]
+        big = np.maximum(np.abs(lc), np.abs(ls))
+        small = np.minimum(np.abs(lc), np.abs(ls))
+        ref = (big + abs(cov) * small * np.sign(lc * ls)).sum()
+        assert mc[c] == pytest.approx(ref, rel=2e-5), c
+        if not same_sign and abs(cov) == 1.0:      # opposite signs: the filter's L1 norm
+            assert ref == pytest.approx(np.abs(lc + ls).sum(), rel=1e-12)
