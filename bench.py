@@ -376,7 +376,9 @@ def bench_batched(args) -> dict:
     sim_shape = sim.shape
     del sim
     ts = 0.0
-    for n in nets:
+    for i, n in enumerate(nets):
+        if i % 8 == 0:
+            log(f"[bench] serial trials {i + 1}..{min(i + 8, len(nets))}")
         s1 = GpuSim(n, record=False)
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -129,10 +129,15 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
     tp.failed = 0;
     /* weights: leaves */
     const size_t nW = (size_t)nb * maxNear * bs * bs;
-    num *W = malloc(sizeof(num) * nW);
-    num *F = malloc(sizeof(num) * (size_t)maxLGN * nV1);
-    for (size_t i = 0; i < nW; ++i) W[i] = conMat[i] != 0.0 ? leaf(conMat[i]) : K(0.0);
-    for (size_t i = 0; i < (size_t)maxLGN * nV1; ++i) F[i] = leaf(sLGN[i]);
+    /* tape leaves of the weights (only with a tape: without one they are
+     * plain constants read straight from conMat and sLGN) */
+    num *W = NULL, *F = NULL;
+    if (tp.on) {
+        W = malloc(sizeof(num) * nW);
+        F = malloc(sizeof(num) * (size_t)maxLGN * nV1);
+        for (size_t i = 0; i < nW; ++i) W[i] = conMat[i] != 0.0 ? leaf(conMat[i]) : K(0.0);
+        for (size_t i = 0; i < (size_t)maxLGN * nV1; ++i) F[i] = leaf(sLGN[i]);
+    }
     st64 *S = malloc(sizeof(st64) * nV1);
     for (uint32_t n = 0; n < nV1; ++n) {
         S[n].v = K(init[VC_SNAP_V * nV1 + n]);
@@ -151,8 +156,19 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
     uint8_t *spk = calloc(nV1, 1), *spkNext = calloc(nV1, 1);
     int64_t *outId = malloc(sizeof(int64_t) * (size_t)T * nV1);
     num *ogN = malloc(sizeof(num) * NG * nV1), *ohN = malloc(sizeof(num) * NG * nV1);
+    /* the previous step's spiking neurons of each block, ascending */
+    uint32_t *spkList = malloc(sizeof(uint32_t) * nV1), *spkOff = malloc(sizeof(uint32_t) * (nb + 1));
+    const int tape = tp.on;
     for (uint32_t it = 0; it < T; ++it) {
         const float *lgn = lgnSInfo + (size_t)it * nLGN;
+        spkOff[0] = 0;
+        for (uint32_t b = 0, k = 0; b < nb; ++b) {
+            for (uint32_t pre = 0; pre < bs; ++pre)
+                if (spk[b * bs + pre]) spkList[k++] = pre;
+            spkOff[b + 1] = k;
+        }
+        /* without a tape every neuron's step is independent of the others' */
+#pragma omp parallel for schedule(static) if (!tape)
         for (uint32_t n = 0; n < nV1; ++n) {
             const uint32_t b = n / bs, post = n % bs;
             const int t = type_of64(p, post);
@@ -160,10 +176,11 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
             /* recurrent increments of the previous step's spikes */
             for (uint32_t ib = 0; ib < nNear[b]; ++ib) {
                 const uint32_t bid = nbr[b * maxNear + ib];
-                for (uint32_t pre = 0; pre < bs; ++pre) {
+                for (uint32_t k = spkOff[bid]; k < spkOff[bid + 1]; ++k) {
+                    const uint32_t pre = spkList[k];
                     const uint32_t pg = bid * bs + pre;
-                    if (!spk[pg]) continue;
-                    num w = W[(((size_t)b * maxNear + ib) * bs + pre) * bs + post];
+                    const size_t wi = (((size_t)b * maxNear + ib) * bs + pre) * bs + post;
+                    num w = W ? W[wi] : K(conMat[wi]);
                     if (w.id < 0 && w.v == 0.0) continue;
                     if (pre < p->nE) {
                         for (uint32_t ig = 0; ig < p->ngE; ++ig) {
@@ -212,7 +229,7 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
                 if (!(sI >= 1.0f)) continue;
                 const double nsp = floor((double)sI), tsp = ((double)sI - nsp) * dt;
                 for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
-                    num f = scl(F[lid], p->pFF[t * p->ngFF + ig] * nsp);
+                    num f = scl(F ? F[lid] : K(sLGN[lid]), p->pFF[t * p->ngFF + ig] * nsp);
                     input(&o.gF[ig], &o.hF[ig], f, K(dt - tsp), p->riseFF[ig], p->decayFF[ig],
                           p->dodFF[ig]);
                     if (backing && s.tB.v - tsp > 0.0)
@@ -323,7 +340,7 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
         free(adj);
     }
     free(W); free(F); free(S); free(og); free(oh); free(ogN); free(ohN);
-    free(spk); free(spkNext); free(outId);
+    free(spk); free(spkNext); free(outId); free(spkList); free(spkOff);
     free(tp.nd);
     tp.nd = NULL;
     tp.n = tp.cap = 0;

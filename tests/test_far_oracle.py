@@ -247,3 +247,16 @@ def test_batched_far_trials_equal_serial(pkg):
         assert int((s >= 1).sum()) > 40
         assert np.array_equal(sb[:, off:off + n.spec.nV1].view(np.uint32), s.view(np.uint32))
         off += n.spec.nV1
+
+
+def test_batch_refuses_mixed_far_settings(pkg):
+    """Trials batched together share one spec for the far gate: a trial whose
+    far connections are delayed (noDelay 1, noFarDelay 0) cannot be batched
+    with one whose far input is off (noFarDelay -1 -> noDelay 1)."""
+    from visual_cortex_amd import batch, network
+    a = network.build_network(network.spec_tiny(nFar=4, noDelay=1, noFarDelay=0, nt=64))
+    b = network.build_network(network.spec_tiny(nFar=4, noDelay=1, noFarDelay=-1, nt=64, seed=7))
+    assert a.far_delayed() and not b.far_delayed()
+    for pair in ((a, b), (b, a)):
+        with pytest.raises(ValueError, match="noFarDelay"):
+            batch.batch_networks(list(pair), device="cpu")

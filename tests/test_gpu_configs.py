@@ -1,0 +1,365 @@
+"""Every benchmark configuration at its real size, HIP path against the oracle.
+
+  C1  plain forward, small (4 blocks x 1024 conductance-LIF, 1024 LGN, 1 s):
+      every output of the first 1600 steps (200 ms) bit for bit; over the
+      whole 8000-step trial at 8 grating orientations, per-neuron rates and
+      orientation tuning (gOSI, OSI) within 2% -- they are in fact identical.
+  C2  plain forward, full patch (64 blocks, 65,536 neurons, 16,384 LGN):
+      every output of 800 steps bit for bit.
+  C3  32 batched trials of 8-96 blocks, 4000 steps (500 ms) each: per step
+      and trial, two checksums of the V1 output bits, the spike counts and the
+      final state equal the 32 trials run alone; the two smallest trials equal
+      the oracle bit for bit.
+  C4  LGN differentiation (16,384 cells, 4000 steps, 63 white-noise frames
+      of 256x256): rates within 1e-5 of float64, the gradient within 1e-4 of
+      the float64 gradient, 256 coordinates against central differences.
+  C5  V1 differentiation (16 blocks = 16,384 LIF neurons, 2000 steps,
+      checkpoints every 100): the forward equals the float32 oracle bit for
+      bit, the backward's recompute equals the forward, and the gradient
+      matches central differences of the float64 loss at the fixed spike
+      pattern on 12 single weights and along 4 random directions through all
+      8.5 M weights.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _tables(g):
+    return dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
+                SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+# ------------------------------------------------------------------ C1
+def test_c1_first_200ms_bitwise(pkg):
+    import oracle
+    from visual_cortex_amd import network as N
+    from visual_cortex_amd.sim import GpuSim
+    net = N.build_network(N.spec_c1())
+    _say(f"C1: {net.spec.nV1} neurons, {net.spec.nLGN} LGN cells, 1600 steps vs the oracle")
+    g = GpuSim(net)
+    o = oracle.OracleSim(net, pkg, tables=_tables(g))
+    g.run(1600)
+    o.run(1600)
+    sg, so = g.sInfo(), o.sInfo()
+    assert np.array_equal(_bits(sg), _bits(so))
+    assert np.array_equal(_bits(g.lgn_sinfo()), _bits(o.lgn_sinfo()))
+    g.finalize()
+    for k, v in g.state_numpy().items():
+        assert np.array_equal(_bits(v), _bits(o.st[k])), k
+    nsp = int((sg >= 1).sum())
+    _say(f"C1: {nsp} V1 spikes in 200 ms, every spike time equal")
+    assert nsp > 5000
+
+
+@pytest.mark.timeout(600)
+def test_c1_full_trial_rates_and_tuning(pkg):
+    """8 orientations x 8000 steps: the per-neuron rates of the HIP path are
+    within 2% of the oracle's (and orientation tuning agrees)."""
+    import torch
+    import oracle
+    from visual_cortex_amd import analysis as A, network as N
+    from visual_cortex_amd.sim import GpuSim
+    spec = N.spec_c1()
+    net0 = N.build_network(spec, with_frames=False)
+    nori, T = 8, spec.nt
+    rg, ro = [], []
+    for k in range(nori):
+        sk = dataclasses.replace(spec, ori_deg=180.0 * k / nori)
+        net = dataclasses.replace(net0, spec=sk, frames=N.grating_frames(sk))
+        g = GpuSim(net, record=False)
+        out = torch.empty(T, spec.nV1, dtype=torch.float32, device="cuda")
+        g.run(T, out=out)
+        o = oracle.OracleSim(net, pkg, tables=_tables(g))
+        o.run(T)
+        sg, so = out.cpu().numpy(), o.sInfo()
+        assert np.array_equal(_bits(sg), _bits(so)), k
+        rg.append(A.rates(sg, spec.dt))
+        ro.append(A.rates(so, spec.dt))
+        _say(f"C1 tuning: orientation {k}: mean rate {rg[-1].mean():.2f} Hz")
+    rg, ro = np.array(rg), np.array(ro)
+    both = ro > 0
+    assert np.all(np.abs(rg - ro)[both] <= 0.02 * ro[both]) and np.all(rg[~both] == 0)
+    fg, fo = A.tuning_curves(list(rg)), A.tuning_curves(list(ro))
+    np.testing.assert_allclose(A.gosi(fg), A.gosi(fo), rtol=0.02, atol=1e-12)
+    np.testing.assert_allclose(A.osi(A.align(fg)), A.osi(A.align(fo)), rtol=0.02, atol=1e-12)
+    assert (fg.max(axis=0) > fg.min(axis=0)).any()
+
+
+# ------------------------------------------------------------------ C2
+@pytest.mark.timeout(600)
+def test_c2_full_patch_bitwise(pkg):
+    import oracle
+    from visual_cortex_amd import network as N
+    from visual_cortex_amd.sim import GpuSim
+    T = 800
+    net = N.build_network(N.spec_c2(nt=T))
+    _say(f"C2: {net.spec.nV1} neurons, {net.spec.nLGN} LGN cells, {T} steps vs the oracle")
+    g = GpuSim(net)
+    g.run(T)
+    sg = g.sInfo()
+    lg = g.lgn_sinfo()
+    g.finalize()
+    st = g.state_numpy()
+    o = oracle.OracleSim(net, pkg, tables=_tables(g))
+    del g
+    o.run(T)
+    assert np.array_equal(_bits(sg), _bits(o.sInfo()))
+    assert np.array_equal(_bits(lg), _bits(o.lgn_sinfo()))
+    for k, v in st.items():
+        assert np.array_equal(_bits(v), _bits(o.st[k])), k
+    nsp = int((sg >= 1).sum())
+    _say(f"C2: {nsp} V1 spikes in {T} steps, every output bit equal")
+    assert nsp > 1000
+
+
+# ------------------------------------------------------------------ C3
+def _checksums(torch, out, trial_of, wgt, ntrial):
+    """Per step and trial: the sum of the output bit patterns and a weighted
+    sum (position-dependent weights), int64."""
+    b = out.view(torch.int32).to(torch.int64)
+    c1 = torch.zeros(out.shape[0], ntrial, dtype=torch.int64, device=out.device)
+    c2 = torch.zeros_like(c1)
+    c1.index_add_(1, trial_of, b)
+    c2.index_add_(1, trial_of, b * wgt)
+    return c1, c2
+
+
+@pytest.mark.timeout(600)
+def test_c3_batched_32_trials(pkg):
+    import torch
+    import oracle
+    from visual_cortex_amd import batch
+    from visual_cortex_amd.sim import GpuSim
+    T, npf = 4000, 64
+    specs = batch.c3_specs(ntrial=32, nt=T)
+    nets = [batch.build_network_fast(s, device="cuda") for s in specs]
+    sizes = [n.spec.nV1 for n in nets]
+    bnet = batch.batch_networks(nets)
+    nV1 = bnet.spec.nV1
+    _say(f"C3: 32 trials, blocks {[s.nblock for s in specs]}, {nV1} neurons, {T} steps")
+    dev = torch.device("cuda")
+    trial_of = torch.repeat_interleave(torch.arange(32, device=dev),
+                                       torch.as_tensor(sizes, device=dev))
+    local = torch.cat([torch.arange(n, device=dev) for n in sizes]).to(torch.int64)
+    wgt = (local * 2654435761) % 2147483647 + 1
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    small = list(np.argsort(sizes, kind="stable")[:2])
+    keep = {t: [] for t in small}
+
+    def run(sim, n_neur, tr, wv, ntr, keep_slices):
+        cs1, cs2, cnt = [], [], torch.zeros(n_neur, dtype=torch.int32, device=dev)
+        out = torch.empty(npf, n_neur, dtype=torch.float32, device=dev)
+        for k0 in range(0, T, npf):
+            o = out[:min(npf, T - k0)]
+            sim.run(o.shape[0], out=o)
+            a, b = _checksums(torch, o, tr, wv, ntr)
+            cs1.append(a)
+            cs2.append(b)
+            cnt += (o >= 1.0).sum(0, dtype=torch.int32)
+            for t, sl in keep_slices.items():
+                sl.append(o[:, offs[t]:offs[t + 1]].cpu())
+        sim.check()
+        sim.finalize()
+        return torch.cat(cs1).cpu(), torch.cat(cs2).cpu(), cnt.cpu(), sim.state_numpy()
+
+    g = GpuSim(bnet, record=False)
+    b1, b2, bcnt, bst = run(g, nV1, trial_of, wgt, 32, keep)
+    del g
+    assert int(bcnt.sum()) > 100000
+    _say(f"C3: batched run done, {int(bcnt.sum())} spikes")
+    tabs = {}
+    for t, n in enumerate(nets):
+        s = GpuSim(n, record=False)
+        if t in small:
+            tabs[t] = _tables(s)
+        z = torch.zeros(sizes[t], dtype=torch.int64, device=dev)
+        s1, s2, scnt, sst = run(s, sizes[t], z, wgt[offs[t]:offs[t + 1]], 1, {})
+        del s
+        assert torch.equal(b1[:, t:t + 1], s1) and torch.equal(b2[:, t:t + 1], s2), t
+        assert torch.equal(bcnt[offs[t]:offs[t + 1]], scnt), t
+        for k, v in sst.items():
+            sl = bst[k][..., offs[t]:offs[t + 1]]
+            assert np.array_equal(_bits(sl), _bits(v)), (t, k)
+    _say("C3: every trial batched equals the trial alone")
+    for t in small:
+        o = oracle.OracleSim(nets[t], pkg, tables=tabs[t])
+        o.run(T)
+        gb = torch.cat(keep[t]).numpy()
+        assert np.array_equal(_bits(gb), _bits(o.sInfo())), t
+        _say(f"C3: trial {t} ({specs[t].nblock} blocks) batched equals the oracle, "
+             f"{int((gb >= 1).sum())} spikes")
+
+
+# ------------------------------------------------------------------ C4
+@pytest.mark.timeout(600)
+def test_c4_lgn_adjoint_full_size(pkg, orc):
+    import torch
+    import oracle as O
+    from visual_cortex_amd import network as N
+    from visual_cortex_amd.lgn_grad import LgnStage
+    spec = N.spec_c2()
+    T = 4000
+    lgn = N.lgn_layer(spec)
+    st = LgnStage(spec, lgn, T)
+    dev = st.dev
+    _say(f"C4: {spec.nLGN} cells, {st.nF} frames of {spec.width}x{spec.height}, {T} steps")
+    frames = N.white_noise_frames(spec, st.nF, seed=11)
+    th0 = st.theta0.clone()
+    with torch.no_grad():
+        tgt = st.rates(torch.as_tensor(N.grating_frames(spec, st.nF), device=dev), th0)
+    F = torch.as_tensor(frames, device=dev).requires_grad_(True)
+    TH = th0.clone().requires_grad_(True)
+    fr = st.rates(F, TH)
+    loss = ((fr - tgt) ** 2).mean()
+    loss.backward()
+    torch.cuda.synchronize()
+    dF, dTH = F.grad.cpu().numpy(), TH.grad.cpu().numpy()
+    tables = dict(SCx=st.SCx.cpu().numpy(), SCy=st.SCy.cpu().numpy(), SW=st.SW.cpu().numpy(),
+                  maxConvol=st.maxConvol.cpu().numpy())
+    g64 = O.LgnGrad64(orc, spec, lgn, tables, T, N)
+    th = th0.cpu().numpy().astype(np.float64)
+    tg = tgt.cpu().numpy()
+    fr64 = g64.rates(frames, th)
+    err = np.abs(fr.detach().cpu().numpy() - fr64).max() / np.abs(fr64).max()
+    _say(f"C4: rates vs float64: max rel error {err:.2e}")
+    assert err < 1e-5
+    df64, dth64 = g64.grad(frames, th, tg)
+    for a, b in ((dF, df64), (dTH, dth64)):
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        _say(f"C4: gradient vs float64: rel norm error {rel:.2e}")
+        assert rel < 1e-4
+    rng = np.random.default_rng(7)
+    eps = 1e-3
+    nz = np.argwhere(df64 != 0)
+    coords = [("f", tuple(c)) for c in nz[rng.choice(len(nz), 128, replace=False)]]
+    cells = rng.choice(th.shape[0], 128, replace=False)
+    coords += [("t", (int(c), int(k))) for c, k in zip(cells, rng.integers(0, 6, 128))]
+    base = frames.astype(np.float64)
+    checked, worst = 0, 0.0
+    for kind, idx in coords:
+        if kind == "f":
+            f, ch, y, x = idx
+            sub = g64.cells_touching(x, y, ch)
+            fp = base.copy(); fp[idx] += eps
+            fm = base.copy(); fm[idx] -= eps
+            lp, pp = g64.loss(fp, th, tg, sub)
+            lm, pm = g64.loss(fm, th, tg, sub)
+            gv, scale = dF[idx], np.abs(dF).max()
+        else:
+            c, k = idx
+            tp = th.copy(); tp[c, k] += eps
+            tm = th.copy(); tm[c, k] -= eps
+            lp, pp = g64.loss(frames, tp, tg, [c])
+            lm, pm = g64.loss(frames, tm, tg, [c])
+            gv, scale = dTH[c, k], np.abs(dTH[:, k]).max()
+        if pp != pm:                  # the +-eps runs cross a kink (clip or rectifier)
+            continue
+        fd = (lp - lm) / (2 * eps)
+        assert abs(gv - fd) <= 1e-3 * abs(fd) + 1e-6 * scale, (kind, idx, gv, fd)
+        worst = max(worst, abs(gv - fd) / max(abs(fd), 1e-30))
+        checked += 1
+    _say(f"C4: finite differences: {checked}/256 checked, worst rel {worst:.2e}")
+    assert checked >= 230
+
+
+# ------------------------------------------------------------------ C5
+@pytest.mark.timeout(900)
+def test_c5_v1_adjoint_full_size(pkg, orc):
+    import torch
+    import oracle as O
+    from visual_cortex_amd import batch, network as N, params as P
+    from visual_cortex_amd.v1_grad import V1Run, soft_bin_counts, soft_count_loss
+    spec = N.spec_c5()
+    T, BIN, TAU = spec.nt, 200, 1.0
+    net = batch.build_network_fast(spec, device="cuda")
+    r = V1Run(net, T, ckpt=100)
+    _say(f"C5: {spec.nV1} LIF neurons, {r.sim.wrec.numel() + r.sim.sLGN.numel()} weights, "
+         f"{T} steps, checkpoints every 100")
+    sL, w = r.sim.sLGN.clone(), r.sim.wrec.clone()
+    gen = torch.Generator(device=r.dev).manual_seed(3)
+    with torch.no_grad():
+        wp = w * torch.exp(0.2 * torch.randn(w.shape, generator=gen, device=r.dev))
+        tgt = soft_bin_counts(r.forward(sL, wp), spec.dt, BIN, TAU)
+    sLq, wq = sL.clone().requires_grad_(True), w.clone().requires_grad_(True)
+    sInfo = r.spikes(sLq, wq)
+    loss = soft_count_loss(sInfo, tgt, spec.dt, BIN, TAU)
+    loss.backward()
+    dsl, dw = sLq.grad.cpu().numpy(), wq.grad.cpu().numpy()
+    s_gpu = sInfo.detach().cpu().numpy()
+    # the backward's segment recompute reproduces the forward bit for bit
+    r.forward(sL, w)
+    r.backward(torch.zeros(T, spec.nV1, device=r.dev), verify=True)
+    # the forward is the float32 oracle's
+    tables = {k: getattr(r.sim, k).cpu().numpy() for k in ("TW", "SW", "SCx", "SCy", "maxConvol")}
+    osim = O.OracleSim(net, pkg, tables=tables)
+    osim.run(T)
+    assert np.array_equal(_bits(osim.sInfo()), _bits(s_gpu))
+    pat = s_gpu >= 1
+    _say(f"C5: forward equals the float32 oracle ({int(pat.sum())} spikes); recompute verified")
+    assert pat.sum() > 10000
+    del osim
+    # float64 loss at the fixed spike pattern, central differences
+    st = P.initial_state(net)
+    init = np.zeros((14, spec.nV1), np.float32)
+    init[0], init[1] = st["v"], st["tBack"]
+    g64 = O.V1Grad64(orc, net, P, r.lgn.cpu().numpy(), init)
+    tg = tgt.cpu().numpy().astype(np.float64)
+    idx_map = net.csr_dense_index()
+
+    def loss64(conMat=None, sLGN=None):
+        out = g64.run(conMat=conMat, sLGN=sLGN, pattern=pat)
+        return O.soft_count_loss64(out, tg, spec.dt, BIN, TAU)[0]
+
+    rng = np.random.default_rng(11)
+    eps = 1e-6
+    cm, sl = g64.conMat, g64.sLGN
+    strict, total = 0, 0
+    for kind in ("w", "s"):
+        grad = dw if kind == "w" else dsl.reshape(-1)
+        top = np.argsort(-np.abs(grad))[:2000]            # well-conditioned coordinates
+        for i in rng.choice(top, 6, replace=False):
+            arr, j = (cm.reshape(-1), idx_map[i]) if kind == "w" else (sl.reshape(-1), i)
+            v0 = arr[j]
+            arr[j] = v0 + eps
+            lp = loss64()
+            arr[j] = v0 - eps
+            lm = loss64()
+            arr[j] = v0
+            fd = (lp - lm) / (2 * eps)
+            g = grad[i]
+            _say(f"C5: {kind}[{i}] adjoint {g:.6e} finite difference {fd:.6e}")
+            assert abs(g - fd) <= 1e-3 * abs(fd) + 1e-5 * np.abs(grad).max()
+            strict += abs(g - fd) <= 1e-3 * abs(fd)
+            total += 1
+    # directional derivatives through every weight: d/de L(w + e u) = grad . u
+    wnz = idx_map[: dw.size]
+    for k in range(4):
+        u_w = rng.standard_normal(dw.size) * np.abs(cm.reshape(-1)[wnz])
+        u_s = rng.standard_normal(sl.size) * np.abs(sl.reshape(-1))
+        e = 1e-4
+        cmp_, cmm = cm.copy(), cm.copy()
+        cmp_.reshape(-1)[wnz] += e * u_w
+        cmm.reshape(-1)[wnz] -= e * u_w
+        slp = sl + e * u_s.reshape(sl.shape)
+        slm = sl - e * u_s.reshape(sl.shape)
+        fd = (loss64(conMat=cmp_, sLGN=slp) - loss64(conMat=cmm, sLGN=slm)) / (2 * e)
+        del cmp_, cmm
+        g = float(dw @ u_w + dsl.reshape(-1) @ u_s)
+        _say(f"C5: direction {k}: adjoint {g:.6e} finite difference {fd:.6e}")
+        assert abs(g - fd) <= 1e-3 * abs(fd)
+        strict += 1
+        total += 1
+    _say(f"C5: {strict}/{total} within 1e-3 relative")
+    assert strict >= total - 1

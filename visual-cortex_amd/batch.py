@@ -164,10 +164,11 @@ def batch_networks(nets: list, device=None) -> N.Network:
     for n in nets:
         s = n.spec
         if (s.blockSize, s.nE, s.dt, s.width, s.height, n.frames.shape, s.noDelay,
-                s.speedOfThought) != (sp0.blockSize, sp0.nE, sp0.dt, sp0.width, sp0.height,
-                                      nets[0].frames.shape, sp0.noDelay, sp0.speedOfThought):
+                s.speedOfThought, n.far_delayed()) != (
+                    sp0.blockSize, sp0.nE, sp0.dt, sp0.width, sp0.height, nets[0].frames.shape,
+                    sp0.noDelay, sp0.speedOfThought, nets[0].far_delayed()):
             raise ValueError("batched trials must share block size, dt, stimulus shape and "
-                             "axonal-delay settings")
+                             "axonal-delay settings (noDelay, noFarDelay, speedOfThought)")
     if device is None:
         c0 = nets[0].csr
         device = c0[0].device if c0 is not None and hasattr(c0[0], "device") else \

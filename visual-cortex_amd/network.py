@@ -141,8 +141,9 @@ class ModelSpec:
 
 
 def spec_c1(**kw) -> ModelSpec:
-    """Plain forward, small: 4 blocks x 1024, 1024 LGN, 128x128 px, 1 s."""
-    return replace(ModelSpec(), **kw)
+    """Plain forward, small: 4 blocks x 1024 conductance-LIF neurons (iModel
+    0, deltaT = a = b = 0 for both types), 1024 LGN, 128x128 px, 1 s."""
+    return replace(lif(ModelSpec()), **kw)
 
 
 def spec_c2(**kw) -> ModelSpec:

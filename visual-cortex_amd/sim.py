@@ -410,16 +410,18 @@ class GpuSim:
                                     ptr(self.lgn_sInfo2[buf]) + a_in * nL * 4,
                                     ptr(self.lgn_fr2[buf]) + a_in * nL * 4, ev),
               "vc_lgn_steps")
+        # the V1 steps wait for this point only: the slice of the next frame
+        # below is read by the next period's LGN steps, on this stream
+        e = torch.cuda.Event()
+        e.record(st)
+        self.lgn_evq.append((b, e))
+        self.lgn_to = b
         c1 = b_in * nL // npf
         if self.fr_frame == k + 1 and c1 > self.fr_cells:
             self._frame_cells(st, k + 1, self.fr_cells, c1)
             self.fr_cells = c1
             if c1 == nL:
                 self.fr_frame, self.fr_cells = k + 2, 0
-        e = torch.cuda.Event()
-        e.record(st)
-        self.lgn_evq.append((b, e))
-        self.lgn_to = b
 
     def _lgn_period(self, k: int):
         """Issue the LGN stage of the whole frame period k (it must be next)."""
