@@ -173,43 +173,53 @@ static double cell_rate(const orc_lgn64 *m, uint32_t c, const double *theta,
 }
 
 /* The loss restricted to the cells in `cells` (all if NULL); *pattern gets
- * the number of clipped samples plus the number of rectified steps, a
- * signature of the activation pattern for finite-difference checks. */
+ * a signature of the activation pattern for finite-difference checks: per
+ * cell an FNV-1a hash of the number of clipped samples of every frame, the
+ * signs of the temporal weights (the normalisation by sum |TW| has a kink
+ * where a weight changes sign) and the rectifier state of every step, summed
+ * over the cells.  Two runs with equal signatures have, with overwhelming
+ * probability, crossed no kink between them (a count alone would miss kinks
+ * crossed in opposite directions). */
 double orc_lgn64_loss(const orc_lgn64 *m, const double *frames, const double *theta,
                       const float *target, const int32_t *cells, int ncells, long *pattern) {
     const int nc = cells ? ncells : (int)m->nLGN;
     const size_t fsz = 3ull * m->W * m->H;
     double total = 0.0;
-    long pat = 0;
+    unsigned long pat = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : total, pat)
     for (int k = 0; k < nc; ++k) {
         const uint32_t c = cells ? (uint32_t)cells[k] : (uint32_t)k;
         const double *th = theta + (size_t)c * 6;
         double *R64 = malloc(sizeof(double) * 2 * (m->nF + 1));
         double *tw = malloc(sizeof(double) * 2 * m->nKS);
+        uint64_t h = 1469598103934665603ull;
+#define ORC_MIX(x) (h = (h ^ (uint64_t)(x)) * 1099511628211ull)
         long clipped = 0;
-        resp64(m, m->initFrame, c, R64, &clipped);
-        clipped = 0;                       /* the pre-stimulus frame is no variable */
-        for (uint32_t f = 0; f < m->nF; ++f)
+        resp64(m, m->initFrame, c, R64, &clipped);   /* the pre-stimulus frame is no variable */
+        for (uint32_t f = 0; f < m->nF; ++f) {
+            clipped = 0;
             resp64(m, frames + f * fsz, c, R64 + 2 * (f + 1), &clipped);
+            ORC_MIX(clipped);
+        }
         tw64(m, c, 0, th[2], th[3], tw);
         tw64(m, c, 1, th[4], th[5], tw + m->nKS);
+        for (uint32_t j = 0; j < 2 * m->nKS; ++j) ORC_MIX(tw[j] < 0.0);
         double acc = 0.0;
-        long act = 0;
         for (uint32_t it = 0; it < m->T; ++it) {
             double a, b, X;
             int on;
             const double fr = cell_rate(m, c, th, R64, tw, it, &a, &b, &X, &on);
             const double d = fr - (double)target[(size_t)it * m->nLGN + c];
             acc += d * d;
-            act += on;
+            ORC_MIX(on);
         }
+#undef ORC_MIX
         total += acc;
-        pat += clipped + act;
+        pat += h;
         free(R64);
         free(tw);
     }
-    if (pattern) *pattern = pat;
+    if (pattern) *pattern = (long)pat;
     return total / ((double)m->T * m->nLGN);
 }
 

@@ -12,7 +12,8 @@
       the oracle bit for bit.
   C4  LGN differentiation (16,384 cells, 4000 steps, 63 white-noise frames
       of 256x256): rates within 1e-5 of float64, the gradient within 1e-4 of
-      the float64 gradient, 256 coordinates against central differences.
+      the float64 gradient, 256 coordinates against central differences
+      (eps 1e-3, narrowed where the +-eps runs straddle a kink).
   C5  V1 differentiation (16 blocks = 16,384 LIF neurons, 2000 steps,
       checkpoints every 100): the forward equals the float32 oracle bit for
       bit, the backward's recompute equals the forward, and the gradient
@@ -241,37 +242,51 @@ def test_c4_lgn_adjoint_full_size(pkg, orc):
         _say(f"C4: gradient vs float64: rel norm error {rel:.2e}")
         assert rel < 1e-4
     rng = np.random.default_rng(7)
-    eps = 1e-3
     nz = np.argwhere(df64 != 0)
     coords = [("f", tuple(c)) for c in nz[rng.choice(len(nz), 128, replace=False)]]
     cells = rng.choice(th.shape[0], 128, replace=False)
     coords += [("t", (int(c), int(k))) for c, k in zip(cells, rng.integers(0, 6, 128))]
     base = frames.astype(np.float64)
-    checked, worst = 0, 0.0
-    for kind, idx in coords:
+
+    def central(kind, idx, e):
         if kind == "f":
             f, ch, y, x = idx
             sub = g64.cells_touching(x, y, ch)
-            fp = base.copy(); fp[idx] += eps
-            fm = base.copy(); fm[idx] -= eps
+            fp = base.copy(); fp[idx] += e
+            fm = base.copy(); fm[idx] -= e
             lp, pp = g64.loss(fp, th, tg, sub)
             lm, pm = g64.loss(fm, th, tg, sub)
-            gv, scale = dF[idx], np.abs(dF).max()
         else:
             c, k = idx
-            tp = th.copy(); tp[c, k] += eps
-            tm = th.copy(); tm[c, k] -= eps
+            tp = th.copy(); tp[c, k] += e
+            tm = th.copy(); tm[c, k] -= e
             lp, pp = g64.loss(frames, tp, tg, [c])
             lm, pm = g64.loss(frames, tm, tg, [c])
-            gv, scale = dTH[c, k], np.abs(dTH[:, k]).max()
-        if pp != pm:                  # the +-eps runs cross a kink (clip or rectifier)
+        return (lp - lm) / (2 * e), pp == pm
+
+    # eps = 1e-3 where the +-eps runs have the same activation pattern; where
+    # they straddle a kink (a clipped contrast, a rectified step, a temporal
+    # weight changing sign) the interval is narrowed to 1e-4, then 1e-5
+    used = {1e-3: 0, 1e-4: 0, 1e-5: 0}
+    worst, checked = 0.0, 0
+    for kind, idx in coords:
+        for e in used:
+            fd, same = central(kind, idx, e)
+            if same:
+                break
+        else:
             continue
-        fd = (lp - lm) / (2 * eps)
-        assert abs(gv - fd) <= 1e-3 * abs(fd) + 1e-6 * scale, (kind, idx, gv, fd)
+        used[e] += 1
+        if kind == "f":
+            gv, scale = dF[idx], np.abs(dF).max()
+        else:
+            gv, scale = dTH[idx], np.abs(dTH[:, idx[1]]).max()
+        assert abs(gv - fd) <= 1e-3 * abs(fd) + 1e-6 * scale, (kind, idx, e, gv, fd)
         worst = max(worst, abs(gv - fd) / max(abs(fd), 1e-30))
         checked += 1
-    _say(f"C4: finite differences: {checked}/256 checked, worst rel {worst:.2e}")
-    assert checked >= 230
+    _say(f"C4: finite differences: {checked}/256 checked (eps 1e-3: {used[1e-3]}, 1e-4: "
+         f"{used[1e-4]}, 1e-5: {used[1e-5]}), worst rel {worst:.2e}")
+    assert checked >= 250
 
 
 # ------------------------------------------------------------------ C5
