@@ -431,7 +431,8 @@ class V1Grad64:
         self.p = params_mod.v1_params_struct(net)
         vp, u32 = C.c_void_p, C.c_uint32
         self.L.orc_v1_lif64.argtypes = ([C.POINTER(o.abi.V1Params), u32, vp, vp, vp, u32, vp, vp,
-                                         vp, vp, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp])
+                                         vp, vp, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                         vp, vp, vp, vp])
         self.L.orc_v1_lif64.restype = C.c_int
         self.lgn = np.ascontiguousarray(lgn, dtype=np.float32)
         self.T = self.lgn.shape[0]
@@ -459,10 +460,34 @@ class V1Grad64:
                                  _fp(self.wstate), sL.shape[0], _fp(self.nLGN_of), _fp(self.idLGN),
                                  _fp(sL), _fp(self.lgn), self.lgn.shape[1], self.net.maxNear,
                                  _fp(self.nNear), _fp(self.nbr), _fp(cm), _fp(pat), _fp(out),
-                                 _fp(g), _fp(dsl), _fp(dcm))
+                                 _fp(g), _fp(dsl), _fp(dcm), None, None, None, None)
         if rc != 0:
             raise RuntimeError(f"orc_v1_lif64 returned {rc}")
         return out if g is None else (out, dsl, dcm)
+
+    def tangent(self, pattern, uConMat=None, uSLGN=None, anchor=None):
+        """Forward-mode derivative of every output [T][nV1] along the weight
+        direction (uConMat dense like conMat, uSLGN like sLGN; None = 0) at
+        the fixed spike pattern; with anchor [T][VC_SNAP_ROWS][nV1] (float32)
+        the primal state follows that trajectory step by step."""
+        sp = self.net.spec
+        pat = np.ascontiguousarray(pattern, dtype=np.uint8)
+        out = np.zeros((self.T, sp.nV1), np.float64)
+        dOut = np.zeros((self.T, sp.nV1), np.float64)
+        uc = None if uConMat is None else np.ascontiguousarray(uConMat, dtype=np.float64)
+        us = None if uSLGN is None else np.ascontiguousarray(uSLGN, dtype=np.float64)
+        an = None if anchor is None else np.ascontiguousarray(anchor, dtype=np.float32)
+        if an is not None and an.shape != (self.T, 14, sp.nV1):
+            raise ValueError(f"anchor must be [{self.T}][14][{sp.nV1}]")
+        rc = self.L.orc_v1_lif64(C.byref(self.p), self.T, _fp(self.init), _fp(self.tonic),
+                                 _fp(self.wstate), self.sLGN.shape[0], _fp(self.nLGN_of),
+                                 _fp(self.idLGN), _fp(self.sLGN), _fp(self.lgn),
+                                 self.lgn.shape[1], self.net.maxNear, _fp(self.nNear),
+                                 _fp(self.nbr), _fp(self.conMat), _fp(pat), _fp(out), None, None,
+                                 None, _fp(us), _fp(uc), _fp(an), _fp(dOut))
+        if rc != 0:
+            raise RuntimeError(f"orc_v1_lif64 returned {rc}")
+        return dOut
 
 
 def soft_bin_counts64(sInfo, dt, bin_steps, tau_s):

@@ -20,6 +20,15 @@
  * once (reverse-mode automatic differentiation).  It is therefore an
  * independent check of the GPU's hand-written reverse sweep; the tape itself
  * is pinned by central finite differences (tests/test_oracle_v1grad.py).
+ *
+ * Every value also carries a tangent (forward-mode automatic differentiation
+ * with dual numbers): given a direction in weight space, the same forward
+ * returns the directional derivative of every output.  With an anchor (the
+ * float32 trajectory's state at the start of every step), the primal state
+ * is reset to it at each step, so the tangent is the derivative of the step
+ * map at exactly that trajectory -- the limit of finite differences at the
+ * fixed spike pattern without the drift of a float64 trajectory forced onto
+ * a float32 spike pattern over a long run.
  */
 #include <math.h>
 #include <stdint.h>
@@ -32,6 +41,7 @@
 typedef struct {
     double v;
     int64_t id;      /* node index, or -1 for a constant */
+    double d;        /* tangent: directional derivative */
 } num;
 
 typedef struct {
@@ -58,21 +68,35 @@ static int64_t push(int64_t a, double da, int64_t b, double db) {
     tp.nd[tp.n] = (node){a, b, da, db};
     return tp.n++;
 }
-static inline num K(double v) { num r = {v, -1}; return r; }
+static inline num K(double v) { num r = {v, -1, 0.0}; return r; }
 static inline num leaf(double v) {
-    num r = {v, -1};
+    num r = {v, -1, 0.0};
     if (tp.on) r.id = push(-2, 0.0, -1, 0.0);   /* a = -2 marks a leaf */
     return r;
 }
-static inline num add(num x, num y) { num r = {x.v + y.v, push(x.id, 1.0, y.id, 1.0)}; return r; }
-static inline num sub(num x, num y) { num r = {x.v - y.v, push(x.id, 1.0, y.id, -1.0)}; return r; }
-static inline num mul(num x, num y) { num r = {x.v * y.v, push(x.id, y.v, y.id, x.v)}; return r; }
-static inline num dvd(num x, num y) {
-    num r = {x.v / y.v, push(x.id, 1.0 / y.v, y.id, -x.v / (y.v * y.v))};
+static inline num add(num x, num y) {
+    num r = {x.v + y.v, push(x.id, 1.0, y.id, 1.0), x.d + y.d};
     return r;
 }
-static inline num ex(num x) { double e = exp(x.v); num r = {e, push(x.id, e, -1, 0.0)}; return r; }
-static inline num scl(num x, double c) { num r = {x.v * c, push(x.id, c, -1, 0.0)}; return r; }
+static inline num sub(num x, num y) {
+    num r = {x.v - y.v, push(x.id, 1.0, y.id, -1.0), x.d - y.d};
+    return r;
+}
+static inline num mul(num x, num y) {
+    num r = {x.v * y.v, push(x.id, y.v, y.id, x.v), x.d * y.v + x.v * y.d};
+    return r;
+}
+static inline num dvd(num x, num y) {
+    const double q = x.v / y.v;
+    num r = {q, push(x.id, 1.0 / y.v, y.id, -x.v / (y.v * y.v)), (x.d - q * y.d) / y.v};
+    return r;
+}
+static inline num ex(num x) {
+    double e = exp(x.v);
+    num r = {e, push(x.id, e, -1, 0.0), e * x.d};
+    return r;
+}
+static inline num scl(num x, double c) { num r = {x.v * c, push(x.id, c, -1, 0.0), x.d * c}; return r; }
 
 /* ------------------------------------------------------------ the model -- */
 #define NG VC_MAX_NG
@@ -110,15 +134,22 @@ static int type_of64(const vc_v1_params *p, uint32_t local) {
  * LIF model), sLGN [maxLGN][nV1] and conMat [nblock][maxNear][bs][bs]
  * (float64 weights), lgnSInfo [T][nLGN].  Writes sInfo [T][nV1].  If gOut
  * ([T][nV1] dL/dsInfo) is given, the gradient is written to dsLGN and dConMat
- * (same shapes as the weights).  Returns 0, or -1 if the tape ran out of
- * memory, -2 for an unsupported model.
+ * (same shapes as the weights).  Tangent mode (no tape): uSLGN and uConMat
+ * (NULL = zero) are a direction in weight space and dOut [T][nV1] receives
+ * the directional derivative of every output; anchor [T][VC_SNAP_ROWS][nV1]
+ * (may be NULL) resets the primal state of every step to a given trajectory
+ * (rows as in a snapshot, the cortical conductances after the step's
+ * recurrent increments).  Returns 0, or -1 if the tape ran out of memory,
+ * -2 for an unsupported model.
  */
 int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const float *tonicDep,
                  const float *wstate, uint32_t maxLGN, const uint32_t *nLGN_of,
                  const uint32_t *idLGN, const double *sLGN, const float *lgnSInfo,
                  uint32_t nLGN, uint32_t maxNear, const uint32_t *nNear, const uint32_t *nbr,
                  const double *conMat, const uint8_t *pattern, double *sInfo,
-                 const double *gOut, double *dsLGN, double *dConMat) {
+                 const double *gOut, double *dsLGN, double *dConMat,
+                 const double *uSLGN, const double *uConMat, const float *anchor,
+                 double *dOut) {
     const uint32_t nV1 = p->nV1, bs = p->blockSize, nb = p->nblock;
     const double dt = p->dt;
     if (p->iModel != 0) return -2;
@@ -181,6 +212,7 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
                     const uint32_t pg = bid * bs + pre;
                     const size_t wi = (((size_t)b * maxNear + ib) * bs + pre) * bs + post;
                     num w = W ? W[wi] : K(conMat[wi]);
+                    if (uConMat) w.d = uConMat[wi];
                     if (w.id < 0 && w.v == 0.0) continue;
                     if (pre < p->nE) {
                         for (uint32_t ig = 0; ig < p->ngE; ++ig) {
@@ -195,6 +227,19 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
                             s.hi[ig] = add(s.hi[ig], mul(sw, oh[ig * nV1 + pg]));
                         }
                     }
+                }
+            }
+            if (anchor) {          /* the primal state of the given trajectory */
+                const float *a = anchor + (size_t)it * VC_SNAP_ROWS * nV1 + n;
+                s.v.v = a[(size_t)VC_SNAP_V * nV1];
+                s.tB.v = a[(size_t)VC_SNAP_TBACK * nV1];
+                for (int ig = 0; ig < NG; ++ig) {
+                    s.gF[ig].v = a[(size_t)(VC_SNAP_GFF + ig) * nV1];
+                    s.hF[ig].v = a[(size_t)(VC_SNAP_HFF + ig) * nV1];
+                    s.ge[ig].v = a[(size_t)(VC_SNAP_GE + ig) * nV1];
+                    s.he[ig].v = a[(size_t)(VC_SNAP_HE + ig) * nV1];
+                    s.gi[ig].v = a[(size_t)(VC_SNAP_GI + ig) * nV1];
+                    s.hi[ig].v = a[(size_t)(VC_SNAP_HI + ig) * nV1];
                 }
             }
             const double vThres = p->vThres[t], gL = p->gL[t], Cm = p->C[t], vT = p->vT[t];
@@ -229,7 +274,9 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
                 if (!(sI >= 1.0f)) continue;
                 const double nsp = floor((double)sI), tsp = ((double)sI - nsp) * dt;
                 for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
-                    num f = scl(F ? F[lid] : K(sLGN[lid]), p->pFF[t * p->ngFF + ig] * nsp);
+                    num fw = F ? F[lid] : K(sLGN[lid]);
+                    if (uSLGN) fw.d = uSLGN[lid];
+                    num f = scl(fw, p->pFF[t * p->ngFF + ig] * nsp);
                     input(&o.gF[ig], &o.hF[ig], f, K(dt - tsp), p->riseFF[ig], p->decayFF[ig],
                           p->dodFF[ig]);
                     if (backing && s.tB.v - tsp > 0.0)
@@ -313,6 +360,7 @@ int orc_v1_lif64(const vc_v1_params *p, uint32_t T, const float *init, const flo
             }
             spkNext[n] = (uint8_t)fired;
             sInfo[(size_t)it * nV1 + n] = out.v;
+            if (dOut) dOut[(size_t)it * nV1 + n] = out.d;
             outId[(size_t)it * nV1 + n] = out.id;
             o.v = vNew;
             o.tB = tBNew;

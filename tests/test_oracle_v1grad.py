@@ -83,3 +83,62 @@ def test_soft_count_loss_gradient(case):
         fd = (O.soft_count_loss64(sp_, tgt, spec.dt, 40, 1.0)[0] -
               O.soft_count_loss64(sm_, tgt, spec.dt, 40, 1.0)[0]) / (2 * e)
         assert g[it[k], n[k]] == pytest.approx(fd, rel=1e-5, abs=1e-12)
+
+
+def test_v1_tangent_mode_equals_tape_and_finite_differences(case):
+    """The forward-mode derivative (dual numbers) along a random direction of
+    all weights equals the tape's reverse-mode gradient dotted with that
+    direction, and the central difference of the float64 loss along it; with
+    the trajectory's own states as the anchor it is unchanged, and anchored to
+    the float32 trajectory it stays close over this short run."""
+    import oracle as O
+    spec, g64, s32 = case
+    pat = s32 >= 1
+    dt, T = spec.dt, s32.shape[0]
+    tgt = np.random.default_rng(0).random((T // 40, spec.nV1)) * 0.5
+    s64 = g64.run(pattern=pat)
+    L, gOut = O.soft_count_loss64(s64, tgt, dt, 40, 1.0)
+    _, dsl, dcm = g64.run(pattern=pat, gOut=gOut)
+    rng = np.random.default_rng(4)
+    uc = rng.standard_normal(g64.conMat.shape) * (g64.conMat != 0) * np.abs(g64.conMat)
+    us = rng.standard_normal(g64.sLGN.shape) * np.abs(g64.sLGN)
+    dOut = g64.tangent(pat, uConMat=uc, uSLGN=us)
+    fwd = float((gOut * dOut).sum())
+    rev = float((dcm * uc).sum() + (dsl * us).sum())
+    assert abs(fwd) > 1e-6
+    assert fwd == pytest.approx(rev, rel=1e-9)
+    e = 1e-7
+    lp = O.soft_count_loss64(g64.run(conMat=g64.conMat + e * uc, sLGN=g64.sLGN + e * us,
+                                     pattern=pat), tgt, dt, 40, 1.0)[0]
+    lm = O.soft_count_loss64(g64.run(conMat=g64.conMat - e * uc, sLGN=g64.sLGN - e * us,
+                                     pattern=pat), tgt, dt, 40, 1.0)[0]
+    assert fwd == pytest.approx((lp - lm) / (2 * e), rel=1e-5)
+    # anchored to the float32 trajectory (the oracle's state at the start of
+    # every step): the same spike pattern, a trajectory within float32
+    # rounding drift of the float64 one, hence nearly the same derivative
+    anchor = _float32_trajectory(g64, T)
+    d_anc = g64.tangent(pat, uConMat=uc, uSLGN=us, anchor=anchor)
+    fa = float((gOut * d_anc).sum())
+    assert fa == pytest.approx(fwd, rel=2e-3)
+    assert not np.array_equal(d_anc, dOut)          # the anchor was applied
+
+
+def _float32_trajectory(g64, T):
+    """[T][14][nV1]: the float32 oracle's state at the start of every step
+    (snapshot rows; the cortical conductances after the step's recurrent
+    increments, which the oracle adds at the end of the step before)."""
+    import oracle as O
+    import importlib
+    pkg = importlib.import_module("visual_cortex_amd")
+    net = g64.net
+    nV1 = net.spec.nV1
+    o = O.OracleSim(net, pkg)
+    out = np.zeros((T, 14, nV1), np.float32)
+    rows = (("v", 0, 1), ("tBack", 1, 1), ("gFF", 2, 2), ("hFF", 4, 2), ("gE", 6, 2),
+            ("hE", 8, 2), ("gI", 10, 2), ("hI", 12, 2))
+    for it in range(T):
+        for k, r0, nr in rows:
+            a = o.st[k].reshape(-1, nV1)
+            out[it, r0:r0 + min(nr, a.shape[0])] = a[:nr]
+        o.run(1, record=False)
+    return out
