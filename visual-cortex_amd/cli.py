@@ -72,6 +72,14 @@ def main(argv=None) -> int:
     t0 = time.time()
     sim.run_with_outputs(sp.nt, paths, outputs, on_step=take_snapshot)
     print(f"simulated {sp.nt} steps ({sp.nt * sp.dt:g} ms) in {time.time() - t0:.1f}s", flush=True)
+    if not cfg.bool("minimal", False):          # src/patch.cu:6952-7028
+        from visual_cortex_amd import formats
+        from visual_cortex_amd.network import view_geometry
+        x0, y0, nvd = view_geometry(sp)
+        sr = cfg.floats("sRatioV1", [1.0])
+        formats.write_patchv1_cfg(cfg.path("fPatchV1_cfg"), sp, sp.nLGN,
+                                  cfg.path("fStimulus"), (nvd, x0, y0, 1.0, 1.0),
+                                  sRatioV1=sr * 4 if len(sr) == 1 else sr)
     return 0
 
 

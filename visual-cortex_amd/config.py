@@ -33,6 +33,10 @@ FILES = {
     "fRawData": ("rawData", "outputFolder", "output_suffix"),
     "fLGN_fr": ("LGN_fr", "outputFolder", "output_suffix"),
     "fLGN_sp": ("LGN_sp", "outputFolder", "output_suffix"),
+    "fLGN_surfaceID": ("LGN_surfaceID", "resourceFolder", "res_suffix"),   # src/patch.cu:2573
+    "fConStats": ("conStats", "inputFolder", "conV1_suffix"),              # src/patch.cu:354, 477
+    "fV1_feature": ("V1_feature", "resourceFolder", "res_suffix"),         # src/patch.cu:350, 422
+    "fPatchV1_cfg": ("patchV1_cfg", "outputFolder", "output_suffix"),      # src/patch.cu:362, 507
 }
 
 # options that hold vectors (everything else is a scalar; the last value wins)

@@ -17,6 +17,10 @@ the reference's formats):
                           neuron presynaptic ids, strengths, distances)
   LGN_V1_idList / sList   src/util/util.h:161-185, src/patch.h:30-77
   fLGN (old LGN setup)    src/patch.cu:2472-2530
+  LGN_vpos<res>.bin       src/patch.cu:1524-1533, 1697-1714, 1758-1772
+  LGN_surfaceID<res>.bin  src/patch.cu:2573-2598
+  conStats<conV1>.bin     src/patch.cu:4119-4133 (ffRatio of every neuron)
+  V1_feature<res>.bin     src/patch.cu:2915-2927 (OD, OP, ... per neuron)
 Outputs:
   rawData<out>.bin        header src/patch.cu:4688-4716; per step sInfo
                           (:6139-6164), the dep/w/v/gFF/hFF block, then
@@ -24,6 +28,8 @@ Outputs:
                           with the semantics of src/readPatchOutput.py:291-351
   LGN_fr<out>.bin         src/patch.cu:4666-4686, per step at :6346
   LGN_sp<out>.bin         per step sInfo[nLGN] (src/patch.cu:6422-6424)
+  patchV1_cfg<out>.bin    the run's parameters, src/patch.cu:6953-7028 (read
+                          by src/readPatchOutput.py:32-88)
 """
 from __future__ import annotations
 
@@ -469,3 +475,134 @@ class StepWriter:
 
     def close(self):
         self.f.close()
+
+
+# ------------------------------------------------------------ LGN positions
+def write_lgn_vpos(path, nLGN_I, nLGN_C, max_ecc, x0, xspan, y0, yspan, x, y, types, polar,
+                   ecc_deg):
+    """LGN_vpos: u32 nLGN_I, nLGN_C; f32 max_ecc; f32 x0, xspan, y0, yspan;
+    f32 x[n], y[n]; u32 type[n]; f32 polar[n] (rad), ecc[n] (deg)."""
+    n = nLGN_I + nLGN_C
+    with open(path, "wb") as f:
+        f.write(struct.pack("<2I", nLGN_I, nLGN_C))
+        f.write(struct.pack("<5f", max_ecc, x0, xspan, y0, yspan))
+        for a, dt in ((x, "<f4"), (y, "<f4"), (types, "<u4"), (polar, "<f4"), (ecc_deg, "<f4")):
+            a = np.ascontiguousarray(a, dtype=dt)
+            if a.size != n:
+                raise ValueError("LGN_vpos arrays must have nLGN_I + nLGN_C entries")
+            f.write(a.tobytes())
+
+
+def read_lgn_vpos(path) -> dict:
+    with open(path, "rb") as f:
+        nI, nC = _u32(f), _u32(f)
+        max_ecc, x0, xspan, y0, yspan = (float(v) for v in _f32(f, 5))
+        n = nI + nC
+        x, y = _f32(f, n), _f32(f, n)
+        types = np.fromfile(f, dtype="<u4", count=n)
+        polar, ecc = _f32(f, n), _f32(f, n)
+        if types.size != n or ecc.size != n:
+            raise ValueError(f"{path}: truncated")
+    return dict(nLGN_I=nI, nLGN_C=nC, nLGN=n, max_ecc=max_ecc, x0=x0, xspan=xspan, y0=y0,
+                yspan=yspan, x=x, y=y, type=types, polar=polar, ecc_deg=ecc)
+
+
+def write_lgn_surface_id(path, sx, sy):
+    """LGN_surfaceID: u32 max_xid, max_yid; i32 x[nLGN], y[nLGN]."""
+    sx = np.ascontiguousarray(sx, dtype="<i4")
+    sy = np.ascontiguousarray(sy, dtype="<i4")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<2I", int(sx.max()), int(sy.max())))
+        f.write(sx.tobytes())
+        f.write(sy.tobytes())
+
+
+def read_lgn_surface_id(path, nLGN: int):
+    """(nsx, nsy, x[nLGN], y[nLGN]): the surface is (max_xid + 1) x (max_yid +
+    1) (src/patch.cu:2582-2594); every id must lie on it, and no two cells may
+    share a position (each cell's spike record has its own texel)."""
+    with open(path, "rb") as f:
+        nsx, nsy = _u32(f) + 1, _u32(f) + 1
+        xy = np.fromfile(f, dtype="<i4", count=2 * nLGN)
+    if xy.size != 2 * nLGN:
+        raise ValueError(f"{path}: truncated")
+    x, y = xy[:nLGN], xy[nLGN:]
+    if nLGN and (x.min() < 0 or y.min() < 0 or x.max() >= nsx or y.max() >= nsy):
+        raise ValueError(f"{path}: a surface id lies outside the {nsx} x {nsy} surface")
+    if np.unique(y.astype(np.int64) * nsx + x).size != nLGN:
+        raise ValueError(f"{path}: two LGN cells share a surface position")
+    return nsx, nsy, x, y
+
+
+# ------------------------------------------------------- connection statistics
+def write_constats(path, nType: int, ffRatio):
+    """conStats: u32 nType, nV1; f32 ffRatio[nV1] (the part the simulator
+    reads; genCon writes further statistics after it)."""
+    ff = np.ascontiguousarray(ffRatio, dtype="<f4")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<2I", nType, ff.size))
+        f.write(ff.tobytes())
+
+
+def read_constats(path, nType: int, nV1: int) -> np.ndarray:
+    with open(path, "rb") as f:
+        nt, n = _u32(f), _u32(f)
+        if nt != nType or n != nV1:
+            raise ValueError(f"{path}: nType {nt}, nV1 {n}; expected {nType}, {nV1}")
+        ff = _f32(f, n)
+    if np.isnan(ff).any():
+        raise ValueError(f"{path}: ffRatio holds NaN")
+    return ff
+
+
+def write_v1_feature(path, features):
+    """V1_feature: u32 nFeature; f32 [nFeature][nV1] (feature 0 ocular
+    dominance, feature 1 orientation preference in [0, 1))."""
+    a = np.ascontiguousarray(features, dtype="<f4")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<I", a.shape[0]))
+        f.write(a.tobytes())
+
+
+def read_v1_feature(path, nV1: int) -> np.ndarray:
+    with open(path, "rb") as f:
+        nF = _u32(f)
+        a = _f32(f, nF * nV1)
+    return a.reshape(nF, nV1)
+
+
+# ------------------------------------------------------------- patchV1_cfg
+def write_patchv1_cfg(path, sp, nLGN: int, stimulus_filename: str, view: tuple,
+                      sRatioV1=(1.0, 1.0, 1.0, 1.0), noisyDep=(0.0, 0.0), virtual_LGN=False,
+                      learning: int = 0, synFail=(0.0, 0.0, 0.0, 0.0), synFailFF=(0.0, 0.0)):
+    """The run's parameters in the reference's patchV1_cfg layout
+    (src/patch.cu:6953-7028), Float = f32: two neuron types (E, I), one
+    level of each in the type hierarchy.  view = (normViewDistance, L_x0,
+    L_y0, R_x0, R_y0).  The initial-condition distributions v0/w0/gFF0/gE0/
+    gI0 (the reference's [mean, std] per type) and spE0/spI0 are written as
+    zeros: this simulator draws its initial state itself."""
+    f4 = lambda v: np.asarray(v, dtype="<f4").tobytes()
+    u4 = lambda v: np.asarray(v, dtype="<u4").tobytes()
+    ngFF, ngE, ngI = len(sp.riseTimeFF), len(sp.riseTimeE), len(sp.riseTimeI)
+    fn = stimulus_filename.encode()
+    flat = lambda rows: [x for r in rows for x in r]
+    parts = [
+        u4([4]), f4([-65.0, 0.0, -75.0]), u4([2, 1, 1]),
+        f4(sp.vR), f4(sp.vThres), f4(sp.gL), f4(sp.vT), u4([1, 1]),
+        u4([sp.nE, sp.blockSize]), f4(sp.sRatioLGN), f4(sRatioV1),
+        f4([sp.frRatioLGN, sp.convolRatio]), u4([sp.frameRate]), u4([len(fn)]), fn,
+        u4([nLGN, sp.nV1, sp.nt]), f4([sp.dt]), f4(view),
+        f4(sp.tonicDep), f4(noisyDep), np.asarray([int(virtual_LGN)], "<i4").tobytes(),
+        np.asarray([sp.seed], "<u8").tobytes(), np.asarray([sp.iModel, learning], "<i4").tobytes(),
+        u4([ngFF, ngE, ngI]),
+        f4(sp.riseTimeFF), f4(sp.decayTimeFF), f4(sp.riseTimeE), f4(sp.decayTimeE),
+        f4(sp.riseTimeI), f4(sp.decayTimeI),
+        f4(flat(sp.pFF)), f4(flat(sp.pE)), f4(flat(sp.pI)),
+        f4(sp.C), f4(sp.tRef), f4(sp.a), f4(sp.b), f4(sp.tau_w), f4(sp.deltaT),
+        f4([0.0, 0.0]), f4([0.0, 0.0]),                       # spE0, spI0
+        f4([0.0] * 4) * 5,                                    # v0, w0, gFF0, gE0, gI0
+        f4(synFail), f4(synFailFF),
+    ]
+    with open(path, "wb") as f:
+        for b in parts:
+            f.write(b)

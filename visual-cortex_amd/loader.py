@@ -5,10 +5,13 @@ inputs drive this simulator and synthetic workloads can be exported.
 Reads (see formats.py for the layouts and reference lines): V1_allpos,
 connectome_cfg, neighborBlock, V1_conMat, V1_delayMat (noDelay = 0), V1_vec
 (far connections; optional here, the reference requires it),
-LGN_V1_idList / LGN_V1_sList,
-fLGN and stimulus.bin.  The neuron and synapse parameters come from the cfg
-keys (config.spec_from_config); sRatioV1 scales the V1 strengths on reading
-and sRatioLGN the LGN ones, as the reference does.
+LGN_V1_idList / LGN_V1_sList, LGN_vpos, LGN_surfaceID, fLGN, conStats,
+V1_feature (when the cfg asks for features) and stimulus.bin.  The neuron and
+synapse parameters come from the cfg keys (config.spec_from_config); sRatioV1
+scales the V1 strengths on reading and sRatioLGN the LGN ones, and the tonic
+drive of every neuron follows from tonicDep, CmoreDep, minTonicRatio and
+boostOri with conStats' ffRatio and V1_feature's orientation preference, as
+the reference computes them.
 """
 from __future__ import annotations
 
@@ -52,12 +55,25 @@ def load_network(cfg: cfgmod.Config) -> Network:
                c50=L["c50"].astype(np.float32), sharpness=L["sharpness"].astype(np.float32),
                spont=L["spont"].astype(np.float32), logA=a, logB=b_)
     frames = st["frames"]
+    # LGN_vpos gives the cell count and types the reference sizes the LGN by
+    vpos = F.read_lgn_vpos(cfg.path("fLGN_vpos"))
+    if vpos["nLGN"] != nL:
+        raise ValueError(f"LGN_vpos holds {vpos['nLGN']} cells, fLGN {nL}")
+    if "type" in L and not np.array_equal(np.asarray(L["type"], np.uint32), vpos["type"]):
+        raise ValueError("LGN_vpos and fLGN disagree on the LGN cell types")
+    if vpos["max_ecc"] > st["range_deg"]:            # src/patch.cu:1536-1538
+        raise ValueError(f"LGN_vpos max_ecc {vpos['max_ecc']} exceeds the stimulus range "
+                         f"{st['range_deg']}")
+    F.read_lgn_surface_id(cfg.path("fLGN_surfaceID"), nL)
     spec = replace(spec, nblock=nblock, blockSize=bs, nE=int(typeAcc[0]), nLGN_override=nL,
                    width=frames.shape[3], height=frames.shape[2],
                    stim_deg=2.0 * st["range_deg"], stim_buffer_deg=st["buffer_deg"],
                    m=int(nof.max()) if nof.size else 0)
-    tonic = np.where((np.arange(nblock * bs) % bs) < spec.nE, spec.tonicDep[0],
-                     spec.tonicDep[1]).astype(np.float32)
+    ffRatio = F.read_constats(cfg.path("fConStats"), nType, nblock * bs)
+    feature = None
+    if feature_needed(cfg):
+        feature = F.read_v1_feature(cfg.path("fV1_feature"), nblock * bs)
+    tonic = tonic_drive(cfg, spec.tonicDep, typeAcc, nblock, bs, ffRatio, feature)
     rec_dist = None
     if spec.noDelay == 0:                    # the reference's own distances
         rec_dist = F.read_delaymat_pairs(cfg.path("fV1_delayMat"), nblock, bs, pairs)
@@ -70,6 +86,112 @@ def load_network(cfg: cfgmod.Config) -> Network:
     return Network(spec=spec, pos=pos, tonicDep=tonic, maxNear=maxNear, nNear=nNear, nbr=nbr,
                    rec_pairs=pairs, nLGN_of=nof, sLGN=sLGN, idLGN=idLGN, lgn=lgn,
                    frames=frames, init_lms=tuple(st["init_lms"]), rec_dist=rec_dist, far=far)
+
+
+def _per_type(vals, nType, key, per=1):
+    """A cfg vector of 1 or nType entries (of `per` values each) -> nType*per."""
+    if len(vals) == per and nType > 1:
+        vals = list(vals) * nType
+    if len(vals) != nType * per:
+        raise ValueError(f"{key} needs {per} or {nType * per} values, got {len(vals)}")
+    return list(vals)
+
+
+def _orientation_options(cfg, nType: int = 2):
+    """(readFeature, fOri[nType], boostOri[2*nType] or None, nOri) after the
+    reference's checks of nOri, iOri and boostOri (src/patch.cu:2843-2910)."""
+    nOri = cfg.int("nOri", 0)
+    read_feature = cfg.bool("readFeature", True)
+    boost = cfg.floats("boostOri")
+    iOri = cfg.floats("iOri")
+    fOri = [0.0] * nType
+    if boost is not None:
+        boost = _per_type(boost, nType, "boostOri", 2)
+    if nOri > 0:
+        if boost is not None:
+            read_feature = True
+            if any(boost[2 * t + 1] != 0 for t in range(nType)) and iOri is None:
+                raise ValueError("when boostOri[std] is set, iOri needs to be set")
+        if iOri is not None:
+            iOri = _per_type([int(x) for x in iOri], nType, "iOri")
+            read_feature = True
+            f32 = np.float32
+            fOri = [float(np.fmod(f32(f32(i % nOri) / f32(nOri)) + f32(0.5), f32(1.0)))
+                    for i in iOri]
+        elif boost is not None:
+            for t in range(nType):
+                boost[2 * t + 1] = 0.0
+    elif boost is not None:
+        for t in range(nType):
+            boost[2 * t + 1] = 0.0
+    return read_feature, fOri, boost, nOri
+
+
+def feature_needed(cfg) -> bool:
+    """Whether the run reads V1_feature: readFeature (default true, or set by
+    nOri > 0 with iOri/boostOri).  The reference then asserts nOri > 0 and
+    both iOri and boostOri (src/patch.cu:2914-2931); so does this loader."""
+    read_feature, _, boost, nOri = _orientation_options(cfg)
+    if read_feature and (nOri <= 0 or cfg.floats("iOri") is None or boost is None):
+        raise ValueError("readFeature needs nOri > 0, iOri and boostOri (the reference "
+                         "asserts them); set readFeature = false otherwise")
+    return read_feature
+
+
+def tonic_drive(cfg, tonicDep, typeAcc, nblock: int, bs: int, ffRatio, feature):
+    """Tonic depolarisation of every neuron (src/patch.cu:4118-4222), float32:
+    tonicDep[type] * boost * (minTonicRatio + (1 - minTonicRatio) * (1 -
+    ffRatio - (1 - max ffRatio)) / (max ffRatio - min ffRatio)) with CmoreDep
+    (the min/max over the type's neurons in every block), tonicDep[type] *
+    boost without; boost is the von Mises orientation boost of boostOri
+    around iOri/nOri when the orientation features are read, else 1."""
+    f32 = np.float32
+    nType = len(typeAcc)
+    typeAcc0 = [0] + [int(x) for x in typeAcc]
+    nV1 = nblock * bs
+    tonic = [f32(x) for x in tonicDep]
+    more = cfg.bool("CmoreDep", True)
+    mtr = cfg.floats("minTonicRatio")
+    if mtr is None:
+        raise ValueError("minTonicRatio needs 1 or nType values")
+    mtr = [f32(x) for x in _per_type(mtr, nType, "minTonicRatio")]
+    read_feature, fOri, boost, nOri = _orientation_options(cfg, nType)
+    ff = np.asarray(ffRatio, f32)
+    local = np.arange(nV1) % bs
+    typ = np.searchsorted(np.asarray(typeAcc0[1:]), local, side="right")
+    mx = np.array([max(ff.reshape(nblock, bs)[:, typeAcc0[j]:typeAcc0[j + 1]].max(), f32(0))
+                   for j in range(nType)], f32)
+    mn = np.array([min(ff.reshape(nblock, bs)[:, typeAcc0[j]:typeAcc0[j + 1]].min(), f32(1))
+                   for j in range(nType)], f32)
+    boosted = np.ones(nV1, f32)
+    if boost is not None and read_feature and nOri > 0:
+        op = np.asarray(feature[1], f32)
+        dOri = (op - np.asarray(fOri, f32)[typ]).astype(f32)
+        dOri = np.where(dOri > f32(0.5), f32(1) - dOri, dOri).astype(f32)
+        dOri = np.where(dOri < f32(-0.5), f32(1) + dOri, dOri).astype(f32)
+        base = np.asarray(boost[0::2], f32)[typ]
+        kap = np.asarray(boost[1::2], f32)[typ]
+        amp = (f32(1) - base).astype(f32)
+        vm = (f32(1) - np.exp(f32(-2) * kap)).astype(f32)
+        cosv = np.cos((dOri.astype(np.float64) * np.pi * 2.0).astype(f32)).astype(f32)
+        vmx = np.where(vm == 0, f32(1), vm).astype(f32)
+        ev = np.exp((kap * (cosv - f32(1))).astype(f32)).astype(f32)
+        bv = (base + (amp * ((ev - f32(1)) + vm)) / vmx).astype(f32)   # a*(e-1+vm)/vm
+        boosted = np.where(vm == 0, base, bv).astype(f32)
+    td = np.asarray(tonic, f32)[typ]
+    if more:
+        if np.any(boosted > 1):
+            raise ValueError("an orientation boost above 1 (the reference asserts boost <= 1)")
+        rel = (f32(1) - ff).astype(f32)
+        minT = (f32(1) - mx[typ]).astype(f32)
+        maxT = (f32(1) - mn[typ]).astype(f32)
+        m = np.asarray(mtr, f32)[typ]
+        span = np.where(mx[typ] > mn[typ], maxT - minT, f32(1)).astype(f32)
+        scaled = ((td * boosted) * (m + ((f32(1) - m) * (rel - minT)) / span)).astype(f32)
+        out = np.where(mx[typ] > mn[typ], scaled, (td * boosted).astype(f32))
+    else:
+        out = (td * boosted).astype(f32)
+    return np.ascontiguousarray(out, np.float32)
 
 
 def scale_v1(pairs, sratio, nE: int):
@@ -94,10 +216,36 @@ def scale_v1(pairs, sratio, nE: int):
     return out
 
 
-def save_network(net: Network, folder, suffix: str = "vc") -> Path:
+def lgn_positions(net: Network):
+    """The LGN_vpos content of a network's cells: all ipsilateral, positions
+    x = ecc cos(polar), y = ecc sin(polar) in degrees, type 0 (as fLGN is
+    written), max_ecc the largest eccentricity (deg)."""
+    L = net.lgn
+    ecc_deg = np.asarray(L["ecc"][0], np.float64) * 180.0 / math.pi
+    pol = np.asarray(L["polar"][0], np.float64)
+    x = (ecc_deg * np.cos(pol)).astype(np.float32)
+    y = (ecc_deg * np.sin(pol)).astype(np.float32)
+    n = x.size
+    return dict(nLGN_I=n, nLGN_C=0, max_ecc=float(ecc_deg.max()) if n else 0.0,
+                x0=float(x.min()), xspan=float(np.ptp(x)), y0=float(y.min()),
+                yspan=float(np.ptp(y)), x=x, y=y, types=np.zeros(n, np.uint32),
+                polar=pol.astype(np.float32), ecc_deg=ecc_deg.astype(np.float32))
+
+
+def lgn_surface_ids(n: int):
+    """A square LGN surface with one texel per cell, row-major."""
+    side = max(1, int(math.ceil(math.sqrt(n))))
+    i = np.arange(n)
+    return (i % side).astype(np.int32), (i // side).astype(np.int32)
+
+
+def save_network(net: Network, folder, suffix: str = "vc", tonic: dict | None = None) -> Path:
     """Write `net` in the reference's input formats under `folder` and return
     the path of a cfg that names them (inputFolder = resourceFolder =
-    outputFolder = folder)."""
+    outputFolder = folder).  The tonic drive is written as it is (CmoreDep =
+    false, no orientation boost) unless `tonic` gives the cfg keys (CmoreDep,
+    minTonicRatio, boostOri, iOri, nOri, readFeature) and the arrays ffRatio
+    [nV1] and feature [nFeature][nV1] of conStats and V1_feature."""
     sp = net.spec
     folder = Path(folder)
     folder.mkdir(parents=True, exist_ok=True)
@@ -117,6 +265,25 @@ def save_network(net: Network, folder, suffix: str = "vc") -> Path:
     F.write_lgn_v1(folder / f"LGN_V1_idList{suf}", folder / f"LGN_V1_sList{suf}", ids, ss)
     L = dict(net.lgn)
     F.write_flgn(folder / f"LGN{suf}", L)
+    vp = lgn_positions(net)
+    F.write_lgn_vpos(folder / f"LGN_vpos{suf}", vp["nLGN_I"], vp["nLGN_C"], vp["max_ecc"],
+                     vp["x0"], vp["xspan"], vp["y0"], vp["yspan"], vp["x"], vp["y"], vp["types"],
+                     vp["polar"], vp["ecc_deg"])
+    F.write_lgn_surface_id(folder / f"LGN_surfaceID{suf}", *lgn_surface_ids(sp.nLGN))
+    tonic = tonic or {}
+    ff = tonic.get("ffRatio")
+    if ff is None:      # a stand-in: the share of LGN strength in each neuron's input
+        fsum = net.sLGN.astype(np.float64).sum(0)
+        rsum = np.zeros(sp.nV1)
+        for b in range(sp.nblock):
+            for ib in range(net.maxNear):
+                pre, post, w = net.rec_pairs[b * net.maxNear + ib] if net.csr is None else \
+                    (np.zeros(0, int), np.zeros(0, int), np.zeros(0))
+                np.add.at(rsum, b * sp.blockSize + post.astype(np.int64), w)
+        ff = (fsum / np.maximum(fsum + rsum, 1e-30)).astype(np.float32)
+    F.write_constats(folder / f"conStats{suf}", 2, ff)
+    if tonic.get("feature") is not None:
+        F.write_v1_feature(folder / f"V1_feature{suf}", tonic["feature"])
     F.write_stimulus(folder / "stimulus.bin", net.frames, net.init_lms,
                      buffer_deg=sp.stim_buffer_deg, range_deg=sp.stim_deg / 2.0)
     t = lambda v: ", ".join(repr(float(x)) for x in v)
@@ -135,6 +302,14 @@ def save_network(net: Network, folder, suffix: str = "vc") -> Path:
     ]
     for k in ("vR", "vThres", "gL", "C", "tRef", "vT", "deltaT", "tau_w", "a", "b", "tonicDep"):
         lines.append(f"{k} = {t(getattr(sp, k))}")
+    keys = dict(CmoreDep="false", minTonicRatio="1.0", readFeature="false")
+    keys.update({k: v for k, v in tonic.items() if k not in ("ffRatio", "feature")})
+    for k, v in keys.items():
+        if isinstance(v, (list, tuple)):
+            for j in range(0, len(v), 2 if k == "boostOri" else len(v)):
+                lines.append(f"{k} = {t(v[j:j + (2 if k == 'boostOri' else len(v))])}")
+        else:
+            lines.append(f"{k} = {str(v).lower() if isinstance(v, bool) else v}")
     for k in ("riseTimeFF", "decayTimeFF", "riseTimeE", "decayTimeE", "riseTimeI", "decayTimeI"):
         lines.append(f"{k} = {t(getattr(sp, k))}")
     for k in ("pFF", "pE", "pI", "pBG"):
