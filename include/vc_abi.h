@@ -99,13 +99,23 @@ typedef struct {
     uint32_t stepWG, gatherTile;
 } vc_v1_params;
 
-/* Persistent V1 state, all [..][nV1] float32, receptor-major. */
+/* Persistent V1 state, all [..][nV1] float32, receptor-major.
+ * gapS and gap (may be NULL = zero) are the reference's d_gapS and d_gap
+ * over the mI = nblock*nI inhibitory neurons, inhibitory neuron l >= nE of
+ * block b at [b*nI + l - nE]: gapS = the sum of its gap-junction strengths
+ * (src/patch.cu:3360-3396, 3686-3723), gap = sum over its junctions of
+ * strength x the partner's voltage (vThres when it spiked) in the step
+ * before (vc_v1_gap).  The step adds gapS to the membrane's a0/a1 and gap
+ * to b0/b1 of inhibitory neurons (src/coredynamics.cuh:107-117,
+ * src/coredynamics.cu:1344, 1536-1543). */
 typedef struct {
     float *v, *w, *tBack;
     float *gFF, *hFF;          /* [ngFF][nV1] */
     float *gE, *hE;            /* [ngE][nV1]  */
     float *gI, *hI;            /* [ngI][nV1]  */
     const float *tonicDep;     /* [nV1]       */
+    const float *gapS;         /* [mI] or NULL */
+    float *gap;                /* [mI] or NULL */
 } vc_v1_state;
 
 /* LGN->V1 synapses, ELL layout transposed like the reference's sLGN
@@ -152,6 +162,17 @@ typedef struct {
      * straight from its row (one round trip fewer than rowMask/rowOff/w);
      * the results are identical.  The adjoint always uses w. */
     const float *wDense;
+    /* Optional gap junctions between inhibitory neurons (InhGap; gapPre NULL
+     * = none): the strictly positive entries of the reference's
+     * V1_gapMat[nblock][maxNear][nI pre][nI post] (already * gapRatio,
+     * src/patch.cu:3688-3723) as an ELL over the mI inhibitory neurons,
+     * junction j of inhibitory neuron i at [j*mI + i], ordered by neighbour
+     * slot, then presynaptic index (the reference's summation order,
+     * src/coredynamics.cu:2524-2540). */
+    uint32_t maxGap;
+    const uint32_t *nGap;      /* [mI] junctions of each inhibitory neuron      */
+    const uint32_t *gapPre;    /* [maxGap][mI] global id of the partner neuron  */
+    const float *gapG;         /* [maxGap][mI] strength                          */
 } vc_rec_conn;
 
 /* Spikes of one step ("per-block spike bookkeeping"): one 64-bit spike mask
@@ -187,7 +208,9 @@ int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state *s,
  * when ffMask is not NULL) and writes out_sInfo +
  * i*outStride (strides in floats; 0 reuses one row).  After the call the
  * last step's spikes are in spk2[cur ^ (nsteps&1)].  (The reference's host
- * loop, src/patch.cu:5966-6913, launched one step at a time.) */
+ * loop, src/patch.cu:5966-6913, launched one step at a time.)  With gap
+ * junctions (rec->gapPre) each step is followed by vc_v1_gap on its output
+ * row, so s->gap holds the last step's gap on return. */
 int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
               const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
               uint32_t cur, const float *lgnSInfo, uint64_t lgnStride,
@@ -204,8 +227,10 @@ int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
  * vc_v1_run; the last step's spikes are left in spk2 as vc_v1_run leaves them.  Every wait is
  * bounded: on a time-out *status (caller-zeroed, sticky) becomes nonzero
  * and the results of the call are invalid.  Returns VC_E_UNSUPPORTED when
- * the grid (nV1/256 workgroups) cannot be co-resident on the device; use
- * vc_v1_run then.  No counterpart in the reference (its host loop launches
+ * the grid (nV1/256 workgroups) cannot be co-resident on the device, or
+ * with gap junctions (s->gapS, s->gap or rec->gapPre set); use vc_v1_run
+ * then.  A call that returns an error, or has nsteps = 0, launches nothing
+ * (the buffer is not cleared): the next call may pass the same epoch.  No counterpart in the reference (its host loop launches
  * each step, src/patch.cu:5966-6913). */
 int vc_v1_run_persistent(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                          const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
@@ -225,7 +250,8 @@ uint64_t vc_v1_persistent_scratch_bytes(const vc_v1_params *p);
  * one per GPU (8.5 MiB, held until the process ends), which limits nV1 to
  * vc_v1_run_persistent_noalloc_max_v1() = 131072 neurons; larger networks
  * get VC_E_INVALID.  The library cycles the epoch itself, per GPU (the first
- * call on a GPU and every 65535th after it clear the array with a kernel).
+ * launching call on a GPU and every 65535th after it clear the array with a
+ * kernel; refused calls and calls of 0 steps use up no epoch).
  * Every call on one GPU shares that array: calls that overlap on one GPU
  * (other streams, other host threads) race and their results are invalid;
  * calls one after the other are safe.  `stream` must belong to the current
@@ -245,6 +271,24 @@ uint32_t vc_v1_run_persistent_noalloc_max_v1(void);
  * (used after the last step, before reading or saving the state). */
 int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                     const vc_rec_conn *rec, const vc_spikes *prev);
+
+/* vc_v1_gap: s->gap[mI] from rec's gap junctions (rec->gapPre must be set):
+ * for inhibitory neuron i, gap = sum over its junctions j, in the ELL's
+ * order, of gapG[j] * v_pre, with v_pre = train[slot*nV1 + gapPre[j]]
+ * replaced by vThres[partner's type] when it is > 0 (a spike), slot = (cur +
+ * depth - lag[j*mI + i]) % depth (lag may be NULL = 0).  train is a ring of
+ * `depth` step outputs (out_sInfo), the newest in slot cur; without axonal
+ * delays pass the step's out_sInfo, depth 1, cur 0, lag NULL
+ * (recal_G_mat_nd, src/coredynamics.cu:2524-2540, 2600); with delays the
+ * lag of each junction is ceil((distance/speedOfThought)/dt) of the pair
+ * (recal_G_mat, src/coredynamics.cu:2237-2260, 2306).  The reference
+ * computes gap after each step's update, for the next step; vc_v1_run does
+ * that itself, callers of vc_v1_step call vc_v1_gap after it (and once
+ * before the first step, on the initial voltages, as the reference's
+ * pre-loop recal_G does, src/patch.cu:5687-5770). */
+int vc_v1_gap(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+              const vc_rec_conn *rec, const float *train, uint32_t depth, uint32_t cur,
+              const uint32_t *lag);
 
 /* depC[nV1] of every neuron from s->tonicDep, as the step kernels compute it
  * (the AdEx constructor's tonic drive, src/coredynamics.cuh:40-49); the

@@ -32,13 +32,15 @@ class Oracle:
         vp, u32, f32, i32 = C.c_void_p, C.c_uint32, C.c_float, C.c_int
         P = C.POINTER
         L.orc_v1_step.argtypes = [P(abi_mod.V1Params)] + [vp] * 14 + [u32, vp, vp, vp, vp, vp, vp, vp,
-                                                                      u32, vp, vp, i32]
+                                                                      u32, vp, vp, i32, vp, vp]
         L.orc_v1_step.restype = i32
         L.orc_v1_rec_delay.argtypes = [P(abi_mod.V1Params)] + [vp] * 4 + [u32] + [vp] * 4 + \
             [f32, vp, u32, u32]
         L.orc_v1_rec_delay.restype = i32
         L.orc_v1_rec_far.argtypes = [P(abi_mod.V1Params)] + [vp] * 8 + [f32, vp, u32, u32]
         L.orc_v1_rec_far.restype = i32
+        L.orc_v1_gap.argtypes = [P(abi_mod.V1Params), u32, vp, vp, vp, vp, f32, vp, u32, u32, vp]
+        L.orc_v1_gap.restype = i32
         L.orc_lgn_frame_response.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp,
                                              C.c_uint64, vp]
         L.orc_lgn_steps.argtypes = [P(abi_mod.LgnParams)] + [vp] * 11 + [u32, u32, vp, vp, vp]
@@ -158,6 +160,27 @@ class OracleSim:
             fmax = float(self.far[3].max()) if self.far[3].size else 0.0
             self.depth = network.train_depth(fmax, sp.speedOfThought, sp.dt)
             self.train = np.zeros((self.depth, sp.nV1), np.float32)
+        # gap junctions between inhibitory neurons: gapS from the strengths
+        # (every inhibitory neuron, whether or not InhGap is on), the gap sums
+        # when InhGap is on, first from the initial voltages (the pre-loop
+        # recal_G, src/patch.cu:5687-5770)
+        self.gapS = net.gap_strength_sums() if net.has_gap() else None
+        self.gap = None
+        if net.gap_on():
+            self.gapMat = np.ascontiguousarray(net.dense_gapmat())
+            self.gapDist = np.ascontiguousarray(net.dense_gapdist()) if self.delay else None
+            self.gap = np.zeros(sp.nblock * sp.nI, np.float32)
+            if self.delay:
+                lag = net.gap_lags(sp.speedOfThought, sp.dt)
+                need = int(lag.max()) + 1 if lag.size else 1
+                if need > self.depth:
+                    self.depth = need
+                    self.train = np.zeros((self.depth, sp.nV1), np.float32)
+        if self.delay or self.far_only:             # rand_spInit: every slot holds v
+            self.train[:] = self.st["v"][None, :]
+        if self.gap is not None:
+            self._gap(self.train if self.delay else self.st["v"][None, :].copy(),
+                      self.depth - 1 if self.delay else 0)
         self.lTR = np.zeros(sp.nLGN, np.float32)
         self.lNL = np.zeros(sp.nLGN, np.float32)
         self.o.L.orc_lgn_init(C.byref(self.lp), _fp(self.lTR), _fp(self.lNL))
@@ -174,6 +197,17 @@ class OracleSim:
         self.lgn_hist = []
         self.fr_hist = []
         self.convol_hist = []
+
+    def _gap(self, train, cur):
+        """The gap sums from the step outputs in ring slot `cur` of train."""
+        sp = self.net.spec
+        train = np.ascontiguousarray(train, dtype=np.float32)
+        rc = self.o.L.orc_v1_gap(C.byref(self.p), self.net.maxNear, _fp(self.nNear),
+                                 _fp(self.nbr), _fp(self.gapMat), _fp(self.gapDist),
+                                 sp.speedOfThought, _fp(train), train.shape[0], cur,
+                                 _fp(self.gap))
+        if rc != 0:
+            raise RuntimeError(f"orc_v1_gap returned {rc}")
 
     def precompute(self):
         from importlib import import_module
@@ -259,7 +293,8 @@ class OracleSim:
                                       _fp(self.nLGN_of), _fp(self.sLGN), _fp(self.idLGN), _fp(lgn_row),
                                       self.net.maxNear, _fp(self.nNear), _fp(self.nbr),
                                       _fp(self.conMat), _fp(self.ipre), _fp(self.npre), _fp(self.og),
-                                      _fp(self.oh), it, _fp(out), None, 0 if self.delay else 1)
+                                      _fp(self.oh), it, _fp(out), None, 0 if self.delay else 1,
+                                      _fp(self.gapS), _fp(self.gap))
             if rc != 0:
                 raise RuntimeError(f"orc_v1_step returned {rc}")
             if self.delay:                  # recal_G_mat after compute_V, same step
@@ -282,6 +317,11 @@ class OracleSim:
                                              _fp(self.train), self.depth, slot)
                 if rc != 0:
                     raise RuntimeError(f"orc_v1_rec_far returned {rc}")
+            if self.gap is not None:        # recal_G's gap sums, for the next step
+                if self.delay:
+                    self._gap(self.train, it % self.depth)
+                else:
+                    self._gap(out[None, :], 0)
             if record:
                 self.sInfo_hist.append(out)
             self.it += 1

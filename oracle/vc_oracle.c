@@ -193,7 +193,7 @@ static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
                                const float *tonicDep, const uint32_t *nLGN,
                                const float *sLGN, const uint32_t *idLGN,
                                const float *lgnSInfo, uint32_t step,
-                               float *out_dep) {
+                               float *out_dep, const float *gapS, const float *gap) {
     const uint32_t nV1 = p->nV1;
     const uint32_t local = n % p->blockSize;
     const int t = orc_type_of(p, local);
@@ -286,12 +286,21 @@ static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
         gI_t0 += gi[ig];
     }
 
+    /* gap junctions of inhibitory neurons: gapS joins a, gap joins b
+     * (src/coredynamics.cuh:107-117; compute_V_collect_spike_learnFF_fast
+     * takes gapS for every inhibitory neuron, src/coredynamics.cu:1344) */
+    float lgapS = 0.0f, lgap = 0.0f;
+    if (local >= p->nE) {
+        uint32_t gi = (n / p->blockSize) * (p->blockSize - p->nE) + (local - p->nE);
+        if (gapS) lgapS = gapS[gi];
+        if (gap) lgap = gap[gi];
+    }
     m.tsp = 0.0f;
     if (m.tBack < dt) {
-        m.a0 = gE_t0 + gI_t0 + m.gL + 0.0f;
-        m.b0 = gE_t0 * ORC_VE + gI_t0 * ORC_VI + m.gL * ORC_VL + m.depC + 0.0f;
-        m.a1 = gE_t1 + gI_t1 + m.gL + 0.0f;
-        m.b1 = gE_t1 * ORC_VE + gI_t1 * ORC_VI + m.gL * ORC_VL + m.depC + 0.0f;
+        m.a0 = gE_t0 + gI_t0 + m.gL + lgapS;
+        m.b0 = gE_t0 * ORC_VE + gI_t0 * ORC_VI + m.gL * ORC_VL + m.depC + lgap;
+        m.a1 = gE_t1 + gI_t1 + m.gL + lgapS;
+        m.b1 = gE_t1 * ORC_VE + gI_t1 * ORC_VI + m.gL * ORC_VL + m.depC + lgap;
         float new_dt = dt - m.tBack;
         if (backing) {
             float A = m.a * (m.v0 - ORC_VL);
@@ -338,7 +347,8 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
                 const float *lgnSInfo, uint32_t maxNear, const uint32_t *nNear,
                 const uint32_t *nbr, const float *conMat, uint32_t *ipre,
                 uint32_t *npre, float *og, float *oh, uint32_t step,
-                float *out_sInfo, float *out_dep, int do_rec) {
+                float *out_sInfo, float *out_dep, int do_rec,
+                const float *gapS, const float *gap) {
     const uint32_t nV1 = p->nV1, bs = p->blockSize, nblock = p->nblock;
     const uint32_t nE = p->nE, nI = bs - nE;
     if (nV1 != nblock * bs) return VC_E_INVALID;
@@ -350,7 +360,7 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
     for (uint32_t n = 0; n < nV1; ++n) {
         sInfo[n] = orc_update_neuron(p, n, v, w, tBack, gFF, hFF, gE, hE, gI,
                                      hI, tonicDep, nLGN, sLGN, idLGN, lgnSInfo,
-                                     step, out_dep);
+                                     step, out_dep, gapS, gap);
     }
     /* compaction, src/coredynamics.cu:1994-2046, ascending order */
     const uint32_t block_ng = p->ngE * nE + p->ngI * nI;
@@ -437,6 +447,57 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
         }
     }
     if (own) free(sInfo);
+    return VC_OK;
+}
+
+/*
+ * Gap sums of the inhibitory neurons after a step, for the next one
+ * (recal_G_mat_nd, src/coredynamics.cu:2524-2540, 2599-2600; with delays
+ * recal_G_mat, :2237-2260, 2305-2306): for inhibitory neuron m of block b,
+ * neighbour slots ib < nNear[b], partners l < nI (block-local index nE + l)
+ * with gapMat[b][ib][l][m] != 0, gap += strength * v, v = the partner's
+ * step output in ring slot cur (without delays) or cur - it2post with
+ * it2post = ceil((delayMat[b][ib][nE+l][nE+m] / speed) / dt) (delays),
+ * replaced by vThres[partner's type] when > 0 (a spike).
+ *   gapMat [nblock][maxNear][nI][nI]; gapDist [nblock][maxNear][nI][nI] =
+ *   the inhibitory-to-inhibitory part of V1_delayMat (distances in mm), or
+ *   NULL (no delays); train [depth][nV1]; gap [nblock*nI] out.
+ * The reference's recal_G_mat_nd_fast indexes gapMat with (i - nE) for a
+ * partner index i < nI, which wraps below zero; this follows the other two
+ * kernels, where i is the block-local index (>= nE).
+ */
+int orc_v1_gap(const vc_v1_params *p, uint32_t maxNear, const uint32_t *nNear,
+               const uint32_t *nbr, const float *gapMat, const float *gapDist,
+               float speed, const float *train, uint32_t depth, uint32_t cur,
+               float *gap) {
+    const uint32_t nV1 = p->nV1, bs = p->blockSize, nE = p->nE, nI = bs - nE;
+    if (nV1 != p->nblock * bs || depth == 0 || cur >= depth) return VC_E_INVALID;
+#pragma omp parallel for schedule(static)
+    for (uint32_t gi = 0; gi < p->nblock * nI; ++gi) {
+        uint32_t b = gi / nI, m = gi % nI;
+        float s = 0.0f;
+        for (uint32_t ib = 0; ib < nNear[b]; ++ib) {
+            uint32_t bid = nbr[b * maxNear + ib];
+            uint64_t local_bid = (uint64_t)b * maxNear + ib;
+            for (uint32_t l = 0; l < nI; ++l) {
+                float strength = gapMat[(local_bid * nI + l) * nI + m];
+                if (strength == 0.0f) continue;
+                uint32_t slot = cur;
+                if (gapDist) {
+                    float d = gapDist[(local_bid * nI + l) * nI + m];
+                    float time2post = d / speed;
+                    uint32_t it2post = (uint32_t)ceilf(time2post / p->dt);
+                    int64_t j0 = (int64_t)cur - (int64_t)it2post;
+                    while (j0 < 0) j0 += depth;
+                    slot = (uint32_t)(j0 % depth);
+                }
+                float v = train[(uint64_t)slot * nV1 + (uint64_t)bid * bs + nE + l];
+                if (v > 0.0f) v = p->vThres[orc_type_of(p, nE + l)];
+                s += strength * v;
+            }
+        }
+        gap[gi] = s;
+    }
     return VC_OK;
 }
 

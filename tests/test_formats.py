@@ -28,8 +28,7 @@ def test_cfg_semantics(pkg):
     assert c.path("fV1_conMat") == "/abs/conmat.bin-abc.bin"       # '!' drops the folder only
     sp = config.spec_from_config(c)
     assert sp.pFF == ((0.9, 0.1), (0.8, 0.2)) and sp.dt == 0.25
-    with pytest.raises(NotImplementedError):
-        config.spec_from_config(config.Config("InhGap = true"))
+    assert config.spec_from_config(config.Config("InhGap = true")).InhGap == 1
     with pytest.raises(NotImplementedError):
         config.spec_from_config(config.Config("noisyDep = 0.1, 0"))
     sp = config.spec_from_config(config.Config("noDelay = 0\nnoFarDelay = 0\nspeedOfThought = 0.5"))

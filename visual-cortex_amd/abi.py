@@ -58,7 +58,8 @@ class V1Params(C.Structure):
 
 class V1State(C.Structure):
     _fields_ = [("v", vp), ("w", vp), ("tBack", vp), ("gFF", vp), ("hFF", vp),
-                ("gE", vp), ("hE", vp), ("gI", vp), ("hI", vp), ("tonicDep", vp)]
+                ("gE", vp), ("hE", vp), ("gI", vp), ("hI", vp), ("tonicDep", vp),
+                ("gapS", vp), ("gap", vp)]
 
 
 class FFConn(C.Structure):
@@ -71,7 +72,8 @@ class FFEvents(C.Structure):
 
 class RecConn(C.Structure):
     _fields_ = [("maxNear", u32), ("nNear", vp), ("nbr", vp), ("rowMask", vp),
-                ("rowOff", vp), ("w", vp), ("wDense", vp)]
+                ("rowOff", vp), ("w", vp), ("wDense", vp),
+                ("maxGap", u32), ("nGap", vp), ("gapPre", vp), ("gapG", vp)]
 
 
 class FarConn(C.Structure):
@@ -134,6 +136,8 @@ SIGNATURES = {
                                      C.c_uint64, vp],
     "vc_v1_run_persistent_noalloc_max_v1": [],
     "vc_v1_dep_current": [vp, C.POINTER(V1Params), C.POINTER(V1State), vp],
+    "vc_v1_gap": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn), vp, u32,
+                  u32, vp],
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],

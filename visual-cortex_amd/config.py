@@ -27,6 +27,8 @@ FILES = {
     "fV1_delayMat": ("V1_delayMat", "inputFolder", "conV1_suffix"),   # src/patch.cu:352, 467-470
     "fNeighborBlock": ("neighborBlock", "inputFolder", "conV1_suffix"),
     "fV1_vec": ("V1_vec", "inputFolder", "conV1_suffix"),             # src/patch.cu:355, 482-486
+    "fV1_gapMat": ("V1_gapMat", "inputFolder", "conV1_suffix"),       # src/patch.cu:353, 472-476
+    "fV1_gapVec": ("V1_gapVec", "inputFolder", "conV1_suffix"),       # src/patch.cu:356, 487-491
     "fLGN_V1_ID": ("LGN_V1_idList", "inputFolder", "conLGN_suffix"),
     "fLGN_V1_s": ("LGN_V1_sList", "inputFolder", "conLGN_suffix"),
     "fLGN": ("LGN", "outputFolder", "output_suffix"),
@@ -204,8 +206,9 @@ def spec_from_config(cfg: Config, base=None):
     kw["noFarDelay"] = cfg.int("noFarDelay", 1)
     if kw["noFarDelay"] and not kw.get("noDelay", sp.noDelay):
         raise ValueError("noFarDelay = 1 with noDelay = 0: delay or no delay is the question")
-    if cfg.bool("InhGap", False):
-        raise NotImplementedError("InhGap (gap junctions) is not built")
+    # gap junctions between inhibitory neurons (src/patch.cu:238); their
+    # strengths come from V1_gapMat / V1_gapVec, scaled by gapRatio (loader)
+    kw["InhGap"] = 1 if cfg.bool("InhGap", False) else 0
     if cfg.int("learning", 0) != 0:
         raise NotImplementedError("learning != 0 (plasticity) is not built")
     if cfg.has("flat_retina") and not cfg.bool("flat_retina"):

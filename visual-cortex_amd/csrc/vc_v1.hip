@@ -668,10 +668,12 @@ __device__ __forceinline__ FFAcc ff_stage(const vc_v1_params &p, const V1Const &
 
 // Stage d of one neuron: x holds the state at the start of the step with the
 // cortical g/h already incremented by the previous step's spikes, acc the FF
-// receptors of ff_stage; on return x holds the state after the step.
-// Returns sInfo.
+// receptors of ff_stage; on return x holds the state after the step.  gapS
+// and gap are the neuron's gap-junction terms (0 for excitatory neurons and
+// without gap junctions).  Returns sInfo.
 __device__ __forceinline__ float membrane_stage(const vc_v1_params &p, const V1Const &k, int t,
-                                                float depC, NState &x, const FFAcc &acc) {
+                                                float depC, NState &x, const FFAcc &acc,
+                                                float gapS, float gap) {
     const float dt = p.dt;
     const float vR = p.vR[t], vThres = p.vThres[t], gL = p.gL[t], C = p.C[t];
     const float tRef = p.tRef[t], vT = p.vT[t], deltaT = p.deltaT[t];
@@ -740,10 +742,10 @@ __device__ __forceinline__ float membrane_stage(const vc_v1_params &p, const V1C
     float v = v0, w = w0, tsp = 0.0f;
     bool spiked = false;
     if (tBack < dt) {
-        const float a0 = gE_t0 + gI_t0 + gL + 0.0f;
-        const float b0 = gE_t0 * VC_VE + gI_t0 * VC_VI + gL * VC_VL + depC + 0.0f;
-        const float a1 = gE_t1 + gI_t1 + gL + 0.0f;
-        const float b1 = gE_t1 * VC_VE + gI_t1 * VC_VI + gL * VC_VL + depC + 0.0f;
+        const float a0 = gE_t0 + gI_t0 + gL + gapS;
+        const float b0 = gE_t0 * VC_VE + gI_t0 * VC_VI + gL * VC_VL + depC + gap;
+        const float a1 = gE_t1 + gI_t1 + gL + gapS;
+        const float b1 = gE_t1 * VC_VE + gI_t1 * VC_VI + gL * VC_VL + depC + gap;
         const float new_dt = dt - tBack;
         if (backing) {
             const float A = a * (v0 - VC_VL);
@@ -952,11 +954,43 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     const float depC = dep_current(p, t, s.tonicDep[n]);
     if (out_dep) out_dep[n] = depC;
     const FFAcc acc = ff_stage<kEvents>(p, k, ff, t, n, b, step, x, f, lgnSInfo, ffMask);
-    const float sInfo = membrane_stage(p, k, t, depC, x, acc);
+    float gapS = 0.0f, gap = 0.0f;           // inhibitory neurons' gap-junction terms
+    if (local >= p.nE) {
+        const uint32_t gi = b * (p.blockSize - p.nE) + (local - p.nE);
+        if (s.gapS) gapS = s.gapS[gi];
+        if (s.gap) gap = s.gap[gi];
+    }
+    const float sInfo = membrane_stage(p, k, t, depC, x, acc, gapS, gap);
     store_state(p, s, n, x);
     out_sInfo[n] = sInfo;
     const RecOut noOut{nullptr, 0u, 0u, 0u};
     compact<false>(p, next, noOut, n, local, lane, sInfo);
+}
+
+// ------------------------------------------------------ gap junctions --
+// The inhibitory neurons' gap sums of recal_G_mat_nd / recal_G_mat
+// (src/coredynamics.cu:2524-2540, 2237-2260): one lane per inhibitory
+// neuron, its junctions in the ELL's order (neighbour slot, then partner
+// index: the reference's order), a partner's step output replaced by its
+// type's vThres when it spiked.  lag[j] < depth (checked by the caller).
+__global__ __launch_bounds__(256) void k_v1_gap(vc_v1_params p, vc_rec_conn rec,
+                                                const float *__restrict__ train, uint32_t depth,
+                                                uint32_t cur, const uint32_t *__restrict__ lag,
+                                                uint32_t mI, float *__restrict__ gap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mI) return;
+    const uint32_t cnt = rec.nGap[i];
+    float sum = 0.0f;
+    for (uint32_t j = 0; j < cnt; ++j) {
+        const uint64_t e = (uint64_t)j * mI + i;
+        const uint32_t pre = rec.gapPre[e];
+        uint32_t slot = cur;
+        if (lag) slot = (cur + depth - lag[e] % depth) % depth;
+        float v = train[(uint64_t)slot * p.nV1 + pre];
+        if (v > 0.0f) v = p.vThres[type_of(p, pre % p.blockSize)];
+        sum = sum + rec.gapG[e] * v;
+    }
+    gap[i] = sum;
 }
 
 // ----------------------------------------------- persistent multi-step --
@@ -1066,7 +1100,7 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
             x.gI[ig] = x.gI[ig] + lgI[ig];
             x.hI[ig] = x.hI[ig] + lhI[ig];
         }
-        const float sInfo = membrane_stage(p, k, t, depC, x, acc);
+        const float sInfo = membrane_stage(p, k, t, depC, x, acc, 0.0f, 0.0f);
         out_sInfo[q * outStride + n] = sInfo;
         const RecOut ro{words, rec_of((q + 1u) & 1u), ovf_of((q + 1u) & 1u),
                         (epoch << 16) + q + 2u};
@@ -1176,7 +1210,42 @@ int check_params(const vc_v1_params *p, const vc_rec_conn *rec) {
 
 
 
+// Gap junctions need s->gap to write to; the ELL arrays come together.
+int gap_ok(const vc_v1_params *p, const vc_v1_state *s, const vc_rec_conn *rec,
+           const char *who) {
+    if (rec->gapPre && (!rec->nGap || !rec->gapG || !s->gap || rec->maxGap == 0)) {
+        vc_set_error("%s: gap junctions need nGap, gapPre, gapG, maxGap > 0 and s->gap", who);
+        return VC_E_INVALID;
+    }
+    if (rec->gapPre && p->nE >= p->blockSize) {
+        vc_set_error("%s: gap junctions without inhibitory neurons", who);
+        return VC_E_INVALID;
+    }
+    return VC_OK;
+}
+
+bool has_gap(const vc_v1_state *s, const vc_rec_conn *rec) {
+    return s->gapS || s->gap || rec->gapPre;
+}
+
 }  // namespace
+
+extern "C" int vc_v1_gap(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                         const vc_rec_conn *rec, const float *train, uint32_t depth,
+                         uint32_t cur, const uint32_t *lag) {
+    int rc = check_params(p, rec);
+    if (rc) return rc;
+    if (!s || !rec || !train || !rec->gapPre || depth == 0 || cur >= depth) {
+        vc_set_error("vc_v1_gap: bad argument (gap junctions, train, depth > cur needed)");
+        return VC_E_INVALID;
+    }
+    if ((rc = gap_ok(p, s, rec, "vc_v1_gap"))) return rc;
+    const uint32_t mI = p->nblock * (p->blockSize - p->nE);
+    hipLaunchKernelGGL(k_v1_gap, dim3((mI + 255) / 256), dim3(256), 0, (hipStream_t)stream, *p,
+                       *rec, train, depth, cur, lag, mI, s->gap);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
 
 extern "C" int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                           const vc_ff_conn *ff, const vc_rec_conn *rec,
@@ -1221,6 +1290,10 @@ extern "C" int vc_v1_step_snapshot(void *stream, const vc_v1_params *p, const vc
         vc_set_error("vc_v1_step_snapshot: null argument");
         return VC_E_INVALID;
     }
+    if (has_gap(s, rec)) {
+        vc_set_error("vc_v1_step_snapshot: the adjoint's forward has no gap junctions");
+        return VC_E_UNSUPPORTED;
+    }
     if (prev->mask == next->mask) {
         vc_set_error("vc_v1_step_snapshot: prev and next spike buffers must differ");
         return VC_E_INVALID;
@@ -1261,9 +1334,11 @@ extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state 
         vc_set_error("vc_v1_run: event-driven LGN input needs maxLGN <= 32 (got %u)", ff->maxLGN);
         return VC_E_INVALID;
     }
+    if ((rc = gap_ok(p, s, rec, "vc_v1_run"))) return rc;
     V1Const k = make_const(p);
     uint32_t g, wg;
     if ((rc = step_shape(p, rec, &g, &wg))) return rc;
+    const uint32_t mI = p->nblock * (p->blockSize - p->nE);
     for (uint32_t i = 0; i < nsteps; ++i) {
         const uint32_t nx = 1u - cur;
         if (ffMask)
@@ -1274,6 +1349,10 @@ extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state 
             launch_step<true, false>(g, wg, (hipStream_t)stream, *p, k, *s, *ff, *rec, spk2[cur],
                                      spk2[nx], lgnSInfo + i * lgnStride, nullptr, it0 + i,
                                      out_sInfo + i * outStride, nullptr, nullptr, nullptr);
+        if (rec->gapPre && mI)          // recal_G's gap sums of this step, for the next
+            hipLaunchKernelGGL(k_v1_gap, dim3((mI + 255) / 256), dim3(256), 0,
+                               (hipStream_t)stream, *p, *rec, out_sInfo + i * outStride, 1u, 0u,
+                               nullptr, mI, s->gap);
         cur = nx;
     }
     VC_HIP_CHECK(hipGetLastError());
@@ -1291,13 +1370,15 @@ namespace {
 // The persistent launch behind vc_v1_run_persistent and its allocation-free
 // variant: `words` is the record buffer, `clearWords32` the number of 32-bit
 // words cleared (by a fill kernel, so that the call may be captured) when
-// epoch is 1.
+// epoch is 1.  *launched (may be null) tells whether kernels were enqueued:
+// a refused call, or one of 0 steps, uses up no epoch.
 int run_persistent(const char *who, void *stream, const vc_v1_params *p, const vc_v1_state *s,
                    const vc_ff_conn *ff, const vc_rec_conn *rec, const vc_spikes *spk2,
                    uint32_t cur, const float *lgnSInfo, uint64_t lgnStride, uint32_t *ffMask,
                    uint64_t maskStride, uint32_t it0, uint32_t nsteps, float *out_sInfo,
                    uint64_t outStride, uint64_t *words, uint32_t epoch, uint64_t clearWords32,
-                   uint32_t *status) {
+                   uint32_t *status, bool *launched = nullptr) {
+    if (launched) *launched = false;
     int rc = check_params(p, rec);
     if (rc) return rc;
     if (!s || !ff || !rec || !spk2 || !lgnSInfo || !out_sInfo || !words || !status || cur > 1) {
@@ -1314,6 +1395,11 @@ int run_persistent(const char *who, void *stream, const vc_v1_params *p, const v
     }
     if (rec->maxNear > 64u) {
         vc_set_error("%s: more than 64 neighbour blocks", who);
+        return VC_E_UNSUPPORTED;
+    }
+    if (has_gap(s, rec)) {
+        vc_set_error("%s: gap junctions need the partners' voltages of every step (vc_v1_run)",
+                     who);
         return VC_E_UNSUPPORTED;
     }
     if (epoch == 0 || epoch > 0xffffu || nsteps > 0xfffdu) {
@@ -1350,6 +1436,7 @@ int run_persistent(const char *who, void *stream, const vc_v1_params *p, const v
         return VC_E_UNSUPPORTED;
     }
     hipStream_t st = (hipStream_t)stream;
+    if (launched) *launched = true;
     if (epoch == 1) {               // a new cycle of epochs: no tag of an earlier one may match
         hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((clearWords32 + 255) / 256)), dim3(256),
                            0, st, (uint32_t *)words, (uint32_t)clearWords32, 0u);
@@ -1430,25 +1517,25 @@ extern "C" int vc_v1_run_persistent_noalloc(void *stream, const vc_v1_params *p,
         vc_set_error("%s: device ordinal %d beyond %d", who, dev, kMaxDevices);
         return VC_E_UNSUPPORTED;
     }
-    uint64_t *words;
-    uint32_t epoch;
-    {
-        std::lock_guard<std::mutex> lock(g_noalloc_mu);
-        NoAllocDevice &d = g_noalloc[dev];
-        if (!d.words) {
-            void *addr = nullptr;
-            VC_HIP_CHECK(hipGetSymbolAddress(&addr, HIP_SYMBOL(noalloc_scratch)));
-            d.words = (uint64_t *)addr;
-        }
-        d.epoch = d.epoch % 0xffffu + 1u;
-        words = d.words;
-        epoch = d.epoch;
+    // The lock covers the whole call (it only enqueues work): the epoch is
+    // used up only by a call that launches, so that epoch 1, which clears the
+    // whole array, is never skipped.  The clear matters because a later call
+    // with more neurons must not meet tags that an earlier cycle of epochs
+    // left beyond this call's records.
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocDevice &d = g_noalloc[dev];
+    if (!d.words) {
+        void *addr = nullptr;
+        VC_HIP_CHECK(hipGetSymbolAddress(&addr, HIP_SYMBOL(noalloc_scratch)));
+        d.words = (uint64_t *)addr;
     }
-    // epoch 1 clears the whole array: a later call with more neurons must not
-    // meet tags that an earlier cycle of epochs left beyond this call's records
-    return run_persistent(who, stream, p, s, ff, rec, spk2, cur, lgnSInfo, lgnStride, ffMask,
-                          maskStride, it0, nsteps, out_sInfo, outStride, words, epoch,
-                          kNoAllocWords * 2u, status);
+    const uint32_t epoch = d.epoch % 0xffffu + 1u;
+    bool launched = false;
+    const int rc = run_persistent(who, stream, p, s, ff, rec, spk2, cur, lgnSInfo, lgnStride,
+                                  ffMask, maskStride, it0, nsteps, out_sInfo, outStride, d.words,
+                                  epoch, kNoAllocWords * 2u, status, &launched);
+    if (launched) d.epoch = epoch;
+    return rc;
 }
 
 extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,

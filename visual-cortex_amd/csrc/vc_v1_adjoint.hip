@@ -577,6 +577,10 @@ extern "C" int vc_v1_adjoint_run(void *stream, const vc_v1_params *p, const vc_v
         vc_set_error("vc_v1_adjoint_run: only the LIF model (iModel 0) is differentiated");
         return VC_E_UNSUPPORTED;
     }
+    if (s->gapS || s->gap || rec->gapPre) {
+        vc_set_error("vc_v1_adjoint_run: gap junctions are not differentiated");
+        return VC_E_UNSUPPORTED;
+    }
     for (uint32_t t = 0; t < p->nType; ++t) {
         if (p->a[t] != 0.0f || p->deltaT[t] != 0.0f) {
             vc_set_error("vc_v1_adjoint_run: type %u has a = %g, deltaT = %g (conductance LIF "

@@ -203,11 +203,15 @@ def write_delaymat(path, net):
             f.write(M.tobytes())
 
 
-def read_delaymat_pairs(path, nblock, blockSize, pairs):
+def read_delaymat_pairs(path, nblock, blockSize, pairs, extra=None):
     """Stream V1_delayMat block by block and keep the distances of the
-    connected pairs, aligned with read_conmat_pairs' (pre, post, w) lists."""
+    connected pairs, aligned with read_conmat_pairs' (pre, post, w) lists.
+    extra: more (pre, post, ...) lists per (b, ib) in block-local indices
+    (e.g. the gap junctions, offset by nE); their distances are returned
+    as a second list."""
     bs = blockSize
     out = []
+    out2 = []
     with open(path, "rb") as f:
         maxNear = _u32(f, 1)
         if len(pairs) != nblock * maxNear:
@@ -221,7 +225,94 @@ def read_delaymat_pairs(path, nblock, blockSize, pairs):
             for ib in range(maxNear):
                 pre, post, _ = pairs[b * maxNear + ib]
                 out.append(M[ib, pre, post].astype(np.float32))
+                if extra is not None:
+                    pre, post = extra[b * maxNear + ib][:2]
+                    out2.append(M[ib, pre, post].astype(np.float32))
+    return (out, out2) if extra is not None else out
+
+
+# -------------------------------------------------------------- gap junctions
+def write_gapmat(path, net):
+    """Dense V1_gapMat [nblock][nearNeighborBlock][nI pre][nI post] of a
+    Network (header u32 nearNeighborBlock; src/patch.cu:3462-3473,
+    3688-3690): the strengths as they are (the reader scales by gapRatio)."""
+    sp = net.spec
+    nI = sp.nI
+    with open(path, "wb") as f:
+        f.write(struct.pack("<I", net.maxNear))
+        for b in range(sp.nblock):
+            M = np.zeros((net.maxNear, nI, nI), "<f4")
+            if net.gap is not None:
+                for ib in range(net.maxNear):
+                    pre, post, g = net.gap[b * net.maxNear + ib]
+                    M[ib, pre, post] = g
+            f.write(M.tobytes())
+
+
+def read_gapmat_pairs(path, nblock, nI, maxNear, nNear):
+    """Stream V1_gapMat block by block: per (b, ib) the (pre, post, g) of its
+    nonzero entries (inhibitory indices, sorted by (pre, post)); the slots
+    past a block's nNear are dropped, as the reference never reads them.
+    Strengths must be >= 0 (src/patch.cu:3711)."""
+    out = []
+    z = np.zeros(0, np.uint32)
+    with open(path, "rb") as f:
+        mn = _u32(f, 1)
+        if mn != maxNear:
+            raise ValueError(f"{path}: {mn} neighbour slots, V1_conMat has {maxNear}")
+        for b in range(nblock):
+            M = np.fromfile(f, "<f4", maxNear * nI * nI)
+            if M.size != maxNear * nI * nI:
+                raise ValueError(f"{path}: truncated at block {b}")
+            M = M.reshape(maxNear, nI, nI)
+            if (M < 0).any():
+                raise ValueError(f"{path}: negative gap-junction strength in block {b}")
+            for ib in range(maxNear):
+                if ib >= int(nNear[b]):
+                    out.append((z, z, np.zeros(0, np.float32)))
+                    continue
+                pre, post = np.nonzero(M[ib])          # row-major: sorted by (pre, post)
+                out.append((pre.astype(np.uint32), post.astype(np.uint32),
+                            M[ib, pre, post].astype(np.float32)))
     return out
+
+
+def write_gapvec(path, mI, gap_far=None):
+    """V1_gapVec: u32 nGapVec[mI], then for every inhibitory neuron with
+    junctions u32 id[n], f32 strength[n], f32 distance[n] (src/patch.cu
+    :3362-3405).  gap_far: (off [mI+1], pre, g, dist) or None (no junctions)."""
+    with open(path, "wb") as f:
+        if gap_far is None:
+            f.write(np.zeros(mI, "<u4").tobytes())
+            return
+        off, pre, g, dist = gap_far
+        cnt = np.diff(off.astype(np.int64))
+        f.write(cnt.astype("<u4").tobytes())
+        for i in np.nonzero(cnt)[0]:
+            a, b = int(off[i]), int(off[i + 1])
+            f.write(np.asarray(pre[a:b], "<u4").tobytes())
+            f.write(np.asarray(g[a:b], "<f4").tobytes())
+            f.write(np.asarray(dist[a:b], "<f4").tobytes())
+
+
+def read_gapvec(path, mI):
+    """(off u32 [mI+1], pre u32, g f32, dist f32) of a V1_gapVec file, the
+    strengths as they are."""
+    with open(path, "rb") as f:
+        cnt = np.fromfile(f, "<u4", mI).astype(np.int64)
+        if cnt.size != mI:
+            raise ValueError(f"{path}: truncated count table")
+        off = np.zeros(mI + 1, np.int64)
+        np.cumsum(cnt, out=off[1:])
+        pre = np.zeros(int(off[-1]), np.uint32)
+        g = np.zeros(int(off[-1]), np.float32)
+        dist = np.zeros(int(off[-1]), np.float32)
+        for i in np.nonzero(cnt)[0]:
+            a, b, n = int(off[i]), int(off[i + 1]), int(cnt[i])
+            pre[a:b] = np.fromfile(f, "<u4", n)
+            g[a:b] = np.fromfile(f, "<f4", n)
+            dist[a:b] = np.fromfile(f, "<f4", n)
+    return off.astype(np.uint32), pre, g, dist
 
 
 # ------------------------------------------------------------------ LGN -> V1

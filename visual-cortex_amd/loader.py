@@ -4,7 +4,8 @@ inputs drive this simulator and synthetic workloads can be exported.
 
 Reads (see formats.py for the layouts and reference lines): V1_allpos,
 connectome_cfg, neighborBlock, V1_conMat, V1_delayMat (noDelay = 0), V1_vec
-(far connections; optional here, the reference requires it),
+(far connections; optional here, the reference requires it), V1_gapMat and
+V1_gapVec (gap junctions, times gapRatio; optional here too),
 LGN_V1_idList / LGN_V1_sList, LGN_vpos, LGN_surfaceID, fLGN, conStats,
 V1_feature (when the cfg asks for features) and stimulus.bin.  The neuron and
 synapse parameters come from the cfg keys (config.spec_from_config); sRatioV1
@@ -74,9 +75,16 @@ def load_network(cfg: cfgmod.Config) -> Network:
     if feature_needed(cfg):
         feature = F.read_v1_feature(cfg.path("fV1_feature"), nblock * bs)
     tonic = tonic_drive(cfg, spec.tonicDep, typeAcc, nblock, bs, ffRatio, feature)
+    gap, gap_far, gap_dist = read_gap_junctions(cfg, nblock, bs, int(typeAcc[0]), maxNear, nNear)
     rec_dist = None
     if spec.noDelay == 0:                    # the reference's own distances
-        rec_dist = F.read_delaymat_pairs(cfg.path("fV1_delayMat"), nblock, bs, pairs)
+        if gap is not None:
+            nE = int(typeAcc[0])
+            shifted = [(p[0] + nE, p[1] + nE) for p in gap]
+            rec_dist, gap_dist = F.read_delaymat_pairs(cfg.path("fV1_delayMat"), nblock, bs,
+                                                       pairs, extra=shifted)
+        else:
+            rec_dist = F.read_delaymat_pairs(cfg.path("fV1_delayMat"), nblock, bs, pairs)
     far = None
     vec = Path(cfg.path("fV1_vec"))
     if vec.exists():
@@ -85,7 +93,39 @@ def load_network(cfg: cfgmod.Config) -> Network:
             far = None
     return Network(spec=spec, pos=pos, tonicDep=tonic, maxNear=maxNear, nNear=nNear, nbr=nbr,
                    rec_pairs=pairs, nLGN_of=nof, sLGN=sLGN, idLGN=idLGN, lgn=lgn,
-                   frames=frames, init_lms=tuple(st["init_lms"]), rec_dist=rec_dist, far=far)
+                   frames=frames, init_lms=tuple(st["init_lms"]), rec_dist=rec_dist, far=far,
+                   gap=gap, gap_dist=gap_dist, gap_far=gap_far)
+
+
+def read_gap_junctions(cfg, nblock: int, bs: int, nE: int, maxNear: int, nNear):
+    """(gap, gap_far, None) from V1_gapMat and V1_gapVec, strengths times
+    gapRatio (one inhibitory type: a single value, src/patch.cu:736-744,
+    3395, 3718).  Files that are absent hold no junctions here (the
+    reference requires both); gapRatio must be given when a file holds a
+    junction."""
+    nI = bs - nE
+    mI = nblock * nI
+    gp, gv = Path(cfg.path("fV1_gapMat")), Path(cfg.path("fV1_gapVec"))
+    gap = F.read_gapmat_pairs(gp, nblock, nI, maxNear, nNear) if gp.exists() else None
+    far = F.read_gapvec(gv, mI) if gv.exists() else None
+    if gap is not None and not any(p[0].size for p in gap):
+        gap = None
+    if far is not None and int(far[0][-1]) == 0:
+        far = None
+    if gap is None and far is None:
+        return None, None, None
+    ratio = cfg.floats("gapRatio")
+    if not ratio or len(ratio) != 1:
+        raise ValueError("gapRatio must hold one value (nTypeI = 1) when V1_gapMat or V1_gapVec "
+                         "holds a junction (src/patch.cu:736-744)")
+    r = np.float32(ratio[0])
+    if r < 0:
+        raise ValueError("gapRatio must be >= 0")
+    if gap is not None:
+        gap = [(a, b, (g * r).astype(np.float32)) for a, b, g in gap]
+    if far is not None:
+        far = (far[0], far[1], (far[2] * r).astype(np.float32), far[3])
+    return gap, far, None
 
 
 def _per_type(vals, nType, key, per=1):
@@ -257,6 +297,8 @@ def save_network(net: Network, folder, suffix: str = "vc", tonic: dict | None = 
     F.write_conmat(folder / f"V1_conMat{suf}", net)
     F.write_delaymat(folder / f"V1_delayMat{suf}", net)
     F.write_v1_vec(folder / f"V1_vec{suf}", sp.nV1, getattr(net, "far", None))
+    F.write_gapmat(folder / f"V1_gapMat{suf}", net)
+    F.write_gapvec(folder / f"V1_gapVec{suf}", sp.nblock * sp.nI, net.gap_far)
     # LGN_V1 lists hold the un-scaled strengths (read_LGN multiplies by sRatioLGN)
     isE = (np.arange(sp.nV1) % sp.blockSize) < sp.nE
     sR = np.where(isE, sp.sRatioLGN[0], sp.sRatioLGN[1]).astype(np.float32)
@@ -292,6 +334,7 @@ def save_network(net: Network, folder, suffix: str = "vc", tonic: dict | None = 
         f"res_suffix = {suffix}", f"conV1_suffix = {suffix}", f"conLGN_suffix = {suffix}",
         f"output_suffix = {suffix}", "flat_retina = true", f"noDelay = {sp.noDelay}",
         f"speedOfThought = {sp.speedOfThought}", "learning = 0",
+        f"InhGap = {'true' if sp.InhGap else 'false'}", "gapRatio = 1.0",
         f"noFarDelay = {sp.noDelay if sp.noFarDelay < 0 else sp.noFarDelay}",
         f"seed = {sp.seed}", f"dt = {sp.dt}", f"nt = {sp.nt}", f"frameRate = {sp.frameRate}",
         f"iModel = {sp.iModel}", f"tau = {sp.tau}", f"nKernelSample = {sp.nKernelSample}",

@@ -91,6 +91,13 @@ class ModelSpec:
     # neighbours (strength sFar times lognormal)
     nFar: int = 0
     sFar: float = 0.05
+    # gap junctions between inhibitory neurons (the reference's InhGap): on
+    # when InhGap is 1; synthetic networks get nGap partners per inhibitory
+    # neuron among the inhibitory neurons of its block's near neighbours
+    # (strength sGap times lognormal)
+    InhGap: int = 0
+    nGap: int = 0
+    sGap: float = 0.2
     # background Poisson drive (extension)
     bgRate: float = 2000.0        # Hz
     bgStrength: tuple = (0.08, 0.12)
@@ -209,6 +216,16 @@ class Network:
     # far connections as CSR over post neurons, in the V1_vec file's order:
     # (off u32 [nV1+1], pre u32, w f32 already * sRatioV1, dist f32 mm)
     far: tuple | None = None
+    # gap junctions between inhibitory neurons of near-neighbour blocks, per
+    # (b, ib) like rec_pairs: (pre u32, post u32 -- indices among the nI
+    # inhibitory neurons of the blocks --, g f32 already * gapRatio, > 0),
+    # sorted by (pre, post); gap_dist: their distances (mm) when read from a
+    # V1_delayMat file, else computed from pos
+    gap: list | None = None
+    gap_dist: list | None = None
+    # far gap junctions (V1_gapVec) as CSR over the mI inhibitory neurons:
+    # (off u32 [mI+1], pre u32 global id, g f32 already * gapRatio, dist f32)
+    gap_far: tuple | None = None
 
     def n_far(self) -> int:
         return 0 if self.far is None else int(self.far[0][-1])
@@ -255,6 +272,135 @@ class Network:
         b = np.zeros(nV1, bool)
         b[self.far[1][self.far[1] < nV1].astype(np.int64)] = True
         return np.packbits(b.reshape(-1, 64), axis=-1, bitorder="little").reshape(-1).view(np.int64)
+
+    # ---- gap junctions --------------------------------------------------------
+    def has_gap(self) -> bool:
+        """Any gap-junction strength (near or far) present."""
+        near = self.gap is not None and any(p[0].size for p in self.gap)
+        far = self.gap_far is not None and int(self.gap_far[0][-1]) > 0
+        return near or far
+
+    def gap_on(self) -> bool:
+        """Gap currents are computed (InhGap and some near junction)."""
+        return bool(self.spec.InhGap) and self.gap is not None and \
+            any(p[0].size for p in self.gap)
+
+    def _gap_post_major(self):
+        """Every near junction ordered by post inhibitory neuron (global index
+        b*nI + post), then neighbour slot, then partner: (post_gi, slot ib,
+        pre local index among the inhibitory neurons, g, dist or None)."""
+        sp = self.spec
+        nI = sp.nI
+        P, S, L, G, D = [], [], [], [], []
+        for b in range(sp.nblock):
+            for ib in range(int(self.nNear[b])):
+                pre, post, g = self.gap[b * self.maxNear + ib]
+                if pre.size == 0:
+                    continue
+                P.append(b * nI + post.astype(np.int64))
+                S.append(np.full(pre.size, ib, np.int64))
+                L.append(pre.astype(np.int64))
+                G.append(np.asarray(g, np.float32))
+                D.append(self._gap_pair_dist(b, ib, pre, post))
+        if not P:
+            z = np.zeros(0, np.int64)
+            return z, z, z, np.zeros(0, np.float32), np.zeros(0, np.float32)
+        post, ib, pre, g, d = (np.concatenate(x) for x in (P, S, L, G, D))
+        order = np.lexsort((pre, ib, post))
+        return post[order], ib[order], pre[order], g[order], d[order]
+
+    def _gap_pair_dist(self, b, ib, pre, post):
+        if self.gap_dist is not None:
+            return np.asarray(self.gap_dist[b * self.maxNear + ib], np.float32)
+        nE = self.spec.nE
+        bs = self.spec.blockSize
+        pos = np.asarray(self.pos, dtype=np.float64)
+        gpre = int(self.nbr[b, ib]) * bs + nE + pre.astype(np.int64)
+        gpost = b * bs + nE + post.astype(np.int64)
+        x = (pos[gpre, 0] - pos[gpost, 0]).astype(np.float32)
+        y = (pos[gpre, 1] - pos[gpost, 1]).astype(np.float32)
+        return np.sqrt(x * x + y * y)
+
+    def gap_ell(self):
+        """(maxGap, nGap u32 [mI], pre u32 [maxGap, mI] global partner id,
+        g f32 [maxGap, mI]): the near junctions as vc_rec_conn's ELL, in the
+        reference's summation order (neighbour slot, then partner)."""
+        sp = self.spec
+        mI, nI, bs = sp.nblock * sp.nI, sp.nI, sp.blockSize
+        post, ib, pre, g, _ = self._gap_post_major()
+        cnt = np.bincount(post, minlength=mI)
+        mg = max(int(cnt.max()) if cnt.size else 0, 1)
+        start = np.zeros(mI, np.int64)
+        np.cumsum(cnt[:-1], out=start[1:])
+        j = np.arange(post.size) - start[post]
+        b = post // nI
+        P = np.zeros((mg, mI), np.uint32)
+        G = np.zeros((mg, mI), np.float32)
+        P[j, post] = self.nbr[b, ib].astype(np.int64) * bs + sp.nE + pre
+        G[j, post] = g
+        return mg, cnt.astype(np.uint32), P, G
+
+    def gap_lags(self, speed: float, dt: float) -> np.ndarray:
+        """u32 [maxGap, mI], aligned with gap_ell: ceil((distance / speed) /
+        dt) in float32, the steps a junction's partner voltage travels with
+        axonal delays (recal_G_mat, src/coredynamics.cu:2249-2256)."""
+        sp = self.spec
+        mI = sp.nblock * sp.nI
+        post, ib, pre, g, d = self._gap_post_major()
+        mg, cnt, _, _ = self.gap_ell()
+        start = np.zeros(mI, np.int64)
+        np.cumsum(cnt[:-1].astype(np.int64), out=start[1:])
+        j = np.arange(post.size) - start[post]
+        f = np.float32
+        t = d.astype(f) / f(speed)
+        lag = np.zeros((mg, mI), np.uint32)
+        lag[j, post] = np.ceil(t / f(dt)).astype(np.uint32)
+        return lag
+
+    def gap_strength_sums(self) -> np.ndarray:
+        """gapS f32 [mI]: per inhibitory neuron, 0 + its far strengths (file
+        order) + its near strengths (neighbour slot, then partner), added one
+        at a time in float32 (src/patch.cu:3367-3397, 3695-3721)."""
+        sp = self.spec
+        mI = sp.nblock * sp.nI
+        acc = np.zeros(mI, np.float32)
+        if self.gap_far is not None:
+            off, _, g, _ = self.gap_far
+            cnt = np.diff(off.astype(np.int64))
+            for j in range(int(cnt.max()) if cnt.size else 0):
+                has = cnt > j
+                idx = off[:-1].astype(np.int64) + j
+                acc[has] = acc[has] + g[idx[has]]
+        if self.gap is not None:
+            mg, cnt, _, G = self.gap_ell()
+            for j in range(mg):
+                has = cnt > j
+                acc[has] = acc[has] + G[j, has]
+        return acc
+
+    def dense_gapmat(self) -> np.ndarray:
+        """The reference's V1_gapMat layout [nblock, maxNear, nI pre, nI post]."""
+        sp = self.spec
+        M = np.zeros((sp.nblock, self.maxNear, sp.nI, sp.nI), np.float32)
+        if self.gap is not None:
+            for b in range(sp.nblock):
+                for ib in range(self.maxNear):
+                    pre, post, g = self.gap[b * self.maxNear + ib]
+                    M[b, ib, pre, post] = g
+        return M
+
+    def dense_gapdist(self) -> np.ndarray:
+        """The distances of the junctions in the V1_gapMat layout (the
+        inhibitory-to-inhibitory part of V1_delayMat), zero elsewhere."""
+        sp = self.spec
+        M = np.zeros((sp.nblock, self.maxNear, sp.nI, sp.nI), np.float32)
+        if self.gap is not None:
+            for b in range(sp.nblock):
+                for ib in range(int(self.nNear[b])):
+                    pre, post, g = self.gap[b * self.maxNear + ib]
+                    if pre.size:
+                        M[b, ib, pre, post] = self._gap_pair_dist(b, ib, pre, post)
+        return M
 
     # ---- recurrent formats -------------------------------------------------
     def masked_csr_torch(self, device="cuda"):
@@ -581,6 +727,8 @@ def build_network(spec: ModelSpec, with_frames: bool = True) -> Network:
     )
     if spec.nFar > 0:
         net.far = far_connections(spec, pos, nbr, nNear)
+    if spec.nGap > 0:
+        net.gap = gap_junctions(spec, pos, nbr, nNear, maxNear)
     if with_frames:
         net.frames = grating_frames(spec)
     return net
@@ -627,6 +775,52 @@ def far_connections(spec: ModelSpec, pos: np.ndarray, nbr, nNear):
     y = (pos[pre, 1] - pos[post, 1]).astype(np.float32)
     dist = np.sqrt(x * x + y * y).astype(np.float32)
     return off, pre.astype(np.uint32), w, dist
+
+
+def gap_junctions(spec: ModelSpec, pos: np.ndarray, nbr, nNear, maxNear: int) -> list:
+    """Synthetic gap junctions: spec.nGap partners per inhibitory neuron
+    among the inhibitory neurons of its block and its near neighbours (not
+    itself), drawn by a Gaussian of the distance (sigma_um), strength sGap
+    times lognormal; per (b, ib) like Network.gap.  From a generator of its
+    own, so that the rest of the network does not depend on nGap."""
+    rng = np.random.default_rng(spec.seed + 104729)
+    bs, nb, nE, nI = spec.blockSize, spec.nblock, spec.nE, spec.nI
+    sig_mm = spec.sigma_um / 1000.0
+    out = [None] * (nb * maxNear)
+    z = np.zeros(0, np.uint32)
+    for b in range(nb):
+        posts = b * bs + nE + np.arange(nI)
+        slots = np.arange(int(nNear[b]))
+        cand = np.concatenate([int(nbr[b, ib]) * bs + nE + np.arange(nI) for ib in slots]) \
+            if slots.size else np.zeros(0, np.int64)
+        cand_slot = np.repeat(slots, nI)
+        k = min(spec.nGap, max(cand.size - 1, 0))
+        if k == 0:
+            for ib in range(maxNear):
+                out[b * maxNear + ib] = (z, z, np.zeros(0, np.float32))
+            continue
+        d2 = ((pos[posts, None, :] - pos[None, cand, :]) ** 2).sum(-1)
+        wgt = np.exp(-d2 / (2 * sig_mm * sig_mm))
+        wgt[posts[:, None] == cand[None, :]] = 0.0
+        k = min(k, int((wgt > 0).sum(1).min()))
+        if k == 0:
+            for ib in range(maxNear):
+                out[b * maxNear + ib] = (z, z, np.zeros(0, np.float32))
+            continue
+        sel = _weighted_topk(wgt, k, rng)                    # [nI, k]
+        g = (spec.sGap * rng.lognormal(0.0, 0.5, size=sel.shape)).astype(np.float32)
+        post_l = np.broadcast_to(np.arange(nI)[:, None], sel.shape)
+        for ib in range(maxNear):
+            if ib < nNear[b]:
+                m = cand_slot[sel] == ib
+                pl = ((cand[sel][m] - nE) % bs).astype(np.uint32)
+                ql = post_l[m].astype(np.uint32)
+                gg = g[m]
+                order = np.lexsort((ql, pl))
+                out[b * maxNear + ib] = (pl[order], ql[order], gg[order])
+            else:
+                out[b * maxNear + ib] = (z, z, np.zeros(0, np.float32))
+    return out
 
 
 def lgn_layer(spec: ModelSpec, seed: int | None = None) -> dict:

@@ -46,9 +46,18 @@ class GpuSim:
         st = initial_state(net) if state is None else state
         self.state = {k: T(v) for k, v in st.items()}
         self.tonic = T(net.tonicDep, np.float32)
+        # gap junctions between inhibitory neurons: gapS (every inhibitory
+        # neuron with a junction, as the reference's fast update kernel takes
+        # it whether or not InhGap is on) and, with InhGap, the gap sums
+        mI = sp.nblock * sp.nI
+        self.gap_on = net.gap_on()
+        self.gapS = T(net.gap_strength_sums()) if net.has_gap() else None
+        self.gap = torch.zeros(mI, dtype=torch.float32, device=d) if self.gap_on else None
+        if (self.gapS is not None or self.gap_on) and getattr(net, "batch", None) is not None:
+            raise NotImplementedError("gap junctions with batched trials are not built")
         self.s = abi.V1State(*(ptr(self.state[k]) for k in
                                ("v", "w", "tBack", "gFF", "hFF", "gE", "hE", "gI", "hI")),
-                             ptr(self.tonic))
+                             ptr(self.tonic), ptr(self.gapS), ptr(self.gap))
         # ---- FF synapses
         self.nLGN_of = T(net.nLGN_of.astype(np.uint32).view(np.int32))
         self.sLGN = T(net.sLGN, np.float32)
@@ -89,6 +98,17 @@ class GpuSim:
         self.wDense = self._dense_weights() if dense_weights else None
         self.rec = abi.RecConn(net.maxNear, ptr(self.nNear), ptr(self.nbr), ptr(self.rowMask),
                                ptr(self.rowOff), ptr(self.wrec), ptr(self.wDense))
+        self.gap_lag = None
+        if self.gap_on:
+            if net.gap_far is not None and int(net.gap_far[0][-1]) > 0:
+                raise NotImplementedError("far gap junctions (V1_gapVec) with InhGap are not "
+                                          "built")
+            mg, cnt, P, G = net.gap_ell()
+            self.gap_t = [T(cnt.view(np.int32)), T(P.view(np.int32)), T(G)]
+            self.rec.maxGap = mg
+            self.rec.nGap, self.rec.gapPre, self.rec.gapG = (ptr(x) for x in self.gap_t)
+            if getattr(sp, "noDelay", 1) == 0:
+                self.gap_lag = T(net.gap_lags(sp.speedOfThought, sp.dt).view(np.int32))
         # ---- axonal delays (noDelay 0): spike-train ring of the last `depth`
         # step outputs and spike masks; every step is one vc_v1_step (no
         # undelayed increments) followed by vc_v1_rec_delay
@@ -113,6 +133,8 @@ class GpuSim:
         self.oh = [torch.zeros(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d) for _ in range(2)]
         self.spk = [abi.Spikes(ptr(self.masks[i]), ptr(self.og[i]), ptr(self.oh[i])) for i in range(2)]
         self.spk2 = (abi.Spikes * 2)(*self.spk)
+        if self.gapS is not None or self.gap_on:
+            persistent = False          # gap junctions need every step's voltages (vc_v1_run)
         if self.delay or self.far_only:
             zero = torch.zeros(nT, dtype=torch.int64, device=d)
             self.dog = torch.empty(abi.VC_MAX_NG, sp.nV1, dtype=torch.float32, device=d)
@@ -150,6 +172,11 @@ class GpuSim:
         self.lgn_hist: list[torch.Tensor] = []
         self.sInfo_step = torch.zeros(sp.nV1, dtype=torch.float32, device=d)
         self.sInfo_last = self.sInfo_step          # V1 output of the last step run (a view)
+        if self.gap_on:              # the pre-loop recal_G's gap sums (initial voltages)
+            if self.delay:
+                self._gap_after(self.train, (self.slot_off + self.depth - 1) % self.depth)
+            else:
+                self._gap_after(self.state["v"], 0)
         self.s_lgn.wait_stream(self.s_v1)          # set-up work precedes the pipeline
         # launch shape of the one-step kernel (runs whose grid cannot be
         # resident at once): timed per workload when asked, else the default
@@ -200,9 +227,22 @@ class GpuSim:
             self.far = abi.FarConn(mf, *(ptr(x) for x in self.far_t))
             fmax = float(net.far[3].max()) if net.far[3].size else 0.0
             self.depth = max(self.depth, train_depth(fmax, self.speed, sp.dt))
-        self.train = torch.zeros(self.depth, sp.nV1, dtype=torch.float32, device=d)
+        if self.gap_lag is not None:         # the junctions' delays read the ring too
+            self.depth = max(self.depth, int(self.gap_lag.max().item()) + 1)
+        # every slot starts with the initial voltages (rand_spInit without
+        # initial spikes, src/coredynamics.cu:119-133)
+        self.train = self.state["v"][None, :].repeat(self.depth, 1).contiguous()
         self.trainMask = torch.zeros(self.depth, sp.nV1 // 64, dtype=torch.int64, device=d)
         self.slot_off = 0            # ring slot of step it is (it + slot_off) % depth
+
+    def _gap_after(self, train: torch.Tensor, cur: int):
+        """The gap sums after a step (recal_G's), from the step outputs in
+        slot `cur` of `train` ([depth, nV1], or one row [nV1]), the junctions'
+        delays applied when the ring is the delay path's."""
+        depth = train.shape[0] if train.dim() == 2 else 1
+        lag = ptr(self.gap_lag) if (self.gap_lag is not None and depth > 1) else None
+        check(self.lib.vc_v1_gap(self._hs(self.s_v1), C.byref(self.p), C.byref(self.s),
+                                 C.byref(self.rec), ptr(train), depth, cur, lag), "vc_v1_gap")
 
     def _delay_step(self, buf: int, k_in: int, it: int, sInfo=None, dep=None, gprev=None):
         """Step `it` with axonal delays: the update writes ring slot `slot`,
@@ -225,6 +265,8 @@ class GpuSim:
             check(self.lib.vc_v1_rec_far(h, C.byref(self.p), C.byref(self.s), C.byref(self.far),
                                          ptr(self.train), ptr(self.trainMask), self.depth,
                                          slot, self.speed, ptr(self.status)), "vc_v1_rec_far")
+        if self.gap_on:
+            self._gap_after(self.train, slot)
         if sInfo is not None:
             sInfo.copy_(self.train[slot])
         return slot
@@ -248,6 +290,8 @@ class GpuSim:
         check(self.lib.vc_v1_rec_far(h, C.byref(self.p), C.byref(self.s), C.byref(self.far),
                                      ptr(self.train), ptr(self.trainMask), self.depth, slot,
                                      self.speed, ptr(self.status)), "vc_v1_rec_far")
+        if self.gap_on:
+            self._gap_after(self.train[slot], 0)
         if sInfo is not None:
             sInfo.copy_(self.train[slot])
         return slot
@@ -488,7 +532,13 @@ class GpuSim:
             if events is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e0.record(sv)
-            if self.delay:
+            if self.delay and self.gap_on:
+                # one step at a time: each step's gap sums follow it
+                for i in range(n):
+                    row = out[done + i] if out is not None else None
+                    slot = self._delay_step(buf, k_in + i, it + i, sInfo=row)
+                self.sInfo_last = out[done + n - 1] if out is not None else self.train[slot]
+            elif self.delay:
                 # the n steps in one native call (vc_v1_delay_run)
                 slot0 = (it + self.slot_off) % self.depth
                 check(self.lib.vc_v1_delay_run(
@@ -604,16 +654,20 @@ class GpuSim:
                                           C.byref(self.spk[self.cur]), C.byref(self.spk[nxt]),
                                           ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, mptr,
                                           it, ptr(sInfo), ptr(dep), ptr(gprev)), "vc_v1_step")
+                if self.gap_on:
+                    self._gap_after(sInfo, 0)
                 self.cur = nxt
             self.it += 1
             self.sInfo_last = sInfo
             if raw:
                 if held is not None:
                     gE, gI, hE, hI = (x.cpu().numpy() for x in g_parts(gprev))
-                    raw.write_step(*held, gE, gI, hE, hI)
+                    raw.write_step(*held[:6], gE, gI, hE, hI, gap=held[6])
+                # the gap sums after the step (recal_G's), zero without InhGap
                 held = (sInfo.cpu().numpy(), dep.cpu().numpy(), st["w"].cpu().numpy(),
                         st["v"].cpu().numpy(), st["gFF"][:ngFF].cpu().numpy(),
-                        st["hFF"][:ngFF].cpu().numpy())
+                        st["hFF"][:ngFF].cpu().numpy(),
+                        self.gap.cpu().numpy() if self.gap is not None else None)
             if self.it % npf == 0:
                 e = torch.cuda.Event()
                 e.record(self.s_v1)
@@ -623,8 +677,8 @@ class GpuSim:
         if raw:
             self.finalize()
             torch.cuda.synchronize()
-            raw.write_step(*held, st["gE"].cpu().numpy(), st["gI"].cpu().numpy(),
-                           st["hE"].cpu().numpy(), st["hI"].cpu().numpy())
+            raw.write_step(*held[:6], st["gE"].cpu().numpy(), st["gI"].cpu().numpy(),
+                           st["hE"].cpu().numpy(), st["hI"].cpu().numpy(), gap=held[6])
             raw.close()
         for w in (fr_w, sp_w):
             if w:

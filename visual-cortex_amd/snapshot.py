@@ -200,7 +200,8 @@ def save(sim, path) -> dict:
     block = dict(depC=dep.cpu().numpy(), w=st["w"], v=st["v"], gFF=st["gFF"][:ngFF],
                  hFF=st["hFF"][:ngFF], gE=st["gE"][:ngE], gI=st["gI"][:ngI],
                  hE=st["hE"][:ngE], hI=st["hI"][:ngI],
-                 gap=np.zeros(sp.nblock * sp.nI, np.float32))
+                 gap=sim.gap.cpu().numpy() if getattr(sim, "gap", None) is not None
+                 else np.zeros(sp.nblock * sp.nI, np.float32))
     write_file(path, h, dict(leftTimeRate=lg[0], lastNegLogRand=lg[1]), dict(tBack=st["tBack"]),
                train, block,
                keys=(int(sim.p.seed_lo), int(sim.p.seed_hi)))
@@ -235,6 +236,8 @@ def restore(sim, path, as_init: bool = False) -> dict:
     for k, ng in (("gFF", ngFF), ("hFF", ngFF), ("gE", ngE), ("gI", ngI), ("hE", ngE),
                   ("hI", ngI)):
         st[k][:ng].copy_(T(d[k]))
+    if getattr(sim, "gap", None) is not None:       # the gap sums after the last step
+        sim.gap.copy_(T(d["gap"]))
     sim.lTR.copy_(T(d["leftTimeRate"]))
     sim.lNL.copy_(T(d["lastNegLogRand"]))
     ring = d["spikeTrain"]
