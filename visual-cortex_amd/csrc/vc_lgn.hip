@@ -202,7 +202,7 @@ __device__ __forceinline__ float tree32_regs(float a[32]) {
 }
 
 constexpr int kMaxFrame = 8;
-constexpr int kWin = 8;                     // rate window of k_lgn_spike
+constexpr int kWin = 16;                    // rate window of k_lgn_spike
 
 
 // Stage a (temporal part) and the rate of stage b for every (cell, step):
@@ -301,6 +301,12 @@ void launch_rate(hipStream_t st, uint64_t waves, const vc_lgn_params &p, const v
 
 // Stage b, the time-rescaled Poisson draw (get_spike,
 // src/discrete_input_convol.cu:1587-1622): sequential in time per cell.
+// The steps are taken 64 at a time: first every step's draw (no memory
+// round trip inside the loop: the rates are prefetched, the outputs stored
+// without waiting), then the deliveries of the window's spikes to the
+// synapse bits of the V1 neurons they feed, in groups of kDelGroup (spike,
+// cell) pairs whose target lists are loaded together.
+constexpr int kDelGroup = 4;
 __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state st,
                                                    uint32_t it0, uint32_t nsteps,
                                                    const float *__restrict__ frIn,
@@ -309,7 +315,6 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = c < p.nLGN;            // every lane runs the step loop (ballots)
     const uint32_t lane = vc_lane();
-    const uint32_t cbase = c - lane;           // cell of lane 0 of this wavefront
     float lTR = valid ? st.leftTimeRate[c] : 0.0f;
     float lNL = valid ? st.lastNegLogRand[c] : 1.0f;
     uint32_t klo = p.seed_lo, khi = p.seed_hi, ent = c;     // Philox key, entity id
@@ -319,76 +324,132 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
         khi = p.trialSeed[2 * tr + 1];
         ent = c - p.trialLgnOff[tr];
     }
-    auto step_body = [&](uint32_t s, float fr) {
+    // this cell's target list, read once
+    uint32_t myk0 = 0, myk1 = 0;
+    if (ev.mask != nullptr && valid) {
+        myk0 = ev.revOff[c];
+        myk1 = ev.revOff[c + 1];
+    }
+    // one step of one cell: rate = fr/1000 and rT = dt*rate come precomputed
+    // (the same operations, taken out of the time-sequential chain, which
+    // without a spike is one compare and one add per step)
+    auto draw = [&](uint32_t s, float rate, float rT) -> float {
         const uint32_t it = it0 + s;
-        float sInfo = 0.0f;
-        if (valid) {
-            const float rate = fr / 1000.0f;
-            const float rT = p.dt * rate;
-            float next_rT = lNL - lTR;
-            uint32_t nsp = 0;
-            float tsp = 0.0f;
-            float lNLn = lNL;
-            if (next_rT > rT) {
-                lTR += rT;
-            } else {
-                float old_rT = 0.0f;
-                uint32_t kdraw = 0;
-                vc_u32x4 pack;
-                const float tr = p.tRef * rate;
-                do {
-                    if ((kdraw & 3u) == 0u)
-                        pack = vc_draw4(klo, khi, ent, it, VC_STREAM_LGN_SPIKE, kdraw >> 2,
-                                        p.trial);
-                    const float u = vc_u01_open0(pack.v[kdraw & 3u]);
-                    kdraw++;
-                    const float x = -vc_logf(u);
-                    lNLn = (x > tr) ? x : tr;
-                    old_rT = next_rT;
-                    next_rT += lNLn;
-                    nsp++;
-                } while (next_rT <= rT);
-                lTR = rT - old_rT;
-                tsp = old_rT / (rate * (float)nsp);
-            }
-            sInfo = (float)nsp + tsp / p.dt;
-            if (sInfo > 0.0f) lNL = lNLn;
-            sInfoOut[(size_t)s * p.nLGN + c] = sInfo;
+        float next_rT = lNL - lTR;
+        uint32_t nsp = 0;
+        float tsp = 0.0f;
+        float lNLn = lNL;
+        if (next_rT > rT) {
+            lTR += rT;
+            return 0.0f;                       // nsp + tsp/dt with nsp = tsp = 0
+        } else {
+            float old_rT = 0.0f;
+            uint32_t kdraw = 0;
+            vc_u32x4 pack;
+            const float tr = p.tRef * rate;
+            do {
+                if ((kdraw & 3u) == 0u)
+                    pack = vc_draw4(klo, khi, ent, it, VC_STREAM_LGN_SPIKE, kdraw >> 2, p.trial);
+                const float u = vc_u01_open0(pack.v[kdraw & 3u]);
+                kdraw++;
+                const float x = -vc_logf(u);
+                lNLn = (x > tr) ? x : tr;
+                old_rT = next_rT;
+                next_rT += lNLn;
+                nsp++;
+            } while (next_rT <= rT);
+            lTR = rT - old_rT;
+            tsp = old_rT / (rate * (float)nsp);
         }
-        // deliver this step's spikes to the synapse bits of the V1 neurons they
-        // feed: the wavefront walks its spiking cells, lanes over targets
-        if (ev.mask != nullptr) {
-            uint64_t sm = __ballot(valid && sInfo >= 1.0f);
-            uint32_t *row = ev.mask + (size_t)s * ev.nV1;
-            while (sm) {
-                const uint32_t l = (uint32_t)__builtin_ctzll(sm);
-                sm &= sm - 1ull;
-                const uint32_t cl = cbase + l;
-                const uint32_t k0 = ev.revOff[cl], k1 = ev.revOff[cl + 1];
-                for (uint32_t k = k0 + lane; k < k1; k += 64u) {
-                    const uint32_t tg = ev.revTgt[k];
-                    atomicOr(row + (tg >> 5), 1u << (tg & 31u));
-                }
-            }
-        }
+        const float sInfo = (float)nsp + tsp / p.dt;
+        if (sInfo > 0.0f) lNL = lNLn;
+        return sInfo;
     };
-    // the rates arrive kWin steps at a time, the next window loaded while the
-    // current one is drawn
-    float cur[kWin], nxt[kWin];
-#pragma unroll
-    for (uint32_t j = 0; j < (uint32_t)kWin; ++j)
-        cur[j] = (valid && j < nsteps) ? frIn[(size_t)j * p.nLGN + c] : 0.0f;
-    for (uint32_t s0 = 0; s0 < nsteps; s0 += (uint32_t)kWin) {
-#pragma unroll
-        for (uint32_t j = 0; j < (uint32_t)kWin; ++j) {
-            const uint32_t sn = s0 + (uint32_t)kWin + j;
-            nxt[j] = (valid && sn < nsteps) ? frIn[(size_t)sn * p.nLGN + c] : 0.0f;
-        }
+    for (uint32_t w0 = 0; w0 < nsteps; w0 += 64u) {
+        const uint32_t wn = (nsteps - w0) < 64u ? (nsteps - w0) : 64u;
+        // ---- the draws of the window; the rates arrive kWin steps at a time
+        uint64_t spiked = 0;                   // bit s: this cell spiked in step w0 + s
+        float cur[kWin], nxt[kWin];
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)kWin; ++j)
-            if (s0 + j < nsteps) step_body(s0 + j, cur[j]);
+            cur[j] = (valid && j < wn) ? frIn[(size_t)(w0 + j) * p.nLGN + c] : 0.0f;
+        for (uint32_t s0 = 0; s0 < wn; s0 += (uint32_t)kWin) {
 #pragma unroll
-        for (uint32_t j = 0; j < (uint32_t)kWin; ++j) cur[j] = nxt[j];
+            for (uint32_t j = 0; j < (uint32_t)kWin; ++j) {
+                const uint32_t sn = s0 + (uint32_t)kWin + j;
+                nxt[j] = (valid && sn < wn) ? frIn[(size_t)(w0 + sn) * p.nLGN + c] : 0.0f;
+            }
+            float rate[kWin], rT[kWin];
+#pragma unroll
+            for (uint32_t j = 0; j < (uint32_t)kWin; ++j) {
+                rate[j] = cur[j] / 1000.0f;
+                rT[j] = p.dt * rate[j];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < (uint32_t)kWin; ++j) {
+                const uint32_t sl = s0 + j;
+                if (valid && sl < wn) {
+                    const float sInfo = draw(w0 + sl, rate[j], rT[j]);
+                    sInfoOut[(size_t)(w0 + sl) * p.nLGN + c] = sInfo;
+                    if (sInfo >= 1.0f) spiked |= 1ull << sl;
+                }
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < (uint32_t)kWin; ++j) cur[j] = nxt[j];
+        }
+        if (ev.mask == nullptr) continue;
+        // ---- deliveries, in (step, cell) order: the wavefront walks the
+        // spikes kDelGroup at a time, lanes over each spike's targets; the
+        // group's target loads are issued together before its atomics
+        uint32_t sstep = 0;
+        uint64_t sm = __ballot(spiked & 1ull);
+        auto next_pair = [&](uint32_t &l, uint32_t &sp) -> bool {
+            while (sm == 0ull) {
+                if (++sstep >= wn) return false;
+                sm = __ballot((spiked >> sstep) & 1ull);
+            }
+            l = (uint32_t)__builtin_ctzll(sm);
+            sm &= sm - 1ull;
+            sp = sstep;
+            return true;
+        };
+        for (;;) {
+            uint32_t gl[kDelGroup], gs[kDelGroup], gk0[kDelGroup], gk1[kDelGroup];
+            uint32_t ng = 0;
+#pragma unroll
+            for (int q = 0; q < kDelGroup; ++q) {
+                gl[q] = gs[q] = gk0[q] = gk1[q] = 0u;
+                if ((uint32_t)q == ng && next_pair(gl[q], gs[q])) {
+                    gk0[q] = vc_readlane_u32(myk0, gl[q]);
+                    gk1[q] = vc_readlane_u32(myk1, gl[q]);
+                    ++ng;
+                }
+            }
+            if (ng == 0) break;
+            // the first two 64-target chunks of every pair in the group
+            uint32_t tg[kDelGroup][2];
+#pragma unroll
+            for (int q = 0; q < kDelGroup; ++q)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t k = gk0[q] + 64u * (uint32_t)h + lane;
+                    tg[q][h] = ((uint32_t)q < ng && k < gk1[q]) ? ev.revTgt[k] : 0xffffffffu;
+                }
+#pragma unroll
+            for (int q = 0; q < kDelGroup; ++q) {
+                if ((uint32_t)q >= ng) break;
+                uint32_t *row = ev.mask + (size_t)(w0 + gs[q]) * ev.nV1;
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    if (tg[q][h] != 0xffffffffu)
+                        atomicOr(row + (tg[q][h] >> 5), 1u << (tg[q][h] & 31u));
+                for (uint32_t k = gk0[q] + 128u + lane; k < gk1[q]; k += 64u) {
+                    const uint32_t t2 = ev.revTgt[k];
+                    atomicOr(row + (t2 >> 5), 1u << (t2 & 31u));
+                }
+            }
+            if (ng < (uint32_t)kDelGroup) break;
+        }
     }
     if (valid) {
         st.leftTimeRate[c] = lTR;
