@@ -123,3 +123,37 @@ def test_c1_first_200_steps_bitwise(pkg):
     sg, so = g.sInfo(), o.sInfo()
     assert (so >= 1).sum() > 0
     assert np.array_equal(sg.view(np.uint32), so.view(np.uint32))
+
+
+def test_short_runs_bitwise(pkg):
+    """Runs of a few steps at a time (the LGN stage issued per run, in chunks
+    shorter than a frame period: the lane-per-cell rate kernel) give the
+    oracle's V1 outputs and LGN state bit for bit."""
+    import torch
+    import oracle
+    from visual_cortex_amd import network
+    from visual_cortex_amd.sim import GpuSim
+    _flush("test_short_runs_bitwise: tiny, runs of 5/20/7/32/13/1/64/... steps")
+    net = network.build_network(network.spec_tiny(bgStrength=(0.4, 0.4)))
+    sp = net.spec
+    g = GpuSim(net, record=False)
+    tables = dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
+                  SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
+    o = oracle.OracleSim(net, pkg, tables=tables)
+    sizes = [5, 20, 7, 32, 13, 1, 64, 30, 2, 26, 56]      # 256 steps: the LGN state of
+    assert sum(sizes) % g.tm["ntPerFrame"] == 0               # both sides is at a period end
+    outs = []
+    for n in sizes:
+        out = torch.empty(n, sp.nV1, dtype=torch.float32, device="cuda")
+        g.run(n, out=out)
+        outs.append(out)
+    T = sum(sizes)
+    o.run(T)
+    sg = torch.cat(outs).cpu().numpy()
+    so = o.sInfo()
+    assert int((so >= 1).sum()) > 100
+    diff = np.argwhere(sg.view(np.uint32) != so.view(np.uint32))
+    assert diff.size == 0, f"first sInfo difference at (step, neuron) {diff[0]}"
+    torch.cuda.synchronize()
+    assert np.array_equal(g.lTR.cpu().numpy().view(np.uint32), o.lTR.view(np.uint32))
+    assert np.array_equal(g.lNL.cpu().numpy().view(np.uint32), o.lNL.view(np.uint32))

@@ -289,6 +289,92 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
     if (s < nsteps) frOut[(size_t)s * p.nLGN + c] = fr;
 }
 
+// k_lgn_rate for a short run of steps (at most kShortRun, e.g. a chunk that
+// ends a V1 launch early): lane = cell, each wavefront a slice of kStepsPerWave
+// steps in sequence (blockIdx.y), so that every lane works (lane = step would
+// leave most of a wavefront idle).  The cell's temporal weights are
+// registers, its frame responses a per-lane LDS row; the tap -> frame walk is
+// wave-uniform.  Per (cell, step) the products, the tap-sum tree and the rate
+// are those of k_lgn_rate, bit for bit.
+constexpr uint32_t kShortRun = 32;
+constexpr uint32_t kStepsPerWave = 4;
+template <int kNT>
+__global__ __launch_bounds__(64) void k_lgn_rate_cells(vc_lgn_params p, vc_lgn_state st,
+                                                       const float *__restrict__ Rring,
+                                                       uint32_t it0, uint32_t nsteps,
+                                                       float *__restrict__ frOut) {
+    constexpr int kF = kMaxFrame + 1;
+    constexpr int kRow = 2 * kF + 1;                   // odd stride: no bank conflicts
+    __shared__ float sR[64 * kRow];
+    const uint32_t ln = threadIdx.x;
+    const uint32_t c = blockIdx.x * 64u + ln;
+    const bool valid = c < p.nLGN;
+    const uint32_t cc = valid ? c : 0u;
+    const uint32_t npf = p.ntPerFrame;
+    // frames under the run's windows: from step 0's first tap on
+    const int64_t base0 = (int64_t)it0 + (int64_t)p.iKernelSampleT0 - (int64_t)p.nRetrace + 1;
+    const int64_t fw = floor_div(base0, (int64_t)npf);
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int d = 0; d < kF; ++d) {
+            int64_t slot = (fw + d) % (int64_t)p.maxFrame;
+            if (slot < 0) slot += p.maxFrame;
+            sR[ln * kRow + r * kF + d] = Rring[((size_t)slot * p.nLGN + cc) * 2 + r];
+        }
+    float tw[2][kNT];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int i = 0; i < kNT; ++i) tw[r][i] = st.TW[((size_t)cc * 2 + r) * kNT + i];
+    const float kC = st.kC[cc], kS = st.kS[cc], mx = st.maxConvol[cc];
+    const float c50 = st.c50[cc], sh = st.sharpness[cc], lA = st.logA[cc], lB = st.logB[cc];
+    const uint32_t dq = p.kernelSampleInterval / npf, dr = p.kernelSampleInterval % npf;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t s1 = (blockIdx.y + 1u) * kStepsPerWave < nsteps ? (blockIdx.y + 1u) * kStepsPerWave
+                                                                    : nsteps;
+    for (uint32_t s = blockIdx.y * kStepsPerWave; s < s1; ++s) {
+        const int64_t base = base0 + (int64_t)s;
+        const int64_t q0 = floor_div(base, (int64_t)npf);
+        const uint32_t rr0 = (uint32_t)(base - q0 * (int64_t)npf);
+        const int dd0 = (int)(q0 - fw);
+        float conv[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            float a0[32], a1[32];
+            uint32_t rr = rr0;
+            int dd = dd0;
+#pragma unroll
+            for (int i = 0; i < kMaxTaps; ++i) {
+                float v = 0.0f;
+                if (i < kNT) {
+                    v = sR[ln * kRow + r * kF + dd] * tw[r][i];
+                    rr += dr;
+                    dd += (int)dq;
+                    if (rr >= npf) {
+                        rr -= npf;
+                        dd += 1;
+                    }
+                }
+                if (i < 32) a0[i] = v; else a1[i - 32] = v;
+            }
+            const float P0 = tree32_regs(a0);
+            const float P1 = tree32_regs(a1);
+            const float sum = ((P0 + 0.0f) + (0.0f + 0.0f)) + ((P1 + 0.0f) + (0.0f + 0.0f));
+            conv[r] = 0.0f + sum;
+        }
+        const float convolC = conv[0] * kC;
+        const float convolS = conv[1] * kS;
+        const float convol = convolC + convolS;
+        float current = convol * p.convolRatio;
+        if (current < 0.0f) current = 0.0f;
+        const float fr = p.frRatio * mx * transform(c50, sh, lA, lB, current / mx);
+        if (valid) frOut[(size_t)s * p.nLGN + c] = fr;
+    }
+}
+
 // k_lgn_rate with the tap count fixed when it is the configurations' 40.
 void launch_rate(hipStream_t st, uint64_t waves, const vc_lgn_params &p, const vc_lgn_state &s,
                  const float *R, uint32_t it0, uint32_t nsteps, float *fr) {
@@ -751,8 +837,14 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
         return VC_E_INVALID;
     }
     if (nsteps == 0 || p->nLGN == 0) return VC_OK;
-    const uint64_t waves = (uint64_t)p->nLGN * ((nsteps + 63u) / 64u);
-    launch_rate((hipStream_t)stream, waves, *p, *s, Rring, it0, nsteps, fr);
+    if (nsteps <= kShortRun && p->nKernelSample == 40) {
+        hipLaunchKernelGGL(k_lgn_rate_cells<40>,
+                           dim3((p->nLGN + 63u) / 64u, (nsteps + kStepsPerWave - 1u) / kStepsPerWave),
+                           dim3(64), 0, (hipStream_t)stream, *p, *s, Rring, it0, nsteps, fr);
+    } else {
+        const uint64_t waves = (uint64_t)p->nLGN * ((nsteps + 63u) / 64u);
+        launch_rate((hipStream_t)stream, waves, *p, *s, Rring, it0, nsteps, fr);
+    }
     VC_HIP_CHECK(hipGetLastError());
     vc_ff_events e{};
     if (ev) {

@@ -157,7 +157,9 @@ class GpuSim:
         # ---- streams (default priority) and the LGN pipeline state
         self.s_v1 = torch.cuda.current_stream(d)
         self.s_lgn = torch.cuda.Stream(device=d)
-        self.lgn_evq = []                # (end step, event) of issued LGN chunks
+        self.lgn_evq = []                # (end step, event, on V1 stream) of issued LGN chunks
+        self.slice_q = []                # frame shares queued by chunks on the V1 stream
+        self.lgn_tail = None             # event after the LGN stream's latest work
         self.v1_done = [None, None]      # event: V1 steps reading buffer i finished
         self.lgn_to = 0                  # the LGN stage is issued for steps < lgn_to
         # frames < fr_frame have their spatial response in the ring; frame
@@ -421,12 +423,16 @@ class GpuSim:
             ctr, self.trialFrameStride, ptr(self.Rring[fidx % self.ringF])),
             "vc_lgn_frame_response_cells")
 
-    def _lgn_chunk(self, a: int, b: int):
-        """Issue, on the LGN stream, the LGN stage of steps a .. b-1 (inside one
-        frame period k, a = lgn_to): the rest of frame k's spatial response if
-        it is not complete, the temporal part and the spike draw of the steps
-        into rows a..b-1 (mod ntPerFrame) of buffer k % 2, and the share
-        (b - a)/ntPerFrame of frame k+1's spatial response."""
+    def _lgn_chunk(self, a: int, b: int, on_v1: bool = False):
+        """Issue the LGN stage of steps a .. b-1 (inside one frame period k, a =
+        lgn_to): the rest of frame k's spatial response if it is not complete,
+        the temporal part and the spike draw of the steps into rows a..b-1
+        (mod ntPerFrame) of buffer k % 2, and the share (b - a)/ntPerFrame of
+        frame k+1's spatial response.  On the LGN stream, or with on_v1 on the
+        V1 stream (for the V1 launch that follows at once: it then needs no
+        wait on another stream, which can hold the host until the awaited work
+        is done); the frame share is then queued for the LGN stream
+        (_lgn_flush_slices, after the V1 launch)."""
         npf = self.tm["ntPerFrame"]
         sp = self.net.spec
         nL = sp.nLGN
@@ -434,10 +440,19 @@ class GpuSim:
         b_in = b - k * npf
         if a != self.lgn_to or not 0 < b_in <= npf or b <= a:
             raise ValueError(f"LGN chunk {a}..{b} out of order (issued up to {self.lgn_to})")
-        st = self.s_lgn
+        st = self.s_v1 if on_v1 else self.s_lgn
+        if on_v1:
+            # earlier LGN-stream work (frame shares): waited for only when it is
+            # still running (a wait on another stream's event costs the host
+            # tens of microseconds)
+            if self.lgn_tail is not None and not self.lgn_tail.query():
+                st.wait_event(self.lgn_tail)
+            self.lgn_tail = None
+        else:
+            self._lgn_flush_slices()
         buf = k % 2
         if a_in == 0:
-            if self.v1_done[buf] is not None:
+            if self.v1_done[buf] is not None and not on_v1:
                 st.wait_event(self.v1_done[buf])       # V1 of period k-2 released it
             with torch.cuda.stream(st):
                 self.lgn_at[buf][0].copy_(self.lTR)
@@ -455,38 +470,62 @@ class GpuSim:
                                     ptr(self.lgn_fr2[buf]) + a_in * nL * 4, ev),
               "vc_lgn_steps")
         # the V1 steps wait for this point only: the slice of the next frame
-        # below is read by the next period's LGN steps, on this stream
+        # below is read by the next period's LGN steps, on the LGN stream
         e = torch.cuda.Event()
         e.record(st)
-        self.lgn_evq.append((b, e))
+        self.lgn_evq.append((b, e, on_v1))
         self.lgn_to = b
         c1 = b_in * nL // npf
         if self.fr_frame == k + 1 and c1 > self.fr_cells:
-            self._frame_cells(st, k + 1, self.fr_cells, c1)
+            if on_v1:
+                self.slice_q.append((k + 1, self.fr_cells, c1, e))
+            else:
+                self._frame_cells(st, k + 1, self.fr_cells, c1)
             self.fr_cells = c1
             if c1 == nL:
                 self.fr_frame, self.fr_cells = k + 2, 0
+        if not on_v1:
+            self._lgn_mark_tail()
+
+    def _lgn_mark_tail(self):
+        """An event after the LGN stream's latest work (see _lgn_chunk)."""
+        self.lgn_tail = torch.cuda.Event()
+        self.lgn_tail.record(self.s_lgn)
+
+    def _lgn_flush_slices(self):
+        """Issue the frame shares queued by chunks on the V1 stream, on the LGN
+        stream after those chunks."""
+        if not self.slice_q:
+            return
+        for f, c0, c1, e in self.slice_q:
+            self.s_lgn.wait_event(e)
+            self._frame_cells(self.s_lgn, f, c0, c1)
+        self.slice_q = []
+        self._lgn_mark_tail()
 
     def _lgn_period(self, k: int):
         """Issue the LGN stage of the whole frame period k (it must be next)."""
         npf = self.tm["ntPerFrame"]
         self._lgn_chunk(k * npf, (k + 1) * npf)
 
-    def _lgn_issue_to(self, end: int):
+    def _lgn_issue_to(self, end: int, on_v1: bool = False):
         """Issue the LGN stage up to step `end` in chunks ending at period
-        boundaries."""
+        boundaries (on the V1 stream with on_v1, see _lgn_chunk)."""
         npf = self.tm["ntPerFrame"]
         while self.lgn_to < end:
             a = self.lgn_to
-            self._lgn_chunk(a, min(end, (a // npf + 1) * npf))
+            self._lgn_chunk(a, min(end, (a // npf + 1) * npf), on_v1)
 
     def _lgn_wait(self, stream, end: int):
-        """Make `stream` wait for the LGN chunk that covers step end-1."""
+        """Make `stream` wait for the LGN chunk that covers step end-1 (nothing
+        to wait for when that chunk was issued on the V1 stream itself)."""
         while self.lgn_evq and self.lgn_evq[0][0] < end:
             self.lgn_evq.pop(0)
         if not self.lgn_evq:
             raise RuntimeError(f"LGN stage of step {end - 1} not issued")
-        stream.wait_event(self.lgn_evq[0][1])
+        b, e, on_v1 = self.lgn_evq[0]
+        if not (on_v1 and stream is self.s_v1):
+            stream.wait_event(e)
 
     def run(self, nsteps: int, out: torch.Tensor | None = None, events: list | None = None):
         """Advance `nsteps` steps.  V1 sInfo of step i goes to out[i] if `out`
@@ -515,9 +554,8 @@ class GpuSim:
             k = it // npf
             k_in = it % npf
             n = min(npf - k_in, nsteps - done)
-            self._lgn_issue_to(lgn_end(it))
-            if it + n < end_all:                       # one launch ahead
-                self._lgn_issue_to(lgn_end(it + n))
+            # this launch's LGN steps, if not issued ahead: on the V1 stream
+            self._lgn_issue_to(lgn_end(it), on_v1=True)
             buf = k % 2
             self._lgn_wait(sv, it + n if not self.record else (k + 1) * npf)
             if k_in == 0 and self.record:
@@ -567,6 +605,7 @@ class GpuSim:
                     e = torch.cuda.Event()
                     e.record(sv)
                     self.v1_done[buf] = e
+                self._lgn_ahead(end_all, lgn_end)
                 continue
             args = (self._hs(sv), C.byref(self.p), C.byref(self.s), C.byref(self.ff),
                     C.byref(self.rec), self.spk2, self.cur,
@@ -601,6 +640,15 @@ class GpuSim:
                 e = torch.cuda.Event()
                 e.record(sv)
                 self.v1_done[buf] = e
+            self._lgn_ahead(end_all, lgn_end)
+
+    def _lgn_ahead(self, end_all: int, lgn_end):
+        """After a V1 launch: the frame shares its chunk queued, then the LGN
+        stage of the next launch of the call, on the LGN stream (it overlaps
+        this launch)."""
+        self._lgn_flush_slices()
+        if self.it < end_all:
+            self._lgn_issue_to(lgn_end(self.it))
 
     def run_with_outputs(self, nsteps: int, paths: dict, outputs: dict, on_step=None):
         """Run `nsteps` steps one launch at a time and write the reference's
@@ -715,6 +763,8 @@ class GpuSim:
         self.lgn_to = it0
         self.fr_frame, self.fr_cells = K, 0
         self.lgn_evq = []
+        self.slice_q = []
+        self.lgn_tail = None
         self.v1_done = [None, None]
         torch.cuda.synchronize()
         self.s_lgn.wait_stream(self.s_v1)
