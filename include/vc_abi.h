@@ -97,6 +97,21 @@ typedef struct {
      * flight per lane in the recurrent gather (4, 8 or 16).  Results do not
      * depend on them (visual-cortex_amd/tune.py picks them by timing). */
     uint32_t stepWG, gatherTile;
+    /* Synaptic failure and voltage noise (src/patch.cu:273-280, cfg keys of
+     * the same names; 0 = off).  A failing synapse's strength f becomes
+     * f * (1 - p + sqrt(p (1 - p) / nSyn) * N(0,1)) each time it transmits
+     * (src/coredynamics.cu:1476-1480, 2766-2769): synFailFF/synPerConFF per
+     * post type for LGN inputs, synFail/synPerCon [pre type][post type] for
+     * V1 inputs.  noisyDep per type: sigma = sqrt(2 noisyDep (vT - vR) r dt)
+     * N(0,1), r = the neuron's tonicDep when noiseOnTonic, else 1; white
+     * (tauNoise 0: sigma/dt added to both RK2 slopes) or Ornstein-Uhlenbeck
+     * (tauNoise > 0, state in vc_v1_state.lastNoise), src/coredynamics.cu
+     * :1424-1441, 1549-1554, src/coredynamics.cuh:79-87. */
+    float synFailFF[VC_MAX_NTYPE], synPerConFF[VC_MAX_NTYPE];
+    float synFail[VC_MAX_NTYPE * VC_MAX_NTYPE], synPerCon[VC_MAX_NTYPE * VC_MAX_NTYPE];
+    float noisyDep[VC_MAX_NTYPE];
+    float tauNoise;
+    uint32_t noiseOnTonic;
 } vc_v1_params;
 
 /* Persistent V1 state, all [..][nV1] float32, receptor-major.
@@ -116,6 +131,8 @@ typedef struct {
     const float *tonicDep;     /* [nV1]       */
     const float *gapS;         /* [mI] or NULL */
     float *gap;                /* [mI] or NULL */
+    float *lastNoise;          /* [nV1] OU noise of the last step (tauNoise > 0),
+                                  the reference's last_noise; else NULL */
 } vc_v1_state;
 
 /* LGN->V1 synapses, ELL layout transposed like the reference's sLGN
@@ -268,9 +285,17 @@ int vc_v1_run_persistent_noalloc(void *stream, const vc_v1_params *p, const vc_v
 uint32_t vc_v1_run_persistent_noalloc_max_v1(void);
 
 /* Apply the recurrent increments of `prev` to gE/hE/gI/hI without stepping
- * (used after the last step, before reading or saving the state). */
+ * (used after the last step, before reading or saving the state).  With
+ * synaptic failure (p->synFail) it returns VC_E_UNSUPPORTED: use
+ * vc_v1_rec_apply_at. */
 int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                     const vc_rec_conn *rec, const vc_spikes *prev);
+
+/* vc_v1_rec_apply for the spikes fired in step `step`: the synaptic-failure
+ * draws of a spike are keyed by the step it was fired in, as vc_v1_step
+ * keys those of the spikes it applies (its step - 1). */
+int vc_v1_rec_apply_at(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                       const vc_rec_conn *rec, const vc_spikes *prev, uint32_t step);
 
 /* vc_v1_gap: s->gap[mI] from rec's gap junctions (rec->gapPre must be set):
  * for inhibitory neuron i, gap = sum over its junctions j, in the ELL's

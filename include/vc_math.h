@@ -147,10 +147,57 @@ VC_HD float vc_u01_open1(uint32_t x) {
 }
 
 /* Random streams (the third counter word). */
-#define VC_STREAM_LGN_SPIKE 1u
-#define VC_STREAM_LGN_INIT  2u
-#define VC_STREAM_V1_BG     3u
-#define VC_STREAM_V1_INIT   4u
+#define VC_STREAM_LGN_SPIKE  1u
+#define VC_STREAM_LGN_INIT   2u
+#define VC_STREAM_V1_BG      3u
+#define VC_STREAM_V1_INIT    4u
+#define VC_STREAM_V1_FAILFF  5u   /* LGN->V1 synaptic failure, pack = synapse slot     */
+#define VC_STREAM_V1_FAIL    6u   /* V1->V1 synaptic failure, pack = presynaptic id    */
+#define VC_STREAM_V1_NOISE   7u   /* voltage noise (noisyDep)                          */
+
+/* ---- cos(2 pi x) and a standard normal ---------------------------------- */
+/* cos(2 pi x) for x in [0, 1).  The reduction to z in [0, 1/8] is exact in
+ * binary32 (1 - x, 1/2 - y and 1/4 - z are exact for the ranges they are
+ * taken on); then the Taylor series of cos (degree 8) or sin (degree 9) of
+ * theta = 2 pi z, |theta| <= pi/4, in Horner form in theta^2 (truncation
+ * below 3e-8). */
+VC_HD float vc_cos2pif(float x) {
+    float y = x > 0.5f ? 1.0f - x : x;             /* cos(2 pi x) = cos(2 pi (1 - x))  */
+    float sgn = 1.0f;
+    if (y > 0.25f) { y = 0.5f - y; sgn = -1.0f; }  /* cos(pi - t) = -cos t             */
+    float r;
+    if (y <= 0.125f) {
+        float t = y * 6.28318548f;
+        float t2 = t * t;
+        float q = 2.48015873e-5f;                  /* 1/8!  */
+        q = q * t2 - 1.38888889e-3f;               /* 1/6!  */
+        q = q * t2 + 4.16666667e-2f;               /* 1/4!  */
+        q = q * t2 - 0.5f;                         /* 1/2!  */
+        r = q * t2 + 1.0f;
+    } else {                                       /* cos(2 pi y) = sin(2 pi (1/4 - y)) */
+        float t = (0.25f - y) * 6.28318548f;
+        float t2 = t * t;
+        float q = 2.75573192e-6f;                  /* 1/9!  */
+        q = q * t2 - 1.98412698e-4f;               /* 1/7!  */
+        q = q * t2 + 8.33333333e-3f;               /* 1/5!  */
+        q = q * t2 - 0.166666667f;                 /* 1/3!  */
+        r = t + t * (q * t2);
+    }
+    return sgn * r;
+}
+
+/* A standard normal from two 32-bit draws by the Box-Muller transform
+ * (Box and Muller, "A note on the generation of random normal deviates",
+ * Ann. Math. Stat. 29, 1958): sqrt(-2 log u1) cos(2 pi u2), u1 in (0, 1],
+ * u2 in [0, 1).  It replaces cuRAND's curand_normal (src/coredynamics.cu
+ * :1430, 1479, 2769); the square root is the correctly rounded one on both
+ * sides. */
+VC_HD float vc_normal(uint32_t a, uint32_t b) {
+    const float u1 = vc_u01_open0(a);
+    const float u2 = vc_u01_open1(b);
+    const float r = __builtin_sqrtf(-2.0f * vc_logf(u1));
+    return r * vc_cos2pif(u2);
+}
 
 /* Counter layout: (entity id, step, stream | (pack << 8), trial). */
 VC_HD vc_u32x4 vc_draw4(uint32_t seed_lo, uint32_t seed_hi, uint32_t entity,

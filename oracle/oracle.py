@@ -32,7 +32,7 @@ class Oracle:
         vp, u32, f32, i32 = C.c_void_p, C.c_uint32, C.c_float, C.c_int
         P = C.POINTER
         L.orc_v1_step.argtypes = [P(abi_mod.V1Params)] + [vp] * 14 + [u32, vp, vp, vp, vp, vp, vp, vp,
-                                                                      u32, vp, vp, i32, vp, vp]
+                                                                      u32, vp, vp, i32, vp, vp, vp]
         L.orc_v1_step.restype = i32
         L.orc_v1_rec_delay.argtypes = [P(abi_mod.V1Params)] + [vp] * 4 + [u32] + [vp] * 4 + \
             [f32, vp, u32, u32]
@@ -49,6 +49,10 @@ class Oracle:
         L.orc_expf.argtypes = [f32]; L.orc_expf.restype = f32
         L.orc_logf.argtypes = [f32]; L.orc_logf.restype = f32
         L.orc_philox.argtypes = [P(C.c_uint32 * 4), u32, u32, P(C.c_uint32 * 4)]
+        L.orc_cos2pi_vec.argtypes = [u32, vp, vp]
+        L.orc_normal_vec.argtypes = [u32, vp, vp, vp]
+        L.orc_draw_normal.argtypes = [u32] * 7
+        L.orc_draw_normal.restype = f32
         L.orc_cond_single.argtypes = [P(f32), P(f32), f32, f32, f32, f32]
         L.orc_cond_decay.argtypes = [P(f32), P(f32), f32, f32, f32]
         L.orc_block_reduce_export.argtypes = [vp, i32, i32]; L.orc_block_reduce_export.restype = f32
@@ -176,6 +180,11 @@ class OracleSim:
                 if need > self.depth:
                     self.depth = need
                     self.train = np.zeros((self.depth, sp.nV1), np.float32)
+        # the Ornstein-Uhlenbeck noise state (src/patch.cu:4086-4088: zeros)
+        self.lastNoise = np.zeros(sp.nV1, np.float32)
+        if any(x > 0 for x in sp.synFail) and (self.delay or self.far_only):
+            raise NotImplementedError("synaptic failure between V1 neurons with conduction "
+                                      "delays is not built")
         if self.delay or self.far_only:             # rand_spInit: every slot holds v
             self.train[:] = self.st["v"][None, :]
         if self.gap is not None:
@@ -294,7 +303,7 @@ class OracleSim:
                                       self.net.maxNear, _fp(self.nNear), _fp(self.nbr),
                                       _fp(self.conMat), _fp(self.ipre), _fp(self.npre), _fp(self.og),
                                       _fp(self.oh), it, _fp(out), None, 0 if self.delay else 1,
-                                      _fp(self.gapS), _fp(self.gap))
+                                      _fp(self.gapS), _fp(self.gap), _fp(self.lastNoise))
             if rc != 0:
                 raise RuntimeError(f"orc_v1_step returned {rc}")
             if self.delay:                  # recal_G_mat after compute_V, same step

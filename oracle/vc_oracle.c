@@ -106,6 +106,19 @@ void orc_cond_decay(float *g, float *h, float ddt, float rise, float decay) {
 
 float orc_expf(float x) { return vc_expf(x); }
 float orc_logf(float x) { return vc_logf(x); }
+void orc_cos2pi_vec(uint32_t n, const float *x, float *out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = vc_cos2pif(x[i]);
+}
+void orc_normal_vec(uint32_t n, const uint32_t *a, const uint32_t *b, float *out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = vc_normal(a[i], b[i]);
+}
+/* The normal of counter (ent, step, stream | pack << 8, trial) under key
+ * (klo, khi): the draw the noise and failure terms use. */
+float orc_draw_normal(uint32_t klo, uint32_t khi, uint32_t ent, uint32_t step,
+                      uint32_t stream, uint32_t pack, uint32_t trial) {
+    vc_u32x4 r = vc_draw4(klo, khi, ent, step, stream, pack, trial);
+    return vc_normal(r.v[0], r.v[1]);
+}
 void orc_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
     vc_u32x4 c;
     for (int i = 0; i < 4; ++i) c.v[i] = ctr[i];
@@ -143,14 +156,16 @@ static void orc_adex_init(orc_adex *m, const vc_v1_params *p, int t, float w0,
     }
 }
 
-/* src/coredynamics.cuh:75-87 with noise0 = noise1 = 0 */
-static void orc_rk2(orc_adex *m, float dt) {
+/* src/coredynamics.cuh:75-87: noise0 joins the first slope, noise1 the
+ * second */
+static void orc_rk2(orc_adex *m, float dt, float noise0, float noise1) {
     float dTgL = m->deltaT * m->gL;
     float fk1, fk2;
     if (m->deltaT > 0.0f)
         fk1 = (-m->a0 * m->v0 + m->b0 + dTgL * vc_expf((m->v0 - m->vT) / m->deltaT) - m->w0) / m->C;
     else
         fk1 = (-m->a0 * m->v0 + m->b0 - m->w0) / m->C;
+    fk1 += noise0;
     float gk1 = (m->a * (m->v0 - ORC_VL) - m->w0) / m->tau_w;
     float v1 = m->v0 + dt * fk1;
     float w1 = m->w0 + dt * gk1;
@@ -158,6 +173,7 @@ static void orc_rk2(orc_adex *m, float dt) {
         fk2 = (-m->a1 * v1 + m->b1 + dTgL * vc_expf((v1 - m->vT) / m->deltaT) - w1) / m->C;
     else
         fk2 = (-m->a1 * v1 + m->b1 - w1) / m->C;
+    fk2 += noise1;
     float gk2 = (m->a * (v1 - ORC_VL) - w1) / m->tau_w;
     m->w = m->w0 + dt * (gk1 + gk2) / 2.0f;
     m->v = m->v0 + dt * (fk1 + fk2) / 2.0f;
@@ -169,18 +185,39 @@ static inline int orc_type_of(const vc_v1_params *p, uint32_t local) {
     return (int)p->nType - 1;
 }
 
+/* The Philox key of neuron n's draws: the run's seed and n, or with batched
+ * trials its trial's seed and its id within the trial (off = the trial's
+ * first neuron). */
+typedef struct { uint32_t lo, hi, ent, off; } orc_key;
+static inline orc_key orc_key_of(const vc_v1_params *p, uint32_t n) {
+    orc_key k = {p->seed_lo, p->seed_hi, n, 0u};
+    if (p->blockTrial) {
+        uint32_t tr = p->blockTrial[n / p->blockSize];
+        k.lo = p->trialSeed[2 * tr];
+        k.hi = p->trialSeed[2 * tr + 1];
+        k.off = p->trialV1Off[tr];
+        k.ent = n - k.off;
+    }
+    return k;
+}
+
+/* Synaptic failure (src/coredynamics.cu:1476-1480 for LGN synapses,
+ * :2766-2769 for V1 ones): strength f times 1 - p + sqrt(p (1 - p) / nSyn)
+ * N(0, 1); the normal is drawn for (post neuron, step, stream, pack). */
+static inline float orc_fail(float f, float pf, float nSyn, orc_key k, uint32_t step,
+                             uint32_t stream, uint32_t pack, uint32_t trial) {
+    float sd = sqrtf(pf * (1.0f - pf) / nSyn);
+    vc_u32x4 r = vc_draw4(k.lo, k.hi, k.ent, step, stream, pack, trial);
+    float rnd = vc_normal(r.v[0], r.v[1]);
+    return f * (1.0f - pf + sd * rnd);
+}
+
 /* Background drive (extension): one Philox draw per neuron-step gives the
  * Poisson count (inversion) and the event time. */
 static inline uint32_t orc_bg_draw(const vc_v1_params *p, int t, uint32_t n,
                                    uint32_t step, float *frac) {
-    uint32_t klo = p->seed_lo, khi = p->seed_hi, ent = n;
-    if (p->blockTrial) {                 /* batched trials: the trial's key, local id */
-        uint32_t tr = p->blockTrial[n / p->blockSize];
-        klo = p->trialSeed[2 * tr];
-        khi = p->trialSeed[2 * tr + 1];
-        ent = n - p->trialV1Off[tr];
-    }
-    vc_u32x4 r = vc_draw4(klo, khi, ent, step, VC_STREAM_V1_BG, 0, p->trial);
+    orc_key k = orc_key_of(p, n);
+    vc_u32x4 r = vc_draw4(k.lo, k.hi, k.ent, step, VC_STREAM_V1_BG, 0, p->trial);
     float u = vc_u01_open0(r.v[0]);
     *frac = vc_u01_open1(r.v[1]);
     return vc_poisson_inv(u, p->bgLambda[t], p->bgP0[t]);
@@ -193,7 +230,8 @@ static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
                                const float *tonicDep, const uint32_t *nLGN,
                                const float *sLGN, const uint32_t *idLGN,
                                const float *lgnSInfo, uint32_t step,
-                               float *out_dep, const float *gapS, const float *gap) {
+                               float *out_dep, const float *gapS, const float *gap,
+                               float *lastNoise) {
     const uint32_t nV1 = p->nV1;
     const uint32_t local = n % p->blockSize;
     const int t = orc_type_of(p, local);
@@ -223,8 +261,28 @@ static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
         g0[ig] = g1[ig]; h0[ig] = h1[ig];
         orc_decay(&g1[ig], &h1[ig], dt, p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
     }
+    /* voltage noise, src/coredynamics.cu:1424-1445: white (tauNoise = 0)
+     * enters both slopes as noise1/dt, Ornstein-Uhlenbeck (tauNoise > 0) as
+     * the previous value (noise0) and the new one (noise1) */
+    const orc_key key = orc_key_of(p, n);
+    const float tau = p->tauNoise;
+    float noise0 = 0.0f;
+    if (tau > 0.0f) noise0 = lastNoise ? lastNoise[n] : 0.0f;
+    float ratio = p->noiseOnTonic ? tonicDep[n] : 1.0f;
+    float noise1 = sqrtf(2.0f * p->noisyDep[t] * (p->vT[t] - p->vR[t]) * ratio * dt);
+    if (noise1 > 0.0f) {
+        vc_u32x4 r = vc_draw4(key.lo, key.hi, key.ent, step, VC_STREAM_V1_NOISE, 0, p->trial);
+        noise1 *= vc_normal(r.v[0], r.v[1]);
+        if (tau > 0.0f) {
+            noise1 /= tau;
+            float hk1 = -noise0 / tau;
+            float hk2 = -(noise0 + dt * hk1 + noise1) / tau;
+            noise1 += noise0 + dt * (hk1 + hk2) / 2.0f;
+            if (lastNoise) lastNoise[n] = noise1;
+        }
+    }
     const int backing = (m.tBack < dt) && (m.tBack > 0.0f);
-    if (out_dep) out_dep[n] = m.depC;
+    if (out_dep) out_dep[n] = m.depC + noise1;
     if (backing)
         for (uint32_t ig = 0; ig < p->ngFF; ++ig)
             orc_decay(&g0[ig], &h0[ig], m.tBack, p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
@@ -237,6 +295,9 @@ static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
         uint32_t nsp = (uint32_t)floorf(s);
         if (nsp > 0) {
             float tsp = (s - (float)nsp) * dt;
+            if (p->synFailFF[t] > 0.0f)
+                f = orc_fail(f, p->synFailFF[t], p->synPerConFF[t], key, step,
+                             VC_STREAM_V1_FAILFF, i, p->trial);
             if (f > 0.0f) {
                 float ddt = 0.0f;
                 if (backing) ddt = m.tBack - tsp;
@@ -306,7 +367,10 @@ static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
             float A = m.a * (m.v0 - ORC_VL);
             m.w0 = (m.w0 - A) * vc_expf(-m.tBack / m.tau_w) + A;
         }
-        orc_rk2(&m, new_dt);
+        if (tau > 0.0f)
+            orc_rk2(&m, new_dt, noise0, noise1);
+        else
+            orc_rk2(&m, new_dt, noise1 / dt, noise1 / dt);
         if (m.v > m.vThres) {
             float ratio = (m.vThres - m.v0) / (m.v - m.v0);
             m.tsp = m.tBack + ratio * new_dt;
@@ -338,7 +402,9 @@ static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
  *   the reference's spike lists and unit increments (E list first).
  *   conMat: dense [nblock][maxNear][bs][bs].
  * If do_rec is 0 the recurrent increments are not applied (the lists are
- * still produced).
+ * still produced).  gapS/gap: the inhibitory neurons' gap-junction terms
+ * (NULL: none); lastNoise: the Ornstein-Uhlenbeck noise state (NULL
+ * without it).
  */
 int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
                 float *gFF, float *hFF, float *gE, float *hE, float *gI,
@@ -348,7 +414,7 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
                 const uint32_t *nbr, const float *conMat, uint32_t *ipre,
                 uint32_t *npre, float *og, float *oh, uint32_t step,
                 float *out_sInfo, float *out_dep, int do_rec,
-                const float *gapS, const float *gap) {
+                const float *gapS, const float *gap, float *lastNoise) {
     const uint32_t nV1 = p->nV1, bs = p->blockSize, nblock = p->nblock;
     const uint32_t nE = p->nE, nI = bs - nE;
     if (nV1 != nblock * bs) return VC_E_INVALID;
@@ -360,7 +426,7 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
     for (uint32_t n = 0; n < nV1; ++n) {
         sInfo[n] = orc_update_neuron(p, n, v, w, tBack, gFF, hFF, gE, hE, gI,
                                      hI, tonicDep, nLGN, sLGN, idLGN, lgnSInfo,
-                                     step, out_dep, gapS, gap);
+                                     step, out_dep, gapS, gap, lastNoise);
     }
     /* compaction, src/coredynamics.cu:1994-2046, ascending order */
     const uint32_t block_ng = p->ngE * nE + p->ngI * nI;
@@ -407,6 +473,14 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
             int t = orc_type_of(p, l);
             float lgE[VC_MAX_NG] = {0}, lhE[VC_MAX_NG] = {0};
             float lgI[VC_MAX_NG] = {0}, lhI[VC_MAX_NG] = {0};
+            /* synaptic failure by presynaptic type (E: the first, I: the
+             * last) and this neuron's type, drawn per (this neuron, step,
+             * presynaptic neuron) */
+            const orc_key key = orc_key_of(p, n);
+            const float pfE = p->synFail[0 * p->nType + t];
+            const float pfI = p->synFail[(p->nType - 1) * p->nType + t];
+            const float nsE = p->synPerCon[0 * p->nType + t];
+            const float nsI = p->synPerCon[(p->nType - 1) * p->nType + t];
             for (uint32_t ib = 0; ib < nNear[b]; ++ib) {
                 uint32_t bid = nbr[b * maxNear + ib];
                 uint32_t cE = npre[bid], cI = npre[nblock + bid];
@@ -414,6 +488,9 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
                 for (uint32_t i = 0; i < cE; ++i) {
                     uint32_t pre = ipre[bid * bs + i];
                     float strength = conMat[(local_bid * bs + pre) * bs + l];
+                    if (strength > 0.0f && pfE > 0.0f)
+                        strength = orc_fail(strength, pfE, nsE, key, step, VC_STREAM_V1_FAIL,
+                                            bid * bs + pre - key.off, p->trial);
                     if (strength > 0.0f) {
                         for (uint32_t ig = 0; ig < p->ngE; ++ig) {
                             float str = strength * p->pE[t * p->ngE + ig];
@@ -426,6 +503,9 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
                 for (uint32_t i = 0; i < cI; ++i) {
                     uint32_t pre = ipre[bid * bs + cE + i];
                     float strength = conMat[(local_bid * bs + pre) * bs + l];
+                    if (strength > 0.0f && pfI > 0.0f)
+                        strength = orc_fail(strength, pfI, nsI, key, step, VC_STREAM_V1_FAIL,
+                                            bid * bs + pre - key.off, p->trial);
                     if (strength > 0.0f) {
                         for (uint32_t ig = 0; ig < p->ngI; ++ig) {
                             float str = strength * p->pI[t * p->ngI + ig];

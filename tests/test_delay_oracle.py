@@ -178,7 +178,7 @@ def test_infinite_speed_gives_undelayed_increments(pkg):
                       fp(t.st["gI"]), fp(t.st["hI"]), fp(t.tonic), fp(t.nLGN_of), fp(t.sLGN),
                       fp(t.idLGN), fp(lgn_row), net.maxNear, fp(t.nNear), fp(t.nbr),
                       fp(t.conMat), fp(t.ipre), fp(t.npre), fp(t.og), fp(t.oh), 0, fp(out),
-                      None, 0, None, None)
+                      None, 0, None, None, None)
     assert np.array_equal(out.view(np.uint32), s.view(np.uint32))
     # delayed at speed 1e9 mm/ms from the same spikes
     c = osim(pkg, net)

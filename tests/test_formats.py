@@ -29,8 +29,20 @@ def test_cfg_semantics(pkg):
     sp = config.spec_from_config(c)
     assert sp.pFF == ((0.9, 0.1), (0.8, 0.2)) and sp.dt == 0.25
     assert config.spec_from_config(config.Config("InhGap = true")).InhGap == 1
+    # noise and failure keys as src/minimal5.cfg:75-120 writes them
+    sp = config.spec_from_config(config.Config(
+        "tau_noise = 2.5\nnoiseOnTonic = false\nnoisyDep = 0.025, 0.05\nsynFailFF = 0.1\n"
+        "synFail = 0.2, 0.0\nsynFail = -0.5, 0.3\nsynPerConFF = 50, 60.0\n"
+        "synPerCon = 5.0, 7.5\nsynPerCon = 4.26, 7.88"))
+    assert sp.tau_noise == 2.5 and sp.noiseOnTonic == 0 and sp.noisyDep == (0.025, 0.05)
+    assert sp.synFailFF == (0.1, 0.1) and sp.synPerConFF == (50.0, 60.0)
+    assert sp.synFail == (0.2, 0.0, 0.0, 0.3)                     # negative rates become 0
+    assert sp.synPerCon == (5.0, 7.5, 4.26, 7.88)
+    assert config.spec_from_config(config.Config("")).noiseOnTonic == 1   # the default
+    with pytest.raises(ValueError):
+        config.spec_from_config(config.Config("synFail = 0.1, 0.1"))     # needs nType x nType
     with pytest.raises(NotImplementedError):
-        config.spec_from_config(config.Config("noisyDep = 0.1, 0"))
+        config.spec_from_config(config.Config("spE0 = 0.1, 0"))
     sp = config.spec_from_config(config.Config("noDelay = 0\nnoFarDelay = 0\nspeedOfThought = 0.5"))
     assert sp.noDelay == 0 and sp.speedOfThought == 0.5 and sp.noFarDelay == 0
     sp = config.spec_from_config(config.Config(""))

@@ -189,7 +189,7 @@ def test_recurrent_sum_brute_force_and_lists(pkg):
                     fp(t.st["hFF"]), fp(t.st["gE"]), fp(t.st["hE"]), fp(t.st["gI"]), fp(t.st["hI"]),
                     fp(t.tonic), fp(t.nLGN_of), fp(t.sLGN), fp(t.idLGN), fp(lgn_row), net.maxNear,
                     fp(t.nNear), fp(t.nbr), fp(t.conMat), fp(t.ipre), fp(t.npre), fp(t.og), fp(t.oh),
-                    0, fp(out), None, 0, None, None)
+                    0, fp(out), None, 0, None, None, None)
     dgE = sim.st["gE"].astype(np.float64) - t.st["gE"].astype(np.float64)
     dhI = sim.st["hI"].astype(np.float64) - t.st["hI"].astype(np.float64)
     dt = sp.dt

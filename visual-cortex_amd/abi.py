@@ -53,13 +53,17 @@ class V1Params(C.Structure):
         ("seed_lo", u32), ("seed_hi", u32), ("trial", u32),
         ("blockTrial", vp), ("trialV1Off", vp), ("trialSeed", vp),
         ("stepWG", u32), ("gatherTile", u32),
+        ("synFailFF", _arr(f32, VC_MAX_NTYPE)), ("synPerConFF", _arr(f32, VC_MAX_NTYPE)),
+        ("synFail", _arr(f32, VC_MAX_NTYPE * VC_MAX_NTYPE)),
+        ("synPerCon", _arr(f32, VC_MAX_NTYPE * VC_MAX_NTYPE)),
+        ("noisyDep", _arr(f32, VC_MAX_NTYPE)), ("tauNoise", f32), ("noiseOnTonic", u32),
     ]
 
 
 class V1State(C.Structure):
     _fields_ = [("v", vp), ("w", vp), ("tBack", vp), ("gFF", vp), ("hFF", vp),
                 ("gE", vp), ("hE", vp), ("gI", vp), ("hI", vp), ("tonicDep", vp),
-                ("gapS", vp), ("gap", vp)]
+                ("gapS", vp), ("gap", vp), ("lastNoise", vp)]
 
 
 class FFConn(C.Structure):
@@ -140,6 +144,8 @@ SIGNATURES = {
                   u32, vp],
     "vc_v1_rec_apply": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                         C.POINTER(Spikes)],
+    "vc_v1_rec_apply_at": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
+                           C.POINTER(Spikes), u32],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],
     "vc_v1_rec_distances": [vp, C.POINTER(V1Params), C.POINTER(RecConn), vp, vp],
     "vc_v1_rec_delay_table": [vp, C.c_uint64, vp, vp, C.c_float, C.c_float, vp],

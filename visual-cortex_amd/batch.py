@@ -169,6 +169,11 @@ def batch_networks(nets: list, device=None) -> N.Network:
                     sp0.noDelay, sp0.speedOfThought, nets[0].far_delayed()):
             raise ValueError("batched trials must share block size, dt, stimulus shape and "
                              "axonal-delay settings (noDelay, noFarDelay, speedOfThought)")
+        noise = ("noisyDep", "noiseOnTonic", "tau_noise", "synFailFF", "synPerConFF", "synFail",
+                 "synPerCon")
+        if any(getattr(s, k) != getattr(sp0, k) for k in noise):
+            raise ValueError("batched trials must share the noise and synaptic-failure "
+                             "parameters (" + ", ".join(noise) + ")")
     if device is None:
         c0 = nets[0].csr
         device = c0[0].device if c0 is not None and hasattr(c0[0], "device") else \

@@ -193,7 +193,28 @@ def spec_from_config(cfg: Config, base=None):
             kw[key] = _pairs(v, 2, ng)
     if cfg.has("bgRate"):
         kw["bgRate"] = cfg.float("bgRate")
-    for key in ("noisyDep", "synFailFF", "synFail", "spE0", "spI0"):
+    # noise and synaptic failure (src/patch.cu:206-207, 273-280, 647-661,
+    # 860-903): per-type lists take one value for every type; synFail and
+    # synPerCon are [pre type][post type], negative failure rates become 0
+    for key in ("noisyDep", "synFailFF", "synPerConFF"):
+        v = cfg.floats(key)
+        if v is not None:
+            if len(v) not in (1, 2):
+                raise ValueError(f"{key} needs 1 or nType = 2 values, got {len(v)}")
+            kw[key] = (v[0], v[0]) if len(v) == 1 else tuple(v)
+    for key in ("synFail", "synPerCon"):
+        v = cfg.floats(key)
+        if v is not None:
+            if len(v) != 4:
+                raise ValueError(f"{key} needs nType x nType = 4 values, got {len(v)}")
+            kw[key] = tuple(max(x, 0.0) for x in v) if key == "synFail" else tuple(v)
+    if any(x < 0 for x in kw.get("noisyDep", ())):
+        raise ValueError("noisyDep must be >= 0")
+    if cfg.has("noiseOnTonic"):
+        kw["noiseOnTonic"] = 1 if cfg.bool("noiseOnTonic") else 0
+    if cfg.has("tau_noise"):
+        kw["tau_noise"] = cfg.float("tau_noise")
+    for key in ("spE0", "spI0"):
         v = cfg.floats(key)
         if v is not None and any(x != 0 for x in v):
             raise NotImplementedError(f"{key} != 0 is not built (the configs of this path use 0)")

@@ -132,6 +132,35 @@ __device__ __forceinline__ void ld_tagged(const uint64_t *p, uint32_t tag, uint3
     for (int j = 0; j < N; ++j) out[j] = (uint32_t)v[j];
 }
 
+// Philox key of neuron n (block b) and its entity id: with batched trials the
+// trial's own key and the neuron's id within its trial (off = the trial's
+// first neuron), so a trial draws the same numbers batched as alone.
+struct V1Key {
+    uint32_t lo, hi, ent, off;
+};
+__device__ __forceinline__ V1Key v1_key(const vc_v1_params &p, uint32_t b, uint32_t n) {
+    V1Key k{p.seed_lo, p.seed_hi, n, 0u};
+    if (p.blockTrial != nullptr) {
+        const uint32_t tr = p.blockTrial[b];
+        k.lo = p.trialSeed[2 * tr];
+        k.hi = p.trialSeed[2 * tr + 1];
+        k.off = p.trialV1Off[tr];
+        k.ent = n - k.off;
+    }
+    return k;
+}
+
+// Synaptic failure: the strength f of a transmitting synapse times
+// 1 - p + sqrt(p (1 - p) / nSyn) * N(0,1) (src/coredynamics.cu:1476-1480,
+// 2766-2769), the normal drawn from (key, step, stream, pack).
+__device__ __forceinline__ float fail_factor(float f, float pf, float nSyn, const V1Key &k,
+                                             uint32_t step, uint32_t stream, uint32_t pack,
+                                             uint32_t trial) {
+    const float sd = __builtin_sqrtf(pf * (1.0f - pf) / nSyn);
+    const vc_u32x4 r = vc_draw4(k.lo, k.hi, k.ent, step, stream, pack, trial);
+    return f * (1.0f - pf + sd * vc_normal(r.v[0], r.v[1]));
+}
+
 // Adds one spike's increments (strength w) to the calling lane's sums in the
 // reference's order.
 __device__ __forceinline__ void add_spike(const vc_v1_params &p, float strength, uint32_t inh,
@@ -171,13 +200,15 @@ struct RecIn {
 // Adds the recurrent increments of a batch of K <= 64 spikes listed in
 // brow/bslot (this wavefront's LDS rows) to the calling lane's sums.  With
 // kRec the increments of spike k are the four tagged words at ri.words +
-// bsrc[k]; otherwise they are read from prev.og/oh.
-template <bool kCoh, bool kRec, int kG>
+// bsrc[k]; otherwise they are read from prev.og/oh.  kSt: synaptic failure
+// may be on (p.synFail); without it the failure draws are compiled out.
+template <bool kCoh, bool kRec, int kG, bool kSt>
 __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_conn &rec,
                                             const vc_spikes &prev, uint32_t K, uint32_t lane,
                                             uint32_t nSeg, uint32_t myseg, int t,
                                             const uint32_t *brow, const uint32_t *bslot,
                                             const uint32_t *bsrc, const RecIn &ri,
+                                            const V1Key &key, uint32_t step,
                                             float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
                                             float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
     // lane kk fetches the row-segment header and the increments of spike kk
@@ -217,6 +248,20 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
     };
     const float pE0 = p.pE[t * p.ngE + 0], pE1 = p.pE[t * p.ngE + (p.ngE > 1 ? 1 : 0)];
     const float pI0 = p.pI[t * p.ngI + 0], pI1 = p.pI[t * p.ngI + (p.ngI > 1 ? 1 : 0)];
+    // synaptic failure of the spike kk's synapse onto this lane's neuron
+    // (presynaptic type: E or I by the spike's flag), 1 when off
+    const float pfE = p.synFail[0 * p.nType + t], pfI = p.synFail[(p.nType - 1u) * p.nType + t];
+    const float nsE = p.synPerCon[0 * p.nType + t], nsI = p.synPerCon[(p.nType - 1u) * p.nType + t];
+    const uint32_t bs = nSeg * 64u;
+    auto failed = [&](float w, uint32_t kk) -> float {
+        if constexpr (!kSt) return w;
+        const uint32_t inh = vc_readlane_u32(isI, kk);
+        const float pf = inh ? pfI : pfE;
+        if (!(pf > 0.0f)) return w;
+        const uint32_t pre = (vc_readlane_u32(slot, kk) / bs) * bs + vc_readlane_u32(row, kk) % bs;
+        return fail_factor(w, pf, inh ? nsI : nsE, key, step, VC_STREAM_V1_FAIL, pre - key.off,
+                           p.trial);
+    };
     if constexpr (dense) {
         // a spike's strengths for this wavefront are one coalesced 256-byte
         // row segment; absent synapses hold +0 and add +0 to the sums
@@ -233,7 +278,8 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
 #pragma unroll
             for (int j = 0; j < kWD; ++j) {
                 const uint32_t kk = k0 + j;
-                if (kk < K)
+                if (kk < K && wv[j] > 0.0f) wv[j] = failed(wv[j], kk);
+                if (kk < K && wv[j] > 0.0f)
                     add_spike(p, wv[j], vc_readlane_u32(isI, kk), pE0, pE1, pI0, pI1,
                               vc_readlane_f32(og0, kk), vc_readlane_f32(oh0, kk),
                               vc_readlane_f32(og1, kk), vc_readlane_f32(oh1, kk), lgE, lhE, lgI,
@@ -258,7 +304,9 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
 #pragma unroll
         for (int j = 0; j < kW; ++j) {
             const uint32_t kk = k0 + j;
-            if (kk < K && ((vc_readlane_u64(rm, kk) >> lane) & 1ull))
+            const bool has = kk < K && ((vc_readlane_u64(rm, kk) >> lane) & 1ull);
+            if (has) wv[j] = failed(wv[j], kk);
+            if (has && wv[j] > 0.0f)
                 add_spike(p, wv[j], vc_readlane_u32(isI, kk), pE0, pE1, pI0, pI1,
                           vc_readlane_f32(og0, kk), vc_readlane_f32(oh0, kk),
                           vc_readlane_f32(og1, kk), vc_readlane_f32(oh1, kk), lgE, lhE, lgI, lhI);
@@ -282,12 +330,13 @@ struct GatherLds {
 // the listed spikes' increments for its own 64 posts, 64 spikes per batch.
 // Lists longer than kList are done in several passes.  Every thread of the
 // workgroup must call it.
-template <bool kCoh, bool kRec, int kG>
+template <bool kCoh, bool kRec, int kG, bool kSt>
 __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_conn &rec,
                                         const vc_spikes &prev, uint32_t b, uint32_t myseg,
                                         uint32_t nSeg, uint32_t nEnt, int t,
                                         const uint64_t *smk, const uint32_t *sbid,
-                                        const RecIn &ri, GatherLds &g,
+                                        const RecIn &ri, GatherLds &g, const V1Key &key,
+                                        uint32_t step,
                                         float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
                                         float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
@@ -351,8 +400,9 @@ __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_c
         const uint32_t K = (total - L0) < (uint32_t)kList ? (total - L0) : (uint32_t)kList;
         for (uint32_t k0 = 0; k0 < K; k0 += (uint32_t)kBatch) {
             const uint32_t kb = (K - k0) < (uint32_t)kBatch ? (K - k0) : (uint32_t)kBatch;
-            flush_batch<kCoh, kRec, kG>(p, rec, prev, kb, lane, nSeg, myseg, t, g.row + k0,
-                                    g.slot + k0, g.src + k0, ri, lgE, lhE, lgI, lhI);
+            flush_batch<kCoh, kRec, kG, kSt>(p, rec, prev, kb, lane, nSeg, myseg, t, g.row + k0,
+                                    g.slot + k0, g.src + k0, ri, key, step, lgE, lhE, lgI,
+                                    lhI);
         }
         __syncthreads();                    // the list is rewritten by the next pass
     }
@@ -498,11 +548,18 @@ struct BGIn {
 // whose cell spiked (sInfo >= 1) do work (reference: nsp_FF > 0).  Every
 // input of a lane goes through the one inlined copy of ff_apply in the loop
 // (no call, so no stack frame).
+// LGN synaptic failure of one neuron (pf = 0: off) and its draws' key.
+struct FailFF {
+    float pf, nSyn;
+    V1Key key;
+    uint32_t step, trial;
+};
+
 __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const FFShape &shBG,
                                           FFPre<true> &f, const vc_ff_conn &ff,
                                           const float *lgnSInfo, uint32_t *ffMask, uint32_t n,
                                           uint32_t nV1, float dt, float tBack, bool backing,
-                                          const BGIn &bg) {
+                                          const BGIn &bg, const FailFF &fl) {
     float evs[kEv];
     if (__ballot(f.bits != 0u) != 0ull) {
 #pragma unroll
@@ -534,6 +591,9 @@ __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const
             }
             nsp = (float)(uint32_t)floorf(sI);
             tsp = (sI - nsp) * dt;
+            if (fl.pf > 0.0f && nsp > 0.0f)
+                fv = fail_factor(fv, fl.pf, fl.nSyn, fl.key, fl.step, VC_STREAM_V1_FAILFF, j,
+                                 fl.trial);
 #pragma unroll
             for (int ig = 0; ig < VC_MAX_NG; ++ig) pm[ig] = shFF.p[ig];
         } else {
@@ -552,7 +612,7 @@ __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const
                                           FFPre<false> &f, const vc_ff_conn &ff,
                                           const float *lgnSInfo, uint32_t *, uint32_t n,
                                           uint32_t nV1, float dt, float tBack, bool backing,
-                                          const BGIn &bg) {
+                                          const BGIn &bg, const FailFF &fl) {
     uint32_t spk = 0;
 #pragma unroll
     for (int j = 0; j < kFF; ++j) spk |= (f.sv[j] >= 1.0f ? 1u : 0u) << j;
@@ -568,8 +628,10 @@ __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const
     for (;;) {
         float fv = 0.0f, sI = 0.0f, pm[VC_MAX_NG];
         bool isBG = false;
+        uint32_t slotj = 0;                              // the synapse's slot in the list
         if (spk != 0u) {
             j = (uint32_t)__builtin_ctz(spk);
+            slotj = j;
             spk &= spk - 1u;
 #pragma unroll
             for (int k = 0; k < kFF; ++k) {      // j is wave-uniform: scalar compares
@@ -580,6 +642,7 @@ __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const
             }
         } else if (i < f.m) {                            // lists longer than kFF
             const uint32_t lid = i * nV1 + n;
+            slotj = i;
             ++i;
             sI = lgnSInfo[ff.idLGN[lid]];
             fv = ff.sLGN[lid];
@@ -598,6 +661,9 @@ __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const
             if (!(sI >= 1.0f)) fv = 0.0f;                // this lane's cell did not spike
             nsp = (float)(uint32_t)floorf(sI);
             tsp = (sI - nsp) * dt;
+            if (fl.pf > 0.0f && fv > 0.0f)
+                fv = fail_factor(fv, fl.pf, fl.nSyn, fl.key, fl.step, VC_STREAM_V1_FAILFF, slotj,
+                                 fl.trial);
         }
 #pragma unroll
         for (int ig = 0; ig < VC_MAX_NG; ++ig) pm[ig] = isBG ? shBG.p[ig] : shFF.p[ig];
@@ -609,7 +675,7 @@ __device__ __forceinline__ void ff_inputs(FFAcc &acc, const FFShape &shFF, const
 // receptors at the end of the step (g1, h1) and, when backing up from
 // refractoriness (0 < tBack < dt), at tBack (g0, h0).  Depends on the
 // neuron's own state only, not on the other neurons' spikes.
-template <bool kEvents>
+template <bool kEvents, bool kSt>
 __device__ __forceinline__ FFAcc ff_stage(const vc_v1_params &p, const V1Const &k,
                                           const vc_ff_conn &ff, int t, uint32_t n, uint32_t b,
                                           uint32_t step, const NState &x, FFPre<kEvents> &f,
@@ -646,24 +712,54 @@ __device__ __forceinline__ FFAcc ff_stage(const vc_v1_params &p, const V1Const &
         shFF.dod[ig] = shBG.dod[ig] = p.dodFF[ig];
     }
     shFF.ng = shBG.ng = p.ngFF;
+    const V1Key key = v1_key(p, b, n);
     // background Poisson drive (extension): one more input after the LGN list
     BGIn bg{0.0f, 0.0f, 0.0f};
     if (p.bgStrength[t] > 0.0f) {
-        uint32_t klo = p.seed_lo, khi = p.seed_hi, ent = n;
-        if (p.blockTrial != nullptr) {
-            const uint32_t tr = p.blockTrial[b];
-            klo = p.trialSeed[2 * tr];
-            khi = p.trialSeed[2 * tr + 1];
-            ent = n - p.trialV1Off[tr];
-        }
-        const vc_u32x4 r = vc_draw4(klo, khi, ent, step, VC_STREAM_V1_BG, 0u, p.trial);
+        const vc_u32x4 r = vc_draw4(key.lo, key.hi, key.ent, step, VC_STREAM_V1_BG, 0u, p.trial);
         const float u = vc_u01_open0(r.v[0]);
         const float frac = vc_u01_open1(r.v[1]);
         const uint32_t nsp = vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
         if (nsp > 0u) bg = BGIn{p.bgStrength[t], (float)nsp, frac * dt};
     }
-    ff_inputs(acc, shFF, shBG, f, ff, lgnSInfo, ffMask, n, p.nV1, dt, tBack, backing, bg);
+    const FailFF fl{kSt ? p.synFailFF[t] : 0.0f, p.synPerConFF[t], key, step, p.trial};
+    ff_inputs(acc, shFF, shBG, f, ff, lgnSInfo, ffMask, n, p.nV1, dt, tBack, backing, bg, fl);
     return acc;
+}
+
+// Voltage noise of one neuron for a step (src/coredynamics.cu:1424-1441):
+// sigma = sqrt(2 noisyDep (vT - vR) r dt) N(0,1), r = tonicDep when
+// noiseOnTonic; Ornstein-Uhlenbeck when tauNoise > 0 (`last` is the previous
+// step's value and becomes this step's).  Returns the terms the RK2 slopes
+// take, n0 and n1 (white: sigma/dt both; OU: last, new), and in *dep the
+// value the reference adds to depC in its dep output (sigma, or the new OU
+// value).  All 0 without noise.
+__device__ __forceinline__ void noise_terms(const vc_v1_params &p, int t, const V1Key &key,
+                                            uint32_t step, float tonic, float &last, float &n0,
+                                            float &n1, float &dep) {
+    const float dt = p.dt, tau = p.tauNoise;
+    const float ratio = p.noiseOnTonic ? tonic : 1.0f;
+    float noise1 = __builtin_sqrtf(2.0f * p.noisyDep[t] * (p.vT[t] - p.vR[t]) * ratio * dt);
+    const float noise0 = tau > 0.0f ? last : 0.0f;
+    if (noise1 > 0.0f) {
+        const vc_u32x4 r = vc_draw4(key.lo, key.hi, key.ent, step, VC_STREAM_V1_NOISE, 0u,
+                                    p.trial);
+        noise1 *= vc_normal(r.v[0], r.v[1]);
+        if (tau > 0.0f) {
+            noise1 /= tau;
+            const float hk1 = -noise0 / tau;
+            const float hk2 = -(noise0 + dt * hk1 + noise1) / tau;
+            noise1 += noise0 + dt * (hk1 + hk2) / 2.0f;
+            last = noise1;
+        }
+    }
+    dep = noise1;
+    if (tau > 0.0f) {
+        n0 = noise0;
+        n1 = noise1;
+    } else {
+        n0 = n1 = noise1 / dt;
+    }
 }
 
 // Stage d of one neuron: x holds the state at the start of the step with the
@@ -673,7 +769,7 @@ __device__ __forceinline__ FFAcc ff_stage(const vc_v1_params &p, const V1Const &
 // without gap junctions).  Returns sInfo.
 __device__ __forceinline__ float membrane_stage(const vc_v1_params &p, const V1Const &k, int t,
                                                 float depC, NState &x, const FFAcc &acc,
-                                                float gapS, float gap) {
+                                                float gapS, float gap, float n0, float n1) {
     const float dt = p.dt;
     const float vR = p.vR[t], vThres = p.vThres[t], gL = p.gL[t], C = p.C[t];
     const float tRef = p.tRef[t], vT = p.vT[t], deltaT = p.deltaT[t];
@@ -754,16 +850,16 @@ __device__ __forceinline__ float membrane_stage(const vc_v1_params &p, const V1C
         const float dTgL = deltaT * gL;
         float fk1, fk2;
         if (deltaT > 0.0f)
-            fk1 = (-a0 * v0 + b0 + dTgL * vc_expf((v0 - vT) / deltaT) - w0) / C;
+            fk1 = (-a0 * v0 + b0 + dTgL * vc_expf((v0 - vT) / deltaT) - w0) / C + n0;
         else
-            fk1 = (-a0 * v0 + b0 - w0) / C;
+            fk1 = (-a0 * v0 + b0 - w0) / C + n0;
         const float gk1 = (a * (v0 - VC_VL) - w0) / tau_w;
         const float v1 = v0 + new_dt * fk1;
         const float w1 = w0 + new_dt * gk1;
         if (deltaT > 0.0f)
-            fk2 = (-a1 * v1 + b1 + dTgL * vc_expf((v1 - vT) / deltaT) - w1) / C;
+            fk2 = (-a1 * v1 + b1 + dTgL * vc_expf((v1 - vT) / deltaT) - w1) / C + n1;
         else
-            fk2 = (-a1 * v1 + b1 - w1) / C;
+            fk2 = (-a1 * v1 + b1 - w1) / C + n1;
         const float gk2 = (a * (v1 - VC_VL) - w1) / tau_w;
         w = w0 + new_dt * (gk1 + gk2) / 2.0f;
         v = v0 + new_dt * (fk1 + fk2) / 2.0f;
@@ -853,7 +949,9 @@ __device__ __forceinline__ void compact(const vc_v1_params &p, const vc_spikes &
 }
 
 // ------------------------------------------------------ one-step kernel --
-template <bool kUpdate, bool kEvents, int kG>
+// kSt: noise or synaptic failure may be on (their draws are compiled out of
+// the other instances).
+template <bool kUpdate, bool kEvents, int kG, bool kSt>
 __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v1_state s,
                                                  vc_ff_conn ff, vc_rec_conn rec,
                                                  vc_spikes prev, vc_spikes next,
@@ -895,8 +993,9 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
     float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
     __syncthreads();
     const RecIn noRec{nullptr, 0u, 0u, 0u, nullptr};
-    rec_gather<false, false, kG>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, noRec, glds,
-                             lgE, lhE, lgI, lhI);
+    // (the failure draws of the spikes of step - 1 are keyed by that step)
+    rec_gather<false, false, kG, kSt>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, noRec, glds,
+                                 v1_key(p, b, n), step - 1u, lgE, lhE, lgI, lhI);
     // conductances at the end of the previous step (reference: after recal_G)
 #pragma unroll
     for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
@@ -951,16 +1050,23 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         return;
     }
     load_rest(p, s, n, x);
-    const float depC = dep_current(p, t, s.tonicDep[n]);
-    if (out_dep) out_dep[n] = depC;
-    const FFAcc acc = ff_stage<kEvents>(p, k, ff, t, n, b, step, x, f, lgnSInfo, ffMask);
+    const float tonic = s.tonicDep[n];
+    const float depC = dep_current(p, t, tonic);
+    float n0 = 0.0f, n1 = 0.0f, dn = 0.0f;
+    if constexpr (kSt) {
+        float last = s.lastNoise ? s.lastNoise[n] : 0.0f;
+        noise_terms(p, t, v1_key(p, b, n), step, tonic, last, n0, n1, dn);
+        if (s.lastNoise) s.lastNoise[n] = last;
+    }
+    if (out_dep) out_dep[n] = depC + dn;
+    const FFAcc acc = ff_stage<kEvents, kSt>(p, k, ff, t, n, b, step, x, f, lgnSInfo, ffMask);
     float gapS = 0.0f, gap = 0.0f;           // inhibitory neurons' gap-junction terms
     if (local >= p.nE) {
         const uint32_t gi = b * (p.blockSize - p.nE) + (local - p.nE);
         if (s.gapS) gapS = s.gapS[gi];
         if (s.gap) gap = s.gap[gi];
     }
-    const float sInfo = membrane_stage(p, k, t, depC, x, acc, gapS, gap);
+    const float sInfo = membrane_stage(p, k, t, depC, x, acc, gapS, gap, n0, n1);
     store_state(p, s, n, x);
     out_sInfo[n] = sInfo;
     const RecOut noOut{nullptr, 0u, 0u, 0u};
@@ -1006,8 +1112,8 @@ __global__ __launch_bounds__(256) void k_v1_gap(vc_v1_params p, vc_rec_conn rec,
 // earlier launches never match.  The vc_spikes buffers sa/sb receive every
 // step's spikes as vc_v1_run leaves them.  Every wait is bounded: on a
 // time-out *status is set and the kernel runs to its end (results invalid,
-// no hang).  The grid must fit on the device at once.
-template <bool kEvents, int kG>
+// no hang).  The grid must fit on the device at once.  kSt as in k_v1_step.
+template <bool kEvents, int kG, bool kSt>
 __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_v1_state s,
                                                   vc_ff_conn ff, vc_rec_conn rec,
                                                   vc_spikes sa, vc_spikes sb,
@@ -1044,7 +1150,10 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
     NState x;
     load_cortical(p, s, n, x);
     load_rest(p, s, n, x);
-    const float depC = dep_current(p, t, s.tonicDep[n]);
+    const float tonic = s.tonicDep[n];
+    const float depC = dep_current(p, t, tonic);
+    const V1Key key = v1_key(p, b, n);
+    float last = kSt && s.lastNoise ? s.lastNoise[n] : 0.0f;      // OU noise state
     // prologue: the spikes of the step before it0 (sa) as records of step -1
     {
         const uint32_t tile = n >> 6;
@@ -1077,7 +1186,7 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
         FFPre<kEvents> f;
         ff_trip1(f, ff, fm, n, nV1);
         ff_trip2(f, ff, lgn, n, nV1);
-        const FFAcc acc = ff_stage<kEvents>(p, k, ff, t, n, b, it0 + q, x, f, lgn, fm);
+        const FFAcc acc = ff_stage<kEvents, kSt>(p, k, ff, t, n, b, it0 + q, x, f, lgn, fm);
         // the neighbours' spike masks of the previous step (tag q+1)
         const RecIn ri{words, rec_of(q & 1u), ovf_of(q & 1u), (epoch << 16) + q + 1u, status};
         __syncthreads();                      // every wavefront is done with smk
@@ -1091,8 +1200,8 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
         float lgE[VC_MAX_NG] = {0.f, 0.f}, lhE[VC_MAX_NG] = {0.f, 0.f};
         float lgI[VC_MAX_NG] = {0.f, 0.f}, lhI[VC_MAX_NG] = {0.f, 0.f};
         __syncthreads();
-        rec_gather<true, true, kG>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, ri, glds, lgE,
-                               lhE, lgI, lhI);
+        rec_gather<true, true, kG, kSt>(p, rec, prev, b, myseg, nSeg, nEnt, t, smk, sbid, ri, glds,
+                                   key, it0 + q - 1u, lgE, lhE, lgI, lhI);
 #pragma unroll
         for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
             x.gE[ig] = x.gE[ig] + lgE[ig];
@@ -1100,13 +1209,16 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
             x.gI[ig] = x.gI[ig] + lgI[ig];
             x.hI[ig] = x.hI[ig] + lhI[ig];
         }
-        const float sInfo = membrane_stage(p, k, t, depC, x, acc, 0.0f, 0.0f);
+        float n0 = 0.0f, n1 = 0.0f, dn;
+        if constexpr (kSt) noise_terms(p, t, key, it0 + q, tonic, last, n0, n1, dn);
+        const float sInfo = membrane_stage(p, k, t, depC, x, acc, 0.0f, 0.0f, n0, n1);
         out_sInfo[q * outStride + n] = sInfo;
         const RecOut ro{words, rec_of((q + 1u) & 1u), ovf_of((q + 1u) & 1u),
                         (epoch << 16) + q + 2u};
         compact<true>(p, next, ro, n, local, lane, sInfo);
     }
     store_state(p, s, n, x);
+    if (kSt && s.lastNoise) s.lastNoise[n] = last;
 }
 
 __global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v) {
@@ -1126,6 +1238,16 @@ __global__ void k_clear_masks(uint64_t *mask, uint32_t nTiles) {
     if (i < nTiles) mask[i] = 0ull;
 }
 
+// Voltage noise or synaptic failure on: the kernels' instances with the draws.
+bool has_noise_or_failure(const vc_v1_params *p) {
+    for (uint32_t t = 0; t < p->nType; ++t) {
+        if (p->noisyDep[t] > 0.0f || p->synFailFF[t] > 0.0f) return true;
+        for (uint32_t u = 0; u < p->nType; ++u)
+            if (p->synFail[t * p->nType + u] > 0.0f) return true;
+    }
+    return false;
+}
+
 // Launch of the one-step kernel with gather variant g (0 = dense rows, else
 // the masked-CSR tile) and wg threads per workgroup.
 template <bool kUpdate, bool kEvents>
@@ -1135,14 +1257,24 @@ void launch_step(uint32_t g, uint32_t wg, hipStream_t st, const vc_v1_params &p,
                  const float *lgn, uint32_t *fm, uint32_t step, float *o_sInfo, float *o_dep,
                  float *o_g, float *o_snap) {
     const dim3 grid(p.nV1 / wg), block(wg);
-#define VC_STEP(G)                                                                              \
-    hipLaunchKernelGGL((k_v1_step<kUpdate, kEvents, G>), grid, block, 0, st, p, k, s, ff, rec,  \
-                       prev, next, lgn, fm, step, o_sInfo, o_dep, o_g, o_snap)
-    switch (g) {
-    case 0: VC_STEP(0); break;
-    case 4: VC_STEP(4); break;
-    case 16: VC_STEP(16); break;
-    default: VC_STEP(8); break;
+    const bool sto = has_noise_or_failure(&p);
+#define VC_STEP(G, S)                                                                           \
+    hipLaunchKernelGGL((k_v1_step<kUpdate, kEvents, G, S>), grid, block, 0, st, p, k, s, ff,    \
+                       rec, prev, next, lgn, fm, step, o_sInfo, o_dep, o_g, o_snap)
+    if (sto) {
+        switch (g) {
+        case 0: VC_STEP(0, true); break;
+        case 4: VC_STEP(4, true); break;
+        case 16: VC_STEP(16, true); break;
+        default: VC_STEP(8, true); break;
+        }
+    } else {
+        switch (g) {
+        case 0: VC_STEP(0, false); break;
+        case 4: VC_STEP(4, false); break;
+        case 16: VC_STEP(16, false); break;
+        default: VC_STEP(8, false); break;
+        }
     }
 #undef VC_STEP
 }
@@ -1194,6 +1326,21 @@ int check_params(const vc_v1_params *p, const vc_rec_conn *rec) {
         }
         if (p->deltaT[t] < 0.0f) { vc_set_error("deltaT < 0"); return VC_E_INVALID; }
     }
+    for (uint32_t t = 0; t < p->nType; ++t) {
+        if (!(p->synFailFF[t] >= 0.0f && p->synFailFF[t] <= 1.0f) ||
+            (p->synFailFF[t] > 0.0f && !(p->synPerConFF[t] > 0.0f)) || !(p->noisyDep[t] >= 0.0f)) {
+            vc_set_error("type %u: synFailFF in [0, 1] with synPerConFF > 0, noisyDep >= 0", t);
+            return VC_E_INVALID;
+        }
+        for (uint32_t u = 0; u < p->nType; ++u) {
+            const float f = p->synFail[t * p->nType + u];
+            if (!(f >= 0.0f && f <= 1.0f) || (f > 0.0f && !(p->synPerCon[t * p->nType + u] > 0.0f))) {
+                vc_set_error("synFail[%u][%u] must be in [0, 1] with synPerCon > 0", t, u);
+                return VC_E_INVALID;
+            }
+        }
+    }
+    if (!(p->tauNoise >= 0.0f)) { vc_set_error("tauNoise < 0"); return VC_E_INVALID; }
     if ((uint64_t)p->nblock * (rec ? rec->maxNear : 1) * p->blockSize >= (1ull << 32)) {
         vc_set_error("nblock * maxNear * blockSize must be < 2^32");
         return VC_E_UNSUPPORTED;
@@ -1209,6 +1356,17 @@ int check_params(const vc_v1_params *p, const vc_rec_conn *rec) {
 }
 
 
+
+// The OU noise needs its state array.
+int noise_ok(const vc_v1_params *p, const vc_v1_state *s, const char *who) {
+    bool noisy = false;
+    for (uint32_t t = 0; t < p->nType; ++t) noisy = noisy || p->noisyDep[t] > 0.0f;
+    if (noisy && p->tauNoise > 0.0f && !s->lastNoise) {
+        vc_set_error("%s: Ornstein-Uhlenbeck noise (tauNoise > 0) needs s->lastNoise", who);
+        return VC_E_INVALID;
+    }
+    return VC_OK;
+}
 
 // Gap junctions need s->gap to write to; the ELL arrays come together.
 int gap_ok(const vc_v1_params *p, const vc_v1_state *s, const vc_rec_conn *rec,
@@ -1258,6 +1416,7 @@ extern "C" int vc_v1_step(void *stream, const vc_v1_params *p, const vc_v1_state
         vc_set_error("vc_v1_step: null argument");
         return VC_E_INVALID;
     }
+    if ((rc = noise_ok(p, s, "vc_v1_step"))) return rc;
     if (prev->mask == next->mask) {
         vc_set_error("vc_v1_step: prev and next spike buffers must differ");
         return VC_E_INVALID;
@@ -1290,8 +1449,9 @@ extern "C" int vc_v1_step_snapshot(void *stream, const vc_v1_params *p, const vc
         vc_set_error("vc_v1_step_snapshot: null argument");
         return VC_E_INVALID;
     }
-    if (has_gap(s, rec)) {
-        vc_set_error("vc_v1_step_snapshot: the adjoint's forward has no gap junctions");
+    if (has_gap(s, rec) || has_noise_or_failure(p)) {
+        vc_set_error("vc_v1_step_snapshot: the adjoint's forward has no gap junctions, noise "
+                     "or synaptic failure");
         return VC_E_UNSUPPORTED;
     }
     if (prev->mask == next->mask) {
@@ -1335,6 +1495,7 @@ extern "C" int vc_v1_run(void *stream, const vc_v1_params *p, const vc_v1_state 
         return VC_E_INVALID;
     }
     if ((rc = gap_ok(p, s, rec, "vc_v1_run"))) return rc;
+    if ((rc = noise_ok(p, s, "vc_v1_run"))) return rc;
     V1Const k = make_const(p);
     uint32_t g, wg;
     if ((rc = step_shape(p, rec, &g, &wg))) return rc;
@@ -1402,6 +1563,7 @@ int run_persistent(const char *who, void *stream, const vc_v1_params *p, const v
                      who);
         return VC_E_UNSUPPORTED;
     }
+    if ((rc = noise_ok(p, s, who))) return rc;
     if (epoch == 0 || epoch > 0xffffu || nsteps > 0xfffdu) {
         vc_set_error("%s: epoch %u (1..65535) or %u steps (<= 65533) out of range", who, epoch,
                      nsteps);
@@ -1412,10 +1574,15 @@ int run_persistent(const char *who, void *stream, const vc_v1_params *p, const v
     const uint32_t grid = p->nV1 / wg;
     constexpr int kT = (int)kTileMulti;
     const bool dense = rec->wDense != nullptr;
-    const void *fn = ffMask ? (dense ? (const void *)k_v1_multi<true, 0>
-                                     : (const void *)k_v1_multi<true, kT>)
-                            : (dense ? (const void *)k_v1_multi<false, 0>
-                                     : (const void *)k_v1_multi<false, kT>);
+    const bool sto = has_noise_or_failure(p);
+    const void *fn = sto ? (ffMask ? (dense ? (const void *)k_v1_multi<true, 0, true>
+                                            : (const void *)k_v1_multi<true, kT, true>)
+                                   : (dense ? (const void *)k_v1_multi<false, 0, true>
+                                            : (const void *)k_v1_multi<false, kT, true>))
+                         : (ffMask ? (dense ? (const void *)k_v1_multi<true, 0, false>
+                                            : (const void *)k_v1_multi<true, kT, false>)
+                                   : (dense ? (const void *)k_v1_multi<false, 0, false>
+                                            : (const void *)k_v1_multi<false, kT, false>));
     int dev = 0, ncu = 0, per = 0;
     VC_HIP_CHECK(hipGetDevice(&dev));
     VC_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1447,14 +1614,22 @@ int run_persistent(const char *who, void *stream, const vc_v1_params *p, const v
     // bounded, so kernels of other streams can at most delay some of them.
     // (A cooperative launch would also serialise the host with the device.)
     const V1Const k = make_const(p);
-#define VC_MULTI(E, G)                                                                          \
-    hipLaunchKernelGGL((k_v1_multi<E, G>), dim3(grid), dim3(wg), 0, st, *p, k, *s, *ff, *rec,    \
+#define VC_MULTI(E, G, S)                                                                       \
+    hipLaunchKernelGGL((k_v1_multi<E, G, S>), dim3(grid), dim3(wg), 0, st, *p, k, *s, *ff, *rec, \
                        spk2[cur], spk2[1u - cur], lgnSInfo, lgnStride, ffMask, maskStride, it0,  \
                        nsteps, out_sInfo, outStride, words, epoch, status)
-    if (ffMask) {
-        if (dense) VC_MULTI(true, 0); else VC_MULTI(true, kT);
+    if (sto) {
+        if (ffMask) {
+            if (dense) VC_MULTI(true, 0, true); else VC_MULTI(true, kT, true);
+        } else {
+            if (dense) VC_MULTI(false, 0, true); else VC_MULTI(false, kT, true);
+        }
     } else {
-        if (dense) VC_MULTI(false, 0); else VC_MULTI(false, kT);
+        if (ffMask) {
+            if (dense) VC_MULTI(true, 0, false); else VC_MULTI(true, kT, false);
+        } else {
+            if (dense) VC_MULTI(false, 0, false); else VC_MULTI(false, kT, false);
+        }
     }
 #undef VC_MULTI
     VC_HIP_CHECK(hipGetLastError());
@@ -1538,8 +1713,24 @@ extern "C" int vc_v1_run_persistent_noalloc(void *stream, const vc_v1_params *p,
     return rc;
 }
 
+static bool any_syn_fail(const vc_v1_params *p) {
+    for (uint32_t i = 0; i < p->nType * p->nType; ++i)
+        if (p->synFail[i] > 0.0f) return true;
+    return false;
+}
+
 extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                                const vc_rec_conn *rec, const vc_spikes *prev) {
+    if (p && any_syn_fail(p)) {
+        vc_set_error("vc_v1_rec_apply: with synaptic failure use vc_v1_rec_apply_at (the "
+                     "failure draws are keyed by the spikes' step)");
+        return VC_E_UNSUPPORTED;
+    }
+    return vc_v1_rec_apply_at(stream, p, s, rec, prev, 0xffffffffu);
+}
+
+extern "C" int vc_v1_rec_apply_at(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                                  const vc_rec_conn *rec, const vc_spikes *prev, uint32_t step) {
     int rc = check_params(p, rec);
     if (rc) return rc;
     if (!s || !rec || !prev) {
@@ -1550,14 +1741,11 @@ extern "C" int vc_v1_rec_apply(void *stream, const vc_v1_params *p, const vc_v1_
     uint32_t wg = wg_size(p->blockSize);
     vc_ff_conn ff{};
     vc_spikes nx{};
-    if (rec->wDense)
-        hipLaunchKernelGGL((k_v1_step<false, false, 0>), dim3(p->nV1 / wg), dim3(wg), 0,
-                           (hipStream_t)stream, *p, k, *s, ff, *rec, *prev, nx, nullptr, nullptr,
-                           0u, nullptr, nullptr, nullptr, nullptr);
-    else
-        hipLaunchKernelGGL((k_v1_step<false, false, (int)kTileMulti>), dim3(p->nV1 / wg),
-                           dim3(wg), 0, (hipStream_t)stream, *p, k, *s, ff, *rec, *prev, nx,
-                           nullptr, nullptr, 0u, nullptr, nullptr, nullptr, nullptr);
+    // (the gather of a k_v1_step launched for step s + 1 keys its failure
+    // draws by s, the step of the spikes)
+    launch_step<false, false>(rec->wDense ? 0u : kTileMulti, wg, (hipStream_t)stream, *p, k, *s,
+                              ff, *rec, *prev, nx, nullptr, nullptr, step + 1u, nullptr, nullptr,
+                              nullptr, nullptr);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
 }

@@ -582,6 +582,15 @@ extern "C" int vc_v1_adjoint_run(void *stream, const vc_v1_params *p, const vc_v
         return VC_E_UNSUPPORTED;
     }
     for (uint32_t t = 0; t < p->nType; ++t) {
+        bool off = p->noisyDep[t] == 0.0f && p->synFailFF[t] == 0.0f;
+        for (uint32_t u = 0; u < p->nType; ++u) off = off && p->synFail[t * p->nType + u] == 0.0f;
+        if (!off) {
+            vc_set_error("vc_v1_adjoint_run: voltage noise and synaptic failure are not "
+                         "differentiated");
+            return VC_E_UNSUPPORTED;
+        }
+    }
+    for (uint32_t t = 0; t < p->nType; ++t) {
         if (p->a[t] != 0.0f || p->deltaT[t] != 0.0f) {
             vc_set_error("vc_v1_adjoint_run: type %u has a = %g, deltaT = %g (conductance LIF "
                          "needs both 0)", t, p->a[t], p->deltaT[t]);

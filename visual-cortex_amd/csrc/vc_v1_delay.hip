@@ -558,6 +558,18 @@ bool rec_ok(const vc_v1_params *p, const vc_rec_conn *rec, const char *who) {
     return true;
 }
 
+// The delayed recurrent kernels (k_rec_delay, k_far_eval) deliver every
+// spike whole: they have no synaptic-failure draw.
+bool rec_failure_refused(const vc_v1_params *p, const char *who) {
+    for (uint32_t i = 0; i < VC_MAX_NTYPE * VC_MAX_NTYPE; ++i)
+        if (p->synFail[i] > 0.0f) {
+            vc_set_error("%s: synaptic failure between V1 neurons (synFail) is not "
+                         "implemented with conduction delays", who);
+            return true;
+        }
+    return false;
+}
+
 }  // namespace
 
 extern "C" int vc_v1_rec_distances(void *stream, const vc_v1_params *p, const vc_rec_conn *rec,
@@ -612,6 +624,7 @@ static int rec_delay_launch(void *stream, const vc_v1_params *p, const vc_v1_sta
                             const uint64_t *trainMask, uint32_t depth, uint32_t cur,
                             uint32_t window, uint32_t *status, float *copyOut) {
     if (!rec_ok(p, rec, "vc_v1_rec_delay")) return VC_E_INVALID;
+    if (rec_failure_refused(p, "vc_v1_rec_delay")) return VC_E_UNSUPPORTED;
     if (!s || !s->gE || !s->hE || !s->gI || !s->hI || !syn || !train || !trainMask) {
         vc_set_error("vc_v1_rec_delay: null argument");
         return VC_E_INVALID;
@@ -662,6 +675,7 @@ extern "C" int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_st
         return VC_E_INVALID;
     }
     if (far->maxFar == 0 || p->nV1 == 0) return VC_OK;
+    if (rec_failure_refused(p, "vc_v1_rec_far")) return VC_E_UNSUPPORTED;
     if (!far->nFar || !far->pre || !far->w || !far->dist || !far->preMask || !far->outOff ||
         !far->outPost || !far->outJ || !far->hit || !train || !trainMask) {
         vc_set_error("vc_v1_rec_far: null argument (every vc_far_conn array, train and "
@@ -713,6 +727,7 @@ extern "C" int vc_v1_delay_run(void *stream, const vc_v1_params *p, const vc_v1_
         vc_set_error("vc_v1_delay_run: need slot0 (%u) < depth (%u)", slot0, depth);
         return VC_E_INVALID;
     }
+    if (rec_failure_refused(p, "vc_v1_delay_run")) return VC_E_UNSUPPORTED;
     const uint32_t nTiles = p->nV1 / 64u;
     const vc_spikes none = {(uint64_t *)zeroMask, og, oh};
     for (uint32_t i = 0; i < nsteps; ++i) {

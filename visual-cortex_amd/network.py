@@ -98,6 +98,20 @@ class ModelSpec:
     InhGap: int = 0
     nGap: int = 0
     sGap: float = 0.2
+    # voltage noise (src/coredynamics.cu:1424-1445): sigma = sqrt(2 noisyDep
+    # (vT - vR) r dt), r = the neuron's tonicDep when noiseOnTonic (else 1);
+    # white when tau_noise is 0, Ornstein-Uhlenbeck with time scale
+    # tau_noise (ms) otherwise
+    noisyDep: tuple = (0.0, 0.0)
+    noiseOnTonic: int = 1
+    tau_noise: float = 0.0
+    # synaptic failure (src/coredynamics.cu:1476-1480, 2766-2769): rate per
+    # post type for LGN synapses, per [pre type][post type] for V1 ones,
+    # with the synapses per connection that set the spread of the factor
+    synFailFF: tuple = (0.0, 0.0)
+    synPerConFF: tuple = (1.0, 1.0)
+    synFail: tuple = (0.0, 0.0, 0.0, 0.0)
+    synPerCon: tuple = (1.0, 1.0, 1.0, 1.0)
     # background Poisson drive (extension)
     bgRate: float = 2000.0        # Hz
     bgStrength: tuple = (0.08, 0.12)

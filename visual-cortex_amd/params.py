@@ -33,6 +33,14 @@ def v1_params_struct(net: Network) -> abi.V1Params:
         p.bgLambda[t] = float(lam)
         p.bgP0[t] = float(np.float32(math.exp(-float(lam))))
         p.bgStrength[t] = sp.bgStrength[t]
+        p.noisyDep[t] = sp.noisyDep[t]
+        p.synFailFF[t] = sp.synFailFF[t]
+        p.synPerConFF[t] = sp.synPerConFF[t]
+        for u in range(2):
+            p.synFail[t * 2 + u] = sp.synFail[t * 2 + u]
+            p.synPerCon[t * 2 + u] = sp.synPerCon[t * 2 + u]
+    p.tauNoise = sp.tau_noise
+    p.noiseOnTonic = 1 if sp.noiseOnTonic else 0
 
     def shape(prefix, rise, decay):
         r = getattr(p, "rise" + prefix)

@@ -55,9 +55,15 @@ class GpuSim:
         self.gap = torch.zeros(mI, dtype=torch.float32, device=d) if self.gap_on else None
         if (self.gapS is not None or self.gap_on) and getattr(net, "batch", None) is not None:
             raise NotImplementedError("gap junctions with batched trials are not built")
+        # the Ornstein-Uhlenbeck noise state, zeros at the start
+        # (src/patch.cu:4086-4088)
+        self.lastNoise = torch.zeros(sp.nV1, dtype=torch.float32, device=d)
+        if any(x > 0 for x in sp.synFail) and (sp.noDelay == 0 or net.n_far() > 0):
+            raise NotImplementedError("synaptic failure between V1 neurons (synFail) with "
+                                      "conduction delays or far connections is not built")
         self.s = abi.V1State(*(ptr(self.state[k]) for k in
                                ("v", "w", "tBack", "gFF", "hFF", "gE", "hE", "gI", "hI")),
-                             ptr(self.tonic), ptr(self.gapS), ptr(self.gap))
+                             ptr(self.tonic), ptr(self.gapS), ptr(self.gap), ptr(self.lastNoise))
         # ---- FF synapses
         self.nLGN_of = T(net.nLGN_of.astype(np.uint32).view(np.int32))
         self.sLGN = T(net.sLGN, np.float32)
@@ -287,8 +293,8 @@ class GpuSim:
                                   C.byref(self.ring_spk[slot]),
                                   ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, mptr, it,
                                   ptr(self.train[slot]), ptr(dep), ptr(gprev)), "vc_v1_step")
-        check(self.lib.vc_v1_rec_apply(h, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
-                                       C.byref(self.ring_spk[slot])), "vc_v1_rec_apply")
+        check(self.lib.vc_v1_rec_apply_at(h, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
+                                          C.byref(self.ring_spk[slot]), it), "vc_v1_rec_apply_at")
         check(self.lib.vc_v1_rec_far(h, C.byref(self.p), C.byref(self.s), C.byref(self.far),
                                      ptr(self.train), ptr(self.trainMask), self.depth, slot,
                                      self.speed, ptr(self.status)), "vc_v1_rec_far")
@@ -776,8 +782,11 @@ class GpuSim:
         recurrent input."""
         if self.delay or self.far_only:
             return
-        check(self.lib.vc_v1_rec_apply(None, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
-                                       C.byref(self.spk[self.cur])), "vc_v1_rec_apply")
+        # (the pending spikes are those of step it - 1, which keys their
+        # synaptic-failure draws)
+        check(self.lib.vc_v1_rec_apply_at(None, C.byref(self.p), C.byref(self.s),
+                                          C.byref(self.rec), C.byref(self.spk[self.cur]),
+                                          (self.it - 1) & 0xffffffff), "vc_v1_rec_apply_at")
         # the increments are now in the state: clear the pending list
         check(self.lib.vc_spikes_clear(None, self.net.spec.nV1, C.byref(self.spk[self.cur])),
               "vc_spikes_clear")

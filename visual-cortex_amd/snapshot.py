@@ -19,7 +19,11 @@ A generator slot is 48 bytes, the size of the reference's MRG32k3a state.
 The random numbers here come from counter-based Philox streams (the stream
 id, the entity and the step are the counter), so a slot holds that stream's
 identity instead of a state: u32 'PHLX' tag, u32 entity, u32 key low, u32 key
-high, u32 stream id (0xffffffff: none), zeros.  The frame-ring fields are
+high, u32 stream id (0xffffffff: none), zeros -- except that the rNoisy slot
+of a neuron (the voltage-noise stream) holds in its sixth word the float32
+bits of the neuron's Ornstein-Uhlenbeck noise value, the only state that
+stream carries from step to step (the reference keeps it in last_noise and
+does not write it).  The frame-ring fields are
 this simulator's: currentFrame frames have entered, iFrameHead =
 currentFrame mod maxFrame, the parvo phase fields and iStatus are 0.  With no
 axonal delays trainDepth is 1 and the spike train is the last step's output;
@@ -50,18 +54,32 @@ TAG = 0x584C4850                  # 'PHLX'
 NO_STREAM = 0xFFFFFFFF
 STREAM_LGN_SPIKE = 1              # the Philox stream ids of the kernels
 STREAM_V1_BG = 3
+STREAM_V1_NOISE = 7
 HEADER_U32 = 11
 
 
-def generator_slots(n: int, key_lo: int, key_hi: int, stream: int) -> bytes:
-    """n 48-byte generator slots naming Philox stream `stream` of entities 0..n-1."""
+def generator_slots(n: int, key_lo: int, key_hi: int, stream: int, carried=None) -> bytes:
+    """n 48-byte generator slots naming Philox stream `stream` of entities
+    0..n-1; `carried` (float32[n]) is the value the stream carries between
+    steps, in the sixth word."""
     a = np.zeros((n, GEN_BYTES // 4), np.uint32)
     a[:, 0] = TAG
     a[:, 1] = np.arange(n, dtype=np.uint32)
     a[:, 2] = key_lo
     a[:, 3] = key_hi
     a[:, 4] = stream
+    if carried is not None:
+        a[:, 5] = np.ascontiguousarray(carried, dtype=np.float32).view(np.uint32)
     return a.tobytes()
+
+
+def carried_values(slots: bytes, n: int) -> np.ndarray:
+    """The float32 values the generator slots carry (zeros for slots that are
+    not this format's, such as the reference's MRG32k3a states)."""
+    a = np.frombuffer(slots, np.uint32, n * (GEN_BYTES // 4)).reshape(n, GEN_BYTES // 4)
+    out = a[:, 5].copy().view(np.float32)
+    out[a[:, 0] != TAG] = 0.0
+    return out
 
 
 def state_block_floats(nV1: int, iModel: int, ngFF: int, ngE: int, ngI: int, mI: int) -> int:
@@ -77,8 +95,8 @@ def file_size(nV1, nLGN, iModel, ngFF, ngE, ngI, mI, trainDepth=1) -> int:
 def write_file(path, h: dict, lgn: dict, v1: dict, spike_train: np.ndarray, block: dict,
                keys=(0, 0)):
     """Write a snapshot from host arrays.  h: the header fields; lgn:
-    leftTimeRate, lastNegLogRand; v1: tBack; block: depC, w (iModel 1), v, gFF,
-    hFF, gE, gI, hE, hI, gap."""
+    leftTimeRate, lastNegLogRand; v1: tBack, optionally lastNoise; block:
+    depC, w (iModel 1), v, gFF, hFF, gE, gI, hE, hI, gap."""
     f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).tobytes()
     nV1, nLGN = h["nV1"], h["nLGN"]
     with open(path, "wb") as f:
@@ -92,7 +110,8 @@ def write_file(path, h: dict, lgn: dict, v1: dict, spike_train: np.ndarray, bloc
         f.write(f32(lgn["lastNegLogRand"]))
         f.write(generator_slots(nLGN, keys[0], keys[1], STREAM_LGN_SPIKE))
         f.write(generator_slots(nV1, keys[0], keys[1], STREAM_V1_BG))
-        f.write(generator_slots(nV1, keys[0], keys[1], NO_STREAM))
+        f.write(generator_slots(nV1, keys[0], keys[1], STREAM_V1_NOISE,
+                                carried=v1.get("lastNoise", np.zeros(nV1, np.float32))))
         f.write(f32(v1["tBack"]))
         f.write(f32(spike_train))
         f.write(f32(block["depC"]))
@@ -143,6 +162,7 @@ def read_file(path) -> dict:
     out["lgn_generators"] = raw(nLGN * GEN_BYTES)
     out["rGenCond"] = raw(nV1 * GEN_BYTES)
     out["rNoisy"] = raw(nV1 * GEN_BYTES)
+    out["lastNoise"] = carried_values(out["rNoisy"], nV1)
     out["tBack"] = floats(nV1)
     out["spikeTrain"] = floats(h["trainDepth"] * nV1).reshape(h["trainDepth"], nV1)
     ngFF, ngE, ngI = h["ngTypeFF"], h["ngTypeE"], h["ngTypeI"]
@@ -202,8 +222,8 @@ def save(sim, path) -> dict:
                  hE=st["hE"][:ngE], hI=st["hI"][:ngI],
                  gap=sim.gap.cpu().numpy() if getattr(sim, "gap", None) is not None
                  else np.zeros(sp.nblock * sp.nI, np.float32))
-    write_file(path, h, dict(leftTimeRate=lg[0], lastNegLogRand=lg[1]), dict(tBack=st["tBack"]),
-               train, block,
+    write_file(path, h, dict(leftTimeRate=lg[0], lastNegLogRand=lg[1]),
+               dict(tBack=st["tBack"], lastNoise=sim.lastNoise.cpu().numpy()), train, block,
                keys=(int(sim.p.seed_lo), int(sim.p.seed_hi)))
     return h
 
@@ -238,6 +258,7 @@ def restore(sim, path, as_init: bool = False) -> dict:
         st[k][:ng].copy_(T(d[k]))
     if getattr(sim, "gap", None) is not None:       # the gap sums after the last step
         sim.gap.copy_(T(d["gap"]))
+    sim.lastNoise.copy_(T(d["lastNoise"]))
     sim.lTR.copy_(T(d["leftTimeRate"]))
     sim.lNL.copy_(T(d["lastNegLogRand"]))
     ring = d["spikeTrain"]

@@ -69,9 +69,14 @@ def tune_step(sim, steps: int = 64, reps: int = 2, verbose: bool = False) -> tup
     mask0 = sim.ffMask2[0].clone() if sim.ev is not None else None
     st0 = {k: v.clone() for k, v in sim.state.items()}
     spk_t = [(sim.masks[i].clone(), sim.og[i].clone(), sim.oh[i].clone()) for i in range(2)]
+    # (with the gap sums and the noise state, which the steps also write)
+    st0["lastNoise"] = sim.lastNoise.clone()
+    if sim.gap is not None:
+        st0["gap"] = sim.gap.clone()
     st = {k: v.clone() for k, v in st0.items()}
     s = abi.V1State(*(ptr(st[k]) for k in ("v", "w", "tBack", "gFF", "hFF", "gE", "hE", "gI",
-                                           "hI")), ptr(sim.tonic))
+                                           "hI")), ptr(sim.tonic), ptr(sim.gapS),
+                    ptr(st.get("gap")), ptr(st["lastNoise"]))
     spk = [(m.clone(), g.clone(), h.clone()) for m, g, h in spk_t]
     spk2 = (abi.Spikes * 2)(*[abi.Spikes(ptr(m), ptr(g), ptr(h)) for m, g, h in spk])
     mask = mask0.clone() if mask0 is not None else None
