@@ -23,7 +23,11 @@
  *   vc_v1_step             compute_V_collect_spike_learnFF_fast
  *                            src/coredynamics.cu:1235-2059 (stages c-FF, d, e)
  *                            fused with recal_G_mat_nd_fast for the previous
- *                            step's spikes, src/coredynamics.cu:2609-2922
+ *                            step's spikes, src/coredynamics.cu:2609-2922;
+ *                            for a type with tRef < dt the multi-spike
+ *                            compute_V_collect_spike_learnFF, :427-1233, and
+ *                            its dense gather recal_G_mat_nd, :2364-2607
+ *                            (the same sums over the same spikes)
  *   vc_v1_rec_apply        recal_G_mat_nd_fast on its own (the pending
  *                            recurrent increments of the last step)
  *   vc_v1_rec_delay        recal_G_mat (axonal delays, noDelay = 0)
@@ -204,7 +208,13 @@ typedef struct {
 
 /* One simulation step on all V1 neurons: apply the recurrent increments of
  * `prev` (the previous step's spikes), decay, LGN + background input, RK2
- * update, spike time, refractory handling, compaction into `next`.
+ * update, spike time, refractory handling, compaction into `next`.  A
+ * neuron whose type has tRef < dt may fire several times in the step: after
+ * each spike whose refractory period ends inside it the rest of the step is
+ * integrated again from vR, and out_sInfo holds nsp + mean(tsp)/dt, its
+ * increments in `next` being those of nsp spikes at the mean time
+ * (compute_V_collect_spike_learnFF, src/coredynamics.cu:617-836, 1205-1230).
+ * tRef must be at least dt/512.
  * ffMask (may be NULL) is this step's row of vc_ff_events.mask: when given,
  * only the synapses whose bit is set are visited and the row is cleared;
  * when NULL every synapse's LGN cell is read.

@@ -41,6 +41,11 @@ class Oracle:
         L.orc_v1_rec_far.restype = i32
         L.orc_v1_gap.argtypes = [P(abi_mod.V1Params), u32, vp, vp, vp, vp, f32, vp, u32, u32, vp]
         L.orc_v1_gap.restype = i32
+        L.orc_v1_rec_fast.argtypes = [P(abi_mod.V1Params)] + [vp] * 4 + [u32] + [vp] * 7 + [u32]
+        L.orc_v1_rec_fast.restype = i32
+        L.orc_v1_rec_dense.argtypes = [P(abi_mod.V1Params)] + [vp] * 4 + [u32] + [vp] * 4 + [u32]
+        L.orc_v1_rec_dense.restype = i32
+        L.orc_set_multi_all.argtypes = [i32]
         L.orc_lgn_frame_response.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp,
                                              C.c_uint64, vp]
         L.orc_lgn_steps.argtypes = [P(abi_mod.LgnParams)] + [vp] * 11 + [u32, u32, vp, vp, vp]

@@ -17,12 +17,15 @@
  *   - LGN temporal sum,  src/discrete_input_convol.cu:1212-1340,
  *     rate and spikes    :1637-1821, :1587-1635; src/LGN_props.cuh:94-100
  *   - V1 update          src/coredynamics.cu:1235-1629 (non-learning path),
- *                        src/coredynamics.cuh:27-118 (AdEx), src/condShape.h:13-41
+ *                        src/coredynamics.cuh:27-118 (AdEx), src/condShape.h:13-41;
+ *                        several spikes per step (tRef < dt):
+ *                        src/coredynamics.cu:530-836
  *   - spike compaction   src/coredynamics.cu:1994-2046 (lists in ascending
  *                        order: the reference's atomicAdd_block order is
  *                        unspecified, ascending is one of its outcomes)
  *   - recurrent gather   src/coredynamics.cu:2609-2922, dense conMat
- *                        [nblock][maxNear][bs pre][bs post]
+ *                        [nblock][maxNear][bs pre][bs post]; the dense-scan
+ *                        form recal_G_mat_nd, :2364-2607
  *   - delayed recurrent  src/coredynamics.cu:2062-2309 (recal_G_mat, axonal
  *     gather             delays from a ring of past step outputs)
  *   - far connections    src/coredynamics.cu:181-372, 2314-2349 (recal_G_vec
@@ -395,6 +398,300 @@ static float orc_update_neuron(const vc_v1_params *p, uint32_t n, float *v,
     return sInfo;
 }
 
+/* One V1 neuron that may fire several times in a step,
+ * compute_V_collect_spike_learnFF, src/coredynamics.cu:530-836 (the
+ * reference runs it for every neuron when dt > min tRef, :6297).  The state
+ * at the start of the step goes into (g0, ge, gi); g1 collects every input
+ * of the step once (first pass).  The do-while re-integrates the rest of the
+ * step from vR after each spike whose refractory period ends inside it:
+ * g0/ge/gi decay from the previous restart new_t0 to tBack, and the LGN
+ * inputs (list order, then the background one) that arrived in [new_t0,
+ * tBack) are added at tBack.  A synapse's failure factor is drawn in the
+ * first pass and kept (f[i]).  White noise: `noise1 /= dt` on every pass
+ * (:720-723).  sInfo = nsp + mean(tsp)/dt (:815-821). */
+static float orc_update_neuron_multi(const vc_v1_params *p, uint32_t n, float *v,
+                                     float *w, float *tBack, float *gFF, float *hFF,
+                                     float *gE, float *hE, float *gI, float *hI,
+                                     const float *tonicDep, const uint32_t *nLGN,
+                                     const float *sLGN, const uint32_t *idLGN,
+                                     const float *lgnSInfo, uint32_t step,
+                                     float *out_dep, const float *gapS, const float *gap,
+                                     float *lastNoise) {
+    const uint32_t nV1 = p->nV1;
+    const uint32_t local = n % p->blockSize;
+    const int t = orc_type_of(p, local);
+    const float dt = p->dt;
+    orc_adex m;
+    orc_adex_init(&m, p, t, w[n], v[n], tBack[n], tonicDep[n]);
+    uint32_t spikeCount = 0;
+
+    /* :549-580: cortical g decayed to the end of the step; ge/gi keep t = 0 */
+    float ge[VC_MAX_NG], he[VC_MAX_NG], gi[VC_MAX_NG], hi[VC_MAX_NG];
+    float gE_t1 = 0.0f, gI_t1 = 0.0f;
+    for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+        float g = gE[ig * nV1 + n], h = hE[ig * nV1 + n];
+        ge[ig] = g; he[ig] = h;
+        orc_decay(&g, &h, dt, p->riseE[ig], p->decayE[ig], p->dodE[ig]);
+        gE[ig * nV1 + n] = g; hE[ig * nV1 + n] = h;
+        gE_t1 += g;
+    }
+    for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+        float g = gI[ig * nV1 + n], h = hI[ig * nV1 + n];
+        gi[ig] = g; hi[ig] = h;
+        orc_decay(&g, &h, dt, p->riseI[ig], p->decayI[ig], p->dodI[ig]);
+        gI[ig * nV1 + n] = g; hI[ig * nV1 + n] = h;
+        gI_t1 += g;
+    }
+    /* :582-596 */
+    float g0[VC_MAX_NG], h0[VC_MAX_NG], g1[VC_MAX_NG], h1[VC_MAX_NG];
+    for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
+        g1[ig] = gFF[ig * nV1 + n]; h1[ig] = hFF[ig * nV1 + n];
+        g0[ig] = g1[ig]; h0[ig] = h1[ig];
+        orc_decay(&g1[ig], &h1[ig], dt, p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+    }
+    /* :606-622 noise, as in the one-spike update */
+    const orc_key key = orc_key_of(p, n);
+    const float tau = p->tauNoise;
+    float noise0 = 0.0f;
+    if (tau > 0.0f) noise0 = lastNoise ? lastNoise[n] : 0.0f;
+    float ratio0 = p->noiseOnTonic ? tonicDep[n] : 1.0f;
+    float noise1 = sqrtf(2.0f * p->noisyDep[t] * (p->vT[t] - p->vR[t]) * ratio0 * dt);
+    if (noise1 > 0.0f) {
+        vc_u32x4 r = vc_draw4(key.lo, key.hi, key.ent, step, VC_STREAM_V1_NOISE, 0, p->trial);
+        noise1 *= vc_normal(r.v[0], r.v[1]);
+        if (tau > 0.0f) {
+            noise1 /= tau;
+            float hk1 = -noise0 / tau;
+            float hk2 = -(noise0 + dt * hk1 + noise1) / tau;
+            noise1 += noise0 + dt * (hk1 + hk2) / 2.0f;
+            if (lastNoise) lastNoise[n] = noise1;
+        }
+    }
+    int backingUpFromRef = (m.tBack < dt) && (m.tBack > 0.0f);
+    float new_t0 = 0.0f, sInfo = 0.0f;
+    if (out_dep) out_dep[n] = m.depC + noise1;
+    const uint32_t mLGN = nLGN[n];
+    uint32_t bgNsp = 0;
+    float bgTsp = 0.0f;
+    if (p->bgStrength[t] > 0.0f) {
+        float frac;
+        bgNsp = orc_bg_draw(p, t, n, step, &frac);
+        bgTsp = frac * dt;
+    }
+    float lgapS = 0.0f, lgap = 0.0f;
+    if (local >= p->nE) {
+        uint32_t gid = (n / p->blockSize) * (p->blockSize - p->nE) + (local - p->nE);
+        if (gapS) lgapS = gapS[gid];
+        if (gap) lgap = gap[gid];
+    }
+    do {
+        float dtBack = 0.0f;
+        if (backingUpFromRef) dtBack = m.tBack - new_t0;
+        /* :645-651 */
+        for (uint32_t ig = 0; ig < p->ngFF; ++ig)
+            if (backingUpFromRef)
+                orc_decay(&g0[ig], &h0[ig], dtBack, p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+        /* :653-693 the LGN inputs, then the background one (extension) */
+        for (uint32_t i = 0; i < mLGN; ++i) {
+            uint32_t lid = i * nV1 + n;
+            float s = lgnSInfo[idLGN[lid]];
+            uint32_t nsp = (uint32_t)floorf(s);
+            float tsp = (s - (float)nsp) * dt;
+            if (nsp > 0 && tsp >= new_t0) {
+                /* f[i]: the reference draws the failure factor in the first
+                 * pass and keeps it; the draw depends on (neuron, step, i)
+                 * only, so drawing it again in a later pass gives it back */
+                float fi = sLGN[lid];
+                if (p->synFailFF[t] > 0.0f)
+                    fi = orc_fail(fi, p->synFailFF[t], p->synPerConFF[t], key, step,
+                                  VC_STREAM_V1_FAILFF, i, p->trial);
+                if (fi > 0.0f) {
+                    float ddt = 0.0f;
+                    if (backingUpFromRef) ddt = m.tBack - tsp;
+                    for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
+                        float str = fi * p->pFF[t * p->ngFF + ig];
+                        if (backingUpFromRef && ddt > 0.0f)
+                            orc_single_input(&g0[ig], &h0[ig], str * (float)nsp, ddt,
+                                             p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+                        if (spikeCount == 0)
+                            orc_single_input(&g1[ig], &h1[ig], str * (float)nsp, dt - tsp,
+                                             p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+                    }
+                }
+            }
+        }
+        if (bgNsp > 0 && bgTsp >= new_t0) {
+            float f = p->bgStrength[t];
+            float ddt = 0.0f;
+            if (backingUpFromRef) ddt = m.tBack - bgTsp;
+            for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
+                float str = f * p->pBG[t * p->ngFF + ig];
+                if (backingUpFromRef && ddt > 0.0f)
+                    orc_single_input(&g0[ig], &h0[ig], str * (float)bgNsp, ddt,
+                                     p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+                if (spikeCount == 0)
+                    orc_single_input(&g1[ig], &h1[ig], str * (float)bgNsp, dt - bgTsp,
+                                     p->riseFF[ig], p->decayFF[ig], p->dodFF[ig]);
+            }
+        }
+        /* :697-727 collect */
+        float gE_t0 = 0.0f;
+        for (uint32_t ig = 0; ig < p->ngFF; ++ig) {
+            gE_t0 += g0[ig];
+            if (spikeCount == 0) {
+                gE_t1 += g1[ig];
+                gFF[ig * nV1 + n] = g1[ig];
+                hFF[ig * nV1 + n] = h1[ig];
+            }
+        }
+        for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+            if (backingUpFromRef)
+                orc_decay(&ge[ig], &he[ig], dtBack, p->riseE[ig], p->decayE[ig], p->dodE[ig]);
+            gE_t0 += ge[ig];
+        }
+        float gI_t0 = 0.0f;
+        for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+            if (backingUpFromRef)
+                orc_decay(&gi[ig], &hi[ig], dtBack, p->riseI[ig], p->decayI[ig], p->dodI[ig]);
+            gI_t0 += gi[ig];
+        }
+        /* :730-800 stepping */
+        m.tsp = 0.0f;
+        if (m.tBack < dt) {
+            m.a0 = gE_t0 + gI_t0 + m.gL + lgapS;
+            m.b0 = gE_t0 * ORC_VE + gI_t0 * ORC_VI + m.gL * ORC_VL + m.depC + lgap;
+            if (spikeCount == 0) {
+                m.a1 = gE_t1 + gI_t1 + m.gL + lgapS;
+                m.b1 = gE_t1 * ORC_VE + gI_t1 * ORC_VI + m.gL * ORC_VL + m.depC + lgap;
+            }
+            float new_dt = dt - m.tBack;
+            if (backingUpFromRef) {                      /* rk2_vFixedBefore(dtBack) */
+                float A = m.a * (m.v0 - ORC_VL);
+                m.w0 = (m.w0 - A) * vc_expf(-dtBack / m.tau_w) + A;
+            }
+            if (tau > 0.0f) {
+                orc_rk2(&m, new_dt, noise0, noise1);
+            } else {
+                noise1 /= dt;
+                orc_rk2(&m, new_dt, noise1, noise1);
+            }
+            if (m.v > m.vThres) {
+                new_t0 = m.tBack;                        /* compute_spike_time */
+                float ratio = (m.vThres - m.v0) / (m.v - m.v0);
+                m.tsp = new_t0 + ratio * new_dt;
+                m.w0 += ratio * (m.w - m.w0) + m.b;
+                sInfo += m.tsp;
+                spikeCount++;
+                m.tBack = m.tsp + m.tRef;
+                backingUpFromRef = m.tBack < dt;
+                if (backingUpFromRef) m.v0 = m.vR;       /* reset(true) */
+                else m.v = m.vR;                          /* reset(false) */
+            } else {
+                if (m.tBack > 0.0f) m.tBack = 0.0f;
+                backingUpFromRef = 0;
+            }
+        }
+        if (m.tBack >= dt) {                             /* rk2_vFixedAfter */
+            m.v = m.vR;
+            float A = m.a * (m.v - ORC_VL);
+            m.w = (m.w0 - A) * vc_expf(-(dt - m.tsp) / m.tau_w) + A;
+            m.tBack -= dt;
+        }
+    } while (backingUpFromRef);
+    if (spikeCount > 0) {
+        sInfo /= (float)spikeCount * dt;
+        sInfo += (float)spikeCount;
+    } else {
+        sInfo = m.v;
+    }
+    v[n] = m.v;
+    if (p->iModel == 1) w[n] = m.w;
+    tBack[n] = m.tBack;
+    return sInfo;
+}
+
+/* 1: every neuron takes the multi-spike update, as the reference does for
+ * the whole network when dt > min tRef; 0: only types with tRef < dt do
+ * (the two give the same results, which tests/ check). */
+static int orc_multi_all = 0;
+void orc_set_multi_all(int on) { orc_multi_all = on; }
+
+/*
+ * recal_G_mat_nd_fast (src/coredynamics.cu:2609-2922, the gather at
+ * :2721-2916): every post neuron sums, over its neighbour slots ib and the
+ * E list then the I list of block nbr[b][ib] (ipre/npre, og/oh in the
+ * reference's compacted layout, E list first), strength * p[type][ig] *
+ * og/oh.  `step` is the step the spikes were fired in (the failure draws'
+ * key).
+ */
+int orc_v1_rec_fast(const vc_v1_params *p, float *gE, float *hE, float *gI, float *hI,
+                    uint32_t maxNear, const uint32_t *nNear, const uint32_t *nbr,
+                    const float *conMat, const uint32_t *ipre, const uint32_t *npre,
+                    const float *og, const float *oh, uint32_t step) {
+    const uint32_t nV1 = p->nV1, bs = p->blockSize, nblock = p->nblock;
+    const uint32_t nE = p->nE, nI = bs - nE;
+    const uint32_t block_ng = p->ngE * nE + p->ngI * nI;
+#pragma omp parallel for schedule(static)
+    for (uint32_t n = 0; n < nV1; ++n) {
+        uint32_t b = n / bs, l = n % bs;
+        int t = orc_type_of(p, l);
+        float lgE[VC_MAX_NG] = {0}, lhE[VC_MAX_NG] = {0};
+        float lgI[VC_MAX_NG] = {0}, lhI[VC_MAX_NG] = {0};
+        /* synaptic failure by presynaptic type (E: the first, I: the
+         * last) and this neuron's type, drawn per (this neuron, step,
+         * presynaptic neuron) */
+        const orc_key key = orc_key_of(p, n);
+        const float pfE = p->synFail[0 * p->nType + t];
+        const float pfI = p->synFail[(p->nType - 1) * p->nType + t];
+        const float nsE = p->synPerCon[0 * p->nType + t];
+        const float nsI = p->synPerCon[(p->nType - 1) * p->nType + t];
+        for (uint32_t ib = 0; ib < nNear[b]; ++ib) {
+            uint32_t bid = nbr[b * maxNear + ib];
+            uint32_t cE = npre[bid], cI = npre[nblock + bid];
+            uint64_t local_bid = (uint64_t)b * maxNear + ib;
+            for (uint32_t i = 0; i < cE; ++i) {
+                uint32_t pre = ipre[bid * bs + i];
+                float strength = conMat[(local_bid * bs + pre) * bs + l];
+                if (strength > 0.0f && pfE > 0.0f)
+                    strength = orc_fail(strength, pfE, nsE, key, step, VC_STREAM_V1_FAIL,
+                                        bid * bs + pre - key.off, p->trial);
+                if (strength > 0.0f) {
+                    for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+                        float str = strength * p->pE[t * p->ngE + ig];
+                        uint32_t id = block_ng * bid + i;
+                        lgE[ig] += str * og[ig * cE + id];
+                        lhE[ig] += str * oh[ig * cE + id];
+                    }
+                }
+            }
+            for (uint32_t i = 0; i < cI; ++i) {
+                uint32_t pre = ipre[bid * bs + cE + i];
+                float strength = conMat[(local_bid * bs + pre) * bs + l];
+                if (strength > 0.0f && pfI > 0.0f)
+                    strength = orc_fail(strength, pfI, nsI, key, step, VC_STREAM_V1_FAIL,
+                                        bid * bs + pre - key.off, p->trial);
+                if (strength > 0.0f) {
+                    for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+                        float str = strength * p->pI[t * p->ngI + ig];
+                        uint32_t id = block_ng * bid + cE * p->ngE + i;
+                        lgI[ig] += str * og[ig * cI + id];
+                        lhI[ig] += str * oh[ig * cI + id];
+                    }
+                }
+            }
+        }
+        for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+            gE[ig * nV1 + n] += lgE[ig];
+            hE[ig * nV1 + n] += lhE[ig];
+        }
+        for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+            gI[ig * nV1 + n] += lgI[ig];
+            hI[ig * nV1 + n] += lhI[ig];
+        }
+    }
+    return VC_OK;
+}
+
 /*
  * One V1 step in the reference's order: the update kernel (c-FF, d, e) for
  * every neuron, then recal_G for the spikes of this step.
@@ -424,9 +721,15 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
 
 #pragma omp parallel for schedule(static)
     for (uint32_t n = 0; n < nV1; ++n) {
-        sInfo[n] = orc_update_neuron(p, n, v, w, tBack, gFF, hFF, gE, hE, gI,
-                                     hI, tonicDep, nLGN, sLGN, idLGN, lgnSInfo,
-                                     step, out_dep, gapS, gap, lastNoise);
+        const int t = orc_type_of(p, n % bs);
+        if (orc_multi_all || p->tRef[t] < p->dt)
+            sInfo[n] = orc_update_neuron_multi(p, n, v, w, tBack, gFF, hFF, gE, hE, gI,
+                                               hI, tonicDep, nLGN, sLGN, idLGN, lgnSInfo,
+                                               step, out_dep, gapS, gap, lastNoise);
+        else
+            sInfo[n] = orc_update_neuron(p, n, v, w, tBack, gFF, hFF, gE, hE, gI,
+                                         hI, tonicDep, nLGN, sLGN, idLGN, lgnSInfo,
+                                         step, out_dep, gapS, gap, lastNoise);
     }
     /* compaction, src/coredynamics.cu:1994-2046, ascending order */
     const uint32_t block_ng = p->ngE * nE + p->ngI * nI;
@@ -465,68 +768,82 @@ int orc_v1_step(const vc_v1_params *p, float *v, float *w, float *tBack,
             }
         }
     }
-    if (do_rec) {
-        /* recal_G_mat_nd_fast, src/coredynamics.cu:2721-2916 */
+    if (do_rec)
+        orc_v1_rec_fast(p, gE, hE, gI, hI, maxNear, nNear, nbr, conMat, ipre, npre, og, oh,
+                        step);
+    if (own) free(sInfo);
+    return VC_OK;
+}
+
+/*
+ * The dense-scan recurrent gather, recal_G_mat_nd (src/coredynamics.cu
+ * :2364-2607, without learning and gap junctions): every post neuron n
+ * visits, for ib < nNear[b] and every presynaptic index i < bs of block
+ * nbr[b][ib] in order, the pairs with conMat != 0 whose step output
+ * sInfo[pre] > 0 (a spike), draws the failure factor, and for strength > 0
+ * adds strength * p[type][ig] * og/oh of the presynaptic neuron.  og/oh are
+ * read per neuron, as the multi-spike update writes them
+ * (src/coredynamics.cu:1205-1230: single_input(nsp, dt - tsp) with nsp and
+ * tsp from the neuron's sInfo).  `step` is the step the spikes were fired
+ * in (the failure draws' key).
+ */
+int orc_v1_rec_dense(const vc_v1_params *p, float *gE, float *hE, float *gI, float *hI,
+                     uint32_t maxNear, const uint32_t *nNear, const uint32_t *nbr,
+                     const float *conMat, const float *sInfo, uint32_t step) {
+    const uint32_t nV1 = p->nV1, bs = p->blockSize, nE = p->nE;
 #pragma omp parallel for schedule(static)
-        for (uint32_t n = 0; n < nV1; ++n) {
-            uint32_t b = n / bs, l = n % bs;
-            int t = orc_type_of(p, l);
-            float lgE[VC_MAX_NG] = {0}, lhE[VC_MAX_NG] = {0};
-            float lgI[VC_MAX_NG] = {0}, lhI[VC_MAX_NG] = {0};
-            /* synaptic failure by presynaptic type (E: the first, I: the
-             * last) and this neuron's type, drawn per (this neuron, step,
-             * presynaptic neuron) */
-            const orc_key key = orc_key_of(p, n);
-            const float pfE = p->synFail[0 * p->nType + t];
-            const float pfI = p->synFail[(p->nType - 1) * p->nType + t];
-            const float nsE = p->synPerCon[0 * p->nType + t];
-            const float nsI = p->synPerCon[(p->nType - 1) * p->nType + t];
-            for (uint32_t ib = 0; ib < nNear[b]; ++ib) {
-                uint32_t bid = nbr[b * maxNear + ib];
-                uint32_t cE = npre[bid], cI = npre[nblock + bid];
-                uint64_t local_bid = (uint64_t)b * maxNear + ib;
-                for (uint32_t i = 0; i < cE; ++i) {
-                    uint32_t pre = ipre[bid * bs + i];
-                    float strength = conMat[(local_bid * bs + pre) * bs + l];
-                    if (strength > 0.0f && pfE > 0.0f)
-                        strength = orc_fail(strength, pfE, nsE, key, step, VC_STREAM_V1_FAIL,
-                                            bid * bs + pre - key.off, p->trial);
-                    if (strength > 0.0f) {
-                        for (uint32_t ig = 0; ig < p->ngE; ++ig) {
-                            float str = strength * p->pE[t * p->ngE + ig];
-                            uint32_t id = block_ng * bid + i;
-                            lgE[ig] += str * og[ig * cE + id];
-                            lhE[ig] += str * oh[ig * cE + id];
-                        }
+    for (uint32_t n = 0; n < nV1; ++n) {
+        const uint32_t b = n / bs, l = n % bs;
+        const int t = orc_type_of(p, l);
+        const orc_key key = orc_key_of(p, n);
+        float lgE[VC_MAX_NG] = {0}, lhE[VC_MAX_NG] = {0};
+        float lgI[VC_MAX_NG] = {0}, lhI[VC_MAX_NG] = {0};
+        for (uint32_t ib = 0; ib < nNear[b]; ++ib) {
+            const uint32_t bid = nbr[b * maxNear + ib];
+            const uint64_t local_bid = (uint64_t)b * maxNear + ib;
+            for (uint32_t i = 0; i < bs; ++i) {
+                const uint32_t pre = bid * bs + i;
+                float strength = conMat[(local_bid * bs + i) * bs + l];
+                const float s = sInfo[pre];
+                if (!(strength != 0.0f && s > 0.0f)) continue;
+                const int jtype = orc_type_of(p, i);
+                const float pf = p->synFail[jtype * p->nType + t];
+                if (pf > 0.0f)
+                    strength = orc_fail(strength, pf, p->synPerCon[jtype * p->nType + t], key,
+                                        step, VC_STREAM_V1_FAIL, pre - key.off, p->trial);
+                if (!(strength > 0.0f)) continue;
+                const float nsp = floorf(s);
+                const float tsp = (s - nsp) * p->dt;
+                if (i < nE) {
+                    for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+                        float g = 0.0f, h = 0.0f;
+                        orc_single_input(&g, &h, nsp, p->dt - tsp, p->riseE[ig], p->decayE[ig],
+                                         p->dodE[ig]);
+                        const float str = strength * p->pE[t * p->ngE + ig];
+                        lgE[ig] += str * g;
+                        lhE[ig] += str * h;
+                    }
+                } else {
+                    for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+                        float g = 0.0f, h = 0.0f;
+                        orc_single_input(&g, &h, nsp, p->dt - tsp, p->riseI[ig], p->decayI[ig],
+                                         p->dodI[ig]);
+                        const float str = strength * p->pI[t * p->ngI + ig];
+                        lgI[ig] += str * g;
+                        lhI[ig] += str * h;
                     }
                 }
-                for (uint32_t i = 0; i < cI; ++i) {
-                    uint32_t pre = ipre[bid * bs + cE + i];
-                    float strength = conMat[(local_bid * bs + pre) * bs + l];
-                    if (strength > 0.0f && pfI > 0.0f)
-                        strength = orc_fail(strength, pfI, nsI, key, step, VC_STREAM_V1_FAIL,
-                                            bid * bs + pre - key.off, p->trial);
-                    if (strength > 0.0f) {
-                        for (uint32_t ig = 0; ig < p->ngI; ++ig) {
-                            float str = strength * p->pI[t * p->ngI + ig];
-                            uint32_t id = block_ng * bid + cE * p->ngE + i;
-                            lgI[ig] += str * og[ig * cI + id];
-                            lhI[ig] += str * oh[ig * cI + id];
-                        }
-                    }
-                }
-            }
-            for (uint32_t ig = 0; ig < p->ngE; ++ig) {
-                gE[ig * nV1 + n] += lgE[ig];
-                hE[ig * nV1 + n] += lhE[ig];
-            }
-            for (uint32_t ig = 0; ig < p->ngI; ++ig) {
-                gI[ig * nV1 + n] += lgI[ig];
-                hI[ig * nV1 + n] += lhI[ig];
             }
         }
+        for (uint32_t ig = 0; ig < p->ngE; ++ig) {
+            gE[ig * nV1 + n] += lgE[ig];
+            hE[ig * nV1 + n] += lhE[ig];
+        }
+        for (uint32_t ig = 0; ig < p->ngI; ++ig) {
+            gI[ig * nV1 + n] += lgI[ig];
+            hI[ig * nV1 + n] += lhI[ig];
+        }
     }
-    if (own) free(sInfo);
     return VC_OK;
 }
 

@@ -201,7 +201,7 @@ __device__ __forceinline__ float tree32_regs(float a[32]) {
     return a[0];
 }
 
-constexpr int kMaxFrame = 8;
+constexpr int kMaxFrame = 16;              // frames under one wavefront's tap windows
 constexpr int kWin = 16;                    // rate window of k_lgn_spike
 
 
@@ -314,10 +314,13 @@ __global__ __launch_bounds__(64) void k_lgn_rate_cells(vc_lgn_params p, vc_lgn_s
     // frames under the run's windows: from step 0's first tap on
     const int64_t base0 = (int64_t)it0 + (int64_t)p.iKernelSampleT0 - (int64_t)p.nRetrace + 1;
     const int64_t fw = floor_div(base0, (int64_t)npf);
+    // the run's taps span at most (nsteps - 1 + nRetrace) / npf + 2 frames
+    const uint32_t nF = (nsteps - 1u + p.nRetrace) / npf + 2u;
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int d = 0; d < kF; ++d) {
+            if ((uint32_t)d >= nF) break;
             int64_t slot = (fw + d) % (int64_t)p.maxFrame;
             if (slot < 0) slot += p.maxFrame;
             sR[ln * kRow + r * kF + d] = Rring[((size_t)slot * p.nLGN + cc) * 2 + r];

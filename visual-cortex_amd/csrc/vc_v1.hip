@@ -543,6 +543,20 @@ __device__ __forceinline__ void ff_trip2(FFPre<false> &f, const vc_ff_conn &, co
 struct BGIn {
     float f, nsp, tsp;
 };
+// One Philox draw per neuron-step gives the Poisson count (by inversion) and
+// the event time.
+__device__ __forceinline__ BGIn bg_input(const vc_v1_params &p, int t, const V1Key &key,
+                                         uint32_t step) {
+    BGIn bg{0.0f, 0.0f, 0.0f};
+    if (p.bgStrength[t] > 0.0f) {
+        const vc_u32x4 r = vc_draw4(key.lo, key.hi, key.ent, step, VC_STREAM_V1_BG, 0u, p.trial);
+        const float u = vc_u01_open0(r.v[0]);
+        const float frac = vc_u01_open1(r.v[1]);
+        const uint32_t nsp = vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
+        if (nsp > 0u) bg = BGIn{p.bgStrength[t], (float)nsp, frac * p.dt};
+    }
+    return bg;
+}
 
 // LGN synapses in list order, then the background input; only synapses
 // whose cell spiked (sInfo >= 1) do work (reference: nsp_FF > 0).  Every
@@ -713,15 +727,7 @@ __device__ __forceinline__ FFAcc ff_stage(const vc_v1_params &p, const V1Const &
     }
     shFF.ng = shBG.ng = p.ngFF;
     const V1Key key = v1_key(p, b, n);
-    // background Poisson drive (extension): one more input after the LGN list
-    BGIn bg{0.0f, 0.0f, 0.0f};
-    if (p.bgStrength[t] > 0.0f) {
-        const vc_u32x4 r = vc_draw4(key.lo, key.hi, key.ent, step, VC_STREAM_V1_BG, 0u, p.trial);
-        const float u = vc_u01_open0(r.v[0]);
-        const float frac = vc_u01_open1(r.v[1]);
-        const uint32_t nsp = vc_poisson_inv(u, p.bgLambda[t], p.bgP0[t]);
-        if (nsp > 0u) bg = BGIn{p.bgStrength[t], (float)nsp, frac * dt};
-    }
+    const BGIn bg = bg_input(p, t, key, step);
     const FailFF fl{kSt ? p.synFailFF[t] : 0.0f, p.synPerConFF[t], key, step, p.trial};
     ff_inputs(acc, shFF, shBG, f, ff, lgnSInfo, ffMask, n, p.nV1, dt, tBack, backing, bg, fl);
     return acc;
@@ -884,6 +890,229 @@ __device__ __forceinline__ float membrane_stage(const vc_v1_params &p, const V1C
     if (p.iModel == 1) x.w = w;
     x.tBack = tBack;
     return spiked ? 1.0f + tsp / dt : v;
+}
+
+// Stage d when the neuron's refractory period is shorter than dt, so that it
+// may fire several times in one step (compute_V_collect_spike_learnFF,
+// src/coredynamics.cu:617-836; the reference runs it for every neuron when
+// dt > min tRef, and for a type with tRef >= dt it gives what membrane_stage
+// gives).  After each spike whose refractory period ends inside the step
+// (tBack < dt) the rest of the step is integrated again from vR: the
+// conductances at the new tBack are those at the previous restart decayed
+// over dtBack = tBack - new_t0 plus the LGN and background inputs that
+// arrived in [new_t0, tBack) (re-read from the synapse lists; the failure
+// draws are keyed by the synapse, so a re-read finds the same factor);
+// w is carried as the reference carries it (rk2_vFixedBefore over dtBack
+// after compute_spike_time has moved w0 to the spike), and white noise is
+// divided by dt once more in every further pass, as the reference's
+// `noise1 /= dt` inside its loop does.  sInfo = nsp + mean(tsp)/dt.
+// Arguments as membrane_stage, plus what the re-read needs.
+__device__ __forceinline__ float membrane_multi(const vc_v1_params &p, const V1Const &k, int t,
+                                                float depC, NState &x, const FFAcc &acc,
+                                                float gapS, float gap, float n0, float n1,
+                                                const vc_ff_conn &ff, const float *lgnSInfo,
+                                                uint32_t n, uint32_t step, const V1Key &key) {
+    const float dt = p.dt;
+    const uint32_t nV1 = p.nV1;
+    const float vR = p.vR[t], vThres = p.vThres[t], gL = p.gL[t], C = p.C[t];
+    const float tRef = p.tRef[t], vT = p.vT[t], deltaT = p.deltaT[t];
+    const float tau_w = p.tau_w[t], a = p.a[t], bb = p.b[t];
+    const bool white = !(p.tauNoise > 0.0f);
+    float tBack = x.tBack;
+    bool backing = (tBack < dt) && (tBack > 0.0f);
+    // cortical conductances decayed to the end of the step (the next state)
+    float geN[VC_MAX_NG], heN[VC_MAX_NG], giN[VC_MAX_NG], hiN[VC_MAX_NG];
+    float gE_t1 = 0.0f, gI_t1 = 0.0f;
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        float g = x.gE[ig], h = x.hE[ig];
+        if (ig < p.ngE) {
+            decay_const(g, h, k.etrE[ig], k.etdE[ig], p.dodE[ig]);
+            gE_t1 += g;
+        }
+        geN[ig] = g;
+        heN[ig] = h;
+    }
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        float g = x.gI[ig], h = x.hI[ig];
+        if (ig < p.ngI) {
+            decay_const(g, h, k.etrI[ig], k.etdI[ig], p.dodI[ig]);
+            gI_t1 += g;
+        }
+        giN[ig] = g;
+        hiN[ig] = h;
+    }
+    float g0f[VC_MAX_NG], h0f[VC_MAX_NG];    // FF receptors at the current restart
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        g0f[ig] = acc.g0[ig];
+        h0f[ig] = acc.h0[ig];
+        if (ig < p.ngFF) {
+            gE_t1 += acc.g1[ig];
+            x.gFF[ig] = acc.g1[ig];
+            x.hFF[ig] = acc.h1[ig];
+        }
+    }
+    float pFF[VC_MAX_NG], pBG[VC_MAX_NG];
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        const uint32_t pi = t * p.ngFF + (ig < p.ngFF ? ig : 0u);
+        pFF[ig] = p.pFF[pi];
+        pBG[ig] = p.pBG[pi];
+    }
+    const BGIn bg = bg_input(p, t, key, step);
+    const float pf = p.synFailFF[t];
+    const uint32_t m = ff.nLGN[n];
+
+    float v0 = x.v, w0 = x.w, v = v0, w = w0, tsp = 0.0f;
+    float new_t0 = 0.0f, sumTsp = 0.0f, a1 = 0.0f, b1 = 0.0f;
+    uint32_t count = 0;
+    bool first = true;
+    do {
+        const float dtBack = tBack - new_t0;
+        if (!first) {                        // a restart after a spike (backing is set)
+#pragma unroll
+            for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                if (ig < p.ngFF)
+                    decay_var(g0f[ig], h0f[ig], dtBack, p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
+            for (uint32_t i = 0; i < m; ++i) {          // LGN inputs in [new_t0, tBack)
+                const uint32_t lid = i * nV1 + n;
+                const float sI = lgnSInfo[ff.idLGN[lid]];
+                const float nsp = (float)(uint32_t)floorf(sI);
+                if (!(nsp > 0.0f)) continue;
+                const float ts = (sI - nsp) * dt;
+                if (!(ts >= new_t0)) continue;
+                float f = ff.sLGN[lid];
+                if (pf > 0.0f)
+                    f = fail_factor(f, pf, p.synPerConFF[t], key, step, VC_STREAM_V1_FAILFF, i,
+                                    p.trial);
+                const float ddt = tBack - ts;
+                if (f > 0.0f && ddt > 0.0f) {
+#pragma unroll
+                    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                        if (ig < p.ngFF)
+                            single_input(g0f[ig], h0f[ig], f * pFF[ig] * nsp, ddt, p.riseFF[ig],
+                                         p.decayFF[ig], p.dodFF[ig]);
+                }
+            }
+            if (bg.nsp > 0.0f && bg.tsp >= new_t0) {    // the background input
+                const float ddt = tBack - bg.tsp;
+                if (ddt > 0.0f) {
+#pragma unroll
+                    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+                        if (ig < p.ngFF)
+                            single_input(g0f[ig], h0f[ig], bg.f * pBG[ig] * bg.nsp, ddt,
+                                         p.riseFF[ig], p.decayFF[ig], p.dodFF[ig]);
+                }
+            }
+        }
+        float gE_t0 = 0.0f;
+#pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig)
+            if (ig < p.ngFF) gE_t0 += g0f[ig];
+#pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+            if (ig < p.ngE) {
+                if (backing)
+                    decay_var(x.gE[ig], x.hE[ig], dtBack, p.riseE[ig], p.decayE[ig], p.dodE[ig]);
+                gE_t0 += x.gE[ig];
+            }
+        }
+        float gI_t0 = 0.0f;
+#pragma unroll
+        for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+            if (ig < p.ngI) {
+                if (backing)
+                    decay_var(x.gI[ig], x.hI[ig], dtBack, p.riseI[ig], p.decayI[ig], p.dodI[ig]);
+                gI_t0 += x.gI[ig];
+            }
+        }
+        tsp = 0.0f;
+        if (tBack < dt) {
+            const float a0 = gE_t0 + gI_t0 + gL + gapS;
+            const float b0 = gE_t0 * VC_VE + gI_t0 * VC_VI + gL * VC_VL + depC + gap;
+            if (first) {
+                a1 = gE_t1 + gI_t1 + gL + gapS;
+                b1 = gE_t1 * VC_VE + gI_t1 * VC_VI + gL * VC_VL + depC + gap;
+            }
+            const float new_dt = dt - tBack;
+            if (backing) {
+                const float A = a * (v0 - VC_VL);
+                w0 = (w0 - A) * vc_expf(-dtBack / tau_w) + A;
+            }
+            if (!first && white) n0 = n1 = n1 / dt;
+            const float dTgL = deltaT * gL;
+            float fk1, fk2;
+            if (deltaT > 0.0f)
+                fk1 = (-a0 * v0 + b0 + dTgL * vc_expf((v0 - vT) / deltaT) - w0) / C + n0;
+            else
+                fk1 = (-a0 * v0 + b0 - w0) / C + n0;
+            const float gk1 = (a * (v0 - VC_VL) - w0) / tau_w;
+            const float v1 = v0 + new_dt * fk1;
+            const float w1 = w0 + new_dt * gk1;
+            if (deltaT > 0.0f)
+                fk2 = (-a1 * v1 + b1 + dTgL * vc_expf((v1 - vT) / deltaT) - w1) / C + n1;
+            else
+                fk2 = (-a1 * v1 + b1 - w1) / C + n1;
+            const float gk2 = (a * (v1 - VC_VL) - w1) / tau_w;
+            w = w0 + new_dt * (gk1 + gk2) / 2.0f;
+            v = v0 + new_dt * (fk1 + fk2) / 2.0f;
+            if (v > vThres) {
+                new_t0 = tBack;
+                const float ratio = (vThres - v0) / (v - v0);
+                tsp = new_t0 + ratio * new_dt;
+                w0 += ratio * (w - w0) + bb;
+                sumTsp += tsp;
+                ++count;
+                tBack = tsp + tRef;
+                backing = tBack < dt;
+                if (backing)
+                    v0 = vR;
+                else
+                    v = vR;
+            } else {
+                if (tBack > 0.0f) tBack = 0.0f;
+                backing = false;
+            }
+        }
+        if (tBack >= dt) {
+            v = vR;
+            const float A = a * (v - VC_VL);
+            w = (w0 - A) * vc_expf(-(dt - tsp) / tau_w) + A;
+            tBack -= dt;
+        }
+        first = false;
+        // tRef >= dt/1024 (checked on the host) ends the loop within 1025 passes
+    } while (backing && count <= 1024u);
+#pragma unroll
+    for (uint32_t ig = 0; ig < VC_MAX_NG; ++ig) {
+        x.gE[ig] = geN[ig];
+        x.hE[ig] = heN[ig];
+        x.gI[ig] = giN[ig];
+        x.hI[ig] = hiN[ig];
+    }
+    x.v = v;
+    if (p.iModel == 1) x.w = w;
+    x.tBack = tBack;
+    if (count > 0u) return sumTsp / ((float)count * dt) + (float)count;
+    return v;
+}
+
+// Stage d: the multi-spike form for types whose tRef < dt (compiled into
+// the kSt instances only), else the one-spike form.
+template <bool kSt>
+__device__ __forceinline__ float membrane(const vc_v1_params &p, const V1Const &k, int t,
+                                          float depC, NState &x, const FFAcc &acc, float gapS,
+                                          float gap, float n0, float n1, const vc_ff_conn &ff,
+                                          const float *lgnSInfo, uint32_t n, uint32_t step,
+                                          const V1Key &key) {
+    if constexpr (kSt) {
+        if (p.tRef[t] < p.dt)
+            return membrane_multi(p, k, t, depC, x, acc, gapS, gap, n0, n1, ff, lgnSInfo, n,
+                                  step, key);
+    }
+    return membrane_stage(p, k, t, depC, x, acc, gapS, gap, n0, n1);
 }
 
 // Spike records being written (persistent kernel).
@@ -1066,7 +1295,8 @@ __global__ __launch_bounds__(kWG) void k_v1_step(vc_v1_params p, V1Const k, vc_v
         if (s.gapS) gapS = s.gapS[gi];
         if (s.gap) gap = s.gap[gi];
     }
-    const float sInfo = membrane_stage(p, k, t, depC, x, acc, gapS, gap, n0, n1);
+    const float sInfo = membrane<kSt>(p, k, t, depC, x, acc, gapS, gap, n0, n1, ff, lgnSInfo, n,
+                                      step, v1_key(p, b, n));
     store_state(p, s, n, x);
     out_sInfo[n] = sInfo;
     const RecOut noOut{nullptr, 0u, 0u, 0u};
@@ -1211,7 +1441,8 @@ __global__ __launch_bounds__(kWG) void k_v1_multi(vc_v1_params p, V1Const k, vc_
         }
         float n0 = 0.0f, n1 = 0.0f, dn;
         if constexpr (kSt) noise_terms(p, t, key, it0 + q, tonic, last, n0, n1, dn);
-        const float sInfo = membrane_stage(p, k, t, depC, x, acc, 0.0f, 0.0f, n0, n1);
+        const float sInfo = membrane<kSt>(p, k, t, depC, x, acc, 0.0f, 0.0f, n0, n1, ff, lgn, n,
+                                          it0 + q, key);
         out_sInfo[q * outStride + n] = sInfo;
         const RecOut ro{words, rec_of((q + 1u) & 1u), ovf_of((q + 1u) & 1u),
                         (epoch << 16) + q + 2u};
@@ -1238,10 +1469,12 @@ __global__ void k_clear_masks(uint64_t *mask, uint32_t nTiles) {
     if (i < nTiles) mask[i] = 0ull;
 }
 
-// Voltage noise or synaptic failure on: the kernels' instances with the draws.
+// Voltage noise, synaptic failure or a refractory period shorter than dt
+// (several spikes per step): the kernels' instances with the draws and the
+// multi-spike update.
 bool has_noise_or_failure(const vc_v1_params *p) {
     for (uint32_t t = 0; t < p->nType; ++t) {
-        if (p->noisyDep[t] > 0.0f || p->synFailFF[t] > 0.0f) return true;
+        if (p->noisyDep[t] > 0.0f || p->synFailFF[t] > 0.0f || p->tRef[t] < p->dt) return true;
         for (uint32_t u = 0; u < p->nType; ++u)
             if (p->synFail[t * p->nType + u] > 0.0f) return true;
     }
@@ -1319,10 +1552,11 @@ int check_params(const vc_v1_params *p, const vc_rec_conn *rec) {
     }
     if (p->nE > p->blockSize) { vc_set_error("nE > blockSize"); return VC_E_INVALID; }
     for (uint32_t t = 0; t < p->nType; ++t) {
-        if (!(p->tRef[t] >= p->dt)) {
-            vc_set_error("tRef[%u]=%g < dt=%g: the multi-spike-per-step variant "
-                         "(compute_V_collect_spike_learnFF) is not built", t, p->tRef[t], p->dt);
-            return VC_E_UNSUPPORTED;
+        // tRef < dt selects the multi-spike update; a lower bound keeps its
+        // per-step loop short (at most 513 spikes in a step)
+        if (!(p->tRef[t] >= p->dt / 512.0f)) {
+            vc_set_error("tRef[%u]=%g must be at least dt/512 (dt=%g)", t, p->tRef[t], p->dt);
+            return VC_E_INVALID;
         }
         if (p->deltaT[t] < 0.0f) { vc_set_error("deltaT < 0"); return VC_E_INVALID; }
     }
@@ -1450,8 +1684,8 @@ extern "C" int vc_v1_step_snapshot(void *stream, const vc_v1_params *p, const vc
         return VC_E_INVALID;
     }
     if (has_gap(s, rec) || has_noise_or_failure(p)) {
-        vc_set_error("vc_v1_step_snapshot: the adjoint's forward has no gap junctions, noise "
-                     "or synaptic failure");
+        vc_set_error("vc_v1_step_snapshot: the adjoint's forward has no gap junctions, noise, "
+                     "synaptic failure or tRef < dt");
         return VC_E_UNSUPPORTED;
     }
     if (prev->mask == next->mask) {
