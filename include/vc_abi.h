@@ -335,6 +335,73 @@ int vc_v1_dep_current(void *stream, const vc_v1_params *p, const vc_v1_state *s,
 /* Initial V1 state: v, w, tBack, g/h as given; clears a spike buffer. */
 int vc_spikes_clear(void *stream, uint32_t nV1, const vc_spikes *sp);
 
+/* ---- initial state --------------------------------------------------- */
+/* The draws of the reference's initial V1 state, per type (cfg keys of the
+ * same names, src/patch.cu:225-229, 311): w0 [type][mean, std] (AdEx),
+ * gFF0 [type][ig][g mean, g std/mean, h mean, h std/mean], gE0, gI0 alike;
+ * useV0 (use_v0). */
+typedef struct {
+    uint32_t nV1, nblock, blockSize, nType;
+    uint32_t typeAcc[VC_MAX_NTYPE];
+    uint32_t ngFF, ngE, ngI;
+    int32_t iModel;
+    uint32_t useV0;
+    float vR[VC_MAX_NTYPE], vT[VC_MAX_NTYPE];
+    float w0[VC_MAX_NTYPE * 2];
+    float gFF0[VC_MAX_NTYPE * VC_MAX_NG * 4];
+    float gE0[VC_MAX_NTYPE * VC_MAX_NG * 4];
+    float gI0[VC_MAX_NTYPE * VC_MAX_NG * 4];
+} vc_v1_init;
+
+/* The reference's initial V1 state, drawn on the HOST (all pointers are host
+ * memory; no device work): src/patch.cu:4405-4570 with the same generator
+ * (std::default_random_engine seeded with `seed`, one
+ * std::normal_distribution<float> for every truncated-normal draw,
+ * std::uniform_real_distribution<float>) in the same order.  w (iModel 1)
+ * ~ N(w0); v = vR + (vT - vR) * tonicDep, or uniform in [vR, vT) when
+ * useV0 or max tonicDep > 1; gFF/hFF, gE/hE, gI/hI ~ N(mean, mean*relstd)
+ * redrawn while negative, gFF = hFF = 0 for a neuron without LGN input.
+ * Arrays are [ng][nV1].  The reference seeds this engine with its `seed`
+ * after the increments that precede it (seed + 1 + nV1 when the LGN
+ * parameters are read from fLGN, src/patch.cu:2686, 4224-4228). */
+int vc_v1_initial_state(uint64_t seed, const vc_v1_init *q, const float *tonicDep,
+                        const uint32_t *nLGN, float *v, float *w, float *gFF, float *hFF,
+                        float *gE, float *hE, float *gI, float *hI);
+
+/* Initial spikes of rand_spInit (src/coredynamics.cu:5-146): per type a
+ * chance per step, sp0[type*2 + 0] for a neuron with more than SCsplit LGN
+ * inputs and sp0[type*2 + 1] otherwise (the cfg's spE0/spI0 rates times
+ * dt/1000, src/patch.cu:811-837).  depthRef is the reference's trainDepth,
+ * the number of steps of spikes it generates. */
+typedef struct {
+    float sp0[VC_MAX_NTYPE * 2];
+    uint32_t SCsplit;
+    uint32_t depthRef;
+} vc_v1_spinit;
+
+/* rand_spInit on the device: a neuron spikes with its chance at a uniform
+ * time (reference spikeTrain slot 0), which puts v at vR, w (AdEx) and
+ * tBack past the spike; then slots depthRef-1, ..., 1 in that order each
+ * hold a spike with chance * (dt - ref)/dt outside the refractory period
+ * (the neuron's v otherwise, vR inside it).  Reference slot j holds step
+ * j - depthRef of the run (the reference's pre-loop recal_G reads slot
+ * depthRef - 1 as the step before the first), written to ring slot (cur0 +
+ * depth - depthRef + j) % depth of train [depth][nV1] (cur0: the ring slot
+ * of step 0; depth >= depthRef; older slots get v) with its spike mask in
+ * trainMask [depth][nV1/64] (may be NULL).  Updates s->v, s->w, s->tBack.
+ * Draws: Philox counter (neuron, reference slot, VC_STREAM_V1_INIT, trial)
+ * in place of cuRAND's per-neuron stream. */
+int vc_v1_initial_spikes(void *stream, const vc_v1_params *p, const vc_v1_state *s,
+                         const uint32_t *nLGN, const vc_v1_spinit *q, float *train,
+                         uint64_t *trainMask, uint32_t depth, uint32_t cur0);
+
+/* One step's outputs sInfo[nV1] as a spike buffer: each tile's spike mask
+ * and every spike's unit increments at slot tile*64 + rank, as the step
+ * kernels write them; used for the spikes the first step gathers (the
+ * reference's pre-loop recal_G, src/patch.cu:5687-5770). */
+int vc_spikes_compact(void *stream, const vc_v1_params *p, const float *sInfo,
+                      const vc_spikes *sp);
+
 /* ---- axonal delays (noDelay = 0) ----------------------------------------
  * The reference's recal_G_mat, src/coredynamics.cu:2062-2309 (launched at
  * src/patch.cu:6518-6527 in place of recal_G_mat_nd_fast): each spike reaches

@@ -46,6 +46,9 @@ class Oracle:
         L.orc_v1_rec_dense.argtypes = [P(abi_mod.V1Params)] + [vp] * 4 + [u32] + [vp] * 4 + [u32]
         L.orc_v1_rec_dense.restype = i32
         L.orc_set_multi_all.argtypes = [i32]
+        L.orc_v1_initial_spikes.argtypes = [P(abi_mod.V1Params), vp, P(abi_mod.V1SpInit), vp, vp,
+                                            vp, vp, u32, u32]
+        L.orc_v1_initial_spikes.restype = i32
         L.orc_lgn_frame_response.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp,
                                              C.c_uint64, vp]
         L.orc_lgn_steps.argtypes = [P(abi_mod.LgnParams)] + [vp] * 11 + [u32, u32, vp, vp, vp]
@@ -192,9 +195,11 @@ class OracleSim:
                                       "delays is not built")
         if self.delay or self.far_only:             # rand_spInit: every slot holds v
             self.train[:] = self.st["v"][None, :]
+        init_row = self.st["v"][None, :].copy()
+        if state is None and params.has_initial_spikes(sp):
+            init_row = self._initial_spikes(params)
         if self.gap is not None:
-            self._gap(self.train if self.delay else self.st["v"][None, :].copy(),
-                      self.depth - 1 if self.delay else 0)
+            self._gap(self.train if self.delay else init_row, self.depth - 1 if self.delay else 0)
         self.lTR = np.zeros(sp.nLGN, np.float32)
         self.lNL = np.zeros(sp.nLGN, np.float32)
         self.o.L.orc_lgn_init(C.byref(self.lp), _fp(self.lTR), _fp(self.lNL))
@@ -211,6 +216,47 @@ class OracleSim:
         self.lgn_hist = []
         self.fr_hist = []
         self.convol_hist = []
+
+    def _initial_spikes(self, params):
+        """rand_spInit (orc_v1_initial_spikes) over the reference's trainDepth
+        steps before the first, then the pre-loop recal_G of the step before
+        the first (src/patch.cu:5687-5770): the delayed gather at its ring
+        slot, or without delays the dense gather of its spikes (their
+        failure draws keyed by step -1).  Returns that step's outputs."""
+        net, sp, st = self.net, self.net.spec, self.st
+        if self.batch is not None or self.far_only or (self.delay and self.far_on):
+            raise NotImplementedError("initial spikes with batched trials or far connections")
+        D = params.reference_train_depth(net)
+        q = params.spinit_struct(net, D)
+        if self.delay:
+            rc = self.o.L.orc_v1_initial_spikes(C.byref(self.p), _fp(self.nLGN_of), C.byref(q),
+                                                _fp(st["v"]), _fp(st["w"]), _fp(st["tBack"]),
+                                                _fp(self.train), self.depth, 0)
+            if rc != 0:
+                raise RuntimeError(f"orc_v1_initial_spikes returned {rc}")
+            rc = self.o.L.orc_v1_rec_delay(C.byref(self.p), _fp(st["gE"]), _fp(st["hE"]),
+                                           _fp(st["gI"]), _fp(st["hI"]), self.net.maxNear,
+                                           _fp(self.nNear), _fp(self.nbr), _fp(self.conMat),
+                                           _fp(self.delayMat), sp.speedOfThought,
+                                           _fp(self.train), self.depth, self.depth - 1)
+            if rc != 0:
+                raise RuntimeError(f"orc_v1_rec_delay returned {rc}")
+            return self.train[self.depth - 1][None, :].copy()
+        ring = np.zeros((D, sp.nV1), np.float32)
+        rc = self.o.L.orc_v1_initial_spikes(C.byref(self.p), _fp(self.nLGN_of), C.byref(q),
+                                            _fp(st["v"]), _fp(st["w"]), _fp(st["tBack"]),
+                                            _fp(ring), D, 0)
+        if rc != 0:
+            raise RuntimeError(f"orc_v1_initial_spikes returned {rc}")
+        row = np.ascontiguousarray(ring[D - 1])
+        rc = self.o.L.orc_v1_rec_dense(C.byref(self.p), _fp(st["gE"]), _fp(st["hE"]),
+                                       _fp(st["gI"]), _fp(st["hI"]), self.net.maxNear,
+                                       _fp(self.nNear), _fp(self.nbr), _fp(self.conMat), _fp(row),
+                                       0xffffffff)
+        if rc != 0:
+            raise RuntimeError(f"orc_v1_rec_dense returned {rc}")
+        self.init_ring = ring
+        return row[None, :].copy()
 
     def _gap(self, train, cur):
         """The gap sums from the step outputs in ring slot `cur` of train."""

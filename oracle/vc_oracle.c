@@ -617,6 +617,76 @@ static int orc_multi_all = 0;
 void orc_set_multi_all(int on) { orc_multi_all = on; }
 
 /*
+ * rand_spInit, src/coredynamics.cu:5-146 (initial spikes and the spike-train
+ * ring).  Per neuron: type t, chance = sp0[t*2 + 0] when it has more than
+ * SCsplit LGN inputs, else sp0[t*2 + 1]; rand < chance spikes at tsp (ring
+ * slot 0 = 1 + tsp; v = vR; tBack = tRef - (1 - tsp) dt if positive; AdEx w
+ * advanced as :75-79 writes it, A = a (vR - vL) tau_w); then reference
+ * slots i = depthRef-1 down to 1: outside the refractory period (ref < dt)
+ * a spike with chance * (dt - ref)/dt at (dt - ref)/dt of a uniform, else
+ * the neuron's v; inside it vR; ref -= dt after each slot.  Uniforms from
+ * Philox counter (neuron, reference slot, VC_STREAM_V1_INIT, trial), word 0
+ * for the test, word 1 for the time, in (0, 1] as curand_uniform.
+ * Reference slot j goes to ring slot (cur0 + depth - depthRef + j) % depth
+ * of train [depth][nV1]; ring slots of steps before -depthRef get v.
+ */
+int orc_v1_initial_spikes(const vc_v1_params *p, const uint32_t *nLGN, const vc_v1_spinit *q,
+                          float *v, float *w, float *tBack, float *train, uint32_t depth,
+                          uint32_t cur0) {
+    const uint32_t nV1 = p->nV1, D = q->depthRef;
+    if (D == 0 || depth < D || cur0 >= depth) return VC_E_INVALID;
+    for (uint32_t n = 0; n < nV1; ++n) {
+        const orc_key key = orc_key_of(p, n);
+        const int t = orc_type_of(p, n % p->blockSize);
+        const uint32_t iLGN = nLGN ? nLGN[n] : 0;
+        const float chance = iLGN > q->SCsplit ? q->sp0[t * 2 + 0] : q->sp0[t * 2 + 1];
+        const float dt = p->dt, tRef = p->tRef[t];
+        vc_u32x4 r = vc_draw4(key.lo, key.hi, key.ent, 0, VC_STREAM_V1_INIT, 0, p->trial);
+        float rnd = vc_u01_open0(r.v[0]);
+        float ref = 0.0f;
+        float sInfo = v[n];
+        if (chance > 0.0f) {
+            if (rnd < chance) {
+                float tsp = vc_u01_open0(r.v[1]);
+                sInfo = 1.0f + tsp;
+                float tb = tRef - (1.0f - tsp) * dt;
+                if (tb > 0.0f) tBack[n] = tb;
+                float v0 = p->vR[t];
+                v[n] = v0;
+                if (p->iModel == 1) {
+                    float A = p->a[t] * (v0 - ORC_VL) * p->tau_w[t];
+                    float w0 = w[n] + p->b[t];
+                    w[n] = (w0 - A) * vc_expf(-dt * (1.0f - tsp) / p->tau_w[t]) + A;
+                }
+                ref = (1.0f - tsp) * dt + tRef - dt;
+            }
+        }
+        train[(size_t)((cur0 + depth - D) % depth) * nV1 + n] = sInfo;
+        for (uint32_t i = D - 1; i > 0; --i) {
+            float val;
+            if (ref < dt) {
+                if (ref < 0.0f) ref = 0.0f;
+                r = vc_draw4(key.lo, key.hi, key.ent, i, VC_STREAM_V1_INIT, 0, p->trial);
+                if (vc_u01_open0(r.v[0]) < chance * (dt - ref) / dt) {
+                    float tsp = vc_u01_open0(r.v[1]) * (dt - ref) / dt;
+                    val = 1.0f + tsp;
+                    ref = tRef + (1.0f - tsp) * dt;
+                } else {
+                    val = v[n];
+                }
+            } else {
+                val = p->vR[t];
+            }
+            ref -= dt;
+            train[(size_t)((cur0 + depth - D + i) % depth) * nV1 + n] = val;
+        }
+        for (uint32_t k = D; k < depth; ++k)
+            train[(size_t)((cur0 + depth - 1 - k) % depth) * nV1 + n] = v[n];
+    }
+    return VC_OK;
+}
+
+/*
  * recal_G_mat_nd_fast (src/coredynamics.cu:2609-2922, the gather at
  * :2721-2916): every post neuron sums, over its neighbour slots ib and the
  * E list then the I list of block nbr[b][ib] (ipre/npre, og/oh in the

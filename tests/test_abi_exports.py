@@ -38,9 +38,10 @@ def test_struct_sizes_match_header_layout(pkg):
 #include <stdio.h>
 #include "vc_abi.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %d\n", sizeof(vc_v1_params), sizeof(vc_v1_state),
-         sizeof(vc_ff_conn), sizeof(vc_rec_conn), sizeof(vc_spikes), sizeof(vc_lgn_setup),
-         sizeof(vc_lgn_params), sizeof(vc_lgn_state), sizeof(vc_v1_adjoint), VC_SNAP_ROWS);
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %d\n", sizeof(vc_v1_params),
+         sizeof(vc_v1_state), sizeof(vc_ff_conn), sizeof(vc_rec_conn), sizeof(vc_spikes),
+         sizeof(vc_lgn_setup), sizeof(vc_lgn_params), sizeof(vc_lgn_state), sizeof(vc_v1_adjoint),
+         sizeof(vc_v1_init), sizeof(vc_v1_spinit), VC_SNAP_ROWS);
   return 0; }
 '''
     with tempfile.TemporaryDirectory() as d:
@@ -52,7 +53,7 @@ int main(void) {
                                              check=True).stdout.split()))
     mine = [ctypes.sizeof(t) for t in (abi.V1Params, abi.V1State, abi.FFConn, abi.RecConn,
                                        abi.Spikes, abi.LgnSetup, abi.LgnParams, abi.LgnState,
-                                       abi.V1Adjoint)] + [abi.VC_SNAP_ROWS]
+                                       abi.V1Adjoint, abi.V1Init, abi.V1SpInit)] + [abi.VC_SNAP_ROWS]
     assert sizes == mine
 
 

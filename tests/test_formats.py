@@ -41,8 +41,19 @@ def test_cfg_semantics(pkg):
     assert config.spec_from_config(config.Config("")).noiseOnTonic == 1   # the default
     with pytest.raises(ValueError):
         config.spec_from_config(config.Config("synFail = 0.1, 0.1"))     # needs nType x nType
-    with pytest.raises(NotImplementedError):
-        config.spec_from_config(config.Config("spE0 = 0.1, 0"))
+    sp = config.spec_from_config(config.Config("spE0 = 0.1, 0\nspI0 = 2, 3\nSCsplit = 4"))
+    assert sp.spE0 == (0.1, 0.0) and sp.spI0 == (2.0, 3.0) and sp.SCsplit == 4
+    with pytest.raises(ValueError):
+        config.spec_from_config(config.Config("spE0 = 0.1"))             # [type][simple, complex]
+    sp = config.spec_from_config(config.Config(
+        "gFF0 = 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8\n"
+        "gFF0 = 1.1, 1.2, 1.3, 1.4, 1.5, 1.6, 1.7, 1.8\n"
+        "gE0 = 2, 0.1, 3, 0.2, 2, 0.1, 3, 0.2\ngE0 = 4, 0.1, 5, 0.2, 4, 0.1, 5, 0.2\n"
+        "gI0 = 6, 0.1, 7, 0.2\ngI0 = 8, 0.1, 9, 0.2\nw0 = 0.5, 0.1, 0.2, 0.05\nuse_v0 = true"))
+    assert sp.init_gFF0[8] == 1.1 and sp.init_gI0 == (6, 0.1, 7, 0.2, 8, 0.1, 9, 0.2)
+    assert sp.init_w0 == (0.5, 0.1, 0.2, 0.05) and sp.use_v0 == 1
+    with pytest.raises(ValueError):              # the reference requires all three with w0
+        config.spec_from_config(config.Config("gE0 = 2, 0.1, 3, 0.2, 2, 0.1, 3, 0.2"))
     sp = config.spec_from_config(config.Config("noDelay = 0\nnoFarDelay = 0\nspeedOfThought = 0.5"))
     assert sp.noDelay == 0 and sp.speedOfThought == 0.5 and sp.noFarDelay == 0
     sp = config.spec_from_config(config.Config(""))

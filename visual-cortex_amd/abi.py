@@ -89,6 +89,23 @@ class Spikes(C.Structure):
     _fields_ = [("mask", vp), ("og", vp), ("oh", vp)]
 
 
+class V1Init(C.Structure):
+    _fields_ = [
+        ("nV1", u32), ("nblock", u32), ("blockSize", u32), ("nType", u32),
+        ("typeAcc", _arr(u32, VC_MAX_NTYPE)), ("ngFF", u32), ("ngE", u32), ("ngI", u32),
+        ("iModel", i32), ("useV0", u32),
+        ("vR", _arr(f32, VC_MAX_NTYPE)), ("vT", _arr(f32, VC_MAX_NTYPE)),
+        ("w0", _arr(f32, VC_MAX_NTYPE * 2)),
+        ("gFF0", _arr(f32, VC_MAX_NTYPE * VC_MAX_NG * 4)),
+        ("gE0", _arr(f32, VC_MAX_NTYPE * VC_MAX_NG * 4)),
+        ("gI0", _arr(f32, VC_MAX_NTYPE * VC_MAX_NG * 4)),
+    ]
+
+
+class V1SpInit(C.Structure):
+    _fields_ = [("sp0", _arr(f32, VC_MAX_NTYPE * 2)), ("SCsplit", u32), ("depthRef", u32)]
+
+
 class LgnSetup(C.Structure):
     _fields_ = [
         ("nLGN", u32), ("nSample1D", u32), ("nKernelSample", u32),
@@ -147,6 +164,11 @@ SIGNATURES = {
     "vc_v1_rec_apply_at": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn),
                            C.POINTER(Spikes), u32],
     "vc_spikes_clear": [vp, u32, C.POINTER(Spikes)],
+    "vc_v1_initial_state": [C.c_uint64, C.POINTER(V1Init), vp, vp, vp, vp, vp, vp, vp, vp,
+                            vp, vp],
+    "vc_v1_initial_spikes": [vp, C.POINTER(V1Params), C.POINTER(V1State), vp,
+                             C.POINTER(V1SpInit), vp, vp, u32, u32],
+    "vc_spikes_compact": [vp, C.POINTER(V1Params), vp, C.POINTER(Spikes)],
     "vc_v1_rec_distances": [vp, C.POINTER(V1Params), C.POINTER(RecConn), vp, vp],
     "vc_v1_rec_delay_table": [vp, C.c_uint64, vp, vp, C.c_float, C.c_float, vp],
     "vc_v1_rec_delay": [vp, C.POINTER(V1Params), C.POINTER(V1State), C.POINTER(RecConn), vp,

@@ -214,10 +214,32 @@ def spec_from_config(cfg: Config, base=None):
         kw["noiseOnTonic"] = 1 if cfg.bool("noiseOnTonic") else 0
     if cfg.has("tau_noise"):
         kw["tau_noise"] = cfg.float("tau_noise")
+    # initial spikes: [simple, complex] rates per type (Hz), src/patch.cu:270-272, 811-837
     for key in ("spE0", "spI0"):
         v = cfg.floats(key)
-        if v is not None and any(x != 0 for x in v):
-            raise NotImplementedError(f"{key} != 0 is not built (the configs of this path use 0)")
+        if v is not None:
+            if len(v) != 2:
+                raise ValueError(f"the size of {key} is {len(v)} != 1 x 2")
+            kw[key] = tuple(v)
+    if cfg.has("SCsplit"):
+        kw["SCsplit"] = cfg.int("SCsplit")
+    # the initial-state draws (src/patch.cu:225-229, 311, 4405-4570); the
+    # reference requires gFF0, gE0 and gI0 (and w0 with AdEx), so a cfg that
+    # gives them gets the reference's initial state
+    for key, per in (("w0", 2), ("gFF0", 4 * ngFF), ("gE0", 4 * ngE), ("gI0", 4 * ngI)):
+        v = cfg.floats(key)
+        if v is not None:
+            if len(v) != 2 * per:
+                raise ValueError(f"size of {key} is {len(v)}, not nType x {per}")
+            kw["init_" + key] = tuple(v)
+    if any(k in kw for k in ("init_gFF0", "init_gE0", "init_gI0")):
+        for key in ("init_gFF0", "init_gE0", "init_gI0"):
+            if key not in kw:
+                raise ValueError(f"{key[5:]} is required with the other initial distributions")
+        if kw.get("iModel", sp.iModel) == 1 and "init_w0" not in kw:
+            raise ValueError("w0 is required for the AdEx model (iModel 1)")
+    if cfg.has("use_v0"):
+        kw["use_v0"] = 1 if cfg.bool("use_v0") else 0
     if cfg.has("noDelay"):
         kw["noDelay"] = cfg.int("noDelay")
     if cfg.has("speedOfThought"):

@@ -28,6 +28,11 @@ from .network import Network, logistic_ab
 
 
 def load_network(cfg: cfgmod.Config) -> Network:
+    # the reference draws new LGN parameters when useNewLGN is on (its
+    # default, src/patch.cu:313, 1754-2459); this build reads them from fLGN
+    if cfg.bool("useNewLGN", True):
+        raise NotImplementedError("useNewLGN = true (drawing new LGN parameters) is not built: "
+                                  "set useNewLGN = false to read them from fLGN")
     spec = cfgmod.spec_from_config(cfg)
     nblock, bs, pos = F.read_v1_allpos(cfg.path("fV1_allpos"))
     nType, nTypeI, typeAcc = F.read_connectome_cfg(cfg.path("fConnectome_cfg"))
@@ -345,7 +350,7 @@ def save_network(net: Network, folder, suffix: str = "vc", tonic: dict | None = 
     ]
     for k in ("vR", "vThres", "gL", "C", "tRef", "vT", "deltaT", "tau_w", "a", "b", "tonicDep"):
         lines.append(f"{k} = {t(getattr(sp, k))}")
-    keys = dict(CmoreDep="false", minTonicRatio="1.0", readFeature="false")
+    keys = dict(CmoreDep="false", minTonicRatio="1.0", readFeature="false", useNewLGN="false")
     keys.update({k: v for k, v in tonic.items() if k not in ("ffRatio", "feature")})
     for k, v in keys.items():
         if isinstance(v, (list, tuple)):

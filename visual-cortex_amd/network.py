@@ -144,6 +144,21 @@ class ModelSpec:
     # seeds
     seed: int = 1234
     trial: int = 0
+    # the reference's initial-state draws (cfg keys w0, gFF0, gE0, gI0,
+    # use_v0; src/patch.cu:4405-4570): None keeps the synthetic initial state
+    # (v uniform below vT, w and conductances 0); otherwise w0 is [type][mean,
+    # std], gFF0/gE0/gI0 [type][ig][g mean, g std/mean, h mean, h std/mean]
+    init_w0: tuple | None = None
+    init_gFF0: tuple | None = None
+    init_gE0: tuple | None = None
+    init_gI0: tuple | None = None
+    use_v0: int = 0
+    # initial spikes of rand_spInit (src/coredynamics.cu:5-146): per type the
+    # rate in Hz for neurons with more than SCsplit LGN inputs, then for the
+    # others (cfg spE0, spI0, SCsplit)
+    spE0: tuple = (0.0, 0.0)
+    spI0: tuple = (0.0, 0.0)
+    SCsplit: int = 0
     # set when the network comes from the reference's input files
     nLGN_override: int = 0
     stim_buffer_deg: float = 0.0

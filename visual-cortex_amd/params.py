@@ -82,14 +82,93 @@ def lgn_params_struct(net: Network) -> abi.LgnParams:
 
 
 def initial_state(net: Network) -> dict:
-    """Initial V1 state (host arrays): v uniform in [vR, vT), w, tBack, g = 0.
-    A batched network concatenates its trials' own initial states."""
+    """Initial V1 state (host arrays).  With the reference's initial
+    distributions in the spec (cfg keys w0, gFF0, gE0, gI0, use_v0) the
+    reference's own draws (vc_v1_initial_state); otherwise the synthetic
+    state: v uniform in [vR, vR + 0.8 (vT - vR)), w, tBack, g = 0.  A batched
+    network concatenates its trials' own initial states."""
     if getattr(net, "batch", None) is not None:
         parts = []
         for sp_t in net.trial_specs:
+            if sp_t.init_gFF0 is not None:
+                raise NotImplementedError("the reference's initial-state draws with batched "
+                                          "trials are not built")
             parts.append(_initial_state_spec(sp_t))
         return {k: np.concatenate([q[k] for q in parts], axis=-1) for k in parts[0]}
+    if net.spec.init_gFF0 is not None:
+        return reference_initial_state(net)
     return _initial_state_spec(net.spec)
+
+
+def init_struct(sp) -> abi.V1Init:
+    """vc_v1_init of a spec holding the reference's initial distributions."""
+    q = abi.V1Init()
+    q.nV1, q.nblock, q.blockSize, q.nType = sp.nV1, sp.nblock, sp.blockSize, 2
+    q.typeAcc[0], q.typeAcc[1] = sp.nE, sp.blockSize
+    q.ngFF, q.ngE, q.ngI = len(sp.riseTimeFF), len(sp.riseTimeE), len(sp.riseTimeI)
+    q.iModel = sp.iModel
+    q.useV0 = 1 if sp.use_v0 else 0
+    for t in range(2):
+        q.vR[t], q.vT[t] = sp.vR[t], sp.vT[t]
+    for name in ("w0", "gFF0", "gE0", "gI0"):
+        vals = getattr(sp, "init_" + name)
+        if vals is not None:
+            arr = getattr(q, name)
+            for i, x in enumerate(vals):
+                arr[i] = x
+    return q
+
+
+def reference_initial_state(net: Network) -> dict:
+    """The reference's initial V1 state (src/patch.cu:4405-4570), drawn by the
+    library's host function with the reference's generator and order, seeded
+    as the reference seeds it when the LGN parameters come from fLGN: the
+    cfg seed + 1 (logRand_init) + nV1 (the per-neuron engines),
+    src/patch.cu:2686, 4224-4228, 4406."""
+    sp = net.spec
+    lib = abi.load_library(require_gpu=False)
+    n = sp.nV1
+    ngFF, ngE, ngI = len(sp.riseTimeFF), len(sp.riseTimeE), len(sp.riseTimeI)
+    z = lambda *s: np.zeros(s, np.float32)
+    st = dict(v=z(n), w=z(n), tBack=z(n), gFF=z(ngFF, n), hFF=z(ngFF, n), gE=z(ngE, n),
+              hE=z(ngE, n), gI=z(ngI, n), hI=z(ngI, n))
+    tonic = np.ascontiguousarray(net.tonicDep, np.float32)
+    nLGN = np.ascontiguousarray(net.nLGN_of, np.uint32)
+    q = init_struct(sp)
+    seed = (int(sp.seed) + 1 + n) & 0xffffffffffffffff
+    P = lambda a: a.ctypes.data
+    abi.check(lib.vc_v1_initial_state(seed, q, P(tonic), P(nLGN), P(st["v"]), P(st["w"]),
+                                      P(st["gFF"]), P(st["hFF"]), P(st["gE"]), P(st["hE"]),
+                                      P(st["gI"]), P(st["hI"])), "vc_v1_initial_state")
+    return st
+
+
+def spinit_struct(net: Network, depth_ref: int) -> abi.V1SpInit:
+    """vc_v1_spinit: the cfg's initial spike rates as chances per step
+    (rate * dt / 1000, src/patch.cu:811-837), [type][simple, complex]."""
+    sp = net.spec
+    q = abi.V1SpInit()
+    rates = list(sp.spE0) + list(sp.spI0)
+    for i, r in enumerate(rates):
+        q.sp0[i] = float(np.float32(r) * (np.float32(sp.dt) / np.float32(1000.0)))
+    q.SCsplit = sp.SCsplit
+    q.depthRef = depth_ref
+    return q
+
+
+def has_initial_spikes(sp) -> bool:
+    return any(x > 0 for x in tuple(sp.spE0) + tuple(sp.spI0))
+
+
+def reference_train_depth(net: Network) -> int:
+    """The reference's trainDepth, ceil(maxDistance / speedOfThought / dt) + 1
+    over the connected near pairs (src/patch.cu:3807-3813), whatever noDelay
+    is: rand_spInit fills that many steps."""
+    from .network import train_depth
+    sp = net.spec
+    nnz = net.n_rec_synapses_csr()
+    dmax = float(net.rec_distances()[:nnz].max()) if nnz else 0.0
+    return train_depth(dmax, sp.speedOfThought, sp.dt)
 
 
 def _initial_state_spec(sp) -> dict:

@@ -22,7 +22,8 @@ import torch
 from . import abi
 from .abi import check, ptr
 from .network import Network, max_sample_1d, timing, view_geometry
-from .params import initial_state, lgn_params_struct, v1_params_struct
+from .params import (has_initial_spikes, initial_state, lgn_params_struct,
+                     reference_train_depth, spinit_struct, v1_params_struct)
 
 
 class GpuSim:
@@ -180,11 +181,16 @@ class GpuSim:
         self.lgn_hist: list[torch.Tensor] = []
         self.sInfo_step = torch.zeros(sp.nV1, dtype=torch.float32, device=d)
         self.sInfo_last = self.sInfo_step          # V1 output of the last step run (a view)
+        # rand_spInit's initial spikes and the pre-loop recal_G of them (a run
+        # that starts from a given state has none, as a restored one)
+        self.init_row = self.state["v"]
+        if state is None and has_initial_spikes(sp):
+            self._initial_spikes()
         if self.gap_on:              # the pre-loop recal_G's gap sums (initial voltages)
             if self.delay:
                 self._gap_after(self.train, (self.slot_off + self.depth - 1) % self.depth)
             else:
-                self._gap_after(self.state["v"], 0)
+                self._gap_after(self.init_row, 0)
         self.s_lgn.wait_stream(self.s_v1)          # set-up work precedes the pipeline
         # launch shape of the one-step kernel (runs whose grid cannot be
         # resident at once): timed per workload when asked, else the default
@@ -192,6 +198,44 @@ class GpuSim:
         if tune:
             from .tune import tune_step
             self.shape = tune_step(self)
+
+    def _initial_spikes(self):
+        """rand_spInit (vc_v1_initial_spikes, src/coredynamics.cu:5-146) over
+        the reference's trainDepth steps before the first, then the pre-loop
+        recal_G (src/patch.cu:5687-5770), which gathers the step before the
+        first: with axonal delays vc_v1_rec_delay at that step's ring slot;
+        without, its spikes become the spike buffer the first step gathers."""
+        net, sp, d = self.net, self.net.spec, self.dev
+        if self.batch is not None:
+            raise NotImplementedError("initial spikes (spE0/spI0) with batched trials are not "
+                                      "built")
+        if net.n_far() > 0 and net.far_delayed():
+            raise NotImplementedError("initial spikes (spE0/spI0) with far connections are not "
+                                      "built")
+        D = reference_train_depth(net)
+        q = spinit_struct(net, D)
+        h = self._hs(self.s_v1)
+        if self.delay:
+            if D > self.depth:
+                raise ValueError(f"spike-train ring of {self.depth} steps < trainDepth {D}")
+            check(self.lib.vc_v1_initial_spikes(h, C.byref(self.p), C.byref(self.s),
+                                                ptr(self.nLGN_of), C.byref(q), ptr(self.train),
+                                                ptr(self.trainMask), self.depth, self.slot_off),
+                  "vc_v1_initial_spikes")
+            prev = (self.slot_off + self.depth - 1) % self.depth
+            check(self.lib.vc_v1_rec_delay(h, C.byref(self.p), C.byref(self.s), C.byref(self.rec),
+                                           ptr(self.syn), ptr(self.train), ptr(self.trainMask),
+                                           self.depth, prev, self.window, ptr(self.status)),
+                  "vc_v1_rec_delay")
+        else:
+            ring = torch.empty(D, sp.nV1, dtype=torch.float32, device=d)
+            check(self.lib.vc_v1_initial_spikes(h, C.byref(self.p), C.byref(self.s),
+                                                ptr(self.nLGN_of), C.byref(q), ptr(ring), None,
+                                                D, 0), "vc_v1_initial_spikes")
+            self.init_row = ring[D - 1]
+            check(self.lib.vc_spikes_compact(h, C.byref(self.p), ptr(self.init_row),
+                                             C.byref(self.spk[self.cur])), "vc_spikes_compact")
+            self._init_ring = ring
 
     def _setup_delay(self):
         from .network import train_depth
