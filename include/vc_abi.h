@@ -623,6 +623,11 @@ typedef struct {
     const uint32_t *cellTrial;
     const uint32_t *trialLgnOff;
     const uint32_t *trialSeed;
+    /* Frames per ntPerFrame steps (find_denorm, src/util/util.h:250-278;
+     * 0 is taken as 1): time is counted in units of dt/denorm and frame f
+     * covers [f ntPerFrame, (f+1) ntPerFrame) of them, i.e. steps
+     * [f ntPerFrame/denorm, (f+1) ntPerFrame/denorm) (src/patch.cu:4999-5030). */
+    uint32_t denorm;
 } vc_lgn_params;
 
 typedef struct {
@@ -634,9 +639,11 @@ typedef struct {
 } vc_lgn_state;
 
 /* Steps it0 .. it0+nsteps-1 of stages a (temporal part) and b for every LGN
- * cell; the steps must lie within one frame period (no frame enters after
- * it0).  Rring is [maxFrame][nLGN][2]: the spatial responses of frame f are
- * in slot f mod maxFrame (pre-stimulus frames f < 0 included).  Writes
+ * cell; the steps must lie within one period of ntPerFrame steps (denorm
+ * frames).  Rring is [maxFrame][nLGN][2]: the spatial responses of frame f
+ * are in slot f mod maxFrame (pre-stimulus frames f < 0 included); tap i of
+ * step it samples frame floor((it + 1 - nRetrace + iKernelSampleT0 + i
+ * kernelSampleInterval) denorm / ntPerFrame), which must be in the ring.  Writes
  * sInfo[nsteps][nLGN] (nsp + mean tsp/dt) and fr[nsteps][nLGN] (Hz; required,
  * it is also the hand-over buffer between the rate and spike kernels).  If ev
  * is not NULL, every spike of step it0+s is also delivered as bits of

@@ -203,7 +203,11 @@ class OracleSim:
         self.lTR = np.zeros(sp.nLGN, np.float32)
         self.lNL = np.zeros(sp.nLGN, np.float32)
         self.o.L.orc_lgn_init(C.byref(self.lp), _fp(self.lTR), _fp(self.lNL))
-        self.Rring = np.zeros((self.tm["maxFrame"], sp.nLGN, 2), np.float32)
+        # a period of ntPerFrame steps holds denorm frames; its steps reach back
+        # over maxFrame - 1 frames before the first of them
+        self.ringF = self.tm["maxFrame"] - 1 + self.tm["denorm"]
+        self.lp.maxFrame = self.ringF
+        self.Rring = np.zeros((self.ringF, sp.nLGN, 2), np.float32)
         init = np.empty((3, sp.height, sp.width), np.float32)
         for c in range(3):
             init[c] = net.init_lms[c]
@@ -323,13 +327,14 @@ class OracleSim:
             it = self.it
             k_in = it % npf
             if k_in == 0:
-                fidx = it // npf
-                if self.batch is not None:                   # frames [ntrial][nF][3][H][W]
-                    self.Rring[fidx % tm["maxFrame"]] = self.frame_response(
-                        frames[:, fidx % frames.shape[1]], per_trial=True)
-                else:
-                    self.Rring[fidx % tm["maxFrame"]] = self.frame_response(
-                        frames[fidx % frames.shape[0]])
+                den = tm["denorm"]
+                for fidx in range((it // npf) * den, (it // npf + 1) * den):
+                    if self.batch is not None:               # frames [ntrial][nF][3][H][W]
+                        self.Rring[fidx % self.ringF] = self.frame_response(
+                            frames[:, fidx % frames.shape[1]], per_trial=True)
+                    else:
+                        self.Rring[fidx % self.ringF] = self.frame_response(
+                            frames[fidx % frames.shape[0]])
                 cv = np.zeros((npf, sp.nLGN), np.float32) if lgn_only else None
                 self.o.L.orc_lgn_steps(C.byref(self.lp), _fp(t["TW"]), _fp(kC), _fp(kS),
                                        _fp(t["maxConvol"]), _fp(self.lgn["c50"]),

@@ -1248,9 +1248,12 @@ static float orc_lgn_cell_step(const vc_lgn_params *p, const float *TW,
     float prod[2][1024];
     for (int r = 0; r < 2; ++r) {
         for (uint32_t i = 0; i < nKS; ++i) {
+            /* the tap's time in units of dt/denorm (src/patch.cu:4999-5030):
+             * frame f covers [f ntPerFrame, (f+1) ntPerFrame) of them */
             int64_t tau = (int64_t)it + 1 - (int64_t)p->nRetrace +
                           (int64_t)p->iKernelSampleT0 + (int64_t)i * p->kernelSampleInterval;
-            int64_t f = orc_floor_div(tau, (int64_t)p->ntPerFrame);
+            int64_t den = p->denorm ? (int64_t)p->denorm : 1;
+            int64_t f = orc_floor_div(tau * den, (int64_t)p->ntPerFrame);
             int64_t slot = f % (int64_t)p->maxFrame;
             if (slot < 0) slot += p->maxFrame;
             float Rv = Rring[((size_t)slot * p->nLGN + c) * 2 + r];

@@ -72,6 +72,9 @@ TIMING_CASES = [
     (37.5, 30, 125),    # nRetrace 300, not a multiple of 64; interval 10, T0 5
     (35.0, 40, 100),    # nRetrace 280, interval 7 (odd: T0 0), 80 steps per frame
     (40.0, 32, 250),    # 32 steps per frame, interval 10, T0 5, 11 frames in the ring
+    # frames that are not a whole number of steps (denorm > 1, src/util/util.h:250-278):
+    (40.0, 40, 60),     # 8000 Hz / 60 Hz: denorm 3, 400 units of dt/3 per frame
+    (30.0, 30, 90),     # 8000 Hz / 90 Hz: denorm 9, interval 8, T0 4
 ]
 
 
@@ -87,13 +90,13 @@ def test_tap_frame_map_follows_reference_bookkeeping(pkg, tau, nks, rate):
     net = _net(pkg, tau=tau, nKernelSample=nks, frameRate=rate, nLGN_side=4)
     sp = net.spec
     tm = network.timing(sp)
-    npf, nR = tm["ntPerFrame"], tm["nRetrace"]
-    nsteps = nR + 4 * npf + npf // 2
-    nF = nsteps // npf + 3
+    npf, nR, den = tm["ntPerFrame"], tm["nRetrace"], tm["denorm"]
+    nsteps = nR + (4 * npf + npf // 2 if den == 1 else 2 * npf)
+    nF = nsteps * den // npf + 3
     net.frames = _noise_frames(sp, nF, seed=int(tau * 10) + nks)
     net.lgn["k"] = np.stack([np.ones(sp.nLGN, np.float32), np.zeros(sp.nLGN, np.float32)])
     ref = reference_tap_frames(nR, npf, nks, tm["kernelSampleInterval"], tm["iKernelSampleT0"],
-                               nsteps)
+                               nsteps, denorm=den)
     probe = sorted({0, 1, nks // 2, nks - 1})
     for i in probe:
         o = oracle.OracleSim(net, pkg)
@@ -114,6 +117,21 @@ def test_tap_frame_map_follows_reference_bookkeeping(pkg, tau, nks, rate):
         np.testing.assert_array_equal(used, ref[:, i], err_msg=f"tap {i}")
     # the map is not trivial: several frames per step window, and pre-stimulus taps
     assert (ref < 0).any() and len(np.unique(ref[-1])) >= 3
+
+
+def test_find_denorm_is_the_least_whole_frame_count():
+    """find_denorm's denorm (src/util/util.h:250-278) is the least number of
+    frames whose length is a whole number of steps, frameRate /
+    gcd(stepRate, frameRate), for every frame rate up to the step rate of
+    dt 0.125 ms and of a few others; ntPerFrame steps then hold exactly
+    denorm frames."""
+    from visual_cortex_amd import network
+    for stepRate in (8000, 10000, 1000, 2000):
+        for frameRate in range(1, 300):
+            den, _ = network.find_denorm(frameRate, stepRate)
+            assert den == frameRate // math.gcd(stepRate, frameRate), (stepRate, frameRate)
+            k = next(k for k in range(1, frameRate + 1) if (k * stepRate) % frameRate == 0)
+            assert den == k
 
 
 def test_reference_bookkeeping_edge_rule():

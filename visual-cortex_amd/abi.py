@@ -123,7 +123,7 @@ class LgnParams(C.Structure):
         ("iKernelSampleT0", u32), ("nRetrace", u32), ("ntPerFrame", u32), ("maxFrame", u32),
         ("dt", f32), ("convolRatio", f32), ("frRatio", f32), ("tRef", f32),
         ("seed_lo", u32), ("seed_hi", u32), ("trial", u32),
-        ("cellTrial", vp), ("trialLgnOff", vp), ("trialSeed", vp),
+        ("cellTrial", vp), ("trialLgnOff", vp), ("trialSeed", vp), ("denorm", u32),
     ]
 
 

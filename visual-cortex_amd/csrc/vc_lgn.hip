@@ -223,10 +223,12 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
     const uint32_t s = (wave - c * wpc) * 64u + (g & 63u);
     const uint32_t nKS = p.nKernelSample;
     const uint32_t it = it0 + (s < nsteps ? s : 0u);
-    const int64_t base = (int64_t)it + (int64_t)p.iKernelSampleT0 - (int64_t)p.nRetrace + 1;
+    // time in units of dt/denorm: frame f covers [f ntPerFrame, (f+1) ntPerFrame)
+    const int64_t den = p.denorm ? (int64_t)p.denorm : 1;
+    const int64_t base = ((int64_t)it + (int64_t)p.iKernelSampleT0 - (int64_t)p.nRetrace + 1) * den;
     // responses of the frames under the first step's window, wave-uniform
-    const int64_t fw = floor_div((int64_t)(it0 + (wave - c * wpc) * 64u) + (int64_t)p.iKernelSampleT0 -
-                                     (int64_t)p.nRetrace + 1, (int64_t)p.ntPerFrame);
+    const int64_t fw = floor_div(((int64_t)(it0 + (wave - c * wpc) * 64u) + (int64_t)p.iKernelSampleT0 -
+                                  (int64_t)p.nRetrace + 1) * den, (int64_t)p.ntPerFrame);
     // frame responses of this cell under the wavefront's windows, in LDS so
     // that each lane can pick its tap's frame with one read
     __shared__ float sR[256 / 64][2][kMaxFrame + 1];
@@ -246,7 +248,8 @@ __global__ __launch_bounds__(256) void k_lgn_rate(vc_lgn_params p, vc_lgn_state 
     const uint32_t rr0 = (uint32_t)(base - q0 * (int64_t)p.ntPerFrame);
     const int dd0 = (int)(q0 - fw);
     const uint32_t npf = p.ntPerFrame;
-    const uint32_t dq = p.kernelSampleInterval / npf, dr = p.kernelSampleInterval % npf;
+    const uint32_t tstep = p.kernelSampleInterval * (uint32_t)den;   // one tap, in dt/denorm
+    const uint32_t dq = tstep / npf, dr = tstep % npf;
     float conv[2];
 #pragma unroll 1
     for (int r = 0; r < 2; ++r) {              // one region at a time: half the registers
@@ -311,11 +314,12 @@ __global__ __launch_bounds__(64) void k_lgn_rate_cells(vc_lgn_params p, vc_lgn_s
     const bool valid = c < p.nLGN;
     const uint32_t cc = valid ? c : 0u;
     const uint32_t npf = p.ntPerFrame;
+    const uint32_t den = p.denorm ? p.denorm : 1u;           // time in units of dt/denorm
     // frames under the run's windows: from step 0's first tap on
     const int64_t base0 = (int64_t)it0 + (int64_t)p.iKernelSampleT0 - (int64_t)p.nRetrace + 1;
-    const int64_t fw = floor_div(base0, (int64_t)npf);
-    // the run's taps span at most (nsteps - 1 + nRetrace) / npf + 2 frames
-    const uint32_t nF = (nsteps - 1u + p.nRetrace) / npf + 2u;
+    const int64_t fw = floor_div(base0 * (int64_t)den, (int64_t)npf);
+    // the run's taps span at most (nsteps - 1 + nRetrace) denorm / npf + 2 frames
+    const uint32_t nF = ((nsteps - 1u + p.nRetrace) * den) / npf + 2u;
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -332,14 +336,15 @@ __global__ __launch_bounds__(64) void k_lgn_rate_cells(vc_lgn_params p, vc_lgn_s
         for (int i = 0; i < kNT; ++i) tw[r][i] = st.TW[((size_t)cc * 2 + r) * kNT + i];
     const float kC = st.kC[cc], kS = st.kS[cc], mx = st.maxConvol[cc];
     const float c50 = st.c50[cc], sh = st.sharpness[cc], lA = st.logA[cc], lB = st.logB[cc];
-    const uint32_t dq = p.kernelSampleInterval / npf, dr = p.kernelSampleInterval % npf;
+    const uint32_t tstep = p.kernelSampleInterval * den;
+    const uint32_t dq = tstep / npf, dr = tstep % npf;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     const uint32_t s1 = (blockIdx.y + 1u) * kStepsPerWave < nsteps ? (blockIdx.y + 1u) * kStepsPerWave
                                                                     : nsteps;
     for (uint32_t s = blockIdx.y * kStepsPerWave; s < s1; ++s) {
-        const int64_t base = base0 + (int64_t)s;
+        const int64_t base = (base0 + (int64_t)s) * (int64_t)den;
         const int64_t q0 = floor_div(base, (int64_t)npf);
         const uint32_t rr0 = (uint32_t)(base - q0 * (int64_t)npf);
         const int dd0 = (int)(q0 - fw);
@@ -820,14 +825,15 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
         vc_set_error("maxFrame %u / ntPerFrame %u must be positive", p->maxFrame, p->ntPerFrame);
         return VC_E_INVALID;
     }
+    const uint32_t den = p->denorm ? p->denorm : 1u;
     // one wavefront covers 64 steps: its taps span at most kMaxFrame + 1 frames
-    if ((63u + p->nRetrace) / p->ntPerFrame + 1u > (uint32_t)kMaxFrame) {
-        vc_set_error("a %u-step window over %u-step frames spans too many frames", p->nRetrace,
-                     p->ntPerFrame);
+    if ((63u + p->nRetrace) * den / p->ntPerFrame + 1u > (uint32_t)kMaxFrame) {
+        vc_set_error("a %u-step window over frames of %u/%u steps spans too many frames",
+                     p->nRetrace, p->ntPerFrame, den);
         return VC_E_UNSUPPORTED;
     }
     // the window spans nRetrace steps; every tap's frame must be within maxFrame of the first
-    if ((p->nRetrace + p->ntPerFrame - 1) / p->ntPerFrame + 1 > p->maxFrame) {
+    if ((p->nRetrace * den + p->ntPerFrame - 1) / p->ntPerFrame + 1 > p->maxFrame) {
         vc_set_error("maxFrame %u too small for a %u-step window", p->maxFrame, p->nRetrace);
         return VC_E_INVALID;
     }
@@ -836,7 +842,8 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
         return VC_E_INVALID;
     }
     if (nsteps > p->ntPerFrame - it0 % p->ntPerFrame) {
-        vc_set_error("vc_lgn_steps: steps %u..%u cross a frame boundary", it0, it0 + nsteps - 1);
+        vc_set_error("vc_lgn_steps: steps %u..%u cross a period boundary (%u steps)", it0,
+                     it0 + nsteps - 1, p->ntPerFrame);
         return VC_E_INVALID;
     }
     if (nsteps == 0 || p->nLGN == 0) return VC_OK;
@@ -877,9 +884,10 @@ extern "C" int vc_lgn_rates(void *stream, const vc_lgn_params *p, const vc_lgn_s
         vc_set_error("vc_lgn_rates: maxFrame and ntPerFrame must be positive");
         return VC_E_INVALID;
     }
-    if ((63u + p->nRetrace) / p->ntPerFrame + 1u > (uint32_t)kMaxFrame) {
-        vc_set_error("a %u-step window over %u-step frames spans too many frames", p->nRetrace,
-                     p->ntPerFrame);
+    if ((63u + p->nRetrace) * (p->denorm ? p->denorm : 1u) / p->ntPerFrame + 1u >
+        (uint32_t)kMaxFrame) {
+        vc_set_error("a %u-step window over frames of %u/%u steps spans too many frames",
+                     p->nRetrace, p->ntPerFrame, p->denorm);
         return VC_E_UNSUPPORTED;
     }
     if (nsteps == 0 || p->nLGN == 0) return VC_OK;

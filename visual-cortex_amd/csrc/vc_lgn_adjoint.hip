@@ -462,6 +462,11 @@ extern "C" int vc_lgn_rates_adjoint(void *stream, const vc_lgn_params *p, const 
                      "ntPerFrame %u)", p->nKernelSample, p->kernelSampleInterval, p->ntPerFrame);
         return VC_E_UNSUPPORTED;
     }
+    if (p->denorm > 1u) {
+        vc_set_error("vc_lgn_rates_adjoint: frames that are not a whole number of steps "
+                     "(denorm %u) are not built", p->denorm);
+        return VC_E_UNSUPPORTED;
+    }
     if (nsteps == 0 || p->nLGN == 0) return VC_OK;
     const int64_t npf = p->ntPerFrame, itv = p->kernelSampleInterval;
     const int64_t span = (int64_t)(p->nKernelSample - 1) * itv;

@@ -46,6 +46,9 @@ class LgnStage:
             self.dev = torch.device("cuda", torch.cuda.current_device())
         d = self.dev
         tm = timing(spec)
+        if tm["denorm"] != 1:
+            raise NotImplementedError("the LGN adjoint needs frames of a whole number of steps "
+                                      "(denorm 1)")
         self.tm = tm
         self.npf = tm["ntPerFrame"]
         self.nF = -(-self.T // self.npf)

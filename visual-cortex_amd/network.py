@@ -887,8 +887,10 @@ def logistic_ab(spont, c50, K):
 
 
 def n_frames(spec: ModelSpec) -> int:
-    steps_per_frame = round(1000.0 / spec.dt) // spec.frameRate
-    return (spec.nt + steps_per_frame - 1) // steps_per_frame + 1
+    """Stimulus frames a run of spec.nt steps reaches (frame f enters at the
+    first step whose end passes f * ntPerFrame / denorm steps), plus one."""
+    tm = timing(spec)
+    return (spec.nt * tm["denorm"] + tm["ntPerFrame"] - 1) // tm["ntPerFrame"] + 1
 
 
 def grating_frames(spec: ModelSpec, nframe: int | None = None) -> np.ndarray:
@@ -908,20 +910,43 @@ def grating_frames(spec: ModelSpec, nframe: int | None = None) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+def find_denorm(u1: int, u2: int) -> tuple[int, int]:
+    """(denorm, norm) of find_denorm(u1 = frameRate, u2 = stepRate, MorN =
+    true) (src/util/util.h:250-278): denorm is the least number of frames
+    whose length is a whole number of steps; norm/denorm the phase a frame
+    advances by.  With u2 % u1 = n != 0 it takes the largest i <= n dividing
+    both n and u1 (norm = n / i, denorm = u1 / i), and norm = 1, denorm = u1
+    when there is none above 1."""
+    if u2 % u1 == 0:
+        return 1, 1
+    m, n = u1, u2 - u1 * (u2 // u1)
+    for i in range(n, 1, -1):
+        if n % i == 0 and m % i == 0:
+            return m // i, n // i
+    return m, 1
+
+
 def timing(spec: ModelSpec):
-    """Kernel-sampling and frame bookkeeping (src/patch.cu:4949-5046)."""
+    """Kernel-sampling and frame bookkeeping (src/patch.cu:4949-5046).  Time
+    is counted in units of dt/denorm: a frame lasts ntPerFrame of them, so
+    frame f covers steps [f ntPerFrame / denorm, (f+1) ntPerFrame / denorm),
+    and ntPerFrame steps hold exactly denorm frames."""
     stepRate = round(1000.0 / spec.dt)
     if abs(stepRate * spec.dt - 1000.0) > 1e-9:
         raise ValueError("1000/dt must be an integer (src/patch.cu stepRate)")
-    if stepRate % spec.frameRate != 0:
-        raise NotImplementedError("stepRate must be a multiple of frameRate (denorm = 1)")
-    ntPerFrame = stepRate // spec.frameRate
+    if stepRate < spec.frameRate:
+        raise NotImplementedError("frameRate above the step rate (frames shorter than dt)")
+    denorm, _ = find_denorm(spec.frameRate, stepRate)
+    if (stepRate * denorm) % spec.frameRate != 0:          # src/patch.cu:5027
+        raise ValueError("stepRate * denorm is not a multiple of frameRate")
+    ntPerFrame = stepRate * denorm // spec.frameRate
     nRetrace = int(round(spec.tau / spec.dt))
     interval = nRetrace // spec.nKernelSample
     iT0 = interval // 2 if interval % 2 == 0 else 0
-    maxFrame = (nRetrace + ntPerFrame - 1) // ntPerFrame + 1
+    maxFrame = (nRetrace * denorm + ntPerFrame - 1) // ntPerFrame + 1   # src/patch.cu:5043
     return dict(stepRate=stepRate, ntPerFrame=ntPerFrame, nRetrace=nRetrace,
-                kernelSampleInterval=interval, iKernelSampleT0=iT0, maxFrame=maxFrame)
+                kernelSampleInterval=interval, iKernelSampleT0=iT0, maxFrame=maxFrame,
+                denorm=denorm)
 
 
 def max_sample_1d(net: Network) -> int:

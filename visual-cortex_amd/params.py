@@ -71,6 +71,7 @@ def lgn_params_struct(net: Network) -> abi.LgnParams:
     q.nRetrace = tm["nRetrace"]
     q.ntPerFrame = tm["ntPerFrame"]
     q.maxFrame = tm["maxFrame"]
+    q.denorm = tm["denorm"]
     q.dt = sp.dt
     q.convolRatio = sp.convolRatio
     q.frRatio = sp.frRatioLGN

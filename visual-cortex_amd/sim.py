@@ -426,10 +426,10 @@ class GpuSim:
         for c in range(3):
             init[c] = net.init_lms[c]
         self.init_frame = torch.as_tensor(init, device=d)
-        # the ring holds one frame more than the temporal kernel reaches back
-        # over, so the next frame's response can be built while the current
-        # period's steps still read the oldest one
-        self.ringF = tm["maxFrame"] + 1
+        # a period of ntPerFrame steps holds denorm frames; its steps reach back
+        # over maxFrame - 1 frames before the first of them, and the next
+        # period's denorm frames are built while they still run
+        self.ringF = tm["maxFrame"] - 1 + 2 * tm["denorm"]
         self.lp.maxFrame = self.ringF
         self.Rring = torch.zeros(self.ringF, sp.nLGN, 2, **f32)
         self._frame_response(self.init_frame, 0)          # on the current stream
@@ -507,9 +507,9 @@ class GpuSim:
             with torch.cuda.stream(st):
                 self.lgn_at[buf][0].copy_(self.lTR)
                 self.lgn_at[buf][1].copy_(self.lNL)
-        if self.fr_frame == k:                          # pipeline start: frame k whole
-            self._frame_cells(st, k, self.fr_cells, nL)
-            self.fr_frame, self.fr_cells = k + 1, 0
+        D = self.tm["denorm"]
+        for f, c0, c1 in self._frame_pieces((k + 1) * D * nL):   # period k's frames whole
+            self._frame_cells(st, f, c0, c1)
         ev = None
         if self.ev is not None:
             e2 = self.ev2[buf]
@@ -525,17 +525,30 @@ class GpuSim:
         e.record(st)
         self.lgn_evq.append((b, e, on_v1))
         self.lgn_to = b
-        c1 = b_in * nL // npf
-        if self.fr_frame == k + 1 and c1 > self.fr_cells:
+        # the share b_in / npf of the next period's frames
+        for f, c0, c1 in self._frame_pieces((k + 1) * D * nL + b_in * D * nL // npf):
             if on_v1:
-                self.slice_q.append((k + 1, self.fr_cells, c1, e))
+                self.slice_q.append((f, c0, c1, e))
             else:
-                self._frame_cells(st, k + 1, self.fr_cells, c1)
-            self.fr_cells = c1
-            if c1 == nL:
-                self.fr_frame, self.fr_cells = k + 2, 0
+                self._frame_cells(st, f, c0, c1)
         if not on_v1:
             self._lgn_mark_tail()
+
+    def _frame_pieces(self, end: int):
+        """The (frame, c0, c1) pieces of spatial-response work from the current
+        position (frame fr_frame, its first fr_cells cells done) up to the
+        position `end` = frame * nLGN + cells, in order; advances the position."""
+        nL = self.net.spec.nLGN
+        out = []
+        while self.fr_frame * nL + self.fr_cells < end:
+            f, c0 = self.fr_frame, self.fr_cells
+            c1 = min(nL, end - f * nL)
+            out.append((f, c0, c1))
+            if c1 == nL:
+                self.fr_frame, self.fr_cells = f + 1, 0
+            else:
+                self.fr_cells = c1
+        return out
 
     def _lgn_mark_tail(self):
         """An event after the LGN stream's latest work (see _lgn_chunk)."""
@@ -795,11 +808,12 @@ class GpuSim:
                              f"(step {it0}, {npf} steps per frame)")
         torch.cuda.synchronize()
         K = it0 // npf
+        KF = K * tm["denorm"]                    # frames entered before step it0
         self._frame_response(self.init_frame, 0)
         for slot in range(1, self.ringF):
             self.Rring[slot].copy_(self.Rring[0])
         cur = torch.cuda.current_stream(self.dev)
-        for fidx in range(max(0, K - maxF + 1), K):
+        for fidx in range(max(0, KF - maxF + 1), KF):
             self._frame_cells(cur, fidx, 0, sp.nLGN)
         if self.ev is not None:
             for m in self.ffMask2:
@@ -811,7 +825,7 @@ class GpuSim:
         self.cur = 0
         self.it = it0
         self.lgn_to = it0
-        self.fr_frame, self.fr_cells = K, 0
+        self.fr_frame, self.fr_cells = KF, 0
         self.lgn_evq = []
         self.slice_q = []
         self.lgn_tail = None

@@ -206,8 +206,9 @@ def save(sim, path) -> dict:
     sim.check()
     st = {k: v.cpu().numpy() for k, v in sim.state.items()}
     ngFF, ngE, ngI = int(sim.p.ngFF), int(sim.p.ngE), int(sim.p.ngI)
-    h = dict(it0=sim.it, frameCycle=K // maxF, iFrameHead=K % maxF,
-             oldFrameHead=(K - 1) % maxF, currentFrame=K, iModel=sp.iModel, learning=0,
+    nFr = K * tm["denorm"]                  # frames entered (denorm per period)
+    h = dict(it0=sim.it, frameCycle=nFr // maxF, iFrameHead=nFr % maxF,
+             oldFrameHead=(nFr - 1) % maxF, currentFrame=nFr, iModel=sp.iModel, learning=0,
              nV1=sp.nV1, nLGN=sp.nLGN, ngTypeFF=ngFF, ngTypeE=ngE, ngTypeI=ngI,
              nType=int(sim.p.nType), trainDepth=1)
     if getattr(sim, "delay", False) or getattr(sim, "far_only", False):
