@@ -602,6 +602,26 @@ int vc_lgn_frame_response_cells(void *stream, const float *frame, uint32_t width
                                 const float *SW, const uint32_t *coneType,
                                 const uint32_t *cellTrial, uint64_t trialFrameStride, float *R);
 
+/* The per-region sample geometry of every cell, geom[nLGN][2][VC_LGN_GEOM]:
+ * ecc cos(polar), ecc sin(polar), cos and sin of the orientation, the
+ * sample grid's step and half span in x and in y -- everything the flat-
+ * retina texture coordinates of a region's samples need besides the
+ * sample's grid position (src/discrete_input_convol.cu:398-417).  64 bytes
+ * per cell instead of the 16 bytes per sample of the SCx/SCy tables. */
+#define VC_LGN_GEOM 8
+int vc_lgn_sample_geometry(void *stream, const vc_lgn_setup *q, float *geom);
+
+/* vc_lgn_frame_response_cells with the coordinates regenerated from geom
+ * (and the setup's x0, y0, normViewDistance) by the same operations that fill
+ * SCx/SCy: the results are bit-identical, and the coordinate tables are not
+ * read.  nSample1D must be 8, 16 or 32. */
+int vc_lgn_frame_response_geom(void *stream, const float *frame, uint32_t width,
+                               uint32_t height, uint32_t nLGN, uint32_t c0, uint32_t c1,
+                               uint32_t nSample1D, const float *geom, float x0, float y0,
+                               float normViewDistance, const float *SW,
+                               const uint32_t *coneType, const uint32_t *cellTrial,
+                               uint64_t trialFrameStride, float *R);
+
 /* The same for nFrames frames [nFrames][3][height][width] in one launch:
  * R[nFrames][nLGN][2]. */
 int vc_lgn_frame_responses(void *stream, const float *frames, uint32_t nFrames,

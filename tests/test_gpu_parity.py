@@ -47,6 +47,48 @@ def test_frame_response_bitwise(pkg, tiny):
         assert np.array_equal(Rg.view(np.uint32), Ro.view(np.uint32)), f
 
 
+@pytest.mark.parametrize("n1d", [8, 16])
+def test_frame_response_geometry_equals_tables(pkg, n1d):
+    """The frame responses computed from the per-region sample geometry
+    (coordinates regenerated in the kernel) equal those read from the SCx/SCy
+    tables and the oracle's bit for bit, for 64 and 256 samples."""
+    import oracle
+    import torch
+    from visual_cortex_amd import network
+    from visual_cortex_amd.abi import check
+    from visual_cortex_amd.sim import GpuSim, ptr
+    _flush(f"test_frame_response_geometry_equals_tables: {n1d}x{n1d} samples")
+    net = network.build_network(network.spec_tiny(nSpatialSample1D=n1d, nLGN_side=12))
+    g = GpuSim(net)
+    sp = net.spec
+    tables = dict(TW=g.TW.cpu().numpy(), SW=g.SW.cpu().numpy(), SCx=g.SCx.cpu().numpy(),
+                  SCy=g.SCy.cpu().numpy(), maxConvol=g.maxConvol.cpu().numpy())
+    o = oracle.OracleSim(net, pkg, tables=tables)
+    Rt = torch.zeros(sp.nLGN, 2, device="cuda")
+    for f in range(3):
+        check(g.lib.vc_lgn_frame_response(None, ptr(g.frames[f]), sp.width, sp.height, sp.nLGN,
+                                          n1d, ptr(g.SCx), ptr(g.SCy), ptr(g.SW),
+                                          ptr(g.coneType), None, 0, ptr(Rt)),
+              "vc_lgn_frame_response")
+        g._frame_response(g.frames[f], 1)                     # the geometry path
+        torch.cuda.synchronize()
+        Rg = g.Rring[1].cpu().numpy()
+        assert np.abs(Rg).max() > 0
+        assert np.array_equal(Rg.view(np.uint32), Rt.cpu().numpy().view(np.uint32)), f
+        Ro = o.frame_response(net.frames[f])
+        assert np.array_equal(Rg.view(np.uint32), Ro.view(np.uint32)), f
+    # a partial range of cells writes those cells only
+    Rp = torch.full((sp.nLGN, 2), -7.0, device="cuda")
+    frame = g.frames[2]
+    check(g.lib.vc_lgn_frame_response_geom(None, ptr(frame), sp.width, sp.height, sp.nLGN, 5, 77,
+                                           n1d, ptr(g.geom), *g.view, ptr(g.SW), ptr(g.coneType),
+                                           None, 0, ptr(Rp)), "vc_lgn_frame_response_geom")
+    torch.cuda.synchronize()
+    Rp = Rp.cpu().numpy()
+    assert np.array_equal(Rp[5:77].view(np.uint32), Rg[5:77].view(np.uint32))
+    assert (Rp[:5] == -7.0).all() and (Rp[77:] == -7.0).all()
+
+
 CASES = {
     # name: (spec factory kwargs, steps)
     "tiny": (dict(), 400),

@@ -16,6 +16,7 @@ LIB_PATH = PKG_DIR / "libvc_hip.so"
 VC_MAX_NTYPE = 2
 VC_MAX_NG = 2
 VC_OK, VC_E_INVALID, VC_E_UNSUPPORTED, VC_E_HIP = 0, -1, -2, -3
+VC_LGN_GEOM = 8            # floats of one region's sample geometry (vc_lgn_sample_geometry)
 VC_SNAP_ROWS = 14          # snapshot rows: v, tBack, gFF[2], hFF[2], gE[2], hE[2], gI[2], hI[2]
 
 f32 = C.c_float
@@ -192,6 +193,9 @@ SIGNATURES = {
                      C.POINTER(FFEvents)],
     "vc_lgn_init": [vp, C.POINTER(LgnParams), vp, vp],
     "vc_lgn_frame_responses": [vp, vp, u32, u32, u32, u32, u32, vp, vp, vp, vp, vp],
+    "vc_lgn_sample_geometry": [vp, C.POINTER(LgnSetup), vp],
+    "vc_lgn_frame_response_geom": [vp, vp, u32, u32, u32, u32, u32, u32, vp, C.c_float,
+                                   C.c_float, C.c_float, vp, vp, vp, C.c_uint64, vp],
     "vc_lgn_temporal_weights": [vp, C.POINTER(LgnSetup), vp],
     "vc_lgn_rates": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp],
     "vc_lgn_rates_adjoint": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp,

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(1024) void k_lgn_frame_response(
     const float xC = SCx[iC], yC = SCy[iC], xS = SCx[iS], yS = SCy[iS];
     const float sw = SW[s];
     const float IC_S = intensity(tC, frame, W, H, xS, yS);
-    const float IS_S = intensity(tS, frame, W, H, xS, yS);
+    const float IS_S = (tS == tC) ? IC_S : intensity(tS, frame, W, H, xS, yS);
     const float IC_C = intensity(tC, frame, W, H, xC, yC);
     float sumC = IC_S, sumS = IS_S;
     block_reduce_ref2(sumC, sumS, part);
@@ -92,13 +92,22 @@ __global__ __launch_bounds__(1024) void k_lgn_frame_response(
 // tree -- here the half-wave trees of register k give warps 2k and 2k+1 --
 // and then halves over the nS/32 warp partials, here in registers.  The
 // additions are the same, so the results are identical; there is no barrier.
-template <int kPer>
+// With a sample-geometry table (geom, VC_LGN_GEOM floats per region; SCx,
+// SCy unused) the coordinates are regenerated from the sample's grid position
+// with the table kernel's own operations (sample_xy), so they equal the SC
+// tables bit for bit and the 16 bytes per sample are not read.
+struct GeomArgs {
+    const float *geom;
+    float x0, y0, nvd;
+    uint32_t n1D;
+};
+template <int kPer, bool kGeom>
 __global__ __launch_bounds__(256) void k_lgn_frame_response_w(
     const float *__restrict__ frame0, int W, int H, uint32_t nLGN, uint32_t c0, uint32_t c1,
     const float *__restrict__ SCx, const float *__restrict__ SCy,
     const float *__restrict__ SW, const uint32_t *__restrict__ coneType,
     const uint32_t *__restrict__ cellTrial, uint64_t trialFrameStride,
-    uint64_t frameStride, float *__restrict__ R) {
+    uint64_t frameStride, float *__restrict__ R, GeomArgs ga) {
     constexpr uint32_t nS = 64u * kPer, nW = 2u * kPer;
     const uint32_t c = c0 + __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
                                                            (threadIdx.x >> 6));
@@ -108,16 +117,30 @@ __global__ __launch_bounds__(256) void k_lgn_frame_response_w(
     R += (size_t)blockIdx.y * nLGN * 2;
     const float *frame = cellTrial ? frame0 + (size_t)cellTrial[c] * trialFrameStride : frame0;
     const uint32_t tC = coneType[c], tS = coneType[nLGN + c];
+    SampleGeom gC{}, gS{};
+    if (kGeom) {                               // wave-uniform: scalar loads
+        const float *g = ga.geom + (size_t)c * 2 * VC_LGN_GEOM;
+        gC = SampleGeom{g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7]};
+        gS = SampleGeom{g[8], g[9], g[10], g[11], g[12], g[13], g[14], g[15]};
+    }
     float icS[kPer], isS[kPer], icC[kPer], sw[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t smp = lane + 64u * k;
-        const size_t iC = ((size_t)c * 2 + 0) * nS + smp;
-        const size_t iS = ((size_t)c * 2 + 1) * nS + smp;
-        const float xC = SCx[iC], yC = SCy[iC], xS = SCx[iS], yS = SCy[iS];
+        float xC, yC, xS, yS;
+        if (kGeom) {
+            const uint32_t tx = smp % ga.n1D, ty = smp / ga.n1D;
+            sample_xy(gC, ga.x0, ga.y0, ga.nvd, tx, ty, xC, yC);
+            sample_xy(gS, ga.x0, ga.y0, ga.nvd, tx, ty, xS, yS);
+        } else {
+            const size_t iC = ((size_t)c * 2 + 0) * nS + smp;
+            const size_t iS = ((size_t)c * 2 + 1) * nS + smp;
+            xC = SCx[iC]; yC = SCy[iC]; xS = SCx[iS]; yS = SCy[iS];
+        }
         sw[k] = SW[smp];
         icS[k] = intensity(tC, frame, W, H, xS, yS);
-        isS[k] = intensity(tS, frame, W, H, xS, yS);
+        // centre and surround of one cone type see the same surround intensity
+        isS[k] = (tS == tC) ? icS[k] : intensity(tS, frame, W, H, xS, yS);
         icC[k] = intensity(tC, frame, W, H, xC, yC);
     }
     // warp partials, then the halving over them (as block_reduce_ref2)
@@ -168,14 +191,29 @@ hipError_t launch_frame_response(hipStream_t st, uint32_t nS, uint32_t nFrames,
                                  uint32_t nLGN, uint32_t c0, uint32_t c1, const float *SCx,
                                  const float *SCy, const float *SW, const uint32_t *coneType,
                                  const uint32_t *cellTrial, uint64_t trialFrameStride,
-                                 uint64_t frameStride, float *R) {
+                                 uint64_t frameStride, float *R, const GeomArgs *ga = nullptr) {
     if (c1 <= c0) return hipSuccess;
     const uint32_t nc = c1 - c0;
     const dim3 grid((nc + 3u) / 4u, nFrames), block(256);
-#define VC_FR(K)                                                                            \
-    hipLaunchKernelGGL((k_lgn_frame_response_w<K>), grid, block, 0, st, frame, (int)width,  \
-                       (int)height, nLGN, c0, c1, SCx, SCy, SW, coneType, cellTrial,       \
-                       trialFrameStride, frameStride, R)
+    const GeomArgs g0{nullptr, 0.0f, 0.0f, 0.0f, 0u};
+    if (ga) {                         // square grids of 64, 256 or 1024 samples
+        switch (nS) {
+#define VC_FRG(K)                                                                                \
+    hipLaunchKernelGGL((k_lgn_frame_response_w<K, true>), grid, block, 0, st, frame, (int)width, \
+                       (int)height, nLGN, c0, c1, SCx, SCy, SW, coneType, cellTrial,            \
+                       trialFrameStride, frameStride, R, *ga)
+        case 64: VC_FRG(1); break;
+        case 256: VC_FRG(4); break;
+        case 1024: VC_FRG(16); break;
+        default: return hipErrorInvalidValue;
+#undef VC_FRG
+        }
+        return hipGetLastError();
+    }
+#define VC_FR(K)                                                                                  \
+    hipLaunchKernelGGL((k_lgn_frame_response_w<K, false>), grid, block, 0, st, frame, (int)width, \
+                       (int)height, nLGN, c0, c1, SCx, SCy, SW, coneType, cellTrial,             \
+                       trialFrameStride, frameStride, R, g0)
     switch (nS) {
     case 64: VC_FR(1); break;
     case 128: VC_FR(2); break;
@@ -619,24 +657,26 @@ __global__ __launch_bounds__(1024) void k_lgn_sw(vc_lgn_setup q, float *__restri
     SW[threadIdx.x] = sw / tot;
 }
 
+// the per-region sample geometry of every cell: geom[(c*2 + r)*VC_LGN_GEOM ...]
+__global__ __launch_bounds__(256) void k_lgn_geom(vc_lgn_setup q, float *__restrict__ geom) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2u * q.nLGN) return;
+    const uint32_t c = i >> 1, r = i & 1u;
+    const SampleGeom g = sample_geom(q, r * q.nLGN + c);
+    float *o = geom + (size_t)i * VC_LGN_GEOM;
+    o[0] = g.ex; o[1] = g.ey; o[2] = g.coso; o[3] = g.sino;
+    o[4] = g.dw; o[5] = g.wSpan; o[6] = g.dh; o[7] = g.hSpan;
+}
+
 // sample texture coordinates, flat retina (src/discrete_input_convol.cu:398-417)
 __global__ __launch_bounds__(1024) void k_lgn_sc(vc_lgn_setup q, float *__restrict__ SCx,
                                                  float *__restrict__ SCy) {
     const uint32_t c = blockIdx.x, r = blockIdx.y;
     const uint32_t n1D = q.nSample1D, nS = n1D * n1D;
     const uint32_t tx = threadIdx.x % n1D, ty = threadIdx.x / n1D;
-    const uint32_t lid = r * q.nLGN + c;
-    const float rx = q.rw[lid], ry = q.rh[lid];
-    const float wSpan = (float)((double)(q.nsig * rx) / kSqrt2);
-    const float dw = 2 * wSpan / (float)n1D;
-    const float hSpan = (float)((double)(q.nsig * ry) / kSqrt2);
-    const float dh = 2 * hSpan / (float)n1D;
-    const float polar = q.polar[lid], ecc = q.ecc[lid];
-    const float coso = cosf(q.orient[lid]), sino = sinf(q.orient[lid]);
-    const float w = (float)((tx + 0.5) * (double)dw - (double)wSpan);
-    const float h = (float)((ty + 0.5) * (double)dh - (double)hSpan);
-    const float x = q.x0 + (ecc * cosf(polar) + w * coso - h * sino) * q.normViewDistance;
-    const float y = q.y0 + (ecc * sinf(polar) + w * sino + h * coso) * q.normViewDistance;
+    const SampleGeom g = sample_geom(q, r * q.nLGN + c);
+    float x, y;
+    sample_xy(g, q.x0, q.y0, q.normViewDistance, tx, ty, x, y);
     const size_t id = ((size_t)c * 2 + r) * nS + threadIdx.x;
     SCx[id] = x;
     SCy[id] = y;
@@ -783,6 +823,50 @@ extern "C" int vc_lgn_frame_response(void *stream, const float *frame, uint32_t 
                                      uint64_t trialFrameStride, float *R) {
     return vc_lgn_frame_response_cells(stream, frame, width, height, nLGN, 0u, nLGN, nSample1D,
                                        SCx, SCy, SW, coneType, cellTrial, trialFrameStride, R);
+}
+
+extern "C" int vc_lgn_sample_geometry(void *stream, const vc_lgn_setup *q, float *geom) {
+    if (!q || !geom || !q->rw || !q->rh || !q->polar || !q->ecc || !q->orient) {
+        vc_set_error("vc_lgn_sample_geometry: null argument");
+        return VC_E_INVALID;
+    }
+    if (q->nLGN == 0 || q->nSample1D == 0) {
+        vc_set_error("vc_lgn_sample_geometry: empty LGN or sample grid");
+        return VC_E_INVALID;
+    }
+    hipLaunchKernelGGL(k_lgn_geom, dim3((2u * q->nLGN + 255u) / 256u), dim3(256), 0,
+                       (hipStream_t)stream, *q, geom);
+    VC_HIP_CHECK(hipGetLastError());
+    return VC_OK;
+}
+
+extern "C" int vc_lgn_frame_response_geom(void *stream, const float *frame, uint32_t width,
+                                          uint32_t height, uint32_t nLGN, uint32_t c0,
+                                          uint32_t c1, uint32_t nSample1D, const float *geom,
+                                          float x0, float y0, float normViewDistance,
+                                          const float *SW, const uint32_t *coneType,
+                                          const uint32_t *cellTrial, uint64_t trialFrameStride,
+                                          float *R) {
+    const uint32_t nS = nSample1D * nSample1D;
+    if (!frame || !geom || !SW || !coneType || !R) {
+        vc_set_error("vc_lgn_frame_response_geom: null argument");
+        return VC_E_INVALID;
+    }
+    if (nS != 64 && nS != 256 && nS != 1024) {
+        vc_set_error("vc_lgn_frame_response_geom: nSpatialSample1D %u must be 8, 16 or 32",
+                     nSample1D);
+        return VC_E_UNSUPPORTED;
+    }
+    if (width == 0 || height == 0 || nLGN == 0 || c0 > c1 || c1 > nLGN) {
+        vc_set_error("vc_lgn_frame_response_geom: empty frame or LGN, or cells %u..%u outside "
+                     "0..%u", c0, c1, nLGN);
+        return VC_E_INVALID;
+    }
+    const GeomArgs ga{geom, x0, y0, normViewDistance, nSample1D};
+    VC_HIP_CHECK(launch_frame_response((hipStream_t)stream, nS, 1u, frame, width, height, nLGN,
+                                       c0, c1, nullptr, nullptr, SW, coneType, cellTrial,
+                                       trialFrameStride, (uint64_t)0, R, &ga));
+    return VC_OK;
 }
 
 extern "C" int vc_lgn_frame_responses(void *stream, const float *frames, uint32_t nFrames,

@@ -9,6 +9,41 @@ namespace {
 constexpr double kSqrt2 = 1.4142135623730951;  // src/CONST.h:32 (double)
 constexpr int kMaxTaps = 64;                    // temporal kernel samples per region
 
+// What a region's sample texture coordinates need besides the sample's grid
+// position (flat retina, src/discrete_input_convol.cu:398-417): ex = ecc
+// cos(polar), ey = ecc sin(polar), the orientation's cos/sin and the grid's
+// step and half span in each direction (VC_LGN_GEOM floats per region).
+struct SampleGeom {
+    float ex, ey, coso, sino, dw, wSpan, dh, hSpan;
+};
+static_assert(sizeof(SampleGeom) == VC_LGN_GEOM * sizeof(float), "geometry record");
+
+__device__ __forceinline__ SampleGeom sample_geom(const vc_lgn_setup &q, uint32_t lid) {
+    SampleGeom g;
+    const uint32_t n1D = q.nSample1D;
+    const float rx = q.rw[lid], ry = q.rh[lid];
+    g.wSpan = (float)((double)(q.nsig * rx) / kSqrt2);
+    g.dw = 2 * g.wSpan / (float)n1D;
+    g.hSpan = (float)((double)(q.nsig * ry) / kSqrt2);
+    g.dh = 2 * g.hSpan / (float)n1D;
+    const float polar = q.polar[lid], ecc = q.ecc[lid];
+    g.coso = cosf(q.orient[lid]);
+    g.sino = sinf(q.orient[lid]);
+    g.ex = ecc * cosf(polar);
+    g.ey = ecc * sinf(polar);
+    return g;
+}
+
+// Normalised texture coordinates of grid sample (tx, ty) of a region: the
+// reference's expression, evaluated in the same order and precision.
+__device__ __forceinline__ void sample_xy(const SampleGeom &g, float x0, float y0, float nvd,
+                                          uint32_t tx, uint32_t ty, float &x, float &y) {
+    const float w = (float)((tx + 0.5) * (double)g.dw - (double)g.wSpan);
+    const float h = (float)((ty + 0.5) * (double)g.dh - (double)g.hSpan);
+    x = x0 + (g.ex + w * g.coso - h * g.sino) * nvd;
+    y = y0 + (g.ey + w * g.sino + h * g.coso) * nvd;
+}
+
 // block_reduce of src/util/cuda_util.cuh:89-101 for blockDim.x threads (a
 // power of two >= 32).  Every thread gets the result.
 __device__ float block_reduce_ref(float x, float *part /* >= 32 floats */) {

@@ -197,6 +197,35 @@ struct RecIn {
     uint32_t *status;
 };
 
+// One wavefront's copy of a batch's row-segment headers and increments,
+// read by the lanes the spikes reach.
+struct WaveStage {
+    uint32_t rmLo[kBatch], rmHi[kBatch], off[kBatch];
+    float4 inc[kBatch];
+};
+
+// Transpose of the 64 x 64 bit matrix whose row k is lane k's word: on
+// return lane l holds column l (bit k set when bit l of lane k's word was).
+// Six butterfly stages exchange blocks of 32, 16, ..., 1 bits between lanes
+// k and k ^ j.
+__device__ __forceinline__ uint64_t transpose64(uint64_t a, uint32_t lane) {
+    constexpr uint64_t kMask[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
+                                   0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
+                                   0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const uint32_t j = 32u >> s;
+        const uint32_t plo = (uint32_t)__shfl_xor((int)(uint32_t)a, (int)j);
+        const uint32_t phi = (uint32_t)__shfl_xor((int)(uint32_t)(a >> 32), (int)j);
+        const uint64_t b = ((uint64_t)phi << 32) | plo;
+        const bool low = (lane & j) == 0u;
+        const uint64_t lo = low ? a : b, hi = low ? b : a;
+        const uint64_t t = ((lo >> j) ^ hi) & kMask[s];
+        a = low ? (lo ^ (t << j)) : (hi ^ t);
+    }
+    return a;
+}
+
 // Adds the recurrent increments of a batch of K <= 64 spikes listed in
 // brow/bslot (this wavefront's LDS rows) to the calling lane's sums.  With
 // kRec the increments of spike k are the four tagged words at ri.words +
@@ -208,7 +237,7 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
                                             uint32_t nSeg, uint32_t myseg, int t,
                                             const uint32_t *brow, const uint32_t *bslot,
                                             const uint32_t *bsrc, const RecIn &ri,
-                                            const V1Key &key, uint32_t step,
+                                            const V1Key &key, uint32_t step, WaveStage &ws,
                                             float lgE[VC_MAX_NG], float lhE[VC_MAX_NG],
                                             float lgI[VC_MAX_NG], float lhI[VC_MAX_NG]) {
     // lane kk fetches the row-segment header and the increments of spike kk
@@ -289,29 +318,59 @@ __device__ __forceinline__ void flush_batch(const vc_v1_params &p, const vc_rec_
         return;
     }
     load_increments();
-    for (uint32_t k0 = 0; k0 < K; k0 += kW) {
+    // Masked CSR: each lane visits only the spikes whose row segment has a
+    // strength for its own post.  Lane k stages spike k's header and
+    // increments; the transpose of the batch's K x 64 segment masks gives
+    // every lane the set of spikes that reach it, walked in ascending order
+    // (the order of the sums), kW strengths in flight at a time.
+    if (lane < K) {
+        ws.rmLo[lane] = (uint32_t)rm;
+        ws.rmHi[lane] = (uint32_t)(rm >> 32);
+        ws.off[lane] = off;
+        ws.inc[lane] = make_float4(og0, oh0, og1, oh1);
+    }
+    uint64_t col = transpose64(lane < K ? rm : 0ull, lane);
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    while (__ballot(col != 0ull) != 0ull) {
+        uint32_t kk[kW];
         float wv[kW];
 #pragma unroll
         for (int j = 0; j < kW; ++j) {
-            const uint32_t kk = k0 + j;
+            kk[j] = 64u;
             wv[j] = 0.0f;
-            if (kk < K) {
-                const uint64_t rmk = vc_readlane_u64(rm, kk);
-                const uint32_t idx = vc_readlane_u32(off, kk) + vc_mbcnt(rmk);
-                if ((rmk >> lane) & 1ull) wv[j] = rec.w[idx];
+            if (col != 0ull) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(col);
+                col &= col - 1ull;
+                kk[j] = k;
+                const uint64_t rmk = ((uint64_t)ws.rmHi[k] << 32) | ws.rmLo[k];
+                wv[j] = rec.w[ws.off[k] + vc_mbcnt(rmk)];
             }
         }
 #pragma unroll
         for (int j = 0; j < kW; ++j) {
-            const uint32_t kk = k0 + j;
-            const bool has = kk < K && ((vc_readlane_u64(rm, kk) >> lane) & 1ull);
-            if (has) wv[j] = failed(wv[j], kk);
-            if (has && wv[j] > 0.0f)
-                add_spike(p, wv[j], vc_readlane_u32(isI, kk), pE0, pE1, pI0, pI1,
-                          vc_readlane_f32(og0, kk), vc_readlane_f32(oh0, kk),
-                          vc_readlane_f32(og1, kk), vc_readlane_f32(oh1, kk), lgE, lhE, lgI, lhI);
+            const uint32_t k = kk[j];
+            if (k < 64u) {
+                const uint32_t sw = bslot[k];
+                const uint32_t inh = sw >> 31;
+                float w = wv[j];
+                if constexpr (kSt) {
+                    const float pf = inh ? pfI : pfE;
+                    if (pf > 0.0f) {
+                        const uint32_t pre = ((sw & 0x7fffffffu) / bs) * bs + brow[k] % bs;
+                        w = fail_factor(w, pf, inh ? nsI : nsE, key, step, VC_STREAM_V1_FAIL,
+                                        pre - key.off, p.trial);
+                    }
+                }
+                if (w > 0.0f) {
+                    const float4 in = ws.inc[k];
+                    add_spike(p, w, inh, pE0, pE1, pI0, pI1, in.x, in.y, in.z, in.w, lgE, lhE,
+                              lgI, lhI);
+                }
+            }
         }
     }
+    __builtin_amdgcn_wave_barrier();        // the stage is rewritten by the next batch
 }
 
 // LDS of the workgroup's spike list (see rec_gather).
@@ -320,6 +379,7 @@ struct GatherLds {
     uint32_t off[kMaxEnt];                   // list position of entry e's first spike
     uint32_t wsum[kWG / 64];
     uint32_t row[kList], slot[kList], src[kList];
+    WaveStage ws[kWG / 64];                  // one per wavefront (flush_batch)
 };
 
 // Recurrent increments of the spikes of `prev` for the calling neuron.
@@ -401,8 +461,8 @@ __device__ __forceinline__ void rec_gather(const vc_v1_params &p, const vc_rec_c
         for (uint32_t k0 = 0; k0 < K; k0 += (uint32_t)kBatch) {
             const uint32_t kb = (K - k0) < (uint32_t)kBatch ? (K - k0) : (uint32_t)kBatch;
             flush_batch<kCoh, kRec, kG, kSt>(p, rec, prev, kb, lane, nSeg, myseg, t, g.row + k0,
-                                    g.slot + k0, g.src + k0, ri, key, step, lgE, lhE, lgI,
-                                    lhI);
+                                    g.slot + k0, g.src + k0, ri, key, step, g.ws[wv], lgE, lhE,
+                                    lgI, lhI);
         }
         __syncthreads();                    // the list is rewritten by the next pass
     }

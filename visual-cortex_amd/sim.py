@@ -396,6 +396,12 @@ class GpuSim:
         check(self.lib.vc_lgn_precompute(None, C.byref(q), ptr(self.TW), ptr(self.SW),
                                          ptr(self.SCx), ptr(self.SCy), ptr(self.maxConvol)),
               "vc_lgn_precompute")
+        # the frames' spatial responses regenerate the sample coordinates from
+        # the per-region geometry (64 bytes per cell instead of 16 per sample)
+        self.geom = torch.zeros(sp.nLGN, 2, abi.VC_LGN_GEOM, **f32)
+        check(self.lib.vc_lgn_sample_geometry(None, C.byref(q), ptr(self.geom)),
+              "vc_lgn_sample_geometry")
+        self.view = (float(x0), float(y0), float(nvd))
         self.kC = self.lgn_t["k"][0].contiguous()
         self.kS = self.lgn_t["k"][1].contiguous()
         self.lTR = torch.zeros(sp.nLGN, **f32)
@@ -444,11 +450,12 @@ class GpuSim:
 
     def _frame_response(self, frame: torch.Tensor, slot: int):
         sp = self.net.spec
-        check(self.lib.vc_lgn_frame_response(None, ptr(frame), sp.width, sp.height, sp.nLGN,
-                                             sp.nSpatialSample1D, ptr(self.SCx), ptr(self.SCy),
-                                             ptr(self.SW), ptr(self.coneType), None, 0,
-                                             ptr(self.Rring[slot])),
-              "vc_lgn_frame_response")
+        check(self.lib.vc_lgn_frame_response_geom(None, ptr(frame), sp.width, sp.height,
+                                                  sp.nLGN, 0, sp.nLGN, sp.nSpatialSample1D,
+                                                  ptr(self.geom), *self.view, ptr(self.SW),
+                                                  ptr(self.coneType), None, 0,
+                                                  ptr(self.Rring[slot])),
+              "vc_lgn_frame_response_geom")
 
     # ------------------------------------------------------------------ steps
     def _hs(self, stream) -> C.c_void_p:
@@ -467,11 +474,11 @@ class GpuSim:
             return
         sp = self.net.spec
         frame, ctr = self._frame_of(fidx)
-        check(self.lib.vc_lgn_frame_response_cells(
+        check(self.lib.vc_lgn_frame_response_geom(
             self._hs(stream), ptr(frame), sp.width, sp.height, sp.nLGN, c0, c1,
-            sp.nSpatialSample1D, ptr(self.SCx), ptr(self.SCy), ptr(self.SW), ptr(self.coneType),
+            sp.nSpatialSample1D, ptr(self.geom), *self.view, ptr(self.SW), ptr(self.coneType),
             ctr, self.trialFrameStride, ptr(self.Rring[fidx % self.ringF])),
-            "vc_lgn_frame_response_cells")
+            "vc_lgn_frame_response_geom")
 
     def _lgn_chunk(self, a: int, b: int, on_v1: bool = False):
         """Issue the LGN stage of steps a .. b-1 (inside one frame period k, a =
