@@ -472,13 +472,19 @@ typedef struct {
  * during the step in slot `cur` of the ring vc_v1_rec_delay reads (same
  * train, depth and cur): the reference's recal_G_vec with distance-dependent
  * delays (noFarDelay 0, src/coredynamics.cu:181-372) followed by sum_G
- * (:2314-2349).  Call it after vc_v1_rec_delay of the same step, as the
- * reference adds the far sums after recal_G_mat.  Per connection time2post =
+ * (:2314-2349).  With delayed near connections (noDelay 0) call it after
+ * vc_v1_rec_delay of the same step, as the reference adds the far sums after
+ * recal_G_mat.  With undelayed near connections (noDelay 1, noFarDelay 0) the
+ * ring serves the far input alone: step t is vc_v1_step writing ring slot
+ * cur (nothing pending in `prev`), then vc_v1_rec_apply_at(step t) of the
+ * step's own spikes (recal_G_mat_nd), then this call.  Per connection time2post =
  * dist/speed, it2post = ceil(time2post/dt); depth must exceed every it2post
  * (the reference's fTrainDepth = it2post + 1, src/patch.cu:3847).  As the
  * reference, it adds nothing in a step in which no far presynaptic neuron
  * (far->preMask) has train[cur][n] > 1.0 (src/patch.cu:6597-6598, src/stats.cu
- * :114-126); trainMask[depth][nV1/64] holds each step's spike masks, as for
+ * :114-126); with batched trials (p->blockTrial) that gate is per trial,
+ * over the trial's own far presynaptic neurons, as each trial is a run of
+ * its own.  trainMask[depth][nV1/64] holds each step's spike masks, as for
  * vc_v1_rec_delay.  Only the connections of presynaptic neurons that fired
  * inside the ring are visited (far->out*), and far->hit is the calls' work
  * memory (calls on one far->hit must not overlap).  *status (may be NULL)

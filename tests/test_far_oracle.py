@@ -223,15 +223,56 @@ def test_far_gate_on_current_far_spikes(pkg):
                                    atol=1e-7 * max(1.0, ref[k].max()))
 
 
-def test_batched_far_trials_equal_serial(pkg):
+def gate_splits(nets, sinfos):
+    """Steps in which the far gates of the trials differ (src/patch.cu
+    :6597-6598: some far presynaptic neuron of the trial has output > 1.0)
+    while a trial whose gate is closed has far arrivals due in that step
+    (a spike of a far presynaptic neuron it2post or it2post - 1 steps
+    earlier, it2post = ceil(d/speed/dt)), which the closed gate drops: the
+    steps in which a network-wide gate would give other sums."""
+    opened, due = [], []
+    for n, s in zip(nets, sinfos):
+        sp = n.spec
+        off, pre, _, dist = n.far
+        T = s.shape[0]
+        pre = pre.astype(np.int64)
+        opened.append((s[:, np.unique(pre)] > 1.0).any(1))
+        it2 = np.ceil(dist.astype(np.float64) / sp.speedOfThought / sp.dt).astype(np.int64)
+        fired = s[:, pre] >= 1.0                          # [T, connections]
+        d = np.zeros(T, bool)
+        for lag in (0, 1):
+            L = it2 - lag
+            for t in range(T):
+                src = t - L
+                ok = (src >= 0) & (L >= 1)
+                if ok.any() and fired[src[ok], np.flatnonzero(ok)].any():
+                    d[t] = True
+        due.append(d)
+    n = 0
+    for i in range(len(nets)):
+        for j in range(len(nets)):
+            if i != j:
+                n += int((opened[i] & ~opened[j] & due[j]).sum())
+    return n
+
+
+# far input through delayed near connections, and with undelayed ones
+FAR_BATCH = {"delayed": dict(noDelay=0, speedOfThought=0.5),
+             "far_only": dict(noDelay=1, noFarDelay=0, speedOfThought=0.5)}
+
+
+@pytest.mark.parametrize("kind", list(FAR_BATCH))
+def test_batched_far_trials_equal_serial(pkg, kind):
     """Trials with far lists batched into one network: each trial's output
     equals its own run bit for bit -- the far gate is per trial, as each
-    trial is a separate run of the reference (trial 2 fires far less, so
-    many steps open the gate of one trial only)."""
+    trial is a separate run of the reference (the second trial fires far
+    less, so in many steps only the first trial's gate is open while the
+    second has far arrivals that its closed gate drops)."""
     import oracle
     from visual_cortex_amd import batch, network
-    specs = [network.spec_tiny(nblock=nb, seed=300 + i, ori_deg=30.0 * i, bgStrength=(0.4, 0.4),
-                               noDelay=0, speedOfThought=0.5, nFar=10)
+    specs = [network.spec_tiny(nblock=nb, seed=300 + i, ori_deg=30.0 * i,
+                               bgStrength=(0.4, 0.4),
+                               nFar=10, **FAR_BATCH[kind])
              for i, nb in enumerate((4, 2))]
     nets = [network.build_network(s) for s in specs]
     b = batch.batch_networks(nets, device="cpu")
@@ -240,13 +281,16 @@ def test_batched_far_trials_equal_serial(pkg):
     ob.run(96)
     sb = ob.sInfo()
     off = 0
+    serial = []
     for n in nets:
         o = oracle.OracleSim(n, pkg)
         o.run(96)
         s = o.sInfo()
+        serial.append(s)
         assert int((s >= 1).sum()) > 40
         assert np.array_equal(sb[:, off:off + n.spec.nV1].view(np.uint32), s.view(np.uint32))
         off += n.spec.nV1
+    assert gate_splits(nets, serial) > 0
 
 
 def test_batch_refuses_mixed_far_settings(pkg):

@@ -309,27 +309,34 @@ def test_far_with_undelayed_near_bitwise(pkg, speed):
     assert np.array_equal(b.sInfo().view(np.uint32), sg[2 * npf:].view(np.uint32))
 
 
-def test_batched_far_trials_equal_serial(pkg):
-    """Delayed trials with far lists batched into one network: each trial's
-    output equals its own run bit for bit (per-trial far gate)."""
+@pytest.mark.parametrize("kind", ["delayed", "far_only"])
+def test_batched_far_trials_equal_serial(pkg, kind):
+    """Trials with far lists batched into one network, with delayed near
+    connections and with undelayed ones: each trial's output equals its own
+    run bit for bit (per-trial far gate), and there are steps in which one
+    trial's gate is open while another's is closed with far arrivals due."""
+    from test_far_oracle import FAR_BATCH, gate_splits
     from visual_cortex_amd import batch, network
     from visual_cortex_amd.sim import GpuSim
-    _flush("test_batched_far_trials_equal_serial: 3 trials, speed 0.5, 160 steps")
+    _flush(f"test_batched_far_trials_equal_serial[{kind}]: 3 trials, speed 0.5, 160 steps")
     specs = [network.spec_tiny(nblock=nb, seed=310 + i, ori_deg=22.5 * i, bgStrength=(0.4, 0.4),
-                               noDelay=0, speedOfThought=0.5, nFar=10)
+                               nFar=10, **FAR_BATCH[kind])
              for i, nb in enumerate((4, 2, 3))]
     nets = [network.build_network(s) for s in specs]
     bnet = batch.batch_networks(nets, device="cuda")
     g = GpuSim(bnet)
-    assert g.delay and g.far_on
+    assert g.far_on and g.delay == (kind == "delayed") and g.far_only == (kind == "far_only")
     g.run(160)
     sb = g.sInfo()
     off = 0
+    serial = []
     for t, n in enumerate(nets):
         gs = GpuSim(n)
         assert gs.far_on and gs.depth <= g.depth
         gs.run(160)
         ss = gs.sInfo()
+        serial.append(ss)
         assert int((ss >= 1).sum()) > 40, t
         assert np.array_equal(sb[:, off:off + n.spec.nV1].view(np.uint32), ss.view(np.uint32)), t
         off += n.spec.nV1
+    assert gate_splits(nets, serial) > 0
