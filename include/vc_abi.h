@@ -33,6 +33,14 @@
  *   vc_v1_rec_delay        recal_G_mat (axonal delays, noDelay = 0)
  *                            src/coredynamics.cu:2062-2309
  *   vc_v1_rec_delay_table  its per-synapse delay arithmetic, :2173-2186
+ *   vc_lgn_steps_learn     vc_lgn_steps plus the LGN cells' LTP trace of the
+ *                            feed-forward plasticity, LGN_nonlinear's
+ *                            learning part, src/discrete_input_convol.cu
+ *                            :1736-1743, 1775-1821
+ *   vc_v1_learn_ff         the feed-forward plasticity of
+ *                            compute_V_collect_spike_learnFF(_fast) with
+ *                            learning = 3, src/coredynamics.cu:1631-1992
+ *                            (and :843-1233, the same code)
  *   vc_v1_rec_distances    the V1_delayMat distances of genCon
  *                            src/preprocess/connect.cu:424-466
  *   vc_v1_rec_far          recal_G_vec (far connections with delays) +
@@ -677,6 +685,78 @@ typedef struct {
 int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
                  const float *Rring, uint32_t it0, uint32_t nsteps,
                  float *sInfo, float *fr, const vc_ff_events *ev);
+
+/* Plasticity of the LGN->V1 strengths: the reference's learning = 3, the
+ * triplet rule on the feed-forward synapses onto excitatory neurons, with
+ * optional homeostasis of each neuron's total feed-forward strength
+ * (src/coredynamics.cu:1631-1992, src/discrete_input_convol.cu:1736-1821,
+ * the constants of learnFF_pre/learnFF_post, src/condShape.h:94-155, set up
+ * at src/patch.cu:1092-1330, 4332-4352).  The traces decay as
+ * x *= exp(-t/tau) (decay/if_decay, src/condShape.h:44-51). */
+typedef struct {
+    float tauLTP;            /* LGN (pre) trace, ms: lFF_E_pre.tauLTP[0]        */
+    float tauLTD, tauTrip;   /* V1 (post) traces, ms: lFF_E_post.tau[0], [1]    */
+    float tauAvg;            /* filtered post spike count, ms: lFF_E_post.tau[2] */
+    float A_LTP;             /* A_LGN[0] * sRatioLGN[0]                          */
+    float A_ratio;           /* the LTD amplitude (learnFF_post's A_ratio)       */
+    float targetFR;          /* Hz; > 0: the LTD amplitude is scaled by
+                                tauLTP * avg * avg at the pre spike              */
+    float gmax, gmin;        /* strength bounds, already * sRatioLGN[0]          */
+    uint32_t applyHomeo;     /* 0 none, 1 multiplicative, 2 additive             */
+    uint32_t symmetricHomeo; /* relax towards the target from below too          */
+    float expHomeo;          /* exp(-dt/decayTimeHomeoFF)                        */
+    uint32_t rebound;        /* 0: a strength of exactly 0 is left alone         */
+} vc_learn_ff_params;
+
+/* The plastic state, all [nV1] float32 (the reference's vLTD_FF_E,
+ * vTrip_FF_E and vAvgE over [nblock][nE], here indexed by global neuron id:
+ * inhibitory entries are neither read nor written). */
+typedef struct {
+    float *sLGN;             /* [maxLGN][nV1]: the strengths vc_ff_conn points
+                                to, updated in place                            */
+    float *vLTD, *vTrip;     /* post traces, zero at the start of a run          */
+    float *vAvg;             /* filtered post spike count, zero at the start     */
+    float *totalFF;          /* applyHomeo: total strength of each neuron (the
+                                reference's initial sum over its list, in list
+                                order, src/patch.cu:4339-4344); else NULL        */
+    const float *totalFFinf; /* applyHomeo: its target, FF_InfRatio * the initial
+                                total; else NULL                                */
+} vc_learn_ff_state;
+
+/* vc_lgn_steps, and the LTP trace x of each LGN cell (the reference's
+ * surface layers 1..3 of LGNspikeSurface, src/discrete_input_convol.cu
+ * :1736-1743, 1775-1821): xVar[nLGN] holds x at the start of step it0 on
+ * entry (0 before the first step, logRand_init) and at the end of the last
+ * step on exit; for step it0+s, xStart[s*nLGN + c] receives x at the start of
+ * the step and xSpike[s*nLGN + c] x at the cell's spike time: the reference
+ * adds the step's nsp spikes first, then decays to the mean spike time tsp
+ * (x_spike = (x + nsp) exp(-tsp/tauLTP)), and the step ends at x_spike
+ * exp(-(dt - tsp)/tauLTP); without a spike xSpike = xStart and the step ends
+ * at x exp(-dt/tauLTP).  tsp is the draw's own spike time, not the one
+ * recovered from sInfo.  Not for batched trials (p->cellTrial must be NULL). */
+int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
+                       const float *Rring, uint32_t it0, uint32_t nsteps, float *sInfo,
+                       float *fr, const vc_ff_events *ev, float tauLTP, float *xVar,
+                       float *xStart, float *xSpike);
+
+/* One step of the feed-forward plasticity, after that step's vc_v1_step (the
+ * reference learns at the end of the same kernel, after the strengths were
+ * used): for each excitatory neuron n, with nsp = floor(sInfo[n]) and tsp =
+ * (sInfo[n] - nsp) dt its spikes of the step (tsp = dt without), every
+ * synapse i < nLGN[n] whose strength f is not 0 (or every one with rebound):
+ * f is first scaled (applyHomeo 1) or shifted (2) by the homeostatic change
+ * of the neuron's total; if its LGN cell c spiked (lgnSInfo[c] >= 1, at
+ * tspF) f -= the LTD term ltd(t) * exp(-|tspF - t|/tauLTD) * A (t = the step
+ * start when tspF < tsp, else tsp); if the neuron spiked, f += the LTP term
+ * x_pre * exp(-dtp/tauLTP) * trip(tsp) * A_LTP (x_pre = xSpike[c], dtp = tsp
+ * - tspF when tspF < tsp, else xStart[c], dtp = tsp); f is clipped to
+ * [gmin, gmax].  Then the post traces take the step's spikes and decay to the
+ * step end, in the reference's operation order.  sInfo is the step's V1
+ * output row, lgnSInfo/xStart/xSpike the step's rows of vc_lgn_steps_learn.
+ * Not for batched trials (p->blockTrial must be NULL). */
+int vc_v1_learn_ff(void *stream, const vc_v1_params *p, const vc_learn_ff_params *lp,
+                   const vc_ff_conn *ff, const vc_learn_ff_state *ls, const float *sInfo,
+                   const float *lgnSInfo, const float *xStart, const float *xSpike);
 
 /* The rate part of vc_lgn_steps alone (no spike draw), for any run of steps
  * and any maxFrame: fr[nsteps][nLGN] (Hz).  R is read at slot f mod maxFrame

@@ -52,6 +52,9 @@ class Oracle:
         L.orc_lgn_frame_response.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp,
                                              C.c_uint64, vp]
         L.orc_lgn_steps.argtypes = [P(abi_mod.LgnParams)] + [vp] * 11 + [u32, u32, vp, vp, vp]
+        L.orc_lgn_steps_learn.argtypes = [P(abi_mod.LgnParams)] + [vp] * 11 + \
+            [u32, u32, vp, vp, vp, f32, vp, vp, vp]
+        L.orc_v1_learn_ff.argtypes = [P(abi_mod.V1Params), P(abi_mod.LearnFFParams)] + [vp] * 12
         L.orc_lgn_init.argtypes = [P(abi_mod.LgnParams), vp, vp]
         L.orc_lgn_precompute.argtypes = [P(abi_mod.LgnSetup), vp, vp, vp, vp, vp]
         L.orc_expf.argtypes = [f32]; L.orc_expf.restype = f32
@@ -124,7 +127,7 @@ class OracleSim:
         self.nbr = np.ascontiguousarray(net.nbr, dtype=np.uint32)
         self.nNear = np.ascontiguousarray(net.nNear, dtype=np.uint32)
         self.nLGN_of = np.ascontiguousarray(net.nLGN_of, dtype=np.uint32)
-        self.sLGN = np.ascontiguousarray(net.sLGN, dtype=np.float32)
+        self.sLGN = np.array(net.sLGN, dtype=np.float32, order="C")   # a copy: plasticity writes it
         self.idLGN = np.ascontiguousarray(net.idLGN, dtype=np.uint32)
         bs = sp.blockSize
         self.ipre = np.zeros(sp.nV1, np.uint32)
@@ -215,6 +218,20 @@ class OracleSim:
         self.Rring[1:] = self.Rring[0]
         self.lgn_sInfo = np.zeros((self.tm["ntPerFrame"], sp.nLGN), np.float32)
         self.lgn_fr = np.zeros((self.tm["ntPerFrame"], sp.nLGN), np.float32)
+        # plasticity (learning 3): the LGN trace and its per-step values, the
+        # post traces and the homeostatic totals (zero / the initial sums)
+        self.learn = getattr(sp, "learning", 0) == 3
+        if self.learn:
+            if self.batch is not None:
+                raise NotImplementedError("plasticity with batched trials is not built")
+            self.learn_p = params.learn_ff_struct(net)
+            self.xVar = np.zeros(sp.nLGN, np.float32)
+            self.xStart = np.zeros_like(self.lgn_sInfo)
+            self.xSpike = np.zeros_like(self.lgn_sInfo)
+            self.lvar = {k: np.zeros(sp.nV1, np.float32) for k in ("vLTD", "vTrip", "vAvg")}
+            self.totalFF = self.totalFFinf = None
+            if sp.applyHomeo:
+                self.totalFF, self.totalFFinf = params.learn_ff_totals(net)
         self.it = 0
         self.sInfo_hist = []
         self.lgn_hist = []
@@ -336,12 +353,14 @@ class OracleSim:
                         self.Rring[fidx % self.ringF] = self.frame_response(
                             frames[fidx % frames.shape[0]])
                 cv = np.zeros((npf, sp.nLGN), np.float32) if lgn_only else None
-                self.o.L.orc_lgn_steps(C.byref(self.lp), _fp(t["TW"]), _fp(kC), _fp(kS),
-                                       _fp(t["maxConvol"]), _fp(self.lgn["c50"]),
-                                       _fp(self.lgn["sharpness"]), _fp(self.lgn["logA"]),
-                                       _fp(self.lgn["logB"]), _fp(self.lTR), _fp(self.lNL),
-                                       _fp(self.Rring), it, npf, _fp(self.lgn_sInfo),
-                                       _fp(self.lgn_fr), _fp(cv))
+                lrn = self.learn
+                self.o.L.orc_lgn_steps_learn(
+                    C.byref(self.lp), _fp(t["TW"]), _fp(kC), _fp(kS), _fp(t["maxConvol"]),
+                    _fp(self.lgn["c50"]), _fp(self.lgn["sharpness"]), _fp(self.lgn["logA"]),
+                    _fp(self.lgn["logB"]), _fp(self.lTR), _fp(self.lNL), _fp(self.Rring), it,
+                    npf, _fp(self.lgn_sInfo), _fp(self.lgn_fr), _fp(cv),
+                    self.learn_p.tauLTP if lrn else 0.0, _fp(self.xVar) if lrn else None,
+                    _fp(self.xStart) if lrn else None, _fp(self.xSpike) if lrn else None)
                 if record:
                     self.lgn_hist.append(self.lgn_sInfo.copy())
                     self.fr_hist.append(self.lgn_fr.copy())
@@ -387,6 +406,16 @@ class OracleSim:
                     self._gap(self.train, it % self.depth)
                 else:
                     self._gap(out[None, :], 0)
+            if self.learn:                  # the step's plasticity (end of compute_V)
+                lv = self.lvar
+                rc = self.o.L.orc_v1_learn_ff(
+                    C.byref(self.p), C.byref(self.learn_p), _fp(self.nLGN_of), _fp(self.sLGN),
+                    _fp(self.idLGN), _fp(lv["vLTD"]), _fp(lv["vTrip"]), _fp(lv["vAvg"]),
+                    _fp(self.totalFF), _fp(self.totalFFinf), _fp(out), _fp(lgn_row),
+                    _fp(np.ascontiguousarray(self.xStart[k_in])),
+                    _fp(np.ascontiguousarray(self.xSpike[k_in])))
+                if rc != 0:
+                    raise RuntimeError(f"orc_v1_learn_ff returned {rc}")
             if record:
                 self.sInfo_hist.append(out)
             self.it += 1

@@ -30,6 +30,10 @@
  *     gather             delays from a ring of past step outputs)
  *   - far connections    src/coredynamics.cu:181-372, 2314-2349 (recal_G_vec
  *                        with delays, sum_G)
+ *   - plasticity,        src/discrete_input_convol.cu:1736-1821 (the LGN
+ *     learning = 3       trace), src/coredynamics.cu:1631-1992 (the FF_E
+ *                        triplet rule, homeostasis, clip), pinned by float64
+ *                        sums over spike times (tests/test_learn_oracle.py)
  * Reductions follow the reference's block_reduce tree (32-lane shuffle-down
  * halving per warp, then halving over the warp partials,
  * src/util/cuda_util.cuh:22-127).
@@ -1243,7 +1247,8 @@ static float orc_lgn_cell_step(const vc_lgn_params *p, const float *TW,
                                const float *K, const float *A, const float *B,
                                float *leftTimeRate, float *lastNegLogRand,
                                const float *Rring, uint32_t c, uint32_t it,
-                               float *fr_out, float *convol_out) {
+                               float *fr_out, float *convol_out, uint32_t *nsp_out,
+                               float *tsp_out) {
     const uint32_t nKS = p->nKernelSample;
     float prod[2][1024];
     for (int r = 0; r < 2; ++r) {
@@ -1313,7 +1318,59 @@ static float orc_lgn_cell_step(const vc_lgn_params *p, const float *TW,
     float sInfo = (float)nsp + tsp / p->dt;
     if (sInfo > 0.0f) lastNegLogRand[c] = lNL;
     leftTimeRate[c] = lTR;
+    if (nsp_out) *nsp_out = nsp;
+    if (tsp_out) *tsp_out = tsp;
     return sInfo;
+}
+
+/* The LTP trace of an LGN cell over one step (LGN_nonlinear's learning part,
+ * src/discrete_input_convol.cu:1736-1743, 1775-1821): x is the value at the
+ * step start (the surface slot varSlot); with a spike the step's nsp spikes
+ * are added and then x decays to their mean time tsp (the value kept in the
+ * third slot), then to the step end (the other slot); without one x decays
+ * over dt. */
+static void orc_lgn_trace_step(float *x, uint32_t nsp, float tsp, float sInfo, float dt,
+                               float tau, float *xStart, float *xSpike) {
+    float v = *x;
+    *xStart = v;
+    float rest = dt;
+    if (sInfo > 0.0f) {
+        v = v + (float)nsp;
+        v = v * vc_expf(-tsp / tau);
+        rest = dt - tsp;
+    }
+    *xSpike = v;
+    v = v * vc_expf(-rest / tau);
+    *x = v;
+}
+
+/* orc_lgn_steps with the cells' LTP trace (xVar [nLGN] in/out; xStart,
+ * xSpike [nsteps][nLGN] out; xVar NULL = no trace). */
+void orc_lgn_steps_learn(const vc_lgn_params *p, const float *TW, const float *kC,
+                         const float *kS, const float *maxConvol, const float *c50,
+                         const float *K, const float *A, const float *B,
+                         float *leftTimeRate, float *lastNegLogRand,
+                         const float *Rring, uint32_t it0, uint32_t nsteps,
+                         float *sInfo, float *fr, float *convol, float tauLTP,
+                         float *xVar, float *xStart, float *xSpike) {
+    for (uint32_t s = 0; s < nsteps; ++s) {
+        uint32_t it = it0 + s;
+#pragma omp parallel for schedule(static)
+        for (uint32_t c = 0; c < p->nLGN; ++c) {
+            float frv, cv, tsp;
+            uint32_t nsp;
+            float si = orc_lgn_cell_step(p, TW, kC, kS, maxConvol, c50, K, A, B,
+                                         leftTimeRate, lastNegLogRand, Rring, c,
+                                         it, &frv, &cv, &nsp, &tsp);
+            size_t o = (size_t)s * p->nLGN + c;
+            sInfo[o] = si;
+            if (fr) fr[o] = frv;
+            if (convol) convol[o] = cv;
+            if (xVar)
+                orc_lgn_trace_step(&xVar[c], nsp, tsp, si, p->dt, tauLTP, &xStart[o],
+                                   &xSpike[o]);
+        }
+    }
 }
 
 void orc_lgn_steps(const vc_lgn_params *p, const float *TW, const float *kC,
@@ -1322,19 +1379,113 @@ void orc_lgn_steps(const vc_lgn_params *p, const float *TW, const float *kC,
                    float *leftTimeRate, float *lastNegLogRand,
                    const float *Rring, uint32_t it0, uint32_t nsteps,
                    float *sInfo, float *fr, float *convol) {
-    for (uint32_t s = 0; s < nsteps; ++s) {
-        uint32_t it = it0 + s;
+    orc_lgn_steps_learn(p, TW, kC, kS, maxConvol, c50, K, A, B, leftTimeRate,
+                        lastNegLogRand, Rring, it0, nsteps, sInfo, fr, convol, 0.0f, NULL,
+                        NULL, NULL);
+}
+
+/* One step of the feed-forward plasticity with learning = 3 (only the FF_E
+ * rule), compute_V_collect_spike_learnFF_fast, src/coredynamics.cu:1631-1992:
+ * for each excitatory neuron (local id < nE), row 0 of its post variables is
+ * the value at the step start and row 1 the value at its spike time; the
+ * homeostatic change of its total (:1765-1800); per synapse with a strength
+ * (:1803-1927) the LTD term at the LGN cell's spike (whether or not the
+ * neuron spiked) and the LTP term at the neuron's spike (whether or not the
+ * cell spiked), the clip; then the post variables take the step's spikes and
+ * decay to the step end (:1930-1960).  sLGN is updated in place; the trace
+ * arrays are [nV1] (inhibitory entries untouched).  lgnS, xStart, xSpike are
+ * the step's rows over the LGN cells. */
+int orc_v1_learn_ff(const vc_v1_params *p, const vc_learn_ff_params *lp,
+                    const uint32_t *nLGN, float *sLGN, const uint32_t *idLGN,
+                    float *vLTD, float *vTrip, float *vAvg, float *totalFF,
+                    const float *totalFFinf, const float *sInfo, const float *lgnS,
+                    const float *xStart, const float *xSpike) {
+    if (p->blockTrial) return -2;
 #pragma omp parallel for schedule(static)
-        for (uint32_t c = 0; c < p->nLGN; ++c) {
-            float frv, cv;
-            float si = orc_lgn_cell_step(p, TW, kC, kS, maxConvol, c50, K, A, B,
-                                         leftTimeRate, lastNegLogRand, Rring, c,
-                                         it, &frv, &cv);
-            sInfo[(size_t)s * p->nLGN + c] = si;
-            if (fr) fr[(size_t)s * p->nLGN + c] = frv;
-            if (convol) convol[(size_t)s * p->nLGN + c] = cv;
+    for (uint32_t n = 0; n < p->nV1; ++n) {
+        if (n % p->blockSize >= p->nE) continue;
+        const float dt = p->dt;
+        float nsp = 0.0f, tsp = dt;
+        if (sInfo[n] > 0.0f) {
+            nsp = floorf(sInfo[n]);
+            tsp = (sInfo[n] - nsp) * dt;
         }
+        /* lFF[row][0] = LTD, lFF[row][1] = triplet; lAvg[row] */
+        float lFF[2][2] = {{vLTD[n], vTrip[n]}, {0.0f, 0.0f}};
+        float lAvg[2] = {vAvg[n], 0.0f};
+        if (nsp > 0.0f) {
+            lFF[1][0] = lFF[0][0] * vc_expf(-tsp / lp->tauLTD);
+            lFF[1][1] = lFF[0][1] * vc_expf(-tsp / lp->tauTrip);
+            lAvg[1] = lAvg[0] * vc_expf(-tsp / lp->tauAvg);
+        }
+        uint32_t m = nLGN[n];
+        float f0tot = 0.0f, fnew = 0.0f, hc = 0.0f, df_tot = 0.0f;
+        if (lp->applyHomeo) {
+            f0tot = totalFF[n];
+            float finf = totalFFinf[n];
+            float d = f0tot - finf;
+            if (d > 0.0f || lp->symmetricHomeo) {
+                fnew = d * lp->expHomeo + finf;
+                hc = lp->applyHomeo == 1 ? fnew / f0tot : (fnew - f0tot) / (float)m;
+            } else {
+                fnew = f0tot;
+                hc = lp->applyHomeo == 1 ? 1.0f : 0.0f;
+            }
+        }
+        for (uint32_t i = 0; i < m; ++i) {
+            size_t lid = (size_t)i * p->nV1 + n;
+            float f = sLGN[lid];
+            if (f == 0.0f && !lp->rebound) continue;
+            if (lp->applyHomeo == 1) f *= hc;
+            if (lp->applyHomeo == 2) f += hc;
+            uint32_t c = idLGN[lid];
+            float sF = lgnS[c];
+            float nspF = floorf(sF);
+            float tspF = (sF > 0.0f ? sF - nspF : 1.0f) * dt;
+            if (nspF > 0.0f) {                 /* LTD */
+                int row;
+                float del;
+                if (tspF < tsp) { row = 0; del = tspF; }
+                else            { row = 1; del = tspF - tsp; }
+                float A_LTD = lp->A_ratio;
+                if (lp->targetFR > 0.0f) A_LTD *= lp->tauLTP * lAvg[row] * lAvg[row];
+                float df = lFF[row][0] * vc_expf(-del / lp->tauLTD) * A_LTD;
+                f -= df;
+                if (lp->applyHomeo) df_tot -= df;
+            }
+            if (nsp > 0.0f) {                  /* LTP */
+                float pre, del;
+                if (tspF < tsp) { pre = xSpike[c]; del = tsp - tspF; }
+                else            { pre = xStart[c]; del = tsp; }
+                float df = pre * vc_expf(-del / lp->tauLTP) * lFF[1][1] * lp->A_LTP;
+                if (lp->applyHomeo) df_tot += df;
+                f += df;
+            }
+            if (f < lp->gmin) {
+                if (lp->applyHomeo) df_tot += lp->gmin - f;
+                f = lp->gmin;
+            }
+            if (f > lp->gmax) {
+                if (lp->applyHomeo) df_tot -= f - lp->gmax;
+                f = lp->gmax;
+            }
+            sLGN[lid] = f;
+        }
+        if (lp->applyHomeo) totalFF[n] = fnew + df_tot;
+        int row = nsp > 0.0f ? 1 : 0;
+        float del = dt;
+        if (nsp > 0.0f) del -= tsp;
+        lFF[row][0] += nsp;
+        lFF[row][1] += nsp;
+        lFF[row][0] = lFF[row][0] * vc_expf(-del / lp->tauLTD);
+        lFF[row][1] = lFF[row][1] * vc_expf(-del / lp->tauTrip);
+        lAvg[row] += nsp;
+        lAvg[row] = lAvg[row] * vc_expf(-del / lp->tauAvg);
+        vLTD[n] = lFF[row][0];
+        vTrip[n] = lFF[row][1];
+        vAvg[n] = lAvg[row];
     }
+    return 0;
 }
 
 void orc_lgn_init(const vc_lgn_params *p, float *leftTimeRate,

@@ -1,0 +1,345 @@
+"""Pins of the oracle's feed-forward plasticity (learning = 3: the reference's
+FF_E triplet rule, src/coredynamics.cu:1631-1992, its LGN trace,
+src/discrete_input_convol.cu:1736-1821, and its constants, src/condShape.h
+:94-155, src/patch.cu:1092-1330, 4332-4352).
+
+The checks do not follow the oracle's per-step recursion.  They recompute
+what the rule adds up to from the run's spike times, in float64:
+
+* the LGN trace at each step start is the sum over the cell's earlier spikes
+  of nsp exp(-(t - t_j)/tauLTP) with each spike placed at the START of its
+  step (the reference adds the count before decaying to the spike time), and
+  the at-spike value is (start value + nsp) exp(-tsp/tauLTP);
+* a strength's final value is its initial value plus, over every LGN spike
+  at time tp, -A_LTD times the post LTD trace at tp summed over the post
+  spikes of earlier steps (a post spike earlier in the same step is not
+  counted: the reference reads its row at the spike before the increment),
+  and over every post spike at time t, +A_LTP times the pre trace at t
+  (pre spikes of earlier steps, and of the same step when earlier) times
+  the post triplet trace of earlier post spikes;
+* with targetFR > 0 the LTD amplitude is A_ratio tauLTP avg^2, avg the
+  filtered post spike count at the step start (pre before post) or at the
+  post spike (pre after post);
+* the post traces at the end are the sums over all post spikes at their
+  actual times;
+* homeostasis without spikes relaxes each neuron's total exponentially
+  towards its target, total_T = inf + (total_0 - inf) expHomeo^T, and scales
+  (applyHomeo 1) or shifts (2) every strength accordingly; with learning
+  events and clips the kept total stays the sum of the strengths;
+* strengths of exactly 0 stay 0 without rebound; inhibitory neurons are
+  untouched.
+
+Mistaken variants of the float64 model (a same-step post spike counted in
+LTD, pre spikes placed at their spike times, the rate read at the pre spike)
+each disagree with the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+LEARN = dict(learning=3, A_LGN=0.05, r_LTD=1.0, tauLTP=16.8, tauLTD=33.7, tauTrip=114.0,
+             tauAvg=(1000.0, 1000.0), targetFR=(0.0, 0.0), gmaxLGN=100.0, gminLGN=-100.0,
+             bgStrength=(0.4, 0.4), tau=20.0, nKernelSample=20, frRatioLGN=200.0,
+             contrast=0.9, nLGN_side=12, m=12)
+
+
+def _net(**kw):
+    from visual_cortex_amd import network
+    base = dict(LEARN)
+    base.update(kw)
+    return network.build_network(network.spec_tiny(**base))
+
+
+def _run(pkg, net, T, sLGN=None):
+    import oracle
+    if sLGN is not None:
+        net.sLGN = np.ascontiguousarray(sLGN, np.float32)
+    o = oracle.OracleSim(net, pkg)
+    s0 = o.sLGN.copy()
+    o.run(T)
+    lgn = np.concatenate(o.lgn_hist)[:T]
+    return o, s0, o.sInfo(), lgn
+
+
+def _spikes(row_vals, dt):
+    """(count, time within the step) of sInfo values (0 count when < 1)."""
+    n = np.where(row_vals >= 1.0, np.floor(row_vals), 0.0)
+    t = np.where(row_vals >= 1.0, (row_vals - n) * dt, dt)
+    return n, t
+
+
+def stdp64(net, sv1, slgn, s0, mistake=None):
+    """Final strengths [maxLGN][nV1] and post traces (float64) from the
+    spike times, as the docstring of this module describes."""
+    sp = net.spec
+    dt = sp.dt
+    T = sv1.shape[0]
+    from visual_cortex_amd.params import learn_ff_struct
+    lp = learn_ff_struct(net)
+    tLTP, tLTD, tTrip, tAvg = (float(x) for x in (lp.tauLTP, lp.tauLTD, lp.tauTrip, lp.tauAvg))
+    A_LTP, A_ratio, target = float(lp.A_LTP), float(lp.A_ratio), float(lp.targetFR)
+    nV, tV = _spikes(sv1.astype(np.float64), dt)            # [T][nV1]
+    nL, tL = _spikes(slgn.astype(np.float64), dt)           # [T][nLGN]
+    f = s0.astype(np.float64).copy()
+
+    def post_sum(n, t_abs, tau, kmax):
+        k = np.nonzero(nV[:kmax, n])[0]
+        tk = k * dt + tV[k, n]
+        return float((nV[k, n] * np.exp(-(t_abs - tk) / tau)).sum())
+
+    for n in range(sp.nV1):
+        if n % sp.blockSize >= sp.nE:
+            continue
+        post_steps = np.nonzero(nV[:, n])[0]
+        for i in range(int(net.nLGN_of[n])):
+            if s0[i, n] == 0.0:
+                continue
+            c = int(net.idLGN[i, n])
+            pre_steps = np.nonzero(nL[:, c])[0]
+            d = 0.0
+            for t in pre_steps:                                # LTD
+                tp = t * dt + tL[t, c]
+                kmax = t + 1 if (mistake == "ltd_same_step_post" and tV[t, n] < tL[t, c]) else t
+                o1 = post_sum(n, tp, tLTD, kmax)
+                A = A_ratio
+                if target > 0:
+                    early = tL[t, c] < tV[t, n]
+                    tr = tp if mistake == "avg_at_pre" else t * dt + (0.0 if early else tV[t, n])
+                    av = post_sum(n, tr, tAvg, t)
+                    A *= float(lp.tauLTP) * av * av
+                d -= o1 * A
+            for t in post_steps:                               # LTP
+                tq = t * dt + tV[t, n]
+                j = pre_steps[(pre_steps < t) | ((pre_steps == t) & (tL[t, c] < tV[t, n]))]
+                tj = j * dt + (tL[j, c] if mistake == "pre_at_spike_time" else 0.0)
+                r1 = float((nL[j, c] * np.exp(-(tq - tj) / tLTP)).sum())
+                o2 = post_sum(n, tq, tTrip, t)
+                d += A_LTP * r1 * o2
+            f[i, n] += d
+    Tend = T * dt
+    traces = {}
+    for key, tau in (("vLTD", tLTD), ("vTrip", tTrip), ("vAvg", tAvg)):
+        v = np.zeros(sp.nV1)
+        for n in range(sp.nV1):
+            if n % sp.blockSize < sp.nE:
+                v[n] = post_sum(n, Tend, tau, T)
+        traces[key] = v
+    return f, traces
+
+
+def test_lgn_trace_closed_form(pkg):
+    """The LGN trace at every step start and spike against the float64 sums
+    over the cells' spikes placed at their step starts."""
+    net = _net(nt=200)
+    sp = net.spec
+    import oracle
+    o = oracle.OracleSim(net, pkg)
+    npf = o.tm["ntPerFrame"]
+    nL_all, xs, xk = [], [], []
+    for _ in range(4):
+        o.run(npf)
+        nL_all.append(o.lgn_hist[-1])
+        xs.append(o.xStart.copy())
+        xk.append(o.xSpike.copy())
+    L = np.concatenate(nL_all).astype(np.float64)
+    XS, XK = np.concatenate(xs), np.concatenate(xk)
+    n, tsp = _spikes(L, sp.dt)
+    assert (n >= 1).sum() > 50 and (n >= 2).sum() > 0         # multi-spike steps too
+    tau = float(np.float32(sp.tauLTP))
+    T = L.shape[0]
+    want_s = np.zeros_like(L)
+    for t in range(1, T):
+        j = np.arange(t)
+        want_s[t] = (n[:t] * np.exp(-((t - j) * sp.dt)[:, None] / tau)).sum(0)
+    want_k = np.where(n >= 1, (want_s + n) * np.exp(-tsp / tau), want_s)
+    np.testing.assert_allclose(XS, want_s, rtol=2e-5, atol=1e-6)
+    np.testing.assert_allclose(XK, want_k, rtol=2e-5, atol=1e-6)
+    # a variant adding the count after the decay to the spike time fails
+    bad = np.where(n >= 1, want_s * np.exp(-tsp / tau) + n, want_s)
+    assert not np.allclose(XK, bad, rtol=2e-5, atol=1e-6)
+
+
+@pytest.fixture(scope="module")
+def stdp_runs(pkg):
+    out = {}
+    for name, kw in (("plain", {}), ("target", dict(targetFR=(5.0, 5.0), tauAvg=(200.0, 200.0)))):
+        net = _net(nt=600, **kw)
+        o, s0, sv1, slgn = _run(pkg, net, 360)
+        out[name] = (net, o, s0, sv1, slgn)
+    return out
+
+
+@pytest.mark.parametrize("name", ["plain", "target"])
+def test_stdp_event_sums(pkg, stdp_runs, name):
+    net, o, s0, sv1, slgn = stdp_runs[name]
+    sp = net.spec
+    assert (sv1 >= 1).sum() > 1000 and (slgn >= 1).sum() > 200
+    f, tr = stdp64(net, sv1, slgn, s0)
+    E = (np.arange(sp.nV1) % sp.blockSize) < sp.nE
+    moved = np.abs(f - s0)[:, E].max()
+    assert moved > 0.05                                    # the rule did something
+    np.testing.assert_allclose(o.sLGN[:, E], f[:, E], rtol=0, atol=2e-4 * max(1.0, moved))
+    assert np.array_equal(o.sLGN[:, ~E], s0[:, ~E])        # inhibitory neurons untouched
+    for k, v in tr.items():
+        np.testing.assert_allclose(o.lvar[k][E], v[E], rtol=2e-5, atol=1e-6, err_msg=k)
+        assert (o.lvar[k][~E] == 0).all()
+
+
+@pytest.mark.parametrize("name,mistake", [("plain", "ltd_same_step_post"),
+                                          ("plain", "pre_at_spike_time"),
+                                          ("target", "avg_at_pre")])
+def test_stdp_pin_detects_mistakes(pkg, stdp_runs, name, mistake):
+    net, o, s0, sv1, slgn = stdp_runs[name]
+    sp = net.spec
+    f, _ = stdp64(net, sv1, slgn, s0, mistake=mistake)
+    E = (np.arange(sp.nV1) % sp.blockSize) < sp.nE
+    moved = np.abs(f - s0)[:, E].max()
+    assert np.abs(o.sLGN[:, E] - f[:, E]).max() > 2e-4 * max(1.0, moved)
+
+
+def _silent(**kw):
+    # no V1 and no LGN spikes: tonic drive below threshold, no background,
+    # no LGN rate
+    return _net(bgStrength=(0.0, 0.0), tonicDep=(0.2, 0.2), frRatioLGN=0.0, **kw)
+
+
+@pytest.mark.parametrize("homeo,sym,ratio", [(1, 1, 0.5), (2, 1, 0.5), (1, 1, 2.0), (1, 0, 2.0),
+                                             (2, 0, 0.5)])
+def test_homeostasis_closed_form(pkg, homeo, sym, ratio):
+    net = _silent(applyHomeo=homeo, symmetricHomeo=sym, FF_InfRatio=ratio, decayTimeHomeoFF=5.0,
+                  gmaxLGN=100.0, gminLGN=-100.0)
+    sp = net.spec
+    T = 120
+    o, s0, sv1, slgn = _run(pkg, net, T)
+    assert (sv1 >= 1).sum() == 0 and (slgn >= 1).sum() == 0
+    from visual_cortex_amd.params import learn_ff_totals
+    tot0, inf = (x.astype(np.float64) for x in learn_ff_totals(net))
+    E = (np.arange(sp.nV1) % sp.blockSize) < sp.nE
+    a = math.exp(-sp.dt / sp.decayTimeHomeoFF)
+    relax = (tot0 - inf > 0) | bool(sym)
+    totT = np.where(relax, inf + (tot0 - inf) * a ** T, tot0)
+    m = net.nLGN_of.astype(np.float64)
+    live = s0 != 0
+    if homeo == 1:
+        want = np.where(live, s0 * (totT / tot0)[None, :], s0)
+    else:
+        want = np.where(live, s0 + ((totT - tot0) / m)[None, :], s0)
+    np.testing.assert_allclose(o.sLGN[:, E], want[:, E], rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(o.totalFF[E], totT[E], rtol=1e-4)
+    if not sym and ratio > 1:
+        assert np.array_equal(o.sLGN, s0)
+
+
+@pytest.mark.parametrize("homeo", [1, 2])
+def test_homeostatic_total_tracks_the_strengths(pkg, homeo):
+    """With learning events and clips, the kept total of each excitatory
+    neuron stays the sum of its strengths (the LTD/LTP/clip changes enter
+    delta_f); a small gmax makes the clips happen."""
+    net = _net(nt=400, applyHomeo=homeo, FF_InfRatio=0.8, decayTimeHomeoFF=50.0, gmaxLGN=1.25,
+               gminLGN=0.95, A_LGN=0.2)
+    sp = net.spec
+    s = net.sLGN.copy()
+    for n in range(sp.nV1):                                 # no pruned entries
+        k = int(net.nLGN_of[n])
+        s[:k, n] = np.where(s[:k, n] == 0, 0.5, s[:k, n])
+    o, s0, sv1, slgn = _run(pkg, net, 240, sLGN=s)
+    E = (np.arange(sp.nV1) % sp.blockSize) < sp.nE
+    assert (s0[:, E] > 1.25).any() and (s0[:, E] < 0.95).any() and (sv1 >= 1).sum() > 500
+    assert o.sLGN[:, E].max() <= np.float32(1.25) and o.sLGN[:, E].min() >= np.float32(0.95)
+    sums = o.sLGN.astype(np.float64).sum(0)
+    np.testing.assert_allclose(o.totalFF[E], sums[E], rtol=2e-5)
+
+
+def test_pruned_strengths_and_rebound(pkg):
+    net = _net(nt=400, A_LGN=0.2)
+    sp = net.spec
+    s = net.sLGN.copy()
+    E = (np.arange(sp.nV1) % sp.blockSize) < sp.nE
+    s[0, E] = 0.0                                          # the first input of every E neuron
+    o, s0, sv1, slgn = _run(pkg, net, 200, sLGN=s)
+    assert (o.sLGN[0, E] == 0).all()
+    assert (o.sLGN[1:, E] != s0[1:, E]).any()
+    o2, _, _, _ = _run(pkg, _net(nt=400, A_LGN=0.2, rebound=1), 200, sLGN=s)
+    assert (o2.sLGN[0, E] != 0).sum() > 10                 # LTP regrows them
+
+
+def test_rule_constants(pkg):
+    """learnFF_post's LTD amplitude with a target rate: r_LTD A_LGN sRatio
+    tauTrip / (tauLTD tauAvg_s^2) / 1000 / targetFR; the bounds scaled by
+    sRatioLGN[0]; exp_homeo = exp(-dt/decayTimeHomeoFF)."""
+    from visual_cortex_amd.params import learn_ff_struct, learn_ff_totals
+    net = _net(sRatioLGN=(0.7, 0.3), r_LTD=0.9, A_LGN=0.025, tauTrip=90.0, tauLTD=26.0,
+               tauAvg=(1000.0, 500.0), targetFR=(5.0, 3.0), gmaxLGN=3.0, gminLGN=0.1,
+               decayTimeHomeoFF=250.0)
+    lp = learn_ff_struct(net)
+    A_LTP = 0.025 * 0.7
+    want = 0.9 * A_LTP * 90.0 / (26.0 * 1.0 ** 2) / 1000.0 / 5.0
+    assert lp.A_LTP == pytest.approx(A_LTP, rel=1e-6)
+    assert lp.A_ratio == pytest.approx(want, rel=1e-6)
+    assert (lp.gmax, lp.gmin) == (pytest.approx(2.1, rel=1e-6), pytest.approx(0.07, rel=1e-6))
+    assert lp.expHomeo == pytest.approx(math.exp(-net.spec.dt / 250.0), rel=1e-7)
+    net0 = _net(targetFR=(0.0, 0.0), r_LTD=0.9, A_LGN=0.025, sRatioLGN=(0.7, 0.3))
+    assert learn_ff_struct(net0).A_ratio == pytest.approx(0.9 * A_LTP, rel=1e-6)
+    tot, inf = learn_ff_totals(_net(FF_InfRatio=0.6))
+    net1 = _net(FF_InfRatio=0.6)
+    want_tot = np.array([net1.sLGN[:int(k), n].astype(np.float64).sum()
+                         for n, k in enumerate(net1.nLGN_of)])
+    np.testing.assert_allclose(tot, want_tot, rtol=1e-6)
+    np.testing.assert_allclose(inf, 0.6 * want_tot, rtol=1e-6)
+
+
+def test_learning_config_keys(pkg):
+    from visual_cortex_amd import config
+    base = ("learning = 3\ntauLTP = 13\ntauLTD = 26\ntauTrip = 90.0\nr_LTD = 1.0\nA_LGN = 0.025\n"
+            "tauAvg = 1000.0\ntargetFR = 5.0, 5.0\ngmaxLGN = 3.0\ngminLGN = 0.0\n")
+    sp = config.spec_from_config(config.Config(base))
+    assert (sp.learning, sp.tauLTP, sp.tauTrip, sp.A_LGN, sp.targetFR, sp.learnData_FF) == \
+        (3, 13.0, 90.0, 0.025, (5.0, 5.0), 1)
+    assert sp.tauAvg == (1000.0, 1000.0) and sp.applyHomeo == 0 and sp.symmetricHomeo == 1
+    for mode in (1, 2, 4):
+        with pytest.raises(NotImplementedError, match="learning"):
+            config.spec_from_config(config.Config(base.replace("learning = 3", f"learning = {mode}")))
+    with pytest.raises(ValueError, match="tauAvg"):
+        config.spec_from_config(config.Config(base.replace("tauAvg = 1000.0\n", "")))
+    with pytest.raises(ValueError, match="tauTrip"):
+        config.spec_from_config(config.Config(base.replace("tauLTD = 26", "tauLTD = 26, 30")))
+    with pytest.raises(ValueError, match="tauQ"):
+        config.spec_from_config(config.Config(base + "tauQ = 20.0\n"))
+    with pytest.raises(NotImplementedError, match="learnData_FF"):
+        config.spec_from_config(config.Config(base + "learnData_FF = 2\n"))
+
+
+def test_learning_refusals(pkg):
+    """Plastic runs of batched trials are refused."""
+    import oracle
+    from visual_cortex_amd import batch, network
+    nets = [_net(seed=5 + i) for i in range(2)]
+    b = batch.batch_networks(nets, device="cpu")
+    with pytest.raises(NotImplementedError, match="batched"):
+        oracle.OracleSim(b, pkg)
+    assert network.spec_tiny().learning == 0
+
+
+def test_slgn_file_layout(pkg, tmp_path):
+    """The sLGN file of learnData_FF = 1: the reference's header field order
+    and sizes (src/patch.cu:4767-4777), then one [max_LGNperV1][nV1] record
+    per sampleInterval_LGN_V1 steps."""
+    import struct
+    from visual_cortex_amd import formats
+    net = _net(nt=250, sampleInterval_LGN_V1=100, sRatioLGN=(0.7, 0.3), gmaxLGN=3.0,
+               FF_InfRatio=0.8)
+    sp = net.spec
+    path = tmp_path / "sLGN_x.bin"
+    w = formats.SLGNWriter(path, sp, net.sLGN.shape[0])
+    recs = [net.sLGN + k for k in range(2)]
+    for r in recs:
+        w.write(r)
+    w.close()
+    b = path.read_bytes()
+    head = 4 * (2 + 1 + 2 + 2 + 2 + 1 + 1)
+    assert len(b) == head + 2 * net.sLGN.size * 4
+    assert struct.unpack_from("<IIfII2fII2f", b, 0) == pytest.approx(
+        (250, 100, sp.dt, sp.nV1, net.sLGN.shape[0], 0.7, 0.3, 1, 0, 3.0, 0.8), rel=1e-6)
+    d = formats.read_slgn(path)
+    assert d["sLGN"].shape == (2,) + net.sLGN.shape
+    assert np.array_equal(d["sLGN"][1], recs[1].astype(np.float32))

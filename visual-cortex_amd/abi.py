@@ -107,6 +107,18 @@ class V1SpInit(C.Structure):
     _fields_ = [("sp0", _arr(f32, VC_MAX_NTYPE * 2)), ("SCsplit", u32), ("depthRef", u32)]
 
 
+class LearnFFParams(C.Structure):
+    _fields_ = [("tauLTP", f32), ("tauLTD", f32), ("tauTrip", f32), ("tauAvg", f32),
+                ("A_LTP", f32), ("A_ratio", f32), ("targetFR", f32), ("gmax", f32),
+                ("gmin", f32), ("applyHomeo", u32), ("symmetricHomeo", u32),
+                ("expHomeo", f32), ("rebound", u32)]
+
+
+class LearnFFState(C.Structure):
+    _fields_ = [("sLGN", vp), ("vLTD", vp), ("vTrip", vp), ("vAvg", vp), ("totalFF", vp),
+                ("totalFFinf", vp)]
+
+
 class LgnSetup(C.Structure):
     _fields_ = [
         ("nLGN", u32), ("nSample1D", u32), ("nKernelSample", u32),
@@ -191,6 +203,10 @@ SIGNATURES = {
                                     C.c_uint64, vp],
     "vc_lgn_steps": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp, vp,
                      C.POINTER(FFEvents)],
+    "vc_lgn_steps_learn": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp,
+                           vp, C.POINTER(FFEvents), C.c_float, vp, vp, vp],
+    "vc_v1_learn_ff": [vp, C.POINTER(V1Params), C.POINTER(LearnFFParams), C.POINTER(FFConn),
+                       C.POINTER(LearnFFState), vp, vp, vp, vp],
     "vc_lgn_init": [vp, C.POINTER(LgnParams), vp, vp],
     "vc_lgn_frame_responses": [vp, vp, u32, u32, u32, u32, u32, vp, vp, vp, vp, vp],
     "vc_lgn_sample_geometry": [vp, C.POINTER(LgnSetup), vp],

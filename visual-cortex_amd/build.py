@@ -10,7 +10,8 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 SRC = [PKG / "csrc" / f for f in ("vc_abi.hip", "vc_lgn.hip", "vc_lgn_adjoint.hip", "vc_v1.hip",
-                               "vc_v1_adjoint.hip", "vc_v1_delay.hip", "vc_v1_init.hip")]
+                               "vc_v1_adjoint.hip", "vc_v1_delay.hip", "vc_v1_init.hip",
+                               "vc_v1_learn.hip")]
 OUT = PKG / "libvc_hip.so"
 OBJ_DIR = PKG / "_build"
 

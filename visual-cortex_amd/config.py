@@ -35,6 +35,7 @@ FILES = {
     "fRawData": ("rawData", "outputFolder", "output_suffix"),
     "fLGN_fr": ("LGN_fr", "outputFolder", "output_suffix"),
     "fLGN_sp": ("LGN_sp", "outputFolder", "output_suffix"),
+    "f_sLGN": ("sLGN", "outputFolder", "output_suffix"),                   # src/patch.cu:367, 4755
     "fLGN_surfaceID": ("LGN_surfaceID", "resourceFolder", "res_suffix"),   # src/patch.cu:2573
     "fConStats": ("conStats", "inputFolder", "conV1_suffix"),              # src/patch.cu:354, 477
     "fV1_feature": ("V1_feature", "resourceFolder", "res_suffix"),         # src/patch.cu:350, 422
@@ -48,6 +49,9 @@ VECTOR_KEYS = {
     "pFF", "pE", "pI", "vR", "vThres", "gL", "C", "tRef", "vT", "deltaT", "tau_w", "a", "b",
     "nTypeHierarchy", "spE0", "spI0", "noisyDep", "tonicDep", "minTonicRatio", "synFailFF",
     "synFail", "boostOri", "iOri", "bgStrength", "pBG",
+    # plasticity (src/patch.cu:239-258)
+    "A_LGN", "r_LTD", "A_V1", "A_Q", "tauQ", "tauLTP", "tauLTD", "tauTrip", "tauAvg",
+    "targetFR", "gmaxLGN", "gmaxE", "gmaxQ", "gminLGN", "gminE", "gminQ",
 }
 
 
@@ -161,6 +165,62 @@ def _pairs(vals, n_types, per):
     return tuple(tuple(vals[t * per:(t + 1) * per]) for t in range(n_types))
 
 
+def learning_from_config(cfg: Config, dt: float) -> dict:
+    """The plasticity keys (src/patch.cu:219-258, 289, 1092-1330).  learning 3
+    (only the FF_E triplet rule) is built; 1, 2 and 4 (cortical E->E and
+    I->E rules) are refused.  The reference's checks that apply to learning 3
+    are made: tauAvg and targetFR of size 1 or 2, tauQ/A_Q/gmaxQ/gminQ of one
+    size, tauLTP/tauLTD/r_LTD of tauTrip's size (one rule, max_nLearnTypeFF_E
+    = 1), gmaxLGN/gminLGN of size 1 (or the number of FF rules)."""
+    mode = cfg.int("learning", 0)
+    if mode == 0:
+        return {}
+    if mode != 3:
+        raise NotImplementedError(f"learning = {mode}: only learning = 3 (the feed-forward "
+                                  "rule onto excitatory neurons) is built")
+    kw = {"learning": 3}
+
+    def need(key, sizes=None):
+        v = cfg.floats(key)
+        if v is None or (sizes is not None and len(v) not in sizes):
+            raise ValueError(f"{key} is not given" if v is None else
+                             f"{key} has {len(v)} values, expected one of {sorted(sizes)}")
+        return v
+    for key in ("tauAvg", "targetFR"):
+        v = need(key, {1, 2})
+        kw[key] = (v[0], v[-1])
+    q = [cfg.floats(k) or [] for k in ("tauQ", "A_Q", "gmaxQ", "gminQ")]
+    if len({len(x) for x in q}) > 1:
+        raise ValueError(f"size of tauQ: {len(q[0])} should be the same as A_Q: {len(q[1])} "
+                         "(and gmaxQ, gminQ)")
+    if cfg.int("nLearnTypeFF_E", 1) != 1:
+        raise NotImplementedError("nLearnTypeFF_E != 1 (max_nLearnTypeFF_E = 1)")
+    trip = need("tauTrip")
+    for key in ("tauLTP", "tauLTD", "r_LTD"):
+        if len(need(key)) != len(trip):
+            raise ValueError(f"the size of {key} should be the same as tauTrip: {len(trip)}")
+    kw.update(tauLTP=cfg.floats("tauLTP")[0], tauLTD=cfg.floats("tauLTD")[0],
+              tauTrip=trip[0], r_LTD=cfg.floats("r_LTD")[0], A_LGN=need("A_LGN")[0])
+    for key in ("gmaxLGN", "gminLGN"):
+        kw[key] = need(key, {1})[0]
+    kw["applyHomeo"] = cfg.int("applyHomeo", 0)
+    if kw["applyHomeo"] not in (0, 1, 2):
+        raise ValueError(f"applyHomeo = {kw['applyHomeo']} (0, 1 or 2)")
+    kw["symmetricHomeo"] = 1 if cfg.bool("symmetricHomeo", True) else 0
+    kw["decayTimeHomeoFF"] = cfg.float("decayTimeHomeoFF", 250.0)
+    kw["FF_InfRatio"] = cfg.float("FF_InfRatio", 1.0)
+    kw["rebound"] = cfg.int("rebound", 0)
+    kw["learnData_FF"] = cfg.int("learnData_FF", 1)
+    if kw["learnData_FF"] > 1:
+        raise NotImplementedError("learnData_FF = 2 (the per-step learning variables) is not "
+                                  "built; 0 or 1 (the strengths every sampleInterval_LGN_V1 "
+                                  "steps)")
+    if cfg.bool("store_dsLGN", False):
+        raise NotImplementedError("store_dsLGN is not built")
+    kw["sampleInterval_LGN_V1"] = cfg.int("sampleInterval_LGN_V1", 100)
+    return kw
+
+
 def spec_from_config(cfg: Config, base=None):
     """ModelSpec from the cfg keys of src/patch.cu:180-374 (keys the model
     uses); everything not given keeps the base spec's value."""
@@ -252,8 +312,7 @@ def spec_from_config(cfg: Config, base=None):
     # gap junctions between inhibitory neurons (src/patch.cu:238); their
     # strengths come from V1_gapMat / V1_gapVec, scaled by gapRatio (loader)
     kw["InhGap"] = 1 if cfg.bool("InhGap", False) else 0
-    if cfg.int("learning", 0) != 0:
-        raise NotImplementedError("learning != 0 (plasticity) is not built")
+    kw.update(learning_from_config(cfg, kw.get("dt", sp.dt)))
     if cfg.has("flat_retina") and not cfg.bool("flat_retina"):
         raise NotImplementedError("only flat_retina = true is built")
     if cfg.bool("virtual_LGN", False):

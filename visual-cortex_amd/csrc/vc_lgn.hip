@@ -439,11 +439,19 @@ void launch_rate(hipStream_t st, uint64_t waves, const vc_lgn_params &p, const v
 // synapse bits of the V1 neurons they feed, in groups of kDelGroup (spike,
 // cell) pairs whose target lists are loaded together.
 constexpr int kDelGroup = 4;
+// The LGN cells' LTP trace of the feed-forward plasticity (NULL var = off).
+struct LgnTrace {
+    float tauLTP;
+    float *var;        // [nLGN] trace at the start of the next step
+    float *xStart;     // [nsteps][nLGN] trace at the start of each step
+    float *xSpike;     // [nsteps][nLGN] trace at the step's spike time
+};
+
 __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state st,
                                                    uint32_t it0, uint32_t nsteps,
                                                    const float *__restrict__ frIn,
                                                    float *__restrict__ sInfoOut,
-                                                   vc_ff_events ev) {
+                                                   vc_ff_events ev, LgnTrace lt) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = c < p.nLGN;            // every lane runs the step loop (ballots)
     const uint32_t lane = vc_lane();
@@ -456,6 +464,8 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
         khi = p.trialSeed[2 * tr + 1];
         ent = c - p.trialLgnOff[tr];
     }
+    const bool trace = lt.var != nullptr;
+    float x = (trace && valid) ? lt.var[c] : 0.0f;
     // this cell's target list, read once
     uint32_t myk0 = 0, myk1 = 0;
     if (ev.mask != nullptr && valid) {
@@ -465,11 +475,12 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
     // one step of one cell: rate = fr/1000 and rT = dt*rate come precomputed
     // (the same operations, taken out of the time-sequential chain, which
     // without a spike is one compare and one add per step)
-    auto draw = [&](uint32_t s, float rate, float rT) -> float {
+    // nsp and tsp: the step's spike count and mean spike time
+    auto draw = [&](uint32_t s, float rate, float rT, uint32_t &nsp, float &tsp) -> float {
         const uint32_t it = it0 + s;
         float next_rT = lNL - lTR;
-        uint32_t nsp = 0;
-        float tsp = 0.0f;
+        nsp = 0;
+        tsp = 0.0f;
         float lNLn = lNL;
         if (next_rT > rT) {
             lTR += rT;
@@ -521,9 +532,25 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
             for (uint32_t j = 0; j < (uint32_t)kWin; ++j) {
                 const uint32_t sl = s0 + j;
                 if (valid && sl < wn) {
-                    const float sInfo = draw(w0 + sl, rate[j], rT[j]);
-                    sInfoOut[(size_t)(w0 + sl) * p.nLGN + c] = sInfo;
+                    uint32_t nsp;
+                    float tsp;
+                    const float sInfo = draw(w0 + sl, rate[j], rT[j], nsp, tsp);
+                    const size_t o = (size_t)(w0 + sl) * p.nLGN + c;
+                    sInfoOut[o] = sInfo;
                     if (sInfo >= 1.0f) spiked |= 1ull << sl;
+                    if (trace) {
+                        // LGN_nonlinear's learning part: the spikes are added
+                        // before the decay to their mean time
+                        lt.xStart[o] = x;
+                        float rest = p.dt;
+                        if (sInfo > 0.0f) {
+                            x += (float)nsp;
+                            x *= vc_expf(-tsp / lt.tauLTP);
+                            rest = p.dt - tsp;
+                        }
+                        lt.xSpike[o] = x;
+                        x *= vc_expf(-rest / lt.tauLTP);
+                    }
                 }
             }
 #pragma unroll
@@ -586,6 +613,7 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
     if (valid) {
         st.leftTimeRate[c] = lTR;
         st.lastNegLogRand[c] = lNL;
+        if (trace) lt.var[c] = x;
     }
 }
 
@@ -894,9 +922,10 @@ extern "C" int vc_lgn_frame_responses(void *stream, const float *frames, uint32_
     return VC_OK;
 }
 
-extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
-                            const float *Rring, uint32_t it0, uint32_t nsteps, float *sInfo,
-                            float *fr, const vc_ff_events *ev) {
+namespace {
+int lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s, const float *Rring,
+              uint32_t it0, uint32_t nsteps, float *sInfo, float *fr, const vc_ff_events *ev,
+              LgnTrace lt) {
     if (!p || !s || !Rring || !sInfo) {
         vc_set_error("vc_lgn_steps: null argument");
         return VC_E_INVALID;
@@ -949,9 +978,36 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
         e = *ev;
     }
     hipLaunchKernelGGL(k_lgn_spike, dim3((p->nLGN + 255) / 256), dim3(256), 0,
-                       (hipStream_t)stream, *p, *s, it0, nsteps, fr, sInfo, e);
+                       (hipStream_t)stream, *p, *s, it0, nsteps, fr, sInfo, e, lt);
     VC_HIP_CHECK(hipGetLastError());
     return VC_OK;
+}
+}  // namespace
+
+extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
+                            const float *Rring, uint32_t it0, uint32_t nsteps, float *sInfo,
+                            float *fr, const vc_ff_events *ev) {
+    return lgn_steps(stream, p, s, Rring, it0, nsteps, sInfo, fr, ev, LgnTrace{});
+}
+
+extern "C" int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
+                                  const float *Rring, uint32_t it0, uint32_t nsteps,
+                                  float *sInfo, float *fr, const vc_ff_events *ev, float tauLTP,
+                                  float *xVar, float *xStart, float *xSpike) {
+    if (!xVar || !xStart || !xSpike) {
+        vc_set_error("vc_lgn_steps_learn: null trace buffer");
+        return VC_E_INVALID;
+    }
+    if (!(tauLTP > 0.0f)) {
+        vc_set_error("vc_lgn_steps_learn: tauLTP %g must be positive", (double)tauLTP);
+        return VC_E_INVALID;
+    }
+    if (p && p->cellTrial) {
+        vc_set_error("vc_lgn_steps_learn: plasticity with batched trials is not built");
+        return VC_E_UNSUPPORTED;
+    }
+    return lgn_steps(stream, p, s, Rring, it0, nsteps, sInfo, fr, ev,
+                     LgnTrace{tauLTP, xVar, xStart, xSpike});
 }
 
 extern "C" int vc_lgn_rates(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,

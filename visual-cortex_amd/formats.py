@@ -553,6 +553,47 @@ def spikes_from_sinfo(sInfo: np.ndarray, dt: float):
     return [np.asarray(x) for x in out]
 
 
+class SLGNWriter:
+    """sLGN<suffix>.bin, the strengths of a plastic run (learnData_FF = 1,
+    src/patch.cu:4767-4777, 6196-6200, 6936-6938): header nt u4,
+    sampleInterval_LGN_V1 u4, dt f4, nV1 u4, max_LGNperV1 u4, sRatioLGN[2]
+    f4, nLearnTypeFF_E u4, nLearnTypeFF_I u4, gmaxLGN[nLearnTypeFF] f4 (as
+    given in the cfg), FF_InfRatio f4; then sLGN [max_LGNperV1][nV1] f4 after
+    every sampleInterval steps (steps S, 2S, ... <= nt; the initial strengths
+    are not written)."""
+
+    def __init__(self, path, sp, maxLGN: int):
+        self.f = open(path, "wb")
+        self.f.write(struct.pack("<IIfII", sp.nt, sp.sampleInterval_LGN_V1, sp.dt, sp.nV1, maxLGN))
+        self.f.write(struct.pack("<2f", *sp.sRatioLGN[:2]))
+        self.f.write(struct.pack("<II", 1, 0))               # learning 3: FF_E only
+        self.f.write(struct.pack("<2f", sp.gmaxLGN, sp.FF_InfRatio))
+
+    def write(self, sLGN: np.ndarray):
+        self.f.write(np.ascontiguousarray(sLGN, "<f4").tobytes())
+
+    def close(self):
+        self.f.close()
+
+
+def read_slgn(path) -> dict:
+    """The header and the records [n][max_LGNperV1][nV1] of an sLGN file."""
+    b = Path(path).read_bytes()
+    nt, S = struct.unpack_from("<II", b, 0)
+    dt, = struct.unpack_from("<f", b, 8)
+    nV1, maxLGN = struct.unpack_from("<II", b, 12)
+    sR = struct.unpack_from("<2f", b, 20)
+    nE, nI = struct.unpack_from("<II", b, 28)
+    o = 36
+    gmax = struct.unpack_from(f"<{nE + nI}f", b, o)
+    o += 4 * (nE + nI)
+    inf, = struct.unpack_from("<f", b, o)
+    o += 4
+    rec = np.frombuffer(b, "<f4", offset=o).reshape(-1, maxLGN, nV1)
+    return dict(nt=nt, sampleInterval=S, dt=dt, nV1=nV1, maxLGN=maxLGN, sRatioLGN=sR,
+                nLearnTypeFF_E=nE, nLearnTypeFF_I=nI, gmaxLGN=gmax, FF_InfRatio=inf, sLGN=rec)
+
+
 class StepWriter:
     """LGN_fr (header nt, nLGN) or LGN_sp (no header) per-step float rows."""
 

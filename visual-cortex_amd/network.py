@@ -159,6 +159,30 @@ class ModelSpec:
     spE0: tuple = (0.0, 0.0)
     spI0: tuple = (0.0, 0.0)
     SCsplit: int = 0
+    # plasticity of the LGN->V1 strengths: learning 3 is the reference's FF_E
+    # triplet rule (src/patch.cu:1092-1330, cfg keys of the same names; the
+    # E entries of tauAvg/targetFR and the first of the per-rule lists); 0 =
+    # off.  Homeostasis of each neuron's total FF strength: applyHomeo 1
+    # (multiplicative) or 2 (additive), relaxing over decayTimeHomeoFF towards
+    # FF_InfRatio x the initial total.  learnData_FF 1 writes the strengths
+    # every sampleInterval_LGN_V1 steps (the reference's sLGN file).
+    learning: int = 0
+    tauLTP: float = 16.8
+    tauLTD: float = 33.7
+    tauTrip: float = 114.0
+    r_LTD: float = 1.0
+    A_LGN: float = 0.0
+    tauAvg: tuple = (1000.0, 1000.0)
+    targetFR: tuple = (0.0, 0.0)
+    gmaxLGN: float = 1.0
+    gminLGN: float = 0.0
+    applyHomeo: int = 0
+    symmetricHomeo: int = 1
+    decayTimeHomeoFF: float = 250.0
+    FF_InfRatio: float = 1.0
+    rebound: int = 0
+    learnData_FF: int = 0
+    sampleInterval_LGN_V1: int = 100
     # set when the network comes from the reference's input files
     nLGN_override: int = 0
     stim_buffer_deg: float = 0.0

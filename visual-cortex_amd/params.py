@@ -183,3 +183,47 @@ def _initial_state_spec(sp) -> dict:
     z = lambda *s: np.zeros(s, np.float32)
     return dict(v=v, w=z(sp.nV1), tBack=z(sp.nV1), gFF=z(ngFF, sp.nV1), hFF=z(ngFF, sp.nV1),
                 gE=z(ngE, sp.nV1), hE=z(ngE, sp.nV1), gI=z(ngI, sp.nV1), hI=z(ngI, sp.nV1))
+
+
+def learn_ff_struct(net: Network) -> abi.LearnFFParams:
+    """The FF_E rule's constants (learnFF_pre / learnFF_post, src/condShape.h
+    :94-155, with the E entries, src/patch.cu:1288-1330), in the reference's
+    float32 / float64 operation order."""
+    sp = net.spec
+    f = np.float32
+    ratio = f(sp.sRatioLGN[0])
+    lp = abi.LearnFFParams()
+    lp.tauLTP, lp.tauLTD, lp.tauTrip = f(sp.tauLTP), f(sp.tauLTD), f(sp.tauTrip)
+    lp.tauAvg = f(sp.tauAvg[0])
+    A_LTP = f(f(sp.A_LGN) * ratio)
+    lp.A_LTP = A_LTP
+    A_ratio = f(f(sp.r_LTD) * A_LTP)                     # = A_LTD
+    target = f(sp.targetFR[0])
+    if target > 0:
+        # tauTrip/(tauLTD*(tauAvg_s*tauAvg_s)) in float, then /1000.0/targetFR
+        # in double, the product rounded once to float
+        ts = f(float(f(sp.tauAvg[0])) / 1000.0)
+        x = f(f(sp.tauTrip) / f(f(sp.tauLTD) * f(ts * ts)))
+        A_ratio = f(float(A_ratio) * (float(x) / 1000.0 / float(target)))
+    lp.A_ratio = A_ratio
+    lp.targetFR = target
+    lp.gmax = f(f(sp.gmaxLGN) * ratio)
+    lp.gmin = f(f(sp.gminLGN) * ratio)
+    lp.applyHomeo = int(sp.applyHomeo)
+    lp.symmetricHomeo = int(sp.symmetricHomeo)
+    # exp_homeo = exponential(-dt/hdFF) on the host (src/patch.cu:1330)
+    lp.expHomeo = f(math.exp(float(f(-f(sp.dt) / f(sp.decayTimeHomeoFF)))))
+    lp.rebound = int(sp.rebound)
+    return lp
+
+
+def learn_ff_totals(net: Network) -> tuple[np.ndarray, np.ndarray]:
+    """(totalFF, totalFF_inf) [nV1]: each neuron's total LGN strength summed
+    in float32 in list order, and FF_InfRatio x it (src/patch.cu:4336-4349)."""
+    sp = net.spec
+    s = np.asarray(net.sLGN, np.float32)
+    n = np.asarray(net.nLGN_of, np.int64)
+    tot = np.zeros(sp.nV1, np.float32)
+    for j in range(s.shape[0]):
+        tot = np.where(j < n, tot + s[j], tot).astype(np.float32)
+    return tot, (tot * np.float32(sp.FF_InfRatio)).astype(np.float32)

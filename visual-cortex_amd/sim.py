@@ -133,6 +133,14 @@ class GpuSim:
         elif self.far_only:
             self.speed = float(sp.speedOfThought)
             self._setup_far(1)
+        # ---- plasticity of the LGN->V1 strengths (learning 3): one step at a
+        # time, each vc_v1_step followed by vc_v1_learn_ff
+        self.learn = getattr(sp, "learning", 0) == 3
+        if getattr(sp, "learning", 0) not in (0, 3):
+            raise NotImplementedError(f"learning = {sp.learning}: only learning = 3 is built")
+        if self.learn:
+            self._setup_learn()
+            persistent = False
         # ---- spike buffers (double buffered)
         nT = sp.nV1 // 64
         self.masks = [torch.zeros(nT, dtype=torch.int64, device=d) for _ in range(2)]
@@ -198,6 +206,50 @@ class GpuSim:
         if tune:
             from .tune import tune_step
             self.shape = tune_step(self)
+
+    def _setup_learn(self):
+        """The plastic state: the post traces (zero at the start, as the
+        reference's cudaMemset of learnVar), the homeostatic totals, the LGN
+        cells' trace and the rule's constants."""
+        from .params import learn_ff_struct, learn_ff_totals
+        net, sp, d = self.net, self.net.spec, self.dev
+        if self.batch is not None:
+            raise NotImplementedError("plasticity (learning) with batched trials is not built")
+        z = dict(dtype=torch.float32, device=d)
+        self.lvar = {k: torch.zeros(sp.nV1, **z) for k in ("vLTD", "vTrip", "vAvg")}
+        self.totalFF = self.totalFFinf = None
+        if sp.applyHomeo:
+            tot, inf = learn_ff_totals(net)
+            self.totalFF = torch.as_tensor(tot, device=d)
+            self.totalFFinf = torch.as_tensor(inf, device=d)
+        self.learn_p = learn_ff_struct(net)
+        self.learn_s = abi.LearnFFState(ptr(self.sLGN), ptr(self.lvar["vLTD"]),
+                                        ptr(self.lvar["vTrip"]), ptr(self.lvar["vAvg"]),
+                                        ptr(self.totalFF), ptr(self.totalFFinf))
+        self.xVar = torch.zeros(sp.nLGN, **z)         # the LGN cells' LTP trace
+
+    def _learn_step(self, buf: int, k_in: int, sInfo: torch.Tensor):
+        """The plasticity of the step whose V1 outputs are `sInfo`, LGN row
+        k_in of buffer buf (after that step's update, as the reference learns
+        at the end of compute_V)."""
+        nL = self.net.spec.nLGN
+        o = k_in * nL * 4
+        check(self.lib.vc_v1_learn_ff(self._hs(self.s_v1), C.byref(self.p), C.byref(self.learn_p),
+                                      C.byref(self.ff), C.byref(self.learn_s), ptr(sInfo),
+                                      ptr(self.lgn_sInfo2[buf]) + o, ptr(self.xStart2[buf]) + o,
+                                      ptr(self.xSpike2[buf]) + o), "vc_v1_learn_ff")
+
+    def learn_state_numpy(self) -> dict:
+        """The plastic state on the host: sLGN [maxLGN][nV1], the post traces,
+        the totals (with homeostasis) and the LGN trace at the end of the
+        issued LGN steps."""
+        torch.cuda.synchronize()
+        out = {k: v.cpu().numpy() for k, v in self.lvar.items()}
+        out["sLGN"] = self.sLGN.cpu().numpy()
+        out["xVar"] = self.xVar.cpu().numpy()
+        if self.totalFF is not None:
+            out["totalFF"] = self.totalFF.cpu().numpy()
+        return out
 
     def _initial_spikes(self):
         """rand_spInit (vc_v1_initial_spikes, src/coredynamics.cu:5-146) over
@@ -444,6 +496,9 @@ class GpuSim:
         # double buffers: the LGN stage of frame period k+1 runs on its own
         # stream while the V1 steps of period k run
         self.lgn_sInfo2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
+        if self.learn:                  # the LGN trace at each step's start and spike
+            self.xStart2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
+            self.xSpike2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
         self.lgn_fr2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
         self.lgn_sInfo = self.lgn_sInfo2[0]
         self.lgn_fr = self.lgn_fr2[0]
@@ -521,11 +576,21 @@ class GpuSim:
         if self.ev is not None:
             e2 = self.ev2[buf]
             ev = C.byref(abi.FFEvents(e2.revOff, e2.revTgt, e2.mask + a_in * sp.nV1 * 4, e2.nV1))
-        check(self.lib.vc_lgn_steps(self._hs(st), C.byref(self.lp), C.byref(self.ls),
-                                    ptr(self.Rring), a, b - a,
-                                    ptr(self.lgn_sInfo2[buf]) + a_in * nL * 4,
-                                    ptr(self.lgn_fr2[buf]) + a_in * nL * 4, ev),
-              "vc_lgn_steps")
+        if self.learn:
+            check(self.lib.vc_lgn_steps_learn(self._hs(st), C.byref(self.lp), C.byref(self.ls),
+                                              ptr(self.Rring), a, b - a,
+                                              ptr(self.lgn_sInfo2[buf]) + a_in * nL * 4,
+                                              ptr(self.lgn_fr2[buf]) + a_in * nL * 4, ev,
+                                              self.learn_p.tauLTP, ptr(self.xVar),
+                                              ptr(self.xStart2[buf]) + a_in * nL * 4,
+                                              ptr(self.xSpike2[buf]) + a_in * nL * 4),
+                  "vc_lgn_steps_learn")
+        else:
+            check(self.lib.vc_lgn_steps(self._hs(st), C.byref(self.lp), C.byref(self.ls),
+                                        ptr(self.Rring), a, b - a,
+                                        ptr(self.lgn_sInfo2[buf]) + a_in * nL * 4,
+                                        ptr(self.lgn_fr2[buf]) + a_in * nL * 4, ev),
+                  "vc_lgn_steps")
         # the V1 steps wait for this point only: the slice of the next frame
         # below is read by the next period's LGN steps, on the LGN stream
         e = torch.cuda.Event()
@@ -640,7 +705,29 @@ class GpuSim:
             if events is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e0.record(sv)
-            if self.delay and self.gap_on:
+            if self.learn:
+                # one step at a time, each followed by its plasticity
+                for i in range(n):
+                    row = out[done + i] if out is not None else None
+                    if self.delay:
+                        slot = self._delay_step(buf, k_in + i, it + i, sInfo=row)
+                        srow = self.train[slot]
+                    elif self.far_only:
+                        slot = self._far_only_step(buf, k_in + i, it + i, sInfo=row)
+                        srow = self.train[slot]
+                    else:
+                        srow = row if row is not None else self.sInfo_step
+                        m1 = mptr + i * sp.nV1 * 4 if mptr is not None else None
+                        check(self.lib.vc_v1_run(
+                            self._hs(sv), C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                            C.byref(self.rec), self.spk2, self.cur,
+                            ptr(self.lgn_sInfo2[buf]) + (k_in + i) * sp.nLGN * 4, sp.nLGN, m1,
+                            sp.nV1, it + i, 1, ptr(srow), 0), "vc_v1_run")
+                        self.cur ^= 1
+                    self._learn_step(buf, k_in + i, srow)
+                self.sInfo_last = srow
+                self.n_v1_launches += n
+            elif self.delay and self.gap_on:
                 # one step at a time: each step's gap sums follow it
                 for i in range(n):
                     row = out[done + i] if out is not None else None
@@ -664,7 +751,7 @@ class GpuSim:
                     row = out[done + i] if out is not None else None
                     slot = self._far_only_step(buf, k_in + i, it + i, sInfo=row)
                 self.sInfo_last = out[done + n - 1] if out is not None else self.train[slot]
-            if self.delay or self.far_only:
+            if self.delay or self.far_only or self.learn:
                 if events is not None:
                     e1 = torch.cuda.Event(enable_timing=True)
                     e1.record(sv)
@@ -775,6 +862,8 @@ class GpuSim:
                 if self.gap_on:
                     self._gap_after(sInfo, 0)
                 self.cur = nxt
+            if self.learn:
+                self._learn_step(buf, k_in, sInfo)
             self.it += 1
             self.sInfo_last = sInfo
             if raw:

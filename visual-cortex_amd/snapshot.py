@@ -189,6 +189,8 @@ def save(sim, path) -> dict:
     npf, maxF = tm["ntPerFrame"], tm["maxFrame"]
     if getattr(net, "batch", None) is not None:
         raise ValueError("snapshots of batched trials are not supported")
+    if getattr(sim, "learn", False):
+        raise NotImplementedError("snapshots of plastic runs (learning != 0) are not built")
     if sim.it % npf != 0:
         raise ValueError(f"snapshot at step {sim.it}: snapshots are taken at frame-period "
                          f"boundaries ({npf} steps)")
@@ -234,6 +236,8 @@ def restore(sim, path, as_init: bool = False) -> dict:
     as_init (the reference's asInit): take the state as the initial
     condition and start at step 0; otherwise resume at the snapshot's step."""
     net, sp = sim.net, sim.net.spec
+    if getattr(sim, "learn", False):
+        raise NotImplementedError("snapshots of plastic runs (learning != 0) are not built")
     d = read_file(path)
     h = d["header"]
     ngFF, ngE, ngI = int(sim.p.ngFF), int(sim.p.ngE), int(sim.p.ngI)
