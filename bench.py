@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--far", type=int, default=0,
                     help="this many far (long-range) inputs per neuron (delayed by distance: "
                          "with --delay, or alone with undelayed near connections)")
+    ap.add_argument("--learn", type=float, default=0.0,
+                    help="c1/c2 with the feed-forward plasticity (learning 3) at this A_LGN")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (x4) for HBM counter calibration")
     args = ap.parse_args()
@@ -135,7 +137,7 @@ def main():
         print(json.dumps(bench_v1_adjoint(args)), flush=True)
         return
     line = bench_plain(args)
-    if args.config == "c2" and not args.no_extras and not args.delay and not args.far:
+    if args.config == "c2" and not args.no_extras and not args.delay and not args.far and not args.learn:
         import torch
         torch.cuda.empty_cache()
         line["batched_trials"] = bench_batched(args)
@@ -158,6 +160,11 @@ def bench_plain(args) -> dict:
         spec = network.replace(spec, noDelay=0, speedOfThought=args.delay, nFar=args.far)
     elif args.far:
         spec = network.replace(spec, noDelay=1, noFarDelay=0, speedOfThought=1.0, nFar=args.far)
+    if args.learn > 0:
+        # the reference's evoked0.cfg time constants, a 5 Hz target rate
+        spec = network.replace(spec, learning=3, A_LGN=args.learn, tauLTP=16.8, tauLTD=33.7,
+                               tauTrip=114.0, r_LTD=1.0, tauAvg=(1000.0, 1000.0),
+                               targetFR=(5.0, 5.0), gmaxLGN=100.0, gminLGN=0.0)
     K, W = args.steps, args.warmup
     # the configuration's whole stimulus (2 s / 1 s of frames), whatever --steps is
     spec = network.replace(spec, nt=max(spec.nt, PREROLL_STEPS + W + K))
@@ -244,6 +251,8 @@ def bench_plain(args) -> dict:
         kernel = "k_v1_step+k_rec_delay" + ("+k_far_mark+k_far_eval" if sim.far_on else "")
     elif sim.far_only:
         kernel = "k_v1_step+k_v1_step(apply)+k_far_mark+k_far_eval"
+    if sim.learn:
+        kernel += "+k_v1_learn_ff"
     workload = (f"plain forward, {'full patch' if args.config == 'c2' else 'small'}: "
                 f"{spec.nblock} blocks x {spec.blockSize} neurons, {spec.nLGN} LGN, m={spec.m}, "
                 f"K={spec.KE}+{spec.KI}, {spec.width}x{spec.height} px, "
@@ -253,6 +262,8 @@ def bench_plain(args) -> dict:
         workload += f", axonal delays at {spec.speedOfThought} mm/ms (ring of {sim.depth} steps)"
     if sim.far_on:
         workload += f", {spec.nFar} far inputs per neuron"
+    if sim.learn:
+        workload += f", feed-forward plasticity (learning 3, A_LGN {spec.A_LGN})"
     traffic = None
     if args.config == "c2" and not sim.delay and not sim.far_on and sim.persistent:
         traffic = pmc_traffic("k_v1_multi_full_patch", steps_per_launch)

@@ -733,11 +733,14 @@ typedef struct {
  * (x_spike = (x + nsp) exp(-tsp/tauLTP)), and the step ends at x_spike
  * exp(-(dt - tsp)/tauLTP); without a spike xSpike = xStart and the step ends
  * at x exp(-dt/tauLTP).  tsp is the draw's own spike time, not the one
- * recovered from sInfo.  Not for batched trials (p->cellTrial must be NULL). */
+ * recovered from sInfo.  spikeBits[s*nWords + c/64] bit c%64 (nWords =
+ * (nLGN + 63)/64) is set when cell c spiked in step it0+s, the bitmap
+ * vc_v1_learn_ff tests before it reads a cell's row.  Not for batched trials
+ * (p->cellTrial must be NULL). */
 int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
                        const float *Rring, uint32_t it0, uint32_t nsteps, float *sInfo,
                        float *fr, const vc_ff_events *ev, float tauLTP, float *xVar,
-                       float *xStart, float *xSpike);
+                       float *xStart, float *xSpike, uint64_t *spikeBits);
 
 /* One step of the feed-forward plasticity, after that step's vc_v1_step (the
  * reference learns at the end of the same kernel, after the strengths were
@@ -752,11 +755,13 @@ int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc_lgn_state 
  * - tspF when tspF < tsp, else xStart[c], dtp = tsp); f is clipped to
  * [gmin, gmax].  Then the post traces take the step's spikes and decay to the
  * step end, in the reference's operation order.  sInfo is the step's V1
- * output row, lgnSInfo/xStart/xSpike the step's rows of vc_lgn_steps_learn.
- * Not for batched trials (p->blockTrial must be NULL). */
+ * output row, lgnSInfo/xStart/xSpike/lgnSpikeBits the step's rows of
+ * vc_lgn_steps_learn over nLGN cells.  Not for batched trials (p->blockTrial
+ * must be NULL). */
 int vc_v1_learn_ff(void *stream, const vc_v1_params *p, const vc_learn_ff_params *lp,
                    const vc_ff_conn *ff, const vc_learn_ff_state *ls, const float *sInfo,
-                   const float *lgnSInfo, const float *xStart, const float *xSpike);
+                   const float *lgnSInfo, const float *xStart, const float *xSpike,
+                   const uint64_t *lgnSpikeBits, uint32_t nLGN);
 
 /* The rate part of vc_lgn_steps alone (no spike draw), for any run of steps
  * and any maxFrame: fr[nsteps][nLGN] (Hz).  R is read at slot f mod maxFrame

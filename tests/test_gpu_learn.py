@@ -79,10 +79,15 @@ def test_learn_kernel_random_inputs_bitwise(pkg, homeo, rebound):
     ff = abi.FFConn(ptr(gnl), ptr(gs), ptr(gil), sL.shape[0])
     ls = abi.LearnFFState(ptr(gs), ptr(g["vLTD"]), ptr(g["vTrip"]), ptr(g["vAvg"]),
                           ptr(gt) if homeo else None, ptr(gi) if homeo else None)
-    rows = [T(x) for x in (sInfo, lgn, xs, xk)]
+    # the step's LGN spike bitmap (bit c%64 of word c/64: lgn[c] >= 1)
+    nW = (nL + 63) // 64
+    spk = np.zeros(nW * 64, bool)
+    spk[:nL] = lgn >= 1.0
+    bits = np.packbits(spk.reshape(nW, 64), axis=-1, bitorder="little").reshape(-1).view(np.int64)
+    rows = [T(x) for x in (sInfo, lgn, xs, xk, bits)]
     lib = abi.load_library()
     abi.check(lib.vc_v1_learn_ff(None, C.byref(p), C.byref(lp), C.byref(ff), C.byref(ls),
-                                 *(ptr(r) for r in rows)), "vc_v1_learn_ff")
+                                 *(ptr(r) for r in rows), nL), "vc_v1_learn_ff")
     torch.cuda.synchronize()
     assert (hs != sL).sum() > 100
     assert np.array_equal(_bits(gs.cpu().numpy()), _bits(hs))
@@ -149,6 +154,15 @@ def test_lgn_trace_bitwise(pkg):
     buf = 1                                   # period 1's rows
     assert np.array_equal(_bits(g.xStart2[buf].cpu().numpy()), _bits(o.xStart))
     assert np.array_equal(_bits(g.xSpike2[buf].cpu().numpy()), _bits(o.xSpike))
+    # the spike bitmaps of the period's steps
+    spk = o.lgn_hist[-1] >= 1.0
+    nW = g.nWords
+    want = np.zeros((npf, nW * 64), bool)
+    want[:, :spk.shape[1]] = spk
+    want = np.packbits(want.reshape(npf, nW, 64), axis=-1, bitorder="little")   # [npf][nW][8]
+    assert spk.sum() > 20
+    got = g.lbits2[buf].cpu().numpy().view(np.uint8).reshape(npf, nW, 8)
+    assert np.array_equal(got, want)
     # the GPU's carried trace is that of the LGN steps it issued ahead
     o.run(npf)
     g._lgn_issue_to(3 * npf)
@@ -174,11 +188,40 @@ def test_plastic_run_with_outputs_equals_run(pkg, tmp_path):
         assert np.array_equal(_bits(la[k]), _bits(lb[k])), k
 
 
-def test_plastic_snapshot_refused(pkg, tmp_path):
-    from visual_cortex_amd import snapshot
+@pytest.mark.parametrize("homeo", [0, 1])
+def test_plastic_snapshot_resume(pkg, tmp_path, homeo):
+    """A plastic run's snapshot (the strengths, the post variables and the
+    LGN trace after the reference's LGN section) resumes bit for bit without
+    homeostasis; with it the totals restart from the network's initial
+    strengths, as the reference's (it sums them before reading the file)."""
+    from visual_cortex_amd import params, snapshot
     from visual_cortex_amd.sim import GpuSim
-    _flush("test_plastic_snapshot_refused")
-    g = GpuSim(_net(nt=600))
-    g.run(g.tm["ntPerFrame"])
-    with pytest.raises(NotImplementedError, match="plastic"):
-        snapshot.save(g, tmp_path / "snap.bin")
+    _flush(f"test_plastic_snapshot_resume[homeo {homeo}]")
+    net = _net(nt=900, applyHomeo=homeo, FF_InfRatio=0.8, A_LGN=0.1)
+    sp = net.spec
+    a = GpuSim(net)
+    npf = a.tm["ntPerFrame"]
+    K = max(1, 200 // npf)
+    a.run(K * npf)
+    path = tmp_path / "snapShot_learn.bin"
+    h = snapshot.save(a, path)
+    assert h["learning"] == 3
+    assert path.stat().st_size == snapshot.file_size(
+        sp.nV1, sp.nLGN, sp.iModel, int(a.p.ngFF), int(a.p.ngE), int(a.p.ngI),
+        sp.nblock * sp.nI, learn=(sp.nblock, sp.nE, net.sLGN.shape[0]))
+    n2 = npf // 2 + 37
+    a.run(n2)
+    ref = a.sInfo()[K * npf:]
+    la = a.learn_state_numpy()
+    b = GpuSim(net)
+    snapshot.restore(b, path)
+    if homeo:
+        tot, _ = params.learn_ff_totals(net)
+        assert np.array_equal(_bits(b.totalFF.cpu().numpy()), _bits(tot))
+        return
+    b.run(n2)
+    lb = b.learn_state_numpy()
+    assert (ref >= 1).sum() > 0
+    assert np.array_equal(_bits(b.sInfo()), _bits(ref))
+    for k in ("sLGN", "vLTD", "vTrip", "vAvg", "xVar"):
+        assert np.array_equal(_bits(la[k]), _bits(lb[k])), k

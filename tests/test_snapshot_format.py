@@ -85,3 +85,48 @@ def test_truncated_file_is_rejected(pkg, tmp_path):
     q.write_bytes(raw[:len(raw) - 4 * nV1 * 3])
     with pytest.raises(ValueError):
         S.read_file(q)
+
+
+def test_plastic_sections_layout(pkg, tmp_path):
+    """A plastic run's sections sit between the LGN generators and the V1
+    ones (src/patch.cu:6845-6851): sLGN [maxLGN][nV1], the post variables in
+    the reference's learnVarFF layout (vLTD_FF_E, vTrip_FF_E over
+    [nblock][nE], vAvgE [nblock][nE][2], vAvgI [nblock][nI]; 2 nE nblock +
+    nV1 + nE nblock floats), the LGN trace [nLGN]."""
+    from visual_cortex_amd import snapshot as S
+    rng = np.random.default_rng(9)
+    nblock, bs, nE, maxLGN, nLGN, ngFF, ngE, ngI = 3, 128, 96, 5, 40, 1, 1, 1
+    nV1, mI = nblock * bs, nblock * (bs - nE)
+    f = lambda *s: rng.standard_normal(s).astype(np.float32)
+    h = dict(it0=64, frameCycle=0, iFrameHead=1, oldFrameHead=0, currentFrame=1, iModel=0,
+             learning=3, nV1=nV1, nLGN=nLGN, ngTypeFF=ngFF, ngTypeE=ngE, ngTypeI=ngI, nType=2,
+             trainDepth=1)
+    isE = (np.arange(nV1) % bs) < nE
+    lv = {k: np.where(isE, f(nV1), 0).astype(np.float32) for k in ("vLTD", "vTrip", "vAvg")}
+    post = S.post_to_file(lv, nblock, bs, nE)
+    assert post.size == 2 * nE * nblock + nV1 + nE * nblock
+    k = nE * nblock
+    assert np.array_equal(post[:nE], lv["vLTD"][:nE])              # block 0, eid = local id
+    assert np.array_equal(post[k + nE:k + 2 * nE], lv["vTrip"][bs:bs + nE])   # block 1
+    assert np.array_equal(post[2 * k:2 * k + 4], [lv["vAvg"][0], 0, lv["vAvg"][1], 0])
+    learn = dict(sLGN=f(maxLGN, nV1), post=post, pre=f(nLGN))
+    block = dict(depC=f(nV1), v=f(nV1), gFF=f(ngFF, nV1), hFF=f(ngFF, nV1), gE=f(ngE, nV1),
+                 gI=f(ngI, nV1), hE=f(ngE, nV1), hI=f(ngI, nV1), gap=f(mI))
+    p = tmp_path / "snap_learn.bin"
+    S.write_file(p, h, dict(leftTimeRate=f(nLGN), lastNegLogRand=f(nLGN)), dict(tBack=f(nV1)),
+                 f(1, nV1), block, learn=learn)
+    raw = p.read_bytes()
+    assert len(raw) == S.file_size(nV1, nLGN, 0, ngFF, ngE, ngI, mI,
+                                   learn=(nblock, nE, maxLGN))
+    assert struct.unpack_from("<2i", raw, 44) == (0, 3)
+    o = 80 + nLGN * (8 + S.GEN_BYTES)
+    assert np.array_equal(np.frombuffer(raw, np.float32, maxLGN * nV1, o),
+                          learn["sLGN"].reshape(-1))
+    with pytest.raises(ValueError, match="plastic"):
+        S.read_file(p)
+    d = S.read_file(p, learn=(nblock, nE, maxLGN))
+    assert np.array_equal(d["sLGN"], learn["sLGN"]) and np.array_equal(d["pre"], learn["pre"])
+    back = S.post_from_file(d["post"], nblock, bs, nE)
+    for name in lv:
+        assert np.array_equal(back[name], lv[name]), name
+    assert np.array_equal(d["gap"], block["gap"])

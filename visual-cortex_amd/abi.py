@@ -204,9 +204,9 @@ SIGNATURES = {
     "vc_lgn_steps": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp, vp,
                      C.POINTER(FFEvents)],
     "vc_lgn_steps_learn": [vp, C.POINTER(LgnParams), C.POINTER(LgnState), vp, u32, u32, vp,
-                           vp, C.POINTER(FFEvents), C.c_float, vp, vp, vp],
+                           vp, C.POINTER(FFEvents), C.c_float, vp, vp, vp, vp],
     "vc_v1_learn_ff": [vp, C.POINTER(V1Params), C.POINTER(LearnFFParams), C.POINTER(FFConn),
-                       C.POINTER(LearnFFState), vp, vp, vp, vp],
+                       C.POINTER(LearnFFState), vp, vp, vp, vp, vp, u32],
     "vc_lgn_init": [vp, C.POINTER(LgnParams), vp, vp],
     "vc_lgn_frame_responses": [vp, vp, u32, u32, u32, u32, u32, vp, vp, vp, vp, vp],
     "vc_lgn_sample_geometry": [vp, C.POINTER(LgnSetup), vp],

@@ -72,8 +72,6 @@ def main(argv=None) -> int:
     # the strengths of a plastic run every sampleInterval_LGN_V1 steps
     # (learnData_FF = 1, src/patch.cu:6196-6200, 6936-6938)
     slgn_w = None
-    if sim.learn:
-        print("snapshots of plastic runs are not built: none will be written", flush=True)
     if sim.learn and sp.learnData_FF == 1 and not cfg.bool("minimal", False):
         from visual_cortex_amd import formats
         slgn_w = formats.SLGNWriter(cfg.path("f_sLGN"), sp, int(net.sLGN.shape[0]))
@@ -81,8 +79,7 @@ def main(argv=None) -> int:
     def after_step(g, i):
         if slgn_w is not None and (i + 1) % max(sp.sampleInterval_LGN_V1, 1) == 0:
             slgn_w.write(g.sLGN.cpu().numpy())
-        if not g.learn:
-            take_snapshot(g, i)
+        take_snapshot(g, i)
 
     t0 = time.time()
     sim.run_with_outputs(sp.nt, paths, outputs, on_step=after_step)

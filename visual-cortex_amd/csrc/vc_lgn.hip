@@ -445,6 +445,7 @@ struct LgnTrace {
     float *var;        // [nLGN] trace at the start of the next step
     float *xStart;     // [nsteps][nLGN] trace at the start of each step
     float *xSpike;     // [nsteps][nLGN] trace at the step's spike time
+    uint64_t *bits;    // [nsteps][(nLGN + 63)/64] bit c%64 of word c/64: cell c spiked
 };
 
 __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state st,
@@ -555,6 +556,17 @@ __global__ __launch_bounds__(256) void k_lgn_spike(vc_lgn_params p, vc_lgn_state
             }
 #pragma unroll
             for (uint32_t j = 0; j < (uint32_t)kWin; ++j) cur[j] = nxt[j];
+        }
+        if (trace) {
+            // the window's spike bitmaps: lane s stores the wavefront's word of
+            // step w0 + s (a wavefront holds 64 consecutive cells)
+            uint64_t mine = 0;
+            for (uint32_t s = 0; s < wn; ++s) {
+                const uint64_t b = __ballot((spiked >> s) & 1ull);
+                if (lane == s) mine = b;
+            }
+            const uint32_t nWords = (p.nLGN + 63u) / 64u;
+            if (lane < wn && (c >> 6) < nWords) lt.bits[(size_t)(w0 + lane) * nWords + (c >> 6)] = mine;
         }
         if (ev.mask == nullptr) continue;
         // ---- deliveries, in (step, cell) order: the wavefront walks the
@@ -993,8 +1005,9 @@ extern "C" int vc_lgn_steps(void *stream, const vc_lgn_params *p, const vc_lgn_s
 extern "C" int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
                                   const float *Rring, uint32_t it0, uint32_t nsteps,
                                   float *sInfo, float *fr, const vc_ff_events *ev, float tauLTP,
-                                  float *xVar, float *xStart, float *xSpike) {
-    if (!xVar || !xStart || !xSpike) {
+                                  float *xVar, float *xStart, float *xSpike,
+                                  uint64_t *spikeBits) {
+    if (!xVar || !xStart || !xSpike || !spikeBits) {
         vc_set_error("vc_lgn_steps_learn: null trace buffer");
         return VC_E_INVALID;
     }
@@ -1007,7 +1020,7 @@ extern "C" int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc
         return VC_E_UNSUPPORTED;
     }
     return lgn_steps(stream, p, s, Rring, it0, nsteps, sInfo, fr, ev,
-                     LgnTrace{tauLTP, xVar, xStart, xSpike});
+                     LgnTrace{tauLTP, xVar, xStart, xSpike, spikeBits});
 }
 
 extern "C" int vc_lgn_rates(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,

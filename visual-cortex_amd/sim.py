@@ -237,7 +237,9 @@ class GpuSim:
         check(self.lib.vc_v1_learn_ff(self._hs(self.s_v1), C.byref(self.p), C.byref(self.learn_p),
                                       C.byref(self.ff), C.byref(self.learn_s), ptr(sInfo),
                                       ptr(self.lgn_sInfo2[buf]) + o, ptr(self.xStart2[buf]) + o,
-                                      ptr(self.xSpike2[buf]) + o), "vc_v1_learn_ff")
+                                      ptr(self.xSpike2[buf]) + o,
+                                      ptr(self.lbits2[buf]) + k_in * self.nWords * 8, nL),
+              "vc_v1_learn_ff")
 
     def learn_state_numpy(self) -> dict:
         """The plastic state on the host: sLGN [maxLGN][nV1], the post traces,
@@ -499,6 +501,9 @@ class GpuSim:
         if self.learn:                  # the LGN trace at each step's start and spike
             self.xStart2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
             self.xSpike2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
+            self.nWords = (sp.nLGN + 63) // 64       # the LGN spike bitmap of each step
+            self.lbits2 = [torch.zeros(tm["ntPerFrame"], self.nWords, dtype=torch.int64,
+                                       device=self.dev) for _ in range(2)]
         self.lgn_fr2 = [torch.zeros(tm["ntPerFrame"], sp.nLGN, **f32) for _ in range(2)]
         self.lgn_sInfo = self.lgn_sInfo2[0]
         self.lgn_fr = self.lgn_fr2[0]
@@ -583,7 +588,8 @@ class GpuSim:
                                               ptr(self.lgn_fr2[buf]) + a_in * nL * 4, ev,
                                               self.learn_p.tauLTP, ptr(self.xVar),
                                               ptr(self.xStart2[buf]) + a_in * nL * 4,
-                                              ptr(self.xSpike2[buf]) + a_in * nL * 4),
+                                              ptr(self.xSpike2[buf]) + a_in * nL * 4,
+                                              ptr(self.lbits2[buf]) + a_in * self.nWords * 8),
                   "vc_lgn_steps_learn")
         else:
             check(self.lib.vc_lgn_steps(self._hs(st), C.byref(self.lp), C.byref(self.ls),

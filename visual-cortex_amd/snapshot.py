@@ -9,6 +9,10 @@ File layout, in order (little endian; u32 unless noted):
     iModel (i32), learning (i32)
     nV1, nLGN, ngTypeFF, ngTypeE, ngTypeI, nType, trainDepth
     LGN  leftTimeRate f32[nLGN], lastNegLogRand f32[nLGN], generator[nLGN]
+    plastic runs (learning != 0; src/patch.cu:6845-6851, 5556-5563):
+         sLGN f32[maxLGN][nV1]; the post variables f32 vLTD_FF_E[nblock][nE],
+         vTrip_FF_E[nblock][nE], vAvgE[nblock][nE][2], vAvgI[nblock][nI]
+         (learning 3: no FF_I rule); the LGN trace f32[nLGN]
     V1   rGenCond generator[nV1], rNoisy generator[nV1], tBack f32[nV1]
     spikeTrain f32[trainDepth][nV1]
     state block f32: depC[nV1], (w[nV1] if iModel == 1), v[nV1],
@@ -86,17 +90,51 @@ def state_block_floats(nV1: int, iModel: int, ngFF: int, ngE: int, ngI: int, mI:
     return nV1 * (2 + (1 if iModel == 1 else 0) + 2 * ngFF + 2 * (ngE + ngI)) + mI
 
 
-def file_size(nV1, nLGN, iModel, ngFF, ngE, ngI, mI, trainDepth=1) -> int:
-    """Bytes of a snapshot (the reference's section sizes, src/patch.cu:5479, 3902-3907)."""
+def learn_floats(nV1: int, nblock: int, nE: int, maxLGN: int, nLGN: int) -> tuple:
+    """Float counts of the plastic sections (learning 3): sLGN, the post
+    variables (learnVarFFsize, src/patch.cu:4366-4368) and the LGN trace
+    (LGN_learnPreSize, :2620)."""
+    return maxLGN * nV1, 2 * nE * nblock + nV1 + nE * nblock, nLGN
+
+
+def file_size(nV1, nLGN, iModel, ngFF, ngE, ngI, mI, trainDepth=1, learn=None) -> int:
+    """Bytes of a snapshot (the reference's section sizes, src/patch.cu:5479, 3902-3907);
+    learn = (nblock, nE, maxLGN) for a plastic run."""
+    extra = 4 * sum(learn_floats(nV1, learn[0], learn[1], learn[2], nLGN)) if learn else 0
     return (4 * HEADER_U32 + 4 * 2 + 4 * 7 + nLGN * (8 + GEN_BYTES) + nV1 * (2 * GEN_BYTES + 4)
-            + 4 * trainDepth * nV1 + 4 * state_block_floats(nV1, iModel, ngFF, ngE, ngI, mI))
+            + 4 * trainDepth * nV1 + 4 * state_block_floats(nV1, iModel, ngFF, ngE, ngI, mI)
+            + extra)
+
+
+def post_to_file(lvar: dict, nblock: int, bs: int, nE: int) -> np.ndarray:
+    """The post variables [nV1] (excitatory entries) in the reference's
+    learnVarFF layout (vAvgE's second slot and vAvgI are 0 with learning 3)."""
+    nV1 = nblock * bs
+    eid = np.arange(nV1).reshape(nblock, bs)[:, :nE].reshape(-1)
+    avg = np.zeros((nblock * nE, 2), np.float32)
+    avg[:, 0] = lvar["vAvg"][eid]
+    return np.concatenate([lvar["vLTD"][eid], lvar["vTrip"][eid], avg.reshape(-1),
+                           np.zeros(nblock * (bs - nE), np.float32)]).astype(np.float32)
+
+
+def post_from_file(post: np.ndarray, nblock: int, bs: int, nE: int) -> dict:
+    """post_to_file's inverse: [nV1] arrays, inhibitory entries 0."""
+    nV1, k = nblock * bs, nblock * nE
+    eid = np.arange(nV1).reshape(nblock, bs)[:, :nE].reshape(-1)
+    out = {}
+    for name, a in (("vLTD", post[:k]), ("vTrip", post[k:2 * k]), ("vAvg", post[2 * k:4 * k:2])):
+        v = np.zeros(nV1, np.float32)
+        v[eid] = a
+        out[name] = v
+    return out
 
 
 def write_file(path, h: dict, lgn: dict, v1: dict, spike_train: np.ndarray, block: dict,
-               keys=(0, 0)):
+               keys=(0, 0), learn: dict | None = None):
     """Write a snapshot from host arrays.  h: the header fields; lgn:
     leftTimeRate, lastNegLogRand; v1: tBack, optionally lastNoise; block:
-    depC, w (iModel 1), v, gFF, hFF, gE, gI, hE, hI, gap."""
+    depC, w (iModel 1), v, gFF, hFF, gE, gI, hE, hI, gap; learn (h learning
+    != 0): sLGN, post (the file layout), pre (the LGN trace)."""
     f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).tobytes()
     nV1, nLGN = h["nV1"], h["nLGN"]
     with open(path, "wb") as f:
@@ -109,6 +147,9 @@ def write_file(path, h: dict, lgn: dict, v1: dict, spike_train: np.ndarray, bloc
         f.write(f32(lgn["leftTimeRate"]))
         f.write(f32(lgn["lastNegLogRand"]))
         f.write(generator_slots(nLGN, keys[0], keys[1], STREAM_LGN_SPIKE))
+        if h.get("learning", 0):
+            for k in ("sLGN", "post", "pre"):
+                f.write(f32(learn[k]))
         f.write(generator_slots(nV1, keys[0], keys[1], STREAM_V1_BG))
         f.write(generator_slots(nV1, keys[0], keys[1], STREAM_V1_NOISE,
                                 carried=v1.get("lastNoise", np.zeros(nV1, np.float32))))
@@ -121,9 +162,11 @@ def write_file(path, h: dict, lgn: dict, v1: dict, spike_train: np.ndarray, bloc
             f.write(f32(block[k]))
 
 
-def read_file(path) -> dict:
+def read_file(path, learn=None) -> dict:
     """Read a snapshot written by write_file (or by the reference, whose
-    generator slots are then MRG32k3a states, returned as raw bytes)."""
+    generator slots are then MRG32k3a states, returned as raw bytes).  A
+    plastic run's file needs learn = (nblock, nE, maxLGN) for the section
+    sizes (the header does not hold them)."""
     with open(path, "rb") as f:
         data = f.read()
     o = 0
@@ -151,8 +194,8 @@ def read_file(path) -> dict:
              "iStatus", "currentTimeSlot")
     h = dict(zip(names, take("<11I")))
     h["iModel"], h["learning"] = take("<2i")
-    if h["learning"]:
-        raise ValueError("snapshots of plastic runs (learning != 0) are not supported")
+    if h["learning"] and learn is None:
+        raise ValueError("a snapshot of a plastic run: pass learn = (nblock, nE, maxLGN)")
     h.update(zip(("nV1", "nLGN", "ngTypeFF", "ngTypeE", "ngTypeI", "nType", "trainDepth"),
                  take("<7I")))
     nV1, nLGN = h["nV1"], h["nLGN"]
@@ -160,6 +203,11 @@ def read_file(path) -> dict:
     out["leftTimeRate"] = floats(nLGN)
     out["lastNegLogRand"] = floats(nLGN)
     out["lgn_generators"] = raw(nLGN * GEN_BYTES)
+    if h["learning"]:
+        nS, nP, nQ = learn_floats(nV1, learn[0], learn[1], learn[2], nLGN)
+        out["sLGN"] = floats(nS).reshape(learn[2], nV1)
+        out["post"] = floats(nP)
+        out["pre"] = floats(nQ)
     out["rGenCond"] = raw(nV1 * GEN_BYTES)
     out["rNoisy"] = raw(nV1 * GEN_BYTES)
     out["lastNoise"] = carried_values(out["rNoisy"], nV1)
@@ -189,19 +237,20 @@ def save(sim, path) -> dict:
     npf, maxF = tm["ntPerFrame"], tm["maxFrame"]
     if getattr(net, "batch", None) is not None:
         raise ValueError("snapshots of batched trials are not supported")
-    if getattr(sim, "learn", False):
-        raise NotImplementedError("snapshots of plastic runs (learning != 0) are not built")
     if sim.it % npf != 0:
         raise ValueError(f"snapshot at step {sim.it}: snapshots are taken at frame-period "
                          f"boundaries ({npf} steps)")
     K = sim.it // npf
     sim.finalize()                          # pending increments into the state
     torch.cuda.synchronize()
+    learn = getattr(sim, "learn", False)
     if sim.lgn_to <= sim.it:
         # nothing of period K issued yet: the live LGN state is its start
         lgn = torch.stack([sim.lTR, sim.lNL])
+        xpre = sim.xVar if learn else None
     else:
         lgn = sim.lgn_at[K % 2]             # the LGN state at the start of period K
+        xpre = sim.xStart2[K % 2][0] if learn else None   # the trace at its first step
     dep = torch.empty(sp.nV1, dtype=torch.float32, device=sim.dev)
     check(sim.lib.vc_v1_dep_current(None, C.byref(sim.p), C.byref(sim.s), ptr(dep)),
           "vc_v1_dep_current")
@@ -210,7 +259,8 @@ def save(sim, path) -> dict:
     ngFF, ngE, ngI = int(sim.p.ngFF), int(sim.p.ngE), int(sim.p.ngI)
     nFr = K * tm["denorm"]                  # frames entered (denorm per period)
     h = dict(it0=sim.it, frameCycle=nFr // maxF, iFrameHead=nFr % maxF,
-             oldFrameHead=(nFr - 1) % maxF, currentFrame=nFr, iModel=sp.iModel, learning=0,
+             oldFrameHead=(nFr - 1) % maxF, currentFrame=nFr, iModel=sp.iModel,
+             learning=int(sp.learning) if learn else 0,
              nV1=sp.nV1, nLGN=sp.nLGN, ngTypeFF=ngFF, ngTypeE=ngE, ngTypeI=ngI,
              nType=int(sim.p.nType), trainDepth=1)
     if getattr(sim, "delay", False) or getattr(sim, "far_only", False):
@@ -225,9 +275,15 @@ def save(sim, path) -> dict:
                  hE=st["hE"][:ngE], hI=st["hI"][:ngI],
                  gap=sim.gap.cpu().numpy() if getattr(sim, "gap", None) is not None
                  else np.zeros(sp.nblock * sp.nI, np.float32))
+    lrn = None
+    if learn:
+        lv = {k: v.cpu().numpy() for k, v in sim.lvar.items()}
+        lrn = dict(sLGN=sim.sLGN.cpu().numpy(), post=post_to_file(lv, sp.nblock, sp.blockSize,
+                                                                   sp.nE),
+                   pre=xpre.cpu().numpy())
     write_file(path, h, dict(leftTimeRate=lg[0], lastNegLogRand=lg[1]),
                dict(tBack=st["tBack"], lastNoise=sim.lastNoise.cpu().numpy()), train, block,
-               keys=(int(sim.p.seed_lo), int(sim.p.seed_hi)))
+               keys=(int(sim.p.seed_lo), int(sim.p.seed_hi)), learn=lrn)
     return h
 
 
@@ -236,10 +292,12 @@ def restore(sim, path, as_init: bool = False) -> dict:
     as_init (the reference's asInit): take the state as the initial
     condition and start at step 0; otherwise resume at the snapshot's step."""
     net, sp = sim.net, sim.net.spec
-    if getattr(sim, "learn", False):
-        raise NotImplementedError("snapshots of plastic runs (learning != 0) are not built")
-    d = read_file(path)
+    learn = getattr(sim, "learn", False)
+    d = read_file(path, learn=(sp.nblock, sp.nE, int(sim.sLGN.shape[0])) if learn else None)
     h = d["header"]
+    if bool(h["learning"]) != learn:
+        raise ValueError(f"snapshot {path} has learning = {h['learning']}, this run "
+                         f"{sp.learning}")
     ngFF, ngE, ngI = int(sim.p.ngFF), int(sim.p.ngE), int(sim.p.ngI)
     mine = (sp.iModel, sp.nV1, sp.nLGN, ngFF, ngE, ngI)
     theirs = (h["iModel"], h["nV1"], h["nLGN"], h["ngTypeFF"], h["ngTypeE"], h["ngTypeI"])
@@ -266,6 +324,21 @@ def restore(sim, path, as_init: bool = False) -> dict:
     sim.lastNoise.copy_(T(d["lastNoise"]))
     sim.lTR.copy_(T(d["leftTimeRate"]))
     sim.lNL.copy_(T(d["lastNegLogRand"]))
+    if learn:
+        # the strengths and the post variables (src/patch.cu:5556-5563); the
+        # homeostatic totals stay those of the network's initial strengths,
+        # which the reference computes before it reads the snapshot
+        # (:4336-4349); the LGN trace is the file's (the reference's resumed
+        # run restarts it from 0, its surface being cleared at start-up)
+        sim.sLGN.copy_(T(d["sLGN"]))
+        for k, v in post_from_file(d["post"], sp.nblock, sp.blockSize, sp.nE).items():
+            sim.lvar[k].copy_(T(v))
+        sim.xVar.copy_(T(d["pre"]))
+        if sim.totalFF is not None:
+            from .params import learn_ff_totals
+            tot, inf = learn_ff_totals(net)
+            sim.totalFF.copy_(T(tot))
+            sim.totalFFinf.copy_(T(inf))
     ring = d["spikeTrain"]
     if getattr(sim, "delay", False) or getattr(sim, "far_only", False):
         it0 = 0 if as_init else h["it0"]
