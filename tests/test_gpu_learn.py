@@ -36,18 +36,22 @@ def _net(**kw):
     return network.build_network(network.spec_tiny(**base))
 
 
-@pytest.mark.parametrize("homeo,rebound", [(0, 0), (1, 0), (2, 1)])
-def test_learn_kernel_random_inputs_bitwise(pkg, homeo, rebound):
+# m > 32 with homeostasis takes the one-lane-per-neuron kernel, the others
+# the four-wavefront one
+@pytest.mark.parametrize("homeo,rebound,m", [(0, 0, 12), (1, 0, 12), (2, 1, 12), (1, 0, 40),
+                                             (0, 1, 40)])
+def test_learn_kernel_random_inputs_bitwise(pkg, homeo, rebound, m):
     """vc_v1_learn_ff against orc_v1_learn_ff on random rows: spikes and
     silent neurons, 0-2 LGN spikes, zero strengths, values past the bounds."""
     import oracle
     from visual_cortex_amd import abi, params
     from visual_cortex_amd.sim import ptr
-    _flush(f"test_learn_kernel_random_inputs_bitwise[homeo {homeo}, rebound {rebound}]")
+    _flush(f"test_learn_kernel_random_inputs_bitwise[homeo {homeo}, rebound {rebound}, m {m}]")
     net = _net(applyHomeo=homeo, rebound=rebound, targetFR=(4.0, 4.0), tauAvg=(300.0, 300.0),
-               gmaxLGN=1.3, gminLGN=0.3, FF_InfRatio=0.9, symmetricHomeo=0)
+               gmaxLGN=1.3, gminLGN=0.3, FF_InfRatio=0.9, symmetricHomeo=0, m=m)
+    assert net.sLGN.shape[0] == m
     sp = net.spec
-    rng = np.random.default_rng(7 + homeo)
+    rng = np.random.default_rng(7 + homeo + m)
     p = params.v1_params_struct(net)
     lp = params.learn_ff_struct(net)
     sL = net.sLGN.astype(np.float32).copy()

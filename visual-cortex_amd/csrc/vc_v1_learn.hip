@@ -164,25 +164,31 @@ __global__ __launch_bounds__(256) void k_v1_learn_ff(
     ls.vAvg[n] = avg;
 }
 
-// Without homeostasis the synapses of a neuron are independent (no running
-// total), so four wavefronts share each 64 neurons: wavefront w takes
-// synapses w, w+4, w+8, ... (at most 8 for m <= 32, loaded together).  Four
-// times the wavefronts of k_v1_learn_ff hide the load latency that one
-// wavefront per SIMD cannot.  Wavefront 0 stores the post traces after a
-// barrier (every wavefront has read them by then).
-template <bool kLds>
+// Four wavefronts share each 64 neurons: wavefront w takes synapses w, w+4,
+// w+8, ... (at most 8 for m <= 32, loaded together).  Four times the
+// wavefronts of k_v1_learn_ff hide the load latency that one wavefront per
+// SIMD cannot.  Wavefront 0 stores the post traces after a barrier (every
+// wavefront has read them by then).  With homeostasis (kHomeo, maxLGN <= 32)
+// each synapse's changes to the running total (-LTD, +LTP, the clip) go to
+// LDS, and wavefront 0 adds them in synapse order, as the reference's
+// sequential loop does, so the total is the same float.
+constexpr uint32_t kHomeoSyn = 32;
+
+template <bool kLds, bool kHomeo>
 __global__ __launch_bounds__(256) void k_v1_learn_ff_split(
     vc_v1_params p, vc_learn_ff_params lp, vc_ff_conn ff, vc_learn_ff_state ls,
     const float *__restrict__ sInfo, const float *__restrict__ lgnS,
     const float *__restrict__ xStart, const float *__restrict__ xSpike,
     const uint64_t *__restrict__ bitsG, uint32_t nWords) {
     __shared__ uint64_t bitsL[kLds ? kMaxLdsWords : 1];
+    __shared__ float terms[kHomeo ? kHomeoSyn * 3 * 64 : 1];   // [i][3][lane]
     if (kLds) {
         for (uint32_t w = threadIdx.x; w < nWords; w += blockDim.x) bitsL[w] = bitsG[w];
         __syncthreads();
     }
     const uint64_t *bits = kLds ? bitsL : bitsG;
     const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t n = blockIdx.x * 64u + (threadIdx.x & 63u);
     const bool mine = n < p.nV1 && n % p.blockSize < p.nE;
     const float dt = p.dt;
@@ -204,6 +210,19 @@ __global__ __launch_bounds__(256) void k_v1_learn_ff_split(
             avg1 = decayed(avg0, lp.tauAvg, tsp);
         }
         m = ff.nLGN[n];
+    }
+    float newTotal = 0.0f, change = 0.0f;
+    if (kHomeo && mine) {                                   // as k_v1_learn_ff
+        const float total0 = ls.totalFF[n];
+        const float inf = ls.totalFFinf[n];
+        const float d = total0 - inf;
+        if (d > 0.0f || lp.symmetricHomeo) {
+            newTotal = d * lp.expHomeo + inf;
+            change = (lp.applyHomeo == 1u) ? newTotal / total0 : (newTotal - total0) / (float)m;
+        } else {
+            newTotal = total0;
+            change = (lp.applyHomeo == 1u) ? 1.0f : 0.0f;
+        }
     }
     constexpr uint32_t kPer = 8;
     for (uint32_t j0 = 0; wave + 4u * j0 < m; j0 += kPer) {
@@ -237,8 +256,20 @@ __global__ __launch_bounds__(256) void k_v1_learn_ff_split(
             const uint32_t i = wave + 4u * (j0 + k);
             if (i >= m) break;
             const float f0 = fs[k];
-            if (f0 == 0.0f && !lp.rebound) continue;
+            float tL = 0.0f, tP = 0.0f, tC = 0.0f;          // changes to the running total
+            if (f0 == 0.0f && !lp.rebound) {
+                if (kHomeo) {
+                    terms[(i * 3 + 0) * 64 + lane] = 0.0f;
+                    terms[(i * 3 + 1) * 64 + lane] = 0.0f;
+                    terms[(i * 3 + 2) * 64 + lane] = 0.0f;
+                }
+                continue;
+            }
             float f = f0;
+            if (kHomeo) {
+                if (lp.applyHomeo == 1u) f *= change;
+                else f += change;
+            }
             const float sF = sFs[k];
             const float nspF = floorf(sF);
             const float tspF = (sF > 0.0f ? sF - nspF : 1.0f) * dt;
@@ -247,17 +278,42 @@ __global__ __launch_bounds__(256) void k_v1_learn_ff_split(
                 const float av = early ? avg0 : avg1;
                 float A = lp.A_ratio;
                 if (lp.targetFR > 0.0f) A *= lp.tauLTP * av * av;
-                f -= decayed(early ? ltd0 : ltd1, lp.tauLTD, early ? tspF : tspF - tsp) * A;
+                const float df = decayed(early ? ltd0 : ltd1, lp.tauLTD, early ? tspF : tspF - tsp) * A;
+                f -= df;
+                tL = -df;
             }
-            if (nsp > 0.0f)
-                f += decayed(xp[k], lp.tauLTP, early ? tsp - tspF : tsp) * trip1 * lp.A_LTP;
-            if (f < lp.gmin) f = lp.gmin;
-            if (f > lp.gmax) f = lp.gmax;
+            if (nsp > 0.0f) {
+                const float df = decayed(xp[k], lp.tauLTP, early ? tsp - tspF : tsp) * trip1 * lp.A_LTP;
+                f += df;
+                tP = df;
+            }
+            if (f < lp.gmin) {
+                tC = lp.gmin - f;
+                f = lp.gmin;
+            }
+            if (f > lp.gmax) {
+                tC = -(f - lp.gmax);
+                f = lp.gmax;
+            }
+            if (kHomeo) {
+                terms[(i * 3 + 0) * 64 + lane] = tL;
+                terms[(i * 3 + 1) * 64 + lane] = tP;
+                terms[(i * 3 + 2) * 64 + lane] = tC;
+            }
             if (__float_as_uint(f) != __float_as_uint(f0)) ls.sLGN[(size_t)i * p.nV1 + n] = f;
         }
     }
     __syncthreads();
     if (wave != 0 || !mine) return;
+    if (kHomeo) {                          // the reference's order: synapse by synapse
+        float deltaF = 0.0f;
+        for (uint32_t i = 0; i < m; ++i) {
+            deltaF += terms[(i * 3 + 0) * 64 + lane];
+            deltaF += terms[(i * 3 + 1) * 64 + lane];
+            deltaF += terms[(i * 3 + 2) * 64 + lane];
+        }
+        ls.totalFF[n] = newTotal + deltaF;
+    }
     const float rest = (nsp > 0.0f) ? dt - tsp : dt;
     float ltd = (nsp > 0.0f) ? ltd1 : ltd0;
     float trip = (nsp > 0.0f) ? trip1 : trip0;
@@ -314,17 +370,27 @@ extern "C" int vc_v1_learn_ff(void *stream, const vc_v1_params *p, const vc_lear
         return VC_E_INVALID;
     }
     const uint32_t nWords = (nLGN + 63u) / 64u;
-    if (lp->applyHomeo == 0u) {           // independent synapses: 4 wavefronts per 64 neurons
-        const dim3 grid((p->nV1 + 63u) / 64u);
-        if (nWords <= kMaxLdsWords)
-            hipLaunchKernelGGL(k_v1_learn_ff_split<true>, grid, dim3(256), 0, (hipStream_t)stream,
-                               *p, *lp, *ff, *ls, sInfo, lgnSInfo, xStart, xSpike, lgnSpikeBits,
-                               nWords);
-        else
-            hipLaunchKernelGGL(k_v1_learn_ff_split<false>, grid, dim3(256), 0,
+    const dim3 grid((p->nV1 + 63u) / 64u);
+    const bool lds = nWords <= kMaxLdsWords;
+    if (lp->applyHomeo == 0u) {           // 4 wavefronts per 64 neurons
+        if (lds)
+            hipLaunchKernelGGL((k_v1_learn_ff_split<true, false>), grid, dim3(256), 0,
                                (hipStream_t)stream, *p, *lp, *ff, *ls, sInfo, lgnSInfo, xStart,
                                xSpike, lgnSpikeBits, nWords);
-    } else if (nWords <= kMaxLdsWords)
+        else
+            hipLaunchKernelGGL((k_v1_learn_ff_split<false, false>), grid, dim3(256), 0,
+                               (hipStream_t)stream, *p, *lp, *ff, *ls, sInfo, lgnSInfo, xStart,
+                               xSpike, lgnSpikeBits, nWords);
+    } else if (ff->maxLGN <= kHomeoSyn) {  // the same, the total's terms added in order
+        if (lds)
+            hipLaunchKernelGGL((k_v1_learn_ff_split<true, true>), grid, dim3(256), 0,
+                               (hipStream_t)stream, *p, *lp, *ff, *ls, sInfo, lgnSInfo, xStart,
+                               xSpike, lgnSpikeBits, nWords);
+        else
+            hipLaunchKernelGGL((k_v1_learn_ff_split<false, true>), grid, dim3(256), 0,
+                               (hipStream_t)stream, *p, *lp, *ff, *ls, sInfo, lgnSInfo, xStart,
+                               xSpike, lgnSpikeBits, nWords);
+    } else if (lds)
         hipLaunchKernelGGL(k_v1_learn_ff<true>, dim3((p->nV1 + 255u) / 256u), dim3(256), 0,
                            (hipStream_t)stream, *p, *lp, *ff, *ls, sInfo, lgnSInfo, xStart,
                            xSpike, lgnSpikeBits, nWords);
