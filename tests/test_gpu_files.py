@@ -56,3 +56,38 @@ def test_cli_delayed_run_from_files(pkg, tmp_path, monkeypatch):
     assert np.array_equal(raw["sInfo"].view(np.uint32), s_direct.view(np.uint32))
     st = g.state_numpy()
     assert np.array_equal(raw["gE"][-1], st["gE"]) and np.array_equal(raw["hI"][-1], st["hI"])
+
+
+def test_cli_plastic_run_writes_slgn(pkg, tmp_path, monkeypatch):
+    """learning = 3 from the cfg: the CLI's outputs and its sLGN file (one
+    record every sampleInterval_LGN_V1 steps) equal the direct plastic run's
+    spikes and strengths."""
+    from visual_cortex_amd import cli, formats, loader, network
+    from visual_cortex_amd.sim import GpuSim
+    print("test_cli_plastic_run_writes_slgn: tiny, 192 steps, learning 3", flush=True)
+    monkeypatch.chdir(tmp_path)
+    keys = dict(tauLTP=16.8, tauLTD=33.7, tauTrip=114.0, r_LTD=1.0, A_LGN=0.05,
+                tauAvg=(1000.0, 1000.0), targetFR=(5.0, 5.0), gmaxLGN=5.0, gminLGN=0.0)
+    net = network.build_network(network.spec_tiny(nt=192, bgStrength=(0.4, 0.4), learning=3,
+                                                  sampleInterval_LGN_V1=64, learnData_FF=1,
+                                                  **keys))
+    cfg = loader.save_network(net, tmp_path / "run")
+    with open(cfg, "a") as f:
+        f.write("rawData = true\nhWrite = true\nlearning = 3\nsampleInterval_LGN_V1 = 64\n")
+        for k, v in keys.items():
+            f.write(f"{k} = {', '.join(str(x) for x in v) if isinstance(v, tuple) else v}\n")
+    assert cli.main(["-c", str(cfg)]) == 0
+    raw = formats.read_raw_data(tmp_path / "run" / "rawData-vc.bin")
+    paths = sorted((tmp_path / "run").glob("sLGN*"))
+    assert len(paths) == 1, paths
+    d = formats.read_slgn(paths[0])
+    assert (d["nt"], d["sampleInterval"], d["nLearnTypeFF_E"], d["nLearnTypeFF_I"]) == (192, 64, 1, 0)
+    assert d["sLGN"].shape == (3, net.sLGN.shape[0], net.spec.nV1)
+    g = GpuSim(net)
+    assert g.learn
+    for k in range(3):
+        g.run(64)
+        want = g.learn_state_numpy()["sLGN"]
+        assert np.array_equal(d["sLGN"][k].view(np.uint32), want.view(np.uint32)), k
+    assert (d["sLGN"][-1] != net.sLGN).sum() > 50
+    assert np.array_equal(raw["sInfo"].view(np.uint32), g.sInfo().view(np.uint32))
