@@ -513,7 +513,10 @@ int vc_v1_rec_far(void *stream, const vc_v1_params *p, const vc_v1_state *s,
  * ffMask + i*maskStride its event mask (or NULL); out_sInfo + i*outStride
  * receives a copy of step i's outputs when out_sInfo is not NULL.  zeroMask
  * is nV1/64 zero words; og, oh [VC_MAX_NG][nV1] are work rows.  The same
- * launches as the per-step calls, issued from one host call. */
+ * launches as the per-step calls, issued from one host call.  syn NULL: the
+ * near connections are undelayed (noDelay 1 with delayed far connections,
+ * noFarDelay 0) and vc_v1_rec_apply_at of the step's own spikes
+ * (recal_G_mat_nd) takes the place of vc_v1_rec_delay (window unused). */
 int vc_v1_delay_run(void *stream, const vc_v1_params *p, const vc_v1_state *s,
                     const vc_ff_conn *ff, const vc_rec_conn *rec, const uint32_t *syn,
                     const vc_far_conn *far, float *train, uint64_t *trainMask,

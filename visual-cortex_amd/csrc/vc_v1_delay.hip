@@ -738,9 +738,18 @@ extern "C" int vc_v1_delay_run(void *stream, const vc_v1_params *p, const vc_v1_
                             ffMask ? ffMask + i * maskStride : nullptr, it0 + i, row, nullptr,
                             nullptr);
         if (rc != VC_OK) return rc;
-        // the delayed gather also copies the step's outputs to out_sInfo
-        rc = rec_delay_launch(stream, p, s, rec, syn, train, trainMask, depth, slot, window,
-                              status, out_sInfo ? out_sInfo + i * outStride : nullptr);
+        if (syn) {
+            // the delayed gather also copies the step's outputs to out_sInfo
+            rc = rec_delay_launch(stream, p, s, rec, syn, train, trainMask, depth, slot, window,
+                                  status, out_sInfo ? out_sInfo + i * outStride : nullptr);
+        } else {
+            // undelayed near connections: the step's own spikes (recal_G_mat_nd)
+            rc = vc_v1_rec_apply_at(stream, p, s, rec, &ring, it0 + i);
+            if (rc == VC_OK && out_sInfo)
+                VC_HIP_CHECK(hipMemcpyAsync(out_sInfo + i * outStride, row,
+                                            (size_t)p->nV1 * sizeof(float),
+                                            hipMemcpyDeviceToDevice, (hipStream_t)stream));
+        }
         if (rc != VC_OK) return rc;
         if (far && far->maxFar) {
             rc = vc_v1_rec_far(stream, p, s, far, train, trainMask, depth, slot, speed, status);

@@ -752,6 +752,20 @@ class GpuSim:
                     "vc_v1_delay_run")
                 slot = (it + n - 1 + self.slot_off) % self.depth
                 self.sInfo_last = out[done + n - 1] if out is not None else self.train[slot]
+            elif self.far_only and not self.gap_on:
+                # the n steps in one native call, undelayed near connections (syn NULL)
+                slot0 = (it + self.slot_off) % self.depth
+                check(self.lib.vc_v1_delay_run(
+                    self._hs(sv), C.byref(self.p), C.byref(self.s), C.byref(self.ff),
+                    C.byref(self.rec), None, C.byref(self.far), ptr(self.train),
+                    ptr(self.trainMask), self.depth, self.window, slot0,
+                    ptr(self.lgn_sInfo2[buf]) + k_in * sp.nLGN * 4, sp.nLGN, mptr, sp.nV1,
+                    it, n, self.speed, ptr(self._keep), ptr(self.dog), ptr(self.doh),
+                    optr if out is not None else None, sp.nV1, ptr(self.status)),
+                    "vc_v1_delay_run")
+                slot = (it + n - 1 + self.slot_off) % self.depth
+                self.sInfo_last = out[done + n - 1] if out is not None else self.train[slot]
+                self.n_v1_launches += n
             elif self.far_only:
                 for i in range(n):
                     row = out[done + i] if out is not None else None
