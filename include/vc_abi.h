@@ -738,8 +738,8 @@ typedef struct {
  * at x exp(-dt/tauLTP).  tsp is the draw's own spike time, not the one
  * recovered from sInfo.  spikeBits[s*nWords + c/64] bit c%64 (nWords =
  * (nLGN + 63)/64) is set when cell c spiked in step it0+s, the bitmap
- * vc_v1_learn_ff tests before it reads a cell's row.  Not for batched trials
- * (p->cellTrial must be NULL). */
+ * vc_v1_learn_ff tests before it reads a cell's row.  Batched trials learn
+ * as each alone (every quantity is per cell). */
 int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc_lgn_state *s,
                        const float *Rring, uint32_t it0, uint32_t nsteps, float *sInfo,
                        float *fr, const vc_ff_events *ev, float tauLTP, float *xVar,
@@ -759,8 +759,9 @@ int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc_lgn_state 
  * [gmin, gmax].  Then the post traces take the step's spikes and decay to the
  * step end, in the reference's operation order.  sInfo is the step's V1
  * output row, lgnSInfo/xStart/xSpike/lgnSpikeBits the step's rows of
- * vc_lgn_steps_learn over nLGN cells.  Not for batched trials (p->blockTrial
- * must be NULL). */
+ * vc_lgn_steps_learn over nLGN cells.  Batched trials (p->blockTrial) learn
+ * as each alone: a neuron's synapses and LGN cells lie in its trial, and the
+ * rule's constants are shared. */
 int vc_v1_learn_ff(void *stream, const vc_v1_params *p, const vc_learn_ff_params *lp,
                    const vc_ff_conn *ff, const vc_learn_ff_state *ls, const float *sInfo,
                    const float *lgnSInfo, const float *xStart, const float *xSpike,

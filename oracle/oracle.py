@@ -222,8 +222,6 @@ class OracleSim:
         # post traces and the homeostatic totals (zero / the initial sums)
         self.learn = getattr(sp, "learning", 0) == 3
         if self.learn:
-            if self.batch is not None:
-                raise NotImplementedError("plasticity with batched trials is not built")
             self.learn_p = params.learn_ff_struct(net)
             self.xVar = np.zeros(sp.nLGN, np.float32)
             self.xStart = np.zeros_like(self.lgn_sInfo)

@@ -1400,7 +1400,6 @@ int orc_v1_learn_ff(const vc_v1_params *p, const vc_learn_ff_params *lp,
                     float *vLTD, float *vTrip, float *vAvg, float *totalFF,
                     const float *totalFFinf, const float *sInfo, const float *lgnS,
                     const float *xStart, const float *xSpike) {
-    if (p->blockTrial) return -2;
 #pragma omp parallel for schedule(static)
     for (uint32_t n = 0; n < p->nV1; ++n) {
         if (n % p->blockSize >= p->nE) continue;

@@ -229,3 +229,29 @@ def test_plastic_snapshot_resume(pkg, tmp_path, homeo):
     assert np.array_equal(_bits(b.sInfo()), _bits(ref))
     for k in ("sLGN", "vLTD", "vTrip", "vAvg", "xVar"):
         assert np.array_equal(_bits(la[k]), _bits(lb[k])), k
+
+
+def test_batched_plastic_trials_equal_serial(pkg):
+    """Plastic trials batched into one network (homeostasis on) learn as each
+    alone, bit for bit, on the GPU."""
+    from visual_cortex_amd import batch
+    from visual_cortex_amd.sim import GpuSim
+    _flush("test_batched_plastic_trials_equal_serial: 3 trials, 200 steps")
+    nets = [_net(nt=600, seed=40 + i, nblock=nb, ori_deg=22.5 * i, applyHomeo=1,
+                 FF_InfRatio=0.9) for i, nb in enumerate((2, 4, 3))]
+    g = GpuSim(batch.batch_networks(nets, device="cuda"))
+    assert g.learn
+    g.run(200)
+    sb, lb = g.sInfo(), g.learn_state_numpy()
+    off = 0
+    for t, n in enumerate(nets):
+        gs = GpuSim(n)
+        gs.run(200)
+        k = n.spec.nV1
+        ls = gs.learn_state_numpy()
+        assert (ls["sLGN"] != n.sLGN).sum() > 20, t
+        assert np.array_equal(_bits(sb[:, off:off + k]), _bits(gs.sInfo())), t
+        for key in ("sLGN", "vLTD", "vTrip", "vAvg", "totalFF"):
+            a = lb[key][..., off:off + k]
+            assert np.array_equal(_bits(a), _bits(ls[key])), (t, key)
+        off += k

@@ -309,15 +309,35 @@ def test_learning_config_keys(pkg):
         config.spec_from_config(config.Config(base + "learnData_FF = 2\n"))
 
 
-def test_learning_refusals(pkg):
-    """Plastic runs of batched trials are refused."""
+def test_batched_plastic_trials_equal_serial(pkg):
+    """Plastic trials batched into one network learn as each alone: the
+    outputs, the strengths and the post traces of each trial equal its own
+    run bit for bit; trials with other rule parameters are refused."""
     import oracle
-    from visual_cortex_amd import batch, network
-    nets = [_net(seed=5 + i) for i in range(2)]
+    from visual_cortex_amd import batch
+    nets = [_net(seed=5 + i, nblock=nb, ori_deg=30.0 * i, applyHomeo=1, FF_InfRatio=0.9)
+            for i, nb in enumerate((2, 3))]
     b = batch.batch_networks(nets, device="cpu")
-    with pytest.raises(NotImplementedError, match="batched"):
-        oracle.OracleSim(b, pkg)
-    assert network.spec_tiny().learning == 0
+    ob = oracle.OracleSim(b, pkg)
+    ob.run(160)
+    sb = ob.sInfo()
+    off = 0
+    for n in nets:
+        o = oracle.OracleSim(n, pkg)
+        s0 = o.sLGN.copy()
+        o.run(160)
+        k = n.spec.nV1
+        assert (o.sLGN != s0).sum() > 20
+        assert np.array_equal(sb[:, off:off + k].view(np.uint32), o.sInfo().view(np.uint32))
+        assert np.array_equal(ob.sLGN[:, off:off + k].view(np.uint32), o.sLGN.view(np.uint32))
+        for key in ("vLTD", "vTrip", "vAvg"):
+            assert np.array_equal(ob.lvar[key][off:off + k].view(np.uint32),
+                                  o.lvar[key].view(np.uint32)), key
+        assert np.array_equal(ob.totalFF[off:off + k].view(np.uint32), o.totalFF.view(np.uint32))
+        off += k
+    with pytest.raises(ValueError, match="plasticity"):
+        batch.batch_networks([nets[0], _net(seed=9, applyHomeo=1, FF_InfRatio=0.8)],
+                             device="cpu")
 
 
 def test_slgn_file_layout(pkg, tmp_path):

@@ -174,6 +174,13 @@ def batch_networks(nets: list, device=None) -> N.Network:
         if any(getattr(s, k) != getattr(sp0, k) for k in noise):
             raise ValueError("batched trials must share the noise and synaptic-failure "
                              "parameters (" + ", ".join(noise) + ")")
+        learn = ("learning", "tauLTP", "tauLTD", "tauTrip", "r_LTD", "A_LGN", "tauAvg",
+                 "targetFR", "gmaxLGN", "gminLGN", "applyHomeo", "symmetricHomeo",
+                 "decayTimeHomeoFF", "FF_InfRatio", "rebound")
+        if any(getattr(s, k) != getattr(sp0, k) for k in learn) or (
+                s.learning and s.sRatioLGN[0] != sp0.sRatioLGN[0]):
+            raise ValueError("batched trials must share the plasticity parameters ("
+                             + ", ".join(learn) + ", sRatioLGN[0])")
     if device is None:
         c0 = nets[0].csr
         device = c0[0].device if c0 is not None and hasattr(c0[0], "device") else \

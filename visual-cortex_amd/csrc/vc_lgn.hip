@@ -1015,10 +1015,6 @@ extern "C" int vc_lgn_steps_learn(void *stream, const vc_lgn_params *p, const vc
         vc_set_error("vc_lgn_steps_learn: tauLTP %g must be positive", (double)tauLTP);
         return VC_E_INVALID;
     }
-    if (p && p->cellTrial) {
-        vc_set_error("vc_lgn_steps_learn: plasticity with batched trials is not built");
-        return VC_E_UNSUPPORTED;
-    }
     return lgn_steps(stream, p, s, Rring, it0, nsteps, sInfo, fr, ev,
                      LgnTrace{tauLTP, xVar, xStart, xSpike, spikeBits});
 }

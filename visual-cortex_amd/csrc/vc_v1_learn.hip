@@ -360,10 +360,6 @@ extern "C" int vc_v1_learn_ff(void *stream, const vc_v1_params *p, const vc_lear
         vc_set_error("vc_v1_learn_ff: the trace time constants must be positive");
         return VC_E_INVALID;
     }
-    if (p->blockTrial) {
-        vc_set_error("vc_v1_learn_ff: plasticity with batched trials is not built");
-        return VC_E_UNSUPPORTED;
-    }
     if (p->nV1 == 0 || p->blockSize == 0 || p->nV1 != p->nblock * p->blockSize) {
         vc_set_error("vc_v1_learn_ff: nV1 %u != nblock %u x blockSize %u", p->nV1, p->nblock,
                      p->blockSize);

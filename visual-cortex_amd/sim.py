@@ -213,8 +213,6 @@ class GpuSim:
         cells' trace and the rule's constants."""
         from .params import learn_ff_struct, learn_ff_totals
         net, sp, d = self.net, self.net.spec, self.dev
-        if self.batch is not None:
-            raise NotImplementedError("plasticity (learning) with batched trials is not built")
         z = dict(dtype=torch.float32, device=d)
         self.lvar = {k: torch.zeros(sp.nV1, **z) for k in ("vLTD", "vTrip", "vAvg")}
         self.totalFF = self.totalFFinf = None
